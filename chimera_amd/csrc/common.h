@@ -1,0 +1,103 @@
+// Shared host/device declarations of libchimera_hip (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/chimera_hip.h"
+
+namespace chm {
+
+void set_error(const std::string &msg);
+
+#define CHM_HIP(expr)                                                                \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            chm::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));        \
+            return CHM_E_HIP;                                                        \
+        }                                                                            \
+    } while (0)
+
+#define CHM_REQUIRE(cond, code, msg)                                                 \
+    do {                                                                             \
+        if (!(cond)) {                                                               \
+            chm::set_error(msg);                                                     \
+            return code;                                                             \
+        }                                                                            \
+    } while (0)
+
+#define CHM_LAUNCH_CHECK()                                                           \
+    do {                                                                             \
+        hipError_t e_ = hipGetLastError();                                           \
+        if (e_ != hipSuccess) {                                                      \
+            chm::set_error(std::string("kernel launch: ") + hipGetErrorString(e_));   \
+            return CHM_E_HIP;                                                        \
+        }                                                                            \
+    } while (0)
+
+constexpr int kMaxNsp = 6;
+
+// A device buffer owned by the library.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int ensure(size_t n) {
+        if (n <= bytes && p) return CHM_OK;
+        release();
+        if (n == 0) return CHM_OK;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+            return CHM_E_HIP;
+        }
+        bytes = n;
+        return CHM_OK;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace chm
+
+struct chm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // tables
+    int nsp = 0, nk = 0;
+    double s02 = 1.0;
+    chm::DevBuf k, f, delta, lam;
+    std::vector<double> h_k;
+    // topology
+    int natoms = 0, npairs = 0, nshell = 1;
+    chm::DevBuf i_idx, j_idx, shift, shell, sig2;
+    chm::DevBuf drtab, c3tab, s2tab;
+    // scratch for the forward path
+    chm::DevBuf pos, species, r, evec, ptype, partial, chi, rows_in, rows_out;
+};
+
+struct chm_ft {
+    chm_ctx *ctx = nullptr;
+    int nchan = 0, nk = 0, nR = 0, nsp = 0;
+    std::vector<int> chan_edge;
+    std::vector<int> edge_first, edge_nkw;  // channels of each edge (contiguous)
+    chm::DevBuf D;       // (nsp, nk, nkw_e * nR * 2): per edge, channels edge-major
+    chm::DevBuf in, out, extra;
+};
+
+namespace chm {
+// Generic FP64 GEMM on the context stream: C[b] (M x N) = A[b] (M x K) @ B[b] (K x N),
+// row-major with leading dimensions and per-batch element strides.
+int gemm_f64(hipStream_t st, int nbatch, int M, int N, int K, const double *A, int lda,
+             long sA, const double *B, int ldb, long sB, double *C, int ldc, long sC);
+}  // namespace chm

@@ -1,0 +1,554 @@
+// Context, tables, topology and the pair-space path sum (single scattering) for gfx950.
+//
+// Path sum layout: a workgroup owns one configuration and a contiguous run of pairs;
+// its 256 lanes own k points (lane, lane+256, ...).  Pair records are staged through
+// LDS in tiles of 256 (one coalesced load per tile, k-independent parts precomputed
+// by the loading lane), the per-k tables f[b](k), delta[t](k) sit in LDS indexed
+// [row][k] so a wavefront's 64 lanes read 64 consecutive words.  Every pair record
+// read from HBM is reused for all nk points.  Each workgroup writes one partial
+// chi (nsp, nk); a second kernel sums the partials in a fixed order, so results are
+// deterministic.
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+namespace chm {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+}  // namespace chm
+
+extern "C" const char *chm_last_error(void) { return chm::g_err.c_str(); }
+extern "C" int chm_version(void) { return 1; }
+
+extern "C" int chm_ctx_create(int device, chm_ctx **out) {
+    CHM_REQUIRE(out != nullptr, CHM_E_ARG, "chm_ctx_create: out is NULL");
+    int ndev = 0;
+    CHM_HIP(hipGetDeviceCount(&ndev));
+    CHM_REQUIRE(device >= 0 && device < ndev, CHM_E_ARG, "chm_ctx_create: no such device");
+    CHM_HIP(hipSetDevice(device));
+    auto *c = new chm_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        chm::set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        return CHM_E_HIP;
+    }
+    *out = c;
+    return CHM_OK;
+}
+
+extern "C" int chm_ctx_destroy(chm_ctx *ctx) {
+    if (!ctx) return CHM_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CHM_OK;
+}
+
+extern "C" int chm_ctx_sync(chm_ctx *ctx) {
+    CHM_REQUIRE(ctx, CHM_E_ARG, "null context");
+    CHM_HIP(hipStreamSynchronize(ctx->stream));
+    return CHM_OK;
+}
+
+namespace {
+
+int upload(chm_ctx *ctx, chm::DevBuf &b, const void *h, size_t n) {
+    int rc = b.ensure(n);
+    if (rc) return rc;
+    if (n) CHM_HIP(hipMemcpyAsync(b.p, h, n, hipMemcpyHostToDevice, ctx->stream));
+    return CHM_OK;
+}
+
+int download(chm_ctx *ctx, void *h, const chm::DevBuf &b, size_t n) {
+    if (n) CHM_HIP(hipMemcpyAsync(h, b.p, n, hipMemcpyDeviceToHost, ctx->stream));
+    CHM_HIP(hipStreamSynchronize(ctx->stream));
+    return CHM_OK;
+}
+
+}  // namespace
+
+extern "C" int chm_fm_set_tables(chm_ctx *ctx, int nsp, int nk, const double *k,
+                                 const double *f, const double *delta, const double *lam,
+                                 double s02) {
+    CHM_REQUIRE(ctx, CHM_E_ARG, "null context");
+    CHM_REQUIRE(nsp >= 2, CHM_E_ARG, "need at least two species");
+    CHM_REQUIRE(nsp <= chm::kMaxNsp, CHM_E_ARG, "at most 6 species are compiled in");
+    CHM_REQUIRE(nk >= 2 && k && f && delta && lam, CHM_E_ARG, "bad tables");
+    CHM_HIP(hipSetDevice(ctx->device));
+    ctx->nsp = nsp;
+    ctx->nk = nk;
+    ctx->s02 = s02;
+    ctx->h_k.assign(k, k + nk);
+    int rc;
+    if ((rc = upload(ctx, ctx->k, k, sizeof(double) * nk))) return rc;
+    if ((rc = upload(ctx, ctx->f, f, sizeof(double) * nsp * nk))) return rc;
+    if ((rc = upload(ctx, ctx->delta, delta, sizeof(double) * nsp * nsp * nk))) return rc;
+    if ((rc = upload(ctx, ctx->lam, lam, sizeof(double) * nk))) return rc;
+    CHM_HIP(hipStreamSynchronize(ctx->stream));
+    return CHM_OK;
+}
+
+extern "C" int chm_fm_set_cumulants(chm_ctx *ctx, int nshell, const double *drtab,
+                                    const double *c3tab, const double *s2tab) {
+    CHM_REQUIRE(ctx && ctx->nsp > 0, CHM_E_STATE, "set tables first");
+    CHM_REQUIRE(nshell >= 1 && drtab && c3tab && s2tab, CHM_E_ARG, "bad cumulant tables");
+    CHM_REQUIRE(ctx->npairs == 0 || nshell == ctx->nshell, CHM_E_ARG,
+                "cumulant tables must have one entry per shell of the topology");
+    CHM_HIP(hipSetDevice(ctx->device));
+    size_t n3 = sizeof(double) * nshell * ctx->nsp * ctx->nsp;
+    int rc;
+    if ((rc = upload(ctx, ctx->drtab, drtab, n3))) return rc;
+    if ((rc = upload(ctx, ctx->s2tab, s2tab, n3))) return rc;
+    if ((rc = upload(ctx, ctx->c3tab, c3tab, sizeof(double) * nshell))) return rc;
+    ctx->nshell = nshell;
+    CHM_HIP(hipStreamSynchronize(ctx->stream));
+    return CHM_OK;
+}
+
+extern "C" int chm_fm_set_topology(chm_ctx *ctx, int natoms, int npairs, const int32_t *i_idx,
+                                   const int32_t *j_idx, const double *shift,
+                                   const int32_t *shell, const double *sig2) {
+    CHM_REQUIRE(ctx && ctx->nsp > 0, CHM_E_STATE, "set tables first");
+    CHM_REQUIRE(natoms > 0 && npairs > 0 && i_idx && j_idx && shift && shell && sig2,
+                CHM_E_ARG, "bad topology");
+    int nshell = 0;
+    for (int p = 0; p < npairs; ++p) {
+        CHM_REQUIRE(i_idx[p] >= 0 && i_idx[p] < natoms && j_idx[p] >= 0 && j_idx[p] < natoms,
+                    CHM_E_ARG, "pair index out of range");
+        CHM_REQUIRE(shell[p] >= 0, CHM_E_ARG, "negative shell index");
+        if (shell[p] + 1 > nshell) nshell = shell[p] + 1;
+    }
+    CHM_HIP(hipSetDevice(ctx->device));
+    ctx->natoms = natoms;
+    ctx->npairs = npairs;
+    int rc;
+    if ((rc = upload(ctx, ctx->i_idx, i_idx, sizeof(int32_t) * npairs))) return rc;
+    if ((rc = upload(ctx, ctx->j_idx, j_idx, sizeof(int32_t) * npairs))) return rc;
+    if ((rc = upload(ctx, ctx->shift, shift, sizeof(double) * 3 * npairs))) return rc;
+    if ((rc = upload(ctx, ctx->shell, shell, sizeof(int32_t) * npairs))) return rc;
+    if ((rc = upload(ctx, ctx->sig2, sig2, sizeof(double) * npairs))) return rc;
+    // cumulant tables default to zero, sized for this topology's shells
+    std::vector<double> z3((size_t)nshell * ctx->nsp * ctx->nsp, 0.0), z1(nshell, 0.0);
+    ctx->nshell = nshell;
+    if ((rc = upload(ctx, ctx->drtab, z3.data(), sizeof(double) * z3.size()))) return rc;
+    if ((rc = upload(ctx, ctx->s2tab, z3.data(), sizeof(double) * z3.size()))) return rc;
+    if ((rc = upload(ctx, ctx->c3tab, z1.data(), sizeof(double) * z1.size()))) return rc;
+    CHM_HIP(hipStreamSynchronize(ctx->stream));
+    return CHM_OK;
+}
+
+extern "C" int chm_fm_set_shift(chm_ctx *ctx, const double *shift) {
+    CHM_REQUIRE(ctx && ctx->npairs > 0 && shift, CHM_E_STATE, "set topology first");
+    CHM_HIP(hipSetDevice(ctx->device));
+    int rc = upload(ctx, ctx->shift, shift, sizeof(double) * 3 * ctx->npairs);
+    if (rc) return rc;
+    CHM_HIP(hipStreamSynchronize(ctx->stream));
+    return CHM_OK;
+}
+
+extern "C" int chm_fm_set_sig2(chm_ctx *ctx, const double *sig2) {
+    CHM_REQUIRE(ctx && ctx->npairs > 0 && sig2, CHM_E_STATE, "set topology first");
+    CHM_HIP(hipSetDevice(ctx->device));
+    int rc = upload(ctx, ctx->sig2, sig2, sizeof(double) * ctx->npairs);
+    if (rc) return rc;
+    CHM_HIP(hipStreamSynchronize(ctx->stream));
+    return CHM_OK;
+}
+
+// ---------------------------------------------------------------- distances, types
+
+__global__ void k_distances(const double *__restrict__ pos, const int *__restrict__ iidx,
+                            const int *__restrict__ jidx, const double *__restrict__ shift,
+                            double *__restrict__ r, double *__restrict__ evec, int npairs,
+                            int natoms, int nconf) {
+    long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long)npairs * nconf) return;
+    int ic = (int)(tid / npairs), p = (int)(tid % npairs);
+    const double *pi = pos + ((long)ic * natoms + iidx[p]) * 3;
+    const double *pj = pos + ((long)ic * natoms + jidx[p]) * 3;
+    double dx = pj[0] + shift[3 * p] - pi[0];
+    double dy = pj[1] + shift[3 * p + 1] - pi[1];
+    double dz = pj[2] + shift[3 * p + 2] - pi[2];
+    double rr = sqrt(dx * dx + dy * dy + dz * dz);
+    r[tid] = rr;
+    if (evec) {
+        double inv = 1.0 / rr;
+        evec[3 * tid] = dx * inv;
+        evec[3 * tid + 1] = dy * inv;
+        evec[3 * tid + 2] = dz * inv;
+    }
+}
+
+__global__ void k_ptypes(const int *__restrict__ species, const int *__restrict__ iidx,
+                         const int *__restrict__ jidx, int *__restrict__ pt, int npairs,
+                         int natoms, int nconf, int nsp) {
+    long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long)npairs * nconf) return;
+    int ic = (int)(tid / npairs), p = (int)(tid % npairs);
+    pt[tid] = species[(long)ic * natoms + iidx[p]] * nsp + species[(long)ic * natoms + jidx[p]];
+}
+
+static int run_distances(chm_ctx *ctx, int nconf, bool want_evec) {
+    long tot = (long)nconf * ctx->npairs;
+    int rc = ctx->r.ensure(sizeof(double) * tot);
+    if (rc) return rc;
+    if (want_evec && (rc = ctx->evec.ensure(sizeof(double) * 3 * tot))) return rc;
+    dim3 grid((unsigned)((tot + 255) / 256));
+    k_distances<<<grid, 256, 0, ctx->stream>>>(ctx->pos.as<double>(), ctx->i_idx.as<int>(),
+                                                ctx->j_idx.as<int>(), ctx->shift.as<double>(),
+                                                ctx->r.as<double>(),
+                                                want_evec ? ctx->evec.as<double>() : nullptr,
+                                                ctx->npairs, ctx->natoms, nconf);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+static int check_species(const chm_ctx *ctx, int nconf, const int32_t *species) {
+    long n = (long)nconf * ctx->natoms;
+    for (long q = 0; q < n; ++q)
+        CHM_REQUIRE(species[q] >= 0 && species[q] < ctx->nsp, CHM_E_ARG,
+                    "species index out of range");
+    return CHM_OK;
+}
+
+extern "C" int chm_fm_distances(chm_ctx *ctx, int nconf, const double *pos, double *r,
+                                double *evec) {
+    CHM_REQUIRE(ctx && ctx->npairs > 0, CHM_E_STATE, "set topology first");
+    CHM_REQUIRE(nconf > 0 && pos && r, CHM_E_ARG, "bad arguments");
+    CHM_HIP(hipSetDevice(ctx->device));
+    int rc = upload(ctx, ctx->pos, pos, sizeof(double) * 3 * (size_t)nconf * ctx->natoms);
+    if (rc) return rc;
+    if ((rc = run_distances(ctx, nconf, evec != nullptr))) return rc;
+    long tot = (long)nconf * ctx->npairs;
+    if (evec) {
+        CHM_HIP(hipMemcpyAsync(evec, ctx->evec.p, sizeof(double) * 3 * tot,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    }
+    return download(ctx, r, ctx->r, sizeof(double) * tot);
+}
+
+extern "C" int chm_fm_ptypes(chm_ctx *ctx, int nconf, const int32_t *species, int32_t *pt) {
+    CHM_REQUIRE(ctx && ctx->npairs > 0, CHM_E_STATE, "set topology first");
+    CHM_REQUIRE(nconf > 0 && species && pt, CHM_E_ARG, "bad arguments");
+    int rc = check_species(ctx, nconf, species);
+    if (rc) return rc;
+    CHM_HIP(hipSetDevice(ctx->device));
+    if ((rc = upload(ctx, ctx->species, species, sizeof(int32_t) * (size_t)nconf * ctx->natoms)))
+        return rc;
+    long tot = (long)nconf * ctx->npairs;
+    if ((rc = ctx->ptype.ensure(sizeof(int32_t) * tot))) return rc;
+    k_ptypes<<<dim3((unsigned)((tot + 255) / 256)), 256, 0, ctx->stream>>>(
+        ctx->species.as<int>(), ctx->i_idx.as<int>(), ctx->j_idx.as<int>(), ctx->ptype.as<int>(),
+        ctx->npairs, ctx->natoms, nconf, ctx->nsp);
+    CHM_LAUNCH_CHECK();
+    return download(ctx, pt, ctx->ptype, sizeof(int32_t) * tot);
+}
+
+// ------------------------------------------------------------------ path sum
+
+namespace {
+
+constexpr int kTile = 256;   // pairs staged per LDS tile == block size
+constexpr int kKq = 2;       // k points per lane: nk <= 512
+constexpr double kInvPi = 0.31830988618379067154;
+constexpr double kTwoPi = 6.28318530717958647692;
+
+struct PairRec {
+    double r;     // r_eff = r + dr[shell][type]
+    double ir2;   // 1 / r_eff^2
+    double s2;    // sigma^2 + s2tab[shell][type]
+    double c3;    // c3tab[shell]
+    int t;        // pair type
+    int sa;       // absorber species
+};
+
+// Stage a tile of pairs: lane q loads pair p0 + q and precomputes its k-independent
+// parts.
+__device__ inline void stage_tile(PairRec *tile, int p0, int npairs, int nsp,
+                                  const double *__restrict__ r, const int *__restrict__ pt,
+                                  const int *__restrict__ shell, const double *__restrict__ sig2,
+                                  const double *__restrict__ drtab,
+                                  const double *__restrict__ c3tab,
+                                  const double *__restrict__ s2tab) {
+    int q = threadIdx.x;
+    int p = p0 + q;
+    PairRec rec;
+    if (p < npairs) {
+        int t = pt[p];
+        int sh = shell[p];
+        double re = r[p] + drtab[sh * nsp * nsp + t];
+        rec.r = re;
+        rec.ir2 = 1.0 / (re * re);
+        rec.s2 = sig2[p] + s2tab[sh * nsp * nsp + t];
+        rec.c3 = c3tab[sh];
+        rec.t = t;
+        rec.sa = t / nsp;
+    } else {
+        rec.r = 1.0;
+        rec.ir2 = 0.0;  // contributes exactly 0
+        rec.s2 = 0.0;
+        rec.c3 = 0.0;
+        rec.t = 0;
+        rec.sa = 0;
+    }
+    tile[q] = rec;
+}
+
+template <int NSP, bool F32>
+__global__ __launch_bounds__(256) void k_chi_pairs(
+    const double *__restrict__ r_all, const int *__restrict__ pt_all,
+    const int *__restrict__ shell, const double *__restrict__ sig2,
+    const double *__restrict__ ftab, const double *__restrict__ dtab,
+    const double *__restrict__ lamtab, const double *__restrict__ ktab,
+    const double *__restrict__ drtab, const double *__restrict__ c3tab,
+    const double *__restrict__ s2tab, double *__restrict__ partial, int npairs, int nk,
+    int pairs_per_wg, double s02) {
+    extern __shared__ double smem[];
+    // LDS: f (NSP, nk), delta (NSP*NSP, nk), pair tile
+    double *sf = smem;
+    double *sd = sf + NSP * nk;
+    PairRec *tile = reinterpret_cast<PairRec *>(sd + NSP * NSP * nk);
+
+    const int ic = blockIdx.y;
+    const int w = blockIdx.x;
+    const int nwg = gridDim.x;
+    for (int q = threadIdx.x; q < NSP * nk; q += blockDim.x) sf[q] = ftab[q];
+    for (int q = threadIdx.x; q < NSP * NSP * nk; q += blockDim.x) sd[q] = dtab[q];
+
+    // per-lane k constants
+    double kk[kKq], c1[kKq], c2[kKq], pref[kKq], c3k[kKq];
+    bool live[kKq];
+#pragma unroll
+    for (int u = 0; u < kKq; ++u) {
+        int ik = threadIdx.x + u * blockDim.x;
+        live[u] = ik < nk;
+        double kv = live[u] ? ktab[ik] : 0.0;
+        double lv = live[u] ? lamtab[ik] : 1.0;
+        kk[u] = kv;
+        c1[u] = -2.0 / lv;
+        c2[u] = -2.0 * kv * kv;
+        pref[u] = (kv > 0.0) ? s02 / kv : 0.0;
+        c3k[u] = (4.0 / 3.0) * kv * kv * kv;
+    }
+    double acc[kKq][NSP];
+#pragma unroll
+    for (int u = 0; u < kKq; ++u)
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) acc[u][s] = 0.0;
+
+    const double *rc = r_all + (long)ic * npairs;
+    const int *tc = pt_all + (long)ic * npairs;
+    const int pbeg = w * pairs_per_wg;
+    const int pend = min(npairs, pbeg + pairs_per_wg);
+
+    for (int p0 = pbeg; p0 < pend; p0 += kTile) {
+        __syncthreads();
+        stage_tile(tile, p0, pend, NSP, rc, tc, shell, sig2, drtab, c3tab, s2tab);
+        __syncthreads();
+        const int nt = min(kTile, pend - p0);
+        for (int q = 0; q < nt; ++q) {
+            const PairRec rec = tile[q];
+#pragma unroll
+            for (int u = 0; u < kKq; ++u) {
+                if (!live[u]) continue;
+                const int ik = threadIdx.x + u * blockDim.x;
+                const int sb = rec.t - rec.sa * NSP;
+                double v;
+                if (F32) {
+                    // phase reduced in FP64 (revolutions), everything else FP32
+                    double rev = kk[u] * kInvPi * rec.r;
+                    double fr = rev - rint(rev);
+                    float th = (float)(fr * kTwoPi) + (float)sd[rec.t * nk + ik] -
+                               (float)c3k[u] * (float)rec.c3;
+                    float e = expf((float)c1[u] * (float)rec.r + (float)c2[u] * (float)rec.s2);
+                    v = (double)((float)pref[u] * (float)sf[sb * nk + ik] * (float)rec.ir2 * e *
+                                 sinf(th));
+                } else {
+                    double e = exp(c1[u] * rec.r + c2[u] * rec.s2);
+                    double th = 2.0 * kk[u] * rec.r + sd[rec.t * nk + ik] - c3k[u] * rec.c3;
+                    v = pref[u] * sf[sb * nk + ik] * rec.ir2 * e * sin(th);
+                }
+#pragma unroll
+                for (int s = 0; s < NSP; ++s)
+                    if (rec.sa == s) acc[u][s] += v;
+            }
+        }
+    }
+    double *out = partial + ((long)ic * nwg + w) * NSP * nk;
+#pragma unroll
+    for (int u = 0; u < kKq; ++u) {
+        int ik = threadIdx.x + u * blockDim.x;
+        if (!live[u]) continue;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) out[s * nk + ik] = acc[u][s];
+    }
+}
+
+// chi[ic][s][k] = sum over workgroups w (fixed order) of partial[ic][w][s][k]
+__global__ void k_reduce_partials(const double *__restrict__ partial, double *__restrict__ chi,
+                                  int nwg, int n_per_conf, int nconf) {
+    long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long)n_per_conf * nconf) return;
+    int ic = (int)(tid / n_per_conf), q = (int)(tid % n_per_conf);
+    const double *p = partial + (long)ic * nwg * n_per_conf + q;
+    double s = 0.0;
+    for (int w = 0; w < nwg; ++w) s += p[(long)w * n_per_conf];
+    chi[tid] = s;
+}
+
+template <int NSP>
+int launch_chi(chm_ctx *ctx, int nconf, int precision) {
+    const int nk = ctx->nk;
+    const int npairs = ctx->npairs;
+    int ntiles = (npairs + kTile - 1) / kTile;
+    int want = (2048 + nconf - 1) / nconf;  // ~2048 workgroups over the device
+    int nwg = std::max(1, std::min(ntiles, want));
+    int tiles_per = (ntiles + nwg - 1) / nwg;
+    int ppw = tiles_per * kTile;
+    nwg = (npairs + ppw - 1) / ppw;
+    size_t lds = sizeof(double) * (NSP + NSP * NSP) * nk + sizeof(PairRec) * kTile;
+    CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+    int rc = ctx->partial.ensure(sizeof(double) * (size_t)nconf * nwg * NSP * nk);
+    if (rc) return rc;
+    dim3 grid(nwg, nconf);
+    if (precision == CHM_PREC_F32) {
+        hipFuncSetAttribute((const void *)k_chi_pairs<NSP, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k_chi_pairs<NSP, true><<<grid, 256, lds, ctx->stream>>>(
+            ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(),
+            ctx->sig2.as<double>(), ctx->f.as<double>(), ctx->delta.as<double>(),
+            ctx->lam.as<double>(), ctx->k.as<double>(), ctx->drtab.as<double>(),
+            ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), ctx->partial.as<double>(), npairs,
+            nk, ppw, ctx->s02);
+    } else {
+        hipFuncSetAttribute((const void *)k_chi_pairs<NSP, false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k_chi_pairs<NSP, false><<<grid, 256, lds, ctx->stream>>>(
+            ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(),
+            ctx->sig2.as<double>(), ctx->f.as<double>(), ctx->delta.as<double>(),
+            ctx->lam.as<double>(), ctx->k.as<double>(), ctx->drtab.as<double>(),
+            ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), ctx->partial.as<double>(), npairs,
+            nk, ppw, ctx->s02);
+    }
+    CHM_LAUNCH_CHECK();
+    long n = (long)nconf * NSP * nk;
+    if ((rc = ctx->chi.ensure(sizeof(double) * n))) return rc;
+    k_reduce_partials<<<dim3((unsigned)((n + 255) / 256)), 256, 0, ctx->stream>>>(
+        ctx->partial.as<double>(), ctx->chi.as<double>(), nwg, NSP * nk, nconf);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+}  // namespace
+
+namespace chm {
+// Path sum over the context's pos/species buffers already on the device.
+int chi_device(chm_ctx *ctx, int nconf, int precision) {
+    int rc = run_distances(ctx, nconf, false);
+    if (rc) return rc;
+    long tot = (long)nconf * ctx->npairs;
+    if ((rc = ctx->ptype.ensure(sizeof(int32_t) * tot))) return rc;
+    k_ptypes<<<dim3((unsigned)((tot + 255) / 256)), 256, 0, ctx->stream>>>(
+        ctx->species.as<int>(), ctx->i_idx.as<int>(), ctx->j_idx.as<int>(), ctx->ptype.as<int>(),
+        ctx->npairs, ctx->natoms, nconf, ctx->nsp);
+    CHM_LAUNCH_CHECK();
+    switch (ctx->nsp) {
+        case 2: return launch_chi<2>(ctx, nconf, precision);
+        case 3: return launch_chi<3>(ctx, nconf, precision);
+        case 4: return launch_chi<4>(ctx, nconf, precision);
+        case 5: return launch_chi<5>(ctx, nconf, precision);
+        case 6: return launch_chi<6>(ctx, nconf, precision);
+    }
+    set_error("unsupported species count");
+    return CHM_E_ARG;
+}
+}  // namespace chm
+
+extern "C" int chm_fm_chi(chm_ctx *ctx, int nconf, const double *pos, const int32_t *species,
+                          double *chi, int precision) {
+    CHM_REQUIRE(ctx && ctx->npairs > 0, CHM_E_STATE, "set tables and topology first");
+    CHM_REQUIRE(nconf > 0 && pos && species && chi, CHM_E_ARG, "bad arguments");
+    CHM_REQUIRE(precision == CHM_PREC_F64 || precision == CHM_PREC_F32, CHM_E_ARG,
+                "precision must be CHM_PREC_F64 or CHM_PREC_F32");
+    CHM_REQUIRE(ctx->nk <= 256 * kKq, CHM_E_ARG, "at most 512 k points");
+    int rc = check_species(ctx, nconf, species);
+    if (rc) return rc;
+    CHM_HIP(hipSetDevice(ctx->device));
+    if ((rc = upload(ctx, ctx->pos, pos, sizeof(double) * 3 * (size_t)nconf * ctx->natoms)))
+        return rc;
+    if ((rc = upload(ctx, ctx->species, species, sizeof(int32_t) * (size_t)nconf * ctx->natoms)))
+        return rc;
+    if ((rc = chm::chi_device(ctx, nconf, precision))) return rc;
+    return download(ctx, chi, ctx->chi, sizeof(double) * (size_t)nconf * ctx->nsp * ctx->nk);
+}
+
+// ------------------------------------------------------------------ term rows
+
+__global__ void k_term_rows(const double *__restrict__ r, const double *__restrict__ s2,
+                            const double *__restrict__ c3, const int *__restrict__ absorber,
+                            const int *__restrict__ nev, const int *__restrict__ ev,
+                            const double *__restrict__ ftab, const double *__restrict__ dtab,
+                            const double *__restrict__ lamtab, const double *__restrict__ ktab,
+                            double *__restrict__ out, int nrows, int nk, int nsp, double s02,
+                            double amp) {
+    long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long)nrows * nk) return;
+    int q = (int)(tid / nk), ik = (int)(tid % nk);
+    double kv = ktab[ik];
+    if (!(kv > 0.0)) {
+        out[tid] = 0.0;
+        return;
+    }
+    int a = absorber[q], n = nev[q];
+    double F = 1.0, D = 0.0;
+    for (int m = 0; m < n; ++m) {
+        int b = ev[3 * q + m];
+        F *= ftab[b * nk + ik];
+        D += dtab[(a * nsp + b) * nk + ik];
+    }
+    D /= (double)n;
+    double rr = r[q];
+    double v = amp * s02 * F / (kv * rr * rr) *
+               exp(-2.0 * rr / lamtab[ik] - 2.0 * kv * kv * s2[q]) *
+               sin(2.0 * kv * rr + D - (4.0 / 3.0) * kv * kv * kv * c3[q]);
+    out[tid] = v;
+}
+
+extern "C" int chm_fm_term_rows(chm_ctx *ctx, int nrows, const double *r, const double *s2,
+                                const double *c3, const int32_t *absorber, const int32_t *nev,
+                                const int32_t *ev, double amp, double *out) {
+    CHM_REQUIRE(ctx && ctx->nsp > 0, CHM_E_STATE, "set tables first");
+    CHM_REQUIRE(nrows > 0 && r && s2 && c3 && absorber && nev && ev && out, CHM_E_ARG,
+                "bad arguments");
+    for (int q = 0; q < nrows; ++q) {
+        CHM_REQUIRE(absorber[q] >= 0 && absorber[q] < ctx->nsp, CHM_E_ARG, "bad absorber");
+        CHM_REQUIRE(nev[q] >= 1 && nev[q] <= 3, CHM_E_ARG, "1..3 scattering events per row");
+        for (int m = 0; m < nev[q]; ++m)
+            CHM_REQUIRE(ev[3 * q + m] >= 0 && ev[3 * q + m] < ctx->nsp, CHM_E_ARG,
+                        "bad scatterer species");
+        CHM_REQUIRE(r[q] > 0.0, CHM_E_ARG, "row radius must be positive");
+    }
+    CHM_HIP(hipSetDevice(ctx->device));
+    chm::DevBuf dr, ds2, dc3, da, dn, dev;
+    int rc;
+    if ((rc = upload(ctx, dr, r, sizeof(double) * nrows))) return rc;
+    if ((rc = upload(ctx, ds2, s2, sizeof(double) * nrows))) return rc;
+    if ((rc = upload(ctx, dc3, c3, sizeof(double) * nrows))) return rc;
+    if ((rc = upload(ctx, da, absorber, sizeof(int32_t) * nrows))) return rc;
+    if ((rc = upload(ctx, dn, nev, sizeof(int32_t) * nrows))) return rc;
+    if ((rc = upload(ctx, dev, ev, sizeof(int32_t) * 3 * nrows))) return rc;
+    long tot = (long)nrows * ctx->nk;
+    if ((rc = ctx->rows_out.ensure(sizeof(double) * tot))) return rc;
+    k_term_rows<<<dim3((unsigned)((tot + 255) / 256)), 256, 0, ctx->stream>>>(
+        dr.as<double>(), ds2.as<double>(), dc3.as<double>(), da.as<int>(), dn.as<int>(),
+        dev.as<int>(), ctx->f.as<double>(), ctx->delta.as<double>(), ctx->lam.as<double>(),
+        ctx->k.as<double>(), ctx->rows_out.as<double>(), nrows, ctx->nk, ctx->nsp, ctx->s02, amp);
+    CHM_LAUNCH_CHECK();
+    return download(ctx, out, ctx->rows_out, sizeof(double) * tot);
+}

@@ -1,0 +1,464 @@
+"""RMCSampler: parallel-tempering reverse Monte Carlo swap sampler, device-resident.
+
+Drop-in for methods/method_a_rmc.py:63-496 for the reference's swap-only use
+(chimera/fitrun.py:307-315 passes swap_frac=1.0): same constructor, init_state / step /
+run / replica_exchange / best / scales / collect_rungs / posterior_wc, and the result
+attributes samples, history, walltime, exchange_rate, best_verified, chi2, species,
+beta, chi.  Arrays come back as NumPy in rung order, as the reference holds them.
+
+What runs where: every step (proposal, gather of the pairs and paths through the two
+swapped sites, trial chi(R), profiled chi^2, Metropolis, commit) and every replica
+exchange run on the GPU (libchimera_hip: K1-K4, replayed from a captured hipGraph);
+the host only launches and reads back at log / collection points.
+
+Count space: on the ideal lattice every pair term is B[a,b,shell](k) and every
+multiple-scattering path term is T_entry(k), so a configuration's chi(R) is its
+integer counts times the transformed basis.  The basis is built on the device (term
+rows, then the DFT-matrix transform of each edge) when init_state fixes the
+composition (the per-edge normalisation by absorber count).
+
+Decisions (recorded, the reference's own choice cannot be replayed without CuPy):
+  * when all 8 candidates repeat a's species the proposal is a null move (the
+    reference falls back to a displacement, kernels.py:403-421);
+  * acceptance and exchange uniforms are counter-based splitmix64 streams keyed by
+    (seed, step, rung) and (seed, exchange number);
+  * exchange swaps temperatures between slots, not configurations.
+"""
+from __future__ import annotations
+
+import time
+
+import sys
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, f64, i32
+from .lattice import alpha_from_counts
+
+
+def _beta_ladder(tmin, tmax, nrep):
+    return 1.0 / np.geomspace(tmin, tmax, nrep)
+
+
+class RMCSampler:
+    def __init__(self, fm, ft, target, mask, sigma_R, nrep=48, tmin=0.02, tmax=5.0, seed=0,
+                 swap_frac=0.75, disp_sigma=0.015, max_disp=0.30, nsp=None,
+                 collect_after=None, collect_every=25, adapt_every=0, target_accept=0.35,
+                 profile_scale=True, collect_beta=0.5, n_collect=4, energy_prior=None,
+                 ms_library=None):
+        if swap_frac != 1.0:
+            raise ValueError("the device sampler makes swap moves only: pass swap_frac=1.0 "
+                             "(the reference's fit driver does, chimera/fitrun.py:218)")
+        if energy_prior is not None:
+            raise ValueError("energy_prior is not supported by the device sampler")
+        if adapt_every:
+            raise ValueError("ladder adaptation (adapt_every) is not supported")
+        self.fm, self.ft = fm, ft
+        self.nsp = int(nsp) if nsp is not None else int(fm.nsp)
+        self.nrep = int(nrep)
+        self.collect_after = collect_after
+        self.collect_every = int(collect_every)
+        self.collect_beta = None if collect_beta is None else float(collect_beta)
+        self.n_collect = int(n_collect)
+        self.profile_scale = bool(profile_scale)
+        self.samples = []
+        self.seed = int(seed)
+        self.swap_frac = swap_frac
+        self.natoms = fm.natoms
+        self.npairs = fm.npairs
+        self.nk = fm.nk
+        self.ms = ms_library
+
+        i_np, j_np = np.asarray(fm.i_idx), np.asarray(fm.j_idx)
+        counts = np.bincount(i_np, minlength=self.natoms)
+        assert counts.min() == counts.max(), "expected uniform coordination"
+        assert np.all(np.diff(i_np) >= 0), "pairs must be i-sorted"
+        self.ncoord = int(counts[0])
+        key = i_np.astype(np.int64) * self.natoms + j_np
+        rkey = j_np.astype(np.int64) * self.natoms + i_np
+        assert np.array_equal(np.sort(key), np.sort(rkey)), "neighbour list is not symmetric"
+        self._beta_rung = _beta_ladder(tmin, tmax, self.nrep)
+
+        self.complex = bool(np.iscomplexobj(target))
+        if not self.complex:
+            raise ValueError("the device sampler needs a complex chi(R) target (chi(k) data)")
+        target = np.asarray(target)
+        self.mask = np.asarray(mask, bool)
+        self.nchan = int(target.shape[0])
+        self.nkw = self.nchan // self.nsp
+        assert self.nkw * self.nsp == self.nchan, \
+            f"{self.nchan} channels do not divide into {self.nsp} edges"
+        chan_edge = getattr(ft, "chan_edge", None)
+        if chan_edge is not None:
+            assert np.array_equal(np.asarray(chan_edge), np.repeat(np.arange(self.nsp), self.nkw)), \
+                "channels must be ordered edge-major"
+        self.nwin = int(self.mask.sum())
+        self.nfit = self.nwin * self.nchan * 2
+        self._tm = target[:, self.mask]
+        self._sm = np.asarray(sigma_R)[:, self.mask]
+        self.exch_attempts = np.zeros(max(self.nrep - 1, 1))
+        self.exch_accepted = np.zeros(max(self.nrep - 1, 1))
+        self._S = None
+        self._comp = None
+        self._log_cap = 0
+
+    # ------------------------------------------------------------------ basis
+    def _lattice_geometry(self):
+        cell = self.fm.cell
+        a0 = float(cell.a0)
+        unit = a0 / 2.0
+        X = np.asarray(cell.ideal, float) / unit
+        Xi = np.rint(X).astype(np.int64)
+        assert np.allclose(X, Xi, atol=1e-6), "sites are not lattice points"
+        ncell = np.array(cell.ncell, np.int64)
+        box = 2 * ncell
+        Xi %= box
+        par = (Xi[:, 0] & 1) | ((Xi[:, 1] & 1) << 1) | ((Xi[:, 2] & 1) << 2)
+        nbasis = self.natoms // int(np.prod(ncell))
+        bpar = -np.ones(8, np.int32)
+        cellidx = ((Xi[:, 0] >> 1) * ncell[1] + (Xi[:, 1] >> 1)) * ncell[2] + (Xi[:, 2] >> 1)
+        b = np.arange(self.natoms) - cellidx * nbasis
+        for p_, b_ in zip(par, b):
+            if bpar[p_] < 0:
+                bpar[p_] = b_
+            assert bpar[p_] == b_, "site order is not cell-major with a fixed basis order"
+        xyz = (Xi[:, 0] | (Xi[:, 1] << 10) | (Xi[:, 2] << 20)).astype(np.int32)
+        return xyz, box, ncell, nbasis, bpar
+
+    def _shells(self):
+        fm = self.fm
+        d = np.linalg.norm((fm.cell.ideal[fm.j_idx] + fm.d_shift) - fm.cell.ideal[fm.i_idx], axis=1)
+        sh = np.asarray(fm.shell)
+        radii, s2 = [], []
+        for s in range(fm.nshell):
+            m = sh == s
+            r_s = d[m]
+            if r_s.size == 0:
+                radii.append(1.0)
+                s2.append(0.0)
+                continue
+            if np.ptp(r_s) > 1e-9 * r_s.mean():
+                raise ValueError("pairs of one shell differ in length: the count-space sampler "
+                                 "needs the ideal lattice with one radius per shell")
+            sg = np.asarray(fm.d_sig2)[m]
+            if np.ptp(sg) > 1e-12 * max(abs(sg.mean()), 1e-30):
+                raise ValueError("sigma^2 differs within a shell")
+            radii.append(float(r_s.mean()))
+            s2.append(float(sg.mean()))
+        return np.array(radii), np.array(s2)
+
+    def _build(self, species0):
+        fm = self.fm
+        nsp, nk = self.nsp, self.nk
+        natoms_of = np.bincount(species0, minlength=nsp)
+        self._comp = natoms_of
+        radii, s2 = self._shells()
+        B = fm.ss_basis(radii, s2)                         # (nsp, nsp, nshell, nk)
+        nshell = fm.nshell
+        if self.ms is not None:
+            r, ms2, ab, nev, ev = self.ms.table_rows()
+            T = fm.term_rows(r, ms2, np.zeros(r.size), ab, nev, ev, amp=self.ms.amp)
+            nblk = self.ms.nblk
+        else:
+            T, nblk = None, 0
+        extra = getattr(self.ft, "chi_extra", None)
+        has_const = extra is not None
+        nrow_e = nsp * nshell + nblk + (1 if has_const else 0)
+        dev = self.ft.device() if hasattr(self.ft, "device") else None
+        if dev is None:
+            raise ValueError("ft must be a chimera_amd MultiEdgeFT")
+        ncol_e = self.nkw * self.nwin * 2
+        basis_R = np.zeros((nsp, nrow_e, ncol_e))
+        basis_k = np.zeros((nsp, nrow_e, nk))
+        for e in range(nsp):
+            rows = [B[e].reshape(nsp * nshell, nk)]
+            if T is not None:
+                rows.append(T[e * nblk:(e + 1) * nblk])
+            rk = np.concatenate(rows, 0)
+            basis_k[e, :rk.shape[0]] = rk
+            scaled = rk / max(natoms_of[e], 1)
+            if has_const:
+                scaled = np.concatenate([scaled, np.asarray(extra)[e][None]], 0)
+            outR = dev.apply_edge(e, scaled)                # (nrow_e, nkw, nR, 2)
+            basis_R[e] = outR[:, :, self.mask, :].reshape(nrow_e, ncol_e)
+        self._basis_k = basis_k
+        # geometry
+        xyz, box, ncell, nbasis, bpar = self._lattice_geometry()
+        g = _lib.Geom()
+        g.natoms, g.nsp, g.ncoord, g.nshell = self.natoms, nsp, self.ncoord, nshell
+        g.nblk = nblk
+        if self.ms is not None:
+            stencil = np.ascontiguousarray(self.ms.stencil, np.int8)
+            g.nstencil, g.nabsrec = self.ms.nstencil, self.ms.nabsrec
+        else:
+            stencil = None
+            g.nstencil = g.nabsrec = 0
+        g.nkw, g.nwin = self.nkw, self.nwin
+        for d in range(3):
+            g.box[d] = int(box[d])
+            g.ncell[d] = int(ncell[d])
+        g.nbasis = int(nbasis)
+        for q in range(8):
+            g.basis_of_parity[q] = int(bpar[q])
+        g.has_const = 1 if has_const else 0
+        tgt = np.stack([self._tm.real, self._tm.imag], -1).reshape(-1)
+        nbr_j = i32(np.asarray(fm.j_idx).reshape(self.natoms, self.ncoord))
+        nbr_sh = i32(np.asarray(fm.shell).reshape(self.natoms, self.ncoord))
+        h = _lib.vp()
+        check(_lib.load().chm_sampler_create(
+            fm.ctx.h, _lib.C.byref(g), self.nrep, nk, xyz, nbr_j, nbr_sh,
+            None if stencil is None else stencil.ctypes.data, f64(basis_R), f64(basis_k),
+            f64(tgt), f64(self._sm.reshape(-1)), int(self.profile_scale), _lib.C.byref(h)))
+        self._S = h
+        self._stencil_keep = stencil
+        self.nrow_e = nrow_e
+        self.nshell = nshell
+
+    # ------------------------------------------------------------------ state
+    def init_state(self, species0, pos0):
+        species0 = i32(species0)
+        pos0 = np.asarray(pos0, float)
+        if not np.allclose(pos0, self.fm.cell.ideal, atol=1e-9):
+            raise ValueError("the device sampler starts from the ideal lattice positions")
+        comp = np.bincount(species0, minlength=self.nsp)
+        if self._S is None or not np.array_equal(comp, self._comp):
+            self._destroy()
+            self._build(species0)
+            if self._log_cap:
+                self.set_recording(self._log_cap, self._series_cap)
+        check(_lib.load().chm_sampler_init(self._S, species0, f64(self._beta_rung),
+                                           int(self.seed) & ((1 << 64) - 1)))
+        self.samples = []
+        self._t_run = 0.0
+
+    def set_recording(self, log_steps=0, series_steps=0, series_rung=None):
+        """Keep the move log (a, b, accepted, rung per step and replica) and the
+        pair-count series of one rung (default: nearest collect_beta) on the device."""
+        self._log_cap, self._series_cap = int(log_steps), int(series_steps)
+        if series_rung is None:
+            series_rung = int(self.collect_rungs()[0]) if series_steps else -1
+        self._series_rung = series_rung
+        if self._S is not None:
+            check(_lib.load().chm_sampler_set_recording(self._S, self._log_cap, self._series_cap,
+                                                        int(series_rung)))
+
+    def _read(self, what, shape, dtype):
+        out = np.empty(shape, dtype)
+        check(_lib.load().chm_sampler_read(self._S, what, out.ctypes.data, out.nbytes))
+        return out
+
+    # ------------------------------------------------------------- properties
+    @property
+    def species(self):
+        return self._read(0, (self.nrep, self.natoms), np.int32)
+
+    @property
+    def chi2(self):
+        return self._read(1, (self.nrep,), np.float64)
+
+    @property
+    def best_chi2(self):
+        return self._read(3, (self.nrep,), np.float64)
+
+    @property
+    def counts(self):
+        return self._read(2, (self.nrep, self.nsp * self.nrow_e), np.int32)
+
+    @property
+    def chi(self):
+        """Unnormalised chi(k) (nrep, nsp, nk), rung order."""
+        return self._read(11, (self.nrep, self.nsp, self.nk), np.float64)
+
+    @property
+    def beta(self):
+        return self._beta_rung.copy()
+
+    @property
+    def count(self):
+        return np.repeat(self._comp[None].astype(float), self.nrep, 0)
+
+    def move_log(self, nsteps):
+        return self._read(6, (self._log_cap, self.nrep, 4), np.int32)[:nsteps]
+
+    def pair_series(self, nsteps):
+        nss = self.nsp * self.nsp * self.nshell
+        return self._read(7, (self._series_cap, nss), np.int32)[:nsteps].reshape(
+            -1, self.nsp, self.nsp, self.nshell)
+
+    def accepted_per_rung(self):
+        return self._read(9, (self.nrep,), np.int64)
+
+    # ------------------------------------------------------------------ moves
+    def step(self):
+        """One swap step for every replica; returns the accepted flags (rung order)."""
+        before = self.accepted_per_rung()
+        check(_lib.load().chm_sampler_run(self._S, 1, 0))
+        return (self.accepted_per_rung() - before) > 0
+
+    def replica_exchange(self, return_rates=False):
+        """One exchange sweep on the device (method_a_rmc.py:332-373).  Returns the
+        number of swaps (and per-pair 0/1 rates of this sweep, NaN where not tried)."""
+        m = max(self.nrep - 1, 1)
+        ea0 = self._read(8, (2, m), np.int64)
+        check(_lib.load().chm_sampler_exchange(self._S))
+        ea1 = self._read(8, (2, m), np.int64)
+        tried = (ea1[0] - ea0[0]) > 0
+        ok = (ea1[1] - ea0[1]) > 0
+        nsw = int(ok.sum())
+        if return_rates:
+            rates = np.where(tried, ok.astype(float), np.nan)
+            filled = np.where(np.isnan(rates), np.nanmean(rates) if tried.any() else 0.3, rates)
+            return nsw, filled
+        return nsw
+
+    def run(self, nsteps, exchange_every=25, log_every=500, callback=None):
+        """Run the chain on the device (method_a_rmc.py:401-455)."""
+        hist = []
+        t0 = time.time()
+        lib = _lib.load()
+        acc0 = self.accepted_per_rung().sum()
+        it = 0
+        marks = set()
+        if self.collect_after is not None:
+            marks.update(s for s in range(int(self.collect_after), nsteps)
+                         if (s + 1) % self.collect_every == 0)
+        marks.update(s for s in range(nsteps) if (s + 1) % log_every == 0 or s == 0)
+        marks = sorted(marks)
+        for m in marks:
+            n = m + 1 - it
+            if n > 0:
+                check(lib.chm_sampler_run(self._S, n, int(exchange_every)))
+                it = m + 1
+            if (self.collect_after is not None and m >= self.collect_after
+                    and (m + 1) % self.collect_every == 0):
+                self._collect()
+            if (m + 1) % log_every == 0 or m == 0:
+                best = float(self.best_chi2.min())
+                acc = (self.accepted_per_rung().sum() - acc0) / ((m + 1) * self.nrep)
+                hist.append((m + 1, best, float(acc), time.time() - t0))
+                if callback:
+                    callback(m + 1, best, hist[-1][2])
+        if it < nsteps:
+            check(lib.chm_sampler_run(self._S, nsteps - it, int(exchange_every)))
+        self.walltime = time.time() - t0
+        self.history = hist
+        ea = self._read(8, (2, max(self.nrep - 1, 1)), np.int64)
+        self.exch_attempts, self.exch_accepted = ea[0].astype(float), ea[1].astype(float)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            self.exchange_rate = np.where(self.exch_attempts > 0,
+                                          self.exch_accepted / np.maximum(self.exch_attempts, 1),
+                                          np.nan)
+        self.best_verified = self.verify_best()
+        return hist
+
+    def verify_best(self):
+        """chi^2 of the retained best, recomputed from scratch: pair-space FP64 path sum
+        + multiple-scattering counts + transform (method_a_rmc.py:439-448)."""
+        bsp, _, _ = self.best()
+        return float(self.chi2_of(bsp[None])[0])
+
+    def chi_k_of(self, species):
+        """Unnormalised chi(k) (nconf, nsp, nk) of configurations from scratch on the
+        device: pair-space path sum plus multiple scattering from the entry counts."""
+        species = i32(species).reshape(-1, self.natoms)
+        n = species.shape[0]
+        pos = np.repeat(self.fm.cell.ideal[None], n, 0)
+        chi = self.fm.chi(pos, species, normalise=False)
+        if self.ms is not None:
+            C = np.empty((n, self.nsp * self.nrow_e), np.int32)
+            check(_lib.load().chm_sampler_counts(self._S, n, species, C))
+            C.reshape(n, self.nsp, self.nrow_e)[:, :, :self.nsp * self.nshell] = 0
+            chi_ms = np.empty((n, self.nsp, self.nk))
+            check(_lib.load().chm_sampler_chik(self._S, n, C, chi_ms))
+            chi += chi_ms
+        return chi
+
+    def counts_of(self, species):
+        """Basis-row counts (nconf, nsp*nrow_e) of configurations, on the device."""
+        species = i32(species).reshape(-1, self.natoms)
+        C = np.empty((species.shape[0], self.nsp * self.nrow_e), np.int32)
+        check(_lib.load().chm_sampler_counts(self._S, species.shape[0], species, C))
+        return C
+
+    def set_target(self, target, sigma_R):
+        """Replace the target and sigma (same channels and window); the state is reset
+        by the next init_state."""
+        target = np.asarray(target)
+        assert target.shape[0] == self.nchan
+        self._tm = target[:, self.mask]
+        self._sm = np.asarray(sigma_R)[:, self.mask]
+        self._destroy()
+        self._comp = None
+
+    def chi2_of(self, species):
+        """Reduced chi^2 of configurations (nconf, natoms), from scratch on the device."""
+        species = i32(species).reshape(-1, self.natoms)
+        n = species.shape[0]
+        chi = self.chi_k_of(species)
+        cnt = np.stack([(species == s).sum(1) for s in range(self.nsp)], 1)
+        chi = chi / np.maximum(cnt, 1)[:, :, None]
+        z = self.ft.to_R(chi)[:, :, self.mask]               # (n, nchan, nwin)
+        zz = np.ascontiguousarray(np.stack([z.real, z.imag], -1).reshape(n, -1))
+        out = np.empty(n)
+        check(_lib.load().chm_sampler_chi2(self._S, n, zz, out))
+        return out
+
+    def scales(self, chi_un=None):
+        """Profiled per-edge amplitudes (nrep, nchan) of the current states."""
+        chi = self.chi / np.maximum(self.count, 1.0)[:, :, None]
+        m = self.ft.to_R(chi)[:, :, self.mask]
+        num = (m.real * self._tm.real[None] + m.imag * self._tm.imag[None]).sum(-1)
+        den = (m.real ** 2 + m.imag ** 2).sum(-1)
+        num = np.repeat(num.reshape(-1, self.nsp, self.nkw).sum(-1), self.nkw, 1)
+        den = np.repeat(den.reshape(-1, self.nsp, self.nkw).sum(-1), self.nkw, 1)
+        return num / np.maximum(den, 1e-30)
+
+    # ------------------------------------------------------------- posterior
+    def collect_rungs(self):
+        beta = self._beta_rung
+        k = min(self.n_collect, self.nrep)
+        if self.collect_beta is None:
+            return np.argsort(beta)[::-1][:k]
+        return np.argsort(np.abs(np.log(beta) - np.log(self.collect_beta)))[:k]
+
+    def _collect(self):
+        idx = self.collect_rungs()
+        sp = self.species[idx]
+        c2 = self.chi2[idx]
+        for a, b in zip(sp, c2):
+            self.samples.append((a.astype(np.int32), float(b)))
+
+    def posterior_wc(self, shell=0):
+        """Warren-Cowley alpha (upper triangle) of shell `shell` over the collected
+        samples: (mean, std, n) (method_a_rmc.py:477-491)."""
+        from .lattice import warren_cowley
+        if not self.samples:
+            raise RuntimeError("no samples collected; set collect_after in the constructor")
+        n = self.nsp
+        arr = np.array([warren_cowley(sp, self.fm.i_idx, self.fm.j_idx, self.fm.shell, shell,
+                                      n)[np.triu_indices(n)] for sp, _ in self.samples])
+        return arr.mean(0), arr.std(0), len(arr)
+
+    def alpha_series(self, nsteps, shell=0):
+        """(nsteps, nsp, nsp) Warren-Cowley alpha of the recorded rung, from its pair
+        counts after every step."""
+        N = self.pair_series(nsteps)[..., shell].astype(float)
+        return np.array([alpha_from_counts(Nt, self._comp) for Nt in N])
+
+    def best(self):
+        bc = self.best_chi2
+        b = int(bc.argmin())
+        sp = self._read(4, (self.nrep, self.natoms), np.int32)[b]
+        return sp, np.asarray(self.fm.cell.ideal).copy(), float(bc[b])
+
+    # ------------------------------------------------------------- teardown
+    def _destroy(self):
+        if self._S is not None and _lib._lib is not None:
+            _lib._lib.chm_sampler_destroy(self._S)
+        self._S = None
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        self._destroy()

@@ -1,0 +1,211 @@
+"""Multiple-scattering path classes and the per-site incidence stencil (host setup).
+
+Mirrors chimera/mslibrary.py: classes under the cubic group O_h and time reversal
+(:51-62, :86-132), library entries class-major with slot 0 fastest (:190-198),
+per-supercell entry counts (SupercellMS, :341-492).
+
+The class set is found by a breadth-first walk over lattice offsets: every path
+absorber -> s1 -> ... -> absorber with 3 or 4 legs, no zero-length leg and half path
+length <= rmax_ms (the reference reads it from a FEFF10 paths.dat, :135-149).
+
+The labelling of a class's directed images must be the reference's, because an
+entry index is s_abs * nblk + cls_off + sum_q s_q nsp^q over the slot labels: the
+canonical representative is the smallest image under O_h x reversal, slots are
+numbered by first appearance along it, and images are generated forward over O_h,
+then reversed over O_h, the first image found keeping its labels (:100-124).
+
+For the device the directed instances are turned into a translation-invariant
+stencil: images of one class with the same site -> slot map give the same entry for
+every configuration and are merged with a weight (a path and its reverse); then for
+every role a site can play (absorber, or slot q) one record holds the other sites as
+offsets from it.  A swap visits only the records of the two swapped sites.
+"""
+from __future__ import annotations
+
+import itertools
+from collections import deque
+
+import numpy as np
+
+from .lattice import lattice_offsets
+
+
+def _ops():
+    out = []
+    for perm in itertools.permutations(range(3)):
+        for sg in itertools.product((1, -1), repeat=3):
+            A = np.zeros((3, 3), np.int64)
+            for r, c in enumerate(perm):
+                A[r, c] = sg[r]
+            out.append(A)
+    return out
+
+
+OPS = _ops()
+
+
+def _canonical(seq):
+    S0 = np.asarray(seq, np.int64)
+    best = None
+    for s in (S0, S0[::-1]):
+        for A in OPS:
+            t = tuple(map(tuple, s @ A.T))
+            if best is None or t < best:
+                best = t
+    return best
+
+
+class PathClass:
+    def __init__(self, rep):
+        self.rep = np.asarray(rep, np.int64)
+        self.nleg = self.rep.shape[0] + 1
+        slot_of, seen = [], {}
+        for q in map(tuple, self.rep):
+            if q == (0, 0, 0):
+                slot_of.append(-1)
+            else:
+                seen.setdefault(q, len(seen))
+                slot_of.append(seen[q])
+        self.slot_of = np.array(slot_of, np.int64)
+        self.nslot = len(seen)
+        imgs = {}
+        for rev in (False, True):
+            s = self.rep[::-1] if rev else self.rep
+            so = self.slot_of[::-1] if rev else self.slot_of
+            for A in OPS:
+                t = tuple(map(tuple, s @ A.T))
+                if t not in imgs:
+                    imgs[t] = so.copy()
+        self.images = [(np.array(k, np.int64), v) for k, v in imgs.items()]
+        self.deg = len(self.images)
+
+    def reff_units(self):
+        pts = np.vstack([np.zeros((1, 3)), self.rep, np.zeros((1, 3))])
+        return 0.5 * float(np.linalg.norm(np.diff(pts, axis=0), axis=1).sum())
+
+
+def find_classes(lattice, a0, rmax_ms, legs=(3, 4)):
+    """All path classes with nleg in `legs` and half length <= rmax_ms (A)."""
+    unit = a0 / 2.0
+    lim = 2.0 * rmax_ms / unit + 1e-9                 # full path length budget (units)
+    steps = lattice_offsets(lattice, lim / 2.0)       # one leg is at most half the path
+    steplen = np.sqrt((steps ** 2).sum(1))
+    found = set()
+    nmax = max(legs)
+    # state: (sites so far after the absorber, length so far)
+    queue = deque([((), 0.0)])
+    while queue:
+        seq, L = queue.popleft()
+        prev = np.array(seq[-1]) if seq else np.zeros(3, np.int64)
+        nxt = prev[None, :] + steps
+        back = np.sqrt((nxt ** 2).sum(1))
+        ok = L + steplen + back <= lim
+        for q, l in zip(nxt[ok], steplen[ok]):
+            tq = tuple(int(v) for v in q)
+            s2 = seq + (tq,)
+            n_leg = len(s2) + 1
+            if any(tq):  # a path may close only from a site other than the absorber
+                if n_leg in legs:
+                    found.add(_canonical(s2))
+            if n_leg < nmax:
+                queue.append((s2, L + l))
+    keys = sorted(found, key=lambda k: (len(k), PathClass(k).reff_units(), k))
+    return [PathClass(k) for k in keys]
+
+
+class MSLibrary:
+    """Path classes, entries, device stencil and synthetic table rows of a supercell.
+
+    Synthetic entry table (the reference reads FEFF10 output, mslibrary.py:294-313):
+    F_e = product of f over the path's scattering events, delta_e = mean over the
+    events of delta[absorber, scatterer], lambda the single-scattering one; row value
+    amp s02 F_e/(k r^2) exp(-2r/lambda - 2k^2 c_ms sigma2_nn r/(2 r_nn)) sin(2kr + delta_e)
+    with r the class's half path length.
+    """
+
+    def __init__(self, cell, nsp, rmax_ms, sigma2_nn=0.006, r_nn=None, c_ms=1.3, amp=1.0):
+        self.cell = cell
+        self.nsp = int(nsp)
+        self.classes = find_classes(cell.lattice, cell.a0, rmax_ms)
+        self.unit = cell.unit
+        self.nblk = int(sum(self.nsp ** c.nslot for c in self.classes))
+        self.cls_off = np.cumsum([0] + [self.nsp ** c.nslot for c in self.classes])[:-1]
+        for c in self.classes:
+            if 2 * np.abs(c.rep).max() >= cell.box_units.min():
+                raise ValueError("supercell too small for the paths")
+        self.sigma2_nn = float(sigma2_nn)
+        self.r_nn = float(r_nn) if r_nn is not None else float(
+            np.sqrt((lattice_offsets(cell.lattice, 2.0) ** 2).sum(1)).min() * self.unit)
+        self.c_ms, self.amp = float(c_ms), float(amp)
+        self._build_stencil()
+
+    # ------------------------------------------------------------- entries
+    def entries(self):
+        """[(class index, slot species)] in library order."""
+        out = []
+        for g, cl in enumerate(self.classes):
+            for code in range(self.nsp ** cl.nslot):
+                out.append((g, tuple((code // self.nsp ** q) % self.nsp for q in range(cl.nslot))))
+        return out
+
+    def table_rows(self):
+        """Per-row parameters of the entry table (absorber-major): r, s2, absorber,
+        nev, ev (scatterer species of each event)."""
+        ent = self.entries()
+        n = self.nsp * len(ent)
+        r = np.empty(n)
+        s2 = np.empty(n)
+        ab = np.empty(n, np.int32)
+        nev = np.empty(n, np.int32)
+        ev = np.zeros((n, 3), np.int32)
+        q = 0
+        for a in range(self.nsp):
+            for g, sp in ent:
+                cl = self.classes[g]
+                rr = cl.reff_units() * self.unit
+                seq = [a if t < 0 else sp[t] for t in cl.slot_of]
+                r[q] = rr
+                s2[q] = self.c_ms * self.sigma2_nn * rr / (2.0 * self.r_nn)
+                ab[q] = a
+                nev[q] = len(seq)
+                ev[q, :len(seq)] = seq
+                q += 1
+        return r, s2, ab, nev, ev
+
+    # ------------------------------------------------------------- stencil
+    def _build_stencil(self):
+        recs = []       # (abs offset, [slot offsets], nslot, w, cls_off), absorber role
+        for g, cl in enumerate(self.classes):
+            merged = {}
+            for img, so in cl.images:
+                M = tuple(tuple(int(v) for v in img[int(np.argmax(so == q))])
+                          for q in range(cl.nslot))
+                merged[M] = merged.get(M, 0) + 1
+            for M, w in merged.items():
+                recs.append((M, w, int(self.cls_off[g])))
+        self.nabsrec = len(recs)
+        stencil = []
+        for M, w, off in recs:
+            stencil.append(self._rec((0, 0, 0), M, w, off))
+        for M, w, off in recs:
+            for q in range(len(M)):
+                o = np.array(M[q])
+                stencil.append(self._rec(tuple(-o), tuple(tuple(np.array(m) - o) for m in M),
+                                         w, off))
+        self.stencil = np.ascontiguousarray(np.array(stencil, np.int64).astype(np.int8).reshape(-1, 16))
+        self.nstencil = self.stencil.shape[0]
+        self.directed_per_absorber = int(sum(c.deg for c in self.classes))
+
+    @staticmethod
+    def _rec(abs_off, slots, w, cls_off):
+        r = np.zeros(16, np.int64)
+        r[0:3] = abs_off
+        for q, s in enumerate(slots):
+            r[3 + 3 * q:6 + 3 * q] = s
+        r[12] = len(slots)
+        r[13] = w
+        r[14] = cls_off & 0xFF
+        r[15] = cls_off >> 8
+        if np.abs(r[:12]).max() > 127 or cls_off >= 1 << 15 or w > 127:
+            raise ValueError("stencil record does not fit its int8 fields")
+        return r

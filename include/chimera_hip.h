@@ -1,0 +1,213 @@
+/*
+ * chimera_hip.h — C ABI of the MI355X (gfx950) chi(k) forward model, windowed chi(R)
+ * transform and parallel-tempering RMC swap sampler.
+ *
+ * The reference (dluccioni/CHIMERA) has no C ABI: its boundary is a Python object API
+ * over CuPy arrays. Each entry point below names the reference interface it backs
+ * (file:line in the reference tree). Conventions:
+ *   - every function returns 0 on success, a negative CHM_E_* code on failure, and
+ *     chm_last_error() then holds a message (thread-local);
+ *   - all pointer arguments are HOST pointers unless the name says otherwise; the
+ *     library owns every device allocation;
+ *   - one HIP stream per context; a context (and everything created from it) is not
+ *     thread-safe: call it from one host thread at a time;
+ *   - floating point is IEEE double unless stated; species are int32 indices.
+ */
+#ifndef CHIMERA_HIP_H
+#define CHIMERA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHM_OK 0
+#define CHM_E_ARG (-1)     /* bad argument (maps to ValueError) */
+#define CHM_E_HIP (-2)     /* HIP runtime failure */
+#define CHM_E_STATE (-3)   /* call out of order (maps to RuntimeError) */
+#define CHM_E_ASSERT (-4)  /* a reference assertion failed (maps to AssertionError) */
+
+#define CHM_PREC_F64 0     /* FP64 path sum */
+#define CHM_PREC_F32 1     /* FP32 path sum (FP64 phase reduction), FP64 accumulation */
+
+typedef struct chm_ctx chm_ctx;
+typedef struct chm_ft chm_ft;
+typedef struct chm_sampler chm_sampler;
+
+const char *chm_last_error(void);
+int chm_version(void);
+
+/* Context: device + stream.  Replaces the CuPy device/null stream the reference
+ * uses implicitly (exafs_gpu/forward.py:20, methods/method_a_rmc.py:38). */
+int chm_ctx_create(int device, chm_ctx **out);
+int chm_ctx_destroy(chm_ctx *ctx);
+int chm_ctx_sync(chm_ctx *ctx);
+
+/* ---------------------------------------------------------------- forward model */
+
+/* Scattering tables: k (nk), f (nsp, nk), delta (nsp*nsp, nk), lambda (nk), s02.
+ * ForwardModel.__init__ table upload (exafs_gpu/forward.py:61-86); nsp >= 2 as
+ * kernels.module (exafs_gpu/kernels.py:674-675). */
+int chm_fm_set_tables(chm_ctx *ctx, int nsp, int nk, const double *k, const double *f,
+                      const double *delta, const double *lam, double s02);
+
+/* Fixed topology: ordered pairs i, j (npairs), periodic shift (npairs, 3), shell
+ * (npairs), sigma^2 (npairs).  ForwardModel.__init__ (forward.py:65-92) and the
+ * in-place d_shift / d_sig2 updates its callers make (chimera/model.py:263, 278). */
+int chm_fm_set_topology(chm_ctx *ctx, int natoms, int npairs, const int32_t *i_idx,
+                        const int32_t *j_idx, const double *shift, const int32_t *shell,
+                        const double *sig2);
+int chm_fm_set_shift(chm_ctx *ctx, const double *shift);
+int chm_fm_set_sig2(chm_ctx *ctx, const double *sig2);
+
+/* Cumulant tables: dr and s2 (nshell*nsp*nsp), c3 (nshell).
+ * ForwardModel.set_dr / set_sig2_types / set_c3 (forward.py:104-145). */
+int chm_fm_set_cumulants(chm_ctx *ctx, int nshell, const double *drtab, const double *c3tab,
+                         const double *s2tab);
+
+/* Pair distances r (nconf, npairs) and unit vectors evec (nconf, npairs, 3) from
+ * positions (nconf, natoms, 3).  ForwardModel.distances (forward.py:148-157),
+ * kernel compute_r (kernels.py:53-78).  evec may be NULL. */
+int chm_fm_distances(chm_ctx *ctx, int nconf, const double *pos, double *r, double *evec);
+
+/* Pair types s_i*nsp + s_j (nconf, npairs).  ForwardModel.ptypes (forward.py:159-166),
+ * kernel build_ptype (kernels.py:81-96). */
+int chm_fm_ptypes(chm_ctx *ctx, int nconf, const int32_t *species, int32_t *ptype);
+
+/* Full single-scattering path sum, UNNORMALISED, chi (nconf, nsp, nk).
+ * ForwardModel.chi (forward.py:169-188) / kernel chi_forward (kernels.py:99-163).
+ * The k = 0 term is defined as 0.  precision: CHM_PREC_F64 or CHM_PREC_F32. */
+int chm_fm_chi(chm_ctx *ctx, int nconf, const double *pos, const int32_t *species,
+               double *chi, int precision);
+
+/* Term table rows: row q gets
+ *   s02 * prod_{m<nev} f[ev_m](k) / (k r^2) * exp(-2 r/lambda(k) - 2 k^2 s2)
+ *       * sin(2 k r + mean_m delta[a*nsp+ev_m](k) - 4/3 k^3 c3)
+ * scaled by amp, with the row's (r, s2, c3, a, nev, ev[3]).  It builds the count-space
+ * basis B[a,b,s](k) (chimera/shellmodel.py:290-336) and the multiple-scattering
+ * entry table (chimera/mslibrary.py:304-313).  out is (nrows, nk) host. */
+int chm_fm_term_rows(chm_ctx *ctx, int nrows, const double *r, const double *s2,
+                     const double *c3, const int32_t *absorber, const int32_t *nev,
+                     const int32_t *ev /* (nrows, 3) */, double amp, double *out);
+
+/* ------------------------------------------------------------------ transform */
+
+/* One windowed transform per channel, folded with the model-grid resample into a
+ * DFT matrix D_c (nk_model, nR) complex on the device.
+ * FourierTransform.__init__ (exafs_gpu/fourier.py:39-67) + MultiEdgeFT
+ * (chimera/multift.py:34-56).  idx/frac/weight are (nchan, nfft): the resample
+ * index, clamped fraction (already times `inside`) and W(k) k^w; kuni (nfft);
+ * R (nR) any grid; norm = dk/sqrt(pi). */
+int chm_ft_create(chm_ctx *ctx, int nchan, int nk_model, int nfft, int nR,
+                  const int32_t *chan_edge, const int32_t *idx, const double *frac,
+                  const double *inside, const double *weight, const double *kuni,
+                  const double *R, double norm, chm_ft **out);
+int chm_ft_destroy(chm_ft *ft);
+
+/* chi (nrows, nsp, nk_model) -> out (nrows, nchan, nR, 2) [re, im], chi_extra
+ * (nsp, nk_model) or NULL added first.  FourierTransform.to_R (fourier.py:75-80) fanned
+ * out per channel by MultiEdgeFT.to_R (multift.py:58-63). */
+int chm_ft_apply(chm_ft *ft, int nrows, const double *chi, const double *chi_extra,
+                 double *out);
+
+/* Rows (nrows, nk_model) through the channels of one edge e only ->
+ * out (nrows, nkw_e, nR, 2).  Used to transform count-space basis rows. */
+int chm_ft_apply_edge(chm_ft *ft, int edge, int nrows, const double *rows, double *out);
+
+/* -------------------------------------------------------------------- sampler */
+
+/* Problem geometry of a count-space swap sampler (every pair and path term depends
+ * only on the species at fixed lattice sites: the reference's swap-only use,
+ * chimera/fitrun.py:307-315 with swap_frac = 1). */
+typedef struct {
+    int32_t natoms, nsp, ncoord, nshell;
+    int32_t nblk;        /* multiple-scattering entries per absorber, 0 = none      */
+    int32_t nstencil;    /* multiple-scattering incidence records per site           */
+    int32_t nabsrec;     /* of which the first nabsrec have the site as absorber     */
+    int32_t nkw, nwin;   /* channels per edge; fit-window R points                  */
+    int32_t box[3];      /* supercell in units of a0/2                              */
+    int32_t ncell[3];    /* conventional cells                                      */
+    int32_t nbasis;      /* sites per conventional cell                             */
+    int32_t basis_of_parity[8]; /* basis index of parity (x&1)|(y&1)<<1|(z&1)<<2   */
+    int32_t has_const;   /* one constant row per edge (MultiEdgeFT.chi_extra)       */
+} chm_geom;
+
+/* Create a sampler.
+ *   nbr_j, nbr_shell (natoms, ncoord): the i-sorted pair list of the forward model
+ *       (methods/method_a_rmc.py:106-117: nbr_pairs = arange reshaped);
+ *   stencil (nstencil, 16) int8: per incidence record [abs dx,dy,dz, slot0 dx,dy,dz,
+ *       slot1 .., slot2 .., nslot, weight, cls_off lo, cls_off hi] relative to the site;
+ *   basis_R (nsp, nrow_e, nkw*nwin*2): the transformed count-space basis per edge,
+ *       already divided by the edge's absorber count (row order: SS rows b*nshell+s,
+ *       then nblk MS entries, then the constant row);
+ *   basis_k (nsp, nrow_e, nk) untransformed, for reading chi(k) back;
+ *   target, sigma (nsp*nkw, nwin, 2) and (nsp*nkw, nwin): the fit window only.
+ * RMCSampler.__init__ (methods/method_a_rmc.py:64-163). */
+int chm_sampler_create(chm_ctx *ctx, const chm_geom *g, int nrep, int nk,
+                       const int32_t *site_xyz, const int32_t *nbr_j,
+                       const int32_t *nbr_shell, const int8_t *stencil,
+                       const double *basis_R, const double *basis_k,
+                       const double *target, const double *sigma, int profile_scale,
+                       chm_sampler **out);
+int chm_sampler_destroy(chm_sampler *s);
+
+/* Start every replica from species0 (natoms); betas per rung (nrep); seed.
+ * RMCSampler.init_state (method_a_rmc.py:166-184). */
+int chm_sampler_init(chm_sampler *s, const int32_t *species0, const double *beta,
+                     uint64_t seed);
+
+/* Run nsteps swap steps (one proposal per replica each), replica exchange every
+ * exchange_every steps (0 = never).  Device-resident; the steps between host calls
+ * are replayed from a captured hipGraph.  RMCSampler.run/step/replica_exchange
+ * (method_a_rmc.py:250-308, 332-373, 401-455). */
+int chm_sampler_run(chm_sampler *s, int nsteps, int exchange_every);
+
+/* Record options: move log (a, b, accepted, rung per replica per step) and the SS
+ * count vector of the rung nearest collect_beta each step (for alpha series / ESS).
+ * Capacity in steps; 0 disables. */
+int chm_sampler_set_recording(chm_sampler *s, int log_capacity, int series_capacity,
+                              int series_rung);
+
+/* Read state back (host buffers).  what:
+ *   0 species (nrep, natoms) int32, rung order    1 chi2 (nrep) double, rung order
+ *   2 counts (nrep, nsp*nrow_e) int32, rung order  3 best chi2 (nrep) double, rung order
+ *   4 best species (nrep, natoms) int32            5 rung_of_slot (nrep) int32
+ *   6 move log (n, nrep, 4) int32 [a, b, acc, rung], slot order, n = steps logged
+ *   7 series (n, nsp*nsp*nshell) int32             8 exchange attempts/accepts (2, nrep-1) int64
+ *   9 accepted moves per rung (nrep) int64         10 step counter (1) int64
+ *   11 chi(k) (nrep, nsp, nk) double, rung order   12 trial chi(R) of the last step
+ *      (nrep, nsp*nkw*nwin*2) double, slot order
+ * RMCSampler attributes species/chi2/chi/best (method_a_rmc.py:166-184, 493-496). */
+int chm_sampler_read(chm_sampler *s, int what, void *host, int64_t nbytes);
+
+/* Entry counts (nconf, nsp*nrow_e) of arbitrary configurations (nconf, natoms), from
+ * scratch: SupercellMS.counts (chimera/mslibrary.py:436-451) and the pair counts
+ * N[a,b,s] of ShellModel.counts_of (chimera/shellmodel.py:379-386) in one pass. */
+int chm_sampler_counts(chm_sampler *s, int nconf, const int32_t *species, int32_t *counts);
+
+/* Reduced chi^2 (with this sampler's target, sigma, window and profiling) of nrows
+ * window transforms chiR (nrows, nsp*nkw*nwin*2).  RMCSampler._chi2
+ * (method_a_rmc.py:211-227) as used for best_verified (:439-448). */
+int chm_sampler_chi2(chm_sampler *s, int nrows, const double *chiR, double *c2);
+
+/* One replica-exchange sweep now (method_a_rmc.py:332-373); run() also exchanges on
+ * its own every exchange_every steps. */
+int chm_sampler_exchange(chm_sampler *s);
+
+/* Unnormalised chi(k) (nconf, nsp, nk) of count vectors (nconf, nsp*nrow_e):
+ * counts @ basis_k per edge (SupercellMS.chi_of, chimera/mslibrary.py:453-457, and the
+ * pair-count model ShellModel.chi_k, chimera/shellmodel.py:411-413). */
+int chm_sampler_chik(chm_sampler *s, int nconf, const int32_t *counts, double *chi);
+
+/* Average device time (ms) per launch of the step's kernels, from HIP events on the
+ * sampler's stream over `iters` launches each: ms[0] propose+gather, ms[1] trial
+ * transform GEMM, ms[2] accept, ms[3] one whole step.  The state stays consistent
+ * (no move is committed while timing), but the step counter advances.  No counterpart
+ * in the reference (it has no profiler hooks). */
+int chm_sampler_profile(chm_sampler *s, int iters, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHIMERA_HIP_H */

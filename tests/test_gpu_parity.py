@@ -1,0 +1,242 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle on the same inputs."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import forward as OF
+from oracle import fourier as OT
+from oracle import rmc as OR
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _problem(name, lat, nc, a0, els, rmax, ms, nrep, seed=0):
+    from chimera_amd import workloads as W
+    _say(f"[build] {name} {nc} ms={ms} nrep={nrep}")
+    return W.Problem(W.Workload(name, lat, nc, a0, els, rmax, ms, nrep), seed=seed)
+
+
+def _oracle_problem(P):
+    from oracle.build import problem_from
+    return problem_from(P)
+
+
+@pytest.fixture(scope="module")
+def tern108():
+    P = _problem("t108", "fcc", (3, 3, 3), 3.59, ("Cr", "Co", "Ni"), 5.2, True, 6)
+    return P, _oracle_problem(P)
+
+
+@pytest.fixture(scope="module")
+def bin500():
+    P = _problem("b500", "fcc", (5, 5, 5), 3.59, ("Fe", "Ni"), 5.2, False, 8)
+    return P, _oracle_problem(P)
+
+
+def test_path_sum_f64_f32(tern108):
+    P, O = tern108
+    _say("[test] path sum fp64/fp32 vs oracle")
+    rng = np.random.default_rng(0)
+    sp = np.stack([P.cfg0, P.ref_species, rng.permutation(P.cfg0)]).astype(np.int32)
+    pos = np.repeat(P.cell.ideal[None], 3, 0)
+    pos[2] += rng.normal(0, 0.03, pos[2].shape)          # displaced configuration too
+    ref = np.stack([OF.chi_pairs(P.k, P.tables.f, P.tables.delta, P.tables.lam, P.tables.s02,
+                                 pos[c], sp[c], P.i, P.j, P.shift, P.sig2, normalise=False)
+                    for c in range(3)])
+    scale = np.abs(ref).max()
+    got = P.fm.chi(pos, sp, normalise=False)
+    assert np.abs(got - ref).max() / scale < 1e-12
+    got32 = P.fm.chi(pos, sp, normalise=False, precision="f32")
+    assert np.abs(got32 - ref).max() / scale < 1e-5
+    # normalised by atoms of the absorber species (forward.py:183-187)
+    gn = P.fm.chi(pos[:1], sp[:1])
+    cnt = np.bincount(sp[0], minlength=3)
+    np.testing.assert_allclose(gn[0], ref[0] / cnt[:, None], rtol=1e-12, atol=1e-14 * scale)
+
+
+def test_cumulants_dr_c3_s2(tern108):
+    P, O = tern108
+    _say("[test] cumulant tables")
+    ns, nsp = P.fm.nshell, P.nsp
+    rng = np.random.default_rng(1)
+    dr = rng.normal(0, 0.01, (ns, nsp, nsp))
+    c3 = rng.normal(0, 1e-3, ns)
+    s2 = np.abs(rng.normal(0, 1e-3, (ns, nsp, nsp)))
+    P.fm.set_dr(dr).set_c3(c3).set_sig2_types(s2)
+    try:
+        sp = P.cfg0[None]
+        got = P.fm.chi(P.cell.ideal[None], sp, normalise=False)[0]
+        ref = OF.chi_pairs(P.k, P.tables.f, P.tables.delta, P.tables.lam, P.tables.s02,
+                           P.cell.ideal, P.cfg0, P.i, P.j, P.shift, P.sig2, shell=P.shell, dr=dr,
+                           c3=c3, s2types=s2, normalise=False)
+        assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12
+        B = P.fm.ss_basis(P.radii, np.full(ns, P.w.sigma2))
+        Bo = OF.ss_basis(P.k, P.tables.f, P.tables.delta, P.tables.lam, P.tables.s02, P.radii,
+                         np.full(ns, P.w.sigma2), dr=dr, c3=c3, s2types=s2)
+        assert np.abs(B - Bo).max() / np.abs(Bo).max() < 1e-13
+    finally:
+        P.fm.set_dr(np.zeros((ns, nsp, nsp))).set_c3(np.zeros(ns)).set_sig2_types(
+            np.zeros((ns, nsp, nsp)))
+
+
+def test_distances_ptypes(tern108):
+    P, O = tern108
+    pos = P.cell.ideal[None] + np.random.default_rng(2).normal(0, 0.02, (1,) + P.cell.ideal.shape)
+    r, ev = P.fm.distances(pos)
+    d = (pos[0][P.j] + P.shift) - pos[0][P.i]
+    np.testing.assert_allclose(r[0], np.linalg.norm(d, axis=1), rtol=1e-15)
+    np.testing.assert_allclose(ev[0], d / np.linalg.norm(d, axis=1)[:, None], rtol=1e-14, atol=1e-15)
+    pt = P.fm.ptypes(P.cfg0[None])
+    assert np.array_equal(pt[0], P.cfg0[P.i] * 3 + P.cfg0[P.j])
+
+
+def test_transform_fft_grid_and_window_grid(tern108):
+    P, O = tern108
+    _say("[test] transform")
+    from chimera_amd.fourier import FourierTransform
+    rng = np.random.default_rng(3)
+    chi = rng.standard_normal((4, P.k.size))
+    ft = FourierTransform(P.k, 3.0, 14.0, 2, 1.0)              # FFT grid, 1024 points
+    got = ft.to_R(chi)
+    ref = OT.Transform(P.k, 3.0, 14.0, 2, 1.0).to_R(chi)
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12
+    chi3 = rng.standard_normal((2, P.nsp, P.k.size))
+    got = P.ft.to_R(chi3)
+    ref = np.stack([OT.multi_to_R(O.transforms, P.chan_edge, c, R=P.R) for c in chi3])
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12
+
+
+def test_entry_counts_exact(tern108):
+    P, O = tern108
+    _say("[test] entry counts")
+    S = P.sampler
+    rng = np.random.default_rng(4)
+    sps = np.stack([P.cfg0, P.ref_species, rng.permutation(P.cfg0)]).astype(np.int32)
+    C = S.counts_of(sps)
+    nss = P.nsp * P.fm.nshell
+    nrow_e = S.nrow_e
+    for c in range(3):
+        Co = O.ms.counts(sps[c])
+        N = OF.pair_counts(sps[c], P.i, P.j, P.shell, P.nsp, P.fm.nshell)
+        Cg = C[c].reshape(P.nsp, nrow_e)
+        assert np.array_equal(Cg[:, :nss].reshape(N.shape), N)
+        assert np.array_equal(Cg[:, nss:].reshape(-1), Co)
+
+
+def test_chi_of_counts_matches_pair_space(tern108):
+    P, O = tern108
+    S = P.sampler
+    got = S.chi_k_of(P.ref_species[None])[0]
+    ref = O.chi_full(P.ref_species.astype(np.int64))
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12
+    S.init_state(P.cfg0, P.cell.ideal)
+    cnt = np.bincount(P.cfg0, minlength=3).astype(float)
+    c2_ref = O.chi2(O.chi_full(P.cfg0.astype(np.int64)), cnt)
+    np.testing.assert_allclose(S.chi2, c2_ref, rtol=1e-10)
+    np.testing.assert_allclose(S.chi, np.repeat(O.chi_full(P.cfg0)[None], S.nrep, 0),
+                               rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def _compare_trajectory(P, O, nsteps, exch, seed):
+    S = P.sampler
+    S.seed = seed
+    S.set_recording(log_steps=nsteps)
+    S.init_state(P.cfg0, P.cell.ideal)
+    _say(f"[run] device {nsteps} steps nrep={S.nrep} exchange_every={exch}")
+    S.run(nsteps, exchange_every=exch, log_every=10 ** 9)
+    log = S.move_log(nsteps)                                    # (steps, slot, 4)
+    R = OR.PlainSampler(O, nrep=S.nrep, seed=seed)
+    R.init_state(P.cfg0)
+    _say("[run] oracle")
+    R.run(nsteps, exchange_every=exch)
+    for t in range(nsteps):
+        by_rung = {int(r[3]): r for r in log[t]}
+        for rung in range(S.nrep):
+            a, b, acc, c2 = R.log[t][rung]
+            g = by_rung[rung]
+            assert (int(g[0]), int(g[2])) == (a, acc), f"step {t} rung {rung}"
+            if acc or b != a:
+                assert int(g[1]) == b
+    assert np.array_equal(S.species, R.species)
+    np.testing.assert_allclose(S.chi2, R.chi2, rtol=1e-9)
+    np.testing.assert_allclose(S.best_chi2, R.best_chi2, rtol=1e-9)
+    return S, R
+
+
+def test_sampler_trajectory_ternary_ms(tern108):
+    P, O = tern108
+    S, R = _compare_trajectory(P, O, 60, 10, seed=5)
+    assert S.accepted_per_rung().sum() > 0
+    # counts tracked incrementally equal a recount from scratch
+    C = S.counts
+    assert np.array_equal(C, S.counts_of(S.species))
+    np.testing.assert_allclose(S.best_verified, S.best_chi2.min(), rtol=1e-9)
+
+
+def test_sampler_trajectory_binary_ss(bin500):
+    P, O = bin500
+    _compare_trajectory(P, O, 40, 5, seed=11)
+
+
+def test_null_moves_when_candidates_repeat(bin500):
+    P, O = bin500
+    S = P.sampler
+    sp = np.zeros(P.cell.natoms, np.int32)
+    sp[:3] = 1                                   # almost every proposal finds no partner
+    S.set_recording(log_steps=20)
+    S.init_state(sp, P.cell.ideal)
+    S.run(20, exchange_every=0, log_every=10 ** 9)
+    R = OR.PlainSampler(O, nrep=S.nrep, seed=S.seed)
+    R.init_state(sp)
+    R.run(20, exchange_every=0)
+    assert np.array_equal(S.species, R.species)
+    log = S.move_log(20)
+    nulls = sum(1 for t in range(20) for r in R.log[t] if r[0] == r[1])
+    assert nulls > 0 and (log[..., 2] == 0).sum() >= nulls
+    S.init_state(P.cfg0, P.cell.ideal)
+
+
+def test_replay_determinism(tern108):
+    P, O = tern108
+    S = P.sampler
+    S.set_recording(log_steps=40)
+    logs, sps = [], []
+    for _ in range(2):
+        S.init_state(P.cfg0, P.cell.ideal)
+        S.run(40, exchange_every=7, log_every=10 ** 9)
+        logs.append(S.move_log(40).copy())
+        sps.append(S.species.copy())
+    assert np.array_equal(logs[0], logs[1]) and np.array_equal(sps[0], sps[1])
+    # replaying the recorded moves on the CPU reproduces the species bit-exactly
+    sp = np.repeat(P.cfg0[None], S.nrep, 0).astype(np.int64)
+    for t in range(40):
+        for slot in range(S.nrep):
+            a, b, acc, rung = logs[0][t, slot]
+            if acc:
+                sp[slot, a], sp[slot, b] = sp[slot, b], sp[slot, a]
+    rung_of = S._read(5, (S.nrep,), np.int32)
+    assert np.array_equal(sp[np.argsort(rung_of)], sps[0])
+
+
+def test_workload_4000_consistency():
+    """Full-size workload: incremental counts == recount, chi^2 from counts == chi^2 from
+    the pair-space path sum + entry counts, composition conserved."""
+    from chimera_amd import workloads as W
+    _say("[build] fcc_crconi_4000")
+    P = W.build("fcc_crconi_4000")
+    S = P.sampler
+    c2_0 = S.chi2.copy()
+    _say("[run] 2000 steps")
+    S.run(2000, exchange_every=25, log_every=10 ** 9)
+    sp = S.species
+    assert all(np.array_equal(np.bincount(s, minlength=3), np.bincount(P.cfg0, minlength=3))
+               for s in sp)
+    assert np.array_equal(S.counts, S.counts_of(sp))
+    np.testing.assert_allclose(S.chi2, S.chi2_of(sp), rtol=1e-9)
+    assert S.best_chi2.min() < c2_0.min()

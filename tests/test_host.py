@@ -1,0 +1,190 @@
+"""Host-side code of the product against the oracle (no GPU), and the C ABI surface."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from chimera_amd import _lib
+from chimera_amd.lattice import CrystalSupercell, alpha_from_counts, warren_cowley
+from chimera_amd.mspaths import MSLibrary, find_classes
+from chimera_amd.scattering import k_grid, synthetic
+from oracle import lattice as OL
+from oracle import mspaths as OM
+from oracle import tables as OTab
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("lat,nc,a0,rmax", [("fcc", (3, 3, 3), 3.59, 5.2),
+                                             ("fcc", (5, 5, 5), 3.59, 5.2),
+                                             ("bcc", (4, 4, 4), 2.87, 4.1),
+                                             ("bcc", (8, 4, 4), 3.0, 4.4)])
+def test_neighbour_list_matches_oracle(lat, nc, a0, rmax):
+    c = CrystalSupercell(lat, nc, a0)
+    i, j, sh = c.neighbour_list(rmax)
+    o = OL.Supercell(lat, nc, a0)
+    i2, j2, sh2 = o.neighbour_list(rmax)
+    assert np.array_equal(i, i2) and np.array_equal(j, j2)
+    np.testing.assert_allclose(sh, sh2, atol=1e-12)
+    s1, r1 = c.shells(i, j, sh)
+    s2, r2 = OL.shell_index(o.ideal, i2, j2, sh2)
+    assert np.array_equal(s1, s2)
+    np.testing.assert_allclose(r1, r2, rtol=1e-14)
+
+
+def test_too_small_box_is_rejected():
+    with pytest.raises(ValueError):
+        CrystalSupercell("fcc", (2, 2, 2), 3.59).neighbour_list(5.2)
+
+
+def test_synthetic_tables_match_oracle():
+    t = synthetic(("Cr", "Co", "Ni"), k_grid())
+    k, f, d, lam, s02 = OTab.synthetic_ss(OTab.k_grid(), [24, 27, 28])
+    np.testing.assert_array_equal(t.k, k)
+    np.testing.assert_allclose(t.f, f, rtol=1e-15)
+    np.testing.assert_allclose(t.delta, d, rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(t.lam, lam, rtol=1e-15)
+    assert t.s02 == s02
+
+
+@pytest.mark.parametrize("a0,rmax", [(3.56, 5.5), (3.59, 5.2)])
+def test_ms_classes_match_oracle(a0, rmax):
+    mine = find_classes("fcc", a0, rmax)
+    ref = OM.enumerate_classes("fcc", a0, rmax)
+    assert [c.rep.tolist() for c in mine] == [c.rep.tolist() for c in ref]
+    for a, b in zip(mine, ref):
+        assert len(a.images) == len(b.images)
+        for (ia, sa), (ib, sb) in zip(a.images, b.images):
+            assert np.array_equal(ia, ib) and np.array_equal(sa, sb)
+
+
+def _emulate_counts(lib, cell, species, i, j, shell, nsp):
+    """CPU emulation of the device's full-count kernel over the stencil records (the
+    same index arithmetic), to check the stencil format before any GPU run."""
+    nshell = int(shell.max()) + 1
+    nss = nsp * nshell
+    nrow_e = nss + lib.nblk
+    out = np.zeros(nsp * nrow_e, np.int64)
+    np.add.at(out, species[i] * nrow_e + species[j] * nshell + shell, 1)
+    st = lib.stencil.astype(np.int64)
+    xyz = cell.xyz % cell.box_units
+    for rec in st[:lib.nabsrec]:
+        nslot, w = rec[12], rec[13]
+        off = (rec[14] & 0xFF) | (rec[15] << 8)
+        e = species * nrow_e + nss + off
+        pw = 1
+        for q in range(1, nslot + 1):
+            e = e + species[cell.site_of(xyz + rec[3 * q:3 * q + 3])] * pw
+            pw *= nsp
+        np.add.at(out, e, w)
+    return out
+
+
+def _emulate_delta(lib, cell, species, i, j, shell, nsp, a, b):
+    """CPU emulation of the device's swap gather (SS neighbour items + MS stencil of
+    both sites, paths through both counted once)."""
+    nshell = int(shell.max()) + 1
+    nss = nsp * nshell
+    nrow_e = nss + lib.nblk
+    ncoord = i.size // cell.natoms
+    out = np.zeros(nsp * nrow_e, np.int64)
+    na, nb = species[b], species[a]
+    new = species.copy()
+    new[a], new[b] = na, nb
+    for x in (a, b):
+        for o in range(ncoord):
+            y = j[x * ncoord + o]
+            if x == b and y == a:
+                continue
+            sh = shell[x * ncoord + o]
+            for (u, v) in ((x, y), (y, x)):
+                out[species[u] * nrow_e + species[v] * nshell + sh] -= 1
+                out[new[u] * nrow_e + new[v] * nshell + sh] += 1
+    xyz = cell.xyz % cell.box_units
+    for x in (a, b):
+        for rec in lib.stencil.astype(np.int64):
+            nslot, w = rec[12], rec[13]
+            off = (rec[14] & 0xFF) | (rec[15] << 8)
+            sites = [int(cell.site_of(xyz[x] + rec[3 * q:3 * q + 3])) for q in range(nslot + 1)]
+            if x == b and a in sites:
+                continue
+            eo = species[sites[0]] * nrow_e + nss + off
+            en = new[sites[0]] * nrow_e + nss + off
+            pw = 1
+            for q in range(1, nslot + 1):
+                eo += species[sites[q]] * pw
+                en += new[sites[q]] * pw
+                pw *= nsp
+            out[eo] -= w
+            out[en] += w
+    return out
+
+
+def test_stencil_counts_match_oracle():
+    cell = CrystalSupercell("fcc", (3, 3, 3), 3.59)
+    i, j, sh = cell.neighbour_list(5.2)
+    shell, radii = cell.shells(i, j, sh)
+    lib = MSLibrary(cell, 3, 5.2)
+    ref_cl = OM.enumerate_classes("fcc", 3.59, 5.2)
+    sc = OM.SupercellPaths(ref_cl, 3, cell.ideal, 3.59, (3, 3, 3))
+    assert lib.nblk == sc.nblk
+    sp = OL.random_config(108, [1, 1, 1], 11).astype(np.int64)
+    got = _emulate_counts(lib, cell, sp, i, j, shell, 3)
+    nss = 3 * 4
+    nrow_e = nss + lib.nblk
+    for a in range(3):
+        ms = got[a * nrow_e + nss:(a + 1) * nrow_e]
+        assert np.array_equal(ms, sc.counts(sp)[a * sc.nblk:(a + 1) * sc.nblk])
+    N = np.zeros((3, 3, 4), np.int64)
+    np.add.at(N, (sp[i], sp[j], shell), 1)
+    assert np.array_equal(got.reshape(3, nrow_e)[:, :nss].reshape(3, 3, 4), N)
+    # the swap gather gives exactly the count change of the swap
+    rng = np.random.default_rng(1)
+    done = 0
+    while done < 6:
+        a, b = (int(x) for x in rng.integers(0, 108, 2))
+        if sp[a] == sp[b]:
+            continue
+        d = _emulate_delta(lib, cell, sp, i, j, shell, 3, a, b)
+        sp2 = sp.copy()
+        sp2[a], sp2[b] = sp[b], sp[a]
+        assert np.array_equal(_emulate_counts(lib, cell, sp2, i, j, shell, 3) - got, d)
+        done += 1
+
+
+def test_stencil_sizes_workload_4000():
+    cell = CrystalSupercell("fcc", (10, 10, 10), 3.56)
+    lib = MSLibrary(cell, 3, 5.5)
+    assert len(lib.classes) == 25 and lib.nblk == 327
+    assert lib.directed_per_absorber == 984
+    w = lib.stencil[:lib.nabsrec, 13].astype(int)
+    assert w.sum() == 984                          # merged weights cover every directed path
+    assert np.abs(lib.stencil[:, :12]).max() < 20  # offsets stay inside the 20-unit box
+
+
+def test_alpha_from_counts_matches_warren_cowley():
+    cell = CrystalSupercell("fcc", (3, 3, 3), 3.59)
+    i, j, sh = cell.neighbour_list(5.2)
+    shell, _ = cell.shells(i, j, sh)
+    sp = OL.random_config(108, [1, 1, 1], 2)
+    N = np.zeros((3, 3))
+    m = shell == 0
+    np.add.at(N, (sp[i[m]], sp[j[m]]), 1)
+    np.testing.assert_allclose(alpha_from_counts(N, np.bincount(sp)),
+                               OL.warren_cowley(sp, i, shell, j, 0, 3), rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(warren_cowley(sp, i, j, shell, 0, 3),
+                               OL.warren_cowley(sp, i, shell, j, 0, 3), rtol=1e-14, atol=1e-14)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libchimera_hip.so not built (run __graft_entry__.build())")
+    hdr = open(os.path.join(ROOT, "include", "chimera_hip.h")).read()
+    declared = set(re.findall(r"\b(chm_[a-z0-9_]+)\s*\(", hdr))
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == set(_lib.EXPORTS)
+    assert _lib.load().chm_version() >= 1

@@ -1,0 +1,258 @@
+"""Pins the FP64 oracle: published values, closed forms, invariants, brute force."""
+import numpy as np
+import pytest
+
+from oracle import forward as OF
+from oracle import fourier as OT
+from oracle import lattice as OL
+from oracle import mspaths as OM
+from oracle import rmc as OR
+from oracle import tables as OTab
+
+
+def test_splitmix64_published_sequence():
+    # Vigna's splitmix64 seeded with 0: first three outputs (public reference values)
+    x = 0
+    outs = []
+    for _ in range(3):
+        outs.append(OR.splitmix64(x))
+        x = (x + OR.GOLDEN) & OR.M64
+    assert outs == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_u01_range_and_resolution():
+    assert OR.u01(0) == 0.0
+    assert OR.u01(OR.M64) < 1.0
+    assert OR.u01(1 << 11) == 2.0 ** -53
+
+
+@pytest.fixture(scope="module")
+def fcc108():
+    cell = OL.Supercell("fcc", (3, 3, 3), 3.59)
+    i, j, sh = cell.neighbour_list(5.2)
+    shell, radii = OL.shell_index(cell.ideal, i, j, sh)
+    return cell, i, j, sh, shell, radii
+
+
+def test_fcc_shells(fcc108):
+    cell, i, j, sh, shell, radii = fcc108
+    assert i.size == 54 * cell.natoms
+    assert np.bincount(shell).tolist() == [12 * 108, 6 * 108, 24 * 108, 12 * 108]
+    a0 = 3.59
+    np.testing.assert_allclose(radii, [a0 / np.sqrt(2), a0, a0 * np.sqrt(1.5), a0 * np.sqrt(2)],
+                               rtol=1e-12)
+
+
+def test_reverse_index_involution(fcc108):
+    cell, i, j, sh, shell, radii = fcc108
+    rev = OL.reverse_index(i, j, cell.natoms)
+    assert np.array_equal(rev[rev], np.arange(i.size))
+    assert np.array_equal(i[rev], j) and np.array_equal(shell[rev], shell)
+
+
+def test_neighbour_list_brute_force():
+    # every ordered pair within rmax found once, against an explicit all-images search
+    cell = OL.Supercell("bcc", (3, 3, 3), 2.87)
+    i, j, sh = cell.neighbour_list(3.0)
+    got = set(zip(i.tolist(), j.tolist()))
+    want = set()
+    for a in range(cell.natoms):
+        for b in range(cell.natoms):
+            for img in cell._images:
+                d = cell.ideal[b] + img - cell.ideal[a]
+                if 1e-6 < np.linalg.norm(d) <= 3.0:
+                    want.add((a, b))
+    assert got == want
+    assert np.bincount(i).tolist() == [14] * cell.natoms   # bcc 8 + 6
+
+
+def test_warren_cowley_special_cases(fcc108):
+    cell, i, j, sh, shell, radii = fcc108
+    # L1_0: alternate (001) planes -> first-shell alpha_AB = 1 - (8/12)/0.5 = -1/3
+    z = np.rint(cell.ideal[:, 2] / (3.59 / 2)).astype(int)
+    sp = (z % 2).astype(np.int32)
+    a = OL.warren_cowley(sp, i, shell, j, 0, 2)
+    assert abs(a[0, 1] + 1.0 / 3.0) < 1e-12 and abs(a[1, 0] + 1.0 / 3.0) < 1e-12
+    # random alloys average alpha -> 0
+    vals = [OL.warren_cowley(OL.random_config(108, [1, 1], s), i, shell, j, 0, 2)[0, 1]
+            for s in range(40)]
+    assert abs(np.mean(vals)) < 0.03
+
+
+def test_integer_counts_rounding():
+    assert OL.integer_counts([1, 1, 1], 4000).tolist() == [1333, 1333, 1334]
+    assert OL.integer_counts([0.5, 0.5], 500).tolist() == [250, 250]
+
+
+def test_ms_classes_reference_figures():
+    # the reference's own numbers: 25 fcc classes to 5.3 A, 327 ternary entries per
+    # absorber (chimera/mslibrary.py:11,16; README.md:267)
+    cl = OM.enumerate_classes("fcc", 3.56, 5.3)
+    assert len(cl) == 25
+    assert len(OM.entry_list(cl, 3)) == 327
+    assert len(OM.entry_list(cl, 2)) == 122
+    assert sum(c.deg for c in cl) == 984
+    assert sum(c.nleg == 3 for c in cl) + sum(c.nleg == 4 for c in cl) == 25
+    # every image is a distinct directed path of the class's length
+    for c in cl:
+        L = c.reff_units()
+        for img, so in c.images:
+            pts = np.vstack([np.zeros((1, 3)), img, np.zeros((1, 3))])
+            assert abs(0.5 * np.linalg.norm(np.diff(pts, axis=0), axis=1).sum() - L) < 1e-9
+
+
+def test_ms_counts_and_delta_brute_force(fcc108):
+    cell, i, j, sh, shell, radii = fcc108
+    cl = OM.enumerate_classes("fcc", 3.59, 5.2)
+    sc = OM.SupercellPaths(cl, 3, cell.ideal, 3.59, (3, 3, 3))
+    assert sc.npath == 108 * sum(c.deg for c in cl)
+    sp = OL.random_config(108, [1, 1, 1], 3)
+    C = sc.counts(sp)
+    assert C.sum() == sc.npath
+    # species marginal: each absorber species appears deg times per atom
+    nb = sc.nblk
+    for a in range(3):
+        assert C[a * nb:(a + 1) * nb].sum() == (sp == a).sum() * sum(c.deg for c in cl)
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        a, b = rng.integers(0, 108, 2)
+        if sp[a] == sp[b]:
+            continue
+        d = sc.delta_counts(sp, a, b, sp[b], sp[a])
+        sp2 = sp.copy()
+        sp2[a], sp2[b] = sp[b], sp[a]
+        assert np.array_equal(sc.counts(sp2) - C, d)
+
+
+def test_pair_space_equals_count_space(fcc108):
+    cell, i, j, sh, shell, radii = fcc108
+    k, f, d, lam, s02 = OTab.synthetic_ss(OTab.k_grid(), [24, 27, 28])
+    sp = OL.random_config(108, [1, 1, 1], 1)
+    chi = OF.chi_pairs(k, f, d, lam, s02, cell.ideal, sp, i, j, sh, 0.006, normalise=False)
+    N = OF.pair_counts(sp, i, j, shell, 3, 4)
+    B = OF.ss_basis(k, f, d, lam, s02, radii, np.full(4, 0.006))
+    np.testing.assert_allclose(OF.chi_counts(N, B), chi, rtol=0, atol=1e-13 * np.abs(chi).max())
+    assert np.all(chi[:, 0] == 0.0)       # the k = 0 term is 0
+
+
+def test_pair_term_closed_form():
+    k = np.array([0.0, 1.0, 4.0])
+    f = np.array([0.0, 2.0, 3.0])
+    d = np.array([0.5, 0.25, -1.0])
+    lam = np.array([6.0, 6.3, 7.2])
+    v = OTab.pair_term(k, 2.5, f, d, lam, 0.006, 0.9, c3=0.001)
+    want = [0.0,
+            0.9 * 2.0 / (1 * 6.25) * np.exp(-5 / 6.3 - 2 * 0.006) * np.sin(5 + 0.25 - 4 / 3 * 0.001),
+            0.9 * 3.0 / (4 * 6.25) * np.exp(-5 / 7.2 - 32 * 0.006) * np.sin(20 - 1 - 4 / 3 * 64 * 0.001)]
+    np.testing.assert_allclose(v, want, rtol=1e-15)
+
+
+def test_synthetic_tables_spot_values():
+    k, f, d, lam, s02 = OTab.synthetic_ss(OTab.k_grid(), [26, 28])
+    assert k.size == 321 and k[80] == 4.0
+    assert f[0, 80] == 1.0 and abs(f[1, 80] - 28 / 26) < 1e-15
+    assert abs(d[0, 1, 0] - (1.2 + 0.05 * 28)) < 1e-15 and lam[0] == 6.0
+
+
+def test_fft_equals_direct_dft_and_hanning():
+    k = OTab.k_grid()
+    t = OT.Transform(k, 3.0, 14.0, 2, 1.0)
+    rng = np.random.default_rng(0)
+    chi = rng.standard_normal(k.size)
+    z_fft = t.to_R(chi)
+    z_dft = t.dft(chi, t.R)
+    np.testing.assert_allclose(z_dft, z_fft, rtol=0, atol=1e-12 * np.abs(z_fft).max())
+    # Hanning: flat top 1, half height at kmin and kmax, 239 non-zero points
+    w = OT.hanning_window(t.k_uniform, 3.0, 14.0, 1.0)
+    assert w[60] == 0.5 and w[280] == 0.5 and w[100] == 1.0
+    assert np.count_nonzero(t.weight) == 239
+
+
+def test_transform_peak_position():
+    k = OTab.k_grid()
+    t = OT.Transform(k, 3.0, 14.0, 0, 1.0)
+    R = np.arange(1.0, 5.0, 0.005)
+    z = t.dft(np.sin(2 * k * 2.7), R)
+    assert abs(R[np.argmax(np.abs(z))] - 2.7) < 0.02
+
+
+def _toy_problem():
+    cell = OL.Supercell("fcc", (3, 3, 3), 3.59)
+    i, j, sh = cell.neighbour_list(5.2)
+    shell, radii = OL.shell_index(cell.ideal, i, j, sh)
+    k, f, d, lam, s02 = OTab.synthetic_ss(OTab.k_grid(), [26, 28])
+    R = np.arange(1.4, 5.0001, 0.02)
+    trs = [OT.Transform(k, 3.0, 14.0, w, 1.0) for e in range(2) for w in (1, 2, 3)]
+    ce = np.repeat(np.arange(2), 3)
+    sp_t = OL.random_config(108, [1, 1], 99)
+    chi_t = OF.chi_pairs(k, f, d, lam, s02, cell.ideal, sp_t, i, j, sh, 0.006)
+    tgt = OT.multi_to_R(trs, ce, chi_t, R=R)
+    P = OR.Problem(k, f, d, lam, s02, cell.ideal, i, j, sh, shell, 0.006, trs, ce, tgt,
+                   np.full(tgt.shape, 0.05), R=R)
+    return P
+
+
+def test_profiled_scale_minimises_chi2():
+    P = _toy_problem()
+    sp = OL.random_config(108, [1, 1], 5)
+    chi = P.chi_full(sp) / np.bincount(sp)[:, None]
+    m = P.to_R(chi)
+    s = P.profiled_scale(m)
+    def c2(scale):
+        d = (m * scale[:, None] - P.tm) / P.sm
+        return (np.abs(d) ** 2).sum()
+    base = c2(s)
+    for e in range(2):
+        for eps in (1e-3, -1e-3):
+            s2 = s.copy()
+            s2[3 * e:3 * e + 3] *= 1 + eps
+            assert c2(s2) > base
+
+
+def test_affected_pairs_dedup_brute_force():
+    P = _toy_problem()
+    for a, b in [(0, 1), (0, 50), (7, 7), (3, 107)]:
+        aff = OR.affected_pairs(a, b, P.nbr, P.rev, P.i, P.j)
+        want = sorted(np.nonzero((P.i == a) | (P.j == a) | ((P.i == b) | (P.j == b)) * (a != b))[0])
+        assert sorted(aff.tolist()) == want
+        assert len(set(aff.tolist())) == len(aff)
+
+
+def test_incremental_chi_matches_recompute():
+    P = _toy_problem()
+    sp = OL.random_config(108, [1, 1], 7)
+    chi = P.chi_full(sp)
+    rng = np.random.default_rng(3)
+    for _ in range(10):
+        a, b = rng.integers(0, 108, 2)
+        if sp[a] == sp[b]:
+            continue
+        d, _ = P.delta_chi(sp, a, b, sp[b], sp[a])
+        sp[a], sp[b] = sp[b], sp[a]
+        chi = chi + d
+    np.testing.assert_allclose(chi, P.chi_full(sp), rtol=0, atol=1e-12 * np.abs(chi).max())
+
+
+def test_exchange_acceptance_detailed_balance():
+    # two rungs, two states: P(swap) pi(x) == P(swap back) pi(x')
+    b1, b2, nfit = 2.0, 0.5, 10.0
+    for E1, E2 in [(1.0, 1.3), (1.3, 1.0), (2.0, 2.0)]:
+        def acc(Ea, Eb):
+            d = (b1 - b2) * (Ea - Eb) * nfit
+            return 1.0 if d >= 0 else np.exp(max(d, -60))
+        pi = np.exp(-nfit * (b1 * E1 + b2 * E2))
+        pi_sw = np.exp(-nfit * (b1 * E2 + b2 * E1))
+        assert abs(pi * acc(E1, E2) - pi_sw * acc(E2, E1)) <= 1e-12 * max(pi, pi_sw)
+
+
+def test_sampler_runs_and_improves():
+    P = _toy_problem()
+    S = OR.PlainSampler(P, nrep=4, seed=1)
+    S.init_state(OL.random_config(108, [1, 1], 5))
+    c0 = S.chi2.copy()
+    S.run(30, exchange_every=10)
+    assert S.best_chi2.min() <= c0.min()
+    # tracked chi equals a recompute for every replica
+    for c in range(4):
+        np.testing.assert_allclose(S.chi[c], P.chi_full(S.species[c]), rtol=0,
+                                   atol=1e-11 * np.abs(S.chi[c]).max())
