@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=200)
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "f64"])
+    ap.add_argument("--reanchor-every", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (replicas shard across GPUs by "
@@ -79,6 +81,8 @@ def main():
     t_build = time.time()
     P = W.build(args.workload)
     S = P.sampler
+    S.precision, S.reanchor_every = args.precision, args.reanchor_every
+    S.init_state(P.cfg0, P.cell.ideal)
     print(f"[bench] built in {time.time() - t_build:.1f} s; warmup {args.warmup}",
           file=sys.stderr, flush=True)
     S.run(args.warmup, exchange_every=args.exchange_every, log_every=10 ** 9)
@@ -94,19 +98,34 @@ def main():
     from chimera_amd import _lib
     ms = np.zeros(4)
     _lib.check(_lib.load().chm_sampler_profile(S._S, args.profile_iters, ms))
-    k1, k2, k3, kstep = ms
-    g_nrow = S.nrow_e
-    ncol_e = S.nkw * S.nwin * 2
-    flops_k2 = 2.0 * S.nrep * P.nsp * g_nrow * ncol_e
-    bytes_k2 = 8.0 * P.nsp * g_nrow * ncol_e                   # basis read once per launch
-    kern = {"propose_gather_ms": k1, "trial_gemm_ms": k2, "accept_ms": k3, "step_ms": kstep}
-    dom = max(("propose_gather", k1), ("trial_gemm", k2), ("accept", k3), key=lambda x: x[1])
-    # dominant kernel roofline: the trial GEMM's algorithmic FP64 FLOPs
-    ach = flops_k2 / (k2 * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
-            "kernel": "k_trial_gemm", "dominant_by_time": dom[0],
-            "hbm_view_GBs": round(bytes_k2 / (k2 * 1e-3) / 1e9, 1)}
+    k_prop, k2, k_slot, kstep = ms
+    nrow, ncol_e, nrep, nsp = S.nrow_e, S.nkw * S.nwin * 2, S.nrep, P.nsp
+    ncol_tot = nsp * ncol_e
+    # algorithmic work of one launch of each kernel of the step
+    flops_k2 = 2.0 * nrep * nsp * nrow * ncol_e                     # one real GEMM
+    ms_stencil = 0 if P.ms is None else P.ms.nstencil
+    nct = -(-ncol_e // 16)
+    bytes_slot = nrep * (P.cell.natoms + 2 * ms_stencil * 16 + 8 * nsp * nct * 4
+                         + 4 * nsp * nrow * 3 + 2 * 8 * ncol_tot)
+    kern = {"slot_step_ms": k_slot, "trial_transform_ms": k2, "step_ms": kstep,
+            "slot_propose_only_ms": k_prop}
+    if args.precision == "bf16x3":
+        peak2, kname = 2500.0 / 3.0, "k_gemm_delta_bf16"        # split-BF16: BF16 dense / 3
+    else:
+        peak2, kname = FP64_PEAK_TFLOPS, "k_gemm_abs_f64"
+    rooflines = {
+        "k_slot_step": {"bound": "hbm", "achieved": bytes_slot / (k_slot * 1e-3) / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+        kname: {"bound": "mfma", "achieved": flops_k2 / (k2 * 1e-3) / 1e12, "peak": peak2,
+                "unit": "TFLOP/s"},
+    }
+    dom = "k_slot_step" if k_slot >= k2 else kname
+    r = rooflines[dom]
+    roof = {"bound": r["bound"], "achieved": round(r["achieved"], 3), "peak": r["peak"],
+            "unit": r["unit"], "frac": round(r["achieved"] / r["peak"], 5), "traffic": None,
+            "kernel": dom,
+            "all": {k: {"achieved": round(v["achieved"], 3), "unit": v["unit"],
+                        "frac": round(v["achieved"] / v["peak"], 5)} for k, v in rooflines.items()}}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -115,6 +134,7 @@ def main():
 
     # alpha and R-factor of the best state (reported beside the rate)
     bsp, _, bc2 = S.best()
+    nanchor = int(S.reanchors().sum()) if args.precision == "bf16x3" else 0
     out = {
         "metric": "RMC swap moves/s & alpha_1 ESS/s at fixed R-factor, learned vs uniform proposal",
         "value": round(moves, 1),
@@ -122,7 +142,8 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * dt / args.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64 chi^2; " + ("bf16x3 MFMA delta + f64 accumulator" if args.precision == "bf16x3"
+                                    else "f64 transform"),
         "data": "synthetic (seeded tables, seeded reference configuration + noise)",
         "config": {"workload": args.workload, "natoms": int(P.cell.natoms),
                    "replicas": int(S.nrep), "edges": int(P.nsp), "channels": int(P.nchan),
@@ -133,6 +154,7 @@ def main():
         "cpu_baseline": cpu,
         "gpu_over_cpu": None if cpu is None else round(moves / cpu["value"], 1),
         "best_chi2": round(float(bc2), 5),
+        "fp64_reanchors": nanchor,
     }
     print(json.dumps(out))
 

@@ -68,6 +68,7 @@ _SIGS = {
     "chm_sampler_exchange": (C.c_int, [vp]),
     "chm_sampler_chik": (C.c_int, [vp, C.c_int, i32p, f64p]),
     "chm_sampler_profile": (C.c_int, [vp, C.c_int, f64p]),
+    "chm_sampler_set_precision": (C.c_int, [vp, C.c_int, C.c_int]),
 }
 
 EXPORTS = tuple(_SIGS)

@@ -24,6 +24,7 @@
 //
 // Computing chi(R)_trial from the exact integer counts every step means the running
 // state never drifts: there is no floating-point accumulator to re-anchor.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -51,6 +52,8 @@ struct Geo {
     int box[3], ncell[3], nbasis;
     int bpar[8];
     int has_const;
+    int nct, nkb;   // 16-column tiles per edge, 32-row K blocks per edge
+    int bxyz[12];   // coordinates (units a0/2) of each basis site in the cell
 };
 
 __device__ __forceinline__ int site_index(const Geo &g, int x, int y, int z) {
@@ -99,104 +102,6 @@ __device__ __forceinline__ void load_species(uint8_t *lsp, const uint8_t *gsp, i
     const uint4 *src = reinterpret_cast<const uint4 *>(gsp);
     uint4 *dst = reinterpret_cast<uint4 *>(lsp);
     for (int q = threadIdx.x; q < natoms_pad / 16; q += blockDim.x) dst[q] = src[q];
-}
-
-// ---------------------------------------------------------------- K1
-// grid (nsplit, nrep); block 256.  LDS: species (natoms_pad B) + hist (nrow_tot int).
-__global__ __launch_bounds__(256) void k_propose_gather(
-    Geo g, const uint8_t *__restrict__ species, const int *__restrict__ rung_of,
-    const uint64_t *__restrict__ stepc_arr, uint64_t seed, const int *__restrict__ site_xyz,
-    const int *__restrict__ nbr_j, const int *__restrict__ nbr_sh,
-    const int4 *__restrict__ stencil, int *__restrict__ dcounts, int *__restrict__ prop_out) {
-    extern __shared__ int lds[];
-    int *hist = lds;
-    uint8_t *lsp = reinterpret_cast<uint8_t *>(lds + ((g.nrow_tot + 3) & ~3));
-    __shared__ Prop P;
-    const int s = blockIdx.y;
-    load_species(lsp, species + (long)s * g.natoms_pad, g.natoms_pad);
-    for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) hist[q] = 0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        P = propose(lsp, g.natoms, seed, stepc_arr[s] + 1, (uint64_t)rung_of[s]);
-        if (blockIdx.x == 0) {
-            int *o = prop_out + 5 * s;
-            o[0] = P.a;
-            o[1] = P.b;
-            o[2] = P.na;
-            o[3] = P.nb;
-            o[4] = P.sw;
-        }
-    }
-    __syncthreads();
-    const Prop p = P;
-    if (!p.sw) return;
-    const int nss = 2 * g.ncoord;
-    const int nitems = nss + 2 * g.nstencil;
-    const int stride = gridDim.x * blockDim.x;
-    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < nitems; it += stride) {
-        if (it < nss) {
-            // ordered pairs x->y and y->x for every neighbour y of x in {a, b}
-            const int x = (it < g.ncoord) ? p.a : p.b;
-            const int o = (it < g.ncoord) ? it : it - g.ncoord;
-            const int y = nbr_j[x * g.ncoord + o];
-            if (x == p.b && y == p.a) continue;  // covered from a's side
-            const int sh = nbr_sh[x * g.ncoord + o];
-            const int sx = lsp[x], sy = lsp[y];
-            const int nx = (x == p.a) ? p.na : p.nb;
-            const int ny = (y == p.a) ? p.na : ((y == p.b) ? p.nb : sy);
-            atomicAdd(&hist[sx * g.nrow_e + sy * g.nshell + sh], -1);
-            atomicAdd(&hist[nx * g.nrow_e + ny * g.nshell + sh], 1);
-            atomicAdd(&hist[sy * g.nrow_e + sx * g.nshell + sh], -1);
-            atomicAdd(&hist[ny * g.nrow_e + nx * g.nshell + sh], 1);
-        } else {
-            const int m = it - nss;
-            const int x = (m < g.nstencil) ? p.a : p.b;
-            const int4 rec = stencil[(m < g.nstencil) ? m : m - g.nstencil];
-            const int8_t *rb = reinterpret_cast<const int8_t *>(&rec);
-            const int nslot = rb[12], wgt = rb[13];
-            const int cls_off = (int)(uint8_t)rb[14] | ((int)rb[15] << 8);
-            const int xyz = site_xyz[x];
-            const int x0 = xyz & 1023, y0 = (xyz >> 10) & 1023, z0 = (xyz >> 20) & 1023;
-            int sites[4];
-            bool has_a = false;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (q > nslot) {
-                    sites[q] = -1;
-                    continue;
-                }
-                int X = wrap(x0 + rb[3 * q], g.box[0]);
-                int Y = wrap(y0 + rb[3 * q + 1], g.box[1]);
-                int Z = wrap(z0 + rb[3 * q + 2], g.box[2]);
-                sites[q] = site_index(g, X, Y, Z);
-                has_a |= (sites[q] == p.a);
-            }
-            if (x == p.b && has_a) continue;  // a path through both sites counts once
-            int eo = 0, en = 0, pw = 1;
-#pragma unroll
-            for (int q = 1; q < 4; ++q) {
-                if (q > nslot) break;
-                const int st = sites[q];
-                const int so = lsp[st];
-                const int sn = (st == p.a) ? p.na : ((st == p.b) ? p.nb : so);
-                eo += so * pw;
-                en += sn * pw;
-                pw *= g.nsp;
-            }
-            const int sa0 = lsp[sites[0]];
-            const int sa1 = (sites[0] == p.a) ? p.na : ((sites[0] == p.b) ? p.nb : sa0);
-            eo += sa0 * g.nrow_e + g.nss_rows + cls_off;
-            en += sa1 * g.nrow_e + g.nss_rows + cls_off;
-            if (eo != en) {
-                atomicAdd(&hist[eo], -wgt);
-                atomicAdd(&hist[en], wgt);
-            }
-        }
-    }
-    __syncthreads();
-    int *dc = dcounts + (long)s * g.nrow_tot;
-    for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x)
-        if (hist[q] != 0) atomicAdd(&dc[q], hist[q]);
 }
 
 // ---------------------------------------------------------------- full counts
@@ -251,23 +156,68 @@ __global__ __launch_bounds__(256) void k_full_counts(
 }
 
 // ---------------------------------------------------------------- K2
-// mtrial[s][e*ncol_e + col] = sum_k (counts + dcounts)[s][e*nrow_e + k] * basisR[e][k][col]
-// Workgroup: 16 columns x 16 K-slices, RB replica rows held per lane.
-constexpr int RB = 32;
-constexpr int NCW = 16;
-constexpr int KS = 16;
+// Both trial-transform kernels end in the same epilogue: for a 32-slot x 16-column tile
+// of one edge, m' (the trial chi(R) window values) is written out and, per slot, four
+// sums of the residual d = m' - t are reduced over the tile's 16 columns:
+//   U1 = sum d t,  U2 = sum d^2,  W1 = sum w d t,  W2 = sum w d^2      (w = 1/sigma^2)
+// The profiled amplitude s = sum m't / sum m'^2 (method_a_rmc.py:190-209) then has
+// s - 1 = -(U1 + U2) / (U2 + 2 U1 + T0) and the edge's chi^2 nfit share is
+// s^2 W2 + 2 s (s-1) W1 + (s-1)^2 Q0, with T0 = sum t^2, Q0 = sum w t^2: every term
+// is small near a fit, so nothing cancels (the plain expansion in m' loses ~6 digits).
+constexpr int RB = 32;    // replica slots per tile
+constexpr int NCW = 16;   // columns per tile
+constexpr int KS = 16;    // K slices of the FP64 kernel
 
-__global__ __launch_bounds__(256) void k_trial_gemm(Geo g, int nrep, const int *__restrict__ counts,
-                                                     const int *__restrict__ dcounts,
-                                                     const double *__restrict__ basisR,
-                                                     double *__restrict__ mtrial) {
+struct TileOut {
+    const double *target;   // (ncol_tot)
+    const double *wcol;     // (ncol_tot) 1/sigma^2 of each column
+    double *mtrial;         // (nrep, ncol_tot)
+    double *partials;       // (nrep, nsp, nct, 4)
+};
+
+// lanes t..t+15 (t % 16 == 0) hold the 16 columns of one slot
+__device__ __forceinline__ void tile_epilogue(const Geo &g, const TileOut &O, int nrep, int slot,
+                                              int e, int ct, int col, double mp) {
+    const bool valid = slot < nrep && col < g.ncol_e;
+    double p1 = 0.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
+    if (valid) {
+        const long gc = (long)e * g.ncol_e + col;
+        O.mtrial[(long)slot * g.ncol_tot + gc] = mp;
+        const double t = O.target[gc], w = O.wcol[gc];
+        const double d = mp - t;
+        p1 = d * t;
+        p2 = d * d;
+        q1 = p1 * w;
+        q2 = p2 * w;
+    }
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) {
+        p1 += __shfl_xor(p1, m, 16);
+        p2 += __shfl_xor(p2, m, 16);
+        q1 += __shfl_xor(q1, m, 16);
+        q2 += __shfl_xor(q2, m, 16);
+    }
+    if (slot < nrep && (threadIdx.x & 15) == 0) {
+        double *P = O.partials + (((long)slot * g.nsp + e) * g.nct + ct) * 4;
+        P[0] = p1;
+        P[1] = p2;
+        P[2] = q1;
+        P[3] = q2;
+    }
+}
+
+// Absolute FP64: m' = (counts + delta) @ basis_R.  Workgroup: 16 columns x 16 K-slices,
+// RB replica rows held per lane; the K-slice partials are summed in a fixed order.
+__global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int *__restrict__ counts,
+                                                       const int *__restrict__ dcounts,
+                                                       const double *__restrict__ basisR, TileOut O) {
     extern __shared__ double dl[];
     const int e = blockIdx.z;
     const int r0 = blockIdx.y * RB;
-    const int c0 = blockIdx.x * NCW;
+    const int ct = blockIdx.x;
+    const int c0 = ct * NCW;
     const int K = g.nrow_e;
     const int ld = RB + 1;
-    // stage the coefficients of this edge for RB replicas: As[k][m]
     for (int q = threadIdx.x; q < RB * K; q += blockDim.x) {
         int m = q / K, k = q % K;
         int slot = r0 + m;
@@ -296,50 +246,185 @@ __global__ __launch_bounds__(256) void k_trial_gemm(Geo g, int nrep, const int *
         }
     }
     __syncthreads();
-    // partials P[ks][m][tn]
     double *P = dl;
 #pragma unroll
     for (int m = 0; m < RB; ++m) P[(ks * RB + m) * NCW + tn] = acc[m];
     __syncthreads();
-    for (int q = threadIdx.x; q < RB * NCW; q += blockDim.x) {
-        int m = q / NCW, t = q % NCW;
-        int slot = r0 + m, cc = c0 + t;
-        if (slot >= nrep || cc >= g.ncol_e) continue;
+    // thread t: column t % 16, slots t/16 and t/16 + 16
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int m = t / NCW + 16 * h, cl = t % NCW;
         double sum = 0.0;
-        for (int z = 0; z < KS; ++z) sum += P[(z * RB + m) * NCW + t];
-        mtrial[(long)slot * g.ncol_tot + (long)e * g.ncol_e + cc] = sum;
+        for (int z = 0; z < KS; ++z) sum += P[(z * RB + m) * NCW + cl];
+        tile_epilogue(g, O, nrep, r0 + m, e, ct, c0 + cl, sum);
     }
 }
 
-// ---------------------------------------------------------------- K3
-struct AccArgs {
-    const double *target;   // (ncol_tot) fit window [re, im] per (chan, j)
-    const double *wsig;     // (ncol_tot/2) 1/sigma^2
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __bf16 bf16_of_small_int(int v) {
+    // |v| <= 128: exactly representable; the float's top 16 bits are the bf16
+    float f = (float)v;
+    return __builtin_bit_cast(__bf16, (unsigned short)(__builtin_bit_cast(unsigned int, f) >> 16));
+}
+
+// Incremental split-BF16: delta(R) = delta_counts @ basis_R with basis_R = hi + mid + lo
+// (three bf16 planes) and the small integer deltas exact in bf16 (|d| > 128 split into
+// 256*hi + lo), on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; then
+// m' = m_cur (FP64 running accumulator) + delta.  Workgroup = 4 waves splitting the K
+// blocks of one 32-slot x 16-column tile; their partial tiles are summed in LDS.
+// Operand layouts of the 16x16x32 MFMA: lane l holds A[l%16][8(l/16)+i] and
+// B[8(l/16)+i][l%16] (i < 8); D lane l holds D[4(l/16)+i][l%16] (i < 4).
+__global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const int *__restrict__ dcounts,
+                                                          const bf16x8 *__restrict__ Tb,
+                                                          const double *__restrict__ mcur, TileOut O) {
+    __shared__ float red[4][RB][NCW + 1];
+    const int e = blockIdx.z;
+    const int r0 = blockIdx.y * RB;
+    const int ct = blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int per = (g.nkb + 3) / 4;
+    const int kb0 = wave * per, kb1 = min(g.nkb, kb0 + per);
+    f32x4 acc[2], acch[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[rb][i] = acch[rb][i] = 0.f;
+    const int K = g.nrow_e;
+    for (int kb = kb0; kb < kb1; ++kb) {
+        const bf16x8 *Bp = Tb + ((((long)e * g.nct + ct) * g.nkb + kb) * 3) * 64 + lane;
+        const bf16x8 b0 = Bp[0], b1 = Bp[64], b2 = Bp[128];
+        const int k0 = kb * 32 + 8 * (lane >> 4);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+            const int slot = r0 + rb * 16 + (lane & 15);
+            bf16x8 alo, ahi;
+            bool hi_any = false;
+            const int *src = dcounts + (long)slot * g.nrow_tot + (long)e * K;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                int v = (slot < nrep && k0 + i < K) ? src[k0 + i] : 0;
+                int lo = ((v + 128) & 255) - 128;
+                int hi = (v - lo) >> 8;
+                hi_any |= (hi != 0);
+                alo[i] = bf16_of_small_int(lo);
+                ahi[i] = bf16_of_small_int(hi);
+            }
+            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b0, acc[rb], 0, 0, 0);
+            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b1, acc[rb], 0, 0, 0);
+            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b2, acc[rb], 0, 0, 0);
+            if (__any(hi_any)) {
+                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b0, acch[rb], 0, 0, 0);
+                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b1, acch[rb], 0, 0, 0);
+                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b2, acch[rb], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            red[wave][rb * 16 + 4 * (lane >> 4) + i][lane & 15] = acc[rb][i] + 256.f * acch[rb][i];
+    __syncthreads();
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int m = t / NCW + 16 * h, cl = t % NCW;
+        const float d = red[0][m][cl] + red[1][m][cl] + red[2][m][cl] + red[3][m][cl];
+        const int slot = r0 + m, col = ct * NCW + cl;
+        double mp = 0.0;
+        if (slot < nrep && col < g.ncol_e)
+            mp = mcur[(long)slot * g.ncol_tot + (long)e * g.ncol_e + col] + (double)d;
+        tile_epilogue(g, O, nrep, slot, e, ct, col, mp);
+    }
+}
+
+// basis_R (nsp, nrow_e, ncol_e) FP64 -> three bf16 planes in MFMA B-operand order:
+// Tb[e][ct][kb][plane][lane] = 8 bf16 of column ct*16 + lane%16, rows kb*32 + 8(lane/16) + i.
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+    unsigned int u = __builtin_bit_cast(unsigned int, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_float(unsigned short h) {
+    return __builtin_bit_cast(float, (unsigned int)h << 16);
+}
+
+__global__ void k_split_bf16(Geo g, const double *__restrict__ basisR, unsigned short *__restrict__ Tb) {
+    long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)g.nsp * g.nct * g.nkb * 64 * 8;
+    if (tid >= n) return;
+    int i = (int)(tid % 8);
+    long r = tid / 8;
+    int lane = (int)(r % 64);
+    r /= 64;
+    int kb = (int)(r % g.nkb);
+    r /= g.nkb;
+    int ct = (int)(r % g.nct);
+    int e = (int)(r / g.nct);
+    int col = ct * 16 + (lane & 15);
+    int k = kb * 32 + 8 * (lane >> 4) + i;
+    double v = (col < g.ncol_e && k < g.nrow_e) ? basisR[((long)e * g.nrow_e + k) * g.ncol_e + col] : 0.0;
+    unsigned short h0 = bf16_rne((float)v);
+    double r1 = v - (double)bf16_to_float(h0);
+    unsigned short h1 = bf16_rne((float)r1);
+    double r2 = r1 - (double)bf16_to_float(h1);
+    unsigned short h2 = bf16_rne((float)r2);
+    long base = ((((long)e * g.nct + ct) * g.nkb + kb) * 3) * 64 * 8 + (long)lane * 8 + i;
+    Tb[base] = h0;
+    Tb[base + 64 * 8] = h1;
+    Tb[base + 2 * 64 * 8] = h2;
+}
+
+// ---------------------------------------------------------------- K3+K1 per slot
+// One workgroup (1024 lanes) per replica slot.  Accept part (step t): the chi^2 of the
+// trial from the GEMM's per-tile sums, Metropolis with a counter-based uniform
+// (method_a_rmc.py:278-285), commit of counts / species / running chi(R), best
+// retention (:301-307), move log, FP64 re-anchor.  Propose part (step t+1):
+// splitmix64 proposal (kernels.py:362-398) and the gather of every directed pair and
+// multiple-scattering path through the two swapped sites (kernels.py:425-468,
+// chimera/mslibrary.py:466-486) into a dense per-slot row of count changes.  Neighbour
+// and path sites come from integer lattice arithmetic on the site index (no dependent
+// table loads); the histogram lives in LDS and is stored with plain coalesced writes.
+struct SlotArgs {
+    const double *partials; // (nrep, nsp, nct, 4)
+    const double *q0;       // (nsp, 2): per edge T0 = sum t^2 and Q0 = sum t^2 w
     const double *mtrial;   // (nrep, ncol_tot)
+    const double *target, *wcol;
+    const double *basisR;   // FP64, for re-anchoring
+    double *mcur;           // (nrep, ncol_tot) running accumulator (incremental mode)
     const double *beta;     // per rung
-    const int *prop;        // (nrep, 5)
-    int *counts, *dcounts;
+    int *prop;              // (nrep, 5) a, b, new a, new b, is-swap
+    int *counts, *dcounts;  // (nrep, nrow_tot)
     uint8_t *species, *best_sp;
     double *c2, *best_c2;
     const int *rung_of;
     uint64_t *stepc;
     long long *acc_rung;
+    int *anchor;            // accepted moves since the last FP64 re-anchor, per slot
+    long long *nanchor;     // re-anchors done, per slot
+    const int *ss_sten;     // (ncoord) packed dx,dy,dz,shell (int8 each)
+    const int4 *ms_sten;    // (nstencil) 16-byte path records
+    int reanchor_every;     // 0: never
+    int incremental;
     int *log;               // (log_cap, nrep, 4) or null
     int log_cap;
     int *series;            // (series_cap, nsp*nsp*nshell) or null
     int series_cap, series_rung;
     uint64_t seed;
     int profile;
-    int mode;               // 0 step, 1 init, 2 evaluate rows of mtrial into c2
+    int accept;             // 0: none, 1: step, 2: init (store chi^2, no decision)
+    int propose;            // 1: propose + gather the next step
     double nfit;
 };
 
 __device__ double block_sum(double v, double *red) {
-    // fixed-order tree over the 256 lanes
     __syncthreads();
     red[threadIdx.x] = v;
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
         if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
         __syncthreads();
     }
@@ -348,103 +433,286 @@ __device__ double block_sum(double v, double *red) {
     return r;
 }
 
-__global__ __launch_bounds__(256) void k_accept(Geo g, AccArgs A) {
+// chi^2 from the per-edge residual sums {U1, U2, W1, W2}
+__device__ double chi2_of_sums(const Geo &g, const double (*S)[4], const double *q0, int profile,
+                               double nfit) {
+    double tot = 0.0;
+    for (int e = 0; e < g.nsp; ++e) {
+        const double U1 = S[e][0], U2 = S[e][1], W1 = S[e][2], W2 = S[e][3];
+        const double T0 = q0[2 * e], Q0 = q0[2 * e + 1];
+        double s = 1.0, sm1 = 0.0;
+        if (profile) {
+            const double P2 = fmax(U2 + 2.0 * U1 + T0, 1e-30);
+            const double P1 = U1 + T0;
+            s = P1 / P2;
+            sm1 = (P1 - P2) / P2;
+        }
+        tot += s * s * W2 + 2.0 * s * sm1 * W1 + sm1 * sm1 * Q0;
+    }
+    return tot / nfit;
+}
+
+// integer coordinates (units a0/2) of a site, from its index
+__device__ __forceinline__ void site_xyz_of(const Geo &g, int q, int &x, int &y, int &z) {
+    const int cell = q / g.nbasis, b = q - cell * g.nbasis;
+    const int cz = cell % g.ncell[2];
+    const int cy = (cell / g.ncell[2]) % g.ncell[1];
+    const int cx = cell / (g.ncell[2] * g.ncell[1]);
+    x = 2 * cx + g.bxyz[3 * b];
+    y = 2 * cy + g.bxyz[3 * b + 1];
+    z = 2 * cz + g.bxyz[3 * b + 2];
+}
+
+__global__ __launch_bounds__(1024) void k_slot_step(Geo g, SlotArgs A) {
+    extern __shared__ int lds[];
+    int *hist = lds;
+    uint8_t *lsp = reinterpret_cast<uint8_t *>(lds + ((g.nrow_tot + 3) & ~3));
+    __shared__ double sums[chm::kMaxNsp][4];
+    __shared__ double red[1024];
+    __shared__ int flag;
+    __shared__ Prop P;
+    __shared__ uint64_t step_sh;
+    const int s = blockIdx.x;
+    const long base_sp = (long)s * g.natoms_pad;
+    load_species(lsp, A.species + base_sp, g.natoms_pad);
+    if (threadIdx.x == 0) step_sh = A.stepc[s];
+    if (A.accept && threadIdx.x < 4 * g.nsp) {
+        const int e = threadIdx.x >> 2, j = threadIdx.x & 3;
+        const double *Pp = A.partials + ((long)s * g.nsp + e) * g.nct * 4 + j;
+        double v = 0.0;
+        for (int c = 0; c < g.nct; ++c) v += Pp[4 * c];
+        sums[e][j] = v;
+    }
+    __syncthreads();
+    const int rung = A.rung_of[s];
+    int *cn = A.counts + (long)s * g.nrow_tot;
+    int *dc = A.dcounts + (long)s * g.nrow_tot;
+    if (A.accept) {
+        const int *pp = A.prop + 5 * s;
+        if (threadIdx.x == 0) {
+            const double c2t = chi2_of_sums(g, sums, A.q0, A.profile, A.nfit);
+            int acc = 0;
+            if (A.accept == 2) {
+                A.c2[s] = c2t;
+                A.best_c2[s] = c2t;
+                acc = 1;
+            } else {
+                const uint64_t step = step_sh + 1;
+                if (pp[4]) {
+                    double x = A.beta[rung] * (c2t - A.c2[s]) * A.nfit;
+                    uint64_t h = splitmix64(A.seed ^ (step * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
+                    double u = u01(splitmix64(h));
+                    acc = (x <= 0.0) || (u < exp(-fmin(x, 60.0)));
+                }
+                if (acc) {
+                    lsp[pp[0]] = (uint8_t)pp[2];
+                    lsp[pp[1]] = (uint8_t)pp[3];
+                    A.species[base_sp + pp[0]] = (uint8_t)pp[2];
+                    A.species[base_sp + pp[1]] = (uint8_t)pp[3];
+                    A.c2[s] = c2t;
+                    A.acc_rung[rung] += 1;
+                }
+                if (A.log && step - 1 < (uint64_t)A.log_cap) {
+                    int *L = A.log + ((long)(step - 1) * gridDim.x + s) * 4;
+                    L[0] = pp[0];
+                    L[1] = pp[1];
+                    L[2] = acc;
+                    L[3] = rung;
+                }
+                A.stepc[s] = step;
+                step_sh = step;
+            }
+            flag = acc;
+        }
+        __syncthreads();
+        const int acc = flag;
+        if (acc && A.accept == 1)
+            for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) cn[q] += dc[q];
+        if (A.incremental && acc) {
+            const double *mt = A.mtrial + (long)s * g.ncol_tot;
+            double *mc = A.mcur + (long)s * g.ncol_tot;
+            for (int q = threadIdx.x; q < g.ncol_tot; q += blockDim.x) mc[q] = mt[q];
+        }
+        __syncthreads();
+        const uint64_t step = step_sh;
+        if (A.accept == 1 && A.series && rung == A.series_rung && step - 1 < (uint64_t)A.series_cap) {
+            const int nss = g.nsp * g.nsp * g.nshell;
+            for (int q = threadIdx.x; q < nss; q += blockDim.x) {
+                int a = q / (g.nsp * g.nshell), r = q % (g.nsp * g.nshell);
+                A.series[(long)(step - 1) * nss + q] = cn[a * g.nrow_e + r];
+            }
+        }
+        // FP64 re-anchor of the running accumulator every reanchor_every accepted moves
+        if (A.accept == 1 && A.incremental && A.reanchor_every > 0 && acc) {
+            if (threadIdx.x == 0) {
+                const int n = A.anchor[s] + 1;
+                A.anchor[s] = (n >= A.reanchor_every) ? 0 : n;
+                flag = (n >= A.reanchor_every);
+            }
+            __syncthreads();
+            if (flag) {
+                double *mc = A.mcur + (long)s * g.ncol_tot;
+                for (int q = threadIdx.x; q < g.ncol_tot; q += blockDim.x) {
+                    const int e = q / g.ncol_e, col = q % g.ncol_e;
+                    const double *B = A.basisR + (long)e * g.nrow_e * g.ncol_e + col;
+                    const int *c = cn + e * g.nrow_e;
+                    double v = 0.0;
+                    for (int k = 0; k < g.nrow_e; ++k) v = fma((double)c[k], B[(long)k * g.ncol_e], v);
+                    mc[q] = v;
+                }
+                __syncthreads();
+                for (int e = 0; e < g.nsp; ++e) {
+                    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+                    for (int q = threadIdx.x; q < g.ncol_e; q += blockDim.x) {
+                        const long o = (long)e * g.ncol_e + q;
+                        const double t = A.target[o], w = A.wcol[o], d = mc[o] - t;
+                        l0 += d * t;
+                        l1 += d * d;
+                        l2 += d * t * w;
+                        l3 += d * d * w;
+                    }
+                    l0 = block_sum(l0, red);
+                    l1 = block_sum(l1, red);
+                    l2 = block_sum(l2, red);
+                    l3 = block_sum(l3, red);
+                    if (threadIdx.x == 0) {
+                        sums[e][0] = l0;
+                        sums[e][1] = l1;
+                        sums[e][2] = l2;
+                        sums[e][3] = l3;
+                    }
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    A.c2[s] = chi2_of_sums(g, sums, A.q0, A.profile, A.nfit);
+                    A.nanchor[s] += 1;
+                }
+            }
+        }
+        __syncthreads();
+        const double cur = A.c2[s];
+        if (A.accept == 2 || cur < A.best_c2[s]) {
+            for (int q = threadIdx.x; q < g.natoms_pad; q += blockDim.x) A.best_sp[base_sp + q] = lsp[q];
+            __syncthreads();
+            if (threadIdx.x == 0) A.best_c2[s] = cur;
+        }
+    }
+    if (!A.propose) return;
+    // ------------------------------------------------ propose + gather, step t+1
+    for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) hist[q] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        P = propose(lsp, g.natoms, A.seed, step_sh + 1, (uint64_t)rung);
+        int *o = A.prop + 5 * s;
+        o[0] = P.a;
+        o[1] = P.b;
+        o[2] = P.na;
+        o[3] = P.nb;
+        o[4] = P.sw;
+    }
+    __syncthreads();
+    const Prop p = P;
+    if (p.sw) {
+        int ax, ay, az, bx, by, bz;
+        site_xyz_of(g, p.a, ax, ay, az);
+        site_xyz_of(g, p.b, bx, by, bz);
+        const int nss = 2 * g.ncoord;
+        const int nitems = nss + 2 * g.nstencil;
+        for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+            if (it < nss) {
+                // ordered pairs x->y and y->x for every neighbour y of x in {a, b}
+                const bool on_a = it < g.ncoord;
+                const int sv = A.ss_sten[on_a ? it : it - g.ncoord];
+                const int dx = (int)(int8_t)(sv & 0xFF), dy = (int)(int8_t)((sv >> 8) & 0xFF);
+                const int dz = (int)(int8_t)((sv >> 16) & 0xFF), sh = (sv >> 24) & 0xFF;
+                const int x = on_a ? p.a : p.b;
+                const int y = site_index(g, wrap((on_a ? ax : bx) + dx, g.box[0]),
+                                         wrap((on_a ? ay : by) + dy, g.box[1]),
+                                         wrap((on_a ? az : bz) + dz, g.box[2]));
+                if (x == p.b && y == p.a) continue;  // covered from a's side
+                const int sx = lsp[x], sy = lsp[y];
+                const int nx = (x == p.a) ? p.na : p.nb;
+                const int ny = (y == p.a) ? p.na : ((y == p.b) ? p.nb : sy);
+                atomicAdd(&hist[sx * g.nrow_e + sy * g.nshell + sh], -1);
+                atomicAdd(&hist[nx * g.nrow_e + ny * g.nshell + sh], 1);
+                atomicAdd(&hist[sy * g.nrow_e + sx * g.nshell + sh], -1);
+                atomicAdd(&hist[ny * g.nrow_e + nx * g.nshell + sh], 1);
+            } else {
+                const int m = it - nss;
+                const bool on_a = m < g.nstencil;
+                const int4 rec = A.ms_sten[on_a ? m : m - g.nstencil];
+                const int8_t *rb = reinterpret_cast<const int8_t *>(&rec);
+                const int nslot = rb[12], wgt = rb[13];
+                const int cls_off = (int)(uint8_t)rb[14] | ((int)rb[15] << 8);
+                const int x0 = on_a ? ax : bx, y0 = on_a ? ay : by, z0 = on_a ? az : bz;
+                int sites[4];
+                bool has_a = false;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (q > nslot) {
+                        sites[q] = -1;
+                        continue;
+                    }
+                    sites[q] = site_index(g, wrap(x0 + rb[3 * q], g.box[0]),
+                                          wrap(y0 + rb[3 * q + 1], g.box[1]),
+                                          wrap(z0 + rb[3 * q + 2], g.box[2]));
+                    has_a |= (sites[q] == p.a);
+                }
+                if (!on_a && has_a) continue;  // a path through both sites counts once
+                int eo = 0, en = 0, pw = 1;
+#pragma unroll
+                for (int q = 1; q < 4; ++q) {
+                    if (q > nslot) break;
+                    const int st = sites[q];
+                    const int so = lsp[st];
+                    const int sn = (st == p.a) ? p.na : ((st == p.b) ? p.nb : so);
+                    eo += so * pw;
+                    en += sn * pw;
+                    pw *= g.nsp;
+                }
+                const int sa0 = lsp[sites[0]];
+                const int sa1 = (sites[0] == p.a) ? p.na : ((sites[0] == p.b) ? p.nb : sa0);
+                eo += sa0 * g.nrow_e + g.nss_rows + cls_off;
+                en += sa1 * g.nrow_e + g.nss_rows + cls_off;
+                if (eo != en) {
+                    atomicAdd(&hist[eo], -wgt);
+                    atomicAdd(&hist[en], wgt);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) dc[q] = hist[q];
+}
+
+// chi^2 of arbitrary window transforms (one row per workgroup), direct form:
+// scale from the sums, then sum |s m - t|^2 w (method_a_rmc.py:211-227)
+__global__ __launch_bounds__(256) void k_eval_rows(Geo g, const double *__restrict__ rows,
+                                                    const double *__restrict__ target,
+                                                    const double *__restrict__ wcol, int profile,
+                                                    double nfit, double *__restrict__ out) {
     __shared__ double red[256];
     __shared__ double scale[chm::kMaxNsp];
-    __shared__ int accd;
-    const int s = blockIdx.x;
-    const double *m = A.mtrial + (long)s * g.ncol_tot;
-    const int ncplx_e = g.ncol_e / 2;
-    // per-edge profiled amplitude (method_a_rmc.py:190-209)
+    const double *m = rows + (long)blockIdx.x * g.ncol_tot;
     for (int e = 0; e < g.nsp; ++e) {
         double num = 0.0, den = 0.0;
-        if (A.profile) {
-            for (int q = threadIdx.x; q < ncplx_e; q += blockDim.x) {
-                long o = (long)e * g.ncol_e + 2 * q;
-                double mr = m[o], mi = m[o + 1];
-                num += mr * A.target[o] + mi * A.target[o + 1];
-                den += mr * mr + mi * mi;
-            }
-            num = block_sum(num, red);
-            den = block_sum(den, red);
+        for (int q = threadIdx.x; q < g.ncol_e; q += blockDim.x) {
+            const double v = m[(long)e * g.ncol_e + q];
+            num += v * target[(long)e * g.ncol_e + q];
+            den += v * v;
         }
-        if (threadIdx.x == 0) scale[e] = A.profile ? num / fmax(den, 1e-30) : 1.0;
+        num = block_sum(num, red);
+        den = block_sum(den, red);
+        if (threadIdx.x == 0) scale[e] = profile ? num / fmax(den, 1e-30) : 1.0;
     }
     __syncthreads();
     double res = 0.0;
-    for (int q = threadIdx.x; q < g.ncol_tot / 2; q += blockDim.x) {
-        int e = q / ncplx_e;
-        double se = scale[e];
-        double dr = se * m[2 * q] - A.target[2 * q];
-        double di = se * m[2 * q + 1] - A.target[2 * q + 1];
-        res += (dr * dr + di * di) * A.wsig[q];
+    for (int q = threadIdx.x; q < g.ncol_tot; q += blockDim.x) {
+        const double d = scale[q / g.ncol_e] * m[q] - target[q];
+        res += d * d * wcol[q];
     }
     res = block_sum(res, red);
-    const double c2t = res / A.nfit;
-    const int *pp = A.prop + 5 * s;
-    const int rung = A.rung_of[s];
-    uint64_t step = A.stepc[s] + 1;
-    if (A.mode == 2) {
-        if (threadIdx.x == 0) A.c2[s] = c2t;
-        return;
-    }
-    if (A.mode == 1) {
-        if (threadIdx.x == 0) {
-            A.c2[s] = c2t;
-            A.best_c2[s] = c2t;
-        }
-        for (int q = threadIdx.x; q < g.natoms_pad; q += blockDim.x)
-            A.best_sp[(long)s * g.natoms_pad + q] = A.species[(long)s * g.natoms_pad + q];
-        return;
-    }
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        if (pp[4]) {
-            double x = A.beta[rung] * (c2t - A.c2[s]) * A.nfit;
-            uint64_t h = splitmix64(A.seed ^ (step * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
-            double u = u01(splitmix64(h));
-            acc = (x <= 0.0) || (u < exp(-fmin(x, 60.0)));
-        }
-        accd = acc;
-        if (acc) {
-            A.species[(long)s * g.natoms_pad + pp[0]] = (uint8_t)pp[2];
-            A.species[(long)s * g.natoms_pad + pp[1]] = (uint8_t)pp[3];
-            A.c2[s] = c2t;
-            A.acc_rung[rung] += 1;
-        }
-        if (A.log && step - 1 < (uint64_t)A.log_cap) {
-            int *L = A.log + ((long)(step - 1) * gridDim.x + s) * 4;
-            L[0] = pp[0];
-            L[1] = pp[1];
-            L[2] = acc;
-            L[3] = rung;
-        }
-        A.stepc[s] = step;
-    }
-    __syncthreads();
-    const int acc = accd;
-    int *cn = A.counts + (long)s * g.nrow_tot;
-    int *dc = A.dcounts + (long)s * g.nrow_tot;
-    for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) {
-        if (acc) cn[q] += dc[q];
-        dc[q] = 0;
-    }
-    __syncthreads();
-    if (A.series && rung == A.series_rung && step - 1 < (uint64_t)A.series_cap) {
-        const int nss = g.nsp * g.nsp * g.nshell;
-        for (int q = threadIdx.x; q < nss; q += blockDim.x) {
-            int a = q / (g.nsp * g.nshell), r = q % (g.nsp * g.nshell);
-            A.series[(long)(step - 1) * nss + q] = cn[a * g.nrow_e + r];
-        }
-    }
-    const double cur = A.c2[s];
-    if (cur < A.best_c2[s]) {
-        __syncthreads();
-        for (int q = threadIdx.x; q < g.natoms_pad; q += blockDim.x)
-            A.best_sp[(long)s * g.natoms_pad + q] = A.species[(long)s * g.natoms_pad + q];
-        __syncthreads();
-        if (threadIdx.x == 0) A.best_c2[s] = cur;
-    }
+    if (threadIdx.x == 0) out[blockIdx.x] = res / nfit;
 }
 
 // ---------------------------------------------------------------- K4
@@ -488,14 +756,16 @@ struct chm_sampler {
     chm_ctx *ctx = nullptr;
     Geo g{};
     int nrep = 0, nk = 0, nsplit = 1, profile = 1;
+    int incremental = 0, reanchor_every = 512;
     uint64_t seed = 0;
     bool inited = false;
     long long hstep = 0;
     double nfit = 1.0;
     size_t lds_k1 = 0;
-    chm::DevBuf site_xyz, nbr_j, nbr_sh, stencil, basisR, basisK, target, wsig;
+    chm::DevBuf site_xyz, nbr_j, nbr_sh, ss_sten, stencil, basisR, basisK, target, wcol, q0, Tb;
     chm::DevBuf species, counts, dcounts, c2, best_c2, best_sp, rung_of, slot_of, beta, stepc, iex,
-        prop, mtrial, exch_att, exch_acc, acc_rung, log, series, tmp;
+        prop, mtrial, mcur, partials, anchor, nanchor, exch_att, exch_acc, acc_rung, log, series,
+        tmp;
     int log_cap = 0, series_cap = 0, series_rung = -1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -504,11 +774,24 @@ struct chm_sampler {
 
 namespace {
 
-AccArgs acc_args(chm_sampler *S, int mode) {
-    AccArgs A;
-    A.target = S->target.as<double>();
-    A.wsig = S->wsig.as<double>();
+TileOut tile_out(chm_sampler *S) {
+    TileOut O;
+    O.target = S->target.as<double>();
+    O.wcol = S->wcol.as<double>();
+    O.mtrial = S->mtrial.as<double>();
+    O.partials = S->partials.as<double>();
+    return O;
+}
+
+SlotArgs slot_args(chm_sampler *S, int accept, int propose) {
+    SlotArgs A;
+    A.partials = S->partials.as<double>();
+    A.q0 = S->q0.as<double>();
     A.mtrial = S->mtrial.as<double>();
+    A.target = S->target.as<double>();
+    A.wcol = S->wcol.as<double>();
+    A.basisR = S->basisR.as<double>();
+    A.mcur = S->mcur.as<double>();
     A.beta = S->beta.as<double>();
     A.prop = S->prop.as<int>();
     A.counts = S->counts.as<int>();
@@ -520,6 +803,12 @@ AccArgs acc_args(chm_sampler *S, int mode) {
     A.rung_of = S->rung_of.as<int>();
     A.stepc = S->stepc.as<uint64_t>();
     A.acc_rung = S->acc_rung.as<long long>();
+    A.anchor = S->anchor.as<int>();
+    A.nanchor = S->nanchor.as<long long>();
+    A.ss_sten = S->ss_sten.as<int>();
+    A.ms_sten = S->stencil.as<int4>();
+    A.reanchor_every = S->reanchor_every;
+    A.incremental = S->incremental;
     A.log = S->log_cap ? S->log.as<int>() : nullptr;
     A.log_cap = S->log_cap;
     A.series = S->series_cap ? S->series.as<int>() : nullptr;
@@ -527,7 +816,8 @@ AccArgs acc_args(chm_sampler *S, int mode) {
     A.series_rung = S->series_rung;
     A.seed = S->seed;
     A.profile = S->profile;
-    A.mode = mode;
+    A.accept = accept;
+    A.propose = propose;
     A.nfit = S->nfit;
     return A;
 }
@@ -538,29 +828,44 @@ size_t lds_gemm(const Geo &g) {
     return std::max(a, p);
 }
 
-int launch_gemm(chm_sampler *S) {
+int launch_gemm_abs(chm_sampler *S) {
     const Geo &g = S->g;
-    dim3 grid((g.ncol_e + NCW - 1) / NCW, (S->nrep + RB - 1) / RB, g.nsp);
-    k_trial_gemm<<<grid, 256, lds_gemm(g), S->ctx->stream>>>(
-        g, S->nrep, S->counts.as<int>(), S->dcounts.as<int>(), S->basisR.as<double>(),
-        S->mtrial.as<double>());
+    dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
+    k_gemm_abs_f64<<<grid, 256, lds_gemm(g), S->ctx->stream>>>(
+        g, S->nrep, S->counts.as<int>(), S->dcounts.as<int>(), S->basisR.as<double>(), tile_out(S));
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
 
-int launch_step(chm_sampler *S) {
+int launch_gemm_delta(chm_sampler *S) {
     const Geo &g = S->g;
-    hipStream_t st = S->ctx->stream;
-    k_propose_gather<<<dim3(S->nsplit, S->nrep), 256, S->lds_k1, st>>>(
-        g, S->species.as<uint8_t>(), S->rung_of.as<int>(), S->stepc.as<uint64_t>(), S->seed,
-        S->site_xyz.as<int>(), S->nbr_j.as<int>(), S->nbr_sh.as<int>(), S->stencil.as<int4>(),
-        S->dcounts.as<int>(), S->prop.as<int>());
-    CHM_LAUNCH_CHECK();
-    int rc = launch_gemm(S);
-    if (rc) return rc;
-    k_accept<<<dim3(S->nrep), 256, 0, st>>>(g, acc_args(S, 0));
+    dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
+    k_gemm_delta_bf16<<<grid, 256, 0, S->ctx->stream>>>(
+        g, S->nrep, S->dcounts.as<int>(), reinterpret_cast<const bf16x8 *>(S->Tb.p),
+        S->mcur.as<double>(), tile_out(S));
     CHM_LAUNCH_CHECK();
     return CHM_OK;
+}
+
+int launch_slot(chm_sampler *S, int accept, int propose) {
+    k_slot_step<<<dim3(S->nrep), 1024, S->lds_k1, S->ctx->stream>>>(S->g,
+                                                                   slot_args(S, accept, propose));
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+int launch_gemm(chm_sampler *S) {
+    return S->incremental ? launch_gemm_delta(S) : launch_gemm_abs(S);
+}
+
+// L steps with no exchange inside: propose, then L x (transform, accept [+ propose]).
+int launch_segment(chm_sampler *S, int L) {
+    int rc = launch_slot(S, 0, 1);
+    for (int t = 0; t < L && !rc; ++t) {
+        rc = launch_gemm(S);
+        if (!rc) rc = launch_slot(S, 1, t + 1 < L ? 1 : 0);
+    }
+    return rc;
 }
 
 int launch_exchange(chm_sampler *S) {
@@ -623,12 +928,24 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
     g.nwin = gm->nwin;
     g.ncol_e = gm->nkw * gm->nwin * 2;
     g.ncol_tot = g.nsp * g.ncol_e;
+    g.nct = (g.ncol_e + NCW - 1) / NCW;
+    g.nkb = (g.nrow_e + 31) / 32;
     g.nbasis = gm->nbasis;
     for (int d = 0; d < 3; ++d) {
         g.box[d] = gm->box[d];
         g.ncell[d] = gm->ncell[d];
     }
     for (int q = 0; q < 8; ++q) g.bpar[q] = gm->basis_of_parity[q];
+    CHM_REQUIRE(g.nbasis <= 4, CHM_E_ARG, "at most 4 sites per cell");
+    for (int b = 0; b < g.nbasis; ++b) {
+        int p = -1;
+        for (int q = 0; q < 8; ++q)
+            if (g.bpar[q] == b) p = q;
+        CHM_REQUIRE(p >= 0, CHM_E_ARG, "basis site without a parity pattern");
+        g.bxyz[3 * b] = p & 1;
+        g.bxyz[3 * b + 1] = (p >> 1) & 1;
+        g.bxyz[3 * b + 2] = (p >> 2) & 1;
+    }
     // host-side validation of every index a kernel will follow
     for (long q = 0; q < (long)g.natoms * g.ncoord; ++q) {
         CHM_REQUIRE(nbr_j[q] >= 0 && nbr_j[q] < g.natoms, CHM_E_ARG, "neighbour out of range");
@@ -642,6 +959,57 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         int cell = ((x >> 1) * g.ncell[1] + (y >> 1)) * g.ncell[2] + (z >> 1);
         CHM_REQUIRE(b >= 0 && cell * g.nbasis + b == q, CHM_E_ARG,
                     "site coordinates do not map back to their index");
+    }
+    // the neighbour offsets of site 0 form a stencil that must reproduce every site's list
+    std::vector<int> ss(g.ncoord);
+    {
+        auto xyz_of = [&](int q, int &x, int &y, int &z) {
+            int v = site_xyz[q];
+            x = v & 1023;
+            y = (v >> 10) & 1023;
+            z = (v >> 20) & 1023;
+        };
+        auto centred = [](int d, int n) {
+            d %= n;
+            if (d > n / 2) d -= n;
+            if (d <= -n / 2) d += n;
+            return d;
+        };
+        int x0, y0, z0;
+        xyz_of(0, x0, y0, z0);
+        std::vector<int> off(3 * g.ncoord), shl(g.ncoord);
+        for (int o = 0; o < g.ncoord; ++o) {
+            int x, y, z;
+            xyz_of(nbr_j[o], x, y, z);
+            off[3 * o] = centred(x - x0, g.box[0]);
+            off[3 * o + 1] = centred(y - y0, g.box[1]);
+            off[3 * o + 2] = centred(z - z0, g.box[2]);
+            shl[o] = nbr_shell[o];
+            for (int d = 0; d < 3; ++d)
+                CHM_REQUIRE(off[3 * o + d] > -128 && off[3 * o + d] < 128 && 2 * std::abs(off[3 * o + d]) < g.box[d],
+                            CHM_E_ARG, "neighbour offset too large for the box");
+            ss[o] = (off[3 * o] & 0xFF) | ((off[3 * o + 1] & 0xFF) << 8) |
+                    ((off[3 * o + 2] & 0xFF) << 16) | ((shl[o] & 0xFF) << 24);
+        }
+        std::vector<long long> want(g.ncoord), got(g.ncoord);
+        for (int q = 0; q < g.natoms; ++q) {
+            int x, y, z;
+            xyz_of(q, x, y, z);
+            for (int o = 0; o < g.ncoord; ++o) {
+                int X = ((x + off[3 * o]) % g.box[0] + g.box[0]) % g.box[0];
+                int Y = ((y + off[3 * o + 1]) % g.box[1] + g.box[1]) % g.box[1];
+                int Z = ((z + off[3 * o + 2]) % g.box[2] + g.box[2]) % g.box[2];
+                int b = g.bpar[(X & 1) | ((Y & 1) << 1) | ((Z & 1) << 2)];
+                CHM_REQUIRE(b >= 0, CHM_E_ARG, "neighbour stencil leaves the lattice");
+                int site = (((X >> 1) * g.ncell[1] + (Y >> 1)) * g.ncell[2] + (Z >> 1)) * g.nbasis + b;
+                got[o] = (long long)site * 256 + shl[o];
+                want[o] = (long long)nbr_j[(long)q * g.ncoord + o] * 256 + nbr_shell[(long)q * g.ncoord + o];
+            }
+            std::sort(got.begin(), got.end());
+            std::sort(want.begin(), want.end());
+            CHM_REQUIRE(got == want, CHM_E_ARG,
+                        "the neighbour list is not translation invariant (a Bravais lattice is required)");
+        }
     }
     for (int q = 0; q < g.nstencil; ++q) {
         const int8_t *rb = stencil + 16 * q;
@@ -665,8 +1033,6 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
     S->nk = nk;
     S->profile = profile_scale ? 1 : 0;
     S->nfit = (double)g.ncol_tot;  // nsp*nkw*nwin complex points, Re and Im counted
-    int items = 2 * g.ncoord + 2 * g.nstencil;
-    S->nsplit = std::max(1, std::min(16, (items + 511) / 512));
     S->lds_k1 = sizeof(int) * ((g.nrow_tot + 3) & ~3) + (size_t)g.natoms_pad;
     auto fail = [&](int rc) {
         delete S;
@@ -679,21 +1045,40 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         return CHM_OK;
     };
     int rc;
-    // wsig = 1/sigma^2, target already restricted to the window
-    std::vector<double> ws(g.ncol_tot / 2);
+    // per-column weight 1/sigma^2 (Re and Im share a point's sigma) and, per edge,
+    // Q0 = sum over the window of t^2 w
+    std::vector<double> wc(g.ncol_tot), q0(2 * g.nsp, 0.0);
     for (int q = 0; q < g.ncol_tot / 2; ++q) {
         if (!(sigma[q] > 0.0)) return fail((chm::set_error("sigma must be positive"), CHM_E_ARG));
-        ws[q] = 1.0 / (sigma[q] * sigma[q]);
+        wc[2 * q] = wc[2 * q + 1] = 1.0 / (sigma[q] * sigma[q]);
     }
-    if ((rc = up(S->site_xyz, site_xyz, sizeof(int) * g.natoms)) ||
+    for (int q = 0; q < g.ncol_tot; ++q) {
+        q0[2 * (q / g.ncol_e)] += target[q] * target[q];
+        q0[2 * (q / g.ncol_e) + 1] += target[q] * target[q] * wc[q];
+    }
+    if ((rc = up(S->ss_sten, ss.data(), sizeof(int) * g.ncoord)) ||
+        (rc = up(S->site_xyz, site_xyz, sizeof(int) * g.natoms)) ||
         (rc = up(S->nbr_j, nbr_j, sizeof(int) * g.natoms * g.ncoord)) ||
         (rc = up(S->nbr_sh, nbr_shell, sizeof(int) * g.natoms * g.ncoord)) ||
         (rc = up(S->stencil, g.nstencil ? (const void *)stencil : (const void *)"", 16 * (size_t)g.nstencil)) ||
         (rc = up(S->basisR, basis_R, sizeof(double) * (size_t)g.nsp * g.nrow_e * g.ncol_e)) ||
         (rc = up(S->basisK, basis_k, sizeof(double) * (size_t)g.nsp * g.nrow_e * nk)) ||
         (rc = up(S->target, target, sizeof(double) * g.ncol_tot)) ||
-        (rc = up(S->wsig, ws.data(), sizeof(double) * ws.size())))
+        (rc = up(S->wcol, wc.data(), sizeof(double) * wc.size())) ||
+        (rc = up(S->q0, q0.data(), sizeof(double) * q0.size())))
         return fail(rc);
+    {
+        long nb = (long)g.nsp * g.nct * g.nkb * 3 * 64 * 8;
+        if ((rc = S->Tb.ensure(sizeof(unsigned short) * nb))) return fail(rc);
+        long nt = (long)g.nsp * g.nct * g.nkb * 64 * 8;
+        k_split_bf16<<<dim3((unsigned)((nt + 255) / 256)), 256, 0, ctx->stream>>>(
+            g, S->basisR.as<double>(), S->Tb.as<unsigned short>());
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) {
+            chm::set_error(std::string("split_bf16: ") + hipGetErrorString(le));
+            return fail(CHM_E_HIP);
+        }
+    }
     size_t nr = nrep;
     if ((rc = S->species.ensure(nr * g.natoms_pad)) || (rc = S->best_sp.ensure(nr * g.natoms_pad)) ||
         (rc = S->counts.ensure(sizeof(int) * nr * g.nrow_tot)) ||
@@ -703,13 +1088,17 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         (rc = S->beta.ensure(sizeof(double) * nr)) || (rc = S->stepc.ensure(sizeof(uint64_t) * nr)) ||
         (rc = S->iex.ensure(sizeof(uint64_t))) || (rc = S->prop.ensure(sizeof(int) * 5 * nr)) ||
         (rc = S->mtrial.ensure(sizeof(double) * nr * g.ncol_tot)) ||
+        (rc = S->mcur.ensure(sizeof(double) * nr * g.ncol_tot)) ||
+        (rc = S->partials.ensure(sizeof(double) * nr * g.nsp * g.nct * 4)) ||
+        (rc = S->anchor.ensure(sizeof(int) * nr)) ||
+        (rc = S->nanchor.ensure(sizeof(long long) * nr)) ||
         (rc = S->exch_att.ensure(sizeof(long long) * nr)) ||
         (rc = S->exch_acc.ensure(sizeof(long long) * nr)) ||
         (rc = S->acc_rung.ensure(sizeof(long long) * nr)))
         return fail(rc);
-    hipFuncSetAttribute((const void *)k_trial_gemm, hipFuncAttributeMaxDynamicSharedMemorySize,
+    hipFuncSetAttribute((const void *)k_gemm_abs_f64, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds_gemm(g));
-    hipFuncSetAttribute((const void *)k_propose_gather,
+    hipFuncSetAttribute((const void *)k_slot_step,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)S->lds_k1);
     hipFuncSetAttribute((const void *)k_full_counts, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)S->lds_k1);
@@ -781,6 +1170,8 @@ extern "C" int chm_sampler_init(chm_sampler *S, const int32_t *species0, const d
     CHM_HIP(hipMemsetAsync(S->exch_att.p, 0, S->exch_att.bytes, st));
     CHM_HIP(hipMemsetAsync(S->exch_acc.p, 0, S->exch_acc.bytes, st));
     CHM_HIP(hipMemsetAsync(S->acc_rung.p, 0, S->acc_rung.bytes, st));
+    CHM_HIP(hipMemsetAsync(S->anchor.p, 0, S->anchor.bytes, st));
+    CHM_HIP(hipMemsetAsync(S->nanchor.p, 0, S->nanchor.bytes, st));
     if (S->log_cap) CHM_HIP(hipMemsetAsync(S->log.p, 0xff, S->log.bytes, st));
     S->seed = seed;
     S->hstep = 0;
@@ -791,10 +1182,9 @@ extern "C" int chm_sampler_init(chm_sampler *S, const int32_t *species0, const d
         g, S->species.as<uint8_t>(), S->site_xyz.as<int>(), S->nbr_j.as<int>(), S->nbr_sh.as<int>(),
         S->stencil.as<int4>(), S->counts.as<int>());
     CHM_LAUNCH_CHECK();
-    int rc = launch_gemm(S);
+    int rc = launch_gemm_abs(S);
     if (rc) return rc;
-    k_accept<<<dim3(nrep), 256, 0, st>>>(g, acc_args(S, 1));
-    CHM_LAUNCH_CHECK();
+    if ((rc = launch_slot(S, 2, 0))) return rc;
     CHM_HIP(hipStreamSynchronize(st));
     S->inited = true;
     return CHM_OK;
@@ -808,9 +1198,10 @@ extern "C" int chm_sampler_run(chm_sampler *S, int nsteps, int exchange_every) {
     const int E = exchange_every;
     int rc = 0;
     if (E > 0 && (S->graph_steps != E)) {
+        // one exchange period: propose, E x (transform, accept [+ propose]), exchange
         drop_graph(S);
         CHM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int t = 0; t < E && !rc; ++t) rc = launch_step(S);
+        rc = launch_segment(S, E);
         if (!rc) rc = launch_exchange(S);
         hipGraph_t gr = nullptr;
         hipError_t e = hipStreamEndCapture(st, &gr);
@@ -833,9 +1224,11 @@ extern "C" int chm_sampler_run(chm_sampler *S, int nsteps, int exchange_every) {
             S->hstep += E;
             left -= E;
         } else {
-            if ((rc = launch_step(S))) return rc;
-            S->hstep += 1;
-            left -= 1;
+            long long L = left;
+            if (E > 0) L = std::min<long long>(left, E - S->hstep % E);
+            if ((rc = launch_segment(S, (int)L))) return rc;
+            S->hstep += L;
+            left -= L;
             if (E > 0 && S->hstep % E == 0 && (rc = launch_exchange(S))) return rc;
         }
     }
@@ -954,10 +1347,16 @@ extern "C" int chm_sampler_read(chm_sampler *S, int what, void *host, int64_t nb
                 memcpy(o + r * row, v.data() + (size_t)slot_of[r] * row, 8 * row);
             return CHM_OK;
         }
-        case 12: {
+        case 12:
+        case 13: {
             int64_t n = (int64_t)8 * nrep * g.ncol_tot;
             if ((rc = need(n))) return rc;
-            CHM_HIP(hipMemcpy(host, S->mtrial.p, n, hipMemcpyDeviceToHost));
+            CHM_HIP(hipMemcpy(host, what == 12 ? S->mtrial.p : S->mcur.p, n, hipMemcpyDeviceToHost));
+            return CHM_OK;
+        }
+        case 14: {
+            if ((rc = need((int64_t)8 * nrep))) return rc;
+            CHM_HIP(hipMemcpy(host, S->nanchor.p, 8 * nrep, hipMemcpyDeviceToHost));
             return CHM_OK;
         }
     }
@@ -1008,10 +1407,9 @@ extern "C" int chm_sampler_chi2(chm_sampler *S, int nrows, const double *chiR, d
         return rc;
     CHM_HIP(hipMemcpyAsync(m.p, chiR, sizeof(double) * (size_t)nrows * g.ncol_tot,
                            hipMemcpyHostToDevice, st));
-    AccArgs A = acc_args(S, 2);
-    A.mtrial = m.as<double>();
-    A.c2 = o.as<double>();
-    k_accept<<<dim3(nrows), 256, 0, st>>>(g, A);
+    k_eval_rows<<<dim3(nrows), 256, 0, st>>>(g, m.as<double>(), S->target.as<double>(),
+                                             S->wcol.as<double>(), S->profile, S->nfit,
+                                             o.as<double>());
     CHM_LAUNCH_CHECK();
     CHM_HIP(hipMemcpyAsync(c2, o.p, sizeof(double) * nrows, hipMemcpyDeviceToHost, st));
     CHM_HIP(hipStreamSynchronize(st));
@@ -1054,27 +1452,20 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
     CHM_REQUIRE(S && S->inited && iters > 0 && ms, CHM_E_ARG, "bad arguments");
     CHM_HIP(hipSetDevice(S->ctx->device));
     hipStream_t st = S->ctx->stream;
-    const Geo &g = S->g;
     hipEvent_t e0, e1;
     CHM_HIP(hipEventCreate(&e0));
     CHM_HIP(hipEventCreate(&e1));
     float t = 0.f;
-    int rc = 0;
-    // K1: the same proposal every launch (the step counter does not move); the
-    // accumulated deltas are cleared afterwards
+    int rc = launch_slot(S, 0, 1);          // a proposal pending from here on
+    // per-slot kernel, propose only: the pending proposal is recomputed identically
     CHM_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < iters; ++i) {
-        k_propose_gather<<<dim3(S->nsplit, S->nrep), 256, S->lds_k1, st>>>(
-            g, S->species.as<uint8_t>(), S->rung_of.as<int>(), S->stepc.as<uint64_t>(), S->seed,
-            S->site_xyz.as<int>(), S->nbr_j.as<int>(), S->nbr_sh.as<int>(), S->stencil.as<int4>(),
-            S->dcounts.as<int>(), S->prop.as<int>());
-    }
+    for (int i = 0; i < iters && !rc; ++i) rc = launch_slot(S, 0, 1);
     CHM_HIP(hipEventRecord(e1, st));
+    if (rc) return rc;
     CHM_HIP(hipEventSynchronize(e1));
     CHM_HIP(hipEventElapsedTime(&t, e0, e1));
     ms[0] = t / iters;
-    CHM_HIP(hipMemsetAsync(S->dcounts.p, 0, S->dcounts.bytes, st));
-    // K2 on the current state
+    // the trial transform of the pending proposal
     CHM_HIP(hipEventRecord(e0, st));
     for (int i = 0; i < iters && !rc; ++i) rc = launch_gemm(S);
     CHM_HIP(hipEventRecord(e1, st));
@@ -1082,24 +1473,35 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
     CHM_HIP(hipEventSynchronize(e1));
     CHM_HIP(hipEventElapsedTime(&t, e0, e1));
     ms[1] = t / iters;
-    // K3 with every proposal marked null (nothing is committed)
-    CHM_HIP(hipMemsetAsync(S->prop.p, 0, S->prop.bytes, st));
+    // full steps: accept + propose, transform (each a real step of the chain)
     CHM_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < iters; ++i) k_accept<<<dim3(S->nrep), 256, 0, st>>>(g, acc_args(S, 0));
-    CHM_HIP(hipEventRecord(e1, st));
-    CHM_HIP(hipEventSynchronize(e1));
-    CHM_HIP(hipEventElapsedTime(&t, e0, e1));
-    ms[2] = t / iters;
-    // one full step (K1..K3) launched directly
-    CHM_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < iters && !rc; ++i) rc = launch_step(S);
+    for (int i = 0; i < iters && !rc; ++i) {
+        rc = launch_slot(S, 1, 1);
+        if (!rc) rc = launch_gemm(S);
+    }
     CHM_HIP(hipEventRecord(e1, st));
     if (rc) return rc;
     CHM_HIP(hipEventSynchronize(e1));
     CHM_HIP(hipEventElapsedTime(&t, e0, e1));
     ms[3] = t / iters;
+    ms[2] = ms[3] - ms[1];                   // accept + propose + gather per step
+    rc = launch_slot(S, 1, 0);               // settle: no proposal pending
+    if (rc) return rc;
+    CHM_HIP(hipStreamSynchronize(st));
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+extern "C" int chm_sampler_set_precision(chm_sampler *S, int incremental, int reanchor_every) {
+    CHM_REQUIRE(S, CHM_E_ARG, "null sampler");
+    CHM_REQUIRE(incremental == 0 || incremental == 1, CHM_E_ARG, "incremental must be 0 or 1");
+    CHM_REQUIRE(reanchor_every >= 0, CHM_E_ARG, "negative re-anchor period");
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    CHM_HIP(hipStreamSynchronize(S->ctx->stream));
+    drop_graph(S);
+    S->incremental = incremental;
+    S->reanchor_every = reanchor_every;
+    S->inited = false;   // the accumulator is built by the next init
     return CHM_OK;
 }

@@ -46,7 +46,16 @@ class RMCSampler:
                  swap_frac=0.75, disp_sigma=0.015, max_disp=0.30, nsp=None,
                  collect_after=None, collect_every=25, adapt_every=0, target_accept=0.35,
                  profile_scale=True, collect_beta=0.5, n_collect=4, energy_prior=None,
-                 ms_library=None):
+                 ms_library=None, precision="f64", reanchor_every=512):
+        """Arguments as methods/method_a_rmc.py:64-69, plus `precision`: "f64" (the trial
+        chi(R) recomputed from the integer counts in FP64 every step) or "bf16x3" (the
+        change of chi(R) in split-BF16 MFMA with FP32 accumulation, added to an FP64
+        running chi(R) re-anchored in FP64 every `reanchor_every` accepted moves of a
+        replica).  chi^2 and the acceptance test are FP64 in both."""
+        if precision not in ("f64", "bf16x3"):
+            raise ValueError("precision must be 'f64' or 'bf16x3'")
+        self.precision = precision
+        self.reanchor_every = int(reanchor_every)
         if swap_frac != 1.0:
             raise ValueError("the device sampler makes swap moves only: pass swap_frac=1.0 "
                              "(the reference's fit driver does, chimera/fitrun.py:218)")
@@ -227,6 +236,8 @@ class RMCSampler:
             self._build(species0)
             if self._log_cap:
                 self.set_recording(self._log_cap, self._series_cap)
+        check(_lib.load().chm_sampler_set_precision(self._S, int(self.precision == "bf16x3"),
+                                                    self.reanchor_every))
         check(_lib.load().chm_sampler_init(self._S, species0, f64(self._beta_rung),
                                            int(self.seed) & ((1 << 64) - 1)))
         self.samples = []
@@ -288,6 +299,18 @@ class RMCSampler:
 
     def accepted_per_rung(self):
         return self._read(9, (self.nrep,), np.int64)
+
+    def reanchors(self):
+        """FP64 re-anchors of the running chi(R) per slot (incremental precision)."""
+        return self._read(14, (self.nrep,), np.int64)
+
+    def running_chiR(self):
+        """(nrep, nsp*nkw*nwin*2) running chi(R) accumulator, slot order."""
+        return self._read(13, (self.nrep, self.nsp * self.nkw * self.nwin * 2), np.float64)
+
+    def counts_slot_order(self):
+        rung_of = self._read(5, (self.nrep,), np.int32)
+        return self.counts[rung_of]
 
     # ------------------------------------------------------------------ moves
     def step(self):

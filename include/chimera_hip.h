@@ -177,7 +177,9 @@ int chm_sampler_set_recording(chm_sampler *s, int log_capacity, int series_capac
  *   7 series (n, nsp*nsp*nshell) int32             8 exchange attempts/accepts (2, nrep-1) int64
  *   9 accepted moves per rung (nrep) int64         10 step counter (1) int64
  *   11 chi(k) (nrep, nsp, nk) double, rung order   12 trial chi(R) of the last step
- *      (nrep, nsp*nkw*nwin*2) double, slot order
+ *      (nrep, nsp*nkw*nwin*2) double, slot order     13 running chi(R) accumulator, same
+ *      layout (incremental mode)                      14 FP64 re-anchors done (nrep) int64,
+ *      slot order
  * RMCSampler attributes species/chi2/chi/best (method_a_rmc.py:166-184, 493-496). */
 int chm_sampler_read(chm_sampler *s, int what, void *host, int64_t nbytes);
 
@@ -190,6 +192,15 @@ int chm_sampler_counts(chm_sampler *s, int nconf, const int32_t *species, int32_
  * window transforms chiR (nrows, nsp*nkw*nwin*2).  RMCSampler._chi2
  * (method_a_rmc.py:211-227) as used for best_verified (:439-448). */
 int chm_sampler_chi2(chm_sampler *s, int nrows, const double *chiR, double *c2);
+
+/* Precision of the trial transform.  incremental = 0: chi(R)_trial = counts @ basis_R
+ * in FP64 every step (exact, no drift).  incremental = 1: delta(R) = delta_counts @
+ * basis_R in split-BF16 (3 planes) on MFMA with FP32 accumulation, added to an FP64
+ * running chi(R) that is re-anchored (recomputed in FP64 from the integer counts) every
+ * reanchor_every accepted moves of a replica (0 = never).  chi^2 is FP64 in both.
+ * Takes effect at the next chm_sampler_init.  No counterpart in the reference (its
+ * transform is an FP64 cuFFT of the full trial chi every step, fourier.py:75-80). */
+int chm_sampler_set_precision(chm_sampler *s, int incremental, int reanchor_every);
 
 /* One replica-exchange sweep now (method_a_rmc.py:332-373); run() also exchanges on
  * its own every exchange_every steps. */

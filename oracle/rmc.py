@@ -275,7 +275,7 @@ class PlainSampler:
             a, b, na, nb, sw = propose(sp, self.seed, self.stepc, c)
             u = accept_uniform(self.seed, self.stepc, c)
             if not sw:
-                rec.append((a, b, 0, self.chi2[c]))
+                rec.append((a, b, 0, self.chi2[c], 0.0, u))
                 continue
             dchi, _ = P.delta_chi(sp, a, b, na, nb)
             trial = self.chi[c] + dchi
@@ -287,7 +287,7 @@ class PlainSampler:
                 self.chi[c] = trial
                 self.chi2[c] = c2
             acc_all[c] = acc
-            rec.append((a, b, int(acc), c2))
+            rec.append((a, b, int(acc), c2, x, u))
         imp = self.chi2 < self.best_chi2
         self.best_chi2 = np.where(imp, self.chi2, self.best_chi2)
         self.best_sp[imp] = self.species[imp]
