@@ -48,7 +48,7 @@ __device__ __forceinline__ double u01(uint64_t h) {
 
 struct Geo {
     int natoms, natoms_pad, nsp, ncoord, nshell, nblk, nstencil, nabsrec;
-    int nrow_e, nrow_tot, nss_rows, ncol_e, ncol_tot, nkw, nwin;
+    int nrow_e, rstride, nrow_tot, nss_rows, ncol_e, ncol_tot, nkw, nwin;  // rstride: padded rows per edge
     int box[3], ncell[3], nbasis;
     int bpar[8];
     int has_const;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_full_counts(
         if (it < npairs) {
             const int x = (int)(it / g.ncoord);
             const int y = nbr_j[it];
-            atomicAdd(&hist[lsp[x] * g.nrow_e + lsp[y] * g.nshell + nbr_sh[it]], 1);
+            atomicAdd(&hist[lsp[x] * g.rstride + lsp[y] * g.nshell + nbr_sh[it]], 1);
         } else {
             const long m = it - npairs;
             const int x = (int)(m / g.nabsrec);
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_full_counts(
             const int cls_off = (int)(uint8_t)rb[14] | ((int)rb[15] << 8);
             const int xyz = site_xyz[x];
             const int x0 = xyz & 1023, y0 = (xyz >> 10) & 1023, z0 = (xyz >> 20) & 1023;
-            int e = lsp[x] * g.nrow_e + g.nss_rows + cls_off;
+            int e = lsp[x] * g.rstride + g.nss_rows + cls_off;
             int pw = 1;
             for (int q = 1; q <= nslot; ++q) {
                 int X = wrap(x0 + rb[3 * q], g.box[0]);
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_full_counts(
     int *dc = counts + (long)s * g.nrow_tot;
     for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) {
         int v = hist[q];
-        if (g.has_const && blockIdx.x == 0 && (q % g.nrow_e) == g.nrow_e - 1) v += 1;
+        if (g.has_const && blockIdx.x == 0 && (q % g.rstride) == g.nrow_e - 1) v += 1;
         if (v != 0) atomicAdd(&dc[q], v);
     }
 }
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int
         int slot = r0 + m;
         double v = 0.0;
         if (slot < nrep) {
-            long o = (long)slot * g.nrow_tot + (long)e * K + k;
+            long o = (long)slot * g.nrow_tot + (long)e * g.rstride + k;
             v = (double)(counts[o] + dcounts[o]);
         }
         dl[k * ld + m] = v;
@@ -277,6 +277,8 @@ __device__ __forceinline__ __bf16 bf16_of_small_int(int v) {
 // blocks of one 32-slot x 16-column tile; their partial tiles are summed in LDS.
 // Operand layouts of the 16x16x32 MFMA: lane l holds A[l%16][8(l/16)+i] and
 // B[8(l/16)+i][l%16] (i < 8); D lane l holds D[4(l/16)+i][l%16] (i < 4).
+constexpr int KBW = 4;   // K blocks per wave held in registers (nkb <= 4 * KBW)
+
 __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const int *__restrict__ dcounts,
                                                           const bf16x8 *__restrict__ Tb,
                                                           const double *__restrict__ mcur, TileOut O) {
@@ -284,41 +286,78 @@ __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const 
     const int e = blockIdx.z;
     const int r0 = blockIdx.y * RB;
     const int ct = blockIdx.x;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
+    // epilogue operands first, so their latency hides behind the MFMA phase
+    double mc_pre[2], tg_pre[2], w_pre[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int m = t / NCW + 16 * h, col = ct * NCW + t % NCW, slot = r0 + m;
+        const bool v = slot < nrep && col < g.ncol_e;
+        const long gc = (long)e * g.ncol_e + col;
+        mc_pre[h] = v ? mcur[(long)slot * g.ncol_tot + gc] : 0.0;
+        tg_pre[h] = v ? O.target[gc] : 0.0;
+        w_pre[h] = v ? O.wcol[gc] : 0.0;
+    }
     const int per = (g.nkb + 3) / 4;
     const int kb0 = wave * per, kb1 = min(g.nkb, kb0 + per);
+    // all operands of this wave's K blocks, loaded before any MFMA
+    bf16x8 B[KBW][3];
+    int4 Av[KBW][2][2];
+#pragma unroll
+    for (int q = 0; q < KBW; ++q) {
+        const int kb = kb0 + q;
+        if (kb < kb1) {
+            const bf16x8 *Bp = Tb + ((((long)e * g.nct + ct) * g.nkb + kb) * 3) * 64 + lane;
+            B[q][0] = Bp[0];
+            B[q][1] = Bp[64];
+            B[q][2] = Bp[128];
+            const int k0 = kb * 32 + 8 * (lane >> 4);
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                const int slot = r0 + rb * 16 + (lane & 15);
+                if (slot < nrep) {
+                    const int4 *src = reinterpret_cast<const int4 *>(
+                        dcounts + (long)slot * g.nrow_tot + (long)e * g.rstride + k0);
+                    Av[q][rb][0] = src[0];
+                    Av[q][rb][1] = src[1];
+                } else {
+                    Av[q][rb][0] = make_int4(0, 0, 0, 0);
+                    Av[q][rb][1] = make_int4(0, 0, 0, 0);
+                }
+            }
+        }
+    }
     f32x4 acc[2], acch[2];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[rb][i] = acch[rb][i] = 0.f;
-    const int K = g.nrow_e;
-    for (int kb = kb0; kb < kb1; ++kb) {
-        const bf16x8 *Bp = Tb + ((((long)e * g.nct + ct) * g.nkb + kb) * 3) * 64 + lane;
-        const bf16x8 b0 = Bp[0], b1 = Bp[64], b2 = Bp[128];
-        const int k0 = kb * 32 + 8 * (lane >> 4);
+#pragma unroll
+    for (int q = 0; q < KBW; ++q) {
+        if (kb0 + q >= kb1) break;
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb) {
-            const int slot = r0 + rb * 16 + (lane & 15);
+            const int vals[8] = {Av[q][rb][0].x, Av[q][rb][0].y, Av[q][rb][0].z, Av[q][rb][0].w,
+                                 Av[q][rb][1].x, Av[q][rb][1].y, Av[q][rb][1].z, Av[q][rb][1].w};
             bf16x8 alo, ahi;
             bool hi_any = false;
-            const int *src = dcounts + (long)slot * g.nrow_tot + (long)e * K;
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                int v = (slot < nrep && k0 + i < K) ? src[k0 + i] : 0;
-                int lo = ((v + 128) & 255) - 128;
-                int hi = (v - lo) >> 8;
+                const int v = vals[i];
+                const int lo = ((v + 128) & 255) - 128;
+                const int hi = (v - lo) >> 8;
                 hi_any |= (hi != 0);
                 alo[i] = bf16_of_small_int(lo);
                 ahi[i] = bf16_of_small_int(hi);
             }
-            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b0, acc[rb], 0, 0, 0);
-            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b1, acc[rb], 0, 0, 0);
-            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b2, acc[rb], 0, 0, 0);
+            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[q][0], acc[rb], 0, 0, 0);
+            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[q][1], acc[rb], 0, 0, 0);
+            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[q][2], acc[rb], 0, 0, 0);
             if (__any(hi_any)) {
-                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b0, acch[rb], 0, 0, 0);
-                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b1, acch[rb], 0, 0, 0);
-                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b2, acch[rb], 0, 0, 0);
+                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, B[q][0], acch[rb], 0, 0, 0);
+                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, B[q][1], acch[rb], 0, 0, 0);
+                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, B[q][2], acch[rb], 0, 0, 0);
             }
         }
     }
@@ -328,16 +367,36 @@ __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const 
         for (int i = 0; i < 4; ++i)
             red[wave][rb * 16 + 4 * (lane >> 4) + i][lane & 15] = acc[rb][i] + 256.f * acch[rb][i];
     __syncthreads();
-    const int t = threadIdx.x;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int m = t / NCW + 16 * h, cl = t % NCW;
         const float d = red[0][m][cl] + red[1][m][cl] + red[2][m][cl] + red[3][m][cl];
         const int slot = r0 + m, col = ct * NCW + cl;
-        double mp = 0.0;
-        if (slot < nrep && col < g.ncol_e)
-            mp = mcur[(long)slot * g.ncol_tot + (long)e * g.ncol_e + col] + (double)d;
-        tile_epilogue(g, O, nrep, slot, e, ct, col, mp);
+        const bool valid = slot < nrep && col < g.ncol_e;
+        const double mp = valid ? mc_pre[h] + (double)d : 0.0;
+        double p1 = 0.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
+        if (valid) {
+            O.mtrial[(long)slot * g.ncol_tot + (long)e * g.ncol_e + col] = mp;
+            const double dd = mp - tg_pre[h];
+            p1 = dd * tg_pre[h];
+            p2 = dd * dd;
+            q1 = p1 * w_pre[h];
+            q2 = p2 * w_pre[h];
+        }
+#pragma unroll
+        for (int mm = 8; mm >= 1; mm >>= 1) {
+            p1 += __shfl_xor(p1, mm, 16);
+            p2 += __shfl_xor(p2, mm, 16);
+            q1 += __shfl_xor(q1, mm, 16);
+            q2 += __shfl_xor(q2, mm, 16);
+        }
+        if (slot < nrep && cl == 0) {
+            double *P = O.partials + (((long)slot * g.nsp + e) * g.nct + ct) * 4;
+            P[0] = p1;
+            P[1] = p2;
+            P[2] = q1;
+            P[3] = q2;
+        }
     }
 }
 
@@ -463,6 +522,77 @@ __device__ __forceinline__ void site_xyz_of(const Geo &g, int q, int &x, int &y,
     z = 2 * cz + g.bxyz[3 * b + 2];
 }
 
+struct SwapSite {
+    Prop p;
+    int ax, ay, az, bx, by, bz;
+};
+
+// One gather item of a swap: item < nss is a neighbour of a (first ncoord) or of b; the
+// rest are path records of a's then b's incidence stencil.  Count changes go to `hist`.
+__device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W,
+                                            const int *__restrict__ ss_sten, const uint8_t *lsp,
+                                            int *hist, int it, int nss, const int4 rec) {
+    const Prop &p = W.p;
+    if (it < nss) {
+        // ordered pairs x->y and y->x for every neighbour y of x in {a, b}
+        const bool on_a = it < g.ncoord;
+        const int sv = ss_sten[on_a ? it : it - g.ncoord];
+        const int dx = (int)(int8_t)(sv & 0xFF), dy = (int)(int8_t)((sv >> 8) & 0xFF);
+        const int dz = (int)(int8_t)((sv >> 16) & 0xFF), sh = (sv >> 24) & 0xFF;
+        const int x = on_a ? p.a : p.b;
+        const int y = site_index(g, wrap((on_a ? W.ax : W.bx) + dx, g.box[0]),
+                                 wrap((on_a ? W.ay : W.by) + dy, g.box[1]),
+                                 wrap((on_a ? W.az : W.bz) + dz, g.box[2]));
+        if (x == p.b && y == p.a) return;  // covered from a's side
+        const int sx = lsp[x], sy = lsp[y];
+        const int nx = (x == p.a) ? p.na : p.nb;
+        const int ny = (y == p.a) ? p.na : ((y == p.b) ? p.nb : sy);
+        atomicAdd(&hist[sx * g.rstride + sy * g.nshell + sh], -1);
+        atomicAdd(&hist[nx * g.rstride + ny * g.nshell + sh], 1);
+        atomicAdd(&hist[sy * g.rstride + sx * g.nshell + sh], -1);
+        atomicAdd(&hist[ny * g.rstride + nx * g.nshell + sh], 1);
+        return;
+    }
+    const int m = it - nss;
+    const bool on_a = m < g.nstencil;
+    const int8_t *rb = reinterpret_cast<const int8_t *>(&rec);
+    const int nslot = rb[12], wgt = rb[13];
+    const int cls_off = (int)(uint8_t)rb[14] | ((int)rb[15] << 8);
+    const int x0 = on_a ? W.ax : W.bx, y0 = on_a ? W.ay : W.by, z0 = on_a ? W.az : W.bz;
+    int sites[4];
+    bool has_a = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q > nslot) {
+            sites[q] = -1;
+            continue;
+        }
+        sites[q] = site_index(g, wrap(x0 + rb[3 * q], g.box[0]), wrap(y0 + rb[3 * q + 1], g.box[1]),
+                              wrap(z0 + rb[3 * q + 2], g.box[2]));
+        has_a |= (sites[q] == p.a);
+    }
+    if (!on_a && has_a) return;  // a path through both sites counts once
+    int eo = 0, en = 0, pw = 1;
+#pragma unroll
+    for (int q = 1; q < 4; ++q) {
+        if (q > nslot) break;
+        const int st = sites[q];
+        const int so = lsp[st];
+        const int sn = (st == p.a) ? p.na : ((st == p.b) ? p.nb : so);
+        eo += so * pw;
+        en += sn * pw;
+        pw *= g.nsp;
+    }
+    const int sa0 = lsp[sites[0]];
+    const int sa1 = (sites[0] == p.a) ? p.na : ((sites[0] == p.b) ? p.nb : sa0);
+    eo += sa0 * g.rstride + g.nss_rows + cls_off;
+    en += sa1 * g.rstride + g.nss_rows + cls_off;
+    if (eo != en) {
+        atomicAdd(&hist[eo], -wgt);
+        atomicAdd(&hist[en], wgt);
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_slot_step(Geo g, SlotArgs A) {
     extern __shared__ int lds[];
     int *hist = lds;
@@ -476,14 +606,44 @@ __global__ __launch_bounds__(1024) void k_slot_step(Geo g, SlotArgs A) {
     const long base_sp = (long)s * g.natoms_pad;
     load_species(lsp, A.species + base_sp, g.natoms_pad);
     if (threadIdx.x == 0) step_sh = A.stepc[s];
-    if (A.accept && threadIdx.x < 4 * g.nsp) {
-        const int e = threadIdx.x >> 2, j = threadIdx.x & 3;
+    // per-edge sums over the column tiles: (e, j) groups of CT lanes, then a tree
+    const int G = 4 * g.nsp;
+    int CT = (int)blockDim.x / G;
+    if (CT > g.nct) CT = g.nct;
+    const int t = threadIdx.x;
+    const int gi = t / CT, c0 = t - gi * CT;
+    if (A.accept && t < G * CT) {
+        const int e = gi >> 2, j = gi & 3;
         const double *Pp = A.partials + ((long)s * g.nsp + e) * g.nct * 4 + j;
         double v = 0.0;
-        for (int c = 0; c < g.nct; ++c) v += Pp[4 * c];
-        sums[e][j] = v;
+        for (int c = c0; c < g.nct; c += CT) v += Pp[4 * c];
+        red[t] = v;
+    }
+    // prefetch this lane's multiple-scattering records (independent of the proposal)
+    constexpr int RMAX = 8;
+    int4 recs[RMAX];
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) recs[r] = make_int4(0, 0, 0, 0);
+    const int nss_items = 2 * g.ncoord;
+    if (A.propose) {
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            const int it = t + r * (int)blockDim.x;
+            const int m = it - nss_items;
+            if (m >= 0 && m < 2 * g.nstencil) recs[r] = A.ms_sten[m < g.nstencil ? m : m - g.nstencil];
+        }
     }
     __syncthreads();
+    if (A.accept) {
+        int h2 = 1;
+        while (h2 < CT) h2 <<= 1;
+        for (int h = h2 >> 1; h >= 1; h >>= 1) {
+            if (t < G * CT && c0 < h && c0 + h < CT) red[t] += red[t + h];
+            __syncthreads();
+        }
+        if (t < G * CT && c0 == 0) sums[gi >> 2][gi & 3] = red[t];
+        __syncthreads();
+    }
     const int rung = A.rung_of[s];
     int *cn = A.counts + (long)s * g.nrow_tot;
     int *dc = A.dcounts + (long)s * g.nrow_tot;
@@ -539,7 +699,7 @@ __global__ __launch_bounds__(1024) void k_slot_step(Geo g, SlotArgs A) {
             const int nss = g.nsp * g.nsp * g.nshell;
             for (int q = threadIdx.x; q < nss; q += blockDim.x) {
                 int a = q / (g.nsp * g.nshell), r = q % (g.nsp * g.nshell);
-                A.series[(long)(step - 1) * nss + q] = cn[a * g.nrow_e + r];
+                A.series[(long)(step - 1) * nss + q] = cn[a * g.rstride + r];
             }
         }
         // FP64 re-anchor of the running accumulator every reanchor_every accepted moves
@@ -555,7 +715,7 @@ __global__ __launch_bounds__(1024) void k_slot_step(Geo g, SlotArgs A) {
                 for (int q = threadIdx.x; q < g.ncol_tot; q += blockDim.x) {
                     const int e = q / g.ncol_e, col = q % g.ncol_e;
                     const double *B = A.basisR + (long)e * g.nrow_e * g.ncol_e + col;
-                    const int *c = cn + e * g.nrow_e;
+                    const int *c = cn + e * g.rstride;
                     double v = 0.0;
                     for (int k = 0; k < g.nrow_e; ++k) v = fma((double)c[k], B[(long)k * g.ncol_e], v);
                     mc[q] = v;
@@ -613,72 +773,20 @@ __global__ __launch_bounds__(1024) void k_slot_step(Geo g, SlotArgs A) {
     __syncthreads();
     const Prop p = P;
     if (p.sw) {
-        int ax, ay, az, bx, by, bz;
-        site_xyz_of(g, p.a, ax, ay, az);
-        site_xyz_of(g, p.b, bx, by, bz);
-        const int nss = 2 * g.ncoord;
-        const int nitems = nss + 2 * g.nstencil;
-        for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
-            if (it < nss) {
-                // ordered pairs x->y and y->x for every neighbour y of x in {a, b}
-                const bool on_a = it < g.ncoord;
-                const int sv = A.ss_sten[on_a ? it : it - g.ncoord];
-                const int dx = (int)(int8_t)(sv & 0xFF), dy = (int)(int8_t)((sv >> 8) & 0xFF);
-                const int dz = (int)(int8_t)((sv >> 16) & 0xFF), sh = (sv >> 24) & 0xFF;
-                const int x = on_a ? p.a : p.b;
-                const int y = site_index(g, wrap((on_a ? ax : bx) + dx, g.box[0]),
-                                         wrap((on_a ? ay : by) + dy, g.box[1]),
-                                         wrap((on_a ? az : bz) + dz, g.box[2]));
-                if (x == p.b && y == p.a) continue;  // covered from a's side
-                const int sx = lsp[x], sy = lsp[y];
-                const int nx = (x == p.a) ? p.na : p.nb;
-                const int ny = (y == p.a) ? p.na : ((y == p.b) ? p.nb : sy);
-                atomicAdd(&hist[sx * g.nrow_e + sy * g.nshell + sh], -1);
-                atomicAdd(&hist[nx * g.nrow_e + ny * g.nshell + sh], 1);
-                atomicAdd(&hist[sy * g.nrow_e + sx * g.nshell + sh], -1);
-                atomicAdd(&hist[ny * g.nrow_e + nx * g.nshell + sh], 1);
-            } else {
-                const int m = it - nss;
-                const bool on_a = m < g.nstencil;
-                const int4 rec = A.ms_sten[on_a ? m : m - g.nstencil];
-                const int8_t *rb = reinterpret_cast<const int8_t *>(&rec);
-                const int nslot = rb[12], wgt = rb[13];
-                const int cls_off = (int)(uint8_t)rb[14] | ((int)rb[15] << 8);
-                const int x0 = on_a ? ax : bx, y0 = on_a ? ay : by, z0 = on_a ? az : bz;
-                int sites[4];
-                bool has_a = false;
+        SwapSite W;
+        W.p = p;
+        site_xyz_of(g, p.a, W.ax, W.ay, W.az);
+        site_xyz_of(g, p.b, W.bx, W.by, W.bz);
+        const int nitems = nss_items + 2 * g.nstencil;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (q > nslot) {
-                        sites[q] = -1;
-                        continue;
-                    }
-                    sites[q] = site_index(g, wrap(x0 + rb[3 * q], g.box[0]),
-                                          wrap(y0 + rb[3 * q + 1], g.box[1]),
-                                          wrap(z0 + rb[3 * q + 2], g.box[2]));
-                    has_a |= (sites[q] == p.a);
-                }
-                if (!on_a && has_a) continue;  // a path through both sites counts once
-                int eo = 0, en = 0, pw = 1;
-#pragma unroll
-                for (int q = 1; q < 4; ++q) {
-                    if (q > nslot) break;
-                    const int st = sites[q];
-                    const int so = lsp[st];
-                    const int sn = (st == p.a) ? p.na : ((st == p.b) ? p.nb : so);
-                    eo += so * pw;
-                    en += sn * pw;
-                    pw *= g.nsp;
-                }
-                const int sa0 = lsp[sites[0]];
-                const int sa1 = (sites[0] == p.a) ? p.na : ((sites[0] == p.b) ? p.nb : sa0);
-                eo += sa0 * g.nrow_e + g.nss_rows + cls_off;
-                en += sa1 * g.nrow_e + g.nss_rows + cls_off;
-                if (eo != en) {
-                    atomicAdd(&hist[eo], -wgt);
-                    atomicAdd(&hist[en], wgt);
-                }
-            }
+        for (int r = 0; r < RMAX; ++r) {
+            const int it = t + r * (int)blockDim.x;
+            if (it < nitems) gather_item(g, W, A.ss_sten, lsp, hist, it, nss_items, recs[r]);
+        }
+        for (int it = t + RMAX * (int)blockDim.x; it < nitems; it += blockDim.x) {
+            const int m = it - nss_items;
+            const int4 rec = (m >= 0) ? A.ms_sten[m < g.nstencil ? m : m - g.nstencil] : make_int4(0, 0, 0, 0);
+            gather_item(g, W, A.ss_sten, lsp, hist, it, nss_items, rec);
         }
     }
     __syncthreads();
@@ -923,7 +1031,8 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
     g.nss_rows = gm->nsp * gm->nshell;
     g.has_const = gm->has_const ? 1 : 0;
     g.nrow_e = g.nss_rows + g.nblk + g.has_const;
-    g.nrow_tot = g.nsp * g.nrow_e;
+    g.rstride = (g.nrow_e + 31) & ~31;
+    g.nrow_tot = g.nsp * g.rstride;
     g.nkw = gm->nkw;
     g.nwin = gm->nwin;
     g.ncol_e = gm->nkw * gm->nwin * 2;
@@ -1276,13 +1385,16 @@ extern "C" int chm_sampler_read(chm_sampler *S, int what, void *host, int64_t nb
             return CHM_OK;
         }
         case 2: {
-            if ((rc = need((int64_t)nrep * g.nrow_tot * 4))) return rc;
+            const int nrc = g.nsp * g.nrow_e;
+            if ((rc = need((int64_t)nrep * nrc * 4))) return rc;
             std::vector<int> v((size_t)nrep * g.nrow_tot);
             CHM_HIP(hipMemcpy(v.data(), S->counts.p, 4 * v.size(), hipMemcpyDeviceToHost));
             int *o = static_cast<int *>(host);
             for (int r = 0; r < nrep; ++r)
-                memcpy(o + (size_t)r * g.nrow_tot, v.data() + (size_t)slot_of[r] * g.nrow_tot,
-                       4 * g.nrow_tot);
+                for (int e = 0; e < g.nsp; ++e)
+                    memcpy(o + (size_t)r * nrc + (size_t)e * g.nrow_e,
+                           v.data() + (size_t)slot_of[r] * g.nrow_tot + (size_t)e * g.rstride,
+                           4 * g.nrow_e);
             return CHM_OK;
         }
         case 5: {
@@ -1334,7 +1446,7 @@ extern "C" int chm_sampler_read(chm_sampler *S, int what, void *host, int64_t nb
             k_counts_to_f64<<<dim3((unsigned)((nc + 255) / 256)), 256, 0, st>>>(S->counts.as<int>(),
                                                                               cf, nc);
             CHM_LAUNCH_CHECK();
-            rc = chm::gemm_f64(st, g.nsp, nrep, S->nk, g.nrow_e, cf, g.nrow_tot, g.nrow_e,
+            rc = chm::gemm_f64(st, g.nsp, nrep, S->nk, g.nrow_e, cf, g.nrow_tot, g.rstride,
                                S->basisK.as<double>(), S->nk, (long)g.nrow_e * S->nk, ck,
                                g.nsp * S->nk, S->nk);
             if (rc) return rc;
@@ -1389,9 +1501,14 @@ extern "C" int chm_sampler_counts(chm_sampler *S, int nconf, const int32_t *spec
         g, sp.as<uint8_t>(), S->site_xyz.as<int>(), S->nbr_j.as<int>(), S->nbr_sh.as<int>(),
         S->stencil.as<int4>(), cn.as<int>());
     CHM_LAUNCH_CHECK();
-    CHM_HIP(hipMemcpyAsync(counts, cn.p, sizeof(int) * (size_t)nconf * g.nrow_tot,
-                           hipMemcpyDeviceToHost, st));
+    std::vector<int> v((size_t)nconf * g.nrow_tot);
+    CHM_HIP(hipMemcpyAsync(v.data(), cn.p, sizeof(int) * v.size(), hipMemcpyDeviceToHost, st));
     CHM_HIP(hipStreamSynchronize(st));
+    const int nrc = g.nsp * g.nrow_e;
+    for (int c = 0; c < nconf; ++c)
+        for (int e = 0; e < g.nsp; ++e)
+            memcpy(counts + (size_t)c * nrc + (size_t)e * g.nrow_e,
+                   v.data() + (size_t)c * g.nrow_tot + (size_t)e * g.rstride, 4 * g.nrow_e);
     return CHM_OK;
 }
 
@@ -1431,14 +1548,15 @@ extern "C" int chm_sampler_chik(chm_sampler *S, int nconf, const int32_t *counts
     CHM_HIP(hipSetDevice(S->ctx->device));
     hipStream_t st = S->ctx->stream;
     chm::DevBuf cf, ck;
-    std::vector<double> h((size_t)nconf * g.nrow_tot);
+    const int nrc = g.nsp * g.nrow_e;
+    std::vector<double> h((size_t)nconf * nrc);
     for (size_t q = 0; q < h.size(); ++q) h[q] = (double)counts[q];
     int rc;
     if ((rc = cf.ensure(sizeof(double) * h.size())) ||
         (rc = ck.ensure(sizeof(double) * (size_t)nconf * g.nsp * S->nk)))
         return rc;
     CHM_HIP(hipMemcpyAsync(cf.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, st));
-    rc = chm::gemm_f64(st, g.nsp, nconf, S->nk, g.nrow_e, cf.as<double>(), g.nrow_tot, g.nrow_e,
+    rc = chm::gemm_f64(st, g.nsp, nconf, S->nk, g.nrow_e, cf.as<double>(), nrc, g.nrow_e,
                        S->basisK.as<double>(), S->nk, (long)g.nrow_e * S->nk, ck.as<double>(),
                        g.nsp * S->nk, S->nk);
     if (rc) return rc;

@@ -143,7 +143,7 @@ def test_chi_of_counts_matches_pair_space(tern108):
                                rtol=0, atol=1e-12 * np.abs(ref).max())
 
 
-def _compare_trajectory(P, O, nsteps, exch, seed, tie_tol=0.0):
+def _compare_trajectory(P, O, nsteps, exch, seed, tie_tol=0.0, c2_rtol=1e-9):
     """Device and oracle move by move.  With tie_tol > 0 a differing decision is
     allowed only at a near tie (|x + log u| < tie_tol); the comparison stops there."""
     S = P.sampler
@@ -167,8 +167,8 @@ def _compare_trajectory(P, O, nsteps, exch, seed, tie_tol=0.0):
                 assert tie_tol > 0 and abs(x + np.log(u)) < tie_tol, f"step {t} rung {rung}"
                 return S, R
     assert np.array_equal(S.species, R.species)
-    np.testing.assert_allclose(S.chi2, R.chi2, rtol=1e-9)
-    np.testing.assert_allclose(S.best_chi2, R.best_chi2, rtol=1e-9)
+    np.testing.assert_allclose(S.chi2, R.chi2, rtol=c2_rtol)
+    np.testing.assert_allclose(S.best_chi2, R.best_chi2, rtol=c2_rtol)
     return S, R
 
 
@@ -188,7 +188,7 @@ def test_sampler_trajectory_bf16x3(tern108):
     S = P.sampler
     S.precision, S.reanchor_every = "bf16x3", 16
     try:
-        S2, R = _compare_trajectory(P, O, 60, 10, seed=5, tie_tol=1e-4)
+        S2, R = _compare_trajectory(P, O, 60, 10, seed=5, tie_tol=1e-4, c2_rtol=2e-6)
         # running accumulator vs the exact FP64 transform of the same counts
         m = S.running_chiR()
         C = S.counts_slot_order()
