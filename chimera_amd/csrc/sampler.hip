@@ -169,6 +169,8 @@ constexpr int NCW = 16;   // columns per tile
 constexpr int KS = 16;    // K slices of the FP64 kernel
 
 struct TileOut {
+    uint64_t *stepc;        // per slot: advanced once per transform when bump is set
+    int bump;
     const double *target;   // (ncol_tot)
     const double *wcol;     // (ncol_tot) 1/sigma^2 of each column
     double *mtrial;         // (nrep, ncol_tot)
@@ -259,6 +261,8 @@ __global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int
         for (int z = 0; z < KS; ++z) sum += P[(z * RB + m) * NCW + cl];
         tile_epilogue(g, O, nrep, r0 + m, e, ct, c0 + cl, sum);
     }
+    if (O.bump && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
+        O.stepc[r0 + threadIdx.x] += 1;
 }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -398,6 +402,8 @@ __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const 
             P[3] = q2;
         }
     }
+    if (O.bump && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
+        O.stepc[r0 + threadIdx.x] += 1;
 }
 
 // basis_R (nsp, nrow_e, ncol_e) FP64 -> three bf16 planes in MFMA B-operand order:
@@ -455,12 +461,17 @@ struct SlotArgs {
     const double *basisR;   // FP64, for re-anchoring
     double *mcur;           // (nrep, ncol_tot) running accumulator (incremental mode)
     const double *beta;     // per rung
-    int *prop;              // (nrep, 5) a, b, new a, new b, is-swap
-    int *counts, *dcounts;  // (nrep, nrow_tot)
+    const int *prop_acc;    // (nrep, 5) a, b, new a, new b, is-swap of the step accepted
+    int *prop_next;         // (nrep, 5) the proposal made here (the other parity buffer)
+    const double *c2_prev;  // chi^2 after the previous step (read by every workgroup)
+    double *c2_next;        // chi^2 after this step (written by the lead workgroup)
+    int *counts;            // (nrep, nrow_tot)
+    int *dc_acc;            // count changes of the step being accepted (zeroed after)
+    int *dc_prop;           // count changes of the step being proposed (zero on entry)
     uint8_t *species, *best_sp;
-    double *c2, *best_c2;
+    double *best_c2;
     const int *rung_of;
-    uint64_t *stepc;
+    const uint64_t *stepc;  // the pending step (advanced by the transform kernels)
     long long *acc_rung;
     int *anchor;            // accepted moves since the last FP64 re-anchor, per slot
     long long *nanchor;     // re-anchors done, per slot
@@ -593,24 +604,25 @@ __device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W,
     }
 }
 
-__global__ __launch_bounds__(1024) void k_slot_step(Geo g, SlotArgs A) {
+__global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     extern __shared__ int lds[];
     int *hist = lds;
     uint8_t *lsp = reinterpret_cast<uint8_t *>(lds + ((g.nrow_tot + 3) & ~3));
     __shared__ double sums[chm::kMaxNsp][4];
-    __shared__ double red[1024];
+    __shared__ double red[256];
     __shared__ int flag;
     __shared__ Prop P;
-    __shared__ uint64_t step_sh;
-    const int s = blockIdx.x;
+    const int s = blockIdx.y;
+    const bool lead = blockIdx.x == 0;       // the workgroup that owns the slot's state
+    const int t = threadIdx.x;
     const long base_sp = (long)s * g.natoms_pad;
+    const uint64_t stepc = A.stepc[s];       // written only by the transform kernels
+    const int rung = A.rung_of[s];
     load_species(lsp, A.species + base_sp, g.natoms_pad);
-    if (threadIdx.x == 0) step_sh = A.stepc[s];
     // per-edge sums over the column tiles: (e, j) groups of CT lanes, then a tree
     const int G = 4 * g.nsp;
     int CT = (int)blockDim.x / G;
     if (CT > g.nct) CT = g.nct;
-    const int t = threadIdx.x;
     const int gi = t / CT, c0 = t - gi * CT;
     if (A.accept && t < G * CT) {
         const int e = gi >> 2, j = gi & 3;
@@ -619,21 +631,23 @@ __global__ __launch_bounds__(1024) void k_slot_step(Geo g, SlotArgs A) {
         for (int c = c0; c < g.nct; c += CT) v += Pp[4 * c];
         red[t] = v;
     }
+    if (A.propose)
+        for (int q = t; q < g.nrow_tot; q += blockDim.x) hist[q] = 0;
     // prefetch this lane's multiple-scattering records (independent of the proposal)
     constexpr int RMAX = 8;
     int4 recs[RMAX];
-#pragma unroll
-    for (int r = 0; r < RMAX; ++r) recs[r] = make_int4(0, 0, 0, 0);
     const int nss_items = 2 * g.ncoord;
-    if (A.propose) {
+    const int stride = gridDim.x * blockDim.x;
+    const int first = blockIdx.x * blockDim.x + t;
 #pragma unroll
-        for (int r = 0; r < RMAX; ++r) {
-            const int it = t + r * (int)blockDim.x;
-            const int m = it - nss_items;
-            if (m >= 0 && m < 2 * g.nstencil) recs[r] = A.ms_sten[m < g.nstencil ? m : m - g.nstencil];
-        }
+    for (int r = 0; r < RMAX; ++r) {
+        recs[r] = make_int4(0, 0, 0, 0);
+        const int m = first + r * stride - nss_items;
+        if (A.propose && m >= 0 && m < 2 * g.nstencil)
+            recs[r] = A.ms_sten[m < g.nstencil ? m : m - g.nstencil];
     }
     __syncthreads();
+    int *cn = A.counts + (long)s * g.nrow_tot;
     if (A.accept) {
         int h2 = 1;
         while (h2 < CT) h2 <<= 1;
@@ -643,132 +657,136 @@ __global__ __launch_bounds__(1024) void k_slot_step(Geo g, SlotArgs A) {
         }
         if (t < G * CT && c0 == 0) sums[gi >> 2][gi & 3] = red[t];
         __syncthreads();
-    }
-    const int rung = A.rung_of[s];
-    int *cn = A.counts + (long)s * g.nrow_tot;
-    int *dc = A.dcounts + (long)s * g.nrow_tot;
-    if (A.accept) {
-        const int *pp = A.prop + 5 * s;
-        if (threadIdx.x == 0) {
+        // every workgroup of the slot takes the same (deterministic) decision
+        const int *pp = A.prop_acc + 5 * s;
+        if (t == 0) {
             const double c2t = chi2_of_sums(g, sums, A.q0, A.profile, A.nfit);
             int acc = 0;
             if (A.accept == 2) {
-                A.c2[s] = c2t;
-                A.best_c2[s] = c2t;
                 acc = 1;
+                if (lead) {
+                    A.c2_next[s] = c2t;
+                    A.best_c2[s] = c2t;
+                }
             } else {
-                const uint64_t step = step_sh + 1;
+                const double c2p = A.c2_prev[s];
                 if (pp[4]) {
-                    double x = A.beta[rung] * (c2t - A.c2[s]) * A.nfit;
-                    uint64_t h = splitmix64(A.seed ^ (step * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
+                    double x = A.beta[rung] * (c2t - c2p) * A.nfit;
+                    uint64_t h = splitmix64(A.seed ^ (stepc * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
                     double u = u01(splitmix64(h));
                     acc = (x <= 0.0) || (u < exp(-fmin(x, 60.0)));
                 }
                 if (acc) {
                     lsp[pp[0]] = (uint8_t)pp[2];
                     lsp[pp[1]] = (uint8_t)pp[3];
-                    A.species[base_sp + pp[0]] = (uint8_t)pp[2];
-                    A.species[base_sp + pp[1]] = (uint8_t)pp[3];
-                    A.c2[s] = c2t;
-                    A.acc_rung[rung] += 1;
                 }
-                if (A.log && step - 1 < (uint64_t)A.log_cap) {
-                    int *L = A.log + ((long)(step - 1) * gridDim.x + s) * 4;
-                    L[0] = pp[0];
-                    L[1] = pp[1];
-                    L[2] = acc;
-                    L[3] = rung;
+                if (lead) {
+                    A.c2_next[s] = acc ? c2t : c2p;
+                    if (acc) {
+                        A.species[base_sp + pp[0]] = (uint8_t)pp[2];
+                        A.species[base_sp + pp[1]] = (uint8_t)pp[3];
+                        A.acc_rung[rung] += 1;
+                    }
+                    if (A.log && stepc - 1 < (uint64_t)A.log_cap) {
+                        int *L = A.log + ((long)(stepc - 1) * gridDim.y + s) * 4;
+                        L[0] = pp[0];
+                        L[1] = pp[1];
+                        L[2] = acc;
+                        L[3] = rung;
+                    }
                 }
-                A.stepc[s] = step;
-                step_sh = step;
             }
             flag = acc;
         }
         __syncthreads();
         const int acc = flag;
-        if (acc && A.accept == 1)
-            for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) cn[q] += dc[q];
-        if (A.incremental && acc) {
-            const double *mt = A.mtrial + (long)s * g.ncol_tot;
-            double *mc = A.mcur + (long)s * g.ncol_tot;
-            for (int q = threadIdx.x; q < g.ncol_tot; q += blockDim.x) mc[q] = mt[q];
-        }
-        __syncthreads();
-        const uint64_t step = step_sh;
-        if (A.accept == 1 && A.series && rung == A.series_rung && step - 1 < (uint64_t)A.series_cap) {
-            const int nss = g.nsp * g.nsp * g.nshell;
-            for (int q = threadIdx.x; q < nss; q += blockDim.x) {
-                int a = q / (g.nsp * g.nshell), r = q % (g.nsp * g.nshell);
-                A.series[(long)(step - 1) * nss + q] = cn[a * g.rstride + r];
+        if (lead) {
+            int *dca = A.dc_acc + (long)s * g.nrow_tot;
+            for (int q = t; q < g.nrow_tot; q += blockDim.x) {
+                if (acc && A.accept == 1) cn[q] += dca[q];
+                dca[q] = 0;
             }
-        }
-        // FP64 re-anchor of the running accumulator every reanchor_every accepted moves
-        if (A.accept == 1 && A.incremental && A.reanchor_every > 0 && acc) {
-            if (threadIdx.x == 0) {
-                const int n = A.anchor[s] + 1;
-                A.anchor[s] = (n >= A.reanchor_every) ? 0 : n;
-                flag = (n >= A.reanchor_every);
-            }
-            __syncthreads();
-            if (flag) {
+            if (A.incremental && acc) {
+                const double *mt = A.mtrial + (long)s * g.ncol_tot;
                 double *mc = A.mcur + (long)s * g.ncol_tot;
-                for (int q = threadIdx.x; q < g.ncol_tot; q += blockDim.x) {
-                    const int e = q / g.ncol_e, col = q % g.ncol_e;
-                    const double *B = A.basisR + (long)e * g.nrow_e * g.ncol_e + col;
-                    const int *c = cn + e * g.rstride;
-                    double v = 0.0;
-                    for (int k = 0; k < g.nrow_e; ++k) v = fma((double)c[k], B[(long)k * g.ncol_e], v);
-                    mc[q] = v;
-                }
-                __syncthreads();
-                for (int e = 0; e < g.nsp; ++e) {
-                    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-                    for (int q = threadIdx.x; q < g.ncol_e; q += blockDim.x) {
-                        const long o = (long)e * g.ncol_e + q;
-                        const double t = A.target[o], w = A.wcol[o], d = mc[o] - t;
-                        l0 += d * t;
-                        l1 += d * d;
-                        l2 += d * t * w;
-                        l3 += d * d * w;
-                    }
-                    l0 = block_sum(l0, red);
-                    l1 = block_sum(l1, red);
-                    l2 = block_sum(l2, red);
-                    l3 = block_sum(l3, red);
-                    if (threadIdx.x == 0) {
-                        sums[e][0] = l0;
-                        sums[e][1] = l1;
-                        sums[e][2] = l2;
-                        sums[e][3] = l3;
-                    }
-                }
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    A.c2[s] = chi2_of_sums(g, sums, A.q0, A.profile, A.nfit);
-                    A.nanchor[s] += 1;
+                for (int q = t; q < g.ncol_tot; q += blockDim.x) mc[q] = mt[q];
+            }
+            __syncthreads();
+            if (A.accept == 1 && A.series && rung == A.series_rung && stepc - 1 < (uint64_t)A.series_cap) {
+                const int nss = g.nsp * g.nsp * g.nshell;
+                for (int q = t; q < nss; q += blockDim.x) {
+                    int a = q / (g.nsp * g.nshell), r = q % (g.nsp * g.nshell);
+                    A.series[(long)(stepc - 1) * nss + q] = cn[a * g.rstride + r];
                 }
             }
-        }
-        __syncthreads();
-        const double cur = A.c2[s];
-        if (A.accept == 2 || cur < A.best_c2[s]) {
-            for (int q = threadIdx.x; q < g.natoms_pad; q += blockDim.x) A.best_sp[base_sp + q] = lsp[q];
+            // FP64 re-anchor of the running accumulator every reanchor_every accepted moves
+            if (A.accept == 1 && A.incremental && A.reanchor_every > 0 && acc) {
+                if (t == 0) {
+                    const int n = A.anchor[s] + 1;
+                    A.anchor[s] = (n >= A.reanchor_every) ? 0 : n;
+                    flag = (n >= A.reanchor_every);
+                }
+                __syncthreads();
+                if (flag) {
+                    double *mc = A.mcur + (long)s * g.ncol_tot;
+                    for (int q = t; q < g.ncol_tot; q += blockDim.x) {
+                        const int e = q / g.ncol_e, col = q % g.ncol_e;
+                        const double *B = A.basisR + (long)e * g.nrow_e * g.ncol_e + col;
+                        const int *c = cn + e * g.rstride;
+                        double v = 0.0;
+                        for (int k = 0; k < g.nrow_e; ++k) v = fma((double)c[k], B[(long)k * g.ncol_e], v);
+                        mc[q] = v;
+                    }
+                    __syncthreads();
+                    for (int e = 0; e < g.nsp; ++e) {
+                        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+                        for (int q = t; q < g.ncol_e; q += blockDim.x) {
+                            const long o = (long)e * g.ncol_e + q;
+                            const double tg = A.target[o], w = A.wcol[o], d = mc[o] - tg;
+                            l0 += d * tg;
+                            l1 += d * d;
+                            l2 += d * tg * w;
+                            l3 += d * d * w;
+                        }
+                        l0 = block_sum(l0, red);
+                        l1 = block_sum(l1, red);
+                        l2 = block_sum(l2, red);
+                        l3 = block_sum(l3, red);
+                        if (t == 0) {
+                            sums[e][0] = l0;
+                            sums[e][1] = l1;
+                            sums[e][2] = l2;
+                            sums[e][3] = l3;
+                        }
+                    }
+                    __syncthreads();
+                    if (t == 0) {
+                        A.c2_next[s] = chi2_of_sums(g, sums, A.q0, A.profile, A.nfit);
+                        A.nanchor[s] += 1;
+                    }
+                }
+            }
             __syncthreads();
-            if (threadIdx.x == 0) A.best_c2[s] = cur;
+            const double cur = A.c2_next[s];
+            if (A.accept == 2 || cur < A.best_c2[s]) {
+                for (int q = t; q < g.natoms_pad; q += blockDim.x) A.best_sp[base_sp + q] = lsp[q];
+                __syncthreads();
+                if (t == 0) A.best_c2[s] = cur;
+            }
         }
     }
     if (!A.propose) return;
-    // ------------------------------------------------ propose + gather, step t+1
-    for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) hist[q] = 0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        P = propose(lsp, g.natoms, A.seed, step_sh + 1, (uint64_t)rung);
-        int *o = A.prop + 5 * s;
-        o[0] = P.a;
-        o[1] = P.b;
-        o[2] = P.na;
-        o[3] = P.nb;
-        o[4] = P.sw;
+    // ------------------------------------------------ propose + gather of the next step
+    if (t == 0) {
+        P = propose(lsp, g.natoms, A.seed, stepc + 1, (uint64_t)rung);
+        if (lead) {
+            int *o = A.prop_next + 5 * s;
+            o[0] = P.a;
+            o[1] = P.b;
+            o[2] = P.na;
+            o[3] = P.nb;
+            o[4] = P.sw;
+        }
     }
     __syncthreads();
     const Prop p = P;
@@ -780,17 +798,20 @@ __global__ __launch_bounds__(1024) void k_slot_step(Geo g, SlotArgs A) {
         const int nitems = nss_items + 2 * g.nstencil;
 #pragma unroll
         for (int r = 0; r < RMAX; ++r) {
-            const int it = t + r * (int)blockDim.x;
+            const int it = first + r * stride;
             if (it < nitems) gather_item(g, W, A.ss_sten, lsp, hist, it, nss_items, recs[r]);
         }
-        for (int it = t + RMAX * (int)blockDim.x; it < nitems; it += blockDim.x) {
+        for (int it = first + RMAX * stride; it < nitems; it += stride) {
             const int m = it - nss_items;
             const int4 rec = (m >= 0) ? A.ms_sten[m < g.nstencil ? m : m - g.nstencil] : make_int4(0, 0, 0, 0);
             gather_item(g, W, A.ss_sten, lsp, hist, it, nss_items, rec);
         }
+        __syncthreads();
+        // this workgroup's share of the count changes into the (zeroed) next-step row
+        int *dcp = A.dc_prop + (long)s * g.nrow_tot;
+        for (int q = t; q < g.nrow_tot; q += blockDim.x)
+            if (hist[q] != 0) atomicAdd(&dcp[q], hist[q]);
     }
-    __syncthreads();
-    for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) dc[q] = hist[q];
 }
 
 // chi^2 of arbitrary window transforms (one row per workgroup), direct form:
@@ -875,8 +896,8 @@ struct chm_sampler {
         prop, mtrial, mcur, partials, anchor, nanchor, exch_att, exch_acc, acc_rung, log, series,
         tmp;
     int log_cap = 0, series_cap = 0, series_rung = -1;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};     // exchange period starting at an even /
+    hipGraphExec_t gexec[2] = {nullptr, nullptr}; // odd completed-step count
     int graph_steps = 0;
 };
 
@@ -901,12 +922,9 @@ SlotArgs slot_args(chm_sampler *S, int accept, int propose) {
     A.basisR = S->basisR.as<double>();
     A.mcur = S->mcur.as<double>();
     A.beta = S->beta.as<double>();
-    A.prop = S->prop.as<int>();
     A.counts = S->counts.as<int>();
-    A.dcounts = S->dcounts.as<int>();
     A.species = S->species.as<uint8_t>();
     A.best_sp = S->best_sp.as<uint8_t>();
-    A.c2 = S->c2.as<double>();
     A.best_c2 = S->best_c2.as<double>();
     A.rung_of = S->rung_of.as<int>();
     A.stepc = S->stepc.as<uint64_t>();
@@ -936,49 +954,69 @@ size_t lds_gemm(const Geo &g) {
     return std::max(a, p);
 }
 
-int launch_gemm_abs(chm_sampler *S) {
+// per-step double buffers, indexed by step parity: the count-change rows and the
+// proposal records of step n, and chi^2 after step n
+int *drow(chm_sampler *S, long long n) {
+    return S->dcounts.as<int>() + (size_t)(n & 1) * S->nrep * S->g.nrow_tot;
+}
+int *prow(chm_sampler *S, long long n) { return S->prop.as<int>() + (size_t)(n & 1) * S->nrep * 5; }
+double *c2row(chm_sampler *S, long long n) { return S->c2.as<double>() + (size_t)(n & 1) * S->nrep; }
+
+int launch_gemm_abs(chm_sampler *S, const int *dc, int bump) {
     const Geo &g = S->g;
     dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
+    TileOut O = tile_out(S);
+    O.bump = bump;
     k_gemm_abs_f64<<<grid, 256, lds_gemm(g), S->ctx->stream>>>(
-        g, S->nrep, S->counts.as<int>(), S->dcounts.as<int>(), S->basisR.as<double>(), tile_out(S));
+        g, S->nrep, S->counts.as<int>(), dc, S->basisR.as<double>(), O);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
 
-int launch_gemm_delta(chm_sampler *S) {
+int launch_gemm_delta(chm_sampler *S, const int *dc, int bump) {
     const Geo &g = S->g;
     dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
+    TileOut O = tile_out(S);
+    O.bump = bump;
     k_gemm_delta_bf16<<<grid, 256, 0, S->ctx->stream>>>(
-        g, S->nrep, S->dcounts.as<int>(), reinterpret_cast<const bf16x8 *>(S->Tb.p),
-        S->mcur.as<double>(), tile_out(S));
+        g, S->nrep, dc, reinterpret_cast<const bf16x8 *>(S->Tb.p), S->mcur.as<double>(), O);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
 
-int launch_slot(chm_sampler *S, int accept, int propose) {
-    k_slot_step<<<dim3(S->nrep), 1024, S->lds_k1, S->ctx->stream>>>(S->g,
-                                                                   slot_args(S, accept, propose));
+// accept: 0 none, 1 step n, 2 init (n = 0).  propose: step n + 1.
+int launch_slot(chm_sampler *S, int accept, int propose, long long n) {
+    SlotArgs A = slot_args(S, accept, propose);
+    A.dc_acc = drow(S, n);
+    A.dc_prop = drow(S, n + 1);
+    A.prop_acc = prow(S, n);
+    A.prop_next = prow(S, n + 1);
+    A.c2_prev = c2row(S, n - 1);
+    A.c2_next = c2row(S, n);
+    k_slot_step<<<dim3(S->nsplit, S->nrep), 256, S->lds_k1, S->ctx->stream>>>(S->g, A);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
 
-int launch_gemm(chm_sampler *S) {
-    return S->incremental ? launch_gemm_delta(S) : launch_gemm_abs(S);
+int launch_gemm(chm_sampler *S, const int *dc, int bump) {
+    return S->incremental ? launch_gemm_delta(S, dc, bump) : launch_gemm_abs(S, dc, bump);
 }
 
-// L steps with no exchange inside: propose, then L x (transform, accept [+ propose]).
-int launch_segment(chm_sampler *S, int L) {
-    int rc = launch_slot(S, 0, 1);
+// L steps after h completed ones, no exchange inside: propose step h+1, then for every
+// step n: transform (advances the device step counter), accept [+ propose n+1].
+int launch_segment(chm_sampler *S, long long h, int L) {
+    int rc = launch_slot(S, 0, 1, h);
     for (int t = 0; t < L && !rc; ++t) {
-        rc = launch_gemm(S);
-        if (!rc) rc = launch_slot(S, 1, t + 1 < L ? 1 : 0);
+        const long long n = h + 1 + t;
+        rc = launch_gemm(S, drow(S, n), 1);
+        if (!rc) rc = launch_slot(S, 1, t + 1 < L ? 1 : 0, n);
     }
     return rc;
 }
 
-int launch_exchange(chm_sampler *S) {
+int launch_exchange(chm_sampler *S, long long n) {
     k_exchange<<<1, 256, 0, S->ctx->stream>>>(
-        S->nrep, S->beta.as<double>(), S->c2.as<double>(), S->slot_of.as<int>(),
+        S->nrep, S->beta.as<double>(), c2row(S, n), S->slot_of.as<int>(),
         S->rung_of.as<int>(), S->iex.as<uint64_t>(), S->exch_att.as<long long>(),
         S->exch_acc.as<long long>(), S->seed, S->nfit);
     CHM_LAUNCH_CHECK();
@@ -986,10 +1024,12 @@ int launch_exchange(chm_sampler *S) {
 }
 
 void drop_graph(chm_sampler *S) {
-    if (S->gexec) hipGraphExecDestroy(S->gexec);
-    if (S->graph) hipGraphDestroy(S->graph);
-    S->gexec = nullptr;
-    S->graph = nullptr;
+    for (int q = 0; q < 2; ++q) {
+        if (S->gexec[q]) hipGraphExecDestroy(S->gexec[q]);
+        if (S->graph[q]) hipGraphDestroy(S->graph[q]);
+        S->gexec[q] = nullptr;
+        S->graph[q] = nullptr;
+    }
     S->graph_steps = 0;
 }
 
@@ -1143,6 +1183,11 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
     S->profile = profile_scale ? 1 : 0;
     S->nfit = (double)g.ncol_tot;  // nsp*nkw*nwin complex points, Re and Im counted
     S->lds_k1 = sizeof(int) * ((g.nrow_tot + 3) & ~3) + (size_t)g.natoms_pad;
+    {
+        // workgroups per replica for the gather: ~3 items per lane, at most 16
+        const int items = 2 * g.ncoord + 2 * g.nstencil;
+        S->nsplit = std::max(1, std::min(16, (items + 767) / 768));
+    }
     auto fail = [&](int rc) {
         delete S;
         return rc;
@@ -1191,11 +1236,11 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
     size_t nr = nrep;
     if ((rc = S->species.ensure(nr * g.natoms_pad)) || (rc = S->best_sp.ensure(nr * g.natoms_pad)) ||
         (rc = S->counts.ensure(sizeof(int) * nr * g.nrow_tot)) ||
-        (rc = S->dcounts.ensure(sizeof(int) * nr * g.nrow_tot)) ||
-        (rc = S->c2.ensure(sizeof(double) * nr)) || (rc = S->best_c2.ensure(sizeof(double) * nr)) ||
+        (rc = S->dcounts.ensure(sizeof(int) * 2 * nr * g.nrow_tot)) ||
+        (rc = S->c2.ensure(sizeof(double) * 2 * nr)) || (rc = S->best_c2.ensure(sizeof(double) * nr)) ||
         (rc = S->rung_of.ensure(sizeof(int) * nr)) || (rc = S->slot_of.ensure(sizeof(int) * nr)) ||
         (rc = S->beta.ensure(sizeof(double) * nr)) || (rc = S->stepc.ensure(sizeof(uint64_t) * nr)) ||
-        (rc = S->iex.ensure(sizeof(uint64_t))) || (rc = S->prop.ensure(sizeof(int) * 5 * nr)) ||
+        (rc = S->iex.ensure(sizeof(uint64_t))) || (rc = S->prop.ensure(sizeof(int) * 2 * 5 * nr)) ||
         (rc = S->mtrial.ensure(sizeof(double) * nr * g.ncol_tot)) ||
         (rc = S->mcur.ensure(sizeof(double) * nr * g.ncol_tot)) ||
         (rc = S->partials.ensure(sizeof(double) * nr * g.nsp * g.nct * 4)) ||
@@ -1291,9 +1336,9 @@ extern "C" int chm_sampler_init(chm_sampler *S, const int32_t *species0, const d
         g, S->species.as<uint8_t>(), S->site_xyz.as<int>(), S->nbr_j.as<int>(), S->nbr_sh.as<int>(),
         S->stencil.as<int4>(), S->counts.as<int>());
     CHM_LAUNCH_CHECK();
-    int rc = launch_gemm_abs(S);
+    int rc = launch_gemm_abs(S, drow(S, 0), 0);    // both rows are zero here
     if (rc) return rc;
-    if ((rc = launch_slot(S, 2, 0))) return rc;
+    if ((rc = launch_slot(S, 2, 0, 0))) return rc;
     CHM_HIP(hipStreamSynchronize(st));
     S->inited = true;
     return CHM_OK;
@@ -1306,39 +1351,44 @@ extern "C" int chm_sampler_run(chm_sampler *S, int nsteps, int exchange_every) {
     hipStream_t st = S->ctx->stream;
     const int E = exchange_every;
     int rc = 0;
-    if (E > 0 && (S->graph_steps != E)) {
-        // one exchange period: propose, E x (transform, accept [+ propose]), exchange
+    if (E > 0 && S->graph_steps != E) {
+        // one exchange period per graph: propose, E x (transform, accept [+ propose]),
+        // exchange.  The count-change rows alternate by step parity, so a period that
+        // starts after an odd number of steps (odd E) has its own graph.
         drop_graph(S);
-        CHM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        rc = launch_segment(S, E);
-        if (!rc) rc = launch_exchange(S);
-        hipGraph_t gr = nullptr;
-        hipError_t e = hipStreamEndCapture(st, &gr);
-        if (rc) {
-            if (gr) hipGraphDestroy(gr);
-            return rc;
+        for (int par = 0; par < ((E & 1) ? 2 : 1) && !rc; ++par) {
+            CHM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = launch_segment(S, par, E);
+            if (!rc) rc = launch_exchange(S, par + E);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &gr);
+            if (rc) {
+                if (gr) hipGraphDestroy(gr);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                chm::set_error(std::string("graph capture: ") + hipGetErrorString(e));
+                return CHM_E_HIP;
+            }
+            S->graph[par] = gr;
+            CHM_HIP(hipGraphInstantiate(&S->gexec[par], gr, nullptr, nullptr, 0));
         }
-        if (e != hipSuccess) {
-            chm::set_error(std::string("graph capture: ") + hipGetErrorString(e));
-            return CHM_E_HIP;
-        }
-        S->graph = gr;
-        CHM_HIP(hipGraphInstantiate(&S->gexec, gr, nullptr, nullptr, 0));
+        if (rc) return rc;
         S->graph_steps = E;
     }
     long long left = nsteps;
     while (left > 0) {
         if (E > 0 && S->hstep % E == 0 && left >= E) {
-            CHM_HIP(hipGraphLaunch(S->gexec, st));
+            CHM_HIP(hipGraphLaunch(S->gexec[(E & 1) ? (S->hstep & 1) : 0], st));
             S->hstep += E;
             left -= E;
         } else {
             long long L = left;
             if (E > 0) L = std::min<long long>(left, E - S->hstep % E);
-            if ((rc = launch_segment(S, (int)L))) return rc;
+            if ((rc = launch_segment(S, S->hstep, (int)L))) return rc;
             S->hstep += L;
             left -= L;
-            if (E > 0 && S->hstep % E == 0 && (rc = launch_exchange(S))) return rc;
+            if (E > 0 && S->hstep % E == 0 && (rc = launch_exchange(S, S->hstep))) return rc;
         }
     }
     CHM_HIP(hipStreamSynchronize(st));
@@ -1378,7 +1428,7 @@ extern "C" int chm_sampler_read(chm_sampler *S, int what, void *host, int64_t nb
         case 3: {
             if ((rc = need((int64_t)nrep * 8))) return rc;
             std::vector<double> v(nrep);
-            CHM_HIP(hipMemcpy(v.data(), what == 1 ? S->c2.p : S->best_c2.p, 8 * nrep,
+            CHM_HIP(hipMemcpy(v.data(), what == 1 ? (void *)c2row(S, S->hstep) : S->best_c2.p, 8 * nrep,
                               hipMemcpyDeviceToHost));
             double *o = static_cast<double *>(host);
             for (int r = 0; r < nrep; ++r) o[r] = v[slot_of[r]];
@@ -1536,7 +1586,7 @@ extern "C" int chm_sampler_chi2(chm_sampler *S, int nrows, const double *chiR, d
 extern "C" int chm_sampler_exchange(chm_sampler *S) {
     CHM_REQUIRE(S && S->inited, CHM_E_STATE, "init_state first");
     CHM_HIP(hipSetDevice(S->ctx->device));
-    int rc = launch_exchange(S);
+    int rc = launch_exchange(S, S->hstep);
     if (rc) return rc;
     CHM_HIP(hipStreamSynchronize(S->ctx->stream));
     return CHM_OK;
@@ -1574,28 +1624,22 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
     CHM_HIP(hipEventCreate(&e0));
     CHM_HIP(hipEventCreate(&e1));
     float t = 0.f;
-    int rc = launch_slot(S, 0, 1);          // a proposal pending from here on
-    // per-slot kernel, propose only: the pending proposal is recomputed identically
+    long long h = S->hstep;
+    int rc = launch_slot(S, 0, 1, h);        // step h+1 pending
+    // the trial transform of the pending step, repeated (no counter advance)
     CHM_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < iters && !rc; ++i) rc = launch_slot(S, 0, 1);
-    CHM_HIP(hipEventRecord(e1, st));
-    if (rc) return rc;
-    CHM_HIP(hipEventSynchronize(e1));
-    CHM_HIP(hipEventElapsedTime(&t, e0, e1));
-    ms[0] = t / iters;
-    // the trial transform of the pending proposal
-    CHM_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < iters && !rc; ++i) rc = launch_gemm(S);
+    for (int i = 0; i < iters && !rc; ++i) rc = launch_gemm(S, drow(S, h + 1), 0);
     CHM_HIP(hipEventRecord(e1, st));
     if (rc) return rc;
     CHM_HIP(hipEventSynchronize(e1));
     CHM_HIP(hipEventElapsedTime(&t, e0, e1));
     ms[1] = t / iters;
-    // full steps: accept + propose, transform (each a real step of the chain)
+    // real steps of the chain: transform, then accept + propose
     CHM_HIP(hipEventRecord(e0, st));
     for (int i = 0; i < iters && !rc; ++i) {
-        rc = launch_slot(S, 1, 1);
-        if (!rc) rc = launch_gemm(S);
+        const long long n = h + 1 + i;
+        rc = launch_gemm(S, drow(S, n), 1);
+        if (!rc) rc = launch_slot(S, 1, 1, n);
     }
     CHM_HIP(hipEventRecord(e1, st));
     if (rc) return rc;
@@ -1603,8 +1647,12 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
     CHM_HIP(hipEventElapsedTime(&t, e0, e1));
     ms[3] = t / iters;
     ms[2] = ms[3] - ms[1];                   // accept + propose + gather per step
-    rc = launch_slot(S, 1, 0);               // settle: no proposal pending
+    ms[0] = 0.0;
+    const long long n = h + 1 + iters;       // settle: transform and accept the last proposal
+    if (!rc) rc = launch_gemm(S, drow(S, n), 1);
+    if (!rc) rc = launch_slot(S, 1, 0, n);
     if (rc) return rc;
+    S->hstep = n;
     CHM_HIP(hipStreamSynchronize(st));
     hipEventDestroy(e0);
     hipEventDestroy(e1);
