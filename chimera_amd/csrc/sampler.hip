@@ -904,7 +904,9 @@ struct chm_sampler {
 namespace {
 
 TileOut tile_out(chm_sampler *S) {
-    TileOut O;
+    TileOut O{};
+    O.stepc = S->stepc.as<uint64_t>();
+    O.bump = 0;
     O.target = S->target.as<double>();
     O.wcol = S->wcol.as<double>();
     O.mtrial = S->mtrial.as<double>();
@@ -913,7 +915,7 @@ TileOut tile_out(chm_sampler *S) {
 }
 
 SlotArgs slot_args(chm_sampler *S, int accept, int propose) {
-    SlotArgs A;
+    SlotArgs A{};
     A.partials = S->partials.as<double>();
     A.q0 = S->q0.as<double>();
     A.mtrial = S->mtrial.as<double>();
