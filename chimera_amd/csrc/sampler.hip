@@ -171,6 +171,9 @@ constexpr int KS = 16;    // K slices of the FP64 kernel
 struct TileOut {
     uint64_t *stepc;        // per slot: advanced once per transform when bump is set
     int bump;
+    const int *anchor;      // re-anchor mode: slots with anchor[s] >= anchor_min get
+    int anchor_min;         //   anchor_out[s] = counts[s] @ basis_R (FP64); nothing else
+    double *anchor_out;     //   is written.  Null in the normal mode.
     const double *target;   // (ncol_tot)
     const double *wcol;     // (ncol_tot) 1/sigma^2 of each column
     double *mtrial;         // (nrep, ncol_tot)
@@ -214,19 +217,33 @@ __global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int
                                                        const int *__restrict__ dcounts,
                                                        const double *__restrict__ basisR, TileOut O) {
     extern __shared__ double dl[];
+    __shared__ int due[RB];
+    __shared__ int any_due;
     const int e = blockIdx.z;
     const int r0 = blockIdx.y * RB;
     const int ct = blockIdx.x;
     const int c0 = ct * NCW;
     const int K = g.nrow_e;
     const int ld = RB + 1;
+    const bool reanchor = O.anchor_out != nullptr;
+    if (reanchor) {
+        if (threadIdx.x == 0) any_due = 0;
+        __syncthreads();
+        if (threadIdx.x < RB) {
+            const int slot = r0 + threadIdx.x;
+            due[threadIdx.x] = slot < nrep && O.anchor[slot] >= O.anchor_min;
+            if (due[threadIdx.x]) atomicOr(&any_due, 1);
+        }
+        __syncthreads();
+        if (!any_due) return;
+    }
     for (int q = threadIdx.x; q < RB * K; q += blockDim.x) {
         int m = q / K, k = q % K;
         int slot = r0 + m;
         double v = 0.0;
         if (slot < nrep) {
             long o = (long)slot * g.nrow_tot + (long)e * g.rstride + k;
-            v = (double)(counts[o] + dcounts[o]);
+            v = (double)(counts[o] + (dcounts ? dcounts[o] : 0));
         }
         dl[k * ld + m] = v;
     }
@@ -259,9 +276,15 @@ __global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int
         const int m = t / NCW + 16 * h, cl = t % NCW;
         double sum = 0.0;
         for (int z = 0; z < KS; ++z) sum += P[(z * RB + m) * NCW + cl];
-        tile_epilogue(g, O, nrep, r0 + m, e, ct, c0 + cl, sum);
+        if (reanchor) {
+            if (due[m] && c0 + cl < g.ncol_e)
+                O.anchor_out[(long)(r0 + m) * g.ncol_tot + (long)e * g.ncol_e + c0 + cl] = sum;
+        } else {
+            tile_epilogue(g, O, nrep, r0 + m, e, ct, c0 + cl, sum);
+        }
     }
-    if (O.bump && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
+    if (!reanchor && O.bump && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < RB &&
+        r0 + (int)threadIdx.x < nrep)
         O.stepc[r0 + threadIdx.x] += 1;
 }
 
@@ -460,7 +483,7 @@ struct SlotArgs {
     const double *target, *wcol;
     const double *basisR;   // FP64, for re-anchoring
     double *mcur;           // (nrep, ncol_tot) running accumulator (incremental mode)
-    const double *beta;     // per rung
+    const double *beta_slot;// beta of each slot's current rung (kept by the exchange)
     const int *prop_acc;    // (nrep, 5) a, b, new a, new b, is-swap of the step accepted
     int *prop_next;         // (nrep, 5) the proposal made here (the other parity buffer)
     const double *c2_prev;  // chi^2 after the previous step (read by every workgroup)
@@ -475,6 +498,7 @@ struct SlotArgs {
     long long *acc_rung;
     int *anchor;            // accepted moves since the last FP64 re-anchor, per slot
     long long *nanchor;     // re-anchors done, per slot
+    int reanchor_step;      // the FP64 re-anchor kernel ran before this step's transform
     const int *ss_sten;     // (ncoord) packed dx,dy,dz,shell (int8 each)
     const int4 *ms_sten;    // (nstencil) 16-byte path records
     int reanchor_every;     // 0: never
@@ -613,11 +637,21 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     __shared__ int flag;
     __shared__ Prop P;
     const int s = blockIdx.y;
+    const int nwg = gridDim.x;
     const bool lead = blockIdx.x == 0;       // the workgroup that owns the slot's state
     const int t = threadIdx.x;
+    if (t == 0) flag = 0;
     const long base_sp = (long)s * g.natoms_pad;
+    // every small per-slot input is requested up front, in one round trip
     const uint64_t stepc = A.stepc[s];       // written only by the transform kernels
     const int rung = A.rung_of[s];
+    const double beta = A.beta_slot[s];
+    const double c2p = A.c2_prev ? A.c2_prev[s] : 0.0;
+    const double best = A.best_c2[s];
+    const int anchor = A.anchor[s];
+    int pp[5] = {0, 0, 0, 0, 0};
+    if (A.accept == 1)
+        for (int q = 0; q < 5; ++q) pp[q] = A.prop_acc[5 * s + q];
     load_species(lsp, A.species + base_sp, g.natoms_pad);
     // per-edge sums over the column tiles: (e, j) groups of CT lanes, then a tree
     const int G = 4 * g.nsp;
@@ -631,13 +665,25 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
         for (int c = c0; c < g.nct; c += CT) v += Pp[4 * c];
         red[t] = v;
     }
+    // this workgroup's share of the running chi(R) to copy forward on acceptance
+    constexpr int MC = 4;
+    double mt[MC];
+    const int mchunk = (g.ncol_tot + nwg - 1) / nwg;
+    const int mbeg = blockIdx.x * mchunk, mend = min(g.ncol_tot, mbeg + mchunk);
+    if (A.accept && A.incremental) {
+#pragma unroll
+        for (int r = 0; r < MC; ++r) {
+            const int q = mbeg + t + r * (int)blockDim.x;
+            mt[r] = (q < mend) ? A.mtrial[(long)s * g.ncol_tot + q] : 0.0;
+        }
+    }
     if (A.propose)
         for (int q = t; q < g.nrow_tot; q += blockDim.x) hist[q] = 0;
     // prefetch this lane's multiple-scattering records (independent of the proposal)
     constexpr int RMAX = 8;
     int4 recs[RMAX];
     const int nss_items = 2 * g.ncoord;
-    const int stride = gridDim.x * blockDim.x;
+    const int stride = nwg * blockDim.x;
     const int first = blockIdx.x * blockDim.x + t;
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
@@ -658,7 +704,6 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
         if (t < G * CT && c0 == 0) sums[gi >> 2][gi & 3] = red[t];
         __syncthreads();
         // every workgroup of the slot takes the same (deterministic) decision
-        const int *pp = A.prop_acc + 5 * s;
         if (t == 0) {
             const double c2t = chi2_of_sums(g, sums, A.q0, A.profile, A.nfit);
             int acc = 0;
@@ -669,11 +714,10 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                     A.best_c2[s] = c2t;
                 }
             } else {
-                const double c2p = A.c2_prev[s];
                 if (pp[4]) {
-                    double x = A.beta[rung] * (c2t - c2p) * A.nfit;
+                    const double x = beta * (c2t - c2p) * A.nfit;
                     uint64_t h = splitmix64(A.seed ^ (stepc * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
-                    double u = u01(splitmix64(h));
+                    const double u = u01(splitmix64(h));
                     acc = (x <= 0.0) || (u < exp(-fmin(x, 60.0)));
                 }
                 if (acc) {
@@ -681,7 +725,8 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                     lsp[pp[1]] = (uint8_t)pp[3];
                 }
                 if (lead) {
-                    A.c2_next[s] = acc ? c2t : c2p;
+                    const double cur = acc ? c2t : c2p;
+                    A.c2_next[s] = cur;
                     if (acc) {
                         A.species[base_sp + pp[0]] = (uint8_t)pp[2];
                         A.species[base_sp + pp[1]] = (uint8_t)pp[3];
@@ -694,89 +739,55 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                         L[2] = acc;
                         L[3] = rung;
                     }
+                    // accepted moves since the running chi(R) was last recomputed in FP64
+                    if (A.incremental) {
+                        if (A.reanchor_step && anchor >= A.reanchor_every) {
+                            A.anchor[s] = acc;
+                            A.nanchor[s] += 1;
+                        } else if (acc) {
+                            A.anchor[s] = anchor + 1;
+                        }
+                    }
+                    flag = (cur < best) ? 2 : 0;
                 }
             }
-            flag = acc;
+            flag |= acc;
         }
         __syncthreads();
-        const int acc = flag;
+        const int acc = flag & 1;
+        if (A.incremental && acc) {
+            double *mc = A.mcur + (long)s * g.ncol_tot;
+#pragma unroll
+            for (int r = 0; r < MC; ++r) {
+                const int q = mbeg + t + r * (int)blockDim.x;
+                if (q < mend) mc[q] = mt[r];
+            }
+            for (int q = mbeg + t + MC * (int)blockDim.x; q < mend; q += blockDim.x)
+                mc[q] = A.mtrial[(long)s * g.ncol_tot + q];
+        }
         if (lead) {
             int *dca = A.dc_acc + (long)s * g.nrow_tot;
             for (int q = t; q < g.nrow_tot; q += blockDim.x) {
                 if (acc && A.accept == 1) cn[q] += dca[q];
                 dca[q] = 0;
             }
-            if (A.incremental && acc) {
-                const double *mt = A.mtrial + (long)s * g.ncol_tot;
-                double *mc = A.mcur + (long)s * g.ncol_tot;
-                for (int q = t; q < g.ncol_tot; q += blockDim.x) mc[q] = mt[q];
-            }
-            __syncthreads();
             if (A.accept == 1 && A.series && rung == A.series_rung && stepc - 1 < (uint64_t)A.series_cap) {
+                __syncthreads();
                 const int nss = g.nsp * g.nsp * g.nshell;
                 for (int q = t; q < nss; q += blockDim.x) {
                     int a = q / (g.nsp * g.nshell), r = q % (g.nsp * g.nshell);
                     A.series[(long)(stepc - 1) * nss + q] = cn[a * g.rstride + r];
                 }
             }
-            // FP64 re-anchor of the running accumulator every reanchor_every accepted moves
-            if (A.accept == 1 && A.incremental && A.reanchor_every > 0 && acc) {
-                if (t == 0) {
-                    const int n = A.anchor[s] + 1;
-                    A.anchor[s] = (n >= A.reanchor_every) ? 0 : n;
-                    flag = (n >= A.reanchor_every);
-                }
-                __syncthreads();
-                if (flag) {
-                    double *mc = A.mcur + (long)s * g.ncol_tot;
-                    for (int q = t; q < g.ncol_tot; q += blockDim.x) {
-                        const int e = q / g.ncol_e, col = q % g.ncol_e;
-                        const double *B = A.basisR + (long)e * g.nrow_e * g.ncol_e + col;
-                        const int *c = cn + e * g.rstride;
-                        double v = 0.0;
-                        for (int k = 0; k < g.nrow_e; ++k) v = fma((double)c[k], B[(long)k * g.ncol_e], v);
-                        mc[q] = v;
-                    }
-                    __syncthreads();
-                    for (int e = 0; e < g.nsp; ++e) {
-                        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
-                        for (int q = t; q < g.ncol_e; q += blockDim.x) {
-                            const long o = (long)e * g.ncol_e + q;
-                            const double tg = A.target[o], w = A.wcol[o], d = mc[o] - tg;
-                            l0 += d * tg;
-                            l1 += d * d;
-                            l2 += d * tg * w;
-                            l3 += d * d * w;
-                        }
-                        l0 = block_sum(l0, red);
-                        l1 = block_sum(l1, red);
-                        l2 = block_sum(l2, red);
-                        l3 = block_sum(l3, red);
-                        if (t == 0) {
-                            sums[e][0] = l0;
-                            sums[e][1] = l1;
-                            sums[e][2] = l2;
-                            sums[e][3] = l3;
-                        }
-                    }
-                    __syncthreads();
-                    if (t == 0) {
-                        A.c2_next[s] = chi2_of_sums(g, sums, A.q0, A.profile, A.nfit);
-                        A.nanchor[s] += 1;
-                    }
-                }
-            }
-            __syncthreads();
-            const double cur = A.c2_next[s];
-            if (A.accept == 2 || cur < A.best_c2[s]) {
+            if (A.accept == 2 || (flag & 2)) {
                 for (int q = t; q < g.natoms_pad; q += blockDim.x) A.best_sp[base_sp + q] = lsp[q];
-                __syncthreads();
-                if (t == 0) A.best_c2[s] = cur;
+                if (t == 0 && A.accept == 1) A.best_c2[s] = A.c2_next[s];
             }
         }
     }
     if (!A.propose) return;
     // ------------------------------------------------ propose + gather of the next step
+    __syncthreads();
     if (t == 0) {
         P = propose(lsp, g.natoms, A.seed, stepc + 1, (uint64_t)rung);
         if (lead) {
@@ -847,8 +858,9 @@ __global__ __launch_bounds__(256) void k_eval_rows(Geo g, const double *__restri
 // ---------------------------------------------------------------- K4
 __global__ void k_exchange(int nrep, const double *__restrict__ beta, const double *__restrict__ c2,
                            int *__restrict__ slot_of, int *__restrict__ rung_of,
-                           uint64_t *__restrict__ iex, long long *__restrict__ att,
-                           long long *__restrict__ acc, uint64_t seed, double nfit) {
+                           double *__restrict__ beta_slot, uint64_t *__restrict__ iex,
+                           long long *__restrict__ att, long long *__restrict__ acc, uint64_t seed,
+                           double nfit) {
     __shared__ uint64_t hx;
     if (threadIdx.x == 0) {
         uint64_t n = iex[0] + 1;
@@ -869,6 +881,8 @@ __global__ void k_exchange(int nrep, const double *__restrict__ beta, const doub
             slot_of[i + 1] = si;
             rung_of[sj] = i;
             rung_of[si] = i + 1;
+            beta_slot[sj] = beta[i];
+            beta_slot[si] = beta[i + 1];
         }
     }
 }
@@ -892,7 +906,8 @@ struct chm_sampler {
     double nfit = 1.0;
     size_t lds_k1 = 0;
     chm::DevBuf site_xyz, nbr_j, nbr_sh, ss_sten, stencil, basisR, basisK, target, wcol, q0, Tb;
-    chm::DevBuf species, counts, dcounts, c2, best_c2, best_sp, rung_of, slot_of, beta, stepc, iex,
+    chm::DevBuf species, counts, dcounts, c2, best_c2, best_sp, rung_of, slot_of, beta, beta_slot,
+        stepc, iex,
         prop, mtrial, mcur, partials, anchor, nanchor, exch_att, exch_acc, acc_rung, log, series,
         tmp;
     int log_cap = 0, series_cap = 0, series_rung = -1;
@@ -923,7 +938,7 @@ SlotArgs slot_args(chm_sampler *S, int accept, int propose) {
     A.wcol = S->wcol.as<double>();
     A.basisR = S->basisR.as<double>();
     A.mcur = S->mcur.as<double>();
-    A.beta = S->beta.as<double>();
+    A.beta_slot = S->beta_slot.as<double>();
     A.counts = S->counts.as<int>();
     A.species = S->species.as<uint8_t>();
     A.best_sp = S->best_sp.as<uint8_t>();
@@ -987,8 +1002,9 @@ int launch_gemm_delta(chm_sampler *S, const int *dc, int bump) {
 }
 
 // accept: 0 none, 1 step n, 2 init (n = 0).  propose: step n + 1.
-int launch_slot(chm_sampler *S, int accept, int propose, long long n) {
+int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanchor_step = 0) {
     SlotArgs A = slot_args(S, accept, propose);
+    A.reanchor_step = reanchor_step;
     A.dc_acc = drow(S, n);
     A.dc_prop = drow(S, n + 1);
     A.prop_acc = prow(S, n);
@@ -996,6 +1012,19 @@ int launch_slot(chm_sampler *S, int accept, int propose, long long n) {
     A.c2_prev = c2row(S, n - 1);
     A.c2_next = c2row(S, n);
     k_slot_step<<<dim3(S->nsplit, S->nrep), 256, S->lds_k1, S->ctx->stream>>>(S->g, A);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+int launch_reanchor(chm_sampler *S) {
+    const Geo &g = S->g;
+    dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
+    TileOut O = tile_out(S);
+    O.anchor = S->anchor.as<int>();
+    O.anchor_min = S->reanchor_every;
+    O.anchor_out = S->mcur.as<double>();
+    k_gemm_abs_f64<<<grid, 256, lds_gemm(g), S->ctx->stream>>>(
+        g, S->nrep, S->counts.as<int>(), nullptr, S->basisR.as<double>(), O);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -1008,10 +1037,14 @@ int launch_gemm(chm_sampler *S, const int *dc, int bump) {
 // step n: transform (advances the device step counter), accept [+ propose n+1].
 int launch_segment(chm_sampler *S, long long h, int L) {
     int rc = launch_slot(S, 0, 1, h);
+    // incremental mode: replicas with reanchor_every accepted moves since their last FP64
+    // recompute get the running chi(R) recomputed from the integer counts, here
+    const int anchor = S->incremental && S->reanchor_every > 0;
+    if (!rc && anchor) rc = launch_reanchor(S);
     for (int t = 0; t < L && !rc; ++t) {
         const long long n = h + 1 + t;
         rc = launch_gemm(S, drow(S, n), 1);
-        if (!rc) rc = launch_slot(S, 1, t + 1 < L ? 1 : 0, n);
+        if (!rc) rc = launch_slot(S, 1, t + 1 < L ? 1 : 0, n, anchor && t == 0);
     }
     return rc;
 }
@@ -1019,7 +1052,8 @@ int launch_segment(chm_sampler *S, long long h, int L) {
 int launch_exchange(chm_sampler *S, long long n) {
     k_exchange<<<1, 256, 0, S->ctx->stream>>>(
         S->nrep, S->beta.as<double>(), c2row(S, n), S->slot_of.as<int>(),
-        S->rung_of.as<int>(), S->iex.as<uint64_t>(), S->exch_att.as<long long>(),
+        S->rung_of.as<int>(), S->beta_slot.as<double>(), S->iex.as<uint64_t>(),
+        S->exch_att.as<long long>(),
         S->exch_acc.as<long long>(), S->seed, S->nfit);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
@@ -1241,7 +1275,8 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         (rc = S->dcounts.ensure(sizeof(int) * 2 * nr * g.nrow_tot)) ||
         (rc = S->c2.ensure(sizeof(double) * 2 * nr)) || (rc = S->best_c2.ensure(sizeof(double) * nr)) ||
         (rc = S->rung_of.ensure(sizeof(int) * nr)) || (rc = S->slot_of.ensure(sizeof(int) * nr)) ||
-        (rc = S->beta.ensure(sizeof(double) * nr)) || (rc = S->stepc.ensure(sizeof(uint64_t) * nr)) ||
+        (rc = S->beta.ensure(sizeof(double) * nr)) || (rc = S->beta_slot.ensure(sizeof(double) * nr)) ||
+        (rc = S->stepc.ensure(sizeof(uint64_t) * nr)) ||
         (rc = S->iex.ensure(sizeof(uint64_t))) || (rc = S->prop.ensure(sizeof(int) * 2 * 5 * nr)) ||
         (rc = S->mtrial.ensure(sizeof(double) * nr * g.ncol_tot)) ||
         (rc = S->mcur.ensure(sizeof(double) * nr * g.ncol_tot)) ||
@@ -1316,6 +1351,7 @@ extern "C" int chm_sampler_init(chm_sampler *S, const int32_t *species0, const d
     for (int r = 0; r < nrep; ++r) ident[r] = r;
     CHM_HIP(hipMemcpyAsync(S->species.p, sp.data(), sp.size(), hipMemcpyHostToDevice, st));
     CHM_HIP(hipMemcpyAsync(S->beta.p, beta, sizeof(double) * nrep, hipMemcpyHostToDevice, st));
+    CHM_HIP(hipMemcpyAsync(S->beta_slot.p, beta, sizeof(double) * nrep, hipMemcpyHostToDevice, st));
     CHM_HIP(hipMemcpyAsync(S->rung_of.p, ident.data(), sizeof(int) * nrep, hipMemcpyHostToDevice, st));
     CHM_HIP(hipMemcpyAsync(S->slot_of.p, ident.data(), sizeof(int) * nrep, hipMemcpyHostToDevice, st));
     CHM_HIP(hipMemsetAsync(S->counts.p, 0, S->counts.bytes, st));
@@ -1327,6 +1363,8 @@ extern "C" int chm_sampler_init(chm_sampler *S, const int32_t *species0, const d
     CHM_HIP(hipMemsetAsync(S->exch_acc.p, 0, S->exch_acc.bytes, st));
     CHM_HIP(hipMemsetAsync(S->acc_rung.p, 0, S->acc_rung.bytes, st));
     CHM_HIP(hipMemsetAsync(S->anchor.p, 0, S->anchor.bytes, st));
+    CHM_HIP(hipMemsetAsync(S->c2.p, 0, S->c2.bytes, st));
+    CHM_HIP(hipMemsetAsync(S->best_c2.p, 0, S->best_c2.bytes, st));
     CHM_HIP(hipMemsetAsync(S->nanchor.p, 0, S->nanchor.bytes, st));
     if (S->log_cap) CHM_HIP(hipMemsetAsync(S->log.p, 0xff, S->log.bytes, st));
     S->seed = seed;
