@@ -186,7 +186,7 @@ def test_sampler_trajectory_bf16x3(tern108):
     """Split-BF16 incremental transform with FP64 running chi(R) and re-anchoring."""
     P, O = tern108
     S = P.sampler
-    S.precision, S.reanchor_every = "bf16x3", 16
+    S.precision, S.reanchor_every = "bf16x3", 4
     try:
         S2, R = _compare_trajectory(P, O, 60, 10, seed=5, tie_tol=1e-4, c2_rtol=2e-6)
         # running accumulator vs the exact FP64 transform of the same counts
