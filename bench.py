@@ -83,9 +83,14 @@ def main():
     S = P.sampler
     S.precision, S.reanchor_every = args.precision, args.reanchor_every
     S.init_state(P.cfg0, P.cell.ideal)
+    from chimera_amd import autotune
+    tuned = autotune.tune(S, use_cache=False)
+    print(f"[bench] tuned: {tuned}", file=sys.stderr, flush=True)
     print(f"[bench] built in {time.time() - t_build:.1f} s; warmup {args.warmup}",
           file=sys.stderr, flush=True)
     S.run(args.warmup, exchange_every=args.exchange_every, log_every=10 ** 9)
+    # pair counts of the likelihood-temperature rung after every timed step (alpha series)
+    S.set_recording(log_steps=0, series_steps=args.steps)
     P.fm.ctx.sync()
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -132,8 +137,18 @@ def main():
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(P, args.cpu_seconds)
 
-    # alpha and R-factor of the best state (reported beside the rate)
+    # alpha_1 ESS/s of the likelihood-temperature rung over the timed steps, and the
+    # R-factor of the best state (reported beside the rate)
+    from chimera_amd.ess import ess
+    al = S.alpha_series(args.steps, shell=0)
+    iu = np.triu_indices(P.nsp, 1)
+    ess_pairs = [ess(al[:, a, b]) for a, b in zip(*iu)]
+    ess_min = float(min(ess_pairs))
     bsp, _, bc2 = S.best()
+    cnt = np.bincount(bsp, minlength=P.nsp)
+    chib = S.chi_k_of(bsp[None])[0] / np.maximum(cnt, 1)[:, None]
+    mb = P.ft.to_R(chib[None])[0]
+    rfac = W.r_factor(mb, P.target, P.mask, W.fit_scales(mb, P.target, P.mask, P.chan_edge))
     nanchor = int(S.reanchors().sum()) if args.precision == "bf16x3" else 0
     out = {
         "metric": "RMC swap moves/s & alpha_1 ESS/s at fixed R-factor, learned vs uniform proposal",
@@ -154,7 +169,13 @@ def main():
         "cpu_baseline": cpu,
         "gpu_over_cpu": None if cpu is None else round(moves / cpu["value"], 1),
         "best_chi2": round(float(bc2), 5),
+        "best_r_factor": round(rfac, 6),
+        "alpha1_ess": round(ess_min, 1),
+        "alpha1_ess_per_s": round(ess_min / dt, 2),
+        "alpha1_mean": {f"{a}{b}": round(float(al[:, a, b].mean()), 5) for a, b in zip(*iu)},
         "fp64_reanchors": nanchor,
+        "autotune": {"nsplit": tuned["nsplit"],
+                     "step_ms": {k: round(v, 5) for k, v in tuned["times_ms"].items()}},
     }
     print(json.dumps(out))
 

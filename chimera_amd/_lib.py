@@ -69,6 +69,8 @@ _SIGS = {
     "chm_sampler_chik": (C.c_int, [vp, C.c_int, i32p, f64p]),
     "chm_sampler_profile": (C.c_int, [vp, C.c_int, f64p]),
     "chm_sampler_set_precision": (C.c_int, [vp, C.c_int, C.c_int]),
+    "chm_sampler_set_tuning": (C.c_int, [vp, C.c_int]),
+    "chm_sampler_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int)]),
 }
 
 EXPORTS = tuple(_SIGS)

@@ -507,6 +507,7 @@ struct SlotArgs {
     int log_cap;
     int *series;            // (series_cap, nsp*nsp*nshell) or null
     int series_cap, series_rung;
+    long long rec_base;     // record index = step - 1 - rec_base
     uint64_t seed;
     int profile;
     int accept;             // 0: none, 1: step, 2: init (store chi^2, no decision)
@@ -732,8 +733,9 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                         A.species[base_sp + pp[1]] = (uint8_t)pp[3];
                         A.acc_rung[rung] += 1;
                     }
-                    if (A.log && stepc - 1 < (uint64_t)A.log_cap) {
-                        int *L = A.log + ((long)(stepc - 1) * gridDim.y + s) * 4;
+                    const long long ri = (long long)stepc - 1 - A.rec_base;
+                    if (A.log && ri >= 0 && ri < A.log_cap) {
+                        int *L = A.log + ((long)ri * gridDim.y + s) * 4;
                         L[0] = pp[0];
                         L[1] = pp[1];
                         L[2] = acc;
@@ -771,12 +773,13 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                 if (acc && A.accept == 1) cn[q] += dca[q];
                 dca[q] = 0;
             }
-            if (A.accept == 1 && A.series && rung == A.series_rung && stepc - 1 < (uint64_t)A.series_cap) {
+            const long long ri = (long long)stepc - 1 - A.rec_base;
+            if (A.accept == 1 && A.series && rung == A.series_rung && ri >= 0 && ri < A.series_cap) {
                 __syncthreads();
                 const int nss = g.nsp * g.nsp * g.nshell;
                 for (int q = t; q < nss; q += blockDim.x) {
                     int a = q / (g.nsp * g.nshell), r = q % (g.nsp * g.nshell);
-                    A.series[(long)(stepc - 1) * nss + q] = cn[a * g.rstride + r];
+                    A.series[(long)ri * nss + q] = cn[a * g.rstride + r];
                 }
             }
             if (A.accept == 2 || (flag & 2)) {
@@ -911,6 +914,7 @@ struct chm_sampler {
         prop, mtrial, mcur, partials, anchor, nanchor, exch_att, exch_acc, acc_rung, log, series,
         tmp;
     int log_cap = 0, series_cap = 0, series_rung = -1;
+    long long rec_base = 0;   // steps completed when recording (re)started
     hipGraph_t graph[2] = {nullptr, nullptr};     // exchange period starting at an even /
     hipGraphExec_t gexec[2] = {nullptr, nullptr}; // odd completed-step count
     int graph_steps = 0;
@@ -957,6 +961,7 @@ SlotArgs slot_args(chm_sampler *S, int accept, int propose) {
     A.series = S->series_cap ? S->series.as<int>() : nullptr;
     A.series_cap = S->series_cap;
     A.series_rung = S->series_rung;
+    A.rec_base = S->rec_base;
     A.seed = S->seed;
     A.profile = S->profile;
     A.accept = accept;
@@ -1330,6 +1335,7 @@ extern "C" int chm_sampler_set_recording(chm_sampler *S, int log_capacity, int s
     S->log_cap = log_capacity;
     S->series_cap = series_capacity;
     S->series_rung = series_rung;
+    S->rec_base = S->hstep;
     if (log_capacity) CHM_HIP(hipMemset(S->log.p, 0xff, S->log.bytes));
     return CHM_OK;
 }
@@ -1369,6 +1375,7 @@ extern "C" int chm_sampler_init(chm_sampler *S, const int32_t *species0, const d
     if (S->log_cap) CHM_HIP(hipMemsetAsync(S->log.p, 0xff, S->log.bytes, st));
     S->seed = seed;
     S->hstep = 0;
+    S->rec_base = 0;
     long long npairs = (long long)g.natoms * g.ncoord;
     long long nitems = npairs + (long long)g.natoms * g.nabsrec;
     int nsplit = (int)std::max(1LL, std::min(64LL, nitems / 8192 + 1));
@@ -1709,5 +1716,21 @@ extern "C" int chm_sampler_set_precision(chm_sampler *S, int incremental, int re
     S->incremental = incremental;
     S->reanchor_every = reanchor_every;
     S->inited = false;   // the accumulator is built by the next init
+    return CHM_OK;
+}
+
+extern "C" int chm_sampler_set_tuning(chm_sampler *S, int nsplit) {
+    CHM_REQUIRE(S, CHM_E_ARG, "null sampler");
+    CHM_REQUIRE(nsplit >= 1 && nsplit <= 64, CHM_E_ARG, "nsplit must be 1..64");
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    CHM_HIP(hipStreamSynchronize(S->ctx->stream));
+    drop_graph(S);
+    S->nsplit = nsplit;
+    return CHM_OK;
+}
+
+extern "C" int chm_sampler_get_tuning(chm_sampler *S, int *nsplit) {
+    CHM_REQUIRE(S && nsplit, CHM_E_ARG, "bad arguments");
+    *nsplit = S->nsplit;
     return CHM_OK;
 }

@@ -211,6 +211,13 @@ int chm_sampler_exchange(chm_sampler *s);
  * pair-count model ShellModel.chi_k, chimera/shellmodel.py:411-413). */
 int chm_sampler_chik(chm_sampler *s, int nconf, const int32_t *counts, double *chi);
 
+/* Launch shape of the per-replica kernel: workgroups per replica slot for the
+ * accept/propose/gather kernel (1..64).  Chosen per problem by the autotuner
+ * (chimera_amd/autotune.py); the default is ~3 gather items per lane.  No counterpart
+ * in the reference (its launch shapes are fixed, method_a_rmc.py:253-297). */
+int chm_sampler_set_tuning(chm_sampler *s, int nsplit);
+int chm_sampler_get_tuning(chm_sampler *s, int *nsplit);
+
 /* Average device time (ms) per launch of the step's kernels, from HIP events on the
  * sampler's stream over `iters` launches each: ms[0] propose+gather, ms[1] trial
  * transform GEMM, ms[2] accept, ms[3] one whole step.  The state stays consistent

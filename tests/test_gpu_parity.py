@@ -247,6 +247,29 @@ def test_replay_determinism(tern108):
     assert np.array_equal(sp[np.argsort(rung_of)], sps[0])
 
 
+def test_launch_split_invariance(tern108):
+    """The trajectory does not depend on how many workgroups share a replica's gather
+    (the autotuner's knob): every split takes the same decisions and the integer count
+    changes merge exactly."""
+    from chimera_amd import autotune
+    P, O = tern108
+    S = P.sampler
+    S.set_recording(log_steps=30)
+    outs = []
+    for ns in (1, 3, 8):
+        S.init_state(P.cfg0, P.cell.ideal)
+        from chimera_amd import _lib
+        _lib.check(_lib.load().chm_sampler_set_tuning(S._S, ns))
+        assert autotune.current(S) == ns
+        S.run(30, exchange_every=7, log_every=10 ** 9)
+        outs.append((S.move_log(30).copy(), S.species.copy(), S.chi2.copy(), S.counts.copy()))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+    rec = autotune.tune(S, candidates=(1, 2), iters=5, use_cache=False)
+    assert rec["nsplit"] in (1, 2) and set(rec["times_ms"]) == {"1", "2"}
+
+
 def test_workload_4000_consistency():
     """Full-size workload: incremental counts == recount, chi^2 from counts == chi^2 from
     the pair-space path sum + entry counts, composition conserved."""
