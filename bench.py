@@ -58,6 +58,26 @@ def cpu_baseline(P, seconds=15.0, procs=1):
                       "FP64 NumPy oracle of the reference algorithm, 1 thread"}
 
 
+def recorded_traffic(kernel, nsplit, workload):
+    """HBM-side bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy of known size), when it was
+    measured on this workload at this launch split; else None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_k_slot_step_{workload}.txt")
+    try:
+        lines = [ln.split() for ln in open(path) if ln.strip() and not ln.startswith("#")]
+    except OSError:
+        return None
+    scale = {ln[0]: float(ln[1]) for ln in lines if len(ln) == 2}
+    for ln in lines:
+        if len(ln) > 2 and ln[0] == "kernel" and ln[1] == kernel:
+            f = dict(zip(ln[2::2], ln[3::2]))
+            if int(f["nsplit"]) != nsplit:
+                return None
+            kib = float(f["fetch_kib"]) / scale["fetch_scale"] + float(f["write_kib"]) / scale["write_scale"]
+            return round(kib * 1024.0)
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,7 +90,22 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=200)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "f64"])
     ap.add_argument("--reanchor-every", type=int, default=512)
+    ap.add_argument("--nsplit", type=int, default=0,
+                    help="workgroups per replica in the slot kernel (0: autotune)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the known-size "
+                         "kernel that scales hardware byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (replicas shard across GPUs by "
               "design, but no multi-GPU path is built); use --gpus 1", file=sys.stderr)
@@ -84,7 +119,12 @@ def main():
     S.precision, S.reanchor_every = args.precision, args.reanchor_every
     S.init_state(P.cfg0, P.cell.ideal)
     from chimera_amd import autotune
-    tuned = autotune.tune(S, use_cache=False)
+    if args.nsplit:
+        from chimera_amd import _lib
+        _lib.check(_lib.load().chm_sampler_set_tuning(S._S, args.nsplit))
+        tuned = {"nsplit": args.nsplit, "times_ms": {}}
+    else:
+        tuned = autotune.tune(S, use_cache=False)
     print(f"[bench] tuned: {tuned}", file=sys.stderr, flush=True)
     print(f"[bench] built in {time.time() - t_build:.1f} s; warmup {args.warmup}",
           file=sys.stderr, flush=True)
@@ -103,7 +143,7 @@ def main():
     from chimera_amd import _lib
     ms = np.zeros(4)
     _lib.check(_lib.load().chm_sampler_profile(S._S, args.profile_iters, ms))
-    k_prop, k2, k_slot, kstep = ms
+    _, k2, k_slot, kstep = ms
     nrow, ncol_e, nrep, nsp = S.nrow_e, S.nkw * S.nwin * 2, S.nrep, P.nsp
     ncol_tot = nsp * ncol_e
     # algorithmic work of one launch of each kernel of the step
@@ -112,8 +152,7 @@ def main():
     nct = -(-ncol_e // 16)
     bytes_slot = nrep * (P.cell.natoms + 2 * ms_stencil * 16 + 8 * nsp * nct * 4
                          + 4 * nsp * nrow * 3 + 2 * 8 * ncol_tot)
-    kern = {"slot_step_ms": k_slot, "trial_transform_ms": k2, "step_ms": kstep,
-            "slot_propose_only_ms": k_prop}
+    kern = {"slot_step_ms": k_slot, "trial_transform_ms": k2, "step_ms": kstep}
     if args.precision == "bf16x3":
         peak2, kname = 2500.0 / 3.0, "k_gemm_delta_bf16"        # split-BF16: BF16 dense / 3
     else:
@@ -126,8 +165,10 @@ def main():
     }
     dom = "k_slot_step" if k_slot >= k2 else kname
     r = rooflines[dom]
+    from chimera_amd import autotune as _at
     roof = {"bound": r["bound"], "achieved": round(r["achieved"], 3), "peak": r["peak"],
-            "unit": r["unit"], "frac": round(r["achieved"] / r["peak"], 5), "traffic": None,
+            "unit": r["unit"], "frac": round(r["achieved"] / r["peak"], 5),
+            "traffic": recorded_traffic(dom, _at.current(S), args.workload),
             "kernel": dom,
             "all": {k: {"achieved": round(v["achieved"], 3), "unit": v["unit"],
                         "frac": round(v["achieved"] / v["peak"], 5)} for k, v in rooflines.items()}}
@@ -169,7 +210,7 @@ def main():
         "cpu_baseline": cpu,
         "gpu_over_cpu": None if cpu is None else round(moves / cpu["value"], 1),
         "best_chi2": round(float(bc2), 5),
-        "best_r_factor": round(rfac, 6),
+        "best_r_factor": float(f"{rfac:.4g}"),
         "alpha1_ess": round(ess_min, 1),
         "alpha1_ess_per_s": round(ess_min / dt, 2),
         "alpha1_mean": {f"{a}{b}": round(float(al[:, a, b].mean()), 5) for a, b in zip(*iu)},

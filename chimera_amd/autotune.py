@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _lib
 
-CANDIDATES = (1, 2, 4, 6, 8, 12, 16, 24)
+CANDIDATES = (1, 2, 4, 8, 12, 16, 24, 32, 48, 64)
 
 
 def _cache_path():

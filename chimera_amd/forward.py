@@ -13,6 +13,28 @@ import numpy as np
 from . import _lib
 from ._lib import check, f64, i32
 
+KB = 8.617333262e-5   # Boltzmann constant, eV/K (CODATA 2018)
+
+
+def thermal_sigma2(Phi, cell, i_idx, j_idx, shift, temperature=300.0):
+    """Per-pair mean-square relative displacement along the bond from a harmonic
+    force-constant matrix Phi (3N, 3N) (host helper, exafs_gpu/forward.py:31-47).
+
+    The displacement covariance is C = kB T Phi^+ (pseudo-inverse over the modes with
+    eigenvalue above 1e-6 of the largest, which drops the three translations);
+    sigma^2_ij = e^T (C_ii + C_jj - C_ij - C_ji) e with e the unit bond vector."""
+    Phi = np.asarray(Phi, float)
+    w, V = np.linalg.eigh(Phi)
+    keep = w > 1e-6 * max(1.0, float(w.max()))
+    Vk = V[:, keep]
+    i_idx, j_idx = np.asarray(i_idx), np.asarray(j_idx)
+    d = cell.ideal[j_idx] + np.asarray(shift, float) - cell.ideal[i_idx]
+    e = d / np.linalg.norm(d, axis=1)[:, None]
+    # project each bond direction onto the modes at both ends: u_p = e . (V_i - V_j)
+    Vr = Vk.reshape(-1, 3, Vk.shape[1])                    # (N, 3, modes)
+    u = np.einsum("pa,pam->pm", e, Vr[i_idx] - Vr[j_idx])  # (npairs, modes)
+    return (u * u * (KB * temperature / w[keep])[None, :]).sum(1)
+
 
 class ForwardModel:
     def __init__(self, cell, tables, i_idx, j_idx, shift, sigma2=None, shell=None, device=0):

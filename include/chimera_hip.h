@@ -219,10 +219,12 @@ int chm_sampler_set_tuning(chm_sampler *s, int nsplit);
 int chm_sampler_get_tuning(chm_sampler *s, int *nsplit);
 
 /* Average device time (ms) per launch of the step's kernels, from HIP events on the
- * sampler's stream over `iters` launches each: ms[0] propose+gather, ms[1] trial
- * transform GEMM, ms[2] accept, ms[3] one whole step.  The state stays consistent
- * (no move is committed while timing), but the step counter advances.  No counterpart
- * in the reference (it has no profiler hooks). */
+ * sampler's stream over `iters` launches each: ms[0] = 0 (kept for layout), ms[1] trial
+ * transform GEMM, ms[2] per-replica accept + propose + gather kernel (step minus
+ * transform), ms[3] one whole step.  The timed steps are real steps of the chain
+ * (iters + 1 moves per replica are made, without replica exchange or FP64 re-anchor);
+ * the state stays consistent.  No counterpart in the reference (it has no profiler
+ * hooks). */
 int chm_sampler_profile(chm_sampler *s, int iters, double *ms);
 
 #ifdef __cplusplus

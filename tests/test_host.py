@@ -188,3 +188,25 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     assert declared == set(_lib.EXPORTS)
     assert _lib.load().chm_version() >= 1
+
+
+def test_thermal_sigma2_springs_in_series():
+    """Harmonic chain 0-1-2 along x with springs k1, k2 and free ends: each bond's
+    extension is an independent mode, so <du^2> = kB T / k for a bond and the sum of
+    both for the 0-2 pair; motion across the bond contributes nothing."""
+    from types import SimpleNamespace
+    from chimera_amd.forward import KB, thermal_sigma2
+    k1, k2, T = 2.0, 5.0, 300.0
+    Phi = np.zeros((9, 9))
+    for (a, b), k in (((0, 1), k1), ((1, 2), k2)):
+        for p, q, sgn in ((a, a, 1), (b, b, 1), (a, b, -1), (b, a, -1)):
+            Phi[3 * p, 3 * q] += sgn * k
+    cell = SimpleNamespace(ideal=np.array([[0.0, 0, 0], [2.5, 0, 0], [5.0, 0, 0]]))
+    i = np.array([0, 1, 0, 1])
+    j = np.array([1, 2, 2, 0])
+    s2 = thermal_sigma2(Phi, cell, i, j, np.zeros((4, 3)), temperature=T)
+    kt = KB * T
+    np.testing.assert_allclose(s2, [kt / k1, kt / k2, kt / k1 + kt / k2, kt / k1], rtol=1e-10)
+    # a bond direction perpendicular to the springs sees no relative motion
+    cell2 = SimpleNamespace(ideal=np.array([[0.0, 0, 0], [0, 2.5, 0], [0, 5.0, 0]]))
+    assert np.allclose(thermal_sigma2(Phi, cell2, i[:1], j[:1], np.zeros((1, 3)), T), 0.0)
