@@ -29,80 +29,12 @@
 #include <cstring>
 
 #include "common.h"
+#include "sampler_dev.h"
+#include "sampler_state.h"
 
 namespace {
 
-constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
-constexpr uint64_t kAccSalt = 0xD1B54A32D192ED03ull;
-constexpr uint64_t kExcSalt = 0x94D049BB133111EBull;
-
-__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
-    x += kGolden;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-}
-__device__ __forceinline__ double u01(uint64_t h) {
-    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
-}
-
-struct Geo {
-    int natoms, natoms_pad, nsp, ncoord, nshell, nblk, nstencil, nabsrec;
-    int nrow_e, rstride, nrow_tot, nss_rows, ncol_e, ncol_tot, nkw, nwin;  // rstride: padded rows per edge
-    int box[3], ncell[3], nbasis;
-    int bpar[8];
-    int has_const;
-    int nct, nkb;   // 16-column tiles per edge, 32-row K blocks per edge
-    int bxyz[12];   // coordinates (units a0/2) of each basis site in the cell
-};
-
-__device__ __forceinline__ int site_index(const Geo &g, int x, int y, int z) {
-    int b = g.bpar[(x & 1) | ((y & 1) << 1) | ((z & 1) << 2)];
-    int cell = ((x >> 1) * g.ncell[1] + (y >> 1)) * g.ncell[2] + (z >> 1);
-    return cell * g.nbasis + b;
-}
-
-__device__ __forceinline__ int wrap(int v, int n) {
-    v = (v < 0) ? v + n : v;
-    return (v >= n) ? v - n : v;
-}
-
-struct Prop {
-    int a, b, na, nb, sw;
-};
-
-// kernels.py:378-398: h = splitmix64(seed ^ step*golden ^ key<<32), then swap flag,
-// a, and up to 8 candidates.  swap_frac = 1: the flag is always set.  All 8 candidates
-// repeating a's species makes a null move.
-__device__ Prop propose(const uint8_t *sp, int natoms, uint64_t seed, uint64_t stepc,
-                        uint64_t key) {
-    uint64_t h = splitmix64(seed ^ (stepc * kGolden) ^ (key << 32));
-    h = splitmix64(h);  // swap flag draw, u < 1 always
-    h = splitmix64(h);
-    int a = (int)(u01(h) * natoms);
-    if (a >= natoms) a = natoms - 1;
-    int sa = sp[a];
-    Prop p{a, a, sa, sa, 0};
-    for (int t = 0; t < 8; ++t) {
-        h = splitmix64(h);
-        int c = (int)(u01(h) * natoms);
-        if (c >= natoms) c = natoms - 1;
-        if (sp[c] != sa) {
-            p.b = c;
-            p.na = sp[c];
-            p.nb = sa;
-            p.sw = 1;
-            break;
-        }
-    }
-    return p;
-}
-
-__device__ __forceinline__ void load_species(uint8_t *lsp, const uint8_t *gsp, int natoms_pad) {
-    const uint4 *src = reinterpret_cast<const uint4 *>(gsp);
-    uint4 *dst = reinterpret_cast<uint4 *>(lsp);
-    for (int q = threadIdx.x; q < natoms_pad / 16; q += blockDim.x) dst[q] = src[q];
-}
+using namespace chm_smp;
 
 // ---------------------------------------------------------------- full counts
 // grid (nsplit, nrep).  counts must be zero on entry (the const rows are set here).
@@ -288,9 +220,6 @@ __global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int
         O.stepc[r0 + threadIdx.x] += 1;
 }
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
 __device__ __forceinline__ __bf16 bf16_of_small_int(int v) {
     // |v| <= 128: exactly representable; the float's top 16 bits are the bf16
     float f = (float)v;
@@ -431,15 +360,6 @@ __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const 
 
 // basis_R (nsp, nrow_e, ncol_e) FP64 -> three bf16 planes in MFMA B-operand order:
 // Tb[e][ct][kb][plane][lane] = 8 bf16 of column ct*16 + lane%16, rows kb*32 + 8(lane/16) + i.
-__device__ __forceinline__ unsigned short bf16_rne(float f) {
-    unsigned int u = __builtin_bit_cast(unsigned int, f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_float(unsigned short h) {
-    return __builtin_bit_cast(float, (unsigned int)h << 16);
-}
-
 __global__ void k_split_bf16(Geo g, const double *__restrict__ basisR, unsigned short *__restrict__ Tb) {
     long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = (long)g.nsp * g.nct * g.nkb * 64 * 8;
@@ -484,8 +404,8 @@ struct SlotArgs {
     const double *basisR;   // FP64, for re-anchoring
     double *mcur;           // (nrep, ncol_tot) running accumulator (incremental mode)
     const double *beta_slot;// beta of each slot's current rung (kept by the exchange)
-    const int *prop_acc;    // (nrep, 5) a, b, new a, new b, is-swap of the step accepted
-    int *prop_next;         // (nrep, 5) the proposal made here (the other parity buffer)
+    const int *prop_acc;    // (nrep, kPropInts) a, b, new a, new b, is-swap, log q ratio
+    int *prop_next;         // (nrep, kPropInts) the proposal made here (other parity buffer)
     const double *c2_prev;  // chi^2 after the previous step (read by every workgroup)
     double *c2_next;        // chi^2 after this step (written by the lead workgroup)
     int *counts;            // (nrep, nrow_tot)
@@ -508,6 +428,11 @@ struct SlotArgs {
     int *series;            // (series_cap, nsp*nsp*nshell) or null
     int series_cap, series_rung;
     long long rec_base;     // record index = step - 1 - rec_base
+    uint8_t *pend_occ;      // (nrep, noccp) occupancy of each slot's pending proposal, or null
+    uint8_t *rec_occ;       // (rec_cap, noccp) move records (training data), or null
+    float *rec_y;           //   energy change nfit * (chi2' - chi2) of the proposal
+    int *rec_meta;          //   rung | accepted << 16, or -1 for a null move
+    int rec_cap, nocc, noccp;
     uint64_t seed;
     int profile;
     int accept;             // 0: none, 1: step, 2: init (store chi^2, no decision)
@@ -547,88 +472,6 @@ __device__ double chi2_of_sums(const Geo &g, const double (*S)[4], const double 
     return tot / nfit;
 }
 
-// integer coordinates (units a0/2) of a site, from its index
-__device__ __forceinline__ void site_xyz_of(const Geo &g, int q, int &x, int &y, int &z) {
-    const int cell = q / g.nbasis, b = q - cell * g.nbasis;
-    const int cz = cell % g.ncell[2];
-    const int cy = (cell / g.ncell[2]) % g.ncell[1];
-    const int cx = cell / (g.ncell[2] * g.ncell[1]);
-    x = 2 * cx + g.bxyz[3 * b];
-    y = 2 * cy + g.bxyz[3 * b + 1];
-    z = 2 * cz + g.bxyz[3 * b + 2];
-}
-
-struct SwapSite {
-    Prop p;
-    int ax, ay, az, bx, by, bz;
-};
-
-// One gather item of a swap: item < nss is a neighbour of a (first ncoord) or of b; the
-// rest are path records of a's then b's incidence stencil.  Count changes go to `hist`.
-__device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W,
-                                            const int *__restrict__ ss_sten, const uint8_t *lsp,
-                                            int *hist, int it, int nss, const int4 rec) {
-    const Prop &p = W.p;
-    if (it < nss) {
-        // ordered pairs x->y and y->x for every neighbour y of x in {a, b}
-        const bool on_a = it < g.ncoord;
-        const int sv = ss_sten[on_a ? it : it - g.ncoord];
-        const int dx = (int)(int8_t)(sv & 0xFF), dy = (int)(int8_t)((sv >> 8) & 0xFF);
-        const int dz = (int)(int8_t)((sv >> 16) & 0xFF), sh = (sv >> 24) & 0xFF;
-        const int x = on_a ? p.a : p.b;
-        const int y = site_index(g, wrap((on_a ? W.ax : W.bx) + dx, g.box[0]),
-                                 wrap((on_a ? W.ay : W.by) + dy, g.box[1]),
-                                 wrap((on_a ? W.az : W.bz) + dz, g.box[2]));
-        if (x == p.b && y == p.a) return;  // covered from a's side
-        const int sx = lsp[x], sy = lsp[y];
-        const int nx = (x == p.a) ? p.na : p.nb;
-        const int ny = (y == p.a) ? p.na : ((y == p.b) ? p.nb : sy);
-        atomicAdd(&hist[sx * g.rstride + sy * g.nshell + sh], -1);
-        atomicAdd(&hist[nx * g.rstride + ny * g.nshell + sh], 1);
-        atomicAdd(&hist[sy * g.rstride + sx * g.nshell + sh], -1);
-        atomicAdd(&hist[ny * g.rstride + nx * g.nshell + sh], 1);
-        return;
-    }
-    const int m = it - nss;
-    const bool on_a = m < g.nstencil;
-    const int8_t *rb = reinterpret_cast<const int8_t *>(&rec);
-    const int nslot = rb[12], wgt = rb[13];
-    const int cls_off = (int)(uint8_t)rb[14] | ((int)rb[15] << 8);
-    const int x0 = on_a ? W.ax : W.bx, y0 = on_a ? W.ay : W.by, z0 = on_a ? W.az : W.bz;
-    int sites[4];
-    bool has_a = false;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (q > nslot) {
-            sites[q] = -1;
-            continue;
-        }
-        sites[q] = site_index(g, wrap(x0 + rb[3 * q], g.box[0]), wrap(y0 + rb[3 * q + 1], g.box[1]),
-                              wrap(z0 + rb[3 * q + 2], g.box[2]));
-        has_a |= (sites[q] == p.a);
-    }
-    if (!on_a && has_a) return;  // a path through both sites counts once
-    int eo = 0, en = 0, pw = 1;
-#pragma unroll
-    for (int q = 1; q < 4; ++q) {
-        if (q > nslot) break;
-        const int st = sites[q];
-        const int so = lsp[st];
-        const int sn = (st == p.a) ? p.na : ((st == p.b) ? p.nb : so);
-        eo += so * pw;
-        en += sn * pw;
-        pw *= g.nsp;
-    }
-    const int sa0 = lsp[sites[0]];
-    const int sa1 = (sites[0] == p.a) ? p.na : ((sites[0] == p.b) ? p.nb : sa0);
-    eo += sa0 * g.rstride + g.nss_rows + cls_off;
-    en += sa1 * g.rstride + g.nss_rows + cls_off;
-    if (eo != en) {
-        atomicAdd(&hist[eo], -wgt);
-        atomicAdd(&hist[en], wgt);
-    }
-}
-
 __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     extern __shared__ int lds[];
     int *hist = lds;
@@ -636,6 +479,7 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     __shared__ double sums[chm::kMaxNsp][4];
     __shared__ double red[256];
     __shared__ int flag;
+    __shared__ double dE_sh;
     __shared__ Prop P;
     const int s = blockIdx.y;
     const int nwg = gridDim.x;
@@ -650,9 +494,11 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     const double c2p = A.c2_prev ? A.c2_prev[s] : 0.0;
     const double best = A.best_c2[s];
     const int anchor = A.anchor[s];
-    int pp[5] = {0, 0, 0, 0, 0};
-    if (A.accept == 1)
-        for (int q = 0; q < 5; ++q) pp[q] = A.prop_acc[5 * s + q];
+    int pp[kPropInts] = {0, 0, 0, 0, 0, 0};
+    if (A.accept == 1) {
+#pragma unroll
+        for (int q = 0; q < kPropInts; ++q) pp[q] = A.prop_acc[kPropInts * s + q];
+    }
     load_species(lsp, A.species + base_sp, g.natoms_pad);
     // per-edge sums over the column tiles: (e, j) groups of CT lanes, then a tree
     const int G = 4 * g.nsp;
@@ -680,19 +526,11 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     }
     if (A.propose)
         for (int q = t; q < g.nrow_tot; q += blockDim.x) hist[q] = 0;
-    // prefetch this lane's multiple-scattering records (independent of the proposal)
-    constexpr int RMAX = 8;
-    int4 recs[RMAX];
-    const int nss_items = 2 * g.ncoord;
+    // this lane's first multiple-scattering records (independent of the proposal)
     const int stride = nwg * blockDim.x;
     const int first = blockIdx.x * blockDim.x + t;
-#pragma unroll
-    for (int r = 0; r < RMAX; ++r) {
-        recs[r] = make_int4(0, 0, 0, 0);
-        const int m = first + r * stride - nss_items;
-        if (A.propose && m >= 0 && m < 2 * g.nstencil)
-            recs[r] = A.ms_sten[m < g.nstencil ? m : m - g.nstencil];
-    }
+    int4 recs[kRecPrefetch];
+    prefetch_recs(g, A.ms_sten, first, stride, A.propose, recs);
     __syncthreads();
     int *cn = A.counts + (long)s * g.nrow_tot;
     if (A.accept) {
@@ -715,8 +553,10 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                     A.best_c2[s] = c2t;
                 }
             } else {
+                dE_sh = (c2t - c2p) * A.nfit;
                 if (pp[4]) {
-                    const double x = beta * (c2t - c2p) * A.nfit;
+                    // Metropolis-Hastings: pp[5] holds log q(x'->x) - log q(x->x')
+                    const double x = beta * (c2t - c2p) * A.nfit - (double)__int_as_float(pp[5]);
                     uint64_t h = splitmix64(A.seed ^ (stepc * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
                     const double u = u01(splitmix64(h));
                     acc = (x <= 0.0) || (u < exp(-fmin(x, 60.0)));
@@ -757,6 +597,17 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
         }
         __syncthreads();
         const int acc = flag & 1;
+        if (lead && A.rec_y && A.accept == 1) {
+            // the move record of this step: occupancy of the proposal, energy change, outcome
+            const long idx = (long)(((stepc - 1) * (uint64_t)gridDim.y + (uint64_t)s) % (uint64_t)A.rec_cap);
+            if (t == 0) {
+                A.rec_y[idx] = pp[4] ? (float)dE_sh : 0.f;
+                A.rec_meta[idx] = pp[4] ? (rung | (acc << 16)) : -1;
+            }
+            if (pp[4])
+                for (int q = t; q < A.noccp; q += blockDim.x)
+                    A.rec_occ[idx * A.noccp + q] = A.pend_occ[(long)s * A.noccp + q];
+        }
         if (A.incremental && acc) {
             double *mc = A.mcur + (long)s * g.ncol_tot;
 #pragma unroll
@@ -794,38 +645,26 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     if (t == 0) {
         P = propose(lsp, g.natoms, A.seed, stepc + 1, (uint64_t)rung);
         if (lead) {
-            int *o = A.prop_next + 5 * s;
+            int *o = A.prop_next + kPropInts * s;
             o[0] = P.a;
             o[1] = P.b;
             o[2] = P.na;
             o[3] = P.nb;
             o[4] = P.sw;
+            o[5] = 0;   // symmetric proposal: log q ratio 0
         }
     }
     __syncthreads();
     const Prop p = P;
-    if (p.sw) {
-        SwapSite W;
-        W.p = p;
-        site_xyz_of(g, p.a, W.ax, W.ay, W.az);
-        site_xyz_of(g, p.b, W.bx, W.by, W.bz);
-        const int nitems = nss_items + 2 * g.nstencil;
-#pragma unroll
-        for (int r = 0; r < RMAX; ++r) {
-            const int it = first + r * stride;
-            if (it < nitems) gather_item(g, W, A.ss_sten, lsp, hist, it, nss_items, recs[r]);
-        }
-        for (int it = first + RMAX * stride; it < nitems; it += stride) {
-            const int m = it - nss_items;
-            const int4 rec = (m >= 0) ? A.ms_sten[m < g.nstencil ? m : m - g.nstencil] : make_int4(0, 0, 0, 0);
-            gather_item(g, W, A.ss_sten, lsp, hist, it, nss_items, rec);
-        }
-        __syncthreads();
-        // this workgroup's share of the count changes into the (zeroed) next-step row
-        int *dcp = A.dc_prop + (long)s * g.nrow_tot;
-        for (int q = t; q < g.nrow_tot; q += blockDim.x)
-            if (hist[q] != 0) atomicAdd(&dcp[q], hist[q]);
+    if (lead && A.pend_occ && p.sw) {
+        int ax, ay, az, bx, by, bz;
+        site_xyz_of(g, p.a, ax, ay, az);
+        site_xyz_of(g, p.b, bx, by, bz);
+        for (int q = t; q < A.nocc; q += blockDim.x)
+            A.pend_occ[(long)s * A.noccp + q] =
+                lsp[occ_site(g, A.ss_sten, p.a, p.b, ax, ay, az, bx, by, bz, q)];
     }
+    if (p.sw) gather_swap(g, p, A.ss_sten, A.ms_sten, lsp, hist, first, stride, recs, A.dc_prop);
 }
 
 // chi^2 of arbitrary window transforms (one row per workgroup), direct form:
@@ -898,27 +737,6 @@ __global__ void k_counts_to_f64(const int *__restrict__ counts, double *__restri
 
 }  // namespace
 
-struct chm_sampler {
-    chm_ctx *ctx = nullptr;
-    Geo g{};
-    int nrep = 0, nk = 0, nsplit = 1, profile = 1;
-    int incremental = 0, reanchor_every = 512;
-    uint64_t seed = 0;
-    bool inited = false;
-    long long hstep = 0;
-    double nfit = 1.0;
-    size_t lds_k1 = 0;
-    chm::DevBuf site_xyz, nbr_j, nbr_sh, ss_sten, stencil, basisR, basisK, target, wcol, q0, Tb;
-    chm::DevBuf species, counts, dcounts, c2, best_c2, best_sp, rung_of, slot_of, beta, beta_slot,
-        stepc, iex,
-        prop, mtrial, mcur, partials, anchor, nanchor, exch_att, exch_acc, acc_rung, log, series,
-        tmp;
-    int log_cap = 0, series_cap = 0, series_rung = -1;
-    long long rec_base = 0;   // steps completed when recording (re)started
-    hipGraph_t graph[2] = {nullptr, nullptr};     // exchange period starting at an even /
-    hipGraphExec_t gexec[2] = {nullptr, nullptr}; // odd completed-step count
-    int graph_steps = 0;
-};
 
 namespace {
 
@@ -962,6 +780,14 @@ SlotArgs slot_args(chm_sampler *S, int accept, int propose) {
     A.series_cap = S->series_cap;
     A.series_rung = S->series_rung;
     A.rec_base = S->rec_base;
+    const chm_lp &L = S->lp;
+    A.nocc = L.nocc;
+    A.noccp = L.noccp;
+    A.pend_occ = (L.rec_cap || L.mode) ? L.pend_occ.as<uint8_t>() : nullptr;
+    A.rec_cap = L.rec_cap;
+    A.rec_occ = L.rec_cap ? L.rec_occ.as<uint8_t>() : nullptr;
+    A.rec_y = L.rec_cap ? L.rec_y.as<float>() : nullptr;
+    A.rec_meta = L.rec_cap ? L.rec_meta.as<int>() : nullptr;
     A.seed = S->seed;
     A.profile = S->profile;
     A.accept = accept;
@@ -975,14 +801,6 @@ size_t lds_gemm(const Geo &g) {
     size_t p = sizeof(double) * (size_t)KS * RB * NCW;
     return std::max(a, p);
 }
-
-// per-step double buffers, indexed by step parity: the count-change rows and the
-// proposal records of step n, and chi^2 after step n
-int *drow(chm_sampler *S, long long n) {
-    return S->dcounts.as<int>() + (size_t)(n & 1) * S->nrep * S->g.nrow_tot;
-}
-int *prow(chm_sampler *S, long long n) { return S->prop.as<int>() + (size_t)(n & 1) * S->nrep * 5; }
-double *c2row(chm_sampler *S, long long n) { return S->c2.as<double>() + (size_t)(n & 1) * S->nrep; }
 
 int launch_gemm_abs(chm_sampler *S, const int *dc, int bump) {
     const Geo &g = S->g;
@@ -1021,6 +839,24 @@ int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanch
     return CHM_OK;
 }
 
+// propose step h + 1 after h completed steps (nothing is accepted)
+int launch_propose(chm_sampler *S, long long h) {
+    if (S->lp.mode) {
+        int rc = lp_launch_score(S);
+        return rc ? rc : lp_launch_select(S, h);
+    }
+    return launch_slot(S, 0, 1, h);
+}
+
+// accept step n, then (propose) propose step n + 1
+int launch_accept(chm_sampler *S, long long n, int propose, int reanchor_step = 0) {
+    if (S->lp.mode) {
+        int rc = launch_slot(S, 1, 0, n, reanchor_step);
+        return (rc || !propose) ? rc : launch_propose(S, n);
+    }
+    return launch_slot(S, 1, propose, n, reanchor_step);
+}
+
 int launch_reanchor(chm_sampler *S) {
     const Geo &g = S->g;
     dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
@@ -1041,7 +877,7 @@ int launch_gemm(chm_sampler *S, const int *dc, int bump) {
 // L steps after h completed ones, no exchange inside: propose step h+1, then for every
 // step n: transform (advances the device step counter), accept [+ propose n+1].
 int launch_segment(chm_sampler *S, long long h, int L) {
-    int rc = launch_slot(S, 0, 1, h);
+    int rc = launch_propose(S, h);
     // incremental mode: replicas with reanchor_every accepted moves since their last FP64
     // recompute get the running chi(R) recomputed from the integer counts, here
     const int anchor = S->incremental && S->reanchor_every > 0;
@@ -1049,7 +885,7 @@ int launch_segment(chm_sampler *S, long long h, int L) {
     for (int t = 0; t < L && !rc; ++t) {
         const long long n = h + 1 + t;
         rc = launch_gemm(S, drow(S, n), 1);
-        if (!rc) rc = launch_slot(S, 1, t + 1 < L ? 1 : 0, n, anchor && t == 0);
+        if (!rc) rc = launch_accept(S, n, t + 1 < L ? 1 : 0, anchor && t == 0);
     }
     return rc;
 }
@@ -1064,6 +900,10 @@ int launch_exchange(chm_sampler *S, long long n) {
     return CHM_OK;
 }
 
+
+}  // namespace
+
+namespace chm_smp {
 void drop_graph(chm_sampler *S) {
     for (int q = 0; q < 2; ++q) {
         if (S->gexec[q]) hipGraphExecDestroy(S->gexec[q]);
@@ -1073,8 +913,7 @@ void drop_graph(chm_sampler *S) {
     }
     S->graph_steps = 0;
 }
-
-}  // namespace
+}  // namespace chm_smp
 
 extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, int nk,
                                   const int32_t *site_xyz, const int32_t *nbr_j,
@@ -1152,6 +991,7 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
     }
     // the neighbour offsets of site 0 form a stencil that must reproduce every site's list
     std::vector<int> ss(g.ncoord);
+    int r2max = 0;
     {
         auto xyz_of = [&](int q, int &x, int &y, int &z) {
             int v = site_xyz[q];
@@ -1181,6 +1021,9 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
             ss[o] = (off[3 * o] & 0xFF) | ((off[3 * o + 1] & 0xFF) << 8) |
                     ((off[3 * o + 2] & 0xFF) << 16) | ((shl[o] & 0xFF) << 24);
         }
+        for (int o = 0; o < g.ncoord; ++o)
+            r2max = std::max(r2max, off[3 * o] * off[3 * o] + off[3 * o + 1] * off[3 * o + 1] +
+                                        off[3 * o + 2] * off[3 * o + 2]);
         std::vector<long long> want(g.ncoord), got(g.ncoord);
         for (int q = 0; q < g.natoms; ++q) {
             int x, y, z;
@@ -1282,7 +1125,7 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         (rc = S->rung_of.ensure(sizeof(int) * nr)) || (rc = S->slot_of.ensure(sizeof(int) * nr)) ||
         (rc = S->beta.ensure(sizeof(double) * nr)) || (rc = S->beta_slot.ensure(sizeof(double) * nr)) ||
         (rc = S->stepc.ensure(sizeof(uint64_t) * nr)) ||
-        (rc = S->iex.ensure(sizeof(uint64_t))) || (rc = S->prop.ensure(sizeof(int) * 2 * 5 * nr)) ||
+        (rc = S->iex.ensure(sizeof(uint64_t))) || (rc = S->prop.ensure(sizeof(int) * 2 * kPropInts * nr)) ||
         (rc = S->mtrial.ensure(sizeof(double) * nr * g.ncol_tot)) ||
         (rc = S->mcur.ensure(sizeof(double) * nr * g.ncol_tot)) ||
         (rc = S->partials.ensure(sizeof(double) * nr * g.nsp * g.nct * 4)) ||
@@ -1300,6 +1143,8 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
                         (int)S->lds_k1);
     if (lds_gemm(g) > 160 * 1024 || S->lds_k1 > 160 * 1024)
         return fail((chm::set_error("problem too large for LDS staging"), CHM_E_ARG));
+    S->lp.r2max = r2max;
+    if ((rc = lp_prepare(S))) return fail(rc);
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         chm::set_error(std::string("sampler upload: ") + hipGetErrorString(e));
@@ -1568,6 +1413,14 @@ extern "C" int chm_sampler_read(chm_sampler *S, int what, void *host, int64_t nb
             CHM_HIP(hipMemcpy(host, S->nanchor.p, 8 * nrep, hipMemcpyDeviceToHost));
             return CHM_OK;
         }
+        case 15: {
+            const int64_t n = (int64_t)4 * nrep * S->lp.ncand;
+            if ((rc = need(n))) return rc;
+            std::vector<float> v((size_t)nrep * S->lp.ncand);
+            CHM_HIP(hipMemcpy(v.data(), S->lp.logit.p, n, hipMemcpyDeviceToHost));
+            std::memcpy(host, v.data(), n);
+            return CHM_OK;
+        }
     }
     chm::set_error("unknown read selector");
     return CHM_E_ARG;
@@ -1672,7 +1525,7 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
     CHM_HIP(hipEventCreate(&e1));
     float t = 0.f;
     long long h = S->hstep;
-    int rc = launch_slot(S, 0, 1, h);        // step h+1 pending
+    int rc = launch_propose(S, h);           // step h+1 pending
     // the trial transform of the pending step, repeated (no counter advance)
     CHM_HIP(hipEventRecord(e0, st));
     for (int i = 0; i < iters && !rc; ++i) rc = launch_gemm(S, drow(S, h + 1), 0);
@@ -1686,7 +1539,7 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
     for (int i = 0; i < iters && !rc; ++i) {
         const long long n = h + 1 + i;
         rc = launch_gemm(S, drow(S, n), 1);
-        if (!rc) rc = launch_slot(S, 1, 1, n);
+        if (!rc) rc = launch_accept(S, n, 1);
     }
     CHM_HIP(hipEventRecord(e1, st));
     if (rc) return rc;
@@ -1697,7 +1550,7 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
     ms[0] = 0.0;
     const long long n = h + 1 + iters;       // settle: transform and accept the last proposal
     if (!rc) rc = launch_gemm(S, drow(S, n), 1);
-    if (!rc) rc = launch_slot(S, 1, 0, n);
+    if (!rc) rc = launch_accept(S, n, 0);
     if (rc) return rc;
     S->hstep = n;
     CHM_HIP(hipStreamSynchronize(st));

@@ -1,0 +1,245 @@
+// Device-side pieces shared by the sampler kernels (sampler.hip) and the learned
+// proposal kernels (learned.hip): the counter-based RNG, the lattice geometry and site
+// arithmetic, the uniform swap proposal and the gather of a swap's count changes.
+#pragma once
+
+#include <cstdint>
+
+#include "common.h"
+
+namespace chm_smp {
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+constexpr uint64_t kAccSalt = 0xD1B54A32D192ED03ull;
+constexpr uint64_t kExcSalt = 0x94D049BB133111EBull;
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += kGolden;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t h) {
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+    unsigned int u = __builtin_bit_cast(unsigned int, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_float(unsigned short h) {
+    return __builtin_bit_cast(float, (unsigned int)h << 16);
+}
+
+struct Geo {
+    int natoms, natoms_pad, nsp, ncoord, nshell, nblk, nstencil, nabsrec;
+    int nrow_e, rstride, nrow_tot, nss_rows, ncol_e, ncol_tot, nkw, nwin;  // rstride: padded rows per edge
+    int box[3], ncell[3], nbasis;
+    int bpar[8];
+    int has_const;
+    int nct, nkb;   // 16-column tiles per edge, 32-row K blocks per edge
+    int bxyz[12];   // coordinates (units a0/2) of each basis site in the cell
+};
+
+__device__ __forceinline__ int site_index(const Geo &g, int x, int y, int z) {
+    int b = g.bpar[(x & 1) | ((y & 1) << 1) | ((z & 1) << 2)];
+    int cell = ((x >> 1) * g.ncell[1] + (y >> 1)) * g.ncell[2] + (z >> 1);
+    return cell * g.nbasis + b;
+}
+
+__device__ __forceinline__ int wrap(int v, int n) {
+    v = (v < 0) ? v + n : v;
+    return (v >= n) ? v - n : v;
+}
+
+struct Prop {
+    int a, b, na, nb, sw;
+};
+
+// kernels.py:378-398: h = splitmix64(seed ^ step*golden ^ key<<32), then swap flag,
+// a, and up to 8 candidates.  swap_frac = 1: the flag is always set.  All 8 candidates
+// repeating a's species makes a null move.
+__device__ inline Prop propose(const uint8_t *sp, int natoms, uint64_t seed, uint64_t stepc,
+                        uint64_t key) {
+    uint64_t h = splitmix64(seed ^ (stepc * kGolden) ^ (key << 32));
+    h = splitmix64(h);  // swap flag draw, u < 1 always
+    h = splitmix64(h);
+    int a = (int)(u01(h) * natoms);
+    if (a >= natoms) a = natoms - 1;
+    int sa = sp[a];
+    Prop p{a, a, sa, sa, 0};
+    for (int t = 0; t < 8; ++t) {
+        h = splitmix64(h);
+        int c = (int)(u01(h) * natoms);
+        if (c >= natoms) c = natoms - 1;
+        if (sp[c] != sa) {
+            p.b = c;
+            p.na = sp[c];
+            p.nb = sa;
+            p.sw = 1;
+            break;
+        }
+    }
+    return p;
+}
+
+__device__ __forceinline__ void load_species(uint8_t *lsp, const uint8_t *gsp, int natoms_pad) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(gsp);
+    uint4 *dst = reinterpret_cast<uint4 *>(lsp);
+    for (int q = threadIdx.x; q < natoms_pad / 16; q += blockDim.x) dst[q] = src[q];
+}
+
+// integer coordinates (units a0/2) of a site, from its index
+__device__ __forceinline__ void site_xyz_of(const Geo &g, int q, int &x, int &y, int &z) {
+    const int cell = q / g.nbasis, b = q - cell * g.nbasis;
+    const int cz = cell % g.ncell[2];
+    const int cy = (cell / g.ncell[2]) % g.ncell[1];
+    const int cx = cell / (g.ncell[2] * g.ncell[1]);
+    x = 2 * cx + g.bxyz[3 * b];
+    y = 2 * cy + g.bxyz[3 * b + 1];
+    z = 2 * cz + g.bxyz[3 * b + 2];
+}
+
+struct SwapSite {
+    Prop p;
+    int ax, ay, az, bx, by, bz;
+};
+
+// One gather item of a swap: item < nss is a neighbour of a (first ncoord) or of b; the
+// rest are path records of a's then b's incidence stencil.  Count changes go to `hist`.
+__device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W,
+                                            const int *__restrict__ ss_sten, const uint8_t *lsp,
+                                            int *hist, int it, int nss, const int4 rec) {
+    const Prop &p = W.p;
+    if (it < nss) {
+        // ordered pairs x->y and y->x for every neighbour y of x in {a, b}
+        const bool on_a = it < g.ncoord;
+        const int sv = ss_sten[on_a ? it : it - g.ncoord];
+        const int dx = (int)(int8_t)(sv & 0xFF), dy = (int)(int8_t)((sv >> 8) & 0xFF);
+        const int dz = (int)(int8_t)((sv >> 16) & 0xFF), sh = (sv >> 24) & 0xFF;
+        const int x = on_a ? p.a : p.b;
+        const int y = site_index(g, wrap((on_a ? W.ax : W.bx) + dx, g.box[0]),
+                                 wrap((on_a ? W.ay : W.by) + dy, g.box[1]),
+                                 wrap((on_a ? W.az : W.bz) + dz, g.box[2]));
+        if (x == p.b && y == p.a) return;  // covered from a's side
+        const int sx = lsp[x], sy = lsp[y];
+        const int nx = (x == p.a) ? p.na : p.nb;
+        const int ny = (y == p.a) ? p.na : ((y == p.b) ? p.nb : sy);
+        atomicAdd(&hist[sx * g.rstride + sy * g.nshell + sh], -1);
+        atomicAdd(&hist[nx * g.rstride + ny * g.nshell + sh], 1);
+        atomicAdd(&hist[sy * g.rstride + sx * g.nshell + sh], -1);
+        atomicAdd(&hist[ny * g.rstride + nx * g.nshell + sh], 1);
+        return;
+    }
+    const int m = it - nss;
+    const bool on_a = m < g.nstencil;
+    const int8_t *rb = reinterpret_cast<const int8_t *>(&rec);
+    const int nslot = rb[12], wgt = rb[13];
+    const int cls_off = (int)(uint8_t)rb[14] | ((int)rb[15] << 8);
+    const int x0 = on_a ? W.ax : W.bx, y0 = on_a ? W.ay : W.by, z0 = on_a ? W.az : W.bz;
+    int sites[4];
+    bool has_a = false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q > nslot) {
+            sites[q] = -1;
+            continue;
+        }
+        sites[q] = site_index(g, wrap(x0 + rb[3 * q], g.box[0]), wrap(y0 + rb[3 * q + 1], g.box[1]),
+                              wrap(z0 + rb[3 * q + 2], g.box[2]));
+        has_a |= (sites[q] == p.a);
+    }
+    if (!on_a && has_a) return;  // a path through both sites counts once
+    int eo = 0, en = 0, pw = 1;
+#pragma unroll
+    for (int q = 1; q < 4; ++q) {
+        if (q > nslot) break;
+        const int st = sites[q];
+        const int so = lsp[st];
+        const int sn = (st == p.a) ? p.na : ((st == p.b) ? p.nb : so);
+        eo += so * pw;
+        en += sn * pw;
+        pw *= g.nsp;
+    }
+    const int sa0 = lsp[sites[0]];
+    const int sa1 = (sites[0] == p.a) ? p.na : ((sites[0] == p.b) ? p.nb : sa0);
+    eo += sa0 * g.rstride + g.nss_rows + cls_off;
+    en += sa1 * g.rstride + g.nss_rows + cls_off;
+    if (eo != en) {
+        atomicAdd(&hist[eo], -wgt);
+        atomicAdd(&hist[en], wgt);
+    }
+}
+
+
+constexpr int kRecPrefetch = 8;   // multiple-scattering records per lane loaded ahead
+
+// Gather items of a swap split over the workgroups of one replica slot: item `first +
+// r * stride`.  The first kRecPrefetch records of this lane do not depend on the
+// proposal, so they can be requested before it is known.
+__device__ __forceinline__ void prefetch_recs(const Geo &g, const int4 *__restrict__ ms_sten, int first,
+                                              int stride, bool on, int4 (&recs)[kRecPrefetch]) {
+    const int nss_items = 2 * g.ncoord;
+#pragma unroll
+    for (int r = 0; r < kRecPrefetch; ++r) {
+        recs[r] = make_int4(0, 0, 0, 0);
+        const int m = first + r * stride - nss_items;
+        if (on && m >= 0 && m < 2 * g.nstencil) recs[r] = ms_sten[m < g.nstencil ? m : m - g.nstencil];
+    }
+}
+
+// The count changes of swap p (a real swap) into `hist` (zeroed by the caller, visible
+// to the workgroup), then this workgroup's nonzero bins added to the slot's row `dc`
+// with integer atomics (order-independent, so the merged row is deterministic).
+__device__ __forceinline__ void gather_swap(const Geo &g, const Prop &p, const int *__restrict__ ss_sten,
+                                            const int4 *__restrict__ ms_sten, const uint8_t *lsp,
+                                            int *hist, int first, int stride,
+                                            const int4 (&recs)[kRecPrefetch], int *dc) {
+    SwapSite W;
+    W.p = p;
+    site_xyz_of(g, p.a, W.ax, W.ay, W.az);
+    site_xyz_of(g, p.b, W.bx, W.by, W.bz);
+    const int nss_items = 2 * g.ncoord;
+    const int nitems = nss_items + 2 * g.nstencil;
+#pragma unroll
+    for (int r = 0; r < kRecPrefetch; ++r) {
+        const int it = first + r * stride;
+        if (it < nitems) gather_item(g, W, ss_sten, lsp, hist, it, nss_items, recs[r]);
+    }
+    for (int it = first + kRecPrefetch * stride; it < nitems; it += stride) {
+        const int m = it - nss_items;
+        const int4 rec = (m >= 0) ? ms_sten[m < g.nstencil ? m : m - g.nstencil] : make_int4(0, 0, 0, 0);
+        gather_item(g, W, ss_sten, lsp, hist, it, nss_items, rec);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x)
+        if (hist[q] != 0) atomicAdd(&dc[q], hist[q]);
+}
+
+// Site of occupancy entry q of the candidate swap (a, b): q < ncoord the q-th stencil
+// neighbour of a, then b's, then a and b themselves.  (ax..bz: their coordinates.)
+__device__ __forceinline__ int occ_site(const Geo &g, const int *__restrict__ ss_sten, int a, int b,
+                                        int ax, int ay, int az, int bx, int by, int bz, int q) {
+    if (q >= 2 * g.ncoord) return q == 2 * g.ncoord ? a : b;
+    const bool on_a = q < g.ncoord;
+    const int sv = ss_sten[on_a ? q : q - g.ncoord];
+    const int dx = (int)(int8_t)(sv & 0xFF), dy = (int)(int8_t)((sv >> 8) & 0xFF);
+    const int dz = (int)(int8_t)((sv >> 16) & 0xFF);
+    return site_index(g, wrap((on_a ? ax : bx) + dx, g.box[0]), wrap((on_a ? ay : by) + dy, g.box[1]),
+                      wrap((on_a ? az : bz) + dz, g.box[2]));
+}
+
+// species at a site with the sites xa and xb exchanged (xa < 0: as stored)
+__device__ __forceinline__ int species_swapped(const uint8_t *lsp, int site, int xa, int xb) {
+    if (xa >= 0) {
+        if (site == xa) return lsp[xb];
+        if (site == xb) return lsp[xa];
+    }
+    return lsp[site];
+}
+
+}  // namespace chm_smp

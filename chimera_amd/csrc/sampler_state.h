@@ -1,0 +1,79 @@
+// Host-side state of a sampler (shared by sampler.hip and learned.hip).
+#pragma once
+
+#include "common.h"
+#include "sampler_dev.h"
+
+// Learned swap proposal: network, per-step candidate scores, move records, trainer.
+struct chm_lp {
+    int mode = 0;            // 0: uniform proposals only; 1: learned proposal mixed with uniform
+    int ncand = 256;         // candidates per replica and step (multiple of 32, <= 256)
+    double eps = 0.0;        // probability that a step uses the uniform proposal instead
+    double lam = 1.0;        // logit = -lam * beta * g / 2 + kappa * |dU|
+    double kappa = 0.0;
+    int nocc = 0, noccp = 0; // occupancy entries per candidate (2 ncoord + 2), padded to 16
+    int nfeat = 0, kp = 0;   // one-hot features nsp * nocc, padded to 32
+    int nparam = 0;          // flat FP32 parameters
+    int r2max = 0;           // largest squared stencil offset, units (a0/2)^2
+    int shell0 = 0;          // stencil shell index of the first shell
+    bool have_params = false;
+    chm::DevBuf params, adam_m, adam_v;          // FP32 masters and Adam moments
+    chm::DevBuf w1t, w1, w2t, w2, w3q;           // bf16 copies in both orders; w3 rounded
+    chm::DevBuf logit;                           // (nrep, ncand) scores of the pending step
+    chm::DevBuf pend_occ;                        // (nrep, noccp) occupancy of each pending proposal
+    chm::DevBuf rec_occ, rec_y, rec_meta;        // move-record ring
+    int rec_cap = 0;
+    long long rec_from = 0;                      // step count when recording started
+    long long adam_t = 0;
+    int tr_cap = 0;                              // training batch capacity
+    chm::DevBuf tr_occ, tr_y, tr_w, tr_wsum, tr_Z1, tr_H1, tr_Z2, tr_H2, tr_g, tr_dZ2, tr_dZ1, grad,
+        loss;
+};
+
+struct chm_sampler {
+    chm_ctx *ctx = nullptr;
+    chm_smp::Geo g{};
+    int nrep = 0, nk = 0, nsplit = 1, profile = 1;
+    int incremental = 0, reanchor_every = 512;
+    uint64_t seed = 0;
+    bool inited = false;
+    long long hstep = 0;
+    double nfit = 1.0;
+    size_t lds_k1 = 0;
+    chm::DevBuf site_xyz, nbr_j, nbr_sh, ss_sten, stencil, basisR, basisK, target, wcol, q0, Tb;
+    chm::DevBuf species, counts, dcounts, c2, best_c2, best_sp, rung_of, slot_of, beta, beta_slot,
+        stepc, iex,
+        prop, mtrial, mcur, partials, anchor, nanchor, exch_att, exch_acc, acc_rung, log, series,
+        tmp;
+    int log_cap = 0, series_cap = 0, series_rung = -1;
+    long long rec_base = 0;   // steps completed when recording (re)started
+    hipGraph_t graph[2] = {nullptr, nullptr};     // exchange period starting at an even /
+    hipGraphExec_t gexec[2] = {nullptr, nullptr}; // odd completed-step count
+    int graph_steps = 0;
+    chm_lp lp;
+};
+
+namespace chm_smp {
+
+constexpr int kPropInts = 6;   // a, b, new a, new b, is-swap, log q ratio (float bits)
+
+// per-step double buffers, indexed by step parity: the count-change rows and the
+// proposal records of step n, and chi^2 after step n
+inline int *drow(chm_sampler *S, long long n) {
+    return S->dcounts.as<int>() + (size_t)(n & 1) * S->nrep * S->g.nrow_tot;
+}
+inline int *prow(chm_sampler *S, long long n) {
+    return S->prop.as<int>() + (size_t)(n & 1) * S->nrep * kPropInts;
+}
+inline double *c2row(chm_sampler *S, long long n) {
+    return S->c2.as<double>() + (size_t)(n & 1) * S->nrep;
+}
+
+// learned.hip: score the candidates of step n + 1 (after step n was accepted), then
+// select one, compute its Hastings ratio and gather its count changes into drow(n + 1)
+int lp_launch_score(chm_sampler *S);
+int lp_launch_select(chm_sampler *S, long long n);
+int lp_prepare(chm_sampler *S);   // buffers for the current geometry (at create)
+void drop_graph(chm_sampler *S);  // sampler.hip: forget the captured step graphs
+
+}  // namespace chm_smp

@@ -111,6 +111,7 @@ class RMCSampler:
         self._S = None
         self._comp = None
         self._log_cap = 0
+        self._build_hooks = []   # re-applied to every new device sampler (learned proposal)
 
     # ------------------------------------------------------------------ basis
     def _lattice_geometry(self):
@@ -236,6 +237,8 @@ class RMCSampler:
             self._build(species0)
             if self._log_cap:
                 self.set_recording(self._log_cap, self._series_cap)
+            for hook in self._build_hooks:
+                hook(self)
         check(_lib.load().chm_sampler_set_precision(self._S, int(self.precision == "bf16x3"),
                                                     self.reanchor_every))
         check(_lib.load().chm_sampler_init(self._S, species0, f64(self._beta_rung),

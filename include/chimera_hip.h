@@ -179,7 +179,8 @@ int chm_sampler_set_recording(chm_sampler *s, int log_capacity, int series_capac
  *   11 chi(k) (nrep, nsp, nk) double, rung order   12 trial chi(R) of the last step
  *      (nrep, nsp*nkw*nwin*2) double, slot order     13 running chi(R) accumulator, same
  *      layout (incremental mode)                      14 FP64 re-anchors done (nrep) int64,
- *      slot order
+ *      slot order                                     15 learned-proposal logits of the
+ *      candidates last scored (nrep, ncand) float, slot order
  * RMCSampler attributes species/chi2/chi/best (method_a_rmc.py:166-184, 493-496). */
 int chm_sampler_read(chm_sampler *s, int what, void *host, int64_t nbytes);
 
@@ -226,6 +227,46 @@ int chm_sampler_get_tuning(chm_sampler *s, int *nsplit);
  * the state stays consistent.  No counterpart in the reference (it has no profiler
  * hooks). */
 int chm_sampler_profile(chm_sampler *s, int iters, double *ms);
+
+/* ----------------------------------------------------- learned swap proposal
+ * No counterpart in the reference (its swap proposal is uniform, exafs_gpu/kernels.py:
+ * 362-398).  Per replica and step, ncand state-independent candidate pairs are scored
+ * by an MLP (one-hot species occupancy of both sites' stencil neighbours and of the
+ * sites: nocc = 2 ncoord + 2 entries, nfeat = nsp * nocc features -> 256 -> 256 -> 1,
+ * SiLU, bf16 weights, FP32 accumulation) that predicts the swap's energy change
+ * nfit * dchi2; logit = -lam * beta * g / 2 + kappa * |dU| (dU: change of unlike
+ * first-shell bonds); one candidate is drawn from the softmax and accepted with the
+ * exact Hastings ratio (the reverse probability from the same candidate set scored at
+ * the proposed state).  With probability eps a step uses the uniform proposal. */
+
+/* Sizes: flat FP32 parameter count, occupancy entries, features, padded features kp.
+ * Flat layout: W1 (kp, 256) feature-major | b1 (256) | W2 (256, 256) input-major |
+ * b2 (256) | w3 (256) | b3 (1). */
+int chm_lp_info(chm_sampler *s, int *nparam, int *nocc, int *nfeat, int *kp);
+/* Network parameters (FP32 masters; the scorer uses them rounded to bf16); resets Adam. */
+int chm_lp_set_params(chm_sampler *s, const float *flat);
+int chm_lp_get_params(chm_sampler *s, float *flat);
+/* mode 0: uniform proposals (the plain sampler); mode 1: learned proposal with the
+ * uniform one mixed in at probability eps.  ncand: multiple of 32 in 32..256. */
+int chm_lp_configure(chm_sampler *s, int mode, int ncand, double eps, double lam, double kappa);
+/* Move records: a ring of `capacity` (occupancy, nfit * dchi2, rung | accepted << 16)
+ * entries written by every step (null moves: meta -1).  0 disables. */
+int chm_lp_set_records(chm_sampler *s, int capacity);
+/* Copy the first `rows` ring entries: occ (rows, nocc) uint8, y (rows), meta (rows);
+ * nvalid = entries written so far (capped at the capacity).  Any output may be NULL. */
+int chm_lp_read_records(chm_sampler *s, int64_t *nvalid, uint8_t *occ, float *y, int32_t *meta,
+                        int64_t rows);
+/* Network output g of n occupancy rows (n, nocc) through the scorer's MFMA path. */
+int chm_lp_eval(chm_sampler *s, int n, const uint8_t *occ, float *g);
+/* Loss 0.5 * mean_w (g - y)^2 and its gradient (flat layout) over n rows, weights 0/1,
+ * through the trainer's FP32 forward and backward kernels (no update). */
+int chm_lp_grad(chm_sampler *s, int n, const uint8_t *occ, const float *y, const float *w,
+                float *grad, float *loss);
+/* iters Adam steps (lr; betas 0.9, 0.999) on minibatches of `batch` records drawn from
+ * the ring with a counter-based stream (seed), records of rungs <= max_rung only, targets
+ * clipped to [-clip, clip]; loss of the last minibatch. */
+int chm_lp_train(chm_sampler *s, int iters, int batch, float lr, int max_rung, float clip,
+                 uint64_t seed, float *loss);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,185 @@
+"""Learned swap proposal on the device (through the C ABI) against the oracle:
+network output, gradients, move records and the sampler trajectory step by step."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import learned as LP
+from oracle import rmc as OR
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def tern108():
+    from chimera_amd import workloads as W
+    from oracle.build import problem_from
+    _say("[build] t108 fcc 3x3x3 ternary MS nrep=6")
+    P = W.Problem(W.Workload("t108", "fcc", (3, 3, 3), 3.59, ("Cr", "Co", "Ni"), 5.2, True, 6), seed=0)
+    P.sampler.init_state(P.cfg0, P.cell.ideal)
+    return P, problem_from(P)
+
+
+def _stencil(P):
+    return LP.Stencil(P.cell.ideal, P.i, P.j, P.shift, P.shell, P.cell.box)
+
+
+def _random_params(lp, seed, w3_scale=2.0):
+    rng = np.random.default_rng(seed)
+    flat = lp.initial_params(seed)
+    o = lp.offsets()
+    for name, sc in (("w3", w3_scale / np.sqrt(256)), ("b1", 0.2), ("b2", 0.2), ("b3", 0.5)):
+        off, n = o[name]
+        flat[off:off + n] = (rng.standard_normal(n) * sc).astype(np.float32)
+    return flat
+
+
+def _lp(P, seed=0):
+    from chimera_amd.learned import LearnedProposal
+    S = P.sampler
+    if getattr(S, "_lp_test", None) is None:
+        S._lp_test = LearnedProposal(S, seed=seed)
+    lp = S._lp_test
+    lp.disable()
+    return lp
+
+
+def test_network_output_matches_oracle(tern108):
+    P, O = tern108
+    lp = _lp(P)
+    flat = _random_params(lp, 1)
+    lp.set_params(flat)
+    rng = np.random.default_rng(2)
+    occ = rng.integers(0, P.nsp, (100, lp.nocc)).astype(np.uint8)
+    _say(f"[test] MFMA scorer on {occ.shape} occupancy rows vs oracle")
+    g_dev = lp.eval(occ).astype(np.float64)
+    g_or, _ = LP.forward(flat, occ, P.nsp)
+    err = np.abs(g_dev - g_or)
+    assert np.median(err) < 1e-4 * max(1.0, np.abs(g_or).max())
+    assert np.all(err <= 3e-3 * np.maximum(1.0, np.abs(g_or)))
+
+
+def test_gradients_match_oracle(tern108):
+    P, O = tern108
+    lp = _lp(P)
+    flat = _random_params(lp, 3)
+    lp.set_params(flat)
+    rng = np.random.default_rng(4)
+    n = 64
+    occ = rng.integers(0, P.nsp, (n, lp.nocc)).astype(np.uint8)
+    y = rng.standard_normal(n).astype(np.float32) * 3
+    w = (rng.random(n) < 0.8).astype(np.float32)
+    _say("[test] trainer forward/backward vs oracle gradient")
+    loss_d, g_d = lp.grad(occ, y, w)
+    loss_o, g_o = LP.loss_and_grad(flat, occ, y.astype(np.float64), w.astype(np.float64), P.nsp)
+    assert abs(loss_d - loss_o) <= 1e-3 * abs(loss_o) + 1e-6
+    for name, (off, m) in lp.offsets().items():
+        a, b = g_d[off:off + m].astype(np.float64), g_o[off:off + m]
+        scale = np.abs(b).max() + 1e-12
+        assert np.abs(a - b).max() <= 5e-3 * scale, name
+        if m > 1:
+            assert np.corrcoef(a, b)[0, 1] > 0.9999, name
+
+
+def test_move_records_match_oracle(tern108):
+    """Records of the uniform sampler: the proposal's occupancy before the move, the
+    energy change nfit * dchi2, the rung and the outcome, against the oracle's run."""
+    P, O = tern108
+    S = P.sampler
+    lp = _lp(P)
+    nsteps = 12
+    S.seed = 5
+    S.set_recording(log_steps=nsteps)
+    S.init_state(P.cfg0, P.cell.ideal)
+    lp.record(nsteps * S.nrep)
+    _say(f"[run] device {nsteps} steps with records")
+    S.run(nsteps, exchange_every=0, log_every=10 ** 9)
+    nv, occ, y, meta = lp.records()
+    assert nv == nsteps * S.nrep
+    R = OR.PlainSampler(O, nrep=S.nrep, seed=5)
+    R.init_state(P.cfg0)
+    st = _stencil(P)
+    sp = R.species.copy()
+    for t in range(nsteps):
+        before = R.species.copy()
+        R.step()
+        for rung in range(S.nrep):                 # no exchange: slot == rung
+            a, b, acc, c2, x, u = R.log[t][rung]
+            idx = t * S.nrep + rung
+            if a == b:
+                assert meta[idx] == -1
+                continue
+            want, _, _ = LP.occupancy(before[rung], st, a, b)
+            assert np.array_equal(occ[idx], want), (t, rung)
+            assert meta[idx] == rung | (acc << 16)
+            dE = x / R.beta[rung]
+            assert abs(y[idx] - dE) <= 1e-5 * max(1.0, abs(dE))
+    del sp
+    lp.record(0)
+
+
+def test_learned_trajectory_matches_oracle(tern108):
+    """Device sampler with the learned proposal (eps 0.3 uniform mixture, tilt on) move by
+    move against the oracle's restatement with the same parameters.  A differing choice
+    is allowed only at a near tie of the selecting uniform or of the acceptance; the
+    comparison stops there."""
+    P, O = tern108
+    S = P.sampler
+    lp = _lp(P)
+    flat = _random_params(lp, 7, w3_scale=0.05)
+    lp.set_params(flat)
+    nsteps, exch, seed = 16, 5, 11
+    S.seed = seed
+    S.set_recording(log_steps=nsteps)
+    S.init_state(P.cfg0, P.cell.ideal)
+    lp.enable(ncand=64, eps=0.3, lam=1.0, kappa=0.2)
+    _say(f"[run] device learned {nsteps} steps")
+    S.run(nsteps, exchange_every=exch, log_every=10 ** 9)
+    log = S.move_log(nsteps)
+    lp.disable()
+    R = OR.PlainSampler(O, nrep=S.nrep, seed=seed)
+    R.init_state(P.cfg0)
+    L = LP.LearnedSampler(R, flat, _stencil(P), ncand=64, eps=0.3, lam=1.0, kappa=0.2)
+    _say("[run] oracle learned")
+    L.run(nsteps, exchange_every=exch)
+    margins = {(s, c): m for s, c, m in L.margins}
+    learned_steps = 0
+    for t in range(nsteps):
+        by_rung = {int(r[3]): r for r in log[t]}
+        for rung in range(S.nrep):
+            a, b, acc, c2, x, u = R.log[t][rung]
+            gdev = by_rung[rung]
+            m = margins[(t + 1, rung)]
+            learned_steps += m < 1.0
+            if (int(gdev[0]), int(gdev[1])) != (a, b):
+                assert m < 1e-5, f"step {t} rung {rung}: different pair, margin {m}"
+                return
+            if int(gdev[2]) != acc:
+                assert abs(x + np.log(u)) < 1e-4, f"step {t} rung {rung}"
+                return
+    assert learned_steps > nsteps * S.nrep // 2
+    assert np.array_equal(S.species, R.species)
+    np.testing.assert_allclose(S.chi2, R.chi2, rtol=1e-9)
+
+
+def test_training_reduces_loss(tern108):
+    P, O = tern108
+    S = P.sampler
+    lp = _lp(P)
+    lp.set_params(lp.initial_params(0))
+    S.seed = 3
+    S.init_state(P.cfg0, P.cell.ideal)
+    lp.record(200 * S.nrep)
+    _say("[run] 200 uniform steps with records, then training")
+    S.run(200, exchange_every=25, log_every=10 ** 9)
+    l0 = lp.train(iters=1, batch=256, lr=1e-3, seed=1)
+    l1 = lp.train(iters=300, batch=256, lr=1e-3, seed=2)
+    _say(f"[test] loss {l0:.4g} -> {l1:.4g}")
+    assert np.isfinite(l1) and l1 < 0.5 * l0
+    lp.record(0)

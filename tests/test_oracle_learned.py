@@ -1,0 +1,121 @@
+"""Pins the oracle of the learned swap proposal: bf16 rounding, MLP gradients by
+finite differences, and detailed balance of the proposal + Hastings ratio."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import lattice as OL
+from oracle import learned as LP
+
+
+def test_bf16_rounding_matches_torch_and_ties_to_even():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.standard_normal(2000) * 10.0 ** rng.integers(-6, 6, 2000),
+                        [1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, -(1 + 2 ** -8), 0.0]]).astype(np.float32)
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float64).numpy()
+    np.testing.assert_array_equal(LP.bf16(x), want)
+    assert LP.bf16(1 + 2 ** -8) == 1.0                     # halfway: to the even mantissa
+    assert LP.bf16(1 + 3 * 2 ** -8) == 1 + 2 ** -6
+
+
+def test_mlp_gradient_matches_finite_differences():
+    nsp, nocc, n = 3, 10, 7
+    nfeat = nsp * nocc
+    rng = np.random.default_rng(1)
+    flat = LP.init_params(nfeat, nocc, seed=3, w3_scale=1.0).astype(np.float64)
+    kp, off, _ = LP.layout(nfeat)
+    for name in ("b1", "b2", "b3"):
+        o, m = off[name]
+        flat[o:o + m] = rng.standard_normal(m) * 0.3
+    occ = rng.integers(0, nsp, (n, nocc))
+    y = rng.standard_normal(n)
+    w = rng.random(n) + 0.5
+    loss, grad = LP.loss_and_grad(flat, occ, y, w, nsp, rounding=False)
+    # check a sample of every block, including W1 rows that some row activates
+    o1 = off["W1"][0]
+    active = sorted({nsp * q + int(occ[r, q]) for r in range(n) for q in range(nocc)})
+    idx = [o1 + f * LP.HID + c for f in active[:6] for c in (0, 17, 255)]
+    for name in ("b1", "W2", "b2", "w3", "b3"):
+        o, m = off[name]
+        idx += list(o + rng.choice(m, min(m, 8), replace=False))
+    h = 1e-6
+    for i in idx:
+        fp, fm = flat.copy(), flat.copy()
+        fp[i] += h
+        fm[i] -= h
+        lp_, _ = LP.loss_and_grad(fp, occ, y, w, nsp, rounding=False)
+        lm_, _ = LP.loss_and_grad(fm, occ, y, w, nsp, rounding=False)
+        fd = (lp_ - lm_) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, grad[i])
+    # a W1 row that no candidate activates gets no gradient
+    unused = [f for f in range(nfeat) if f not in active]
+    for f in unused:
+        assert np.all(grad[o1 + f * LP.HID:o1 + (f + 1) * LP.HID] == 0.0)
+
+
+def _toy():
+    cell = OL.Supercell("sc", (2, 2, 2), 3.0)
+    i, j, sh = cell.neighbour_list(1.5 * 3.0)
+    shell = np.round(np.linalg.norm(cell.ideal[j] + sh - cell.ideal[i], axis=1) / 3.0 * 10).astype(int)
+    _, shell = np.unique(shell, return_inverse=True)
+    st = LP.Stencil(cell.ideal, i, j, sh, shell, cell.box)
+    return cell, st
+
+
+def test_stencil_reproduces_the_neighbour_sets():
+    cell = OL.Supercell("fcc", (3, 3, 3), 3.6)
+    i, j, sh = cell.neighbour_list(3.6 * 1.01)
+    shell = np.zeros(i.size, int)
+    st = LP.Stencil(cell.ideal, i, j, sh, shell, cell.box)
+    for q in (0, 5, 77, cell.natoms - 1):
+        assert sorted(st.neighbours(q)) == sorted(j[i == q].tolist())
+
+
+@pytest.mark.parametrize("lam,kappa", [(1.0, 0.0), (0.7, 0.4)])
+def test_learned_proposal_detailed_balance(lam, kappa):
+    """For a FIXED candidate set, the transition matrix of (softmax proposal, Hastings
+    acceptance) over every state of a binary 8-site toy satisfies
+    pi(x) T(x, y) = pi(y) T(y, x); T(y, x) uses y's own forward probabilities."""
+    cell, st = _toy()
+    nsp, n = 2, cell.natoms
+    nfeat = nsp * (2 * st.ncoord + 2)
+    flat = LP.init_params(nfeat, 2 * st.ncoord + 2, seed=5, w3_scale=3.0)
+    # candidates with a repeat, a reversed repeat and a same-site pair
+    cands = [(0, 3), (1, 6), (3, 0), (2, 5), (4, 7), (0, 3), (5, 5), (6, 1), (2, 7)]
+    states = [np.array(c) for c in itertools.product((0, 1), repeat=n) if sum(c) == n // 2]
+    index = {tuple(s): k for k, s in enumerate(states)}
+    rng = np.random.default_rng(7)
+    E = rng.standard_normal(len(states)) * 1.5
+    beta = 0.8
+    T = np.zeros((len(states), len(states)))
+    for k, sp in enumerate(states):
+        lg, _, _ = LP.logits(flat, sp, st, cands, nsp, beta, lam, kappa)
+        if not np.isfinite(lg).any():
+            continue
+        p = np.exp(lg - lg.max())
+        p /= p.sum()
+        for jj, (a, b) in enumerate(cands):
+            if not np.isfinite(lg[jj]):
+                continue
+            y = sp.copy()
+            y[a], y[b] = sp[b], sp[a]
+            ky = index[tuple(y)]
+            lqr = LP.log_q_ratio(flat, sp, st, nsp, cands, lg, jj, beta, lam, kappa)
+            A = min(1.0, np.exp(-beta * (E[ky] - E[k]) + lqr))
+            T[k, ky] += p[jj] * A
+    pi = np.exp(-beta * E)
+    pi /= pi.sum()
+    F = pi[:, None] * T
+    assert np.allclose(F, F.T, rtol=1e-10, atol=1e-15)
+    assert T.sum() > 1.0                                    # moves do happen
+
+
+def test_candidate_stream_is_uniform_and_state_free():
+    c1 = LP.candidates(11, 5, 3, 4000, 256)
+    c2 = LP.candidates(11, 5, 3, 4000, 256)
+    assert c1 == c2 and len(set(c1)) > 250
+    a = np.array([x for x, _ in LP.candidates(1, 1, 0, 1000, 20000)])
+    counts = np.bincount(a // 100, minlength=10)
+    assert counts.min() > 1800 and counts.max() < 2200
