@@ -130,8 +130,9 @@ class Context:
     def sync(self):
         check(load().chm_ctx_sync(self.h))
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # bound at definition: module globals may already be gone at interpreter exit
+        if _finalizing():
             return
         if getattr(self, "h", None) and _lib is not None:
             _lib.chm_ctx_destroy(self.h)

@@ -73,6 +73,7 @@ struct DevBuf {
 struct chm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    int refs = 1;   // the owner's reference plus one per sampler / transform made on it
     // tables
     int nsp = 0, nk = 0;
     double s02 = 1.0;
@@ -96,6 +97,17 @@ struct chm_ft {
 };
 
 namespace chm {
+// A context lives until its owner and every object made on it have been destroyed, in
+// any order (chm_ctx_destroy drops the owner's reference).
+inline void ctx_retain(chm_ctx *c) { ++c->refs; }
+inline void ctx_release(chm_ctx *c) {
+    if (--c->refs > 0) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
 // Generic FP64 GEMM on the context stream: C[b] (M x N) = A[b] (M x K) @ B[b] (K x N),
 // row-major with leading dimensions and per-batch element strides.
 int gemm_f64(hipStream_t st, int nbatch, int M, int N, int K, const double *A, int lda,

@@ -43,10 +43,7 @@ extern "C" int chm_ctx_create(int device, chm_ctx **out) {
 
 extern "C" int chm_ctx_destroy(chm_ctx *ctx) {
     if (!ctx) return CHM_OK;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    hipStreamDestroy(ctx->stream);
-    delete ctx;
+    chm::ctx_release(ctx);
     return CHM_OK;
 }
 

@@ -483,7 +483,9 @@ __global__ __launch_bounds__(256) void k_lp_select(LPArgs A) {
     }
     // the lead workgroup's records are requested only now: its registers held the MLP
     if (lead) prefetch_recs(g, A.ms_sten, first, stride, true, recs);
-    if (p.sw) gather_swap(g, p, A.ss_sten, A.ms_sten, L.lsp, hist, first, stride, recs, A.dc_prop);
+    if (p.sw)
+        gather_swap(g, p, A.ss_sten, A.ms_sten, L.lsp, hist, first, stride, recs,
+                    A.dc_prop + (long)s * g.nrow_tot);
 }
 
 // ------------------------------------------------------------------ evaluation (tests)

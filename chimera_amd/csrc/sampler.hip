@@ -664,7 +664,9 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
             A.pend_occ[(long)s * A.noccp + q] =
                 lsp[occ_site(g, A.ss_sten, p.a, p.b, ax, ay, az, bx, by, bz, q)];
     }
-    if (p.sw) gather_swap(g, p, A.ss_sten, A.ms_sten, lsp, hist, first, stride, recs, A.dc_prop);
+    if (p.sw)
+        gather_swap(g, p, A.ss_sten, A.ms_sten, lsp, hist, first, stride, recs,
+                    A.dc_prop + (long)s * g.nrow_tot);
 }
 
 // chi^2 of arbitrary window transforms (one row per workgroup), direct form:
@@ -1150,16 +1152,19 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         chm::set_error(std::string("sampler upload: ") + hipGetErrorString(e));
         return fail(CHM_E_HIP);
     }
+    chm::ctx_retain(ctx);
     *out = S;
     return CHM_OK;
 }
 
 extern "C" int chm_sampler_destroy(chm_sampler *S) {
     if (!S) return CHM_OK;
-    hipSetDevice(S->ctx->device);
-    hipStreamSynchronize(S->ctx->stream);
+    chm_ctx *ctx = S->ctx;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
     drop_graph(S);
     delete S;
+    chm::ctx_release(ctx);
     return CHM_OK;
 }
 

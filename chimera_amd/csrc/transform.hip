@@ -184,15 +184,18 @@ extern "C" int chm_ft_create(chm_ctx *ctx, int nchan, int nk_model, int nfft, in
         chm::set_error(std::string("build_dft: ") + hipGetErrorString(e));
         return CHM_E_HIP;
     }
+    chm::ctx_retain(ctx);
     *out = ft;
     return CHM_OK;
 }
 
 extern "C" int chm_ft_destroy(chm_ft *ft) {
     if (ft) {
-        hipSetDevice(ft->ctx->device);
-        hipStreamSynchronize(ft->ctx->stream);
+        chm_ctx *ctx = ft->ctx;
+        hipSetDevice(ctx->device);
+        hipStreamSynchronize(ctx->stream);
         delete ft;
+        chm::ctx_release(ctx);
     }
     return CHM_OK;
 }

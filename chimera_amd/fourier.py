@@ -131,8 +131,9 @@ class _DeviceFT:
         check(_lib.load().chm_ft_apply_edge(self.h, int(edge), n, rows, out))
         return out
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # bound at definition: module globals may already be gone at interpreter exit
+        if _finalizing():
             return
         if getattr(self, "h", None) and _lib._lib is not None:
             _lib._lib.chm_ft_destroy(self.h)

@@ -24,6 +24,7 @@ the candidate-set softmax.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -45,7 +46,8 @@ class LearnedProposal:
         self.mode = 0
         self.rec_cap = 0
         self.set_params(self.initial_params(seed))
-        sampler._build_hooks.append(self._reapply)
+        # a weak hook: the sampler must not keep its proposal (and itself) alive in a cycle
+        sampler._build_hooks.append(weakref.WeakMethod(self._reapply))
 
     # ------------------------------------------------------------ parameters
     def offsets(self):

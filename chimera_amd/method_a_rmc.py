@@ -111,7 +111,7 @@ class RMCSampler:
         self._S = None
         self._comp = None
         self._log_cap = 0
-        self._build_hooks = []   # re-applied to every new device sampler (learned proposal)
+        self._build_hooks = []   # weak methods re-applied to every new device sampler
 
     # ------------------------------------------------------------------ basis
     def _lattice_geometry(self):
@@ -237,8 +237,12 @@ class RMCSampler:
             self._build(species0)
             if self._log_cap:
                 self.set_recording(self._log_cap, self._series_cap)
-            for hook in self._build_hooks:
-                hook(self)
+            for ref in list(self._build_hooks):
+                hook = ref()
+                if hook is None:
+                    self._build_hooks.remove(ref)
+                else:
+                    hook(self)
         check(_lib.load().chm_sampler_set_precision(self._S, int(self.precision == "bf16x3"),
                                                     self.reanchor_every))
         check(_lib.load().chm_sampler_init(self._S, species0, f64(self._beta_rung),
@@ -484,7 +488,8 @@ class RMCSampler:
             _lib._lib.chm_sampler_destroy(self._S)
         self._S = None
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # bound at definition: module globals may already be gone at interpreter exit
+        if _finalizing():
             return
         self._destroy()

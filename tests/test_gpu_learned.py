@@ -40,12 +40,15 @@ def _random_params(lp, seed, w3_scale=2.0):
     return flat
 
 
+_LP = {}
+
+
 def _lp(P, seed=0):
     from chimera_amd.learned import LearnedProposal
     S = P.sampler
-    if getattr(S, "_lp_test", None) is None:
-        S._lp_test = LearnedProposal(S, seed=seed)
-    lp = S._lp_test
+    if id(S) not in _LP:
+        _LP[id(S)] = LearnedProposal(S, seed=seed)
+    lp = _LP[id(S)]
     lp.disable()
     return lp
 
@@ -105,9 +108,10 @@ def test_move_records_match_oracle(tern108):
     R = OR.PlainSampler(O, nrep=S.nrep, seed=5)
     R.init_state(P.cfg0)
     st = _stencil(P)
-    sp = R.species.copy()
+    bad = []
     for t in range(nsteps):
         before = R.species.copy()
+        c2_before = R.chi2.copy()
         R.step()
         for rung in range(S.nrep):                 # no exchange: slot == rung
             a, b, acc, c2, x, u = R.log[t][rung]
@@ -118,10 +122,12 @@ def test_move_records_match_oracle(tern108):
             want, _, _ = LP.occupancy(before[rung], st, a, b)
             assert np.array_equal(occ[idx], want), (t, rung)
             assert meta[idx] == rung | (acc << 16)
-            dE = x / R.beta[rung]
-            assert abs(y[idx] - dE) <= 1e-5 * max(1.0, abs(dE))
-    del sp
+            dE = (c2 - c2_before[rung]) * O.nfit
+            if abs(y[idx] - dE) > 1e-5 * max(1.0, abs(dE)):
+                bad.append((t, rung, float(y[idx]), dE, acc))
+    np.testing.assert_allclose(S.chi2, R.chi2, rtol=1e-9)
     lp.record(0)
+    assert not bad, bad[:12]
 
 
 def test_learned_trajectory_matches_oracle(tern108):
