@@ -75,13 +75,15 @@ _SIGS = {
                               C.POINTER(C.c_int)]),
     "chm_lp_set_params": (C.c_int, [vp, vp]),
     "chm_lp_get_params": (C.c_int, [vp, vp]),
-    "chm_lp_configure": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]),
+    "chm_lp_configure": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.c_double]),
     "chm_lp_set_records": (C.c_int, [vp, C.c_int]),
     "chm_lp_read_records": (C.c_int, [vp, C.POINTER(C.c_int64), vp, vp, vp, C.c_int64]),
     "chm_lp_eval": (C.c_int, [vp, C.c_int, vp, vp]),
     "chm_lp_grad": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp]),
     "chm_lp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_uint64,
                                vp]),
+    "chm_lp_profile": (C.c_int, [vp, C.c_int, vp]),
 }
 
 EXPORTS = tuple(_SIGS)

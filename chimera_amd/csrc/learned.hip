@@ -56,14 +56,13 @@ struct LPArgs {
     const double *beta_slot;
     const uint64_t *stepc;
     const int *ss_sten;
-    const int4 *ms_sten;
+    const int *nbr;      // (natoms, ncoord) neighbour sites in stencil order
     uint64_t seed;
     int ncand, nocc, noccp, nfeat, kp, r2max;
-    double eps, lam, kappa;
+    double eps, lam, kappa, beta_max;
     LPW W;
     float *logit;        // (nrep, ncand)
     int *prop_next;      // (nrep, kPropInts)
-    int *dc_prop;        // (nrep, nrow_tot), zero on entry
     uint8_t *pend_occ;   // (nrep, noccp)
 };
 
@@ -78,6 +77,8 @@ __device__ __forceinline__ void candidate(uint64_t h0, int i, int natoms, int &a
 }
 
 __device__ __forceinline__ float silu(float z) { return z / (1.f + expf(-z)); }
+// the scorer's SiLU: hardware exp2 and reciprocal (a few ulp; the trainer keeps silu)
+__device__ __forceinline__ float silu_fast(float z) { return z * __frcp_rn(1.f + __expf(-z)); }
 __device__ __forceinline__ float dsilu(float z) {
     const float s = 1.f / (1.f + expf(-z));
     return s * (1.f + z * (1.f - s));
@@ -96,31 +97,74 @@ __device__ __forceinline__ int dist2(const Geo &g, const int *p, const int *q) {
     return dx * dx + dy * dy + dz * dz;
 }
 
+// the uniform proposal on this step: a state-independent coin, and every slot colder
+// than beta_max (there the network's error would be multiplied by a large beta)
+__device__ __forceinline__ bool lp_uniform(const LPArgs &A, uint64_t h0, double beta) {
+    return u01(splitmix64(h0 ^ kCoinSalt)) < A.eps || beta > A.beta_max;
+}
+
+// site of occupancy entry q of candidate (a, b): the stencil neighbours of a, of b, then
+// a and b themselves
+__device__ __forceinline__ int occ_site_tbl(const LPArgs &A, int a, int b, int q) {
+    const int nc = A.g.ncoord;
+    if (q < nc) return A.nbr[(long)a * nc + q];
+    if (q < 2 * nc) return A.nbr[(long)b * nc + q - nc];
+    return q == 2 * nc ? a : b;
+}
+
 // Occupancy rows of nrows candidates (rows[r] = (a, b)) with sites xa, xb exchanged
 // virtually (xa < 0: as stored) into occ (MR x noccp), the one-hot features into X
-// (MR x ldx bf16, zero beyond nfeat and beyond nrows) and |dU| into du.  Needs the
-// rows' coordinates in rxyz (MR x 6).  Ends with the workgroup synchronised.
-__device__ void build_rows(const LPArgs &A, const uint8_t *lsp, const int2 *rows, const int *rxyz,
-                           int nrows, int xa, int xb, uint8_t *occ, __bf16 *X, int ldx, int *du) {
+// (MR x ldx bf16, zero beyond nfeat and beyond nrows) and dU into du.  shell0[q] marks
+// the first-shell stencil entries.  The neighbour-table loads of a lane are all issued
+// before any is used.  Ends with the workgroup synchronised.
+constexpr int IPT = 8;   // occupancy entries per lane (MR * nocc <= IPT * blockDim.x)
+
+// idx / d for 0 <= idx < 2^20, 0 < d < 2^12 (a float reciprocal; the quotient is exact)
+__device__ __forceinline__ int small_div(int idx, int d, float inv_d) {
+    int r = (int)(((float)idx + 0.5f) * inv_d);
+    return r;
+}
+
+// the neighbour-table loads of this lane's occupancy entries (issued, not waited for)
+__device__ __forceinline__ void row_sites(const LPArgs &A, const int2 *rows, int nrows, int (&site)[IPT]) {
+    const int n = nrows * A.nocc;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int idx = k * blockDim.x + threadIdx.x;
+        site[k] = -1;
+        if (idx < n) {
+            const int r = small_div(idx, A.nocc, 1.f / (float)A.nocc), q = idx - r * A.nocc;
+            site[k] = occ_site_tbl(A, rows[r].x, rows[r].y, q);
+        }
+    }
+}
+
+// Occupancy rows of nrows candidates (rows[r] = (a, b), sites from row_sites) with sites
+// xa, xb exchanged virtually (xa < 0: as stored) into occ (MR x noccp), the one-hot
+// features into X (MR x ldx bf16, zero beyond nfeat and beyond nrows) and dU into du
+// (zeroed by the caller).  shell0[q] marks the first-shell stencil entries.  Ends with
+// the workgroup synchronised.
+__device__ void build_rows(const LPArgs &A, const uint8_t *lsp, const uint8_t *shell0, const int2 *rows,
+                           int nrows, const int (&site)[IPT], int xa, int xb, uint8_t *occ, __bf16 *X,
+                           int ldx, int *du) {
     const Geo &g = A.g;
     const int t = threadIdx.x;
-    if (t < MR) du[t] = 0;
-    __syncthreads();
-    for (int idx = t; idx < nrows * A.nocc; idx += blockDim.x) {
-        const int r = idx / A.nocc, q = idx - r * A.nocc;
+    const int n = nrows * A.nocc;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int idx = k * blockDim.x + t;
+        if (idx >= n) continue;
+        const int r = small_div(idx, A.nocc, 1.f / (float)A.nocc), q = idx - r * A.nocc;
         const int a = rows[r].x, b = rows[r].y;
-        const int *c = rxyz + 6 * r;
-        const int site = occ_site(g, A.ss_sten, a, b, c[0], c[1], c[2], c[3], c[4], c[5], q);
-        const int sp = species_swapped(lsp, site, xa, xb);
+        const int sp = species_swapped(lsp, site[k], xa, xb);
         occ[r * A.noccp + q] = (uint8_t)sp;
         if (q < 2 * g.ncoord) {
             const bool on_a = q < g.ncoord;
-            const int sv = A.ss_sten[on_a ? q : q - g.ncoord];
-            if (((sv >> 24) & 0xFF) == 0) {                 // first shell
+            if (shell0[on_a ? q : q - g.ncoord]) {
                 const int sa = species_swapped(lsp, a, xa, xb), sb = species_swapped(lsp, b, xa, xb);
                 int cnt = 0;
-                if (on_a && site != b) cnt = (sp == sa) - (sp == sb);
-                if (!on_a && site != a) cnt = (sp == sb) - (sp == sa);
+                if (on_a && site[k] != b) cnt = (sp == sa) - (sp == sb);
+                if (!on_a && site[k] != a) cnt = (sp == sb) - (sp == sa);
                 if (cnt) atomicAdd(&du[r], cnt);
             }
         }
@@ -146,99 +190,157 @@ __device__ void build_rows(const LPArgs &A, const uint8_t *lsp, const int2 *rows
     __syncthreads();
 }
 
-// g of the MR rows whose features are in X (LDS): two MFMA layers, 4 waves x 64 hidden
-// units each; H (MR x LDH bf16) and gp (4 x MR) are LDS scratch; result in gout (LDS).
-__device__ void mlp_rows(const LPW &W, const __bf16 *X, int ldx, __bf16 *H, float *gp, float *gout) {
+// first-shell flags of the stencil entries into LDS (at most 256 entries)
+__device__ __forceinline__ void load_shell0(const LPArgs &A, uint8_t *shell0) {
+    for (int q = threadIdx.x; q < A.g.ncoord; q += blockDim.x) shell0[q] = ((A.ss_sten[q] >> 24) & 0xFF) == 0;
+}
+
+// This lane's share of the network: 8 waves x 32 hidden units (two 16-column tiles).  The
+// weight fragments are loaded once, into registers, at the start of a kernel, so their
+// latency overlaps the occupancy gathers.
+constexpr int NT = 512;    // threads per workgroup of the scorer and the selection
+constexpr int KT1 = 12;    // layer-1 K tiles held in registers (kp <= 384 in one pass)
+struct Frag {
+    bf16x8 w1[KT1][2], w2[HID / 32][2];
+    float b1[2], b2[2], w3[2];
+};
+
+__device__ __forceinline__ void load_frag(const LPW &W, Frag &F) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int lr = lane & 15, lk = 8 * (lane >> 4);
+    const int nk1 = W.kp / 32;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int n = 32 * wave + 16 * c + lr;
+        const __bf16 *w1 = W.w1t + (long)n * W.kp + lk;
+        const __bf16 *w2 = W.w2t + (long)n * HID + lk;
+#pragma unroll
+        for (int kt = 0; kt < KT1; ++kt)
+            if (kt < nk1) F.w1[kt][c] = *reinterpret_cast<const bf16x8 *>(w1 + 32 * kt);
+#pragma unroll
+        for (int kt = 0; kt < HID / 32; ++kt) F.w2[kt][c] = *reinterpret_cast<const bf16x8 *>(w2 + 32 * kt);
+        F.b1[c] = W.b1[n];
+        F.b2[c] = W.b2[n];
+        F.w3[c] = W.w3q[n];
+    }
+}
+
+// g of the MR rows whose features are in X (LDS); H (MR x LDH bf16) and gp (8 x MR) are LDS
+// scratch; result in gout (LDS).  Every thread of the (NT-thread) workgroup calls it.
+// With F the weight fragments come from registers (loaded earlier by load_frag);
+// without, each K tile's fragments are loaded one tile ahead (fewer registers).
+template <bool kHeld>
+__device__ void mlp_rows(const LPW &W, const Frag *F, const __bf16 *X, int ldx, __bf16 *H, float *gp,
+                         float *gout) {
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int lr = lane & 15, lk = 8 * (lane >> 4);
-    f32x4 acc[2][4];
+    const int n0 = 32 * wave + lr, n1 = n0 + 16;
+    const __bf16 *w1a = W.w1t + (long)n0 * W.kp + lk, *w1b = W.w1t + (long)n1 * W.kp + lk;
+    const __bf16 *w2a = W.w2t + (long)n0 * HID + lk, *w2b = W.w2t + (long)n1 * HID + lk;
+    float b1[2], b2[2], w3[2];
+    if (kHeld) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            b1[c] = F->b1[c];
+            b2[c] = F->b2[c];
+            w3[c] = F->w3[c];
+        }
+    } else {
+        b1[0] = W.b1[n0];
+        b1[1] = W.b1[n1];
+        b2[0] = W.b2[n0];
+        b2[1] = W.b2[n1];
+        w3[0] = W.w3q[n0];
+        w3[1] = W.w3q[n1];
+    }
+    f32x4 acc[2][2];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // layer 1: (MR x kp) . (kp x 256)
+        for (int c = 0; c < 2; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int nk1 = W.kp / 32;
-    const __bf16 *w1 = W.w1t + (long)(64 * wave + lr) * W.kp + lk;
-    bf16x8 bn[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) bn[c] = *reinterpret_cast<const bf16x8 *>(w1 + (long)16 * c * W.kp);
-    for (int kt = 0; kt < nk1; ++kt) {
-        bf16x8 bc[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) bc[c] = bn[c];
-        if (kt + 1 < nk1) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                bn[c] = *reinterpret_cast<const bf16x8 *>(w1 + (long)16 * c * W.kp + 32 * (kt + 1));
-        }
+    auto step = [&](const bf16x8 &b0, const bf16x8 &b1v, int kt) {
         const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(X + lr * ldx + 32 * kt + lk);
         const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(X + (16 + lr) * ldx + 32 * kt + lk);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1v, acc[0][1], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1v, acc[1][1], 0, 0, 0);
+    };
+    bf16x8 nb0, nb1;
+    if (kHeld) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bc[c], acc[0][c], 0, 0, 0);
-            acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bc[c], acc[1][c], 0, 0, 0);
+        for (int kt = 0; kt < KT1; ++kt)
+            if (kt < nk1) step(F->w1[kt][0], F->w1[kt][1], kt);
+        for (int kt = KT1; kt < nk1; ++kt)   // features beyond the register-held tiles
+            step(*reinterpret_cast<const bf16x8 *>(w1a + 32 * kt),
+                 *reinterpret_cast<const bf16x8 *>(w1b + 32 * kt), kt);
+    } else {
+        nb0 = *reinterpret_cast<const bf16x8 *>(w1a);
+        nb1 = *reinterpret_cast<const bf16x8 *>(w1b);
+        for (int kt = 0; kt < nk1; ++kt) {
+            const bf16x8 b0 = nb0, b1v = nb1;
+            if (kt + 1 < nk1) {
+                nb0 = *reinterpret_cast<const bf16x8 *>(w1a + 32 * (kt + 1));
+                nb1 = *reinterpret_cast<const bf16x8 *>(w1b + 32 * (kt + 1));
+            }
+            step(b0, b1v, kt);
         }
     }
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int n = 64 * wave + 16 * c + lr;
-            const float bias = W.b1[n];
+        for (int c = 0; c < 2; ++c) {
+            const int n = c ? n1 : n0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int row = 16 * r + 4 * (lane >> 4) + i;
-                H[row * LDH + n] = __builtin_bit_cast(__bf16, bf16_rne(silu(acc[r][c][i] + bias)));
+                H[row * LDH + n] = __builtin_bit_cast(__bf16, bf16_rne(silu_fast(acc[r][c][i] + b1[c])));
             }
             acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
     __syncthreads();
-    // layer 2: (MR x 256) . (256 x 256), then . w3
-    const __bf16 *w2 = W.w2t + (long)(64 * wave + lr) * HID + lk;
+    if (!kHeld) {
+        nb0 = *reinterpret_cast<const bf16x8 *>(w2a);
+        nb1 = *reinterpret_cast<const bf16x8 *>(w2b);
+    }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) bn[c] = *reinterpret_cast<const bf16x8 *>(w2 + (long)16 * c * HID);
     for (int kt = 0; kt < HID / 32; ++kt) {
-        bf16x8 bc[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) bc[c] = bn[c];
-        if (kt + 1 < HID / 32) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                bn[c] = *reinterpret_cast<const bf16x8 *>(w2 + (long)16 * c * HID + 32 * (kt + 1));
+        bf16x8 b0, b1v;
+        if (kHeld) {
+            b0 = F->w2[kt][0];
+            b1v = F->w2[kt][1];
+        } else {
+            b0 = nb0;
+            b1v = nb1;
+            if (kt + 1 < HID / 32) {
+                nb0 = *reinterpret_cast<const bf16x8 *>(w2a + 32 * (kt + 1));
+                nb1 = *reinterpret_cast<const bf16x8 *>(w2b + 32 * (kt + 1));
+            }
         }
         const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(H + lr * LDH + 32 * kt + lk);
         const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(H + (16 + lr) * LDH + 32 * kt + lk);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bc[c], acc[0][c], 0, 0, 0);
-            acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bc[c], acc[1][c], 0, 0, 0);
-        }
-    }
-    float part[2][4];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) part[r][i] = 0.f;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int n = 64 * wave + 16 * c + lr;
-        const float bias = W.b2[n], w3 = W.w3q[n];
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) part[r][i] += silu(acc[r][c][i] + bias) * w3;
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1v, acc[0][1], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1v, acc[1][1], 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            float v = part[r][i];
+            float v = silu_fast(acc[r][0][i] + b2[0]) * w3[0] + silu_fast(acc[r][1][i] + b2[1]) * w3[1];
 #pragma unroll
             for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, 16);
             if (lr == 0) gp[wave * MR + 16 * r + 4 * (lane >> 4) + i] = v;
         }
     __syncthreads();
-    if (t < MR) gout[t] = W.b3[0] + ((gp[t] + gp[MR + t]) + (gp[2 * MR + t] + gp[3 * MR + t]));
+    if (t < MR) {
+        float v = 0.f;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) v += gp[w * MR + t];
+        gout[t] = W.b3[0] + v;
+    }
     __syncthreads();
 }
 
@@ -248,9 +350,9 @@ struct RowLds {
     int ldx;
 };
 
-__device__ RowLds carve(const LPArgs &A, char *base, int extra_bytes) {
+__device__ RowLds carve(const LPArgs &A, char *base) {
     RowLds L;
-    char *p = base + extra_bytes;
+    char *p = base;
     L.lsp = reinterpret_cast<uint8_t *>(p);
     p += (A.g.natoms_pad + 15) & ~15;
     L.ldx = A.kp + 8;
@@ -263,33 +365,39 @@ __device__ RowLds carve(const LPArgs &A, char *base, int extra_bytes) {
 }
 
 // ------------------------------------------------------------------ scoring
-// grid (ncand / MR, nrep): logits of the candidates of step stepc + 1 at the current state
-__global__ __launch_bounds__(256) void k_lp_score(LPArgs A) {
+// grid (ncand / MR, nrep), NT threads: logits of the candidates of step stepc + 1 at the
+// current state
+__global__ __launch_bounds__(NT) void k_lp_score(LPArgs A) {
     extern __shared__ __align__(16) char lds_raw[];
     __shared__ int2 rows[MR];
-    __shared__ int rxyz[6 * MR];
     __shared__ int du[MR];
-    __shared__ float gp[4 * MR], gout[MR];
+    __shared__ float gp[(NT / 64) * MR], gout[MR];
+    __shared__ uint8_t shell0[256];
     const Geo &g = A.g;
     const int s = blockIdx.y, t = threadIdx.x;
     const uint64_t n1 = A.stepc[s] + 1;
     const int rung = A.rung_of[s];
     const double beta = A.beta_slot[s];
     const uint64_t h0 = lp_stream(A.seed, n1, rung);
-    if (u01(splitmix64(h0 ^ kCoinSalt)) < A.eps) return;   // a uniform step: nothing to score
-    RowLds L = carve(A, lds_raw, 0);
+    if (lp_uniform(A, h0, beta)) return;   // a uniform step: nothing to score
+    RowLds L = carve(A, lds_raw);
     load_species(L.lsp, A.species + (long)s * g.natoms_pad, g.natoms_pad);
+    load_shell0(A, shell0);
     const int i0 = blockIdx.x * MR;
     if (t < MR) {
         int a, b;
         candidate(h0, i0 + t, g.natoms, a, b);
         rows[t] = make_int2(a, b);
-        site_xyz_of(g, a, rxyz[6 * t], rxyz[6 * t + 1], rxyz[6 * t + 2]);
-        site_xyz_of(g, b, rxyz[6 * t + 3], rxyz[6 * t + 4], rxyz[6 * t + 5]);
+        du[t] = 0;
     }
     __syncthreads();
-    build_rows(A, L.lsp, rows, rxyz, MR, -1, -1, L.occ, L.X, L.ldx, du);
-    mlp_rows(A.W, L.X, L.ldx, L.H, gp, gout);
+    // neighbour loads first, then the weights: the occupancy work waits only for the former
+    int site[IPT];
+    row_sites(A, rows, MR, site);
+    Frag F;
+    load_frag(A.W, F);
+    build_rows(A, L.lsp, shell0, rows, MR, site, -1, -1, L.occ, L.X, L.ldx, du);
+    mlp_rows<true>(A.W, &F, L.X, L.ldx, L.H, gp, gout);
     if (t < MR) {
         const bool valid = L.lsp[rows[t].x] != L.lsp[rows[t].y];
         const float lg = (float)(-A.lam * beta * 0.5 * (double)gout[t] + A.kappa * fabs((double)du[t]));
@@ -298,14 +406,17 @@ __global__ __launch_bounds__(256) void k_lp_score(LPArgs A) {
 }
 
 // ------------------------------------------------------------------ selection
+constexpr int NW = NT / 64;
+
 __device__ __forceinline__ float block_max_f(float v, float *red) {
     for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    v = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    float r = red[0];
+    for (int w = 1; w < NW; ++w) r = fmaxf(r, red[w]);
     __syncthreads();
-    return v;
+    return r;
 }
 
 __device__ __forceinline__ double block_sum_d(double v, double *red) {
@@ -313,12 +424,13 @@ __device__ __forceinline__ double block_sum_d(double v, double *red) {
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    v = (red[0] + red[1]) + (red[2] + red[3]);
+    double r = red[0];
+    for (int w = 1; w < NW; ++w) r += red[w];
     __syncthreads();
-    return v;
+    return r;
 }
 
-// inclusive scan over the 256 lanes (index order); returns (exclusive, inclusive)
+// inclusive scan over the workgroup's lanes (index order); returns (exclusive, inclusive)
 __device__ __forceinline__ void block_scan_d(double v, double *red, double &excl, double &incl) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double x = v;
@@ -337,44 +449,35 @@ __device__ __forceinline__ void block_scan_d(double v, double *red, double &excl
     __syncthreads();
 }
 
-// grid (nsplit, nrep): the proposal of step stepc + 1 (learned, or uniform on the coin),
-// its log q ratio (lead workgroup), and the gather of its count changes (all workgroups)
-__global__ __launch_bounds__(256) void k_lp_select(LPArgs A) {
+// grid (1, nrep), NT threads: the proposal of step stepc + 1 (learned, or uniform on the
+// coin / above beta_max), its log q ratio and its occupancy record; the slot kernel then
+// gathers its count changes
+__global__ __launch_bounds__(NT) void k_lp_select(LPArgs A) {
     extern __shared__ __align__(16) char lds_raw[];
     __shared__ int2 rows[MR];
-    __shared__ int rxyz[6 * MR];
     __shared__ int du[MR];
-    __shared__ float gp[4 * MR], gout[MR];
+    __shared__ float gp[NW * MR], gout[MR];
     __shared__ float lrev[MAXC];
     __shared__ int alist[MAXC];
     __shared__ int nalist, jsel;
-    __shared__ double redd[4];
-    __shared__ float redf[4];
+    __shared__ double redd[NW];
+    __shared__ float redf[NW];
+    __shared__ uint8_t shell0[256];
     __shared__ Prop P;
-    __shared__ float logqr_sh;
     const Geo &g = A.g;
     const int s = blockIdx.y, t = threadIdx.x;
-    const bool lead = blockIdx.x == 0;
-    int *hist = reinterpret_cast<int *>(lds_raw);
-    const int hist_bytes = (int)(sizeof(int) * ((g.nrow_tot + 3) & ~3));
-    RowLds L = carve(A, lds_raw, hist_bytes);
+    RowLds L = carve(A, lds_raw);
     const uint64_t n1 = A.stepc[s] + 1;
     const int rung = A.rung_of[s];
     const double beta = A.beta_slot[s];
     const uint64_t h0 = lp_stream(A.seed, n1, rung);
-    const bool uniform = u01(splitmix64(h0 ^ kCoinSalt)) < A.eps;
+    const bool uniform = lp_uniform(A, h0, beta);
     const float lg = (!uniform && t < A.ncand) ? A.logit[(long)s * A.ncand + t] : -__builtin_huge_valf();
     load_species(L.lsp, A.species + (long)s * g.natoms_pad, g.natoms_pad);
-    for (int q = t; q < g.nrow_tot; q += blockDim.x) hist[q] = 0;
-    const int stride = gridDim.x * blockDim.x;
-    const int first = blockIdx.x * blockDim.x + t;
-    int4 recs[kRecPrefetch];
-    if (!lead) prefetch_recs(g, A.ms_sten, first, stride, true, recs);
-    if (t == 0) {
-        logqr_sh = 0.f;
-        jsel = -1;
-    }
+    load_shell0(A, shell0);
+    if (t == 0) jsel = -1;
     __syncthreads();
+    float logqr = 0.f;
     if (uniform) {
         if (t == 0) P = propose(L.lsp, g.natoms, A.seed, n1, (uint64_t)rung);
         __syncthreads();
@@ -403,8 +506,9 @@ __global__ __launch_bounds__(256) void k_lp_select(LPArgs A) {
             }
         }
         __syncthreads();
-        if (lead && j >= 0) {
-            // Hastings ratio: the candidate set scored at x' = swap(x, a, b)
+        if (j >= 0) {
+            // Hastings ratio: the candidate set scored at x' = swap(x, a, b); only the
+            // candidates within the stencil radius of a or b change their occupancy
             const int a = P.a, b = P.b;
             int pj[3], qj[3];
             site_xyz_of(g, a, pj[0], pj[1], pj[2]);
@@ -431,12 +535,13 @@ __global__ __launch_bounds__(256) void k_lp_select(LPArgs A) {
                     int ca, cb;
                     candidate(h0, alist[c0 + t], g.natoms, ca, cb);
                     rows[t] = make_int2(ca, cb);
-                    site_xyz_of(g, ca, rxyz[6 * t], rxyz[6 * t + 1], rxyz[6 * t + 2]);
-                    site_xyz_of(g, cb, rxyz[6 * t + 3], rxyz[6 * t + 4], rxyz[6 * t + 5]);
                 }
+                if (t < MR) du[t] = 0;
                 __syncthreads();
-                build_rows(A, L.lsp, rows, rxyz, nr, a, b, L.occ, L.X, L.ldx, du);
-                mlp_rows(A.W, L.X, L.ldx, L.H, gp, gout);
+                int site[IPT];
+                row_sites(A, rows, nr, site);
+                build_rows(A, L.lsp, shell0, rows, nr, site, a, b, L.occ, L.X, L.ldx, du);
+                mlp_rows<false>(A.W, nullptr, L.X, L.ldx, L.H, gp, gout);
                 if (t < nr) {
                     const int ca = rows[t].x, cb = rows[t].y;
                     const bool valid = species_swapped(L.lsp, ca, a, b) != species_swapped(L.lsp, cb, a, b);
@@ -452,49 +557,34 @@ __global__ __launch_bounds__(256) void k_lp_select(LPArgs A) {
             const double Z2 = block_sum_d(p2, redd);
             const double D1 = block_sum_d(same ? p : 0.0, redd);
             const double D2 = block_sum_d(same ? p2 : 0.0, redd);
-            if (t == 0) logqr_sh = (float)((log(D2) - log(Z2)) - (log(D1) - log(Z)));
-            // occupancy of the chosen candidate at x, for its move record
-            if (A.pend_occ)
-                for (int q = t; q < A.nocc; q += blockDim.x)
-                    A.pend_occ[(long)s * A.noccp + q] =
-                        L.lsp[occ_site(g, A.ss_sten, a, b, pj[0], pj[1], pj[2], qj[0], qj[1], qj[2], q)];
-            __syncthreads();
+            logqr = (float)((log(D2) - log(Z2)) - (log(D1) - log(Z)));
         }
     }
     const Prop p = P;
-    if (lead) {
-        if (t == 0) {
-            int *o = A.prop_next + kPropInts * s;
-            o[0] = p.a;
-            o[1] = p.b;
-            o[2] = p.na;
-            o[3] = p.nb;
-            o[4] = p.sw;
-            o[5] = __float_as_int(logqr_sh);
-        }
-        if (uniform && A.pend_occ && p.sw) {
-            int ax, ay, az, bx, by, bz;
-            site_xyz_of(g, p.a, ax, ay, az);
-            site_xyz_of(g, p.b, bx, by, bz);
-            for (int q = t; q < A.nocc; q += blockDim.x)
-                A.pend_occ[(long)s * A.noccp + q] =
-                    L.lsp[occ_site(g, A.ss_sten, p.a, p.b, ax, ay, az, bx, by, bz, q)];
-        }
+    if (t == 0) {
+        int *o = A.prop_next + kPropInts * s;
+        o[0] = p.a;
+        o[1] = p.b;
+        o[2] = p.na;
+        o[3] = p.nb;
+        o[4] = p.sw;
+        o[5] = __float_as_int(logqr);
     }
-    // the lead workgroup's records are requested only now: its registers held the MLP
-    if (lead) prefetch_recs(g, A.ms_sten, first, stride, true, recs);
-    if (p.sw)
-        gather_swap(g, p, A.ss_sten, A.ms_sten, L.lsp, hist, first, stride, recs,
-                    A.dc_prop + (long)s * g.nrow_tot);
+    // occupancy of the proposal at x, for its move record
+    if (A.pend_occ && p.sw)
+        for (int q = t; q < A.nocc; q += blockDim.x)
+            A.pend_occ[(long)s * A.noccp + q] = L.lsp[occ_site_tbl(A, p.a, p.b, q)];
 }
 
 // ------------------------------------------------------------------ evaluation (tests)
-// grid ceil(n / MR): g of given occupancy rows through the scorer's MFMA path
-__global__ __launch_bounds__(256) void k_lp_eval(LPArgs A, int n, const uint8_t *__restrict__ occ_in,
-                                                 float *__restrict__ out) {
+// grid ceil(n / MR), NT threads: g of given occupancy rows through the scorer's MFMA path
+__global__ __launch_bounds__(NT) void k_lp_eval(LPArgs A, int n, const uint8_t *__restrict__ occ_in,
+                                                float *__restrict__ out) {
     extern __shared__ __align__(16) char lds_raw[];
-    __shared__ float gp[4 * MR], gout[MR];
-    RowLds L = carve(A, lds_raw, 0);
+    __shared__ float gp[NW * MR], gout[MR];
+    Frag F;
+    load_frag(A.W, F);
+    RowLds L = carve(A, lds_raw);
     const int r0 = blockIdx.x * MR, nr = min(MR, n - r0), t = threadIdx.x;
     for (int idx = t; idx < MR * A.noccp; idx += blockDim.x) {
         const int r = idx / A.noccp, q = idx - r * A.noccp;
@@ -519,7 +609,7 @@ __global__ __launch_bounds__(256) void k_lp_eval(LPArgs A, int n, const uint8_t 
         *reinterpret_cast<bf16x8 *>(L.X + r * L.ldx + 8 * c) = v;
     }
     __syncthreads();
-    mlp_rows(A.W, L.X, L.ldx, L.H, gp, gout);
+    mlp_rows<true>(A.W, &F, L.X, L.ldx, L.H, gp, gout);
     if (t < nr) out[r0 + t] = gout[t];
 }
 
@@ -730,7 +820,7 @@ LPArgs lp_args(chm_sampler *S) {
     A.beta_slot = S->beta_slot.as<double>();
     A.stepc = S->stepc.as<uint64_t>();
     A.ss_sten = S->ss_sten.as<int>();
-    A.ms_sten = S->stencil.as<int4>();
+    A.nbr = S->nbr_st.as<int>();
     A.seed = S->seed;
     A.ncand = L.ncand;
     A.nocc = L.nocc;
@@ -741,6 +831,7 @@ LPArgs lp_args(chm_sampler *S) {
     A.eps = L.eps;
     A.lam = L.lam;
     A.kappa = L.kappa;
+    A.beta_max = L.beta_max;
     const Offs o = offsets(L.kp);
     A.W.w1t = L.w1t.as<__bf16>();
     A.W.w2t = L.w2t.as<__bf16>();
@@ -758,9 +849,6 @@ size_t lds_rows(const chm_sampler *S) {
     const chm_lp &L = S->lp;
     return (size_t)((S->g.natoms_pad + 15) & ~15) + (size_t)MR * (L.kp + 8) * 2 + (size_t)MR * LDH * 2 +
            (size_t)MR * L.noccp;
-}
-size_t lds_select(const chm_sampler *S) {
-    return sizeof(int) * (size_t)((S->g.nrow_tot + 3) & ~3) + lds_rows(S);
 }
 
 int quantise(chm_sampler *S) {
@@ -854,7 +942,8 @@ int lp_prepare(chm_sampler *S) {
     L.nfeat = g.nsp * L.nocc;
     L.kp = (L.nfeat + 31) & ~31;
     L.nparam = offsets(L.kp).n;
-    if (L.kp > 1024) return chm::set_error("learned proposal: too many occupancy features"), CHM_E_ARG;
+    if (L.kp > 1024 || g.ncoord > 256)
+        return chm::set_error("learned proposal: too many occupancy features"), CHM_E_ARG;
     int rc;
     if ((rc = L.params.ensure(4 * (size_t)L.nparam)) || (rc = L.adam_m.ensure(4 * (size_t)L.nparam)) ||
         (rc = L.adam_v.ensure(4 * (size_t)L.nparam)) || (rc = L.w1t.ensure(2 * (size_t)L.kp * HID)) ||
@@ -864,17 +953,18 @@ int lp_prepare(chm_sampler *S) {
         (rc = L.pend_occ.ensure((size_t)S->nrep * L.noccp)))
         return rc;
     CHM_HIP(hipMemsetAsync(L.pend_occ.p, 0, L.pend_occ.bytes, S->ctx->stream));
-    if (lds_select(S) > 160 * 1024) return chm::set_error("learned proposal: LDS too small"), CHM_E_ARG;
+    if (lds_rows(S) > 140 * 1024) return chm::set_error("learned proposal: LDS too small"), CHM_E_ARG;
+    if (MR * L.nocc > IPT * NT) return chm::set_error("learned proposal: stencil too large"), CHM_E_ARG;
     hipFuncSetAttribute((const void *)k_lp_score, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows(S));
     hipFuncSetAttribute((const void *)k_lp_eval, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows(S));
     hipFuncSetAttribute((const void *)k_lp_select, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds_select(S));
+                        (int)lds_rows(S));
     return CHM_OK;
 }
 
 int lp_launch_score(chm_sampler *S) {
     LPArgs A = lp_args(S);
-    k_lp_score<<<dim3(S->lp.ncand / MR, S->nrep), 256, lds_rows(S), S->ctx->stream>>>(A);
+    k_lp_score<<<dim3(S->lp.ncand / MR, S->nrep), NT, lds_rows(S), S->ctx->stream>>>(A);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -882,8 +972,7 @@ int lp_launch_score(chm_sampler *S) {
 int lp_launch_select(chm_sampler *S, long long n) {
     LPArgs A = lp_args(S);
     A.prop_next = prow(S, n + 1);
-    A.dc_prop = drow(S, n + 1);
-    k_lp_select<<<dim3(S->nsplit, S->nrep), 256, lds_select(S), S->ctx->stream>>>(A);
+    k_lp_select<<<dim3(1, S->nrep), NT, lds_rows(S), S->ctx->stream>>>(A);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -925,13 +1014,15 @@ extern "C" int chm_lp_get_params(chm_sampler *S, float *flat) {
     return CHM_OK;
 }
 
-extern "C" int chm_lp_configure(chm_sampler *S, int mode, int ncand, double eps, double lam, double kappa) {
+extern "C" int chm_lp_configure(chm_sampler *S, int mode, int ncand, double eps, double lam, double kappa,
+                                double beta_max) {
     CHM_REQUIRE(S, CHM_E_ARG, "null sampler");
     CHM_REQUIRE(mode == 0 || mode == 1, CHM_E_ARG, "mode must be 0 (uniform) or 1 (learned)");
     CHM_REQUIRE(ncand >= MR && ncand <= MAXC && ncand % MR == 0, CHM_E_ARG,
                 "ncand must be a multiple of 32 in 32..256");
     CHM_REQUIRE(eps >= 0.0 && eps <= 1.0, CHM_E_ARG, "eps must be in [0, 1]");
     CHM_REQUIRE(std::isfinite(lam) && std::isfinite(kappa), CHM_E_ARG, "lam and kappa must be finite");
+    CHM_REQUIRE(beta_max > 0.0, CHM_E_ARG, "beta_max must be positive");
     CHM_REQUIRE(mode == 0 || S->lp.have_params, CHM_E_STATE, "set the network parameters first");
     CHM_HIP(hipSetDevice(S->ctx->device));
     CHM_HIP(hipStreamSynchronize(S->ctx->stream));
@@ -942,6 +1033,7 @@ extern "C" int chm_lp_configure(chm_sampler *S, int mode, int ncand, double eps,
     L.eps = eps;
     L.lam = lam;
     L.kappa = kappa;
+    L.beta_max = beta_max;
     return CHM_OK;
 }
 
@@ -995,7 +1087,7 @@ extern "C" int chm_lp_eval(chm_sampler *S, int n, const uint8_t *occ, float *g) 
     if ((rc = din.ensure((size_t)n * L.nocc)) || (rc = dout.ensure(4 * (size_t)n))) return rc;
     CHM_HIP(hipMemcpyAsync(din.p, occ, (size_t)n * L.nocc, hipMemcpyHostToDevice, st));
     LPArgs A = lp_args(S);
-    k_lp_eval<<<(unsigned)((n + MR - 1) / MR), 256, lds_rows(S), st>>>(A, n, din.as<uint8_t>(), dout.as<float>());
+    k_lp_eval<<<(unsigned)((n + MR - 1) / MR), NT, lds_rows(S), st>>>(A, n, din.as<uint8_t>(), dout.as<float>());
     CHM_LAUNCH_CHECK();
     CHM_HIP(hipMemcpyAsync(g, dout.p, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
     CHM_HIP(hipStreamSynchronize(st));
@@ -1029,6 +1121,46 @@ extern "C" int chm_lp_grad(chm_sampler *S, int n, const uint8_t *occ, const floa
     CHM_HIP(hipMemcpyAsync(grad, L.grad.p, 4 * (size_t)L.nparam, hipMemcpyDeviceToHost, st));
     CHM_HIP(hipMemcpyAsync(loss, L.loss.p, 4, hipMemcpyDeviceToHost, st));
     CHM_HIP(hipStreamSynchronize(st));
+    return CHM_OK;
+}
+
+extern "C" int chm_lp_profile(chm_sampler *S, int iters, double *ms) {
+    CHM_REQUIRE(S && S->inited && iters > 0 && ms, CHM_E_ARG, "bad arguments");
+    chm_lp &L = S->lp;
+    CHM_REQUIRE(L.have_params, CHM_E_STATE, "set the network parameters first");
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    hipStream_t st = S->ctx->stream;
+    chm::DevBuf pr;
+    int rc;
+    if ((rc = pr.ensure(sizeof(int) * (size_t)S->nrep * kPropInts))) return rc;
+    // both kernels on the current state; the selection writes into scratch rows, so the
+    // chain is not advanced (the scores and the pending occupancy are rewritten by the
+    // next real step)
+    LPArgs A = lp_args(S);
+    A.prop_next = pr.as<int>();
+    A.eps = 0.0;   // time the learned path on every slot
+    A.beta_max = 1e300;
+    hipEvent_t e0, e1, e2;
+    CHM_HIP(hipEventCreate(&e0));
+    CHM_HIP(hipEventCreate(&e1));
+    CHM_HIP(hipEventCreate(&e2));
+    CHM_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < iters; ++i)
+        k_lp_score<<<dim3(L.ncand / MR, S->nrep), NT, lds_rows(S), st>>>(A);
+    CHM_HIP(hipEventRecord(e1, st));
+    for (int i = 0; i < iters; ++i)
+        k_lp_select<<<dim3(1, S->nrep), NT, lds_rows(S), st>>>(A);
+    CHM_HIP(hipEventRecord(e2, st));
+    CHM_LAUNCH_CHECK();
+    CHM_HIP(hipEventSynchronize(e2));
+    float t1 = 0.f, t2 = 0.f;
+    CHM_HIP(hipEventElapsedTime(&t1, e0, e1));
+    CHM_HIP(hipEventElapsedTime(&t2, e1, e2));
+    ms[0] = t1 / iters;
+    ms[1] = t2 / iters;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(e2);
     return CHM_OK;
 }
 

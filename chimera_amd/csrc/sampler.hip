@@ -436,7 +436,7 @@ struct SlotArgs {
     uint64_t seed;
     int profile;
     int accept;             // 0: none, 1: step, 2: init (store chi^2, no decision)
-    int propose;            // 1: propose + gather the next step
+    int propose;            // 1: propose + gather the next step; 2: gather prop_next
     double nfit;
 };
 
@@ -479,7 +479,6 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     __shared__ double sums[chm::kMaxNsp][4];
     __shared__ double red[256];
     __shared__ int flag;
-    __shared__ double dE_sh;
     __shared__ Prop P;
     const int s = blockIdx.y;
     const int nwg = gridDim.x;
@@ -553,7 +552,6 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                     A.best_c2[s] = c2t;
                 }
             } else {
-                dE_sh = (c2t - c2p) * A.nfit;
                 if (pp[4]) {
                     // Metropolis-Hastings: pp[5] holds log q(x'->x) - log q(x->x')
                     const double x = beta * (c2t - c2p) * A.nfit - (double)__int_as_float(pp[5]);
@@ -591,6 +589,13 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                         }
                     }
                     flag = (cur < best) ? 2 : 0;
+                    if (A.rec_y) {
+                        // the move record of this step: energy change and outcome
+                        const long idx = (long)(((stepc - 1) * (uint64_t)gridDim.y + (uint64_t)s) %
+                                                (uint64_t)A.rec_cap);
+                        A.rec_y[idx] = pp[4] ? (float)((c2t - c2p) * A.nfit) : 0.f;
+                        A.rec_meta[idx] = pp[4] ? (rung | (acc << 16)) : -1;
+                    }
                 }
             }
             flag |= acc;
@@ -598,12 +603,8 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
         __syncthreads();
         const int acc = flag & 1;
         if (lead && A.rec_y && A.accept == 1) {
-            // the move record of this step: occupancy of the proposal, energy change, outcome
+            // ... and the occupancy of its proposal
             const long idx = (long)(((stepc - 1) * (uint64_t)gridDim.y + (uint64_t)s) % (uint64_t)A.rec_cap);
-            if (t == 0) {
-                A.rec_y[idx] = pp[4] ? (float)dE_sh : 0.f;
-                A.rec_meta[idx] = pp[4] ? (rung | (acc << 16)) : -1;
-            }
             if (pp[4])
                 for (int q = t; q < A.noccp; q += blockDim.x)
                     A.rec_occ[idx * A.noccp + q] = A.pend_occ[(long)s * A.noccp + q];
@@ -642,7 +643,11 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     if (!A.propose) return;
     // ------------------------------------------------ propose + gather of the next step
     __syncthreads();
-    if (t == 0) {
+    if (t == 0 && A.propose == 2) {
+        // the learned proposal (or its uniform fallback) chosen by the selection kernel
+        const int *o = A.prop_next + kPropInts * s;
+        P = Prop{o[0], o[1], o[2], o[3], o[4]};
+    } else if (t == 0) {
         P = propose(lsp, g.natoms, A.seed, stepc + 1, (uint64_t)rung);
         if (lead) {
             int *o = A.prop_next + kPropInts * s;
@@ -656,7 +661,7 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     }
     __syncthreads();
     const Prop p = P;
-    if (lead && A.pend_occ && p.sw) {
+    if (lead && A.pend_occ && p.sw && A.propose == 1) {
         int ax, ay, az, bx, by, bz;
         site_xyz_of(g, p.a, ax, ay, az);
         site_xyz_of(g, p.b, bx, by, bz);
@@ -845,7 +850,8 @@ int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanch
 int launch_propose(chm_sampler *S, long long h) {
     if (S->lp.mode) {
         int rc = lp_launch_score(S);
-        return rc ? rc : lp_launch_select(S, h);
+        if (!rc) rc = lp_launch_select(S, h);
+        return rc ? rc : launch_slot(S, 0, 2, h);
     }
     return launch_slot(S, 0, 1, h);
 }
@@ -992,7 +998,7 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
                     "site coordinates do not map back to their index");
     }
     // the neighbour offsets of site 0 form a stencil that must reproduce every site's list
-    std::vector<int> ss(g.ncoord);
+    std::vector<int> ss(g.ncoord), nbr_st;
     int r2max = 0;
     {
         auto xyz_of = [&](int q, int &x, int &y, int &z) {
@@ -1027,6 +1033,7 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
             r2max = std::max(r2max, off[3 * o] * off[3 * o] + off[3 * o + 1] * off[3 * o + 1] +
                                         off[3 * o + 2] * off[3 * o + 2]);
         std::vector<long long> want(g.ncoord), got(g.ncoord);
+        nbr_st.resize((size_t)g.natoms * g.ncoord);
         for (int q = 0; q < g.natoms; ++q) {
             int x, y, z;
             xyz_of(q, x, y, z);
@@ -1038,6 +1045,7 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
                 CHM_REQUIRE(b >= 0, CHM_E_ARG, "neighbour stencil leaves the lattice");
                 int site = (((X >> 1) * g.ncell[1] + (Y >> 1)) * g.ncell[2] + (Z >> 1)) * g.nbasis + b;
                 got[o] = (long long)site * 256 + shl[o];
+                nbr_st[(size_t)q * g.ncoord + o] = site;
                 want[o] = (long long)nbr_j[(long)q * g.ncoord + o] * 256 + nbr_shell[(long)q * g.ncoord + o];
             }
             std::sort(got.begin(), got.end());
@@ -1097,6 +1105,7 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         q0[2 * (q / g.ncol_e) + 1] += target[q] * target[q] * wc[q];
     }
     if ((rc = up(S->ss_sten, ss.data(), sizeof(int) * g.ncoord)) ||
+        (rc = up(S->nbr_st, nbr_st.data(), sizeof(int) * nbr_st.size())) ||
         (rc = up(S->site_xyz, site_xyz, sizeof(int) * g.natoms)) ||
         (rc = up(S->nbr_j, nbr_j, sizeof(int) * g.natoms * g.ncoord)) ||
         (rc = up(S->nbr_sh, nbr_shell, sizeof(int) * g.natoms * g.ncoord)) ||

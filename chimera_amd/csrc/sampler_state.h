@@ -11,6 +11,7 @@ struct chm_lp {
     double eps = 0.0;        // probability that a step uses the uniform proposal instead
     double lam = 1.0;        // logit = -lam * beta * g / 2 + kappa * |dU|
     double kappa = 0.0;
+    double beta_max = 1e300;  // slots at a larger beta (colder) use the uniform proposal
     int nocc = 0, noccp = 0; // occupancy entries per candidate (2 ncoord + 2), padded to 16
     int nfeat = 0, kp = 0;   // one-hot features nsp * nocc, padded to 32
     int nparam = 0;          // flat FP32 parameters
@@ -41,6 +42,7 @@ struct chm_sampler {
     double nfit = 1.0;
     size_t lds_k1 = 0;
     chm::DevBuf site_xyz, nbr_j, nbr_sh, ss_sten, stencil, basisR, basisK, target, wcol, q0, Tb;
+    chm::DevBuf nbr_st;   // (natoms, ncoord) neighbour sites in stencil order
     chm::DevBuf species, counts, dcounts, c2, best_c2, best_sp, rung_of, slot_of, beta, beta_slot,
         stepc, iex,
         prop, mtrial, mcur, partials, anchor, nanchor, exch_att, exch_acc, acc_rung, log, series,
@@ -70,7 +72,7 @@ inline double *c2row(chm_sampler *S, long long n) {
 }
 
 // learned.hip: score the candidates of step n + 1 (after step n was accepted), then
-// select one, compute its Hastings ratio and gather its count changes into drow(n + 1)
+// select one and compute its Hastings ratio into prow(n + 1) (the slot kernel gathers it)
 int lp_launch_score(chm_sampler *S);
 int lp_launch_select(chm_sampler *S, long long n);
 int lp_prepare(chm_sampler *S);   // buffers for the current geometry (at create)

@@ -35,11 +35,12 @@ HID = 256
 
 
 class LearnedProposal:
-    def __init__(self, sampler, seed=0, ncand=256, eps=0.1, lam=1.0, kappa=0.0):
+    def __init__(self, sampler, seed=0, ncand=256, eps=0.1, lam=1.0, kappa=0.0, beta_max=None):
         if sampler._S is None:
             raise RuntimeError("call init_state on the sampler first")
         self.S = sampler
         self.ncand, self.eps, self.lam, self.kappa = int(ncand), float(eps), float(lam), float(kappa)
+        self.beta_max = 1e300 if beta_max is None else float(beta_max)
         n = [C.c_int(0) for _ in range(4)]
         check(_lib.load().chm_lp_info(sampler._S, *[C.byref(v) for v in n]))
         self.nparam, self.nocc, self.nfeat, self.kp = (int(v.value) for v in n)
@@ -86,7 +87,7 @@ class LearnedProposal:
         return out
 
     # ------------------------------------------------------------ proposal mode
-    def enable(self, on=True, ncand=None, eps=None, lam=None, kappa=None):
+    def enable(self, on=True, ncand=None, eps=None, lam=None, kappa=None, beta_max=None):
         if ncand is not None:
             self.ncand = int(ncand)
         if eps is not None:
@@ -95,9 +96,11 @@ class LearnedProposal:
             self.lam = float(lam)
         if kappa is not None:
             self.kappa = float(kappa)
+        if beta_max is not None:
+            self.beta_max = float(beta_max)
         self.mode = 1 if on else 0
         check(_lib.load().chm_lp_configure(self.S._S, self.mode, self.ncand, self.eps, self.lam,
-                                           self.kappa))
+                                           self.kappa, self.beta_max))
 
     def disable(self):
         self.enable(False)
@@ -145,6 +148,12 @@ class LearnedProposal:
         check(_lib.load().chm_lp_grad(self.S._S, occ.shape[0], occ.ctypes.data, y.ctypes.data,
                                       w.ctypes.data, g.ctypes.data, loss.ctypes.data))
         return float(loss[0]), g
+
+    def profile(self, iters=100):
+        """Device ms per launch of the scorer and of the selection kernel (HIP events)."""
+        ms = np.zeros(2)
+        check(_lib.load().chm_lp_profile(self.S._S, int(iters), ms.ctypes.data))
+        return {"score_ms": float(ms[0]), "select_ms": float(ms[1])}
 
     def logits(self):
         """Logits (nrep, ncand) of the candidates scored last, slot order."""

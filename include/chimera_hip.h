@@ -247,8 +247,10 @@ int chm_lp_info(chm_sampler *s, int *nparam, int *nocc, int *nfeat, int *kp);
 int chm_lp_set_params(chm_sampler *s, const float *flat);
 int chm_lp_get_params(chm_sampler *s, float *flat);
 /* mode 0: uniform proposals (the plain sampler); mode 1: learned proposal with the
- * uniform one mixed in at probability eps.  ncand: multiple of 32 in 32..256. */
-int chm_lp_configure(chm_sampler *s, int mode, int ncand, double eps, double lam, double kappa);
+ * uniform one mixed in at probability eps and used on every slot whose beta exceeds
+ * beta_max (cold rungs).  ncand: multiple of 32 in 32..256. */
+int chm_lp_configure(chm_sampler *s, int mode, int ncand, double eps, double lam, double kappa,
+                     double beta_max);
 /* Move records: a ring of `capacity` (occupancy, nfit * dchi2, rung | accepted << 16)
  * entries written by every step (null moves: meta -1).  0 disables. */
 int chm_lp_set_records(chm_sampler *s, int capacity);
@@ -267,6 +269,11 @@ int chm_lp_grad(chm_sampler *s, int n, const uint8_t *occ, const float *y, const
  * clipped to [-clip, clip]; loss of the last minibatch. */
 int chm_lp_train(chm_sampler *s, int iters, int batch, float lr, int max_rung, float clip,
                  uint64_t seed, float *loss);
+/* Average device time (ms, HIP events on the sampler's stream) of the candidate scorer
+ * (ms[0]) and of the selection + Hastings kernel (ms[1]) over `iters` launches each on
+ * the current state, with every slot on the learned path; the chain does not advance
+ * (the selection writes into a scratch proposal row). */
+int chm_lp_profile(chm_sampler *s, int iters, double *ms);
 
 #ifdef __cplusplus
 }

@@ -24,8 +24,9 @@ kappa * |dU_i| (dU: change of unlike first-shell bonds), -inf where the two site
 hold the same species; j ~ softmax.  Reverse probability: the same candidate set,
 scored at x' = swap(x, a_j, b_j).  Acceptance: x = beta dE - log(q_rev / q_fwd),
 where q sums the probabilities of every candidate naming the same unordered pair.
-With probability eps a step uses the plain uniform proposal instead (a
-state-independent mixture of two kernels that each keep the posterior).
+With probability eps, and on every rung colder than beta_max, a step uses the plain
+uniform proposal instead (a state-independent mixture of two kernels that each keep
+the posterior).
 """
 from __future__ import annotations
 
@@ -299,11 +300,12 @@ class LearnedSampler:
     """The plain sampler (oracle.rmc.PlainSampler) with the learned proposal mixed in:
     the checker of the device sampler in learned mode."""
 
-    def __init__(self, plain, flat, stencil, ncand=256, eps=0.0, lam=1.0, kappa=0.0):
+    def __init__(self, plain, flat, stencil, ncand=256, eps=0.0, lam=1.0, kappa=0.0, beta_max=1e300):
         self.S = plain
         self.flat = np.asarray(flat, np.float32)
         self.st = stencil
         self.ncand, self.eps, self.lam, self.kappa = ncand, eps, lam, kappa
+        self.beta_max = beta_max
         self.margins = []
 
     def step(self):
@@ -315,7 +317,7 @@ class LearnedSampler:
             sp = S.species[c]
             u = accept_uniform(S.seed, S.stepc, c)
             margin = 1.0
-            if uses_uniform(S.seed, S.stepc, c, self.eps):
+            if uses_uniform(S.seed, S.stepc, c, self.eps) or S.beta[c] > self.beta_max:
                 a, b, na, nb, sw = propose(sp, S.seed, S.stepc, c)
                 lq = 0.0
             else:

@@ -131,8 +131,9 @@ def test_move_records_match_oracle(tern108):
 
 
 def test_learned_trajectory_matches_oracle(tern108):
-    """Device sampler with the learned proposal (eps 0.3 uniform mixture, tilt on) move by
-    move against the oracle's restatement with the same parameters.  A differing choice
+    """Device sampler with the learned proposal (eps 0.3 uniform mixture, tilt on, the two
+    coldest rungs uniform) move by move against the oracle's restatement with the same
+    parameters.  A differing choice
     is allowed only at a near tie of the selecting uniform or of the acceptance; the
     comparison stops there."""
     P, O = tern108
@@ -144,14 +145,15 @@ def test_learned_trajectory_matches_oracle(tern108):
     S.seed = seed
     S.set_recording(log_steps=nsteps)
     S.init_state(P.cfg0, P.cell.ideal)
-    lp.enable(ncand=64, eps=0.3, lam=1.0, kappa=0.2)
+    lp.enable(ncand=64, eps=0.3, lam=1.0, kappa=0.2, beta_max=10.0)
     _say(f"[run] device learned {nsteps} steps")
     S.run(nsteps, exchange_every=exch, log_every=10 ** 9)
     log = S.move_log(nsteps)
     lp.disable()
     R = OR.PlainSampler(O, nrep=S.nrep, seed=seed)
     R.init_state(P.cfg0)
-    L = LP.LearnedSampler(R, flat, _stencil(P), ncand=64, eps=0.3, lam=1.0, kappa=0.2)
+    L = LP.LearnedSampler(R, flat, _stencil(P), ncand=64, eps=0.3, lam=1.0, kappa=0.2,
+                          beta_max=10.0)
     _say("[run] oracle learned")
     L.run(nsteps, exchange_every=exch)
     margins = {(s, c): m for s, c, m in L.margins}
@@ -169,7 +171,7 @@ def test_learned_trajectory_matches_oracle(tern108):
             if int(gdev[2]) != acc:
                 assert abs(x + np.log(u)) < 1e-4, f"step {t} rung {rung}"
                 return
-    assert learned_steps > nsteps * S.nrep // 2
+    assert learned_steps > nsteps * S.nrep // 3
     assert np.array_equal(S.species, R.species)
     np.testing.assert_allclose(S.chi2, R.chi2, rtol=1e-9)
 
