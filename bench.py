@@ -58,6 +58,84 @@ def cpu_baseline(P, seconds=15.0, procs=1):
                       "FP64 NumPy oracle of the reference algorithm, 1 thread"}
 
 
+def ess_run(P, S, burn, steps, exchange_every, setup):
+    """Equilibrate from the workload's start configuration, then time `steps` steps and
+    take the alpha_1 ESS (smallest over the unlike pairs) of the likelihood-temperature
+    rung."""
+    from chimera_amd import workloads as W
+    from chimera_amd.ess import ess
+    S.init_state(P.cfg0, P.cell.ideal)
+    setup()
+    S.run(burn, exchange_every=exchange_every, log_every=10 ** 9)
+    a0 = S.accepted_per_rung().copy()
+    S.set_recording(log_steps=0, series_steps=steps)
+    P.fm.ctx.sync()
+    t0 = time.perf_counter()
+    S.run(steps, exchange_every=exchange_every, log_every=10 ** 9)
+    P.fm.ctx.sync()
+    dt = time.perf_counter() - t0
+    cr = int(S.collect_rungs()[0])
+    acc = (S.accepted_per_rung() - a0) / steps
+    al = S.alpha_series(steps, shell=0)
+    iu = np.triu_indices(P.nsp, 1)
+    e = min(ess(al[:, a, b]) for a, b in zip(*iu))
+    bsp, _, bc2 = S.best()
+    cnt = np.bincount(bsp, minlength=P.nsp)
+    chib = S.chi_k_of(bsp[None])[0] / np.maximum(cnt, 1)[:, None]
+    mb = P.ft.to_R(chib[None])[0]
+    rfac = W.r_factor(mb, P.target, P.mask, W.fit_scales(mb, P.target, P.mask, P.chan_edge))
+    return {"us_per_step": round(1e6 * dt / steps, 3), "alpha1_ess": round(e, 1),
+            "alpha1_ess_per_s": round(e / dt, 2), "accept_collect_rung": round(float(acc[cr]), 4),
+            "alpha1_mean": {f"{a}{b}": round(float(al[:, a, b].mean()), 5) for a, b in zip(*iu)},
+            "best_chi2": round(float(bc2), 6), "best_r_factor": float(f"{rfac:.4g}")}
+
+
+def learned_vs_uniform(P, S, args):
+    """Train the learned proposal on move records of the uniform sampler, then compare the
+    two samplers' alpha_1 ESS/s under the same protocol."""
+    from chimera_amd.learned import LearnedProposal
+    lp = LearnedProposal(S, seed=0)
+    S.init_state(P.cfg0, P.cell.ideal)
+    lp.record(args.train_steps * S.nrep)
+    t0 = time.perf_counter()
+    S.run(args.train_steps, exchange_every=args.exchange_every, log_every=10 ** 9)
+    P.fm.ctx.sync()
+    t_rec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss1 = lp.train(iters=args.train_iters // 2, batch=1024, lr=1e-3, seed=1)
+    loss2 = lp.train(iters=args.train_iters - args.train_iters // 2, batch=1024, lr=3e-4, seed=2)
+    t_train = time.perf_counter() - t0
+    nv, occ, y, meta = lp.records(min(lp.rec_cap, 200000))
+    ok = meta >= 0
+    g = lp.eval(occ[ok][:20000])
+    yy = np.clip(y[ok][:20000], -30, 30)
+    corr = float(np.corrcoef(g, yy)[0, 1])
+    lp.record(0)
+    settings = dict(ncand=args.lp_ncand, lam=1.0, kappa=args.lp_kappa, eps=0.05,
+                    beta_max=args.lp_beta_max)
+    print(f"[bench] learned: trained {args.train_iters} its, loss {loss2:.3f}, corr {corr:.3f}; "
+          "ESS runs", file=sys.stderr, flush=True)
+    uni = ess_run(P, S, args.ess_burn, args.ess_steps, args.exchange_every, lambda: lp.disable())
+    lrn = ess_run(P, S, args.ess_burn, args.ess_steps, args.exchange_every,
+                  lambda: lp.enable(**settings))
+    prof = lp.profile(50)
+    lp.disable()
+    # MFMA work of one scorer launch: every slot's candidates through both layers
+    flops = 2.0 * S.nrep * args.lp_ncand * (lp.kp * 256 + 256 * 256 + 256)
+    return {"settings": settings,
+            "training": {"records": int(nv), "record_steps": args.train_steps,
+                         "record_s": round(t_rec, 2), "iters": args.train_iters, "batch": 1024,
+                         "train_s": round(t_train, 2), "loss": round(loss2, 4),
+                         "loss_first_half": round(loss1, 4), "fit_corr": round(corr, 4)},
+            "uniform": uni, "learned": lrn,
+            "alpha1_ess_per_s_ratio": round(lrn["alpha1_ess_per_s"] / uni["alpha1_ess_per_s"], 3),
+            "alpha1_ess_per_step_ratio": round(lrn["alpha1_ess"] / uni["alpha1_ess"], 3),
+            "kernels_ms": {k: round(v, 5) for k, v in prof.items()},
+            "scorer_roofline": {"bound": "mfma", "achieved": round(flops / (prof["score_ms"] * 1e-3) / 1e12, 3),
+                                "peak": 2500.0, "unit": "TFLOP/s",
+                                "frac": round(flops / (prof["score_ms"] * 1e-3) / 1e12 / 2500.0, 6)}}
+
+
 def recorded_traffic(kernel, nsplit, workload):
     """HBM-side bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy of known size), when it was
@@ -90,6 +168,15 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=200)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "f64"])
     ap.add_argument("--reanchor-every", type=int, default=512)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-vs-uniform proposal comparison")
+    ap.add_argument("--ess-steps", type=int, default=100000)
+    ap.add_argument("--ess-burn", type=int, default=10000)
+    ap.add_argument("--train-steps", type=int, default=40000)
+    ap.add_argument("--train-iters", type=int, default=8000)
+    ap.add_argument("--lp-ncand", type=int, default=256)
+    ap.add_argument("--lp-kappa", type=float, default=1.0)
+    ap.add_argument("--lp-beta-max", type=float, default=3.0)
     ap.add_argument("--nsplit", type=int, default=0,
                     help="workgroups per replica in the slot kernel (0: autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -191,6 +278,11 @@ def main():
     mb = P.ft.to_R(chib[None])[0]
     rfac = W.r_factor(mb, P.target, P.mask, W.fit_scales(mb, P.target, P.mask, P.chan_edge))
     nanchor = int(S.reanchors().sum()) if args.precision == "bf16x3" else 0
+    lvu = None
+    if not args.no_learned:
+        lvu = learned_vs_uniform(P, S, args)
+        print(f"[bench] learned/uniform alpha_1 ESS/s: {lvu['alpha1_ess_per_s_ratio']}",
+              file=sys.stderr, flush=True)
     out = {
         "metric": "RMC swap moves/s & alpha_1 ESS/s at fixed R-factor, learned vs uniform proposal",
         "value": round(moves, 1),
@@ -215,6 +307,7 @@ def main():
         "alpha1_ess_per_s": round(ess_min / dt, 2),
         "alpha1_mean": {f"{a}{b}": round(float(al[:, a, b].mean()), 5) for a, b in zip(*iu)},
         "fp64_reanchors": nanchor,
+        "learned_vs_uniform": lvu,
         "autotune": {"nsplit": tuned["nsplit"],
                      "step_ms": {k: round(v, 5) for k, v in tuned["times_ms"].items()}},
     }
