@@ -166,8 +166,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=200)
-    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "f64"])
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "f64", "f32"])
     ap.add_argument("--reanchor-every", type=int, default=512)
+    ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
+                    help="only run the precision study (STEPS steps per mode) and exit")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-uniform proposal comparison")
     ap.add_argument("--ess-steps", type=int, default=100000)
@@ -192,6 +194,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        return
+    if args.precision_study:
+        from chimera_amd import precision_study
+        precision_study.main(["--steps", str(args.precision_study)])
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (replicas shard across GPUs by "
@@ -242,6 +248,8 @@ def main():
     kern = {"slot_step_ms": k_slot, "trial_transform_ms": k2, "step_ms": kstep}
     if args.precision == "bf16x3":
         peak2, kname = 2500.0 / 3.0, "k_gemm_delta_bf16"        # split-BF16: BF16 dense / 3
+    elif args.precision == "f32":
+        peak2, kname = 157.3, "k_gemm_delta_f32"                 # FP32 (vector + matrix) peak
     else:
         peak2, kname = FP64_PEAK_TFLOPS, "k_gemm_abs_f64"
     rooflines = {
@@ -277,7 +285,7 @@ def main():
     chib = S.chi_k_of(bsp[None])[0] / np.maximum(cnt, 1)[:, None]
     mb = P.ft.to_R(chib[None])[0]
     rfac = W.r_factor(mb, P.target, P.mask, W.fit_scales(mb, P.target, P.mask, P.chan_edge))
-    nanchor = int(S.reanchors().sum()) if args.precision == "bf16x3" else 0
+    nanchor = int(S.reanchors().sum()) if args.precision != "f64" else 0
     lvu = None
     if not args.no_learned:
         lvu = learned_vs_uniform(P, S, args)
@@ -290,8 +298,9 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * dt / args.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 chi^2; " + ("bf16x3 MFMA delta + f64 accumulator" if args.precision == "bf16x3"
-                                    else "f64 transform"),
+        "dtype": "f64 chi^2; " + {"bf16x3": "bf16x3 MFMA delta + f64 accumulator",
+                                  "f32": "f32 delta + f64 accumulator",
+                                  "f64": "f64 transform"}[args.precision],
         "data": "synthetic (seeded tables, seeded reference configuration + noise)",
         "config": {"workload": args.workload, "natoms": int(P.cell.natoms),
                    "replicas": int(S.nrep), "edges": int(P.nsp), "channels": int(P.nchan),

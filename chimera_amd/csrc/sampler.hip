@@ -110,6 +110,7 @@ struct TileOut {
     const double *wcol;     // (ncol_tot) 1/sigma^2 of each column
     double *mtrial;         // (nrep, ncol_tot)
     double *partials;       // (nrep, nsp, nct, 4)
+    int acc32;              // the running chi(R) is an FP32 accumulator (precision study)
 };
 
 // lanes t..t+15 (t % 16 == 0) hold the 16 columns of one slot
@@ -210,9 +211,10 @@ __global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int
         for (int z = 0; z < KS; ++z) sum += P[(z * RB + m) * NCW + cl];
         if (reanchor) {
             if (due[m] && c0 + cl < g.ncol_e)
-                O.anchor_out[(long)(r0 + m) * g.ncol_tot + (long)e * g.ncol_e + c0 + cl] = sum;
+                O.anchor_out[(long)(r0 + m) * g.ncol_tot + (long)e * g.ncol_e + c0 + cl] =
+                    O.acc32 ? (double)(float)sum : sum;
         } else {
-            tile_epilogue(g, O, nrep, r0 + m, e, ct, c0 + cl, sum);
+            tile_epilogue(g, O, nrep, r0 + m, e, ct, c0 + cl, O.acc32 ? (double)(float)sum : sum);
         }
     }
     if (!reanchor && O.bump && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < RB &&
@@ -329,7 +331,8 @@ __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const 
         const float d = red[0][m][cl] + red[1][m][cl] + red[2][m][cl] + red[3][m][cl];
         const int slot = r0 + m, col = ct * NCW + cl;
         const bool valid = slot < nrep && col < g.ncol_e;
-        const double mp = valid ? mc_pre[h] + (double)d : 0.0;
+        double mp = valid ? mc_pre[h] + (double)d : 0.0;
+        if (O.acc32) mp = (double)(float)mp;
         double p1 = 0.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
         if (valid) {
             O.mtrial[(long)slot * g.ncol_tot + (long)e * g.ncol_e + col] = mp;
@@ -356,6 +359,69 @@ __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const 
     }
     if (O.bump && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
         O.stepc[r0 + threadIdx.x] += 1;
+}
+
+// Incremental FP32: delta(R) = delta_counts @ basis_R with basis_R rounded to FP32 and FP32
+// FMA (no MFMA: the reference transform in FP32, for the precision study), added to the
+// running chi(R).  Tiling as k_gemm_abs_f64: 16 columns x 16 K slices, 32 slots per lane.
+__global__ __launch_bounds__(256) void k_gemm_delta_f32(Geo g, int nrep, const int *__restrict__ dcounts,
+                                                         const float *__restrict__ basis32,
+                                                         const double *__restrict__ mcur, TileOut O) {
+    extern __shared__ float fl[];
+    const int e = blockIdx.z;
+    const int r0 = blockIdx.y * RB;
+    const int ct = blockIdx.x;
+    const int c0 = ct * NCW;
+    const int K = g.nrow_e;
+    const int ld = RB + 1;
+    for (int q = threadIdx.x; q < RB * K; q += blockDim.x) {
+        const int m = q / K, k = q % K;
+        const int slot = r0 + m;
+        fl[k * ld + m] = (slot < nrep) ? (float)dcounts[(long)slot * g.nrow_tot + (long)e * g.rstride + k] : 0.f;
+    }
+    __syncthreads();
+    const int tn = threadIdx.x % NCW, ks = threadIdx.x / NCW;
+    const int col = c0 + tn;
+    const int kc = (K + KS - 1) / KS;
+    const int kb = ks * kc, ke = min(K, kb + kc);
+    float acc[RB];
+#pragma unroll
+    for (int m = 0; m < RB; ++m) acc[m] = 0.f;
+    if (col < g.ncol_e) {
+        const float *Bc = basis32 + (long)e * K * g.ncol_e + col;
+        for (int k = kb; k < ke; ++k) {
+            const float bv = Bc[(long)k * g.ncol_e];
+            const float *Ak = fl + k * ld;
+#pragma unroll
+            for (int m = 0; m < RB; ++m) acc[m] = fmaf(Ak[m], bv, acc[m]);
+        }
+    }
+    __syncthreads();
+    float *P = fl;
+#pragma unroll
+    for (int m = 0; m < RB; ++m) P[(ks * RB + m) * NCW + tn] = acc[m];
+    __syncthreads();
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int m = t / NCW + 16 * h, cl = t % NCW;
+        float d = 0.f;
+        for (int z = 0; z < KS; ++z) d += P[(z * RB + m) * NCW + cl];
+        const int slot = r0 + m, c = c0 + cl;
+        double mp = 0.0;
+        if (slot < nrep && c < g.ncol_e) {
+            mp = mcur[(long)slot * g.ncol_tot + (long)e * g.ncol_e + c] + (double)d;
+            if (O.acc32) mp = (double)(float)mp;
+        }
+        tile_epilogue(g, O, nrep, slot, e, ct, c, mp);
+    }
+    if (O.bump && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
+        O.stepc[r0 + threadIdx.x] += 1;
+}
+
+__global__ void k_to_f32(long n, const double *__restrict__ in, float *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
 }
 
 // basis_R (nsp, nrow_e, ncol_e) FP64 -> three bf16 planes in MFMA B-operand order:
@@ -755,6 +821,7 @@ TileOut tile_out(chm_sampler *S) {
     O.wcol = S->wcol.as<double>();
     O.mtrial = S->mtrial.as<double>();
     O.partials = S->partials.as<double>();
+    O.acc32 = S->acc32;
     return O;
 }
 
@@ -878,7 +945,20 @@ int launch_reanchor(chm_sampler *S) {
     return CHM_OK;
 }
 
+int launch_gemm_delta32(chm_sampler *S, const int *dc, int bump) {
+    const Geo &g = S->g;
+    dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
+    TileOut O = tile_out(S);
+    O.bump = bump;
+    const size_t lds = std::max(sizeof(float) * (size_t)g.nrow_e * (RB + 1), sizeof(float) * (size_t)KS * RB * NCW);
+    k_gemm_delta_f32<<<grid, 256, lds, S->ctx->stream>>>(g, S->nrep, dc, S->basisR32.as<float>(),
+                                                          S->mcur.as<double>(), O);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
 int launch_gemm(chm_sampler *S, const int *dc, int bump) {
+    if (S->incremental == 2) return launch_gemm_delta32(S, dc, bump);
     return S->incremental ? launch_gemm_delta(S, dc, bump) : launch_gemm_abs(S, dc, bump);
 }
 
@@ -1575,12 +1655,28 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
 
 extern "C" int chm_sampler_set_precision(chm_sampler *S, int incremental, int reanchor_every) {
     CHM_REQUIRE(S, CHM_E_ARG, "null sampler");
-    CHM_REQUIRE(incremental == 0 || incremental == 1, CHM_E_ARG, "incremental must be 0 or 1");
+    const int mode = incremental & 15, acc32 = (incremental & 16) ? 1 : 0;
+    CHM_REQUIRE((incremental & ~31) == 0 && mode >= 0 && mode <= 2 && (mode > 0 || !acc32), CHM_E_ARG,
+                "precision mode must be 0, 1 or 2 (+16 for an FP32 accumulator with 1, 2)");
     CHM_REQUIRE(reanchor_every >= 0, CHM_E_ARG, "negative re-anchor period");
     CHM_HIP(hipSetDevice(S->ctx->device));
     CHM_HIP(hipStreamSynchronize(S->ctx->stream));
     drop_graph(S);
-    S->incremental = incremental;
+    if (mode == 2 && !S->basisR32.p) {
+        const Geo &g = S->g;
+        const long n = (long)g.nsp * g.nrow_e * g.ncol_e;
+        int rc = S->basisR32.ensure(sizeof(float) * n);
+        if (rc) return rc;
+        k_to_f32<<<(unsigned)((n + 255) / 256), 256, 0, S->ctx->stream>>>(n, S->basisR.as<double>(),
+                                                                         S->basisR32.as<float>());
+        CHM_LAUNCH_CHECK();
+        CHM_HIP(hipStreamSynchronize(S->ctx->stream));
+        hipFuncSetAttribute((const void *)k_gemm_delta_f32, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)std::max(sizeof(float) * (size_t)g.nrow_e * (RB + 1),
+                                          sizeof(float) * (size_t)KS * RB * NCW));
+    }
+    S->incremental = mode;
+    S->acc32 = acc32;
     S->reanchor_every = reanchor_every;
     S->inited = false;   // the accumulator is built by the next init
     return CHM_OK;

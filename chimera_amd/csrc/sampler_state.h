@@ -35,7 +35,8 @@ struct chm_sampler {
     chm_ctx *ctx = nullptr;
     chm_smp::Geo g{};
     int nrep = 0, nk = 0, nsplit = 1, profile = 1;
-    int incremental = 0, reanchor_every = 512;
+    int incremental = 0, reanchor_every = 512;   // 0 exact FP64, 1 split-BF16 delta, 2 FP32 delta
+    int acc32 = 0;                               // running chi(R) rounded to FP32 (study only)
     uint64_t seed = 0;
     bool inited = false;
     long long hstep = 0;
@@ -43,6 +44,7 @@ struct chm_sampler {
     size_t lds_k1 = 0;
     chm::DevBuf site_xyz, nbr_j, nbr_sh, ss_sten, stencil, basisR, basisK, target, wcol, q0, Tb;
     chm::DevBuf nbr_st;   // (natoms, ncoord) neighbour sites in stencil order
+    chm::DevBuf basisR32; // FP32 copy of basis_R (FP32-transform mode only)
     chm::DevBuf species, counts, dcounts, c2, best_c2, best_sp, rung_of, slot_of, beta, beta_slot,
         stepc, iex,
         prop, mtrial, mcur, partials, anchor, nanchor, exch_att, exch_acc, acc_rung, log, series,

@@ -46,15 +46,20 @@ class RMCSampler:
                  swap_frac=0.75, disp_sigma=0.015, max_disp=0.30, nsp=None,
                  collect_after=None, collect_every=25, adapt_every=0, target_accept=0.35,
                  profile_scale=True, collect_beta=0.5, n_collect=4, energy_prior=None,
-                 ms_library=None, precision="f64", reanchor_every=512):
+                 ms_library=None, precision="f64", reanchor_every=512, accumulator="f64"):
         """Arguments as methods/method_a_rmc.py:64-69, plus `precision`: "f64" (the trial
-        chi(R) recomputed from the integer counts in FP64 every step) or "bf16x3" (the
-        change of chi(R) in split-BF16 MFMA with FP32 accumulation, added to an FP64
-        running chi(R) re-anchored in FP64 every `reanchor_every` accepted moves of a
-        replica).  chi^2 and the acceptance test are FP64 in both."""
-        if precision not in ("f64", "bf16x3"):
-            raise ValueError("precision must be 'f64' or 'bf16x3'")
+        chi(R) recomputed from the integer counts in FP64 every step), "bf16x3" (the
+        change of chi(R) in split-BF16 MFMA with FP32 accumulation, added to a running
+        chi(R) re-anchored in FP64 every `reanchor_every` accepted moves of a replica, 0 =
+        never) or "f32" (the change in FP32 FMA, same running chi(R)); `accumulator`:
+        "f64" (default) or "f32" (the running chi(R) rounded to FP32 after every update;
+        for precision studies).  chi^2 and the acceptance test are FP64 in every mode."""
+        if precision not in ("f64", "bf16x3", "f32"):
+            raise ValueError("precision must be 'f64', 'bf16x3' or 'f32'")
+        if accumulator not in ("f64", "f32") or (precision == "f64" and accumulator == "f32"):
+            raise ValueError("accumulator must be 'f64' or 'f32' (an incremental precision only)")
         self.precision = precision
+        self.accumulator = accumulator
         self.reanchor_every = int(reanchor_every)
         if swap_frac != 1.0:
             raise ValueError("the device sampler makes swap moves only: pass swap_frac=1.0 "
@@ -193,6 +198,7 @@ class RMCSampler:
             outR = dev.apply_edge(e, scaled)                # (nrow_e, nkw, nR, 2)
             basis_R[e] = outR[:, :, self.mask, :].reshape(nrow_e, ncol_e)
         self._basis_k = basis_k
+        self._basis_R = basis_R
         # geometry
         xyz, box, ncell, nbasis, bpar = self._lattice_geometry()
         g = _lib.Geom()
@@ -243,8 +249,10 @@ class RMCSampler:
                     self._build_hooks.remove(ref)
                 else:
                     hook(self)
-        check(_lib.load().chm_sampler_set_precision(self._S, int(self.precision == "bf16x3"),
-                                                    self.reanchor_every))
+        mode = {"f64": 0, "bf16x3": 1, "f32": 2}[self.precision]
+        if getattr(self, "accumulator", "f64") == "f32":
+            mode |= 16
+        check(_lib.load().chm_sampler_set_precision(self._S, mode, self.reanchor_every))
         check(_lib.load().chm_sampler_init(self._S, species0, f64(self._beta_rung),
                                            int(self.seed) & ((1 << 64) - 1)))
         self.samples = []
@@ -314,6 +322,12 @@ class RMCSampler:
     def running_chiR(self):
         """(nrep, nsp*nkw*nwin*2) running chi(R) accumulator, slot order."""
         return self._read(13, (self.nrep, self.nsp * self.nkw * self.nwin * 2), np.float64)
+
+    def exact_chiR(self):
+        """(nrep, nsp*nkw*nwin*2) chi(R) of every slot recomputed on the host in FP64 from
+        its integer counts (the reference value of the running accumulator)."""
+        C = self.counts_slot_order().astype(np.float64).reshape(self.nrep, self.nsp, self.nrow_e)
+        return np.concatenate([C[:, e] @ self._basis_R[e] for e in range(self.nsp)], axis=1)
 
     def counts_slot_order(self):
         rung_of = self._read(5, (self.nrep,), np.int32)

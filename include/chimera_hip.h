@@ -198,7 +198,10 @@ int chm_sampler_chi2(chm_sampler *s, int nrows, const double *chiR, double *c2);
  * in FP64 every step (exact, no drift).  incremental = 1: delta(R) = delta_counts @
  * basis_R in split-BF16 (3 planes) on MFMA with FP32 accumulation, added to an FP64
  * running chi(R) that is re-anchored (recomputed in FP64 from the integer counts) every
- * reanchor_every accepted moves of a replica (0 = never).  chi^2 is FP64 in both.
+ * reanchor_every accepted moves of a replica (0 = never).  incremental = 2: the same with
+ * delta(R) in FP32 FMA against an FP32 copy of basis_R.  Adding 16 to 1 or 2 rounds the
+ * running chi(R) to FP32 after every update (an FP32 accumulator, for precision studies).
+ * chi^2 is FP64 in every mode.
  * Takes effect at the next chm_sampler_init.  No counterpart in the reference (its
  * transform is an FP64 cuFFT of the full trial chi every step, fourier.py:75-80). */
 int chm_sampler_set_precision(chm_sampler *s, int incremental, int reanchor_every);

@@ -202,6 +202,27 @@ def test_sampler_trajectory_bf16x3(tern108):
         S.init_state(P.cfg0, P.cell.ideal)
 
 
+@pytest.mark.parametrize("prec,acc,rean,tol", [("bf16x3", "f64", 4, 2e-6), ("f32", "f64", 4, 2e-6),
+                                              ("bf16x3", "f32", 0, 1e-4), ("f32", "f32", 0, 1e-4)])
+def test_precision_modes_running_chiR(tern108, prec, acc, rean, tol):
+    """Split-BF16 and FP32 transforms, FP64 and FP32 accumulators: the running chi(R) stays
+    within tol of the exact FP64 transform of the integer counts, and the chain follows
+    the FP64 oracle move by move up to a near tie."""
+    P, O = tern108
+    S = P.sampler
+    S.precision, S.accumulator, S.reanchor_every = prec, acc, rean
+    try:
+        _say(f"[test] precision {prec} acc {acc} reanchor {rean}")
+        _compare_trajectory(P, O, 60, 10, seed=9, tie_tol=1e-3, c2_rtol=tol)
+        m, ex = S.running_chiR(), S.exact_chiR()
+        assert np.abs(m - ex).max() <= tol * np.abs(ex).max()
+        if rean:
+            assert S.reanchors().sum() > 0
+    finally:
+        S.precision, S.accumulator, S.reanchor_every = "f64", "f64", 512
+        S.init_state(P.cfg0, P.cell.ideal)
+
+
 def test_sampler_trajectory_binary_ss(bin500):
     P, O = bin500
     _compare_trajectory(P, O, 40, 5, seed=11)
