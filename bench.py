@@ -136,6 +136,62 @@ def learned_vs_uniform(P, S, args):
                                 "frac": round(flops / (prof["score_ms"] * 1e-3) / 1e12 / 2500.0, 6)}}
 
 
+def batch_bench(args):
+    """Uneven problems stepped together (one launch per kernel per step, one hipGraph per
+    exchange period): aggregate replica swap moves/s."""
+    from chimera_amd import batch as BT
+    log = lambda m: print(m, file=sys.stderr, flush=True)       # noqa: E731
+    specs = BT.batch_specs(args.batch)
+    t0 = time.perf_counter()
+    probs = BT.build(specs)
+    t_build = time.perf_counter() - t0
+    log(f"[bench] built {len(probs)} problems in {t_build:.1f} s")
+    B = BT.Batch(probs, precision=args.precision, reanchor_every=args.reanchor_every, log=log)
+    B.run(args.warmup, exchange_every=args.exchange_every)
+    steps = args.steps
+    dt = B.run(steps, exchange_every=args.exchange_every)
+    nrep = B.replicas
+    moves = nrep * steps / dt
+    prof = B.profile(50)
+    sites = [int(P.cell.natoms) for P in probs]
+    nsplits = [B._split(P.sampler) for P in probs]
+    # the same problems stepped one after another (each its own launches and graphs)
+    B.close()
+    seq_steps = max(200, steps // 10)
+    t_seq = 0.0
+    for P in probs:
+        S = P.sampler
+        S.run(args.exchange_every, exchange_every=args.exchange_every, log_every=10 ** 9)   # graphs
+        P.fm.ctx.sync()
+        t1 = time.perf_counter()
+        S.run(seq_steps, exchange_every=args.exchange_every, log_every=10 ** 9)
+        P.fm.ctx.sync()
+        t_seq += time.perf_counter() - t1
+    seq_moves = nrep * seq_steps / t_seq
+    out = {
+        "metric": "aggregate replica swap moves/s, batched uneven RMC problems",
+        "value": round(moves, 1), "unit": "replica swap moves/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(1e3 * dt / steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 chi^2; " + {"bf16x3": "bf16x3 MFMA delta + f64 accumulator",
+                                  "f32": "f32 delta + f64 accumulator", "f64": "f64 transform"}[args.precision],
+        "data": "synthetic (seeded tables, seeded reference configurations + noise)",
+        "config": {"workload": f"batch of {len(probs)} uneven problems", "problems": len(probs),
+                   "replicas": nrep, "sites_total": int(sum(sites)),
+                   "sites": {str(n): sites.count(n) for n in sorted(set(sites))},
+                   "lattices": {lat: sum(P.w.lattice == lat for P in probs) for lat in ("fcc", "bcc")},
+                   "edges": {str(k): sum(len(P.w.edges or range(P.nsp)) == k for P in probs) for k in (1, 2, 3)},
+                   "exchange_every": args.exchange_every},
+        "kernels_ms": {k: round(v, 5) for k, v in prof.items()},
+        "sequential_moves_per_s": round(seq_moves, 1),
+        "batched_over_sequential": round(moves / seq_moves, 2),
+        "autotune": B.tuned,
+        "nsplit": nsplits,
+        "build_s": round(t_build, 1),
+    }
+    print(json.dumps(out))
+
+
 def recorded_traffic(kernel, nsplit, workload):
     """HBM-side bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy of known size), when it was
@@ -168,6 +224,8 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=200)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "f64", "f32"])
     ap.add_argument("--reanchor-every", type=int, default=512)
+    ap.add_argument("--batch", type=int, default=0, metavar="NPROB",
+                    help="only run the batched benchmark over NPROB uneven problems and exit")
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="only run the precision study (STEPS steps per mode) and exit")
     ap.add_argument("--no-learned", action="store_true",
@@ -194,6 +252,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        return
+    if args.batch:
+        batch_bench(args)
         return
     if args.precision_study:
         from chimera_amd import precision_study

@@ -84,6 +84,10 @@ _SIGS = {
     "chm_lp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_uint64,
                                vp]),
     "chm_lp_profile": (C.c_int, [vp, C.c_int, vp]),
+    "chm_batch_create": (C.c_int, [vp, C.c_int, vp, C.POINTER(vp)]),
+    "chm_batch_run": (C.c_int, [vp, C.c_int, C.c_int]),
+    "chm_batch_destroy": (C.c_int, [vp]),
+    "chm_batch_profile": (C.c_int, [vp, C.c_int, vp]),
 }
 
 EXPORTS = tuple(_SIGS)

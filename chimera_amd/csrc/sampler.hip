@@ -146,15 +146,14 @@ __device__ __forceinline__ void tile_epilogue(const Geo &g, const TileOut &O, in
 
 // Absolute FP64: m' = (counts + delta) @ basis_R.  Workgroup: 16 columns x 16 K-slices,
 // RB replica rows held per lane; the K-slice partials are summed in a fixed order.
-__global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int *__restrict__ counts,
-                                                       const int *__restrict__ dcounts,
-                                                       const double *__restrict__ basisR, TileOut O) {
+__device__ __forceinline__ void gemm_abs_body(const Geo &g, int nrep, const int *__restrict__ counts,
+                                              const int *__restrict__ dcounts,
+                                              const double *__restrict__ basisR, const TileOut &O, int ct,
+                                              int rbi, int e) {
     extern __shared__ double dl[];
     __shared__ int due[RB];
     __shared__ int any_due;
-    const int e = blockIdx.z;
-    const int r0 = blockIdx.y * RB;
-    const int ct = blockIdx.x;
+    const int r0 = rbi * RB;
     const int c0 = ct * NCW;
     const int K = g.nrow_e;
     const int ld = RB + 1;
@@ -217,9 +216,15 @@ __global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int
             tile_epilogue(g, O, nrep, r0 + m, e, ct, c0 + cl, O.acc32 ? (double)(float)sum : sum);
         }
     }
-    if (!reanchor && O.bump && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < RB &&
+    if (!reanchor && O.bump && ct == 0 && e == 0 && threadIdx.x < RB &&
         r0 + (int)threadIdx.x < nrep)
         O.stepc[r0 + threadIdx.x] += 1;
+}
+
+__global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int *__restrict__ counts,
+                                                       const int *__restrict__ dcounts,
+                                                       const double *__restrict__ basisR, TileOut O) {
+    gemm_abs_body(g, nrep, counts, dcounts, basisR, O, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 __device__ __forceinline__ __bf16 bf16_of_small_int(int v) {
@@ -237,13 +242,12 @@ __device__ __forceinline__ __bf16 bf16_of_small_int(int v) {
 // B[8(l/16)+i][l%16] (i < 8); D lane l holds D[4(l/16)+i][l%16] (i < 4).
 constexpr int KBW = 4;   // K blocks per wave held in registers (nkb <= 4 * KBW)
 
-__global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const int *__restrict__ dcounts,
-                                                          const bf16x8 *__restrict__ Tb,
-                                                          const double *__restrict__ mcur, TileOut O) {
+__device__ __forceinline__ void gemm_delta_bf16_body(const Geo &g, int nrep, const int *__restrict__ dcounts,
+                                                     const bf16x8 *__restrict__ Tb,
+                                                     const double *__restrict__ mcur, const TileOut &O, int ct,
+                                                     int rbi, int e) {
     __shared__ float red[4][RB][NCW + 1];
-    const int e = blockIdx.z;
-    const int r0 = blockIdx.y * RB;
-    const int ct = blockIdx.x;
+    const int r0 = rbi * RB;
     const int t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63;
     // epilogue operands first, so their latency hides behind the MFMA phase
@@ -357,20 +361,25 @@ __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const 
             P[3] = q2;
         }
     }
-    if (O.bump && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
+    if (O.bump && ct == 0 && e == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
         O.stepc[r0 + threadIdx.x] += 1;
+}
+
+__global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const int *__restrict__ dcounts,
+                                                          const bf16x8 *__restrict__ Tb,
+                                                          const double *__restrict__ mcur, TileOut O) {
+    gemm_delta_bf16_body(g, nrep, dcounts, Tb, mcur, O, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Incremental FP32: delta(R) = delta_counts @ basis_R with basis_R rounded to FP32 and FP32
 // FMA (no MFMA: the reference transform in FP32, for the precision study), added to the
 // running chi(R).  Tiling as k_gemm_abs_f64: 16 columns x 16 K slices, 32 slots per lane.
-__global__ __launch_bounds__(256) void k_gemm_delta_f32(Geo g, int nrep, const int *__restrict__ dcounts,
-                                                         const float *__restrict__ basis32,
-                                                         const double *__restrict__ mcur, TileOut O) {
+__device__ __forceinline__ void gemm_delta_f32_body(const Geo &g, int nrep, const int *__restrict__ dcounts,
+                                                    const float *__restrict__ basis32,
+                                                    const double *__restrict__ mcur, const TileOut &O, int ct,
+                                                    int rbi, int e) {
     extern __shared__ float fl[];
-    const int e = blockIdx.z;
-    const int r0 = blockIdx.y * RB;
-    const int ct = blockIdx.x;
+    const int r0 = rbi * RB;
     const int c0 = ct * NCW;
     const int K = g.nrow_e;
     const int ld = RB + 1;
@@ -415,8 +424,14 @@ __global__ __launch_bounds__(256) void k_gemm_delta_f32(Geo g, int nrep, const i
         }
         tile_epilogue(g, O, nrep, slot, e, ct, c, mp);
     }
-    if (O.bump && blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
+    if (O.bump && ct == 0 && e == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
         O.stepc[r0 + threadIdx.x] += 1;
+}
+
+__global__ __launch_bounds__(256) void k_gemm_delta_f32(Geo g, int nrep, const int *__restrict__ dcounts,
+                                                         const float *__restrict__ basis32,
+                                                         const double *__restrict__ mcur, TileOut O) {
+    gemm_delta_f32_body(g, nrep, dcounts, basis32, mcur, O, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 __global__ void k_to_f32(long n, const double *__restrict__ in, float *__restrict__ out) {
@@ -501,6 +516,7 @@ struct SlotArgs {
     int rec_cap, nocc, noccp;
     uint64_t seed;
     int profile;
+    int nrep;               // replica slots of the problem
     int accept;             // 0: none, 1: step, 2: init (store chi^2, no decision)
     int propose;            // 1: propose + gather the next step; 2: gather prop_next
     double nfit;
@@ -538,7 +554,9 @@ __device__ double chi2_of_sums(const Geo &g, const double (*S)[4], const double 
     return tot / nfit;
 }
 
-__global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
+// One replica slot s, workgroup `split` of `nsplit` (the body of k_slot_step and of the
+// batched launch over many problems).
+__device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, int s, int split, int nsplit) {
     extern __shared__ int lds[];
     int *hist = lds;
     uint8_t *lsp = reinterpret_cast<uint8_t *>(lds + ((g.nrow_tot + 3) & ~3));
@@ -546,9 +564,8 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     __shared__ double red[256];
     __shared__ int flag;
     __shared__ Prop P;
-    const int s = blockIdx.y;
-    const int nwg = gridDim.x;
-    const bool lead = blockIdx.x == 0;       // the workgroup that owns the slot's state
+    const int nwg = nsplit;
+    const bool lead = split == 0;            // the workgroup that owns the slot's state
     const int t = threadIdx.x;
     if (t == 0) flag = 0;
     const long base_sp = (long)s * g.natoms_pad;
@@ -581,7 +598,7 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     constexpr int MC = 4;
     double mt[MC];
     const int mchunk = (g.ncol_tot + nwg - 1) / nwg;
-    const int mbeg = blockIdx.x * mchunk, mend = min(g.ncol_tot, mbeg + mchunk);
+    const int mbeg = split * mchunk, mend = min(g.ncol_tot, mbeg + mchunk);
     if (A.accept && A.incremental) {
 #pragma unroll
         for (int r = 0; r < MC; ++r) {
@@ -593,7 +610,7 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
         for (int q = t; q < g.nrow_tot; q += blockDim.x) hist[q] = 0;
     // this lane's first multiple-scattering records (independent of the proposal)
     const int stride = nwg * blockDim.x;
-    const int first = blockIdx.x * blockDim.x + t;
+    const int first = split * blockDim.x + t;
     int4 recs[kRecPrefetch];
     prefetch_recs(g, A.ms_sten, first, stride, A.propose, recs);
     __syncthreads();
@@ -639,7 +656,7 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                     }
                     const long long ri = (long long)stepc - 1 - A.rec_base;
                     if (A.log && ri >= 0 && ri < A.log_cap) {
-                        int *L = A.log + ((long)ri * gridDim.y + s) * 4;
+                        int *L = A.log + ((long)ri * A.nrep + s) * 4;
                         L[0] = pp[0];
                         L[1] = pp[1];
                         L[2] = acc;
@@ -657,7 +674,7 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                     flag = (cur < best) ? 2 : 0;
                     if (A.rec_y) {
                         // the move record of this step: energy change and outcome
-                        const long idx = (long)(((stepc - 1) * (uint64_t)gridDim.y + (uint64_t)s) %
+                        const long idx = (long)(((stepc - 1) * (uint64_t)A.nrep + (uint64_t)s) %
                                                 (uint64_t)A.rec_cap);
                         A.rec_y[idx] = pp[4] ? (float)((c2t - c2p) * A.nfit) : 0.f;
                         A.rec_meta[idx] = pp[4] ? (rung | (acc << 16)) : -1;
@@ -670,7 +687,7 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
         const int acc = flag & 1;
         if (lead && A.rec_y && A.accept == 1) {
             // ... and the occupancy of its proposal
-            const long idx = (long)(((stepc - 1) * (uint64_t)gridDim.y + (uint64_t)s) % (uint64_t)A.rec_cap);
+            const long idx = (long)(((stepc - 1) * (uint64_t)A.nrep + (uint64_t)s) % (uint64_t)A.rec_cap);
             if (pp[4])
                 for (int q = t; q < A.noccp; q += blockDim.x)
                     A.rec_occ[idx * A.noccp + q] = A.pend_occ[(long)s * A.noccp + q];
@@ -740,6 +757,10 @@ __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
                     A.dc_prop + (long)s * g.nrow_tot);
 }
 
+__global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
+    slot_step_body(g, A, blockIdx.y, blockIdx.x, gridDim.x);
+}
+
 // chi^2 of arbitrary window transforms (one row per workgroup), direct form:
 // scale from the sums, then sum |s m - t|^2 w (method_a_rmc.py:211-227)
 __global__ __launch_bounds__(256) void k_eval_rows(Geo g, const double *__restrict__ rows,
@@ -771,7 +792,7 @@ __global__ __launch_bounds__(256) void k_eval_rows(Geo g, const double *__restri
 }
 
 // ---------------------------------------------------------------- K4
-__global__ void k_exchange(int nrep, const double *__restrict__ beta, const double *__restrict__ c2,
+__device__ __forceinline__ void exchange_body(int nrep, const double *__restrict__ beta, const double *__restrict__ c2,
                            int *__restrict__ slot_of, int *__restrict__ rung_of,
                            double *__restrict__ beta_slot, uint64_t *__restrict__ iex,
                            long long *__restrict__ att, long long *__restrict__ acc, uint64_t seed,
@@ -800,6 +821,14 @@ __global__ void k_exchange(int nrep, const double *__restrict__ beta, const doub
             beta_slot[si] = beta[i + 1];
         }
     }
+}
+
+__global__ void k_exchange(int nrep, const double *__restrict__ beta, const double *__restrict__ c2,
+                           int *__restrict__ slot_of, int *__restrict__ rung_of,
+                           double *__restrict__ beta_slot, uint64_t *__restrict__ iex,
+                           long long *__restrict__ att, long long *__restrict__ acc, uint64_t seed,
+                           double nfit) {
+    exchange_body(nrep, beta, c2, slot_of, rung_of, beta_slot, iex, att, acc, seed, nfit);
 }
 
 // chi(k) of every slot: counts @ basis_k per edge (for reading back)
@@ -864,6 +893,7 @@ SlotArgs slot_args(chm_sampler *S, int accept, int propose) {
     A.rec_meta = L.rec_cap ? L.rec_meta.as<int>() : nullptr;
     A.seed = S->seed;
     A.profile = S->profile;
+    A.nrep = S->nrep;
     A.accept = accept;
     A.propose = propose;
     A.nfit = S->nfit;
@@ -1695,5 +1725,386 @@ extern "C" int chm_sampler_set_tuning(chm_sampler *S, int nsplit) {
 extern "C" int chm_sampler_get_tuning(chm_sampler *S, int *nsplit) {
     CHM_REQUIRE(S && nsplit, CHM_E_ARG, "bad arguments");
     *nsplit = S->nsplit;
+    return CHM_OK;
+}
+
+// ======================================================================== batches
+// Many independent problems (different supercells, edges, rows, columns, replica counts)
+// stepped together: every kernel of a step is ONE launch for the whole batch, whose
+// workgroups find their problem and tile (or replica slot) in descriptor tables built
+// once; an exchange period of every problem is one captured hipGraph.  The problems keep
+// their own state (each is a chm_sampler), so they can still be read and run alone.
+namespace {
+
+struct GemmProb {
+    Geo g;
+    int nrep;
+    const int *counts;
+    const int *dc;            // count-change rows of the launch's step (parity)
+    const double *basisR;
+    const bf16x8 *Tb;
+    const float *basis32;
+    const double *mcur;
+    TileOut O;
+};
+
+template <int kMode>   // 0 exact FP64, 1 split-BF16 delta, 2 FP32 delta
+__global__ __launch_bounds__(256) void k_gemm_grouped(const GemmProb *__restrict__ probs,
+                                                       const int4 *__restrict__ tiles) {
+    const int4 tl = tiles[blockIdx.x];
+    const GemmProb &P = probs[tl.x];
+    if (kMode == 0)
+        gemm_abs_body(P.g, P.nrep, P.counts, P.dc, P.basisR, P.O, tl.y, tl.z, tl.w);
+    else if (kMode == 1)
+        gemm_delta_bf16_body(P.g, P.nrep, P.dc, P.Tb, P.mcur, P.O, tl.y, tl.z, tl.w);
+    else
+        gemm_delta_f32_body(P.g, P.nrep, P.dc, P.basis32, P.mcur, P.O, tl.y, tl.z, tl.w);
+}
+
+constexpr int kSlotKinds = 5;   // (accept, propose, re-anchor step): 010 110 100 111 101
+constexpr int kKindAccept[kSlotKinds] = {0, 1, 1, 1, 1};
+constexpr int kKindPropose[kSlotKinds] = {1, 1, 0, 1, 0};
+constexpr int kKindReanchor[kSlotKinds] = {0, 0, 0, 1, 1};
+
+struct SlotProb {
+    Geo g;
+    int nsplit;
+    SlotArgs A[2 * kSlotKinds];   // [parity of the step][kind]
+};
+
+__global__ __launch_bounds__(256) void k_slot_grouped(const SlotProb *__restrict__ probs,
+                                                       const int2 *__restrict__ slots, int variant) {
+    const int2 ps = slots[blockIdx.y];
+    const SlotProb &P = probs[ps.x];
+    if ((int)blockIdx.x >= P.nsplit) return;
+    slot_step_body(P.g, P.A[variant], ps.y, blockIdx.x, P.nsplit);
+}
+
+struct ExchProb {
+    int nrep;
+    const double *beta;
+    const double *c2[2];
+    int *slot_of, *rung_of;
+    double *beta_slot;
+    uint64_t *iex;
+    long long *att, *acc;
+    uint64_t seed;
+    double nfit;
+};
+
+__global__ void k_exchange_grouped(const ExchProb *__restrict__ probs, int parity) {
+    const ExchProb &P = probs[blockIdx.x];
+    exchange_body(P.nrep, P.beta, P.c2[parity], P.slot_of, P.rung_of, P.beta_slot, P.iex, P.att, P.acc,
+                  P.seed, P.nfit);
+}
+
+}  // namespace
+
+struct chm_batch {
+    chm_ctx *ctx = nullptr;
+    std::vector<chm_sampler *> S;
+    int mode = 0, anchor = 0, nslot = 0, nsplit_max = 1, ntile = 0;
+    size_t lds_gemm_max = 0, lds_slot_max = 0;
+    chm::DevBuf gemm[2], gemm_nb, reanchor, tiles, slot, slots, exch;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int graph_steps = 0;
+    long long hstep = 0;
+};
+
+namespace {
+
+void batch_drop_graph(chm_batch *B) {
+    for (int q = 0; q < 2; ++q) {
+        if (B->gexec[q]) hipGraphExecDestroy(B->gexec[q]);
+        if (B->graph[q]) hipGraphDestroy(B->graph[q]);
+        B->gexec[q] = nullptr;
+        B->graph[q] = nullptr;
+    }
+    B->graph_steps = 0;
+}
+
+int batch_gemm(chm_batch *B, const GemmProb *probs, int mode) {
+    hipStream_t st = B->ctx->stream;
+    const int4 *tiles = B->tiles.as<int4>();
+    if (mode == 0)
+        k_gemm_grouped<0><<<B->ntile, 256, B->lds_gemm_max, st>>>(probs, tiles);
+    else if (mode == 1)
+        k_gemm_grouped<1><<<B->ntile, 256, 0, st>>>(probs, tiles);
+    else
+        k_gemm_grouped<2><<<B->ntile, 256, B->lds_gemm_max, st>>>(probs, tiles);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+int batch_slot(chm_batch *B, long long n, int kind) {
+    const int variant = (int)(n & 1) * kSlotKinds + kind;
+    k_slot_grouped<<<dim3(B->nsplit_max, B->nslot), 256, B->lds_slot_max, B->ctx->stream>>>(
+        B->slot.as<SlotProb>(), B->slots.as<int2>(), variant);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+int batch_exchange(chm_batch *B, long long n) {
+    k_exchange_grouped<<<(unsigned)B->S.size(), 256, 0, B->ctx->stream>>>(B->exch.as<ExchProb>(),
+                                                                          (int)(n & 1));
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+// as launch_segment, for every problem at once
+int batch_segment(chm_batch *B, long long h, int L) {
+    int rc = batch_slot(B, h, 0);
+    if (!rc && B->anchor) rc = batch_gemm(B, B->reanchor.as<GemmProb>(), 0);
+    for (int t = 0; t < L && !rc; ++t) {
+        const long long n = h + 1 + t;
+        rc = batch_gemm(B, B->gemm[n & 1].as<GemmProb>(), B->mode);
+        const bool prop = t + 1 < L, re = B->anchor && t == 0;
+        const int kind = re ? (prop ? 3 : 4) : (prop ? 1 : 2);
+        if (!rc) rc = batch_slot(B, n, kind);
+    }
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int chm_batch_create(chm_ctx *ctx, int n, chm_sampler *const *samplers, chm_batch **out) {
+    CHM_REQUIRE(ctx && samplers && out && n >= 1, CHM_E_ARG, "bad arguments");
+    auto *B = new chm_batch();
+    B->ctx = ctx;
+    auto fail = [&](int code, const char *msg) {
+        delete B;
+        chm::set_error(msg);
+        return code;
+    };
+    for (int p = 0; p < n; ++p) {
+        chm_sampler *S = samplers[p];
+        if (!S) return fail(CHM_E_ARG, "null sampler");
+        if (S->ctx->device != ctx->device) return fail(CHM_E_ARG, "every sampler must live on the batch's device");
+        if (!S->inited) return fail(CHM_E_STATE, "init every sampler first");
+        if (S->lp.mode) return fail(CHM_E_ARG, "the learned proposal is not batched");
+        if (p && (S->incremental != samplers[0]->incremental || S->hstep != samplers[0]->hstep))
+            return fail(CHM_E_ARG, "batched samplers need one precision mode and one step count");
+        B->S.push_back(S);
+    }
+    chm_sampler *S0 = samplers[0];
+    B->mode = S0->incremental;
+    B->hstep = S0->hstep;
+    for (chm_sampler *S : B->S) B->anchor |= (S->incremental && S->reanchor_every > 0);
+    CHM_HIP(hipSetDevice(ctx->device));
+    // tiles of every problem's transform, slots of every problem's replicas
+    std::vector<int4> tiles;
+    std::vector<int2> slots;
+    std::vector<GemmProb> gp[2], rp, gnb;
+    std::vector<SlotProb> sp(n);
+    std::vector<ExchProb> ep(n);
+    for (int p = 0; p < n; ++p) {
+        chm_sampler *S = B->S[p];
+        const Geo &g = S->g;
+        for (int e = 0; e < g.nsp; ++e)
+            for (int rb = 0; rb < (S->nrep + RB - 1) / RB; ++rb)
+                for (int ct = 0; ct < g.nct; ++ct) tiles.push_back(make_int4(p, ct, rb, e));
+        for (int s = 0; s < S->nrep; ++s) slots.push_back(make_int2(p, s));
+        B->lds_gemm_max = std::max(B->lds_gemm_max, lds_gemm(g));
+        B->lds_slot_max = std::max(B->lds_slot_max, S->lds_k1);
+        B->nsplit_max = std::max(B->nsplit_max, S->nsplit);
+        for (int par = 0; par < 2; ++par) {
+            GemmProb G{};
+            G.g = g;
+            G.nrep = S->nrep;
+            G.counts = S->counts.as<int>();
+            G.dc = drow(S, par);
+            G.basisR = S->basisR.as<double>();
+            G.Tb = reinterpret_cast<const bf16x8 *>(S->Tb.p);
+            G.basis32 = S->basisR32.as<float>();
+            G.mcur = S->mcur.as<double>();
+            G.O = tile_out(S);
+            G.O.bump = 1;
+            gp[par].push_back(G);
+            if (par == 1) {   // for timing: the transform without advancing the step counter
+                G.O.bump = 0;
+                gnb.push_back(G);
+            }
+        }
+        GemmProb R{};
+        R.g = g;
+        R.nrep = S->nrep;
+        R.counts = S->counts.as<int>();
+        R.dc = nullptr;
+        R.basisR = S->basisR.as<double>();
+        R.O = tile_out(S);
+        R.O.anchor = S->anchor.as<int>();
+        R.O.anchor_min = (S->incremental && S->reanchor_every > 0) ? S->reanchor_every : (1 << 30);
+        R.O.anchor_out = S->mcur.as<double>();
+        rp.push_back(R);
+        SlotProb &P = sp[p];
+        P.g = g;
+        P.nsplit = S->nsplit;
+        for (int par = 0; par < 2; ++par)
+            for (int k = 0; k < kSlotKinds; ++k) {
+                SlotArgs A = slot_args(S, kKindAccept[k], kKindPropose[k]);
+                const long long nn = par;   // only the parity of the step matters
+                A.reanchor_step = kKindReanchor[k];
+                A.dc_acc = drow(S, nn);
+                A.dc_prop = drow(S, nn + 1);
+                A.prop_acc = prow(S, nn);
+                A.prop_next = prow(S, nn + 1);
+                A.c2_prev = c2row(S, nn + 1);   // step n - 1 has the other parity
+                A.c2_next = c2row(S, nn);
+                P.A[par * kSlotKinds + k] = A;
+            }
+        ExchProb &E = ep[p];
+        E.nrep = S->nrep;
+        E.beta = S->beta.as<double>();
+        E.c2[0] = c2row(S, 0);
+        E.c2[1] = c2row(S, 1);
+        E.slot_of = S->slot_of.as<int>();
+        E.rung_of = S->rung_of.as<int>();
+        E.beta_slot = S->beta_slot.as<double>();
+        E.iex = S->iex.as<uint64_t>();
+        E.att = S->exch_att.as<long long>();
+        E.acc = S->exch_acc.as<long long>();
+        E.seed = S->seed;
+        E.nfit = S->nfit;
+    }
+    if (B->mode == 2)
+        for (chm_sampler *S : B->S)
+            if (!S->basisR32.p) return fail(CHM_E_STATE, "FP32 basis missing");
+    B->ntile = (int)tiles.size();
+    B->nslot = (int)slots.size();
+    if (B->lds_gemm_max > 160 * 1024 || B->lds_slot_max > 160 * 1024)
+        return fail(CHM_E_ARG, "problem too large for LDS staging");
+    auto up = [&](chm::DevBuf &b, const void *h, size_t bytes) -> int {
+        int rc = b.ensure(bytes);
+        if (rc) return rc;
+        CHM_HIP(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+        return CHM_OK;
+    };
+    int rc;
+    if ((rc = up(B->tiles, tiles.data(), sizeof(int4) * tiles.size())) ||
+        (rc = up(B->slots, slots.data(), sizeof(int2) * slots.size())) ||
+        (rc = up(B->gemm[0], gp[0].data(), sizeof(GemmProb) * n)) ||
+        (rc = up(B->gemm[1], gp[1].data(), sizeof(GemmProb) * n)) ||
+        (rc = up(B->gemm_nb, gnb.data(), sizeof(GemmProb) * n)) ||
+        (rc = up(B->reanchor, rp.data(), sizeof(GemmProb) * n)) ||
+        (rc = up(B->slot, sp.data(), sizeof(SlotProb) * n)) ||
+        (rc = up(B->exch, ep.data(), sizeof(ExchProb) * n))) {
+        delete B;
+        return rc;
+    }
+    hipFuncSetAttribute((const void *)k_gemm_grouped<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)B->lds_gemm_max);
+    hipFuncSetAttribute((const void *)k_gemm_grouped<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)B->lds_gemm_max);
+    hipFuncSetAttribute((const void *)k_slot_grouped, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)B->lds_slot_max);
+    for (chm_sampler *S : B->S) drop_graph(S);
+    chm::ctx_retain(ctx);
+    *out = B;
+    return CHM_OK;
+}
+
+extern "C" int chm_batch_destroy(chm_batch *B) {
+    if (!B) return CHM_OK;
+    chm_ctx *ctx = B->ctx;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    batch_drop_graph(B);
+    delete B;
+    chm::ctx_release(ctx);
+    return CHM_OK;
+}
+
+extern "C" int chm_batch_run(chm_batch *B, int nsteps, int exchange_every) {
+    CHM_REQUIRE(B, CHM_E_ARG, "null batch");
+    CHM_REQUIRE(nsteps >= 0 && exchange_every >= 0, CHM_E_ARG, "negative step count");
+    for (chm_sampler *S : B->S)
+        CHM_REQUIRE(S->hstep == B->hstep && S->inited, CHM_E_STATE,
+                    "a sampler of the batch was re-initialised or run on its own");
+    CHM_HIP(hipSetDevice(B->ctx->device));
+    hipStream_t st = B->ctx->stream;
+    for (chm_sampler *S : B->S)   // work the samplers queued on their own streams comes first
+        if (S->ctx != B->ctx) CHM_HIP(hipStreamSynchronize(S->ctx->stream));
+    const int E = exchange_every;
+    int rc = 0;
+    if (E > 0 && B->graph_steps != E) {
+        batch_drop_graph(B);
+        for (int par = 0; par < ((E & 1) ? 2 : 1) && !rc; ++par) {
+            CHM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = batch_segment(B, par, E);
+            if (!rc) rc = batch_exchange(B, par + E);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &gr);
+            if (rc) {
+                if (gr) hipGraphDestroy(gr);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                chm::set_error(std::string("graph capture: ") + hipGetErrorString(e));
+                return CHM_E_HIP;
+            }
+            B->graph[par] = gr;
+            CHM_HIP(hipGraphInstantiate(&B->gexec[par], gr, nullptr, nullptr, 0));
+        }
+        B->graph_steps = E;
+    }
+    long long left = nsteps;
+    while (left > 0) {
+        if (E > 0 && B->hstep % E == 0 && left >= E) {
+            CHM_HIP(hipGraphLaunch(B->gexec[(E & 1) ? (B->hstep & 1) : 0], st));
+            B->hstep += E;
+            left -= E;
+        } else {
+            long long L = left;
+            if (E > 0) L = std::min<long long>(left, E - B->hstep % E);
+            if ((rc = batch_segment(B, B->hstep, (int)L))) return rc;
+            B->hstep += L;
+            left -= L;
+            if (E > 0 && B->hstep % E == 0 && (rc = batch_exchange(B, B->hstep))) return rc;
+        }
+    }
+    CHM_HIP(hipStreamSynchronize(st));
+    for (chm_sampler *S : B->S) S->hstep = B->hstep;
+    return CHM_OK;
+}
+
+extern "C" int chm_batch_profile(chm_batch *B, int iters, double *ms) {
+    CHM_REQUIRE(B && iters > 0 && ms, CHM_E_ARG, "bad arguments");
+    CHM_HIP(hipSetDevice(B->ctx->device));
+    hipStream_t st = B->ctx->stream;
+    hipEvent_t e0, e1;
+    CHM_HIP(hipEventCreate(&e0));
+    CHM_HIP(hipEventCreate(&e1));
+    float t = 0.f;
+    const long long h = B->hstep;
+    int rc = batch_slot(B, h, 0);                 // step h + 1 pending
+    CHM_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < iters && !rc; ++i) rc = batch_gemm(B, B->gemm_nb.as<GemmProb>(), B->mode);
+    CHM_HIP(hipEventRecord(e1, st));
+    if (rc) return rc;
+    CHM_HIP(hipEventSynchronize(e1));
+    CHM_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms[0] = t / iters;                              // transform of every problem
+    CHM_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < iters && !rc; ++i) {
+        const long long n = h + 1 + i;
+        rc = batch_gemm(B, B->gemm[n & 1].as<GemmProb>(), B->mode);
+        if (!rc) rc = batch_slot(B, n, 1);
+    }
+    CHM_HIP(hipEventRecord(e1, st));
+    if (rc) return rc;
+    CHM_HIP(hipEventSynchronize(e1));
+    CHM_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms[2] = t / iters;                              // whole step
+    ms[1] = ms[2] - ms[0];                          // per-replica kernel
+    const long long n = h + 1 + iters;
+    if (!rc) rc = batch_gemm(B, B->gemm[n & 1].as<GemmProb>(), B->mode);
+    if (!rc) rc = batch_slot(B, n, 2);
+    if (rc) return rc;
+    CHM_HIP(hipStreamSynchronize(st));
+    B->hstep = n;
+    for (chm_sampler *S : B->S) S->hstep = n;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
     return CHM_OK;
 }

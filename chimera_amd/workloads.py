@@ -51,6 +51,7 @@ class Workload:
     alpha_target: float = -0.08
     noise: float = 0.002
     seed: int = 0
+    edges: tuple = None      # measured edges (species indices); None = every species
 
 
 WORKLOADS = {
@@ -156,6 +157,11 @@ class Problem:
         resid0 = (m0 * scales[:, None] - tgt)[:, self.mask]
         sig_data = np.sqrt((np.abs(resid0) ** 2).mean(1) / 2.0)
         sig = sig_data / scales
+        if self.w.edges is not None:
+            # an element that is not a measured edge still scatters; its channels carry
+            # no data: zero weight (the sampler's basis is per absorber species)
+            measured = np.isin(self.chan_edge, np.asarray(self.w.edges))
+            sig = np.where(measured, sig, 1e30)
         target = tgt / scales[:, None]
         sigR = np.repeat(sig[:, None], self.R.size, axis=1)
         return target, sigR, scales

@@ -34,6 +34,7 @@ extern "C" {
 typedef struct chm_ctx chm_ctx;
 typedef struct chm_ft chm_ft;
 typedef struct chm_sampler chm_sampler;
+typedef struct chm_batch chm_batch;
 
 const char *chm_last_error(void);
 int chm_version(void);
@@ -277,6 +278,24 @@ int chm_lp_train(chm_sampler *s, int iters, int batch, float lr, int max_rung, f
  * the current state, with every slot on the learned path; the chain does not advance
  * (the selection writes into a scratch proposal row). */
 int chm_lp_profile(chm_sampler *s, int iters, double *ms);
+
+/* ------------------------------------------------------------------- batches
+ * Many independent problems stepped together: every kernel of a step is one launch for
+ * all of them (descriptor tables of problem x tile and problem x replica slot), and an
+ * exchange period of every problem is one captured hipGraph, launched on ctx's stream.
+ * The samplers must live on ctx's device (any of its contexts), be initialised, share one
+ * precision mode and one step count, and use the uniform
+ * proposal; each keeps its own state and launch split (chm_sampler_set_tuning) and can
+ * be read as usual.  Configure recording before creating the batch.  No counterpart in
+ * the reference (its fit driver loops over problems and seeds on the host,
+ * chimera/fitrun.py:302-333). */
+int chm_batch_create(chm_ctx *ctx, int n, chm_sampler *const *samplers, chm_batch **out);
+int chm_batch_run(chm_batch *b, int nsteps, int exchange_every);
+int chm_batch_destroy(chm_batch *b);
+/* Device time (ms, HIP events) of the batched transform (ms[0]), the batched per-replica
+ * kernel (ms[1]) and one whole batched step (ms[2]), over `iters` steps that are real
+ * steps of every chain (no exchange, no re-anchor). */
+int chm_batch_profile(chm_batch *b, int iters, double *ms);
 
 #ifdef __cplusplus
 }
