@@ -642,7 +642,10 @@ __global__ __launch_bounds__(256) void k_lp_fwd_train(TrainArgs T) {
         if (row < T.B) {
             z = T.b1[n];
             const uint8_t *o = T.occ + (long)row * T.noccp;
-            for (int q = 0; q < T.nocc; ++q) z += bf(T.w1[(long)(T.nsp * q + o[q]) * HID + n]);
+            for (int q = 0; q < T.nocc; ++q) {
+                const int sq = o[q] < T.nsp ? o[q] : 0;   // (rows are validated; never index past W1)
+                z += bf(T.w1[(long)(T.nsp * q + sq) * HID + n]);
+            }
             T.Z1[(long)row * HID + n] = z;
         }
         const float h = bf16_round(silu(z));
@@ -750,7 +753,9 @@ __global__ void k_lp_batch(int B, long long nvalid, int noccp, uint64_t seed, lo
     const int r = blockIdx.x, t = threadIdx.x;
     const uint64_t h = splitmix64(seed ^ ((uint64_t)it * kGolden) ^ ((uint64_t)(r + 1) << 40) ^ kLpSalt);
     const long long idx = (long long)(h % (uint64_t)nvalid);
-    for (int q = t; q < noccp; q += blockDim.x) occ[(long)r * noccp + q] = rec_occ[idx * noccp + q];
+    // a null move's entry has no occupancy: its row gets species 0 (and weight 0)
+    const bool swap = rec_meta[idx] >= 0;
+    for (int q = t; q < noccp; q += blockDim.x) occ[(long)r * noccp + q] = swap ? rec_occ[idx * noccp + q] : 0;
     if (t == 0) {
         const int meta = rec_meta[idx];
         const bool use = meta >= 0 && (meta & 0xFFFF) <= max_rung;
@@ -1047,7 +1052,11 @@ extern "C" int chm_lp_set_records(chm_sampler *S, int capacity) {
     if ((rc = L.rec_occ.ensure((size_t)capacity * L.noccp)) || (rc = L.rec_y.ensure(4 * (size_t)capacity)) ||
         (rc = L.rec_meta.ensure(4 * (size_t)capacity)))
         return rc;
-    if (capacity) CHM_HIP(hipMemset(L.rec_meta.p, 0xFF, 4 * (size_t)capacity));
+    if (capacity) {
+        CHM_HIP(hipMemset(L.rec_meta.p, 0xFF, 4 * (size_t)capacity));
+        CHM_HIP(hipMemset(L.rec_occ.p, 0, (size_t)capacity * L.noccp));
+        CHM_HIP(hipMemset(L.rec_y.p, 0, 4 * (size_t)capacity));
+    }
     L.rec_cap = capacity;
     L.rec_from = S->hstep;
     return CHM_OK;

@@ -561,7 +561,6 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     int *hist = lds;
     uint8_t *lsp = reinterpret_cast<uint8_t *>(lds + ((g.nrow_tot + 3) & ~3));
     __shared__ double sums[chm::kMaxNsp][4];
-    __shared__ double red[256];
     __shared__ int flag;
     __shared__ Prop P;
     const int nwg = nsplit;
@@ -582,17 +581,20 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
         for (int q = 0; q < kPropInts; ++q) pp[q] = A.prop_acc[kPropInts * s + q];
     }
     load_species(lsp, A.species + base_sp, g.natoms_pad);
-    // per-edge sums over the column tiles: (e, j) groups of CT lanes, then a tree
+    // per-edge sums over the column tiles: (e, j) groups of CT lanes (CT-aligned inside a
+    // wave), then a butterfly over the group's lanes (no LDS, no barrier)
     const int G = 4 * g.nsp;
-    int CT = (int)blockDim.x / G;
-    if (CT > g.nct) CT = g.nct;
+    const int CT = (G <= 16) ? 16 : 8;
     const int gi = t / CT, c0 = t - gi * CT;
-    if (A.accept && t < G * CT) {
-        const int e = gi >> 2, j = gi & 3;
-        const double *Pp = A.partials + ((long)s * g.nsp + e) * g.nct * 4 + j;
+    if (A.accept) {
         double v = 0.0;
-        for (int c = c0; c < g.nct; c += CT) v += Pp[4 * c];
-        red[t] = v;
+        if (t < G * CT) {
+            const int e = gi >> 2, j = gi & 3;
+            const double *Pp = A.partials + ((long)s * g.nsp + e) * g.nct * 4 + j;
+            for (int c = c0; c < g.nct; c += CT) v += Pp[4 * c];
+        }
+        for (int m = CT / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, CT);
+        if (t < G * CT && c0 == 0) sums[gi >> 2][gi & 3] = v;
     }
     // this workgroup's share of the running chi(R) to copy forward on acceptance
     constexpr int MC = 4;
@@ -616,14 +618,6 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     __syncthreads();
     int *cn = A.counts + (long)s * g.nrow_tot;
     if (A.accept) {
-        int h2 = 1;
-        while (h2 < CT) h2 <<= 1;
-        for (int h = h2 >> 1; h >= 1; h >>= 1) {
-            if (t < G * CT && c0 < h && c0 + h < CT) red[t] += red[t + h];
-            __syncthreads();
-        }
-        if (t < G * CT && c0 == 0) sums[gi >> 2][gi & 3] = red[t];
-        __syncthreads();
         // every workgroup of the slot takes the same (deterministic) decision
         if (t == 0) {
             const double c2t = chi2_of_sums(g, sums, A.q0, A.profile, A.nfit);

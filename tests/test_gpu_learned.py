@@ -191,3 +191,27 @@ def test_training_reduces_loss(tern108):
     _say(f"[test] loss {l0:.4g} -> {l1:.4g}")
     assert np.isfinite(l1) and l1 < 0.5 * l0
     lp.record(0)
+
+
+def test_training_with_null_move_records(tern108):
+    """Records of null moves carry no occupancy (meta -1): the ring starts zeroed and the
+    trainer gives such rows species 0 and weight 0."""
+    P, O = tern108
+    S = P.sampler
+    lp = _lp(P)
+    sp = np.zeros(P.cell.natoms, np.int32)
+    sp[:4], sp[4:8] = 1, 2                       # most proposals find no partner
+    try:
+        S.init_state(sp, P.cell.ideal)
+        lp.record(100 * S.nrep)
+        _say("[run] 100 steps of a 100:4:4 composition with records, then training")
+        S.run(100, exchange_every=25, log_every=10 ** 9)
+        nv, occ, y, meta = lp.records()
+        nulls = meta[:nv] == -1
+        assert nulls.sum() > 0
+        assert np.all(occ[:nv][nulls] == 0)
+        loss = lp.train(iters=20, batch=256, lr=1e-3, seed=3)
+        assert np.isfinite(loss)
+    finally:
+        lp.record(0)
+        S.init_state(P.cfg0, P.cell.ideal)
