@@ -535,13 +535,16 @@ __device__ double block_sum(double v, double *red) {
     return r;
 }
 
-// chi^2 from the per-edge residual sums {U1, U2, W1, W2}
-__device__ double chi2_of_sums(const Geo &g, const double (*S)[4], const double *q0, int profile,
-                               double nfit) {
+// chi^2 from the per-edge residual sums {U1, U2, W1, W2}; q0v: per edge T0, Q0
+__device__ __forceinline__ double chi2_of_sums(const Geo &g, const double (*S)[4],
+                                               const double (&q0v)[2 * chm::kMaxNsp], int profile,
+                                               double nfit) {
     double tot = 0.0;
-    for (int e = 0; e < g.nsp; ++e) {
+#pragma unroll
+    for (int e = 0; e < chm::kMaxNsp; ++e) {
+        if (e >= g.nsp) break;
         const double U1 = S[e][0], U2 = S[e][1], W1 = S[e][2], W2 = S[e][3];
-        const double T0 = q0[2 * e], Q0 = q0[2 * e + 1];
+        const double T0 = q0v[2 * e], Q0 = q0v[2 * e + 1];
         double s = 1.0, sm1 = 0.0;
         if (profile) {
             const double P2 = fmax(U2 + 2.0 * U1 + T0, 1e-30);
@@ -555,7 +558,10 @@ __device__ double chi2_of_sums(const Geo &g, const double (*S)[4], const double 
 }
 
 // One replica slot s, workgroup `split` of `nsplit` (the body of k_slot_step and of the
-// batched launch over many problems).
+// batched launch over many problems).  Every input that does not depend on the decision
+// is requested before the first barrier (one memory round trip); the decision is taken
+// by every workgroup, and each commits its own share of the counts, running chi(R) and
+// best configuration.
 __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, int s, int split, int nsplit) {
     extern __shared__ int lds[];
     int *hist = lds;
@@ -564,91 +570,136 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     __shared__ int flag;
     __shared__ Prop P;
     const int nwg = nsplit;
-    const bool lead = split == 0;            // the workgroup that owns the slot's state
+    const bool lead = split == 0;            // the workgroup that owns the slot's scalars
     const int t = threadIdx.x;
+    const int nt = blockDim.x;
     if (t == 0) flag = 0;
     const long base_sp = (long)s * g.natoms_pad;
-    // every small per-slot input is requested up front, in one round trip
+    // ------------------------------------------------ every input, in one round trip:
+    // all loads are issued (fixed counts, unconditional where the buffer allows) before
+    // any of them is consumed
     const uint64_t stepc = A.stepc[s];       // written only by the transform kernels
     const int rung = A.rung_of[s];
     const double beta = A.beta_slot[s];
-    const double c2p = A.c2_prev ? A.c2_prev[s] : 0.0;
+    const double c2p = A.c2_prev[s];
     const double best = A.best_c2[s];
     const int anchor = A.anchor[s];
-    int pp[kPropInts] = {0, 0, 0, 0, 0, 0};
-    if (A.accept == 1) {
+    int pp[kPropInts];                       // the proposal being accepted
 #pragma unroll
-        for (int q = 0; q < kPropInts; ++q) pp[q] = A.prop_acc[kPropInts * s + q];
-    }
-    load_species(lsp, A.species + base_sp, g.natoms_pad);
+    for (int q = 0; q < kPropInts; ++q) pp[q] = A.prop_acc[kPropInts * s + q];
+    int pn[5];                               // a proposal written by the learned selection
+#pragma unroll
+    for (int q = 0; q < 5; ++q) pn[q] = A.prop_next[kPropInts * s + q];
+    double q0v[2 * chm::kMaxNsp];            // per edge T0, Q0 (padded with zeros)
+#pragma unroll
+    for (int q = 0; q < 2 * chm::kMaxNsp; ++q) q0v[q] = A.q0[q];
+    // species: one 16-byte vector per lane
+    const int nv = g.natoms_pad / 16;
+    const uint4 *gsp = reinterpret_cast<const uint4 *>(A.species + base_sp);
+    const int4 spv = buf_i128(make_buf(gsp, 16u * nv), t < nv, 16u * t);
     // per-edge sums over the column tiles: (e, j) groups of CT lanes (CT-aligned inside a
-    // wave), then a butterfly over the group's lanes (no LDS, no barrier)
+    // wave), each lane summing tiles c0, c0 + CT, ... in order, then a butterfly over the
+    // group's lanes
     const int G = 4 * g.nsp;
     const int CT = (G <= 16) ? 16 : 8;
     const int gi = t / CT, c0 = t - gi * CT;
-    if (A.accept) {
-        double v = 0.0;
-        if (t < G * CT) {
-            const int e = gi >> 2, j = gi & 3;
-            const double *Pp = A.partials + ((long)s * g.nsp + e) * g.nct * 4 + j;
-            for (int c = c0; c < g.nct; c += CT) v += Pp[4 * c];
-        }
-        for (int m = CT / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, CT);
-        if (t < G * CT && c0 == 0) sums[gi >> 2][gi & 3] = v;
-    }
+    const bool sum_lane = A.accept && t < G * CT;
+    const double *Pp = A.partials + (long)s * g.nsp * g.nct * 4;
+    const Buf bp = make_buf(Pp, 32u * g.nsp * g.nct);
+    const uint32_t poff = 8u * ((gi >> 2) * g.nct * 4 + (gi & 3));
+    constexpr int PR = 8;
+    double pv[PR];
+#pragma unroll
+    for (int r = 0; r < PR; ++r) pv[r] = buf_f64(bp, sum_lane && c0 + r * CT < g.nct, poff + 32u * (c0 + r * CT));
     // this workgroup's share of the running chi(R) to copy forward on acceptance
     constexpr int MC = 4;
     double mt[MC];
     const int mchunk = (g.ncol_tot + nwg - 1) / nwg;
     const int mbeg = split * mchunk, mend = min(g.ncol_tot, mbeg + mchunk);
-    if (A.accept && A.incremental) {
+    const bool mcopy = A.accept && A.incremental;
+    const double *mts = A.mtrial + (long)s * g.ncol_tot;
+    const Buf bmt = make_buf(mts, 8u * g.ncol_tot);
 #pragma unroll
-        for (int r = 0; r < MC; ++r) {
-            const int q = mbeg + t + r * (int)blockDim.x;
-            mt[r] = (q < mend) ? A.mtrial[(long)s * g.ncol_tot + q] : 0.0;
-        }
+    for (int r = 0; r < MC; ++r) {
+        const int q = mbeg + t + r * nt;
+        mt[r] = buf_f64(bmt, mcopy && q < mend, 8u * q);
+    }
+    // ... and of the count commit.  With the pair-count series of this slot's rung being
+    // recorded, the lead workgroup commits every row (it reads the committed counts).
+    int *cn = A.counts + (long)s * g.nrow_tot;
+    int *dca = A.dc_acc + (long)s * g.nrow_tot;
+    const long long ri = (long long)stepc - 1 - A.rec_base;
+    const bool series_on = A.accept == 1 && A.series && rung == A.series_rung && ri >= 0 &&
+                           ri < A.series_cap;
+    const int rchunk = series_on ? (lead ? g.nrow_tot : 0) : (g.nrow_tot + nwg - 1) / nwg;
+    const int rbeg = series_on ? 0 : split * rchunk, rend = min(g.nrow_tot, rbeg + rchunk);
+    const Buf bcn = make_buf(cn, 4u * g.nrow_tot), bdc = make_buf(dca, 4u * g.nrow_tot);
+    constexpr int RC = 2;
+    int cnv[RC], dcv[RC];
+#pragma unroll
+    for (int r = 0; r < RC; ++r) {
+        const int q = rbeg + t + r * nt;
+        dcv[r] = buf_i32(bdc, A.accept && q < rend, 4u * q);
+        cnv[r] = buf_i32(bcn, A.accept == 1 && q < rend, 4u * q);
+    }
+    // this lane's first gather records (independent of the proposal)
+    const int stride = nwg * nt;
+    const int first = split * nt + t;
+    RecPre recs;
+    prefetch_recs(g, A.ss_sten, A.ms_sten, first, stride, A.propose, recs);
+    // the draws of this step's acceptance uniform and of the next uniform proposal
+    // (wave-uniform: scalar arithmetic)
+    double u_acc = 1.0;
+    if (A.accept == 1) {
+        const uint64_t h = splitmix64(A.seed ^ (stepc * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
+        u_acc = u01(splitmix64(h));
+    }
+    PropDraws dr{};
+    if (A.propose == 1) dr = propose_draws(g.natoms, A.seed, stepc + 1, (uint64_t)rung);
+    // ------------------------------------------------ consume
+    if (t < nv) reinterpret_cast<int4 *>(lsp)[t] = spv;
+    for (int q = t + nt; q < nv; q += nt) reinterpret_cast<uint4 *>(lsp)[q] = gsp[q];
+    if (A.accept) {
+        double psum = 0.0;   // out-of-range tiles loaded as 0.0
+#pragma unroll
+        for (int r = 0; r < PR; ++r) psum += pv[r];
+        for (int c = c0 + PR * CT; sum_lane && c < g.nct; c += CT)
+            psum += Pp[(long)((gi >> 2) * g.nct + c) * 4 + (gi & 3)];
+        for (int m = CT / 2; m >= 1; m >>= 1) psum += __shfl_xor(psum, m, CT);
+        if (t < G * CT && c0 == 0) sums[gi >> 2][gi & 3] = psum;
     }
     if (A.propose)
-        for (int q = t; q < g.nrow_tot; q += blockDim.x) hist[q] = 0;
-    // this lane's first multiple-scattering records (independent of the proposal)
-    const int stride = nwg * blockDim.x;
-    const int first = split * blockDim.x + t;
-    int4 recs[kRecPrefetch];
-    prefetch_recs(g, A.ms_sten, first, stride, A.propose, recs);
+        for (int q = t; q < g.nrow_tot; q += nt) hist[q] = 0;
     __syncthreads();
-    int *cn = A.counts + (long)s * g.nrow_tot;
     if (A.accept) {
-        // every workgroup of the slot takes the same (deterministic) decision
+        // ------------------------------------------------ decision (same in every workgroup)
         if (t == 0) {
-            const double c2t = chi2_of_sums(g, sums, A.q0, A.profile, A.nfit);
-            int acc = 0;
-            if (A.accept == 2) {
-                acc = 1;
-                if (lead) {
-                    A.c2_next[s] = c2t;
-                    A.best_c2[s] = c2t;
-                }
-            } else {
+            const double c2t = chi2_of_sums(g, sums, q0v, A.profile, A.nfit);
+            int acc = 1;
+            double cur = c2t;
+            if (A.accept == 1) {
+                acc = 0;
                 if (pp[4]) {
                     // Metropolis-Hastings: pp[5] holds log q(x'->x) - log q(x->x')
                     const double x = beta * (c2t - c2p) * A.nfit - (double)__int_as_float(pp[5]);
-                    uint64_t h = splitmix64(A.seed ^ (stepc * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
-                    const double u = u01(splitmix64(h));
-                    acc = (x <= 0.0) || (u < exp(-fmin(x, 60.0)));
+                    acc = (x <= 0.0) || (u_acc < exp(-fmin(x, 60.0)));
                 }
                 if (acc) {
                     lsp[pp[0]] = (uint8_t)pp[2];
                     lsp[pp[1]] = (uint8_t)pp[3];
                 }
-                if (lead) {
-                    const double cur = acc ? c2t : c2p;
-                    A.c2_next[s] = cur;
+                cur = acc ? c2t : c2p;
+            }
+            flag = acc | ((A.accept == 2 || cur < best) ? 2 : 0);
+            if (lead) {
+                A.c2_next[s] = cur;
+                if (A.accept == 2 || cur < best) A.best_c2[s] = cur;
+                if (A.accept == 1) {
                     if (acc) {
                         A.species[base_sp + pp[0]] = (uint8_t)pp[2];
                         A.species[base_sp + pp[1]] = (uint8_t)pp[3];
-                        A.acc_rung[rung] += 1;
+                        atomicAdd(reinterpret_cast<unsigned long long *>(A.acc_rung + rung), 1ull);
                     }
-                    const long long ri = (long long)stepc - 1 - A.rec_base;
                     if (A.log && ri >= 0 && ri < A.log_cap) {
                         int *L = A.log + ((long)ri * A.nrep + s) * 4;
                         L[0] = pp[0];
@@ -660,12 +711,11 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
                     if (A.incremental) {
                         if (A.reanchor_step && anchor >= A.reanchor_every) {
                             A.anchor[s] = acc;
-                            A.nanchor[s] += 1;
+                            atomicAdd(reinterpret_cast<unsigned long long *>(A.nanchor + s), 1ull);
                         } else if (acc) {
                             A.anchor[s] = anchor + 1;
                         }
                     }
-                    flag = (cur < best) ? 2 : 0;
                     if (A.rec_y) {
                         // the move record of this step: energy change and outcome
                         const long idx = (long)(((stepc - 1) * (uint64_t)A.nrep + (uint64_t)s) %
@@ -675,57 +725,68 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
                     }
                 }
             }
-            flag |= acc;
         }
         __syncthreads();
+        // ------------------------------------------------ commit of this workgroup's share
         const int acc = flag & 1;
-        if (lead && A.rec_y && A.accept == 1) {
-            // ... and the occupancy of its proposal
-            const long idx = (long)(((stepc - 1) * (uint64_t)A.nrep + (uint64_t)s) % (uint64_t)A.rec_cap);
-            if (pp[4])
-                for (int q = t; q < A.noccp; q += blockDim.x)
-                    A.rec_occ[idx * A.noccp + q] = A.pend_occ[(long)s * A.noccp + q];
+        const bool add = acc && A.accept == 1;
+#pragma unroll
+        for (int r = 0; r < RC; ++r) {
+            const int q = rbeg + t + r * nt;
+            if (q < rend && dcv[r] != 0) {
+                if (add) cn[q] = cnv[r] + dcv[r];
+                dca[q] = 0;
+            }
+        }
+        for (int q = rbeg + t + RC * nt; q < rend; q += nt) {
+            const int d = dca[q];
+            if (d != 0) {
+                if (add) cn[q] += d;
+                dca[q] = 0;
+            }
+        }
+        if (series_on && lead) {
+            __syncthreads();
+            const int nss = g.nsp * g.nsp * g.nshell;
+            for (int q = t; q < nss; q += nt) {
+                const int a = q / (g.nsp * g.nshell), r = q % (g.nsp * g.nshell);
+                A.series[(long)ri * nss + q] = cn[a * g.rstride + r];
+            }
         }
         if (A.incremental && acc) {
             double *mc = A.mcur + (long)s * g.ncol_tot;
 #pragma unroll
             for (int r = 0; r < MC; ++r) {
-                const int q = mbeg + t + r * (int)blockDim.x;
+                const int q = mbeg + t + r * nt;
                 if (q < mend) mc[q] = mt[r];
             }
-            for (int q = mbeg + t + MC * (int)blockDim.x; q < mend; q += blockDim.x)
-                mc[q] = A.mtrial[(long)s * g.ncol_tot + q];
+            for (int q = mbeg + t + MC * nt; q < mend; q += nt) mc[q] = A.mtrial[(long)s * g.ncol_tot + q];
         }
-        if (lead) {
-            int *dca = A.dc_acc + (long)s * g.nrow_tot;
-            for (int q = t; q < g.nrow_tot; q += blockDim.x) {
-                if (acc && A.accept == 1) cn[q] += dca[q];
-                dca[q] = 0;
-            }
-            const long long ri = (long long)stepc - 1 - A.rec_base;
-            if (A.accept == 1 && A.series && rung == A.series_rung && ri >= 0 && ri < A.series_cap) {
-                __syncthreads();
-                const int nss = g.nsp * g.nsp * g.nshell;
-                for (int q = t; q < nss; q += blockDim.x) {
-                    int a = q / (g.nsp * g.nshell), r = q % (g.nsp * g.nshell);
-                    A.series[(long)ri * nss + q] = cn[a * g.rstride + r];
-                }
-            }
-            if (A.accept == 2 || (flag & 2)) {
-                for (int q = t; q < g.natoms_pad; q += blockDim.x) A.best_sp[base_sp + q] = lsp[q];
-                if (t == 0 && A.accept == 1) A.best_c2[s] = A.c2_next[s];
-            }
+        if (flag & 2) {
+            const int nv = g.natoms_pad / 16;
+            const int vchunk = (nv + nwg - 1) / nwg;
+            const int vend = min(nv, (split + 1) * vchunk);
+            const uint4 *src = reinterpret_cast<const uint4 *>(lsp);
+            uint4 *dst = reinterpret_cast<uint4 *>(A.best_sp + base_sp);
+            for (int q = split * vchunk + t; q < vend; q += nt) dst[q] = src[q];
+        }
+        if (lead && A.rec_y && A.accept == 1 && pp[4]) {
+            // ... and the occupancy of its proposal
+            const long idx = (long)(((stepc - 1) * (uint64_t)A.nrep + (uint64_t)s) % (uint64_t)A.rec_cap);
+            for (int q = t; q < A.noccp; q += nt) A.rec_occ[idx * A.noccp + q] = A.pend_occ[(long)s * A.noccp + q];
         }
     }
     if (!A.propose) return;
     // ------------------------------------------------ propose + gather of the next step
-    __syncthreads();
-    if (t == 0 && A.propose == 2) {
+    if (A.propose == 2) {
         // the learned proposal (or its uniform fallback) chosen by the selection kernel
-        const int *o = A.prop_next + kPropInts * s;
-        P = Prop{o[0], o[1], o[2], o[3], o[4]};
-    } else if (t == 0) {
-        P = propose(lsp, g.natoms, A.seed, stepc + 1, (uint64_t)rung);
+        const Prop p{pn[0], pn[1], pn[2], pn[3], pn[4]};
+        if (p.sw) gather_swap(g, p, A.ss_sten, A.ms_sten, lsp, hist, first, stride, recs,
+                              A.dc_prop + (long)s * g.nrow_tot);
+        return;
+    }
+    if (t == 0) {
+        P = propose_pick(lsp, dr);
         if (lead) {
             int *o = A.prop_next + kPropInts * s;
             o[0] = P.a;
@@ -738,11 +799,11 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     }
     __syncthreads();
     const Prop p = P;
-    if (lead && A.pend_occ && p.sw && A.propose == 1) {
+    if (lead && A.pend_occ && p.sw) {
         int ax, ay, az, bx, by, bz;
         site_xyz_of(g, p.a, ax, ay, az);
         site_xyz_of(g, p.b, bx, by, bz);
-        for (int q = t; q < A.nocc; q += blockDim.x)
+        for (int q = t; q < A.nocc; q += nt)
             A.pend_occ[(long)s * A.noccp + q] =
                 lsp[occ_site(g, A.ss_sten, p.a, p.b, ax, ay, az, bx, by, bz, q)];
     }
@@ -1199,7 +1260,7 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
     int rc;
     // per-column weight 1/sigma^2 (Re and Im share a point's sigma) and, per edge,
     // Q0 = sum over the window of t^2 w
-    std::vector<double> wc(g.ncol_tot), q0(2 * g.nsp, 0.0);
+    std::vector<double> wc(g.ncol_tot), q0(2 * chm::kMaxNsp, 0.0);   // padded: read as one block
     for (int q = 0; q < g.ncol_tot / 2; ++q) {
         if (!(sigma[q] > 0.0)) return fail((chm::set_error("sigma must be positive"), CHM_E_ARG));
         wc[2 * q] = wc[2 * q + 1] = 1.0 / (sigma[q] * sigma[q]);

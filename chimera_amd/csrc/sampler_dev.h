@@ -35,6 +35,27 @@ __device__ __forceinline__ float bf16_to_float(unsigned short h) {
     return __builtin_bit_cast(float, (unsigned int)h << 16);
 }
 
+// Bounds-checked loads through a buffer resource (raw buffer, stride 0): a lane whose
+// byte offset is at or beyond the resource's size gets zero and makes no memory access,
+// without a branch, so a block of such loads is issued back to back and waited on once.
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    uint32_t bytes;
+};
+__device__ __forceinline__ Buf make_buf(const void *p, uint32_t bytes) {
+    return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000),
+               bytes};
+}
+__device__ __forceinline__ double buf_f64(const Buf &b, bool ok, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(b.r, ok ? off : b.bytes, 0, 0));
+}
+__device__ __forceinline__ int buf_i32(const Buf &b, bool ok, uint32_t off) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(b.r, ok ? off : b.bytes, 0, 0);
+}
+__device__ __forceinline__ int4 buf_i128(const Buf &b, bool ok, uint32_t off) {
+    return __builtin_bit_cast(int4, __builtin_amdgcn_raw_buffer_load_b128(b.r, ok ? off : b.bytes, 0, 0));
+}
+
 struct Geo {
     int natoms, natoms_pad, nsp, ncoord, nshell, nblk, nstencil, nabsrec;
     int nrow_e, rstride, nrow_tot, nss_rows, ncol_e, ncol_tot, nkw, nwin;  // rstride: padded rows per edge
@@ -87,6 +108,37 @@ __device__ inline Prop propose(const uint8_t *sp, int natoms, uint64_t seed, uin
     return p;
 }
 
+// The same proposal in two halves: the draws (a and the 8 candidates) do not depend on
+// the configuration, so they can be made before it is final; the pick reads it.
+struct PropDraws {
+    int a;
+    int c[8];
+};
+__device__ __forceinline__ PropDraws propose_draws(int natoms, uint64_t seed, uint64_t stepc, uint64_t key) {
+    uint64_t h = splitmix64(seed ^ (stepc * kGolden) ^ (key << 32));
+    h = splitmix64(h);
+    h = splitmix64(h);
+    PropDraws d;
+    d.a = min((int)(u01(h) * natoms), natoms - 1);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        h = splitmix64(h);
+        d.c[t] = min((int)(u01(h) * natoms), natoms - 1);
+    }
+    return d;
+}
+__device__ __forceinline__ Prop propose_pick(const uint8_t *sp, const PropDraws &d) {
+    const int sa = sp[d.a];
+    int sc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) sc[t] = sp[d.c[t]];
+    Prop p{d.a, d.a, sa, sa, 0};
+#pragma unroll
+    for (int t = 7; t >= 0; --t)
+        if (sc[t] != sa) p = Prop{d.a, d.c[t], sc[t], sa, 1};
+    return p;
+}
+
 __device__ __forceinline__ void load_species(uint8_t *lsp, const uint8_t *gsp, int natoms_pad) {
     const uint4 *src = reinterpret_cast<const uint4 *>(gsp);
     uint4 *dst = reinterpret_cast<uint4 *>(lsp);
@@ -109,16 +161,16 @@ struct SwapSite {
     int ax, ay, az, bx, by, bz;
 };
 
-// One gather item of a swap: item < nss is a neighbour of a (first ncoord) or of b; the
-// rest are path records of a's then b's incidence stencil.  Count changes go to `hist`.
-__device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W,
-                                            const int *__restrict__ ss_sten, const uint8_t *lsp,
+// One gather item of a swap: item < nss is a neighbour of a (first ncoord) or of b (rec.x
+// holds its packed stencil entry); the rest are path records `rec` of a's then b's
+// incidence stencil.  Count changes go to `hist`.
+__device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W, const uint8_t *lsp,
                                             int *hist, int it, int nss, const int4 rec) {
     const Prop &p = W.p;
     if (it < nss) {
         // ordered pairs x->y and y->x for every neighbour y of x in {a, b}
         const bool on_a = it < g.ncoord;
-        const int sv = ss_sten[on_a ? it : it - g.ncoord];
+        const int sv = rec.x;   // the neighbour stencil entry of this item
         const int dx = (int)(int8_t)(sv & 0xFF), dy = (int)(int8_t)((sv >> 8) & 0xFF);
         const int dz = (int)(int8_t)((sv >> 16) & 0xFF), sh = (sv >> 24) & 0xFF;
         const int x = on_a ? p.a : p.b;
@@ -176,19 +228,40 @@ __device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W,
 }
 
 
-constexpr int kRecPrefetch = 8;   // multiple-scattering records per lane loaded ahead
+constexpr int kRecPrefetch = 4;   // gather records per lane loaded ahead
+
+// The record of gather item `it` (< 2 ncoord + 2 nstencil): the packed neighbour stencil
+// entry (in .x) or the path record.
+__device__ __forceinline__ int4 item_rec(const Geo &g, const int *__restrict__ ss_sten,
+                                         const int4 *__restrict__ ms_sten, int it) {
+    const int nss_items = 2 * g.ncoord;
+    if (it < nss_items) return make_int4(ss_sten[it < g.ncoord ? it : it - g.ncoord], 0, 0, 0);
+    const int m = it - nss_items;
+    return ms_sten[m < g.nstencil ? m : m - g.nstencil];
+}
 
 // Gather items of a swap split over the workgroups of one replica slot: item `first +
 // r * stride`.  The first kRecPrefetch records of this lane do not depend on the
-// proposal, so they can be requested before it is known.
-__device__ __forceinline__ void prefetch_recs(const Geo &g, const int4 *__restrict__ ms_sten, int first,
-                                              int stride, bool on, int4 (&recs)[kRecPrefetch]) {
+// proposal, so they are requested before it is known: both candidate records (neighbour
+// stencil entry and path record) are loaded with clamped, in-bounds indexes and no
+// branch; the gather picks the one its item needs.
+struct RecPre {
+    int4 ms[kRecPrefetch];
+    int ss[kRecPrefetch];
+};
+__device__ __forceinline__ void prefetch_recs(const Geo &g, const int *__restrict__ ss_sten,
+                                              const int4 *__restrict__ ms_sten, int first,
+                                              int stride, bool on, RecPre &R) {
     const int nss_items = 2 * g.ncoord;
+    const int nitems = nss_items + 2 * g.nstencil;
+    const Buf bs = make_buf(ss_sten, 4u * g.ncoord);
+    const Buf bm = make_buf(g.nstencil > 0 ? (const void *)ms_sten : (const void *)ss_sten, 16u * g.nstencil);
 #pragma unroll
     for (int r = 0; r < kRecPrefetch; ++r) {
-        recs[r] = make_int4(0, 0, 0, 0);
-        const int m = first + r * stride - nss_items;
-        if (on && m >= 0 && m < 2 * g.nstencil) recs[r] = ms_sten[m < g.nstencil ? m : m - g.nstencil];
+        const int it = first + r * stride;
+        const int m = it - nss_items;
+        R.ss[r] = buf_i32(bs, on && it < nss_items, 4u * (it < g.ncoord ? it : it - g.ncoord));
+        R.ms[r] = buf_i128(bm, on && m >= 0 && it < nitems, 16u * (m < g.nstencil ? m : m - g.nstencil));
     }
 }
 
@@ -197,8 +270,7 @@ __device__ __forceinline__ void prefetch_recs(const Geo &g, const int4 *__restri
 // with integer atomics (order-independent, so the merged row is deterministic).
 __device__ __forceinline__ void gather_swap(const Geo &g, const Prop &p, const int *__restrict__ ss_sten,
                                             const int4 *__restrict__ ms_sten, const uint8_t *lsp,
-                                            int *hist, int first, int stride,
-                                            const int4 (&recs)[kRecPrefetch], int *dc) {
+                                            int *hist, int first, int stride, const RecPre &R, int *dc) {
     SwapSite W;
     W.p = p;
     site_xyz_of(g, p.a, W.ax, W.ay, W.az);
@@ -208,13 +280,11 @@ __device__ __forceinline__ void gather_swap(const Geo &g, const Prop &p, const i
 #pragma unroll
     for (int r = 0; r < kRecPrefetch; ++r) {
         const int it = first + r * stride;
-        if (it < nitems) gather_item(g, W, ss_sten, lsp, hist, it, nss_items, recs[r]);
+        if (it < nitems)
+            gather_item(g, W, lsp, hist, it, nss_items, it < nss_items ? make_int4(R.ss[r], 0, 0, 0) : R.ms[r]);
     }
-    for (int it = first + kRecPrefetch * stride; it < nitems; it += stride) {
-        const int m = it - nss_items;
-        const int4 rec = (m >= 0) ? ms_sten[m < g.nstencil ? m : m - g.nstencil] : make_int4(0, 0, 0, 0);
-        gather_item(g, W, ss_sten, lsp, hist, it, nss_items, rec);
-    }
+    for (int it = first + kRecPrefetch * stride; it < nitems; it += stride)
+        gather_item(g, W, lsp, hist, it, nss_items, item_rec(g, ss_sten, ms_sten, it));
     __syncthreads();
     for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x)
         if (hist[q] != 0) atomicAdd(&dc[q], hist[q]);
