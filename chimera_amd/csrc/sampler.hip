@@ -218,7 +218,7 @@ __device__ __forceinline__ void gemm_abs_body(const Geo &g, int nrep, const int 
     }
     if (!reanchor && O.bump && ct == 0 && e == 0 && threadIdx.x < RB &&
         r0 + (int)threadIdx.x < nrep)
-        O.stepc[r0 + threadIdx.x] += 1;
+        atomicAdd(reinterpret_cast<unsigned long long *>(O.stepc + r0 + threadIdx.x), 1ull);
 }
 
 __global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int *__restrict__ counts,
@@ -362,7 +362,7 @@ __device__ __forceinline__ void gemm_delta_bf16_body(const Geo &g, int nrep, con
         }
     }
     if (O.bump && ct == 0 && e == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
-        O.stepc[r0 + threadIdx.x] += 1;
+        atomicAdd(reinterpret_cast<unsigned long long *>(O.stepc + r0 + threadIdx.x), 1ull);
 }
 
 __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const int *__restrict__ dcounts,
@@ -425,7 +425,7 @@ __device__ __forceinline__ void gemm_delta_f32_body(const Geo &g, int nrep, cons
         tile_epilogue(g, O, nrep, slot, e, ct, c, mp);
     }
     if (O.bump && ct == 0 && e == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
-        O.stepc[r0 + threadIdx.x] += 1;
+        atomicAdd(reinterpret_cast<unsigned long long *>(O.stepc + r0 + threadIdx.x), 1ull);
 }
 
 __global__ __launch_bounds__(256) void k_gemm_delta_f32(Geo g, int nrep, const int *__restrict__ dcounts,
@@ -656,6 +656,7 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     }
     PropDraws dr{};
     if (A.propose == 1) dr = propose_draws(g.natoms, A.seed, stepc + 1, (uint64_t)rung);
+    __builtin_amdgcn_sched_barrier(0);   // every load issued before the first use
     // ------------------------------------------------ consume
     if (t < nv) reinterpret_cast<int4 *>(lsp)[t] = spv;
     for (int q = t + nt; q < nv; q += nt) reinterpret_cast<uint4 *>(lsp)[q] = gsp[q];
@@ -1148,6 +1149,18 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         g.bxyz[3 * b + 1] = (p >> 1) & 1;
         g.bxyz[3 * b + 2] = (p >> 2) & 1;
     }
+    g.bpar4 = 0;
+    g.bxyz3 = 0;
+    for (int q = 0; q < 8; ++q) {
+        CHM_REQUIRE(g.bpar[q] >= -1 && g.bpar[q] < g.nbasis, CHM_E_ARG, "basis_of_parity out of range");
+        g.bpar4 |= (uint32_t)(g.bpar[q] & 15) << (4 * q);
+    }
+    for (int q = 0; q < 3 * g.nbasis; ++q) g.bxyz3 |= (uint32_t)g.bxyz[q] << q;
+    CHM_REQUIRE(g.natoms < (1 << 20), CHM_E_ARG, "at most 2^20 sites");
+    CHM_REQUIRE(g.nbasis >= 1 && g.ncell[1] >= 1 && g.ncell[2] >= 1, CHM_E_ARG, "empty cell grid");
+    g.rnb = 1.0f / (float)g.nbasis;
+    g.rnc1 = 1.0f / (float)g.ncell[1];
+    g.rnc2 = 1.0f / (float)g.ncell[2];
     // host-side validation of every index a kernel will follow
     for (long q = 0; q < (long)g.natoms * g.ncoord; ++q) {
         CHM_REQUIRE(nbr_j[q] >= 0 && nbr_j[q] < g.natoms, CHM_E_ARG, "neighbour out of range");

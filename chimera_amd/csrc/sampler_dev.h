@@ -64,10 +64,16 @@ struct Geo {
     int has_const;
     int nct, nkb;   // 16-column tiles per edge, 32-row K blocks per edge
     int bxyz[12];   // coordinates (units a0/2) of each basis site in the cell
+    // the same two tables packed into one register each, so a per-lane lookup is a shift
+    // (an indexed read of the arrays above would be a memory load per lane):
+    // bpar4 nibble q = bpar[q] (4-bit two's complement), bxyz3 bit 3b + d = bxyz[3b + d]
+    uint32_t bpar4, bxyz3;
+    float rnb, rnc1, rnc2;   // 1/nbasis, 1/ncell[1], 1/ncell[2] (float) for div_small
 };
 
 __device__ __forceinline__ int site_index(const Geo &g, int x, int y, int z) {
-    int b = g.bpar[(x & 1) | ((y & 1) << 1) | ((z & 1) << 2)];
+    const int par = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2);
+    const int b = (int)(g.bpar4 << (28 - 4 * par)) >> 28;
     int cell = ((x >> 1) * g.ncell[1] + (y >> 1)) * g.ncell[2] + (z >> 1);
     return cell * g.nbasis + b;
 }
@@ -145,15 +151,27 @@ __device__ __forceinline__ void load_species(uint8_t *lsp, const uint8_t *gsp, i
     for (int q = threadIdx.x; q < natoms_pad / 16; q += blockDim.x) dst[q] = src[q];
 }
 
-// integer coordinates (units a0/2) of a site, from its index
+// n / d for 0 <= n < 2^20 and d >= 1, from a float reciprocal of d: the truncated estimate
+// is within one of the quotient (relative error below 2^-21), and one correction each way
+// makes it exact.  A few VALU operations instead of the generic integer division.
+__device__ __forceinline__ int div_small(int n, int d, float rcp) {
+    int q = (int)((float)n * rcp);
+    const int r = n - q * d;
+    q += (r >= d) ? 1 : 0;
+    q -= (r < 0) ? 1 : 0;
+    return q;
+}
+
+// integer coordinates (units a0/2) of a site, from its index (natoms < 2^20)
 __device__ __forceinline__ void site_xyz_of(const Geo &g, int q, int &x, int &y, int &z) {
-    const int cell = q / g.nbasis, b = q - cell * g.nbasis;
-    const int cz = cell % g.ncell[2];
-    const int cy = (cell / g.ncell[2]) % g.ncell[1];
-    const int cx = cell / (g.ncell[2] * g.ncell[1]);
-    x = 2 * cx + g.bxyz[3 * b];
-    y = 2 * cy + g.bxyz[3 * b + 1];
-    z = 2 * cz + g.bxyz[3 * b + 2];
+    const int cell = div_small(q, g.nbasis, g.rnb), b = q - cell * g.nbasis;
+    const int c12 = div_small(cell, g.ncell[2], g.rnc2);
+    const int cz = cell - c12 * g.ncell[2];
+    const int cx = div_small(c12, g.ncell[1], g.rnc1);
+    const int cy = c12 - cx * g.ncell[1];
+    x = 2 * cx + (int)((g.bxyz3 >> (3 * b)) & 1u);
+    y = 2 * cy + (int)((g.bxyz3 >> (3 * b + 1)) & 1u);
+    z = 2 * cz + (int)((g.bxyz3 >> (3 * b + 2)) & 1u);
 }
 
 struct SwapSite {
