@@ -71,6 +71,7 @@ _SIGS = {
     "chm_sampler_set_precision": (C.c_int, [vp, C.c_int, C.c_int]),
     "chm_sampler_set_tuning": (C.c_int, [vp, C.c_int]),
     "chm_sampler_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "chm_sampler_set_pairs": (C.c_int, [vp, C.c_int]),
     "chm_lp_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                               C.POINTER(C.c_int)]),
     "chm_lp_set_params": (C.c_int, [vp, vp]),

@@ -101,7 +101,7 @@ constexpr int NCW = 16;   // columns per tile
 constexpr int KS = 16;    // K slices of the FP64 kernel
 
 struct TileOut {
-    uint64_t *stepc;        // per slot: advanced once per transform when bump is set
+    uint64_t *stepc;        // per slot: advanced by `bump` steps per transform
     int bump;
     const int *anchor;      // re-anchor mode: slots with anchor[s] >= anchor_min get
     int anchor_min;         //   anchor_out[s] = counts[s] @ basis_R (FP64); nothing else
@@ -242,11 +242,62 @@ __device__ __forceinline__ __bf16 bf16_of_small_int(int v) {
 // B[8(l/16)+i][l%16] (i < 8); D lane l holds D[4(l/16)+i][l%16] (i < 4).
 constexpr int KBW = 4;   // K blocks per wave held in registers (nkb <= 4 * KBW)
 
-__device__ __forceinline__ void gemm_delta_bf16_body(const Geo &g, int nrep, const int *__restrict__ dcounts,
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// two integers in [-256, 256] as a packed bf16 pair (exact: the float's upper half)
+__device__ __forceinline__ unsigned pk_bf16_small(int v0, int v1) {
+    return __builtin_amdgcn_perm(__builtin_bit_cast(unsigned, (float)v1), __builtin_bit_cast(unsigned, (float)v0),
+                                 0x07060302u);
+}
+
+// The A operand of one K block: 8 small integer counts per lane as bf16 lo (and hi when
+// |v| > 128), into acc (and acch).  When every value of the wave lies in [-128, 127]
+// (nearly always: a swap changes few counts by small amounts) lo is the value itself,
+// packed with one conversion and one byte permute per pair, and hi is zero.
+__device__ __forceinline__ void mfma_kblock(const int4 (&a)[2], const bf16x8 (&B)[3], f32x4 &acc, f32x4 &acch) {
+    const int vals[8] = {a[0].x, a[0].y, a[0].z, a[0].w, a[1].x, a[1].y, a[1].z, a[1].w};
+    int wide = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) wide |= (vals[i] + 128) & ~255;
+    if (!__any(wide)) {
+        const u32x4 w = {pk_bf16_small(vals[0], vals[1]), pk_bf16_small(vals[2], vals[3]),
+                         pk_bf16_small(vals[4], vals[5]), pk_bf16_small(vals[6], vals[7])};
+        const bf16x8 alo = __builtin_bit_cast(bf16x8, w);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[2], acc, 0, 0, 0);
+        return;
+    }
+    bf16x8 alo, ahi;
+    bool hi_any = false;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int v = vals[i];
+        const int lo = ((v + 128) & 255) - 128;
+        const int hi = (v - lo) >> 8;
+        hi_any |= (hi != 0);
+        alo[i] = bf16_of_small_int(lo);
+        ahi[i] = bf16_of_small_int(hi);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[2], acc, 0, 0, 0);
+    if (__any(hi_any)) {
+        acch = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, B[0], acch, 0, 0, 0);
+        acch = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, B[1], acch, 0, 0, 0);
+        acch = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, B[2], acch, 0, 0, 0);
+    }
+}
+
+// NA = 1: m' = m_cur + dc[0] @ B into O[0].  NA = 3 (a paired round): m1 = m_cur +
+// dc[0] @ B, m2r = m_cur + dc[1] @ B and m2a = m1 + dc[2] @ B into O[0..2], each product
+// accumulated in FP32 on its own and added to the FP64 value, exactly as in single steps.
+template <int NA, int KB>
+__device__ __forceinline__ void gemm_delta_bf16_tile(const Geo &g, int nrep, const int *const (&dc)[NA],
                                                      const bf16x8 *__restrict__ Tb,
-                                                     const double *__restrict__ mcur, const TileOut &O, int ct,
+                                                     const double *__restrict__ mcur, const TileOut *O, int ct,
                                                      int rbi, int e) {
-    __shared__ float red[4][RB][NCW + 1];
+    __shared__ float red[NA][4][RB][NCW + 1];
     const int r0 = rbi * RB;
     const int t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63;
@@ -258,16 +309,16 @@ __device__ __forceinline__ void gemm_delta_bf16_body(const Geo &g, int nrep, con
         const bool v = slot < nrep && col < g.ncol_e;
         const long gc = (long)e * g.ncol_e + col;
         mc_pre[h] = v ? mcur[(long)slot * g.ncol_tot + gc] : 0.0;
-        tg_pre[h] = v ? O.target[gc] : 0.0;
-        w_pre[h] = v ? O.wcol[gc] : 0.0;
+        tg_pre[h] = v ? O[0].target[gc] : 0.0;
+        w_pre[h] = v ? O[0].wcol[gc] : 0.0;
     }
     const int per = (g.nkb + 3) / 4;
     const int kb0 = wave * per, kb1 = min(g.nkb, kb0 + per);
     // all operands of this wave's K blocks, loaded before any MFMA
-    bf16x8 B[KBW][3];
-    int4 Av[KBW][2][2];
+    bf16x8 B[KB][3];
+    int4 Av[NA][KB][2][2];
 #pragma unroll
-    for (int q = 0; q < KBW; ++q) {
+    for (int q = 0; q < KB; ++q) {
         const int kb = kb0 + q;
         if (kb < kb1) {
             const bf16x8 *Bp = Tb + ((((long)e * g.nct + ct) * g.nkb + kb) * 3) * 64 + lane;
@@ -276,93 +327,91 @@ __device__ __forceinline__ void gemm_delta_bf16_body(const Geo &g, int nrep, con
             B[q][2] = Bp[128];
             const int k0 = kb * 32 + 8 * (lane >> 4);
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb) {
-                const int slot = r0 + rb * 16 + (lane & 15);
-                if (slot < nrep) {
-                    const int4 *src = reinterpret_cast<const int4 *>(
-                        dcounts + (long)slot * g.nrow_tot + (long)e * g.rstride + k0);
-                    Av[q][rb][0] = src[0];
-                    Av[q][rb][1] = src[1];
-                } else {
-                    Av[q][rb][0] = make_int4(0, 0, 0, 0);
-                    Av[q][rb][1] = make_int4(0, 0, 0, 0);
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) {
+                    const int slot = r0 + rb * 16 + (lane & 15);
+                    if (slot < nrep) {
+                        const int4 *src = reinterpret_cast<const int4 *>(
+                            dc[a] + (long)slot * g.nrow_tot + (long)e * g.rstride + k0);
+                        Av[a][q][rb][0] = src[0];
+                        Av[a][q][rb][1] = src[1];
+                    } else {
+                        Av[a][q][rb][0] = make_int4(0, 0, 0, 0);
+                        Av[a][q][rb][1] = make_int4(0, 0, 0, 0);
+                    }
                 }
-            }
-        }
-    }
-    f32x4 acc[2], acch[2];
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[rb][i] = acch[rb][i] = 0.f;
-#pragma unroll
-    for (int q = 0; q < KBW; ++q) {
-        if (kb0 + q >= kb1) break;
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-            const int vals[8] = {Av[q][rb][0].x, Av[q][rb][0].y, Av[q][rb][0].z, Av[q][rb][0].w,
-                                 Av[q][rb][1].x, Av[q][rb][1].y, Av[q][rb][1].z, Av[q][rb][1].w};
-            bf16x8 alo, ahi;
-            bool hi_any = false;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int v = vals[i];
-                const int lo = ((v + 128) & 255) - 128;
-                const int hi = (v - lo) >> 8;
-                hi_any |= (hi != 0);
-                alo[i] = bf16_of_small_int(lo);
-                ahi[i] = bf16_of_small_int(hi);
-            }
-            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[q][0], acc[rb], 0, 0, 0);
-            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[q][1], acc[rb], 0, 0, 0);
-            acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, B[q][2], acc[rb], 0, 0, 0);
-            if (__any(hi_any)) {
-                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, B[q][0], acch[rb], 0, 0, 0);
-                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, B[q][1], acch[rb], 0, 0, 0);
-                acch[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, B[q][2], acch[rb], 0, 0, 0);
-            }
         }
     }
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+    for (int a = 0; a < NA; ++a) {
+        f32x4 acc[2], acch[2];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            red[wave][rb * 16 + 4 * (lane >> 4) + i][lane & 15] = acc[rb][i] + 256.f * acch[rb][i];
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[rb][i] = acch[rb][i] = 0.f;
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            if (kb0 + q >= kb1) break;
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) mfma_kblock(Av[a][q][rb], B[q], acc[rb], acch[rb]);
+        }
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                red[a][wave][rb * 16 + 4 * (lane >> 4) + i][lane & 15] = acc[rb][i] + 256.f * acch[rb][i];
+    }
     __syncthreads();
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int m = t / NCW + 16 * h, cl = t % NCW;
-        const float d = red[0][m][cl] + red[1][m][cl] + red[2][m][cl] + red[3][m][cl];
         const int slot = r0 + m, col = ct * NCW + cl;
         const bool valid = slot < nrep && col < g.ncol_e;
-        double mp = valid ? mc_pre[h] + (double)d : 0.0;
-        if (O.acc32) mp = (double)(float)mp;
-        double p1 = 0.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
-        if (valid) {
-            O.mtrial[(long)slot * g.ncol_tot + (long)e * g.ncol_e + col] = mp;
-            const double dd = mp - tg_pre[h];
-            p1 = dd * tg_pre[h];
-            p2 = dd * dd;
-            q1 = p1 * w_pre[h];
-            q2 = p2 * w_pre[h];
-        }
+        double m1 = 0.0;
 #pragma unroll
-        for (int mm = 8; mm >= 1; mm >>= 1) {
-            p1 += __shfl_xor(p1, mm, 16);
-            p2 += __shfl_xor(p2, mm, 16);
-            q1 += __shfl_xor(q1, mm, 16);
-            q2 += __shfl_xor(q2, mm, 16);
-        }
-        if (slot < nrep && cl == 0) {
-            double *P = O.partials + (((long)slot * g.nsp + e) * g.nct + ct) * 4;
-            P[0] = p1;
-            P[1] = p2;
-            P[2] = q1;
-            P[3] = q2;
+        for (int a = 0; a < NA; ++a) {
+            const float d = red[a][0][m][cl] + red[a][1][m][cl] + red[a][2][m][cl] + red[a][3][m][cl];
+            const TileOut &Oa = O[a];
+            double mp = valid ? ((a == 2) ? m1 : mc_pre[h]) + (double)d : 0.0;
+            if (Oa.acc32) mp = (double)(float)mp;
+            if (a == 0) m1 = mp;
+            double p1 = 0.0, p2 = 0.0, q1 = 0.0, q2 = 0.0;
+            if (valid) {
+                Oa.mtrial[(long)slot * g.ncol_tot + (long)e * g.ncol_e + col] = mp;
+                const double dd = mp - tg_pre[h];
+                p1 = dd * tg_pre[h];
+                p2 = dd * dd;
+                q1 = p1 * w_pre[h];
+                q2 = p2 * w_pre[h];
+            }
+#pragma unroll
+            for (int mm = 8; mm >= 1; mm >>= 1) {
+                p1 += __shfl_xor(p1, mm, 16);
+                p2 += __shfl_xor(p2, mm, 16);
+                q1 += __shfl_xor(q1, mm, 16);
+                q2 += __shfl_xor(q2, mm, 16);
+            }
+            if (slot < nrep && cl == 0) {
+                double *P = Oa.partials + (((long)slot * g.nsp + e) * g.nct + ct) * 4;
+                P[0] = p1;
+                P[1] = p2;
+                P[2] = q1;
+                P[3] = q2;
+            }
         }
     }
-    if (O.bump && ct == 0 && e == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
-        atomicAdd(reinterpret_cast<unsigned long long *>(O.stepc + r0 + threadIdx.x), 1ull);
+    if (O[0].bump && ct == 0 && e == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
+        atomicAdd(reinterpret_cast<unsigned long long *>(O[0].stepc + r0 + threadIdx.x),
+                  (unsigned long long)O[0].bump);
+}
+
+__device__ __forceinline__ void gemm_delta_bf16_body(const Geo &g, int nrep, const int *__restrict__ dcounts,
+                                                     const bf16x8 *__restrict__ Tb,
+                                                     const double *__restrict__ mcur, const TileOut &O, int ct,
+                                                     int rbi, int e) {
+    const int *const dc[1] = {dcounts};
+    gemm_delta_bf16_tile<1, KBW>(g, nrep, dc, Tb, mcur, &O, ct, rbi, e);
 }
 
 __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const int *__restrict__ dcounts,
@@ -649,13 +698,16 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     prefetch_recs(g, A.ss_sten, A.ms_sten, first, stride, A.propose, recs);
     // the draws of this step's acceptance uniform and of the next uniform proposal
     // (wave-uniform: scalar arithmetic)
+    // (wave 0 only: lane 0 takes the decision and makes the pick)
     double u_acc = 1.0;
-    if (A.accept == 1) {
-        const uint64_t h = splitmix64(A.seed ^ (stepc * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
-        u_acc = u01(splitmix64(h));
-    }
     PropDraws dr{};
-    if (A.propose == 1) dr = propose_draws(g.natoms, A.seed, stepc + 1, (uint64_t)rung);
+    if (t < 64) {
+        if (A.accept == 1) {
+            const uint64_t h = splitmix64(A.seed ^ (stepc * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
+            u_acc = u01(splitmix64(h));
+        }
+        if (A.propose == 1) dr = propose_draws(g.natoms, A.seed, stepc + 1, (uint64_t)rung);
+    }
     __builtin_amdgcn_sched_barrier(0);   // every load issued before the first use
     // ------------------------------------------------ consume
     if (t < nv) reinterpret_cast<int4 *>(lsp)[t] = spv;
@@ -815,6 +867,439 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
 
 __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
     slot_step_body(g, A, blockIdx.y, blockIdx.x, gridDim.x);
+}
+
+// ---------------------------------------------------------------- paired steps
+// Two Metropolis steps per launch pair, by pre-fetching (Brockwell, J. Comput. Graph.
+// Stat. 15 (2006) 246): the trial of step n+2 depends only on whether step n+1 is
+// accepted, so the transform launch computes the trial of step n+1 (m1 = m + D1 B), of
+// step n+2 after a rejection (m2r = m + D2r B) and after an acceptance (m2a = m1 + D2a B),
+// each in the arithmetic of a single step; the per-replica launch below takes both
+// decisions in order, commits them, and proposes and gathers the next pair: step n+3 on
+// the final configuration, step n+4 on it (r) and on it with step n+3's swap made (a).
+// Decisions, counts, chi^2 and records are those of the steps run one at a time.
+struct PairArgs {
+    const double *partials[3];   // (nrep, nsp, nct, 4) of m1, m2r, m2a
+    const double *mtrial[3];     // (nrep, ncol_tot) m1, m2r, m2a
+    const double *q0;
+    double *mcur;
+    const double *beta_slot;
+    const int *prop_acc[3];      // this round's proposals: step 1, step 2 | rejected, | accepted
+    int *dc_acc[3];              //   and count-change rows (zeroed here)
+    int *prop_next[3];           // next round's (propose 3), or [0] the single step's (propose 1)
+    int *dc_next[3];             //   count-change rows (zero on entry)
+    const double *c2_prev;       // chi^2 before the round (read by every workgroup)
+    double *c2_next;             // chi^2 after it (written by the lead; another buffer)
+    int *counts;
+    uint8_t *species, *best_sp;
+    double *best_c2;
+    const int *rung_of;
+    const uint64_t *stepc;       // n + 2 after the round's transform (n when proposing only)
+    long long *acc_rung;
+    int *anchor;
+    long long *nanchor;
+    int reanchor_step, reanchor_every;
+    const int *ss_sten;
+    const int4 *ms_sten;
+    int *log;
+    int log_cap;
+    int *series;
+    int series_cap, series_rung;
+    long long rec_base;
+    uint64_t seed;
+    int profile;
+    int nrep;
+    int accept;                  // 0: propose only, 1: the round's two steps
+    int propose;                 // 0: nothing, 1: the single step after the round, 3: next round
+    int gr2;                     // largest squared gather offset, units (a0/2)^2
+    double nfit;
+};
+
+__device__ __forceinline__ int min_image(int d, int n) {
+    d = (d < 0) ? d + n : d;   // |d| < n
+    return (2 * d > n) ? d - n : d;
+}
+__device__ __forceinline__ void write_prop(int *o, const Prop &P) {
+    o[0] = P.a;
+    o[1] = P.b;
+    o[2] = P.na;
+    o[3] = P.nb;
+    o[4] = P.sw;
+    o[5] = 0;   // symmetric proposal: log q ratio 0
+}
+
+__device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, int s, int split, int nsplit) {
+    extern __shared__ int lds[];
+    const int R4 = (g.nrow_tot + 3) & ~3;
+    int *hist0 = lds, *hist1 = lds + R4, *hist2 = lds + 2 * R4;
+    uint8_t *lsp = reinterpret_cast<uint8_t *>(lds + 3 * R4);
+    uint8_t *lsp2 = lsp + g.natoms_pad;
+    __shared__ double sums[3][chm::kMaxNsp][4];
+    __shared__ int flag;
+    __shared__ Prop Ps[3];
+    const int nwg = nsplit;
+    const bool lead = split == 0;
+    const int t = threadIdx.x;
+    const int nt = blockDim.x;
+    if (t == 0) flag = 0;
+    const long base_sp = (long)s * g.natoms_pad;
+    // ------------------------------------------------ every input, in one round trip
+    const uint64_t stepc = A.stepc[s];
+    const int rung = A.rung_of[s];
+    const double beta = A.beta_slot[s];
+    const double c2p = A.c2_prev[s];
+    const double best = A.best_c2[s];
+    const int anchor = A.anchor[s];
+    int pr[3][kPropInts];
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+#pragma unroll
+        for (int q = 0; q < kPropInts; ++q) pr[v][q] = A.prop_acc[v][kPropInts * s + q];
+    double q0v[2 * chm::kMaxNsp];
+#pragma unroll
+    for (int q = 0; q < 2 * chm::kMaxNsp; ++q) q0v[q] = A.q0[q];
+    const int nv = g.natoms_pad / 16;
+    const uint4 *gsp = reinterpret_cast<const uint4 *>(A.species + base_sp);
+    const int4 spv = buf_i128(make_buf(gsp, 16u * nv), t < nv, 16u * t);
+    // tile sums of the three trials: (e, j) groups of CT lanes, tiles c0, c0 + CT, ...
+    const int G = 4 * g.nsp;
+    const int CT = (G <= 16) ? 16 : 8;
+    const int gi = t / CT, c0 = t - gi * CT;
+    const bool sum_lane = A.accept && t < G * CT;
+    const uint32_t poff = 8u * ((gi >> 2) * g.nct * 4 + (gi & 3));
+    constexpr int PR = 6;
+    double pv[3][PR];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const Buf bp = make_buf(A.partials[v] + (long)s * g.nsp * g.nct * 4, 32u * g.nsp * g.nct);
+#pragma unroll
+        for (int r = 0; r < PR; ++r)
+            pv[v][r] = buf_f64(bp, sum_lane && c0 + r * CT < g.nct, poff + 32u * (c0 + r * CT));
+    }
+    // this workgroup's share of the running chi(R) (one of the three trials is kept)
+    constexpr int MC = 2;
+    double mt[3][MC];
+    const int mchunk = (g.ncol_tot + nwg - 1) / nwg;
+    const int mbeg = split * mchunk, mend = min(g.ncol_tot, mbeg + mchunk);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const Buf bm = make_buf(A.mtrial[v] + (long)s * g.ncol_tot, 8u * g.ncol_tot);
+#pragma unroll
+        for (int r = 0; r < MC; ++r) {
+            const int q = mbeg + t + r * nt;
+            mt[v][r] = buf_f64(bm, A.accept && q < mend, 8u * q);
+        }
+    }
+    // ... and of the count commit; with the pair-count series of this rung recorded, the
+    // lead commits every row (and writes the series from the values it holds)
+    int *cn = A.counts + (long)s * g.nrow_tot;
+    const long long ri1 = (long long)stepc - 2 - A.rec_base, ri2 = ri1 + 1;
+    const bool series_on = A.accept && A.series && rung == A.series_rung &&
+                           ((ri1 >= 0 && ri1 < A.series_cap) || (ri2 >= 0 && ri2 < A.series_cap));
+    const int rchunk = series_on ? (lead ? g.nrow_tot : 0) : (g.nrow_tot + nwg - 1) / nwg;
+    const int rbeg = series_on ? 0 : split * rchunk, rend = min(g.nrow_tot, rbeg + rchunk);
+    constexpr int RC = 2;
+    int cnv[RC], dcv[3][RC];
+    {
+        const Buf bcn = make_buf(cn, 4u * g.nrow_tot);
+#pragma unroll
+        for (int r = 0; r < RC; ++r) cnv[r] = buf_i32(bcn, A.accept && rbeg + t + r * nt < rend, 4u * (rbeg + t + r * nt));
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            const Buf bd = make_buf(A.dc_acc[v] + (long)s * g.nrow_tot, 4u * g.nrow_tot);
+#pragma unroll
+            for (int r = 0; r < RC; ++r)
+                dcv[v][r] = buf_i32(bd, A.accept && rbeg + t + r * nt < rend, 4u * (rbeg + t + r * nt));
+        }
+    }
+    const int stride = nwg * nt;
+    const int first = split * nt + t;
+    RecPre recs;
+    prefetch_recs(g, A.ss_sten, A.ms_sten, first, stride, A.propose != 0, recs);
+    // the round's acceptance uniforms and the next proposals' draws (wave-uniform)
+    // (wave 0 only: lane 0 takes the decisions and makes the picks)
+    double u1 = 1.0, u2 = 1.0;
+    PropDraws dr3{}, dr4{};
+    if (t < 64) {
+        if (A.accept) {
+            uint64_t h = splitmix64(A.seed ^ ((stepc - 1) * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
+            u1 = u01(splitmix64(h));
+            h = splitmix64(A.seed ^ (stepc * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
+            u2 = u01(splitmix64(h));
+        }
+        if (A.propose) dr3 = propose_draws(g.natoms, A.seed, stepc + 1, (uint64_t)rung);
+        if (A.propose == 3) dr4 = propose_draws(g.natoms, A.seed, stepc + 2, (uint64_t)rung);
+    }
+    __builtin_amdgcn_sched_barrier(0);   // every load issued before the first use
+    // ------------------------------------------------ consume
+    if (t < nv) reinterpret_cast<int4 *>(lsp)[t] = spv;
+    for (int q = t + nt; q < nv; q += nt) reinterpret_cast<uint4 *>(lsp)[q] = gsp[q];
+    if (A.accept) {
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            double psum = 0.0;   // out-of-range tiles loaded as 0.0
+#pragma unroll
+            for (int r = 0; r < PR; ++r) psum += pv[v][r];
+            const double *Pp = A.partials[v] + (long)s * g.nsp * g.nct * 4;
+            for (int c = c0 + PR * CT; sum_lane && c < g.nct; c += CT)
+                psum += Pp[(long)((gi >> 2) * g.nct + c) * 4 + (gi & 3)];
+            for (int m = CT / 2; m >= 1; m >>= 1) psum += __shfl_xor(psum, m, CT);
+            if (t < G * CT && c0 == 0) sums[v][gi >> 2][gi & 3] = psum;
+        }
+    }
+    if (A.propose)
+        for (int q = t; q < 3 * R4; q += nt) lds[q] = 0;
+    __syncthreads();
+    int acc1 = 0, acc2 = 0, bestsel = 0;
+    if (A.accept) {
+        // ------------------------------------------------ both decisions (every workgroup)
+        // chi^2 of the three trials on lanes 0..2 at once
+        double c2v = 0.0;
+        if (t < 3) c2v = chi2_of_sums(g, sums[t], q0v, A.profile, A.nfit);
+        double c2_1 = 0.0, c2_2r = 0.0, c2_2a = 0.0;
+        if (t < 64) {
+            c2_1 = __shfl(c2v, 0);
+            c2_2r = __shfl(c2v, 1);
+            c2_2a = __shfl(c2v, 2);
+        }
+        if (t == 0) {
+            int p1[kPropInts];
+#pragma unroll
+            for (int q = 0; q < kPropInts; ++q) p1[q] = pr[0][q];
+            if (p1[4]) {
+                const double x = beta * (c2_1 - c2p) * A.nfit - (double)__int_as_float(p1[5]);
+                acc1 = (x <= 0.0) || (u1 < exp(-fmin(x, 60.0)));
+            }
+            const double cur1 = acc1 ? c2_1 : c2p;
+            int q2[kPropInts];
+#pragma unroll
+            for (int q = 0; q < kPropInts; ++q) q2[q] = acc1 ? pr[2][q] : pr[1][q];
+            const double c2_2 = acc1 ? c2_2a : c2_2r;
+            if (q2[4]) {
+                const double x = beta * (c2_2 - cur1) * A.nfit - (double)__int_as_float(q2[5]);
+                acc2 = (x <= 0.0) || (u2 < exp(-fmin(x, 60.0)));
+            }
+            const double cur2 = acc2 ? c2_2 : cur1;
+            if (acc1) {
+                lsp[p1[0]] = (uint8_t)p1[2];
+                lsp[p1[1]] = (uint8_t)p1[3];
+            }
+            if (acc2) {
+                lsp[q2[0]] = (uint8_t)q2[2];
+                lsp[q2[1]] = (uint8_t)q2[3];
+            }
+            // best retention after each step, in order (method_a_rmc.py:301-307)
+            double bb = best;
+            if (cur1 < bb) {
+                bb = cur1;
+                bestsel = 1;
+            }
+            if (cur2 < bb) {
+                bb = cur2;
+                bestsel = 2;
+            }
+            flag = acc1 | (acc2 << 1) | (bestsel << 2);
+            if (lead) {
+                A.c2_next[s] = cur2;
+                if (bestsel) A.best_c2[s] = bb;
+                if (acc1) {
+                    A.species[base_sp + p1[0]] = (uint8_t)p1[2];
+                    A.species[base_sp + p1[1]] = (uint8_t)p1[3];
+                }
+                if (acc2) {
+                    A.species[base_sp + q2[0]] = (uint8_t)q2[2];
+                    A.species[base_sp + q2[1]] = (uint8_t)q2[3];
+                }
+                if (acc1 + acc2)
+                    atomicAdd(reinterpret_cast<unsigned long long *>(A.acc_rung + rung), (unsigned long long)(acc1 + acc2));
+                if (A.log) {
+                    if (ri1 >= 0 && ri1 < A.log_cap) {
+                        int *L = A.log + ((long)ri1 * A.nrep + s) * 4;
+                        L[0] = p1[0];
+                        L[1] = p1[1];
+                        L[2] = acc1;
+                        L[3] = rung;
+                    }
+                    if (ri2 >= 0 && ri2 < A.log_cap) {
+                        int *L = A.log + ((long)ri2 * A.nrep + s) * 4;
+                        L[0] = q2[0];
+                        L[1] = q2[1];
+                        L[2] = acc2;
+                        L[3] = rung;
+                    }
+                }
+                if (A.reanchor_step && anchor >= A.reanchor_every) {
+                    A.anchor[s] = acc1 + acc2;
+                    atomicAdd(reinterpret_cast<unsigned long long *>(A.nanchor + s), 1ull);
+                } else if (acc1 + acc2) {
+                    A.anchor[s] = anchor + acc1 + acc2;
+                }
+            }
+        }
+        __syncthreads();
+        acc1 = flag & 1;
+        acc2 = (flag >> 1) & 1;
+        bestsel = flag >> 2;
+        const int v2 = acc1 ? 2 : 1;
+        // ------------------------------------------------ commit of this workgroup's share
+        const int nss = g.nsp * g.nshell;
+        auto commit_row = [&](int q, int c, int d1, int d2r, int d2a) {
+            const int d2 = acc1 ? d2a : d2r;
+            const int cA = c + (acc1 ? d1 : 0), cB = cA + (acc2 ? d2 : 0);
+            if (cB != c) cn[q] = cB;
+            if (d1) A.dc_acc[0][(long)s * g.nrow_tot + q] = 0;
+            if (d2r) A.dc_acc[1][(long)s * g.nrow_tot + q] = 0;
+            if (d2a) A.dc_acc[2][(long)s * g.nrow_tot + q] = 0;
+            if (series_on) {
+                const int a = q / g.rstride, r = q - a * g.rstride;
+                if (a < g.nsp && r < nss) {
+                    const int idx = a * nss + r, ns = g.nsp * nss;
+                    if (ri1 >= 0 && ri1 < A.series_cap) A.series[ri1 * ns + idx] = cA;
+                    if (ri2 >= 0 && ri2 < A.series_cap) A.series[ri2 * ns + idx] = cB;
+                }
+            }
+        };
+#pragma unroll
+        for (int r = 0; r < RC; ++r) {
+            const int q = rbeg + t + r * nt;
+            if (q < rend) commit_row(q, cnv[r], dcv[0][r], dcv[1][r], dcv[2][r]);
+        }
+        for (int q = rbeg + t + RC * nt; q < rend; q += nt) {
+            const long o = (long)s * g.nrow_tot + q;
+            commit_row(q, cn[q], A.dc_acc[0][o], A.dc_acc[1][o], A.dc_acc[2][o]);
+        }
+        // the running chi(R): the trial of the last accepted step
+        const int fv = acc2 ? v2 : (acc1 ? 0 : -1);
+        if (fv >= 0) {
+            double *mc = A.mcur + (long)s * g.ncol_tot;
+#pragma unroll
+            for (int r = 0; r < MC; ++r) {
+                const int q = mbeg + t + r * nt;
+                if (q < mend) mc[q] = (fv == 0) ? mt[0][r] : ((fv == 1) ? mt[1][r] : mt[2][r]);
+            }
+            for (int q = mbeg + t + MC * nt; q < mend; q += nt) mc[q] = A.mtrial[fv][(long)s * g.ncol_tot + q];
+        }
+        if (bestsel) {
+            // the configuration after step 1 (bestsel 1 with step 2 accepted) is the final
+            // one with step 2's swap undone, built in lsp2
+            const uint8_t *src = lsp;
+            if (bestsel == 1 && acc2) {
+                for (int q = t; q < nv; q += nt) reinterpret_cast<uint4 *>(lsp2)[q] = reinterpret_cast<const uint4 *>(lsp)[q];
+                __syncthreads();
+                if (t == 0) {
+                    const int a2 = acc1 ? pr[2][0] : pr[1][0], b2 = acc1 ? pr[2][1] : pr[1][1];
+                    const int na2 = acc1 ? pr[2][2] : pr[1][2], nb2 = acc1 ? pr[2][3] : pr[1][3];
+                    lsp2[a2] = (uint8_t)nb2;
+                    lsp2[b2] = (uint8_t)na2;
+                }
+                __syncthreads();
+                src = lsp2;
+            }
+            const int vchunk = (nv + nwg - 1) / nwg;
+            const int vend = min(nv, (split + 1) * vchunk);
+            uint4 *dst = reinterpret_cast<uint4 *>(A.best_sp + base_sp);
+            for (int q = split * vchunk + t; q < vend; q += nt) dst[q] = reinterpret_cast<const uint4 *>(src)[q];
+        }
+    }
+    if (!A.propose) return;
+    // ------------------------------------------------ propose + gather of the next step(s)
+    if (t == 0) {
+        const Prop P3 = propose_pick(lsp, dr3);
+        Ps[0] = P3;
+        if (A.propose == 3) {
+            Ps[1] = propose_pick(lsp, dr4);
+            // step n+4 after step n+3 is accepted: picked with P3's swap made
+            Ps[2] = P3.sw ? propose_pick_sp(SpOver{lsp, P3.a, P3.b, P3.na, P3.nb}, dr4) : Ps[1];
+        }
+        if (lead) {
+            write_prop(A.prop_next[0] + kPropInts * s, P3);
+            if (A.propose == 3) {
+                write_prop(A.prop_next[1] + kPropInts * s, Ps[1]);
+                write_prop(A.prop_next[2] + kPropInts * s, Ps[2]);
+            }
+        }
+    }
+    __syncthreads();
+    const Prop P3 = Ps[0], P4r = Ps[1], P4a = Ps[2];
+    const bool pair = A.propose == 3;
+    SwapSite W3, W4r, W4a;
+    W3.p = P3;
+    W4r.p = P4r;
+    W4a.p = P4a;
+    site_xyz_of(g, P3.a, W3.ax, W3.ay, W3.az);
+    site_xyz_of(g, P3.b, W3.bx, W3.by, W3.bz);
+    // the a-variant rows equal the r-variant ones when the picks agree and P3's sites lie
+    // outside the gather stencil (squared offsets <= gr2) of P4's two sites
+    bool same = true;
+    if (pair) {
+        site_xyz_of(g, P4r.a, W4r.ax, W4r.ay, W4r.az);
+        site_xyz_of(g, P4r.b, W4r.bx, W4r.by, W4r.bz);
+        if (P3.sw) {
+            same = P4a.a == P4r.a && P4a.b == P4r.b && P4a.na == P4r.na && P4a.nb == P4r.nb &&
+                   P4a.sw == P4r.sw;
+            if (same && P4r.sw) {
+                auto d2 = [&](int ux, int uy, int uz, int vx, int vy, int vz) {
+                    const int dx = min_image(vx - ux, g.box[0]), dy = min_image(vy - uy, g.box[1]),
+                              dz = min_image(vz - uz, g.box[2]);
+                    return dx * dx + dy * dy + dz * dz;
+                };
+                same = d2(W3.ax, W3.ay, W3.az, W4r.ax, W4r.ay, W4r.az) > A.gr2 &&
+                       d2(W3.ax, W3.ay, W3.az, W4r.bx, W4r.by, W4r.bz) > A.gr2 &&
+                       d2(W3.bx, W3.by, W3.bz, W4r.ax, W4r.ay, W4r.az) > A.gr2 &&
+                       d2(W3.bx, W3.by, W3.bz, W4r.bx, W4r.by, W4r.bz) > A.gr2;
+            }
+        }
+        if (!same) {
+            site_xyz_of(g, P4a.a, W4a.ax, W4a.ay, W4a.az);
+            site_xyz_of(g, P4a.b, W4a.bx, W4a.by, W4a.bz);
+        }
+    }
+    const SpOver ov{lsp, P3.a, P3.b, P3.na, P3.nb};
+    const int nss_items = 2 * g.ncoord;
+    const int nitems = nss_items + 2 * g.nstencil;
+    auto items = [&](int it, const int4 rec) {
+        if (P3.sw) gather_item(g, W3, lsp, hist0, it, nss_items, rec);
+        if (pair && P4r.sw) gather_item(g, W4r, lsp, hist1, it, nss_items, rec);
+        if (pair && !same && P4a.sw) gather_item_sp(g, W4a, ov, hist2, it, nss_items, rec);
+    };
+#pragma unroll
+    for (int r = 0; r < kRecPrefetch; ++r) {
+        const int it = first + r * stride;
+        if (it < nitems) items(it, it < nss_items ? make_int4(recs.ss[r], 0, 0, 0) : recs.ms[r]);
+    }
+    for (int it = first + kRecPrefetch * stride; it < nitems; it += stride)
+        items(it, item_rec(g, A.ss_sten, A.ms_sten, it));
+    __syncthreads();
+    int *d3 = A.dc_next[0] + (long)s * g.nrow_tot;
+    int *d4r = pair ? A.dc_next[1] + (long)s * g.nrow_tot : nullptr;
+    int *d4a = pair ? A.dc_next[2] + (long)s * g.nrow_tot : nullptr;
+    for (int q = t; q < g.nrow_tot; q += nt) {
+        const int h0 = hist0[q];
+        if (h0 != 0) atomicAdd(&d3[q], h0);
+        if (pair) {
+            const int h1 = hist1[q], h2 = same ? h1 : hist2[q];
+            if (h1 != 0) atomicAdd(&d4r[q], h1);
+            if (h2 != 0) atomicAdd(&d4a[q], h2);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_slot_pair(Geo g, PairArgs A) {
+    slot_pair_body(g, A, blockIdx.y, blockIdx.x, gridDim.x);
+}
+
+constexpr int kPairKB = 3;   // K blocks per wave in the paired transform
+
+struct PairTile {   // the three trials of a round
+    const int *dc[3];    // D1, D2r, D2a
+    TileOut O[3];        // m1, m2r, m2a (O[0] advances the step counter by two)
+};
+
+__global__ __launch_bounds__(256) void k_gemm_pair_bf16(Geo g, int nrep, const bf16x8 *__restrict__ Tb,
+                                                        const double *__restrict__ mcur, PairTile P) {
+    // one workgroup per tile computes all three trials (B read once); 3 K blocks per wave
+    // (nkb <= 12) keep the three operand sets within 2 workgroups per CU
+    gemm_delta_bf16_tile<3, kPairKB>(g, nrep, P.dc, Tb, mcur, P.O, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // chi^2 of arbitrary window transforms (one row per workgroup), direct form:
@@ -985,14 +1470,15 @@ int launch_gemm_delta(chm_sampler *S, const int *dc, int bump) {
 }
 
 // accept: 0 none, 1 step n, 2 init (n = 0).  propose: step n + 1.
-int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanchor_step = 0) {
+int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanchor_step = 0,
+                const double *c2_prev = nullptr) {
     SlotArgs A = slot_args(S, accept, propose);
     A.reanchor_step = reanchor_step;
     A.dc_acc = drow(S, n);
     A.dc_prop = drow(S, n + 1);
     A.prop_acc = prow(S, n);
     A.prop_next = prow(S, n + 1);
-    A.c2_prev = c2row(S, n - 1);
+    A.c2_prev = c2_prev ? c2_prev : c2row(S, n - 1);
     A.c2_next = c2row(S, n);
     k_slot_step<<<dim3(S->nsplit, S->nrep), 256, S->lds_k1, S->ctx->stream>>>(S->g, A);
     CHM_LAUNCH_CHECK();
@@ -1048,9 +1534,121 @@ int launch_gemm(chm_sampler *S, const int *dc, int bump) {
     return S->incremental ? launch_gemm_delta(S, dc, bump) : launch_gemm_abs(S, dc, bump);
 }
 
+// ------------------------------------------------ paired steps
+int *pdc_row(chm_sampler *S, int k, int v) {
+    return S->pdc.as<int>() + ((size_t)(k & 1) * 3 + v) * S->nrep * S->g.nrow_tot;
+}
+int *pprop_row(chm_sampler *S, int k, int v) {
+    return S->pprop.as<int>() + ((size_t)(k & 1) * 3 + v) * S->nrep * kPropInts;
+}
+double *pc2_row(chm_sampler *S, int k) { return S->pc2.as<double>() + (size_t)(k & 1) * S->nrep; }
+
+bool use_pairs(const chm_sampler *S, int L) {
+    return S->pairs && S->incremental == 1 && !S->lp.mode && !S->lp.rec_cap && L >= 3 &&
+           S->g.nkb <= 4 * kPairKB;
+}
+
+// the three trials of round k (bump: step-counter advance, 2 for a real round)
+int launch_gemm_pair(chm_sampler *S, int k, int bump) {
+    const Geo &g = S->g;
+    const size_t mt = (size_t)S->nrep * g.ncol_tot, pt = (size_t)S->nrep * g.nsp * g.nct * 4;
+    PairTile P{};
+    for (int v = 0; v < 3; ++v) {
+        P.dc[v] = pdc_row(S, k, v);
+        P.O[v] = tile_out(S);
+        P.O[v].mtrial = S->pmt.as<double>() + v * mt;
+        P.O[v].partials = S->ppart.as<double>() + v * pt;
+        P.O[v].bump = v == 0 ? bump : 0;
+    }
+    const int nrb = (S->nrep + RB - 1) / RB;
+    k_gemm_pair_bf16<<<dim3(g.nct, nrb, g.nsp), 256, 0, S->ctx->stream>>>(
+        g, S->nrep, reinterpret_cast<const bf16x8 *>(S->Tb.p), S->mcur.as<double>(), P);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+// round k: accept its two steps (accept 1) and/or propose round k + 1 (propose 3) or the
+// single step n1 (propose 1, into the single-step rows of step n1).  k = -1 proposes
+// round 0.
+int launch_slot_pair(chm_sampler *S, int accept, int propose, int k, const double *c2_prev, double *c2_next,
+                     int reanchor_step, long long n1) {
+    const Geo &g = S->g;
+    const size_t mt = (size_t)S->nrep * g.ncol_tot, pt = (size_t)S->nrep * g.nsp * g.nct * 4;
+    PairArgs A{};
+    for (int v = 0; v < 3; ++v) {
+        A.partials[v] = S->ppart.as<double>() + v * pt;
+        A.mtrial[v] = S->pmt.as<double>() + v * mt;
+        A.prop_acc[v] = pprop_row(S, k, v);
+        A.dc_acc[v] = pdc_row(S, k, v);
+        A.prop_next[v] = pprop_row(S, k + 1, v);
+        A.dc_next[v] = pdc_row(S, k + 1, v);
+    }
+    if (propose == 1) {
+        A.prop_next[0] = prow(S, n1);
+        A.dc_next[0] = drow(S, n1);
+    }
+    A.q0 = S->q0.as<double>();
+    A.mcur = S->mcur.as<double>();
+    A.beta_slot = S->beta_slot.as<double>();
+    A.c2_prev = c2_prev;
+    A.c2_next = c2_next;
+    A.counts = S->counts.as<int>();
+    A.species = S->species.as<uint8_t>();
+    A.best_sp = S->best_sp.as<uint8_t>();
+    A.best_c2 = S->best_c2.as<double>();
+    A.rung_of = S->rung_of.as<int>();
+    A.stepc = S->stepc.as<uint64_t>();
+    A.acc_rung = S->acc_rung.as<long long>();
+    A.anchor = S->anchor.as<int>();
+    A.nanchor = S->nanchor.as<long long>();
+    A.reanchor_step = reanchor_step;
+    A.reanchor_every = S->reanchor_every;
+    A.ss_sten = S->ss_sten.as<int>();
+    A.ms_sten = S->stencil.as<int4>();
+    A.log = S->log_cap ? S->log.as<int>() : nullptr;
+    A.log_cap = S->log_cap;
+    A.series = S->series_cap ? S->series.as<int>() : nullptr;
+    A.series_cap = S->series_cap;
+    A.series_rung = S->series_rung;
+    A.rec_base = S->rec_base;
+    A.seed = S->seed;
+    A.profile = S->profile;
+    A.nrep = S->nrep;
+    A.accept = accept;
+    A.propose = propose;
+    A.gr2 = S->gr2;
+    A.nfit = S->nfit;
+    k_slot_pair<<<dim3(S->nsplit, S->nrep), 256, S->lds_pair, S->ctx->stream>>>(g, A);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+// L >= 3 steps after h as L/2 paired rounds (and one single step if L is odd).  chi^2
+// moves c2row(h) -> pc2(0) -> pc2(1) -> ... and ends in c2row(h + L), where every other
+// path expects it; no launch reads the buffer it writes.
+int launch_segment_pairs(chm_sampler *S, long long h, int L) {
+    const int R = L / 2, odd = L & 1;
+    const int anchor = S->reanchor_every > 0;
+    int rc = launch_slot_pair(S, 0, 3, -1, c2row(S, h), pc2_row(S, 0), 0, h + L);   // round 0's rows
+    if (!rc && anchor) rc = launch_reanchor(S);
+    for (int k = 0; k < R && !rc; ++k) {
+        rc = launch_gemm_pair(S, k, 2);
+        const int prop = (k + 1 < R) ? 3 : (odd ? 1 : 0);
+        const double *in = (k == 0) ? c2row(S, h) : pc2_row(S, k - 1);
+        double *out = (k == R - 1 && !odd) ? c2row(S, h + L) : pc2_row(S, k);
+        if (!rc) rc = launch_slot_pair(S, 1, prop, k, in, out, anchor && k == 0, h + L);
+    }
+    if (odd && !rc) {
+        rc = launch_gemm(S, drow(S, h + L), 1);
+        if (!rc) rc = launch_slot(S, 1, 0, h + L, 0, pc2_row(S, R - 1));
+    }
+    return rc;
+}
+
 // L steps after h completed ones, no exchange inside: propose step h+1, then for every
 // step n: transform (advances the device step counter), accept [+ propose n+1].
 int launch_segment(chm_sampler *S, long long h, int L) {
+    if (use_pairs(S, L)) return launch_segment_pairs(S, h, L);
     int rc = launch_propose(S, h);
     // incremental mode: replicas with reanchor_every accepted moves since their last FP64
     // recompute get the running chi(R) recomputed from the integer counts, here
@@ -1246,6 +1844,13 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         CHM_REQUIRE((q < g.nabsrec) == (rb[0] == 0 && rb[1] == 0 && rb[2] == 0), CHM_E_ARG,
                     "absorber-role records must come first");
     }
+    // largest squared offset a gather visits (neighbours and path sites)
+    int gr2 = r2max;
+    for (int q = 0; q < g.nstencil; ++q) {
+        const int8_t *rb = stencil + 16 * q;
+        for (int z = 0; z <= rb[12]; ++z)
+            gr2 = std::max(gr2, rb[3 * z] * rb[3 * z] + rb[3 * z + 1] * rb[3 * z + 1] + rb[3 * z + 2] * rb[3 * z + 2]);
+    }
     CHM_HIP(hipSetDevice(ctx->device));
     auto *S = new chm_sampler();
     S->ctx = ctx;
@@ -1255,6 +1860,8 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
     S->profile = profile_scale ? 1 : 0;
     S->nfit = (double)g.ncol_tot;  // nsp*nkw*nwin complex points, Re and Im counted
     S->lds_k1 = sizeof(int) * ((g.nrow_tot + 3) & ~3) + (size_t)g.natoms_pad;
+    S->lds_pair = sizeof(int) * 3 * ((g.nrow_tot + 3) & ~3) + 2 * (size_t)g.natoms_pad;
+    S->gr2 = gr2;
     {
         // workgroups per replica for the gather: ~3 items per lane, at most 16
         const int items = 2 * g.ncoord + 2 * g.nstencil;
@@ -1322,7 +1929,12 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         (rc = S->nanchor.ensure(sizeof(long long) * nr)) ||
         (rc = S->exch_att.ensure(sizeof(long long) * nr)) ||
         (rc = S->exch_acc.ensure(sizeof(long long) * nr)) ||
-        (rc = S->acc_rung.ensure(sizeof(long long) * nr)))
+        (rc = S->acc_rung.ensure(sizeof(long long) * nr)) ||
+        (rc = S->pdc.ensure(sizeof(int) * 6 * nr * g.nrow_tot)) ||
+        (rc = S->pprop.ensure(sizeof(int) * 6 * kPropInts * nr)) ||
+        (rc = S->pmt.ensure(sizeof(double) * 3 * nr * g.ncol_tot)) ||
+        (rc = S->ppart.ensure(sizeof(double) * 3 * nr * g.nsp * g.nct * 4)) ||
+        (rc = S->pc2.ensure(sizeof(double) * 2 * nr)))
         return fail(rc);
     hipFuncSetAttribute((const void *)k_gemm_abs_f64, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds_gemm(g));
@@ -1330,6 +1942,9 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)S->lds_k1);
     hipFuncSetAttribute((const void *)k_full_counts, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)S->lds_k1);
+    hipFuncSetAttribute((const void *)k_slot_pair, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)S->lds_pair);
+    if (S->lds_pair > 160 * 1024) S->pairs = 0;   // single steps only
     if (lds_gemm(g) > 160 * 1024 || S->lds_k1 > 160 * 1024)
         return fail((chm::set_error("problem too large for LDS staging"), CHM_E_ARG));
     S->lp.r2max = r2max;
@@ -1409,6 +2024,9 @@ extern "C" int chm_sampler_init(chm_sampler *S, const int32_t *species0, const d
     CHM_HIP(hipMemsetAsync(S->c2.p, 0, S->c2.bytes, st));
     CHM_HIP(hipMemsetAsync(S->best_c2.p, 0, S->best_c2.bytes, st));
     CHM_HIP(hipMemsetAsync(S->nanchor.p, 0, S->nanchor.bytes, st));
+    CHM_HIP(hipMemsetAsync(S->pdc.p, 0, S->pdc.bytes, st));
+    CHM_HIP(hipMemsetAsync(S->pprop.p, 0, S->pprop.bytes, st));
+    CHM_HIP(hipMemsetAsync(S->pc2.p, 0, S->pc2.bytes, st));
     if (S->log_cap) CHM_HIP(hipMemsetAsync(S->log.p, 0xff, S->log.bytes, st));
     S->seed = seed;
     S->hstep = 0;
@@ -1717,6 +2335,39 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
     CHM_HIP(hipEventCreate(&e1));
     float t = 0.f;
     long long h = S->hstep;
+    if (use_pairs(S, 3)) {
+        // paired rounds: the three-trial transform repeated, then real rounds; times
+        // are per step (half a round)
+        int rc = launch_slot_pair(S, 0, 3, -1, c2row(S, h), pc2_row(S, 0), 0, h + 1);   // round 0's rows
+        CHM_HIP(hipEventRecord(e0, st));
+        for (int i = 0; i < iters && !rc; ++i) rc = launch_gemm_pair(S, 0, 0);
+        CHM_HIP(hipEventRecord(e1, st));
+        if (rc) return rc;
+        CHM_HIP(hipEventSynchronize(e1));
+        CHM_HIP(hipEventElapsedTime(&t, e0, e1));
+        ms[1] = t / iters / 2;
+        CHM_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < iters && !rc; ++k) {
+            rc = launch_gemm_pair(S, k, 2);
+            if (!rc) rc = launch_slot_pair(S, 1, 3, k, k ? pc2_row(S, k - 1) : c2row(S, h), pc2_row(S, k), 0, 0);
+        }
+        CHM_HIP(hipEventRecord(e1, st));
+        if (rc) return rc;
+        CHM_HIP(hipEventSynchronize(e1));
+        CHM_HIP(hipEventElapsedTime(&t, e0, e1));
+        ms[3] = t / iters / 2;
+        ms[2] = ms[3] - ms[1];
+        ms[0] = 0.0;
+        // settle: the last proposed round, ending in the canonical chi^2 row
+        if (!rc) rc = launch_gemm_pair(S, iters, 2);
+        if (!rc) rc = launch_slot_pair(S, 1, 0, iters, pc2_row(S, iters - 1), c2row(S, h + 2 * (iters + 1)), 0, 0);
+        if (rc) return rc;
+        S->hstep = h + 2 * (iters + 1);
+        CHM_HIP(hipStreamSynchronize(st));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return CHM_OK;
+    }
     int rc = launch_propose(S, h);           // step h+1 pending
     // the trial transform of the pending step, repeated (no counter advance)
     CHM_HIP(hipEventRecord(e0, st));
@@ -1793,6 +2444,16 @@ extern "C" int chm_sampler_set_tuning(chm_sampler *S, int nsplit) {
 extern "C" int chm_sampler_get_tuning(chm_sampler *S, int *nsplit) {
     CHM_REQUIRE(S && nsplit, CHM_E_ARG, "bad arguments");
     *nsplit = S->nsplit;
+    return CHM_OK;
+}
+
+extern "C" int chm_sampler_set_pairs(chm_sampler *S, int on) {
+    CHM_REQUIRE(S, CHM_E_ARG, "null sampler");
+    CHM_REQUIRE(!on || S->lds_pair <= 160 * 1024, CHM_E_ARG, "problem too large for paired steps");
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    CHM_HIP(hipStreamSynchronize(S->ctx->stream));
+    drop_graph(S);
+    S->pairs = on ? 1 : 0;
     return CHM_OK;
 }
 

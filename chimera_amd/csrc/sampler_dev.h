@@ -133,16 +133,30 @@ __device__ __forceinline__ PropDraws propose_draws(int natoms, uint64_t seed, ui
     }
     return d;
 }
-__device__ __forceinline__ Prop propose_pick(const uint8_t *sp, const PropDraws &d) {
-    const int sa = sp[d.a];
+// species lookups: as stored, or with one swap x made (x.a holds x.na, x.b holds x.nb)
+struct SpPlain {
+    const uint8_t *sp;
+    __device__ __forceinline__ int operator()(int q) const { return sp[q]; }
+};
+struct SpOver {
+    const uint8_t *sp;
+    int a, b, na, nb;
+    __device__ __forceinline__ int operator()(int q) const { return q == a ? na : (q == b ? nb : (int)sp[q]); }
+};
+template <class SP>
+__device__ __forceinline__ Prop propose_pick_sp(const SP &sp, const PropDraws &d) {
+    const int sa = sp(d.a);
     int sc[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) sc[t] = sp[d.c[t]];
+    for (int t = 0; t < 8; ++t) sc[t] = sp(d.c[t]);
     Prop p{d.a, d.a, sa, sa, 0};
 #pragma unroll
     for (int t = 7; t >= 0; --t)
         if (sc[t] != sa) p = Prop{d.a, d.c[t], sc[t], sa, 1};
     return p;
+}
+__device__ __forceinline__ Prop propose_pick(const uint8_t *sp, const PropDraws &d) {
+    return propose_pick_sp(SpPlain{sp}, d);
 }
 
 __device__ __forceinline__ void load_species(uint8_t *lsp, const uint8_t *gsp, int natoms_pad) {
@@ -182,8 +196,9 @@ struct SwapSite {
 // One gather item of a swap: item < nss is a neighbour of a (first ncoord) or of b (rec.x
 // holds its packed stencil entry); the rest are path records `rec` of a's then b's
 // incidence stencil.  Count changes go to `hist`.
-__device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W, const uint8_t *lsp,
-                                            int *hist, int it, int nss, const int4 rec) {
+template <class SP>
+__device__ __forceinline__ void gather_item_sp(const Geo &g, const SwapSite &W, const SP &lsp,
+                                               int *hist, int it, int nss, const int4 rec) {
     const Prop &p = W.p;
     if (it < nss) {
         // ordered pairs x->y and y->x for every neighbour y of x in {a, b}
@@ -196,7 +211,7 @@ __device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W, con
                                  wrap((on_a ? W.ay : W.by) + dy, g.box[1]),
                                  wrap((on_a ? W.az : W.bz) + dz, g.box[2]));
         if (x == p.b && y == p.a) return;  // covered from a's side
-        const int sx = lsp[x], sy = lsp[y];
+        const int sx = lsp(x), sy = lsp(y);
         const int nx = (x == p.a) ? p.na : p.nb;
         const int ny = (y == p.a) ? p.na : ((y == p.b) ? p.nb : sy);
         atomicAdd(&hist[sx * g.rstride + sy * g.nshell + sh], -1);
@@ -229,13 +244,13 @@ __device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W, con
     for (int q = 1; q < 4; ++q) {
         if (q > nslot) break;
         const int st = sites[q];
-        const int so = lsp[st];
+        const int so = lsp(st);
         const int sn = (st == p.a) ? p.na : ((st == p.b) ? p.nb : so);
         eo += so * pw;
         en += sn * pw;
         pw *= g.nsp;
     }
-    const int sa0 = lsp[sites[0]];
+    const int sa0 = lsp(sites[0]);
     const int sa1 = (sites[0] == p.a) ? p.na : ((sites[0] == p.b) ? p.nb : sa0);
     eo += sa0 * g.rstride + g.nss_rows + cls_off;
     en += sa1 * g.rstride + g.nss_rows + cls_off;
@@ -245,6 +260,11 @@ __device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W, con
     }
 }
 
+
+__device__ __forceinline__ void gather_item(const Geo &g, const SwapSite &W, const uint8_t *lsp,
+                                            int *hist, int it, int nss, const int4 rec) {
+    gather_item_sp(g, W, SpPlain{lsp}, hist, it, nss, rec);
+}
 
 constexpr int kRecPrefetch = 4;   // gather records per lane loaded ahead
 

@@ -55,6 +55,12 @@ struct chm_sampler {
     hipGraphExec_t gexec[2] = {nullptr, nullptr}; // odd completed-step count
     int graph_steps = 0;
     chm_lp lp;
+    // paired steps (bf16x3 mode, uniform proposals): two steps per transform + per-replica
+    // launch; round k uses proposal / count-change rows (k & 1) and the chi^2 row (k & 1)
+    int pairs = 1;
+    int gr2 = 0;                 // largest squared gather offset, units (a0/2)^2
+    size_t lds_pair = 0;
+    chm::DevBuf pdc, pprop, pmt, ppart, pc2;
 };
 
 namespace chm_smp {

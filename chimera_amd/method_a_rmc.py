@@ -46,14 +46,18 @@ class RMCSampler:
                  swap_frac=0.75, disp_sigma=0.015, max_disp=0.30, nsp=None,
                  collect_after=None, collect_every=25, adapt_every=0, target_accept=0.35,
                  profile_scale=True, collect_beta=0.5, n_collect=4, energy_prior=None,
-                 ms_library=None, precision="f64", reanchor_every=512, accumulator="f64"):
+                 ms_library=None, precision="f64", reanchor_every=512, accumulator="f64",
+                 paired=True):
         """Arguments as methods/method_a_rmc.py:64-69, plus `precision`: "f64" (the trial
         chi(R) recomputed from the integer counts in FP64 every step), "bf16x3" (the
         change of chi(R) in split-BF16 MFMA with FP32 accumulation, added to a running
         chi(R) re-anchored in FP64 every `reanchor_every` accepted moves of a replica, 0 =
         never) or "f32" (the change in FP32 FMA, same running chi(R)); `accumulator`:
         "f64" (default) or "f32" (the running chi(R) rounded to FP32 after every update;
-        for precision studies).  chi^2 and the acceptance test are FP64 in every mode."""
+        for precision studies).  chi^2 and the acceptance test are FP64 in every mode.
+        `paired` (bf16x3 with the uniform proposal): two steps per transform and
+        per-replica launch, by pre-fetching the trial of the second step for both outcomes
+        of the first; the chain is the same move for move."""
         if precision not in ("f64", "bf16x3", "f32"):
             raise ValueError("precision must be 'f64', 'bf16x3' or 'f32'")
         if accumulator not in ("f64", "f32") or (precision == "f64" and accumulator == "f32"):
@@ -61,6 +65,7 @@ class RMCSampler:
         self.precision = precision
         self.accumulator = accumulator
         self.reanchor_every = int(reanchor_every)
+        self.paired = bool(paired)
         if swap_frac != 1.0:
             raise ValueError("the device sampler makes swap moves only: pass swap_frac=1.0 "
                              "(the reference's fit driver does, chimera/fitrun.py:218)")
@@ -253,6 +258,7 @@ class RMCSampler:
         if getattr(self, "accumulator", "f64") == "f32":
             mode |= 16
         check(_lib.load().chm_sampler_set_precision(self._S, mode, self.reanchor_every))
+        check(_lib.load().chm_sampler_set_pairs(self._S, int(getattr(self, "paired", True))))
         check(_lib.load().chm_sampler_init(self._S, species0, f64(self._beta_rung),
                                            int(self.seed) & ((1 << 64) - 1)))
         self.samples = []

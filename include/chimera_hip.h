@@ -223,6 +223,16 @@ int chm_sampler_chik(chm_sampler *s, int nconf, const int32_t *counts, double *c
 int chm_sampler_set_tuning(chm_sampler *s, int nsplit);
 int chm_sampler_get_tuning(chm_sampler *s, int *nsplit);
 
+/* Paired steps (default on): in the split-BF16 mode with the uniform proposal, each
+ * transform + per-replica launch pair makes two Metropolis steps.  The transform computes
+ * the trial of step n+1 and of step n+2 for both outcomes of step n+1 (pre-fetching,
+ * Brockwell 2006); the per-replica launch takes both decisions in order and proposes the
+ * next pair.  Decisions, counts, chi^2 and records are those of single steps.  Off: one
+ * step per launch pair (also used with the learned proposal, move records, the FP64 and
+ * FP32 modes and segments shorter than 3 steps).  No counterpart in the reference (its
+ * steps are sequential kernel chains, method_a_rmc.py:253-297). */
+int chm_sampler_set_pairs(chm_sampler *s, int on);
+
 /* Average device time (ms) per launch of the step's kernels, from HIP events on the
  * sampler's stream over `iters` launches each: ms[0] = 0 (kept for layout), ms[1] trial
  * transform GEMM, ms[2] per-replica accept + propose + gather kernel (step minus
