@@ -103,6 +103,11 @@ constexpr int KS = 16;    // K slices of the FP64 kernel
 struct TileOut {
     uint64_t *stepc;        // per slot: advanced by `bump` steps per transform
     int bump;
+    int *draws;             // per slot (kDrawInts): the next per-replica launch's acceptance
+    int draw_mode;          //   uniforms and proposal draws, made by an extra workgroup that
+    const int *rung_of;     //   also advances the step counter (null: none; 1 one step,
+    uint64_t seed;          //   2 a paired round)
+    int natoms;
     const int *anchor;      // re-anchor mode: slots with anchor[s] >= anchor_min get
     int anchor_min;         //   anchor_out[s] = counts[s] @ basis_R (FP64); nothing else
     double *anchor_out;     //   is written.  Null in the normal mode.
@@ -241,6 +246,47 @@ __device__ __forceinline__ __bf16 bf16_of_small_int(int v) {
 // Operand layouts of the 16x16x32 MFMA: lane l holds A[l%16][8(l/16)+i] and
 // B[8(l/16)+i][l%16] (i < 8); D lane l holds D[4(l/16)+i][l%16] (i < 4).
 constexpr int KBW = 4;   // K blocks per wave held in registers (nkb <= 4 * KBW)
+
+// The random numbers the next per-replica launch needs, one slot per lane, made while
+// the transform runs: for one step n (pre = n - 1 before the advance) the acceptance
+// uniform of step n and the proposal draws of step n + 1; for a paired round (pre = n)
+// the uniforms of steps n + 1, n + 2 and the draws of steps n + 3, n + 4.  The same
+// counter-based streams as in the per-replica kernel (kernels.py:378-398,
+// method_a_rmc.py:278-285).  This workgroup also advances the step counter.
+constexpr int kDrawInts = 24;   // u (2 x double), draws (2 x (a, c[8])), pad
+__device__ __forceinline__ void compute_draws(const TileOut &O, int nrep, int r0) {
+    const int t = threadIdx.x, slot = r0 + t;
+    if (t >= RB || slot >= nrep) return;
+    const uint64_t pre = O.stepc[slot];
+    const uint64_t rung = (uint64_t)O.rung_of[slot];
+    int *D = O.draws + (long)slot * kDrawInts;
+    const int nst = O.draw_mode == 2 ? 2 : 1;
+    for (int k = 0; k < nst; ++k) {
+        const uint64_t h = splitmix64(O.seed ^ ((pre + 1 + k) * kGolden) ^ (rung << 32) ^ kAccSalt);
+        const double u = u01(splitmix64(h));
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, u);
+        D[2 * k] = (int)(unsigned)b;
+        D[2 * k + 1] = (int)(unsigned)(b >> 32);
+    }
+    for (int k = 0; k < nst; ++k) {
+        const PropDraws d = propose_draws(O.natoms, O.seed, pre + 1 + nst + k, rung);
+        D[4 + 9 * k] = d.a;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) D[5 + 9 * k + i] = d.c[i];
+    }
+    O.stepc[slot] = pre + (uint64_t)O.bump;
+}
+__device__ __forceinline__ double draw_u(const int *D, int k) {
+    return __builtin_bit_cast(double, (unsigned long long)(unsigned)D[2 * k] |
+                                          ((unsigned long long)(unsigned)D[2 * k + 1] << 32));
+}
+__device__ __forceinline__ PropDraws draw_p(const int *D, int k) {
+    PropDraws d;
+    d.a = D[4 + 9 * k];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) d.c[i] = D[5 + 9 * k + i];
+    return d;
+}
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
@@ -401,7 +447,7 @@ __device__ __forceinline__ void gemm_delta_bf16_tile(const Geo &g, int nrep, con
             }
         }
     }
-    if (O[0].bump && ct == 0 && e == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
+    if (O[0].bump && !O[0].draws && ct == 0 && e == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
         atomicAdd(reinterpret_cast<unsigned long long *>(O[0].stepc + r0 + threadIdx.x),
                   (unsigned long long)O[0].bump);
 }
@@ -417,6 +463,10 @@ __device__ __forceinline__ void gemm_delta_bf16_body(const Geo &g, int nrep, con
 __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const int *__restrict__ dcounts,
                                                           const bf16x8 *__restrict__ Tb,
                                                           const double *__restrict__ mcur, TileOut O) {
+    if ((int)blockIdx.x == g.nct) {   // the extra column of workgroups: draws + step counter
+        if (blockIdx.z == 0) compute_draws(O, nrep, blockIdx.y * RB);
+        return;
+    }
     gemm_delta_bf16_body(g, nrep, dcounts, Tb, mcur, O, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -569,6 +619,7 @@ struct SlotArgs {
     int accept;             // 0: none, 1: step, 2: init (store chi^2, no decision)
     int propose;            // 1: propose + gather the next step; 2: gather prop_next
     double nfit;
+    const int *draws;       // (nrep, kDrawInts) made by the transform just before, or null
 };
 
 __device__ double block_sum(double v, double *red) {
@@ -680,16 +731,18 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     const long long ri = (long long)stepc - 1 - A.rec_base;
     const bool series_on = A.accept == 1 && A.series && rung == A.series_rung && ri >= 0 &&
                            ri < A.series_cap;
-    const int rchunk = series_on ? (lead ? g.nrow_tot : 0) : (g.nrow_tot + nwg - 1) / nwg;
-    const int rbeg = series_on ? 0 : split * rchunk, rend = min(g.nrow_tot, rbeg + rchunk);
+    // (the prefetched chunk does not depend on series_on, which needs the step counter)
+    const int rchunk = (g.nrow_tot + nwg - 1) / nwg;
+    const int pbeg = split * rchunk, pend = min(g.nrow_tot, pbeg + rchunk);
+    const int rbeg = series_on ? 0 : pbeg, rend = series_on ? (lead ? g.nrow_tot : 0) : pend;
     const Buf bcn = make_buf(cn, 4u * g.nrow_tot), bdc = make_buf(dca, 4u * g.nrow_tot);
     constexpr int RC = 2;
     int cnv[RC], dcv[RC];
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
-        const int q = rbeg + t + r * nt;
-        dcv[r] = buf_i32(bdc, A.accept && q < rend, 4u * q);
-        cnv[r] = buf_i32(bcn, A.accept == 1 && q < rend, 4u * q);
+        const int q = pbeg + t + r * nt;
+        dcv[r] = buf_i32(bdc, A.accept && q < pend, 4u * q);
+        cnv[r] = buf_i32(bcn, A.accept == 1 && q < pend, 4u * q);
     }
     // this lane's first gather records (independent of the proposal)
     const int stride = nwg * nt;
@@ -698,10 +751,15 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     prefetch_recs(g, A.ss_sten, A.ms_sten, first, stride, A.propose, recs);
     // the draws of this step's acceptance uniform and of the next uniform proposal
     // (wave-uniform: scalar arithmetic)
-    // (wave 0 only: lane 0 takes the decision and makes the pick)
+    // (made by the transform when it ran just before; else here, by wave 0 only: lane 0
+    // takes the decision and makes the pick)
     double u_acc = 1.0;
     PropDraws dr{};
-    if (t < 64) {
+    if (A.draws) {
+        const int *D = A.draws + (long)s * kDrawInts;
+        if (A.accept == 1) u_acc = draw_u(D, 0);
+        if (A.propose == 1) dr = draw_p(D, 0);
+    } else if (t < 64) {
         if (A.accept == 1) {
             const uint64_t h = splitmix64(A.seed ^ (stepc * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
             u_acc = u01(splitmix64(h));
@@ -783,12 +841,18 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
         // ------------------------------------------------ commit of this workgroup's share
         const int acc = flag & 1;
         const bool add = acc && A.accept == 1;
+        // rows rbeg + t + r nt: prefetched when inside this workgroup's chunk (rbeg == pbeg
+        // whenever rend > 0: series_on commits all rows in the lead, whose chunk starts at 0)
 #pragma unroll
         for (int r = 0; r < RC; ++r) {
             const int q = rbeg + t + r * nt;
-            if (q < rend && dcv[r] != 0) {
-                if (add) cn[q] = cnv[r] + dcv[r];
-                dca[q] = 0;
+            if (q < rend) {
+                const bool pf = q < pend;
+                const int d = pf ? dcv[r] : dca[q];
+                if (d != 0) {
+                    if (add) cn[q] = (pf ? cnv[r] : cn[q]) + d;
+                    dca[q] = 0;
+                }
             }
         }
         for (int q = rbeg + t + RC * nt; q < rend; q += nt) {
@@ -913,6 +977,7 @@ struct PairArgs {
     int propose;                 // 0: nothing, 1: the single step after the round, 3: next round
     int gr2;                     // largest squared gather offset, units (a0/2)^2
     double nfit;
+    const int *draws;            // (nrep, kDrawInts) made by the transform just before, or null
 };
 
 __device__ __forceinline__ int min_image(int d, int n) {
@@ -996,20 +1061,22 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
     const long long ri1 = (long long)stepc - 2 - A.rec_base, ri2 = ri1 + 1;
     const bool series_on = A.accept && A.series && rung == A.series_rung &&
                            ((ri1 >= 0 && ri1 < A.series_cap) || (ri2 >= 0 && ri2 < A.series_cap));
-    const int rchunk = series_on ? (lead ? g.nrow_tot : 0) : (g.nrow_tot + nwg - 1) / nwg;
-    const int rbeg = series_on ? 0 : split * rchunk, rend = min(g.nrow_tot, rbeg + rchunk);
+    // (the prefetched chunk does not depend on series_on, which needs the step counter)
+    const int rchunk = (g.nrow_tot + nwg - 1) / nwg;
+    const int pbeg = split * rchunk, pend = min(g.nrow_tot, pbeg + rchunk);
+    const int rbeg = series_on ? 0 : pbeg, rend = series_on ? (lead ? g.nrow_tot : 0) : pend;
     constexpr int RC = 2;
     int cnv[RC], dcv[3][RC];
     {
         const Buf bcn = make_buf(cn, 4u * g.nrow_tot);
 #pragma unroll
-        for (int r = 0; r < RC; ++r) cnv[r] = buf_i32(bcn, A.accept && rbeg + t + r * nt < rend, 4u * (rbeg + t + r * nt));
+        for (int r = 0; r < RC; ++r) cnv[r] = buf_i32(bcn, A.accept && pbeg + t + r * nt < pend, 4u * (pbeg + t + r * nt));
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
             const Buf bd = make_buf(A.dc_acc[v] + (long)s * g.nrow_tot, 4u * g.nrow_tot);
 #pragma unroll
             for (int r = 0; r < RC; ++r)
-                dcv[v][r] = buf_i32(bd, A.accept && rbeg + t + r * nt < rend, 4u * (rbeg + t + r * nt));
+                dcv[v][r] = buf_i32(bd, A.accept && pbeg + t + r * nt < pend, 4u * (pbeg + t + r * nt));
         }
     }
     const int stride = nwg * nt;
@@ -1017,10 +1084,19 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
     RecPre recs;
     prefetch_recs(g, A.ss_sten, A.ms_sten, first, stride, A.propose != 0, recs);
     // the round's acceptance uniforms and the next proposals' draws (wave-uniform)
-    // (wave 0 only: lane 0 takes the decisions and makes the picks)
+    // (made by the transform when it ran just before; else here, by wave 0 only: lane 0
+    // takes the decisions and makes the picks)
     double u1 = 1.0, u2 = 1.0;
     PropDraws dr3{}, dr4{};
-    if (t < 64) {
+    if (A.draws) {
+        const int *D = A.draws + (long)s * kDrawInts;
+        if (A.accept) {
+            u1 = draw_u(D, 0);
+            u2 = draw_u(D, 1);
+        }
+        if (A.propose) dr3 = draw_p(D, 0);
+        if (A.propose == 3) dr4 = draw_p(D, 1);
+    } else if (t < 64) {
         if (A.accept) {
             uint64_t h = splitmix64(A.seed ^ ((stepc - 1) * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
             u1 = u01(splitmix64(h));
@@ -1159,10 +1235,18 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
                 }
             }
         };
+        // rows rbeg + t + r nt: prefetched when inside this workgroup's chunk (rbeg == pbeg
+        // whenever rend > 0: series_on commits all rows in the lead, whose chunk starts at 0)
 #pragma unroll
         for (int r = 0; r < RC; ++r) {
             const int q = rbeg + t + r * nt;
-            if (q < rend) commit_row(q, cnv[r], dcv[0][r], dcv[1][r], dcv[2][r]);
+            if (q < rend) {
+                const long o = (long)s * g.nrow_tot + q;
+                if (q < pend)
+                    commit_row(q, cnv[r], dcv[0][r], dcv[1][r], dcv[2][r]);
+                else
+                    commit_row(q, cn[q], A.dc_acc[0][o], A.dc_acc[1][o], A.dc_acc[2][o]);
+            }
         }
         for (int q = rbeg + t + RC * nt; q < rend; q += nt) {
             const long o = (long)s * g.nrow_tot + q;
@@ -1299,6 +1383,10 @@ __global__ __launch_bounds__(256) void k_gemm_pair_bf16(Geo g, int nrep, const b
                                                         const double *__restrict__ mcur, PairTile P) {
     // one workgroup per tile computes all three trials (B read once); 3 K blocks per wave
     // (nkb <= 12) keep the three operand sets within 2 workgroups per CU
+    if ((int)blockIdx.x == g.nct) {   // the extra column of workgroups: draws + step counter
+        if (blockIdx.z == 0) compute_draws(P.O[0], nrep, blockIdx.y * RB);
+        return;
+    }
     gemm_delta_bf16_tile<3, kPairKB>(g, nrep, P.dc, Tb, mcur, P.O, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -1458,11 +1546,21 @@ int launch_gemm_abs(chm_sampler *S, const int *dc, int bump) {
     return CHM_OK;
 }
 
+// the draws of the next per-replica launch, made by the transform's extra workgroup column
+void set_draws(chm_sampler *S, TileOut &O, int mode) {
+    O.draws = S->draws.as<int>();
+    O.draw_mode = mode;
+    O.rung_of = S->rung_of.as<int>();
+    O.seed = S->seed;
+    O.natoms = S->g.natoms;
+}
+
 int launch_gemm_delta(chm_sampler *S, const int *dc, int bump) {
     const Geo &g = S->g;
-    dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
+    dim3 grid(g.nct + 1, (S->nrep + RB - 1) / RB, g.nsp);
     TileOut O = tile_out(S);
     O.bump = bump;
+    set_draws(S, O, 1);
     k_gemm_delta_bf16<<<grid, 256, 0, S->ctx->stream>>>(
         g, S->nrep, dc, reinterpret_cast<const bf16x8 *>(S->Tb.p), S->mcur.as<double>(), O);
     CHM_LAUNCH_CHECK();
@@ -1479,6 +1577,7 @@ int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanch
     A.prop_acc = prow(S, n);
     A.prop_next = prow(S, n + 1);
     A.c2_prev = c2_prev ? c2_prev : c2row(S, n - 1);
+    A.draws = (accept == 1 && S->incremental == 1) ? S->draws.as<int>() : nullptr;   // see launch_gemm_delta
     A.c2_next = c2row(S, n);
     k_slot_step<<<dim3(S->nsplit, S->nrep), 256, S->lds_k1, S->ctx->stream>>>(S->g, A);
     CHM_LAUNCH_CHECK();
@@ -1560,8 +1659,9 @@ int launch_gemm_pair(chm_sampler *S, int k, int bump) {
         P.O[v].partials = S->ppart.as<double>() + v * pt;
         P.O[v].bump = v == 0 ? bump : 0;
     }
+    set_draws(S, P.O[0], 2);
     const int nrb = (S->nrep + RB - 1) / RB;
-    k_gemm_pair_bf16<<<dim3(g.nct, nrb, g.nsp), 256, 0, S->ctx->stream>>>(
+    k_gemm_pair_bf16<<<dim3(g.nct + 1, nrb, g.nsp), 256, 0, S->ctx->stream>>>(
         g, S->nrep, reinterpret_cast<const bf16x8 *>(S->Tb.p), S->mcur.as<double>(), P);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
@@ -1618,6 +1718,7 @@ int launch_slot_pair(chm_sampler *S, int accept, int propose, int k, const doubl
     A.propose = propose;
     A.gr2 = S->gr2;
     A.nfit = S->nfit;
+    A.draws = accept ? S->draws.as<int>() : nullptr;   // made by launch_gemm_pair just before
     k_slot_pair<<<dim3(S->nsplit, S->nrep), 256, S->lds_pair, S->ctx->stream>>>(g, A);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
@@ -1934,7 +2035,8 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         (rc = S->pprop.ensure(sizeof(int) * 6 * kPropInts * nr)) ||
         (rc = S->pmt.ensure(sizeof(double) * 3 * nr * g.ncol_tot)) ||
         (rc = S->ppart.ensure(sizeof(double) * 3 * nr * g.nsp * g.nct * 4)) ||
-        (rc = S->pc2.ensure(sizeof(double) * 2 * nr)))
+        (rc = S->pc2.ensure(sizeof(double) * 2 * nr)) ||
+        (rc = S->draws.ensure(sizeof(int) * kDrawInts * nr)))
         return fail(rc);
     hipFuncSetAttribute((const void *)k_gemm_abs_f64, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds_gemm(g));

@@ -61,6 +61,7 @@ struct chm_sampler {
     int gr2 = 0;                 // largest squared gather offset, units (a0/2)^2
     size_t lds_pair = 0;
     chm::DevBuf pdc, pprop, pmt, ppart, pc2;
+    chm::DevBuf draws;           // (nrep, 24) random numbers made by the transform's extra workgroup
 };
 
 namespace chm_smp {
