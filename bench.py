@@ -196,7 +196,7 @@ def recorded_traffic(kernel, nsplit, workload):
     """HBM-side bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy of known size), when it was
     measured on this workload at this launch split; else None."""
-    path = os.path.join(ROOT, "profiles", f"traffic_k_slot_step_{workload}.txt")
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.txt")
     try:
         lines = [ln.split() for ln in open(path) if ln.strip() and not ln.startswith("#")]
     except OSError:
@@ -239,6 +239,8 @@ def main():
     ap.add_argument("--lp-beta-max", type=float, default=3.0)
     ap.add_argument("--nsplit", type=int, default=0,
                     help="workgroups per replica in the slot kernel (0: autotune)")
+    ap.add_argument("--single-steps", action="store_true",
+                    help="one Metropolis step per launch pair (paired steps are the default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the known-size "
                          "kernel that scales hardware byte counters) and exit")
@@ -271,6 +273,7 @@ def main():
     P = W.build(args.workload)
     S = P.sampler
     S.precision, S.reanchor_every = args.precision, args.reanchor_every
+    S.paired = not args.single_steps
     S.init_state(P.cfg0, P.cell.ideal)
     from chimera_amd import autotune
     if args.nsplit:
@@ -297,29 +300,41 @@ def main():
     from chimera_amd import _lib
     ms = np.zeros(4)
     _lib.check(_lib.load().chm_sampler_profile(S._S, args.profile_iters, ms))
-    _, k2, k_slot, kstep = ms
+    _, k2, k_slot, kstep = ms          # per step (a paired launch makes two steps)
+    paired = bool(getattr(S, "paired", False)) and args.precision == "bf16x3"
     nrow, ncol_e, nrep, nsp = S.nrow_e, S.nkw * S.nwin * 2, S.nrep, P.nsp
     ncol_tot = nsp * ncol_e
-    # algorithmic work of one launch of each kernel of the step
+    # algorithmic work of one launch of each kernel (per replica: species, path records,
+    # tile sums, counts + count-change rows, running chi(R) copy)
     flops_k2 = 2.0 * nrep * nsp * nrow * ncol_e                     # one real GEMM
     ms_stencil = 0 if P.ms is None else P.ms.nstencil
     nct = -(-ncol_e // 16)
-    bytes_slot = nrep * (P.cell.natoms + 2 * ms_stencil * 16 + 8 * nsp * nct * 4
-                         + 4 * nsp * nrow * 3 + 2 * 8 * ncol_tot)
-    kern = {"slot_step_ms": k_slot, "trial_transform_ms": k2, "step_ms": kstep}
-    if args.precision == "bf16x3":
-        peak2, kname = 2500.0 / 3.0, "k_gemm_delta_bf16"        # split-BF16: BF16 dense / 3
-    elif args.precision == "f32":
-        peak2, kname = 157.3, "k_gemm_delta_f32"                 # FP32 (vector + matrix) peak
+    if paired:
+        # three trials per launch (m1, m2r, m2a); two decisions, two or three gathers over
+        # one read of the path records, three tile-sum sets and count-change rows
+        flops_k2 *= 3
+        bytes_slot = nrep * (P.cell.natoms + 2 * ms_stencil * 16 + 3 * 8 * nsp * nct * 4
+                             + 4 * nsp * nrow * 7 + 2 * 8 * ncol_tot)
+        k2_launch, kslot_launch = 2 * k2, 2 * k_slot
+        kname, sname = "k_gemm_pair_bf16", "k_slot_pair"
     else:
-        peak2, kname = FP64_PEAK_TFLOPS, "k_gemm_abs_f64"
+        bytes_slot = nrep * (P.cell.natoms + 2 * ms_stencil * 16 + 8 * nsp * nct * 4
+                             + 4 * nsp * nrow * 3 + 2 * 8 * ncol_tot)
+        k2_launch, kslot_launch = k2, k_slot
+        kname, sname = {"bf16x3": "k_gemm_delta_bf16", "f32": "k_gemm_delta_f32",
+                        "f64": "k_gemm_abs_f64"}[args.precision], "k_slot_step"
+    kern = {"slot_step_ms": k_slot, "trial_transform_ms": k2, "step_ms": kstep,
+            "slot_launch_ms": kslot_launch, "transform_launch_ms": k2_launch}
+    peak2 = {"bf16x3": 2500.0 / 3.0,                 # split-BF16: BF16 dense / 3
+             "f32": 157.3,                           # FP32 (vector + matrix) peak
+             "f64": FP64_PEAK_TFLOPS}[args.precision]
     rooflines = {
-        "k_slot_step": {"bound": "hbm", "achieved": bytes_slot / (k_slot * 1e-3) / 1e9,
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s"},
-        kname: {"bound": "mfma", "achieved": flops_k2 / (k2 * 1e-3) / 1e12, "peak": peak2,
+        sname: {"bound": "hbm", "achieved": bytes_slot / (kslot_launch * 1e-3) / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+        kname: {"bound": "mfma", "achieved": flops_k2 / (k2_launch * 1e-3) / 1e12, "peak": peak2,
                 "unit": "TFLOP/s"},
     }
-    dom = "k_slot_step" if k_slot >= k2 else kname
+    dom = sname if k_slot >= k2 else kname
     r = rooflines[dom]
     from chimera_amd import autotune as _at
     roof = {"bound": r["bound"], "achieved": round(r["achieved"], 3), "peak": r["peak"],
@@ -366,7 +381,7 @@ def main():
         "config": {"workload": args.workload, "natoms": int(P.cell.natoms),
                    "replicas": int(S.nrep), "edges": int(P.nsp), "channels": int(P.nchan),
                    "R_points": int(S.nwin), "ms_paths": P.ms is not None,
-                   "exchange_every": args.exchange_every},
+                   "exchange_every": args.exchange_every, "paired_steps": paired},
         "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

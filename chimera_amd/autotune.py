@@ -33,8 +33,9 @@ def _cache_path():
 def _key(S):
     ms = S.ms
     return "|".join(str(v) for v in (
-        "v1", S.natoms, S.nsp, S.ncoord, S.nrow_e, S.nkw, S.nwin, S.nrep,
-        0 if ms is None else ms.nstencil, S.precision))
+        "v2", S.natoms, S.nsp, S.ncoord, S.nrow_e, S.nkw, S.nwin, S.nrep,
+        0 if ms is None else ms.nstencil, S.precision,
+        int(bool(getattr(S, "paired", False)) and S.precision == "bf16x3")))
 
 
 def _load():
