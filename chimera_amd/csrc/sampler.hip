@@ -180,7 +180,7 @@ __device__ __forceinline__ void gemm_abs_body(const Geo &g, int nrep, const int 
         double v = 0.0;
         if (slot < nrep) {
             long o = (long)slot * g.nrow_tot + (long)e * g.rstride + k;
-            v = (double)(counts[o] + (dcounts ? dcounts[o] : 0));
+            v = (double)(counts[o] + (dcounts ? dw_row(dcounts + (long)slot * g.nrow_w, e * g.rstride + k) : 0));
         }
         dl[k * ld + m] = v;
     }
@@ -300,8 +300,10 @@ __device__ __forceinline__ unsigned pk_bf16_small(int v0, int v1) {
 // |v| > 128), into acc (and acch).  When every value of the wave lies in [-128, 127]
 // (nearly always: a swap changes few counts by small amounts) lo is the value itself,
 // packed with one conversion and one byte permute per pair, and hi is zero.
-__device__ __forceinline__ void mfma_kblock(const int4 (&a)[2], const bf16x8 (&B)[3], f32x4 &acc, f32x4 &acch) {
-    const int vals[8] = {a[0].x, a[0].y, a[0].z, a[0].w, a[1].x, a[1].y, a[1].z, a[1].w};
+__device__ __forceinline__ void mfma_kblock(const int4 a, const bf16x8 (&B)[3], f32x4 &acc, f32x4 &acch) {
+    // 8 consecutive rows, packed two per word
+    const int vals[8] = {dw_lo(a.x), dw_hi(a.x), dw_lo(a.y), dw_hi(a.y),
+                         dw_lo(a.z), dw_hi(a.z), dw_lo(a.w), dw_hi(a.w)};
     int wide = 0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) wide |= (vals[i] + 128) & ~255;
@@ -362,7 +364,7 @@ __device__ __forceinline__ void gemm_delta_bf16_tile(const Geo &g, int nrep, con
     const int kb0 = wave * per, kb1 = min(g.nkb, kb0 + per);
     // all operands of this wave's K blocks, loaded before any MFMA
     bf16x8 B[KB][3];
-    int4 Av[NA][KB][2][2];
+    int4 Av[NA][KB][2];   // 8 consecutive rows of one slot, two per word
 #pragma unroll
     for (int q = 0; q < KB; ++q) {
         const int kb = kb0 + q;
@@ -377,15 +379,9 @@ __device__ __forceinline__ void gemm_delta_bf16_tile(const Geo &g, int nrep, con
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb) {
                     const int slot = r0 + rb * 16 + (lane & 15);
-                    if (slot < nrep) {
-                        const int4 *src = reinterpret_cast<const int4 *>(
-                            dc[a] + (long)slot * g.nrow_tot + (long)e * g.rstride + k0);
-                        Av[a][q][rb][0] = src[0];
-                        Av[a][q][rb][1] = src[1];
-                    } else {
-                        Av[a][q][rb][0] = make_int4(0, 0, 0, 0);
-                        Av[a][q][rb][1] = make_int4(0, 0, 0, 0);
-                    }
+                    Av[a][q][rb] = (slot < nrep) ? *reinterpret_cast<const int4 *>(
+                                                       dc[a] + (long)slot * g.nrow_w + ((e * g.rstride + k0) >> 1))
+                                                 : make_int4(0, 0, 0, 0);
                 }
         }
     }
@@ -485,7 +481,7 @@ __device__ __forceinline__ void gemm_delta_f32_body(const Geo &g, int nrep, cons
     for (int q = threadIdx.x; q < RB * K; q += blockDim.x) {
         const int m = q / K, k = q % K;
         const int slot = r0 + m;
-        fl[k * ld + m] = (slot < nrep) ? (float)dcounts[(long)slot * g.nrow_tot + (long)e * g.rstride + k] : 0.f;
+        fl[k * ld + m] = (slot < nrep) ? (float)dw_row(dcounts + (long)slot * g.nrow_w, e * g.rstride + k) : 0.f;
     }
     __syncthreads();
     const int tn = threadIdx.x % NCW, ks = threadIdx.x / NCW;
@@ -726,23 +722,25 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     }
     // ... and of the count commit.  With the pair-count series of this slot's rung being
     // recorded, the lead workgroup commits every row (it reads the committed counts).
+    // The chunks are in words of the packed count-change row (rows 2w, 2w+1).
     int *cn = A.counts + (long)s * g.nrow_tot;
-    int *dca = A.dc_acc + (long)s * g.nrow_tot;
+    int *dca = A.dc_acc + (long)s * g.nrow_w;
     const long long ri = (long long)stepc - 1 - A.rec_base;
     const bool series_on = A.accept == 1 && A.series && rung == A.series_rung && ri >= 0 &&
                            ri < A.series_cap;
     // (the prefetched chunk does not depend on series_on, which needs the step counter)
-    const int rchunk = (g.nrow_tot + nwg - 1) / nwg;
-    const int pbeg = split * rchunk, pend = min(g.nrow_tot, pbeg + rchunk);
-    const int rbeg = series_on ? 0 : pbeg, rend = series_on ? (lead ? g.nrow_tot : 0) : pend;
-    const Buf bcn = make_buf(cn, 4u * g.nrow_tot), bdc = make_buf(dca, 4u * g.nrow_tot);
+    const int wchunk = (g.nrow_w + nwg - 1) / nwg;
+    const int pbeg = split * wchunk, pend = min(g.nrow_w, pbeg + wchunk);
+    const int rbeg = series_on ? 0 : pbeg, rend = series_on ? (lead ? g.nrow_w : 0) : pend;
+    const Buf bcn = make_buf(cn, 4u * g.nrow_tot), bdc = make_buf(dca, 4u * g.nrow_w);
     constexpr int RC = 2;
-    int cnv[RC], dcv[RC];
+    int dcv[RC];
+    int2 cnv[RC];
 #pragma unroll
     for (int r = 0; r < RC; ++r) {
-        const int q = pbeg + t + r * nt;
-        dcv[r] = buf_i32(bdc, A.accept && q < pend, 4u * q);
-        cnv[r] = buf_i32(bcn, A.accept == 1 && q < pend, 4u * q);
+        const int w = pbeg + t + r * nt;
+        dcv[r] = buf_i32(bdc, A.accept && w < pend, 4u * w);
+        cnv[r] = buf_i64(bcn, A.accept == 1 && w < pend, 8u * w);
     }
     // this lane's first gather records (independent of the proposal)
     const int stride = nwg * nt;
@@ -841,27 +839,26 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
         // ------------------------------------------------ commit of this workgroup's share
         const int acc = flag & 1;
         const bool add = acc && A.accept == 1;
-        // rows rbeg + t + r nt: prefetched when inside this workgroup's chunk (rbeg == pbeg
+        // words rbeg + t + r nt: prefetched when inside this workgroup's chunk (rbeg == pbeg
         // whenever rend > 0: series_on commits all rows in the lead, whose chunk starts at 0)
+        int2 *cn2 = reinterpret_cast<int2 *>(cn);
+        auto commit_word = [&](int w, int d, int2 c) {
+            if (d != 0) {
+                if (add) cn2[w] = make_int2(c.x + dw_lo(d), c.y + dw_hi(d));
+                dca[w] = 0;
+            }
+        };
 #pragma unroll
         for (int r = 0; r < RC; ++r) {
-            const int q = rbeg + t + r * nt;
-            if (q < rend) {
-                const bool pf = q < pend;
-                const int d = pf ? dcv[r] : dca[q];
-                if (d != 0) {
-                    if (add) cn[q] = (pf ? cnv[r] : cn[q]) + d;
-                    dca[q] = 0;
-                }
+            const int w = rbeg + t + r * nt;
+            if (w < rend) {
+                if (w < pend)
+                    commit_word(w, dcv[r], cnv[r]);
+                else
+                    commit_word(w, dca[w], cn2[w]);
             }
         }
-        for (int q = rbeg + t + RC * nt; q < rend; q += nt) {
-            const int d = dca[q];
-            if (d != 0) {
-                if (add) cn[q] += d;
-                dca[q] = 0;
-            }
-        }
+        for (int w = rbeg + t + RC * nt; w < rend; w += nt) commit_word(w, dca[w], cn2[w]);
         if (series_on && lead) {
             __syncthreads();
             const int nss = g.nsp * g.nsp * g.nshell;
@@ -899,7 +896,7 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
         // the learned proposal (or its uniform fallback) chosen by the selection kernel
         const Prop p{pn[0], pn[1], pn[2], pn[3], pn[4]};
         if (p.sw) gather_swap(g, p, A.ss_sten, A.ms_sten, lsp, hist, first, stride, recs,
-                              A.dc_prop + (long)s * g.nrow_tot);
+                              A.dc_prop + (long)s * g.nrow_w);
         return;
     }
     if (t == 0) {
@@ -926,7 +923,7 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     }
     if (p.sw)
         gather_swap(g, p, A.ss_sten, A.ms_sten, lsp, hist, first, stride, recs,
-                    A.dc_prop + (long)s * g.nrow_tot);
+                    A.dc_prop + (long)s * g.nrow_w);
 }
 
 __global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
@@ -1061,19 +1058,21 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
     const long long ri1 = (long long)stepc - 2 - A.rec_base, ri2 = ri1 + 1;
     const bool series_on = A.accept && A.series && rung == A.series_rung &&
                            ((ri1 >= 0 && ri1 < A.series_cap) || (ri2 >= 0 && ri2 < A.series_cap));
-    // (the prefetched chunk does not depend on series_on, which needs the step counter)
-    const int rchunk = (g.nrow_tot + nwg - 1) / nwg;
-    const int pbeg = split * rchunk, pend = min(g.nrow_tot, pbeg + rchunk);
-    const int rbeg = series_on ? 0 : pbeg, rend = series_on ? (lead ? g.nrow_tot : 0) : pend;
+    // (chunks in words of the packed count-change rows: rows 2w, 2w+1; the prefetched chunk
+    // does not depend on series_on, which needs the step counter)
+    const int wchunk = (g.nrow_w + nwg - 1) / nwg;
+    const int pbeg = split * wchunk, pend = min(g.nrow_w, pbeg + wchunk);
+    const int rbeg = series_on ? 0 : pbeg, rend = series_on ? (lead ? g.nrow_w : 0) : pend;
     constexpr int RC = 2;
-    int cnv[RC], dcv[3][RC];
+    int dcv[3][RC];
+    int2 cnv[RC];
     {
         const Buf bcn = make_buf(cn, 4u * g.nrow_tot);
 #pragma unroll
-        for (int r = 0; r < RC; ++r) cnv[r] = buf_i32(bcn, A.accept && pbeg + t + r * nt < pend, 4u * (pbeg + t + r * nt));
+        for (int r = 0; r < RC; ++r) cnv[r] = buf_i64(bcn, A.accept && pbeg + t + r * nt < pend, 8u * (pbeg + t + r * nt));
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
-            const Buf bd = make_buf(A.dc_acc[v] + (long)s * g.nrow_tot, 4u * g.nrow_tot);
+            const Buf bd = make_buf(A.dc_acc[v] + (long)s * g.nrow_w, 4u * g.nrow_w);
 #pragma unroll
             for (int r = 0; r < RC; ++r)
                 dcv[v][r] = buf_i32(bd, A.accept && pbeg + t + r * nt < pend, 4u * (pbeg + t + r * nt));
@@ -1219,13 +1218,12 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
         const int v2 = acc1 ? 2 : 1;
         // ------------------------------------------------ commit of this workgroup's share
         const int nss = g.nsp * g.nshell;
-        auto commit_row = [&](int q, int c, int d1, int d2r, int d2a) {
+        int2 *cn2 = reinterpret_cast<int2 *>(cn);
+        int *dw0 = A.dc_acc[0] + (long)s * g.nrow_w, *dw1 = A.dc_acc[1] + (long)s * g.nrow_w,
+            *dw2 = A.dc_acc[2] + (long)s * g.nrow_w;
+        auto commit_row = [&](int q, int c, int d1, int d2r, int d2a) {   // row q, returns cB
             const int d2 = acc1 ? d2a : d2r;
             const int cA = c + (acc1 ? d1 : 0), cB = cA + (acc2 ? d2 : 0);
-            if (cB != c) cn[q] = cB;
-            if (d1) A.dc_acc[0][(long)s * g.nrow_tot + q] = 0;
-            if (d2r) A.dc_acc[1][(long)s * g.nrow_tot + q] = 0;
-            if (d2a) A.dc_acc[2][(long)s * g.nrow_tot + q] = 0;
             if (series_on) {
                 const int a = q / g.rstride, r = q - a * g.rstride;
                 if (a < g.nsp && r < nss) {
@@ -1234,24 +1232,29 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
                     if (ri2 >= 0 && ri2 < A.series_cap) A.series[ri2 * ns + idx] = cB;
                 }
             }
+            return cB;
         };
-        // rows rbeg + t + r nt: prefetched when inside this workgroup's chunk (rbeg == pbeg
+        auto commit_word = [&](int w, int2 c, int d1, int d2r, int d2a) {
+            const int lo = commit_row(2 * w, c.x, dw_lo(d1), dw_lo(d2r), dw_lo(d2a));
+            const int hi = commit_row(2 * w + 1, c.y, dw_hi(d1), dw_hi(d2r), dw_hi(d2a));
+            if (lo != c.x || hi != c.y) cn2[w] = make_int2(lo, hi);
+            if (d1) dw0[w] = 0;
+            if (d2r) dw1[w] = 0;
+            if (d2a) dw2[w] = 0;
+        };
+        // words rbeg + t + r nt: prefetched when inside this workgroup's chunk (rbeg == pbeg
         // whenever rend > 0: series_on commits all rows in the lead, whose chunk starts at 0)
 #pragma unroll
         for (int r = 0; r < RC; ++r) {
-            const int q = rbeg + t + r * nt;
-            if (q < rend) {
-                const long o = (long)s * g.nrow_tot + q;
-                if (q < pend)
-                    commit_row(q, cnv[r], dcv[0][r], dcv[1][r], dcv[2][r]);
+            const int w = rbeg + t + r * nt;
+            if (w < rend) {
+                if (w < pend)
+                    commit_word(w, cnv[r], dcv[0][r], dcv[1][r], dcv[2][r]);
                 else
-                    commit_row(q, cn[q], A.dc_acc[0][o], A.dc_acc[1][o], A.dc_acc[2][o]);
+                    commit_word(w, cn2[w], dw0[w], dw1[w], dw2[w]);
             }
         }
-        for (int q = rbeg + t + RC * nt; q < rend; q += nt) {
-            const long o = (long)s * g.nrow_tot + q;
-            commit_row(q, cn[q], A.dc_acc[0][o], A.dc_acc[1][o], A.dc_acc[2][o]);
-        }
+        for (int w = rbeg + t + RC * nt; w < rend; w += nt) commit_word(w, cn2[w], dw0[w], dw1[w], dw2[w]);
         // the running chi(R): the trial of the last accepted step
         const int fv = acc2 ? v2 : (acc1 ? 0 : -1);
         if (fv >= 0) {
@@ -1354,16 +1357,17 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
     for (int it = first + kRecPrefetch * stride; it < nitems; it += stride)
         items(it, item_rec(g, A.ss_sten, A.ms_sten, it));
     __syncthreads();
-    int *d3 = A.dc_next[0] + (long)s * g.nrow_tot;
-    int *d4r = pair ? A.dc_next[1] + (long)s * g.nrow_tot : nullptr;
-    int *d4a = pair ? A.dc_next[2] + (long)s * g.nrow_tot : nullptr;
-    for (int q = t; q < g.nrow_tot; q += nt) {
-        const int h0 = hist0[q];
-        if (h0 != 0) atomicAdd(&d3[q], h0);
+    int *d3 = A.dc_next[0] + (long)s * g.nrow_w;
+    int *d4r = pair ? A.dc_next[1] + (long)s * g.nrow_w : nullptr;
+    int *d4a = pair ? A.dc_next[2] + (long)s * g.nrow_w : nullptr;
+    for (int w = t; w < g.nrow_w; w += nt) {
+        const int h0 = dw_pack(hist0[2 * w], hist0[2 * w + 1]);
+        if (h0 != 0) atomicAdd(&d3[w], h0);
         if (pair) {
-            const int h1 = hist1[q], h2 = same ? h1 : hist2[q];
-            if (h1 != 0) atomicAdd(&d4r[q], h1);
-            if (h2 != 0) atomicAdd(&d4a[q], h2);
+            const int h1 = dw_pack(hist1[2 * w], hist1[2 * w + 1]);
+            const int h2 = same ? h1 : dw_pack(hist2[2 * w], hist2[2 * w + 1]);
+            if (h1 != 0) atomicAdd(&d4r[w], h1);
+            if (h2 != 0) atomicAdd(&d4a[w], h2);
         }
     }
 }
@@ -1826,6 +1830,7 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
     g.nrow_e = g.nss_rows + g.nblk + g.has_const;
     g.rstride = (g.nrow_e + 31) & ~31;
     g.nrow_tot = g.nsp * g.rstride;
+    g.nrow_w = g.nrow_tot / 2;   // rstride is even
     g.nkw = gm->nkw;
     g.nwin = gm->nwin;
     g.ncol_e = gm->nkw * gm->nwin * 2;

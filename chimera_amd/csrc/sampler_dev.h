@@ -52,6 +52,9 @@ __device__ __forceinline__ double buf_f64(const Buf &b, bool ok, uint32_t off) {
 __device__ __forceinline__ int buf_i32(const Buf &b, bool ok, uint32_t off) {
     return (int)__builtin_amdgcn_raw_buffer_load_b32(b.r, ok ? off : b.bytes, 0, 0);
 }
+__device__ __forceinline__ int2 buf_i64(const Buf &b, bool ok, uint32_t off) {
+    return __builtin_bit_cast(int2, __builtin_amdgcn_raw_buffer_load_b64(b.r, ok ? off : b.bytes, 0, 0));
+}
 __device__ __forceinline__ int4 buf_i128(const Buf &b, bool ok, uint32_t off) {
     return __builtin_bit_cast(int4, __builtin_amdgcn_raw_buffer_load_b128(b.r, ok ? off : b.bytes, 0, 0));
 }
@@ -63,6 +66,7 @@ struct Geo {
     int bpar[8];
     int has_const;
     int nct, nkb;   // 16-column tiles per edge, 32-row K blocks per edge
+    int nrow_w;     // 32-bit words per count-change row (two rows per word)
     int bxyz[12];   // coordinates (units a0/2) of each basis site in the cell
     // the same two tables packed into one register each, so a per-lane lookup is a shift
     // (an indexed read of the arrays above would be a memory load per lane):
@@ -81,6 +85,19 @@ __device__ __forceinline__ int site_index(const Geo &g, int x, int y, int z) {
 __device__ __forceinline__ int wrap(int v, int n) {
     v = (v < 0) ? v + n : v;
     return (v >= n) ? v - n : v;
+}
+
+// Count-change rows hold two rows per 32-bit word: row 2w in the low half and row 2w+1 in
+// the high half, each a 16-bit two's-complement value (w = lo + 65536 hi, modulo 2^32).
+// Adding packed words with 32-bit integer atomics adds both halves exactly, so a merged
+// word decodes exactly as long as both final values fit in 16 bits: a swap changes a
+// count by at most 2 x (gather items) x 2 < 2^15.  The transform reads half the bytes.
+__device__ __forceinline__ int dw_lo(int w) { return (int)((unsigned)w << 16) >> 16; }
+__device__ __forceinline__ int dw_hi(int w) { return (int)((unsigned)w - (unsigned)dw_lo(w)) >> 16; }
+__device__ __forceinline__ int dw_pack(int lo, int hi) { return (int)((unsigned)lo + ((unsigned)hi << 16)); }
+__device__ __forceinline__ int dw_row(const int *dw, int q) {   // row q of a packed row
+    const int w = dw[q >> 1];
+    return (q & 1) ? dw_hi(w) : dw_lo(w);
 }
 
 struct Prop {
@@ -304,8 +321,8 @@ __device__ __forceinline__ void prefetch_recs(const Geo &g, const int *__restric
 }
 
 // The count changes of swap p (a real swap) into `hist` (zeroed by the caller, visible
-// to the workgroup), then this workgroup's nonzero bins added to the slot's row `dc`
-// with integer atomics (order-independent, so the merged row is deterministic).
+// to the workgroup), then this workgroup's nonzero bins added to the slot's packed row
+// `dc` with integer atomics (order-independent, so the merged row is deterministic).
 __device__ __forceinline__ void gather_swap(const Geo &g, const Prop &p, const int *__restrict__ ss_sten,
                                             const int4 *__restrict__ ms_sten, const uint8_t *lsp,
                                             int *hist, int first, int stride, const RecPre &R, int *dc) {
@@ -324,8 +341,10 @@ __device__ __forceinline__ void gather_swap(const Geo &g, const Prop &p, const i
     for (int it = first + kRecPrefetch * stride; it < nitems; it += stride)
         gather_item(g, W, lsp, hist, it, nss_items, item_rec(g, ss_sten, ms_sten, it));
     __syncthreads();
-    for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x)
-        if (hist[q] != 0) atomicAdd(&dc[q], hist[q]);
+    for (int w = threadIdx.x; w < g.nrow_w; w += blockDim.x) {
+        const int v = dw_pack(hist[2 * w], hist[2 * w + 1]);
+        if (v != 0) atomicAdd(&dc[w], v);
+    }
 }
 
 // Site of occupancy entry q of the candidate swap (a, b): q < ncoord the q-th stencil
