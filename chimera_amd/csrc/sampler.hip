@@ -365,6 +365,9 @@ __device__ __forceinline__ void gemm_delta_bf16_tile(const Geo &g, int nrep, con
     // all operands of this wave's K blocks, loaded before any MFMA
     bf16x8 B[KB][3];
     int4 Av[NA][KB][2];   // 8 consecutive rows of one slot, two per word
+    Buf bA[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) bA[a] = make_buf(dc[a], 4u * nrep * g.nrow_w);
 #pragma unroll
     for (int q = 0; q < KB; ++q) {
         const int kb = kb0 + q;
@@ -379,9 +382,9 @@ __device__ __forceinline__ void gemm_delta_bf16_tile(const Geo &g, int nrep, con
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb) {
                     const int slot = r0 + rb * 16 + (lane & 15);
-                    Av[a][q][rb] = (slot < nrep) ? *reinterpret_cast<const int4 *>(
-                                                       dc[a] + (long)slot * g.nrow_w + ((e * g.rstride + k0) >> 1))
-                                                 : make_int4(0, 0, 0, 0);
+                    // bounds-checked (slots beyond nrep read zero): no branch, no wait
+                    Av[a][q][rb] = buf_i128(bA[a], slot < nrep,
+                                            4u * ((uint32_t)slot * g.nrow_w + ((e * g.rstride + k0) >> 1)));
                 }
         }
     }

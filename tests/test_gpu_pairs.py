@@ -30,6 +30,12 @@ def tern108():
 
 
 @pytest.fixture(scope="module")
+def tern108_40():
+    # more replica slots than one 32-slot transform row block (the reference's default is 48)
+    return _problem("t108r40", "fcc", (3, 3, 3), 3.59, ("Cr", "Co", "Ni"), 5.2, True, 40)
+
+
+@pytest.fixture(scope="module")
 def bin500():
     return _problem("b500", "fcc", (5, 5, 5), 3.59, ("Fe", "Ni"), 5.2, False, 8)
 
@@ -77,6 +83,18 @@ def test_pairs_equal_single_steps(tern108, nsteps, exch, seed):
         assert single["acc"].sum() > 0
         if exch:   # re-anchoring happens at exchange-period starts
             assert single["reanchors"].sum() > 0
+        _same(single, paired)
+    finally:
+        _restore(P)
+
+
+def test_pairs_two_row_blocks(tern108_40):
+    """40 replicas: two row blocks in every transform and in the draws column."""
+    P = tern108_40
+    try:
+        single = _run(P, False, 33, 7, 12, series=True)
+        paired = _run(P, True, 33, 7, 12, series=True)
+        assert single["acc"].sum() > 0
         _same(single, paired)
     finally:
         _restore(P)
