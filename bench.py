@@ -136,6 +136,37 @@ def learned_vs_uniform(P, S, args):
                                 "frac": round(flops / (prof["score_ms"] * 1e-3) / 1e12 / 2500.0, 6)}}
 
 
+def forward_bench(P, S, iters=10):
+    """Full chi(k) evaluations of the replicas' configurations: the pair-space path sum
+    (FP64 and FP32, with distances and pair types), its adjoint dL/dr, and the
+    relaxed-occupancy path sum, timed with HIP events on the forward model's stream.
+    Work unit: one path term = one (pair, k) evaluation (exp, sin, amplitude)."""
+    fm = P.fm
+    sp = S.species
+    nconf = sp.shape[0]
+    pos = np.repeat(P.cell.ideal[None], nconf, 0)
+    fm.chi(pos, sp)
+    fm.chi(pos, sp, precision="f32")
+    g = np.ones((nconf, P.nsp, P.k.size))
+    fm.grad_pos(pos, sp, g)
+    w = np.eye(P.nsp)[sp]
+    fm.chi_soft(pos, w)
+    terms = float(nconf) * fm.npairs * int(np.count_nonzero(P.k > 0))
+    out = {"configs": nconf, "pairs": int(fm.npairs), "k_points": int(P.k.size),
+           "terms_per_launch": terms}
+    for key, what in (("path_sum_f64", "chi"), ("path_sum_f32", "chi_f32"),
+                      ("adjoint_dr_f64", "adjoint"), ("soft_path_sum_f64", "soft")):
+        ms = fm.profile(what, nconf, iters)
+        out[key] = {"ms": round(ms, 4), "terms_per_s": float(f"{terms / (ms * 1e-3):.4g}")}
+    out["chi_evals_per_s_f32"] = round(nconf / (out["path_sum_f32"]["ms"] * 1e-3), 1)
+    out["chi_evals_per_s_f64"] = round(nconf / (out["path_sum_f64"]["ms"] * 1e-3), 1)
+    # the transcendental bound of an FP32 term (exp, sin and a reciprocal on the
+    # quarter-rate unit: 256 CUs x 16 lanes x 2.4 GHz / 3 = 3.28e12 terms/s)
+    out["path_sum_f32"]["frac_of_transcendental_bound"] = round(
+        out["path_sum_f32"]["terms_per_s"] / 3.28e12, 4)
+    return out
+
+
 def batch_bench(args):
     """Uneven problems stepped together (one launch per kernel per step, one hipGraph per
     exchange period): aggregate replica swap moves/s."""
@@ -228,6 +259,8 @@ def main():
                     help="only run the batched benchmark over NPROB uneven problems and exit")
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="only run the precision study (STEPS steps per mode) and exit")
+    ap.add_argument("--no-forward", action="store_true",
+                    help="skip the full chi(k) path-sum / adjoint timings")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-uniform proposal comparison")
     ap.add_argument("--ess-steps", type=int, default=100000)
@@ -362,6 +395,10 @@ def main():
     mb = P.ft.to_R(chib[None])[0]
     rfac = W.r_factor(mb, P.target, P.mask, W.fit_scales(mb, P.target, P.mask, P.chan_edge))
     nanchor = int(S.reanchors().sum()) if args.precision != "f64" else 0
+    fwd = None
+    if not args.no_forward:
+        print("[bench] forward model", file=sys.stderr, flush=True)
+        fwd = forward_bench(P, S)
     lvu = None
     if not args.no_learned:
         lvu = learned_vs_uniform(P, S, args)
@@ -392,6 +429,7 @@ def main():
         "alpha1_ess_per_s": round(ess_min / dt, 2),
         "alpha1_mean": {f"{a}{b}": round(float(al[:, a, b].mean()), 5) for a, b in zip(*iu)},
         "fp64_reanchors": nanchor,
+        "forward_model": fwd,
         "learned_vs_uniform": lvu,
         "autotune": {"nsplit": tuned["nsplit"],
                      "step_ms": {k: round(v, 5) for k, v in tuned["times_ms"].items()}},

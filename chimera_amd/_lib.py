@@ -65,7 +65,13 @@ _SIGS = {
     "chm_sampler_read": (C.c_int, [vp, C.c_int, vp, C.c_int64]),
     "chm_sampler_counts": (C.c_int, [vp, C.c_int, i32p, i32p]),
     "chm_sampler_chi2": (C.c_int, [vp, C.c_int, f64p, f64p]),
+    "chm_fm_tab_rows": (C.c_int, [vp, C.c_int, f64p, f64p, f64p, f64p, f64p, C.c_double, f64p]),
+    "chm_fm_grad_pos": (C.c_int, [vp, C.c_int, f64p, i32p, f64p, f64p]),
+    "chm_fm_chi_soft": (C.c_int, [vp, C.c_int, f64p, f64p, f64p]),
+    "chm_fm_grad_soft": (C.c_int, [vp, C.c_int, f64p, f64p, f64p, f64p, f64p]),
+    "chm_fm_profile": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, f64p]),
     "chm_sampler_exchange": (C.c_int, [vp]),
+    "chm_sampler_set_beta": (C.c_int, [vp, f64p]),
     "chm_sampler_chik": (C.c_int, [vp, C.c_int, i32p, f64p]),
     "chm_sampler_profile": (C.c_int, [vp, C.c_int, f64p]),
     "chm_sampler_set_precision": (C.c_int, [vp, C.c_int, C.c_int]),

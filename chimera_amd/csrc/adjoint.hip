@@ -1,0 +1,567 @@
+// Adjoints of the pair-space path sum and the relaxed-occupancy path sum (gfx950), and
+// HIP-event timing of the forward-model kernels.
+//
+// dL/dr (hard and soft occupancy): a workgroup owns 256 pairs of one configuration, one
+// pair per lane.  The per-k tables (f, delta or f cos(delta) / f sin(delta), k,
+// -2/lambda) and the configuration's dL/dchi are staged in LDS once; every lane walks
+// the k grid with its pair's accumulators in registers, so all 64 lanes of a wavefront
+// read the same k column of a table row (a handful of distinct LDS words per step).
+// dL/dpos and dL/dw: one lane per (configuration, atom) gathers its pairs through CSR
+// lists built with the topology (pairs where the atom is the absorber, then where it
+// is the scatterer), always in the same order: the gradients are deterministic (no
+// floating-point atomics).
+//
+// Relaxed-occupancy forward: lanes own k points (as the hard path sum), a pair record
+// carries both sites' weight vectors; T_ab = base f_b sin(2kr + delta_ab) is formed from
+// one sincos(2kr) per (pair, k) and the tables f_b cos(delta_ab), f_b sin(delta_ab).
+#include <cmath>
+
+#include "common.h"
+
+namespace chm {
+int distances_device(chm_ctx *ctx, int nconf, bool want_evec);
+int chi_device(chm_ctx *ctx, int nconf, int precision);
+int ptypes_device(chm_ctx *ctx, int nconf);
+int reduce_partials(chm_ctx *ctx, const double *partial, double *out, int nwg, int n_per_conf,
+                    int nconf);
+}  // namespace chm
+
+namespace {
+
+constexpr int kLanes = 256;
+constexpr int kKq = 2;   // k points per lane in the soft forward: nk <= 512
+
+int upload(chm_ctx *ctx, chm::DevBuf &b, const void *h, size_t n) {
+    int rc = b.ensure(n);
+    if (rc) return rc;
+    if (n) CHM_HIP(hipMemcpyAsync(b.p, h, n, hipMemcpyHostToDevice, ctx->stream));
+    return CHM_OK;
+}
+
+int download(chm_ctx *ctx, void *h, const chm::DevBuf &b, size_t n) {
+    if (n) CHM_HIP(hipMemcpyAsync(h, b.p, n, hipMemcpyDeviceToHost, ctx->stream));
+    CHM_HIP(hipStreamSynchronize(ctx->stream));
+    return CHM_OK;
+}
+
+// ---------------------------------------------------------------- hard occupancy
+
+// gr[ic][p] = sum_k gchi[ic][a_p][k] dT_p/dr (r_eff = r + dr, sigma^2 + s2tab, C3 phase)
+template <int NSP>
+__global__ __launch_bounds__(256) void k_adjoint_r(
+    const double *__restrict__ r_all, const int *__restrict__ pt_all,
+    const int *__restrict__ shell, const double *__restrict__ sig2,
+    const double *__restrict__ ftab, const double *__restrict__ dtab,
+    const double *__restrict__ lamtab, const double *__restrict__ ktab,
+    const double *__restrict__ drtab, const double *__restrict__ c3tab,
+    const double *__restrict__ s2tab, const double *__restrict__ gchi,
+    double *__restrict__ gr, int npairs, int nk, double s02) {
+    extern __shared__ double smem[];
+    double *sf = smem;                  // f (NSP, nk)
+    double *sd = sf + NSP * nk;         // delta (NSP*NSP, nk)
+    double *sg = sd + NSP * NSP * nk;   // this configuration's dL/dchi (NSP, nk)
+    double *sk = sg + NSP * nk;         // k
+    double *sl = sk + nk;               // -2 / lambda
+    const int ic = blockIdx.y;
+    for (int q = threadIdx.x; q < NSP * nk; q += kLanes) {
+        sf[q] = ftab[q];
+        sg[q] = gchi[(long)ic * NSP * nk + q];
+    }
+    for (int q = threadIdx.x; q < NSP * NSP * nk; q += kLanes) sd[q] = dtab[q];
+    for (int q = threadIdx.x; q < nk; q += kLanes) {
+        sk[q] = ktab[q];
+        sl[q] = -2.0 / lamtab[q];
+    }
+    __syncthreads();
+    const int p = blockIdx.x * kLanes + threadIdx.x;
+    if (p >= npairs) return;
+    const long o = (long)ic * npairs + p;
+    const int t = pt_all[o];
+    const int sh = shell[p];
+    const int sa = t / NSP, sb = t - sa * NSP;
+    const double re = r_all[o] + drtab[sh * NSP * NSP + t];
+    const double ir = 1.0 / re;
+    const double s2 = sig2[p] + s2tab[sh * NSP * NSP + t];
+    const double c3 = c3tab[sh];
+    const double *gs = sg + sa * nk;
+    const double *fs = sf + sb * nk;
+    const double *ds = sd + t * nk;
+    double acc = 0.0;
+    for (int ik = 0; ik < nk; ++ik) {
+        const double g = gs[ik];
+        const double kv = sk[ik];
+        if (g == 0.0 || !(kv > 0.0)) continue;   // the k = 0 term is defined as 0
+        const double c1 = sl[ik];
+        const double amp = s02 * fs[ik] * ir * ir / kv * exp(c1 * re - 2.0 * kv * kv * s2);
+        double sn, cs;
+        sincos(2.0 * kv * re + ds[ik] - (4.0 / 3.0) * kv * kv * kv * c3, &sn, &cs);
+        acc += g * amp * ((c1 - 2.0 * ir) * sn + 2.0 * kv * cs);
+    }
+    gr[o] = acc;
+}
+
+// gpos[ic][a] = sum_{p: i_p = a} -gr_p e_p + sum_{p: j_p = a} gr_p e_p; with gwi/gwj also
+// gw[ic][a][s] = sum_{p: i_p = a} gwi_p[s] + sum_{p: j_p = a} gwj_p[s].
+__global__ void k_gather_atoms(const double *__restrict__ gr, const double *__restrict__ evec,
+                               const double *__restrict__ gwi, const double *__restrict__ gwj,
+                               const int *__restrict__ ioff, const int *__restrict__ ilist,
+                               const int *__restrict__ joff, const int *__restrict__ jlist,
+                               double *__restrict__ gpos, double *__restrict__ gw, int npairs,
+                               int natoms, int nconf, int nsp) {
+    long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long)natoms * nconf) return;
+    const int ic = (int)(tid / natoms), a = (int)(tid % natoms);
+    const double *g = gr + (long)ic * npairs;
+    const double *e = evec + (long)ic * npairs * 3;
+    double x = 0.0, y = 0.0, z = 0.0;
+    double ws[chm::kMaxNsp];
+    for (int s = 0; s < chm::kMaxNsp; ++s) ws[s] = 0.0;
+    for (int q = ioff[a]; q < ioff[a + 1]; ++q) {
+        const int p = ilist[q];
+        const double v = g[p];
+        x -= v * e[3 * p];
+        y -= v * e[3 * p + 1];
+        z -= v * e[3 * p + 2];
+        if (gw)
+            for (int s = 0; s < nsp; ++s) ws[s] += gwi[((long)ic * npairs + p) * nsp + s];
+    }
+    for (int q = joff[a]; q < joff[a + 1]; ++q) {
+        const int p = jlist[q];
+        const double v = g[p];
+        x += v * e[3 * p];
+        y += v * e[3 * p + 1];
+        z += v * e[3 * p + 2];
+        if (gw)
+            for (int s = 0; s < nsp; ++s) ws[s] += gwj[((long)ic * npairs + p) * nsp + s];
+    }
+    gpos[3 * tid] = x;
+    gpos[3 * tid + 1] = y;
+    gpos[3 * tid + 2] = z;
+    if (gw)
+        for (int s = 0; s < nsp; ++s) gw[tid * nsp + s] = ws[s];
+}
+
+// ---------------------------------------------------------------- soft occupancy
+
+// fc[t][k] = f_b(k) cos(delta_t(k)), fs[t][k] = f_b(k) sin(delta_t(k)), t = a*nsp + b
+__global__ void k_soft_tables(const double *__restrict__ ftab, const double *__restrict__ dtab,
+                              double *__restrict__ fc, double *__restrict__ fs, int nsp, int nk) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nsp * nsp * nk) return;
+    const int t = tid / nk, ik = tid - t * nk;
+    const double f = ftab[(t % nsp) * nk + ik];
+    double sn, cs;
+    sincos(dtab[tid], &sn, &cs);
+    fc[tid] = f * cs;
+    fs[tid] = f * sn;
+}
+
+template <int NSP>
+struct SoftRec {
+    double r, ir2, s2;
+    double wi[NSP], wj[NSP];
+};
+
+template <int NSP>
+__global__ __launch_bounds__(256) void k_chi_soft(
+    const double *__restrict__ r_all, const double *__restrict__ w_all,
+    const int *__restrict__ iidx, const int *__restrict__ jidx,
+    const double *__restrict__ sig2, const double *__restrict__ fctab,
+    const double *__restrict__ fstab, const double *__restrict__ lamtab,
+    const double *__restrict__ ktab, double *__restrict__ partial, int npairs, int natoms,
+    int nk, int pairs_per_wg, double s02) {
+    extern __shared__ double smem[];
+    double *sfc = smem;
+    double *sfs = sfc + NSP * NSP * nk;
+    SoftRec<NSP> *tile = reinterpret_cast<SoftRec<NSP> *>(sfs + NSP * NSP * nk);
+    const int ic = blockIdx.y, wg = blockIdx.x, nwg = gridDim.x;
+    for (int q = threadIdx.x; q < NSP * NSP * nk; q += kLanes) {
+        sfc[q] = fctab[q];
+        sfs[q] = fstab[q];
+    }
+    double kk[kKq], c1[kKq], c2[kKq], pref[kKq];
+    bool live[kKq];
+#pragma unroll
+    for (int u = 0; u < kKq; ++u) {
+        const int ik = threadIdx.x + u * kLanes;
+        live[u] = ik < nk;
+        const double kv = live[u] ? ktab[ik] : 0.0;
+        kk[u] = kv;
+        c1[u] = -2.0 / (live[u] ? lamtab[ik] : 1.0);
+        c2[u] = -2.0 * kv * kv;
+        pref[u] = kv > 0.0 ? s02 / kv : 0.0;
+    }
+    double acc[kKq][NSP];
+#pragma unroll
+    for (int u = 0; u < kKq; ++u)
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) acc[u][s] = 0.0;
+    const double *rc = r_all + (long)ic * npairs;
+    const double *wc = w_all + (long)ic * natoms * NSP;
+    const int pbeg = wg * pairs_per_wg, pend = min(npairs, pbeg + pairs_per_wg);
+    for (int p0 = pbeg; p0 < pend; p0 += kLanes) {
+        __syncthreads();
+        {
+            const int p = p0 + threadIdx.x;
+            SoftRec<NSP> rec;
+            if (p < pend) {
+                const double re = rc[p];
+                rec.r = re;
+                rec.ir2 = 1.0 / (re * re);
+                rec.s2 = sig2[p];
+                const double *wi = wc + (long)iidx[p] * NSP;
+                const double *wj = wc + (long)jidx[p] * NSP;
+#pragma unroll
+                for (int s = 0; s < NSP; ++s) {
+                    rec.wi[s] = wi[s];
+                    rec.wj[s] = wj[s];
+                }
+            } else {
+                rec.r = 1.0;
+                rec.ir2 = 0.0;   // contributes exactly 0
+                rec.s2 = 0.0;
+#pragma unroll
+                for (int s = 0; s < NSP; ++s) rec.wi[s] = rec.wj[s] = 0.0;
+            }
+            tile[threadIdx.x] = rec;
+        }
+        __syncthreads();
+        const int nt = min(kLanes, pend - p0);
+        for (int q = 0; q < nt; ++q) {
+            const SoftRec<NSP> rec = tile[q];
+#pragma unroll
+            for (int u = 0; u < kKq; ++u) {
+                if (!live[u]) continue;
+                const int ik = threadIdx.x + u * kLanes;
+                const double base = pref[u] * rec.ir2 * exp(c1[u] * rec.r + c2[u] * rec.s2);
+                double sn, cs;
+                sincos(2.0 * kk[u] * rec.r, &sn, &cs);
+#pragma unroll
+                for (int a = 0; a < NSP; ++a) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int b = 0; b < NSP; ++b) {
+                        const int t = (a * NSP + b) * nk + ik;
+                        s += rec.wj[b] * (sn * sfc[t] + cs * sfs[t]);
+                    }
+                    acc[u][a] += rec.wi[a] * base * s;
+                }
+            }
+        }
+    }
+    double *out = partial + ((long)ic * nwg + wg) * NSP * nk;
+#pragma unroll
+    for (int u = 0; u < kKq; ++u) {
+        const int ik = threadIdx.x + u * kLanes;
+        if (!live[u]) continue;
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) out[s * nk + ik] = acc[u][s];
+    }
+}
+
+// Per pair: gr (dL/dr), gwi[s] (dL/dw_i[s]), gwj[s] (dL/dw_j[s]) of the soft path sum.
+template <int NSP>
+__global__ __launch_bounds__(256) void k_adjoint_soft(
+    const double *__restrict__ r_all, const double *__restrict__ w_all,
+    const int *__restrict__ iidx, const int *__restrict__ jidx,
+    const double *__restrict__ sig2, const double *__restrict__ fctab,
+    const double *__restrict__ fstab, const double *__restrict__ lamtab,
+    const double *__restrict__ ktab, const double *__restrict__ gchi,
+    double *__restrict__ gr, double *__restrict__ gwi_out, double *__restrict__ gwj_out,
+    int npairs, int natoms, int nk, double s02) {
+    extern __shared__ double smem[];
+    double *sfc = smem;
+    double *sfs = sfc + NSP * NSP * nk;
+    double *sg = sfs + NSP * NSP * nk;
+    double *sk = sg + NSP * nk;
+    double *sl = sk + nk;
+    const int ic = blockIdx.y;
+    for (int q = threadIdx.x; q < NSP * NSP * nk; q += kLanes) {
+        sfc[q] = fctab[q];
+        sfs[q] = fstab[q];
+    }
+    for (int q = threadIdx.x; q < NSP * nk; q += kLanes) sg[q] = gchi[(long)ic * NSP * nk + q];
+    for (int q = threadIdx.x; q < nk; q += kLanes) {
+        sk[q] = ktab[q];
+        sl[q] = -2.0 / lamtab[q];
+    }
+    __syncthreads();
+    const int p = blockIdx.x * kLanes + threadIdx.x;
+    if (p >= npairs) return;
+    const long o = (long)ic * npairs + p;
+    const double re = r_all[o];
+    const double ir = 1.0 / re;
+    const double s2 = sig2[p];
+    double wi[NSP], wj[NSP], gwi[NSP], gwj[NSP];
+    const double *wc = w_all + (long)ic * natoms * NSP;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        wi[s] = wc[(long)iidx[p] * NSP + s];
+        wj[s] = wc[(long)jidx[p] * NSP + s];
+        gwi[s] = gwj[s] = 0.0;
+    }
+    double acc = 0.0;
+    for (int ik = 0; ik < nk; ++ik) {
+        const double kv = sk[ik];
+        if (!(kv > 0.0)) continue;
+        const double c1 = sl[ik];
+        const double base = s02 / kv * ir * ir * exp(c1 * re - 2.0 * kv * kv * s2);
+        const double dfac = c1 - 2.0 * ir;
+        double sn, cs;
+        sincos(2.0 * kv * re, &sn, &cs);
+#pragma unroll
+        for (int a = 0; a < NSP; ++a) {
+            const double ga = sg[a * nk + ik];
+            if (ga == 0.0) continue;
+            double st = 0.0, dt = 0.0;
+#pragma unroll
+            for (int b = 0; b < NSP; ++b) {
+                const int t = (a * NSP + b) * nk + ik;
+                const double snv = sn * sfc[t] + cs * sfs[t];   // f sin(2kr + delta)
+                const double csv = cs * sfc[t] - sn * sfs[t];   // f cos(2kr + delta)
+                st += wj[b] * snv;
+                dt += wj[b] * csv;
+                gwj[b] += ga * wi[a] * base * snv;
+            }
+            gwi[a] += ga * base * st;
+            acc += ga * wi[a] * base * (dfac * st + 2.0 * kv * dt);
+        }
+    }
+    gr[o] = acc;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+        gwi_out[o * NSP + s] = gwi[s];
+        gwj_out[o * NSP + s] = gwj[s];
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+size_t adjoint_lds(int nsp, int nk, bool soft) {
+    return soft ? sizeof(double) * (2 * nsp * nsp + nsp + 2) * nk
+                : sizeof(double) * (nsp + nsp * nsp + nsp + 2) * nk;
+}
+
+template <int NSP>
+int launch_adjoint_r(chm_ctx *ctx, int nconf) {
+    const int nk = ctx->nk, npairs = ctx->npairs;
+    const size_t lds = adjoint_lds(NSP, nk, false);
+    CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+    hipFuncSetAttribute((const void *)k_adjoint_r<NSP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds);
+    dim3 grid((npairs + kLanes - 1) / kLanes, nconf);
+    k_adjoint_r<NSP><<<grid, kLanes, lds, ctx->stream>>>(
+        ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(), ctx->sig2.as<double>(),
+        ctx->f.as<double>(), ctx->delta.as<double>(), ctx->lam.as<double>(), ctx->k.as<double>(),
+        ctx->drtab.as<double>(), ctx->c3tab.as<double>(), ctx->s2tab.as<double>(),
+        ctx->gchi.as<double>(), ctx->gr.as<double>(), npairs, nk, ctx->s02);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+template <int NSP>
+int launch_chi_soft(chm_ctx *ctx, int nconf) {
+    const int nk = ctx->nk, npairs = ctx->npairs;
+    const size_t lds = sizeof(double) * 2 * NSP * NSP * nk + sizeof(SoftRec<NSP>) * kLanes;
+    CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+    int ntiles = (npairs + kLanes - 1) / kLanes;
+    int want = (2048 + nconf - 1) / nconf;
+    int nwg = std::max(1, std::min(ntiles, want));
+    const int ppw = ((ntiles + nwg - 1) / nwg) * kLanes;
+    nwg = (npairs + ppw - 1) / ppw;
+    int rc = ctx->partial.ensure(sizeof(double) * (size_t)nconf * nwg * NSP * nk);
+    if (rc) return rc;
+    hipFuncSetAttribute((const void *)k_chi_soft<NSP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds);
+    k_chi_soft<NSP><<<dim3(nwg, nconf), kLanes, lds, ctx->stream>>>(
+        ctx->r.as<double>(), ctx->w.as<double>(), ctx->i_idx.as<int>(), ctx->j_idx.as<int>(),
+        ctx->sig2.as<double>(), ctx->soft_fc.as<double>(), ctx->soft_fs.as<double>(),
+        ctx->lam.as<double>(), ctx->k.as<double>(), ctx->partial.as<double>(), npairs, ctx->natoms,
+        nk, ppw, ctx->s02);
+    CHM_LAUNCH_CHECK();
+    if ((rc = ctx->chi.ensure(sizeof(double) * (size_t)nconf * NSP * nk))) return rc;
+    return chm::reduce_partials(ctx, ctx->partial.as<double>(), ctx->chi.as<double>(), nwg,
+                                NSP * nk, nconf);
+}
+
+template <int NSP>
+int launch_adjoint_soft(chm_ctx *ctx, int nconf) {
+    const int nk = ctx->nk, npairs = ctx->npairs;
+    const size_t lds = adjoint_lds(NSP, nk, true);
+    CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+    hipFuncSetAttribute((const void *)k_adjoint_soft<NSP>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    dim3 grid((npairs + kLanes - 1) / kLanes, nconf);
+    k_adjoint_soft<NSP><<<grid, kLanes, lds, ctx->stream>>>(
+        ctx->r.as<double>(), ctx->w.as<double>(), ctx->i_idx.as<int>(), ctx->j_idx.as<int>(),
+        ctx->sig2.as<double>(), ctx->soft_fc.as<double>(), ctx->soft_fs.as<double>(),
+        ctx->lam.as<double>(), ctx->k.as<double>(), ctx->gchi.as<double>(), ctx->gr.as<double>(),
+        ctx->gwi.as<double>(), ctx->gwj.as<double>(), npairs, ctx->natoms, nk, ctx->s02);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+#define CHM_NSP_DISPATCH(fn, ...)                                                     \
+    switch (ctx->nsp) {                                                               \
+        case 2: return fn<2>(__VA_ARGS__);                                            \
+        case 3: return fn<3>(__VA_ARGS__);                                            \
+        case 4: return fn<4>(__VA_ARGS__);                                            \
+        case 5: return fn<5>(__VA_ARGS__);                                            \
+        case 6: return fn<6>(__VA_ARGS__);                                            \
+    }                                                                                 \
+    chm::set_error("unsupported species count");                                      \
+    return CHM_E_ARG;
+
+int adjoint_r(chm_ctx *ctx, int nconf) { CHM_NSP_DISPATCH(launch_adjoint_r, ctx, nconf) }
+int chi_soft(chm_ctx *ctx, int nconf) { CHM_NSP_DISPATCH(launch_chi_soft, ctx, nconf) }
+int adjoint_soft(chm_ctx *ctx, int nconf) { CHM_NSP_DISPATCH(launch_adjoint_soft, ctx, nconf) }
+
+int soft_tables(chm_ctx *ctx) {
+    const int n = ctx->nsp * ctx->nsp * ctx->nk;
+    int rc;
+    if ((rc = ctx->soft_fc.ensure(sizeof(double) * n)) || (rc = ctx->soft_fs.ensure(sizeof(double) * n)))
+        return rc;
+    k_soft_tables<<<dim3((n + 255) / 256), 256, 0, ctx->stream>>>(
+        ctx->f.as<double>(), ctx->delta.as<double>(), ctx->soft_fc.as<double>(),
+        ctx->soft_fs.as<double>(), ctx->nsp, ctx->nk);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+int gather_atoms(chm_ctx *ctx, int nconf, bool with_w) {
+    const long tot = (long)nconf * ctx->natoms;
+    int rc = ctx->gpos.ensure(sizeof(double) * 3 * tot);
+    if (rc) return rc;
+    if (with_w && (rc = ctx->gw.ensure(sizeof(double) * ctx->nsp * tot))) return rc;
+    k_gather_atoms<<<dim3((unsigned)((tot + 255) / 256)), 256, 0, ctx->stream>>>(
+        ctx->gr.as<double>(), ctx->evec.as<double>(), with_w ? ctx->gwi.as<double>() : nullptr,
+        with_w ? ctx->gwj.as<double>() : nullptr, ctx->csr_ioff.as<int>(), ctx->csr_i.as<int>(),
+        ctx->csr_joff.as<int>(), ctx->csr_j.as<int>(), ctx->gpos.as<double>(),
+        with_w ? ctx->gw.as<double>() : nullptr, ctx->npairs, ctx->natoms, nconf, ctx->nsp);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+int check_species(const chm_ctx *ctx, int nconf, const int32_t *species) {
+    const long n = (long)nconf * ctx->natoms;
+    for (long q = 0; q < n; ++q)
+        CHM_REQUIRE(species[q] >= 0 && species[q] < ctx->nsp, CHM_E_ARG,
+                    "species index out of range");
+    return CHM_OK;
+}
+
+// Everything the hard adjoint needs on the device: pos -> r, evec; species -> types.
+int prepare_hard(chm_ctx *ctx, int nconf, const double *pos, const int32_t *species,
+                 const double *gchi) {
+    int rc = check_species(ctx, nconf, species);
+    if (rc) return rc;
+    const size_t na = (size_t)nconf * ctx->natoms;
+    if ((rc = upload(ctx, ctx->pos, pos, sizeof(double) * 3 * na))) return rc;
+    if ((rc = upload(ctx, ctx->species, species, sizeof(int32_t) * na))) return rc;
+    if ((rc = upload(ctx, ctx->gchi, gchi, sizeof(double) * (size_t)nconf * ctx->nsp * ctx->nk)))
+        return rc;
+    if ((rc = chm::distances_device(ctx, nconf, true))) return rc;
+    const long tot = (long)nconf * ctx->npairs;
+    if ((rc = ctx->ptype.ensure(sizeof(int32_t) * tot))) return rc;
+    if ((rc = ctx->gr.ensure(sizeof(double) * tot))) return rc;
+    return CHM_OK;
+}
+
+int prepare_soft(chm_ctx *ctx, int nconf, const double *pos, const double *w, const double *gchi) {
+    const size_t na = (size_t)nconf * ctx->natoms;
+    int rc;
+    if ((rc = upload(ctx, ctx->pos, pos, sizeof(double) * 3 * na))) return rc;
+    if ((rc = upload(ctx, ctx->w, w, sizeof(double) * ctx->nsp * na))) return rc;
+    if (gchi && (rc = upload(ctx, ctx->gchi, gchi,
+                             sizeof(double) * (size_t)nconf * ctx->nsp * ctx->nk)))
+        return rc;
+    if ((rc = chm::distances_device(ctx, nconf, gchi != nullptr))) return rc;
+    if ((rc = soft_tables(ctx))) return rc;
+    if (gchi) {
+        const long tot = (long)nconf * ctx->npairs;
+        if ((rc = ctx->gr.ensure(sizeof(double) * tot)) ||
+            (rc = ctx->gwi.ensure(sizeof(double) * ctx->nsp * tot)) ||
+            (rc = ctx->gwj.ensure(sizeof(double) * ctx->nsp * tot)))
+            return rc;
+    }
+    return CHM_OK;
+}
+
+}  // namespace
+
+extern "C" int chm_fm_grad_pos(chm_ctx *ctx, int nconf, const double *pos,
+                               const int32_t *species, const double *gchi, double *gpos) {
+    CHM_REQUIRE(ctx && ctx->npairs > 0, CHM_E_STATE, "set tables and topology first");
+    CHM_REQUIRE(nconf > 0 && pos && species && gchi && gpos, CHM_E_ARG, "bad arguments");
+    CHM_HIP(hipSetDevice(ctx->device));
+    int rc = prepare_hard(ctx, nconf, pos, species, gchi);
+    if (rc) return rc;
+    if ((rc = chm::ptypes_device(ctx, nconf))) return rc;
+    if ((rc = adjoint_r(ctx, nconf))) return rc;
+    if ((rc = gather_atoms(ctx, nconf, false))) return rc;
+    ctx->prof_nconf[2] = nconf;
+    return download(ctx, gpos, ctx->gpos, sizeof(double) * 3 * (size_t)nconf * ctx->natoms);
+}
+
+extern "C" int chm_fm_chi_soft(chm_ctx *ctx, int nconf, const double *pos, const double *w,
+                               double *chi) {
+    CHM_REQUIRE(ctx && ctx->npairs > 0, CHM_E_STATE, "set tables and topology first");
+    CHM_REQUIRE(nconf > 0 && pos && w && chi, CHM_E_ARG, "bad arguments");
+    CHM_REQUIRE(ctx->nk <= kLanes * kKq, CHM_E_ARG, "at most 512 k points");
+    CHM_HIP(hipSetDevice(ctx->device));
+    int rc = prepare_soft(ctx, nconf, pos, w, nullptr);
+    if (rc) return rc;
+    if ((rc = chi_soft(ctx, nconf))) return rc;
+    ctx->prof_nconf[3] = nconf;
+    return download(ctx, chi, ctx->chi, sizeof(double) * (size_t)nconf * ctx->nsp * ctx->nk);
+}
+
+extern "C" int chm_fm_grad_soft(chm_ctx *ctx, int nconf, const double *pos, const double *w,
+                                const double *gchi, double *gpos, double *gw) {
+    CHM_REQUIRE(ctx && ctx->npairs > 0, CHM_E_STATE, "set tables and topology first");
+    CHM_REQUIRE(nconf > 0 && pos && w && gchi && gpos && gw, CHM_E_ARG, "bad arguments");
+    CHM_HIP(hipSetDevice(ctx->device));
+    int rc = prepare_soft(ctx, nconf, pos, w, gchi);
+    if (rc) return rc;
+    if ((rc = adjoint_soft(ctx, nconf))) return rc;
+    if ((rc = gather_atoms(ctx, nconf, true))) return rc;
+    ctx->prof_nconf[4] = nconf;
+    const size_t na = (size_t)nconf * ctx->natoms;
+    CHM_HIP(hipMemcpyAsync(gw, ctx->gw.p, sizeof(double) * ctx->nsp * na, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    return download(ctx, gpos, ctx->gpos, sizeof(double) * 3 * na);
+}
+
+extern "C" int chm_fm_profile(chm_ctx *ctx, int what, int nconf, int iters, double *ms) {
+    CHM_REQUIRE(ctx && ctx->npairs > 0, CHM_E_STATE, "set tables and topology first");
+    CHM_REQUIRE(what >= 0 && what <= 4 && nconf > 0 && iters > 0 && ms, CHM_E_ARG,
+                "bad arguments");
+    CHM_REQUIRE(ctx->prof_nconf[what] == nconf, CHM_E_STATE,
+                "run the profiled call once with this nconf before profiling it");
+    CHM_HIP(hipSetDevice(ctx->device));
+    hipEvent_t e0, e1;
+    CHM_HIP(hipEventCreate(&e0));
+    CHM_HIP(hipEventCreate(&e1));
+    int rc = CHM_OK;
+    auto once = [&]() -> int {
+        switch (what) {
+            case 0: return chm::chi_device(ctx, nconf, CHM_PREC_F64);
+            case 1: return chm::chi_device(ctx, nconf, CHM_PREC_F32);
+            case 2: return adjoint_r(ctx, nconf);
+            case 3: return chi_soft(ctx, nconf);
+            default: return adjoint_soft(ctx, nconf);
+        }
+    };
+    if ((rc = once())) return rc;   // warm
+    hipEventRecord(e0, ctx->stream);
+    for (int it = 0; it < iters && !rc; ++it) rc = once();
+    hipEventRecord(e1, ctx->stream);
+    hipEventSynchronize(e1);
+    float t = 0.f;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    *ms = (double)t / iters;
+    return CHM_OK;
+}

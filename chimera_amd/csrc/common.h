@@ -83,8 +83,15 @@ struct chm_ctx {
     int natoms = 0, npairs = 0, nshell = 1;
     chm::DevBuf i_idx, j_idx, shift, shell, sig2;
     chm::DevBuf drtab, c3tab, s2tab;
+    // pairs of each atom as absorber / as scatterer (CSR, pair order), for the adjoints
+    chm::DevBuf csr_ioff, csr_i, csr_joff, csr_j;
     // scratch for the forward path
     chm::DevBuf pos, species, r, evec, ptype, partial, chi, rows_in, rows_out;
+    // scratch for the adjoints and the relaxed-occupancy path sum
+    chm::DevBuf w, gchi, gr, gwi, gwj, gpos, gw, soft_fc, soft_fs;
+    // nconf of the last call of each profiled kind (chm_fm_profile): path sum FP64,
+    // path sum FP32, hard adjoint, soft path sum, soft adjoint; 0 = none
+    int prof_nconf[5] = {0, 0, 0, 0, 0};
 };
 
 struct chm_ft {

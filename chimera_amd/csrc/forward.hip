@@ -130,6 +130,28 @@ extern "C" int chm_fm_set_topology(chm_ctx *ctx, int natoms, int npairs, const i
     if ((rc = upload(ctx, ctx->shift, shift, sizeof(double) * 3 * npairs))) return rc;
     if ((rc = upload(ctx, ctx->shell, shell, sizeof(int32_t) * npairs))) return rc;
     if ((rc = upload(ctx, ctx->sig2, sig2, sizeof(double) * npairs))) return rc;
+    {
+        std::vector<int> ioff(natoms + 1, 0), joff(natoms + 1, 0), il(npairs), jl(npairs);
+        for (int p = 0; p < npairs; ++p) {
+            ++ioff[i_idx[p] + 1];
+            ++joff[j_idx[p] + 1];
+        }
+        for (int a = 0; a < natoms; ++a) {
+            ioff[a + 1] += ioff[a];
+            joff[a + 1] += joff[a];
+        }
+        std::vector<int> ci(ioff.begin(), ioff.end() - 1), cj(joff.begin(), joff.end() - 1);
+        for (int p = 0; p < npairs; ++p) {
+            il[ci[i_idx[p]]++] = p;
+            jl[cj[j_idx[p]]++] = p;
+        }
+        if ((rc = upload(ctx, ctx->csr_ioff, ioff.data(), sizeof(int) * (natoms + 1)))) return rc;
+        if ((rc = upload(ctx, ctx->csr_joff, joff.data(), sizeof(int) * (natoms + 1)))) return rc;
+        if ((rc = upload(ctx, ctx->csr_i, il.data(), sizeof(int) * npairs))) return rc;
+        if ((rc = upload(ctx, ctx->csr_j, jl.data(), sizeof(int) * npairs))) return rc;
+        CHM_HIP(hipStreamSynchronize(ctx->stream));   // the host vectors go out of scope
+    }
+    for (int &v : ctx->prof_nconf) v = 0;
     // cumulant tables default to zero, sized for this topology's shells
     std::vector<double> z3((size_t)nshell * ctx->nsp * ctx->nsp, 0.0), z1(nshell, 0.0);
     ctx->nshell = nshell;
@@ -445,16 +467,36 @@ int launch_chi(chm_ctx *ctx, int nconf, int precision) {
 }  // namespace
 
 namespace chm {
-// Path sum over the context's pos/species buffers already on the device.
-int chi_device(chm_ctx *ctx, int nconf, int precision) {
-    int rc = run_distances(ctx, nconf, false);
-    if (rc) return rc;
+int distances_device(chm_ctx *ctx, int nconf, bool want_evec) {
+    return run_distances(ctx, nconf, want_evec);
+}
+
+// Pair types of the context's species buffer (nconf configurations) into ctx->ptype.
+int ptypes_device(chm_ctx *ctx, int nconf) {
     long tot = (long)nconf * ctx->npairs;
-    if ((rc = ctx->ptype.ensure(sizeof(int32_t) * tot))) return rc;
+    int rc = ctx->ptype.ensure(sizeof(int32_t) * tot);
+    if (rc) return rc;
     k_ptypes<<<dim3((unsigned)((tot + 255) / 256)), 256, 0, ctx->stream>>>(
         ctx->species.as<int>(), ctx->i_idx.as<int>(), ctx->j_idx.as<int>(), ctx->ptype.as<int>(),
         ctx->npairs, ctx->natoms, nconf, ctx->nsp);
     CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+int reduce_partials(chm_ctx *ctx, const double *partial, double *out, int nwg, int n_per_conf,
+                    int nconf) {
+    long n = (long)nconf * n_per_conf;
+    k_reduce_partials<<<dim3((unsigned)((n + 255) / 256)), 256, 0, ctx->stream>>>(
+        partial, out, nwg, n_per_conf, nconf);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+// Path sum over the context's pos/species buffers already on the device.
+int chi_device(chm_ctx *ctx, int nconf, int precision) {
+    int rc = run_distances(ctx, nconf, false);
+    if (rc) return rc;
+    if ((rc = ptypes_device(ctx, nconf))) return rc;
     switch (ctx->nsp) {
         case 2: return launch_chi<2>(ctx, nconf, precision);
         case 3: return launch_chi<3>(ctx, nconf, precision);
@@ -482,6 +524,7 @@ extern "C" int chm_fm_chi(chm_ctx *ctx, int nconf, const double *pos, const int3
     if ((rc = upload(ctx, ctx->species, species, sizeof(int32_t) * (size_t)nconf * ctx->natoms)))
         return rc;
     if ((rc = chm::chi_device(ctx, nconf, precision))) return rc;
+    ctx->prof_nconf[0] = ctx->prof_nconf[1] = nconf;
     return download(ctx, chi, ctx->chi, sizeof(double) * (size_t)nconf * ctx->nsp * ctx->nk);
 }
 
@@ -548,4 +591,49 @@ extern "C" int chm_fm_term_rows(chm_ctx *ctx, int nrows, const double *r, const 
         ctx->k.as<double>(), ctx->rows_out.as<double>(), nrows, ctx->nk, ctx->nsp, ctx->s02, amp);
     CHM_LAUNCH_CHECK();
     return download(ctx, out, ctx->rows_out, sizeof(double) * tot);
+}
+
+// ------------------------------------------------------- tabulated term rows
+
+// Row q from tabulated F, D, L (nrows, nk) on the model grid:
+//   amp s02 F / (k r^2) exp(-2 r / max(L, 1e-6) - 2 k^2 s2) sin(2 k r + D)
+__global__ void k_tab_rows(const double *__restrict__ r, const double *__restrict__ s2,
+                           const double *__restrict__ F, const double *__restrict__ D,
+                           const double *__restrict__ L, const double *__restrict__ ktab,
+                           double *__restrict__ out, int nrows, int nk, double s02, double amp) {
+    long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long)nrows * nk) return;
+    int q = (int)(tid / nk), ik = (int)(tid % nk);
+    double kv = ktab[ik];
+    if (!(kv > 0.0)) {
+        out[tid] = 0.0;
+        return;
+    }
+    double rr = r[q];
+    out[tid] = amp * s02 * F[tid] / (kv * rr * rr) *
+               exp(-2.0 * rr / fmax(L[tid], 1e-6) - 2.0 * kv * kv * s2[q]) *
+               sin(2.0 * kv * rr + D[tid]);
+}
+
+extern "C" int chm_fm_tab_rows(chm_ctx *ctx, int nrows, const double *r, const double *s2,
+                               const double *F, const double *D, const double *L, double amp,
+                               double *out) {
+    CHM_REQUIRE(ctx && ctx->nsp > 0, CHM_E_STATE, "set tables first");
+    CHM_REQUIRE(nrows > 0 && r && s2 && F && D && L && out, CHM_E_ARG, "bad arguments");
+    for (int q = 0; q < nrows; ++q) CHM_REQUIRE(r[q] > 0.0, CHM_E_ARG, "row radius must be positive");
+    CHM_HIP(hipSetDevice(ctx->device));
+    const size_t n = (size_t)nrows * ctx->nk;
+    chm::DevBuf dr, ds2, dF, dD, dL;
+    int rc;
+    if ((rc = upload(ctx, dr, r, sizeof(double) * nrows))) return rc;
+    if ((rc = upload(ctx, ds2, s2, sizeof(double) * nrows))) return rc;
+    if ((rc = upload(ctx, dF, F, sizeof(double) * n))) return rc;
+    if ((rc = upload(ctx, dD, D, sizeof(double) * n))) return rc;
+    if ((rc = upload(ctx, dL, L, sizeof(double) * n))) return rc;
+    if ((rc = ctx->rows_out.ensure(sizeof(double) * n))) return rc;
+    k_tab_rows<<<dim3((unsigned)((n + 255) / 256)), 256, 0, ctx->stream>>>(
+        dr.as<double>(), ds2.as<double>(), dF.as<double>(), dD.as<double>(), dL.as<double>(),
+        ctx->k.as<double>(), ctx->rows_out.as<double>(), nrows, ctx->nk, ctx->s02, amp);
+    CHM_LAUNCH_CHECK();
+    return download(ctx, out, ctx->rows_out, sizeof(double) * n);
 }

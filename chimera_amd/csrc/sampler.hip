@@ -2412,6 +2412,30 @@ extern "C" int chm_sampler_exchange(chm_sampler *S) {
     return CHM_OK;
 }
 
+extern "C" int chm_sampler_set_beta(chm_sampler *S, const double *beta) {
+    CHM_REQUIRE(S && beta, CHM_E_ARG, "bad arguments");
+    CHM_REQUIRE(S->inited, CHM_E_STATE, "init_state first");
+    const int nrep = S->nrep;
+    for (int r = 0; r < nrep; ++r)
+        CHM_REQUIRE(std::isfinite(beta[r]) && beta[r] > 0.0, CHM_E_ARG, "beta must be finite and > 0");
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    hipStream_t st = S->ctx->stream;
+    CHM_HIP(hipStreamSynchronize(st));
+    // The temperatures travel with the exchange (slot s holds rung rung_of[s]); the
+    // new ladder is written per rung and re-gathered into slot order.
+    std::vector<int> rung_of(nrep);
+    CHM_HIP(hipMemcpy(rung_of.data(), S->rung_of.p, sizeof(int) * nrep, hipMemcpyDeviceToHost));
+    std::vector<double> bslot(nrep);
+    for (int s = 0; s < nrep; ++s) {
+        CHM_REQUIRE(rung_of[s] >= 0 && rung_of[s] < nrep, CHM_E_STATE, "corrupt rung map");
+        bslot[s] = beta[rung_of[s]];
+    }
+    CHM_HIP(hipMemcpyAsync(S->beta.p, beta, sizeof(double) * nrep, hipMemcpyHostToDevice, st));
+    CHM_HIP(hipMemcpyAsync(S->beta_slot.p, bslot.data(), sizeof(double) * nrep, hipMemcpyHostToDevice, st));
+    CHM_HIP(hipStreamSynchronize(st));
+    return CHM_OK;
+}
+
 extern "C" int chm_sampler_chik(chm_sampler *S, int nconf, const int32_t *counts, double *chi) {
     CHM_REQUIRE(S && nconf > 0 && counts && chi, CHM_E_ARG, "bad arguments");
     const Geo &g = S->g;

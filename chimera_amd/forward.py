@@ -165,6 +165,74 @@ class ForwardModel:
             chi /= np.maximum(cnt, 1.0)[:, :, None]
         return chi
 
+    def tab_rows(self, r, s2, F, D, L, amp=1.0):
+        """Device evaluation of term rows from tabulated F, D, L (nrows, nk) on this
+        model's k grid (include/chimera_hip.h, chm_fm_tab_rows)."""
+        n = int(np.asarray(r).size)
+        shp = (n, self.nk)
+        out = np.empty(shp)
+        check(_lib.load().chm_fm_tab_rows(self.ctx.h, n, f64(r), f64(s2), f64(F).reshape(shp),
+                                          f64(D).reshape(shp), f64(L).reshape(shp), float(amp), out))
+        return out
+
+    # ---------------------------------------------------------------- adjoint
+    def grad_pos(self, pos, species, gchi, r=None, evec=None, ptype=None, normalise=True):
+        """dL/dpos (nconf, natoms, 3) given dL/dchi (nconf, nsp, nk) (forward.py:209-234):
+        with `normalise`, gchi is dL/dchi of the normalised chi and is divided by the
+        species counts first.  `r`/`evec`/`ptype` are accepted for signature parity; the
+        device recomputes them."""
+        pos = f64(pos).reshape(-1, self.natoms, 3)
+        species = i32(species).reshape(-1, self.natoms)
+        nconf = pos.shape[0]
+        if species.shape[0] != nconf:
+            raise ValueError("pos and species disagree on nconf")
+        g = np.asarray(gchi, np.float64).reshape(nconf, self.nsp, self.nk)
+        if normalise:
+            cnt = np.stack([(species == s).sum(axis=1) for s in range(self.nsp)], axis=1)
+            g = g / np.maximum(cnt, 1.0)[:, :, None]
+        out = np.empty((nconf, self.natoms, 3))
+        check(_lib.load().chm_fm_grad_pos(self.ctx.h, nconf, pos, species, f64(g), out))
+        return out
+
+    def chi_soft(self, pos, w, r=None, normalise=True):
+        """Relaxed-occupancy chi (nconf, nsp, nk), w (nconf, natoms, nsp) on the simplex
+        (forward.py:190-206); normalised by max(sum of w over sites, 1)."""
+        pos = f64(pos).reshape(-1, self.natoms, 3)
+        nconf = pos.shape[0]
+        w = f64(w).reshape(nconf, self.natoms, self.nsp)
+        chi = np.empty((nconf, self.nsp, self.nk))
+        check(_lib.load().chm_fm_chi_soft(self.ctx.h, nconf, pos, w, chi))
+        if normalise:
+            chi /= np.maximum(w.sum(axis=1), 1.0)[:, :, None]
+        return chi
+
+    def grad_soft(self, pos, w, gchi, r=None, evec=None, normalise=True):
+        """(dL/dpos, dL/dw) for the relaxed-occupancy model (forward.py:236-262), with the
+        derivative of the normalisation by max(sum w, 1) applied as the reference does."""
+        pos = f64(pos).reshape(-1, self.natoms, 3)
+        nconf = pos.shape[0]
+        w = f64(w).reshape(nconf, self.natoms, self.nsp)
+        gchi = np.asarray(gchi, np.float64).reshape(nconf, self.nsp, self.nk)
+        norm = np.maximum(w.sum(axis=1), 1.0)
+        g = gchi / norm[:, :, None] if normalise else gchi
+        gpos = np.empty((nconf, self.natoms, 3))
+        gw = np.empty((nconf, self.natoms, self.nsp))
+        check(_lib.load().chm_fm_grad_soft(self.ctx.h, nconf, pos, w, f64(g), gpos, gw))
+        if normalise:
+            chi_un = self.chi_soft(pos, w, normalise=False)
+            gw -= ((gchi * chi_un).sum(axis=2) / norm ** 2)[:, None, :]
+        return gpos, gw
+
+    def profile(self, what, nconf, iters=20):
+        """Average device ms of a forward-model launch sequence on the inputs of the
+        last call of that kind (include/chimera_hip.h, chm_fm_profile): "chi" / "chi_f32"
+        (path sum with distances and types), "adjoint" (dL/dr of grad_pos), "soft"
+        (chi_soft), "soft_adjoint" (per-pair part of grad_soft)."""
+        code = {"chi": 0, "chi_f32": 1, "adjoint": 2, "soft": 3, "soft_adjoint": 4}[what]
+        ms = np.zeros(1)
+        check(_lib.load().chm_fm_profile(self.ctx.h, code, int(nconf), int(iters), ms))
+        return float(ms[0])
+
     def term_rows(self, r, s2, c3, absorber, nev, ev, amp=1.0):
         """Device evaluation of term-table rows (see include/chimera_hip.h)."""
         n = int(np.asarray(r).size)

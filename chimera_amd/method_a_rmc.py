@@ -71,8 +71,9 @@ class RMCSampler:
                              "(the reference's fit driver does, chimera/fitrun.py:218)")
         if energy_prior is not None:
             raise ValueError("energy_prior is not supported by the device sampler")
-        if adapt_every:
-            raise ValueError("ladder adaptation (adapt_every) is not supported")
+        self.adapt_every = int(adapt_every)
+        self.target_accept = float(target_accept)
+        self.disp_sigma = float(disp_sigma)
         self.fm, self.ft = fm, ft
         self.nsp = int(nsp) if nsp is not None else int(fm.nsp)
         self.nrep = int(nrep)
@@ -177,8 +178,7 @@ class RMCSampler:
         B = fm.ss_basis(radii, s2)                         # (nsp, nsp, nshell, nk)
         nshell = fm.nshell
         if self.ms is not None:
-            r, ms2, ab, nev, ev = self.ms.table_rows()
-            T = fm.term_rows(r, ms2, np.zeros(r.size), ab, nev, ev, amp=self.ms.amp)
+            T = self.ms.table(fm)
             nblk = self.ms.nblk
         else:
             T, nblk = None, 0
@@ -362,24 +362,76 @@ class RMCSampler:
             return nsw, filled
         return nsw
 
+    def adapt(self, window_acc, window_exch):
+        """Retune the temperature ladder (method_a_rmc.py:376-398).  In log beta, with the
+        hottest and coldest rungs fixed, each neighbour gap is scaled by the square root
+        of its exchange rate over the window (clipped to [0.02, 0.98]) relative to the
+        mean rate, then all gaps by the one factor that restores the span: pairs that
+        rarely exchange move closer, pairs that exchange freely move apart.  The new
+        ladder goes to the device; each slot takes the beta of the rung it holds.
+        The displacement step is retuned as the reference does, though swap-only chains
+        make no displacement moves."""
+        if window_acc is not None and window_acc > 0:
+            f = min(max((window_acc / self.target_accept) ** 0.35, 0.7), 1.4)
+            self.disp_sigma = min(max(self.disp_sigma * f, 1e-4), 0.15)
+        if window_exch is None or self.nrep < 4:
+            return
+        logb = np.log(self._beta_rung)
+        rate = np.clip(np.asarray(window_exch, np.float64), 0.02, 0.98)
+        step = np.diff(logb) * (rate / max(rate.mean(), 1e-6)) ** 0.5
+        step = step * ((logb[-1] - logb[0]) / step.sum())
+        ladder = np.exp(np.concatenate([[logb[0]], logb[0] + np.cumsum(step)]))
+        check(_lib.load().chm_sampler_set_beta(self._S, f64(ladder)))
+        self._beta_rung = ladder
+
+    def _exchange_outcomes(self, since):
+        """0/1 outcome per neighbour pair of the exchange sweep made since the attempt /
+        accept counters read `since`, untried pairs filled with the mean of the tried
+        ones (method_a_rmc.py:366-372); returns (rates, counters now)."""
+        ea = self._read(8, (2, max(self.nrep - 1, 1)), np.int64)
+        tried = (ea[0] - since[0]) > 0
+        ok = (ea[1] - since[1]) > 0
+        fill = ok[tried].mean() if tried.any() else 0.3
+        return np.where(tried, ok.astype(np.float64), fill), ea
+
     def run(self, nsteps, exchange_every=25, log_every=500, callback=None):
-        """Run the chain on the device (method_a_rmc.py:401-455)."""
+        """Run the chain on the device (method_a_rmc.py:401-455).  With adapt_every the
+        run stops at every exchange to read that sweep's per-pair outcomes and retunes
+        the ladder every adapt_every steps, as the reference's loop does."""
         hist = []
         t0 = time.time()
         lib = _lib.load()
         acc0 = self.accepted_per_rung().sum()
         it = 0
+        E = int(exchange_every)
+        A = self.adapt_every
         marks = set()
         if self.collect_after is not None:
             marks.update(s for s in range(int(self.collect_after), nsteps)
                          if (s + 1) % self.collect_every == 0)
         marks.update(s for s in range(nsteps) if (s + 1) % log_every == 0 or s == 0)
+        if A:
+            h0 = int(self._read(10, (1,), np.int64)[0])
+            if E:
+                marks.update(s for s in range(nsteps) if (h0 + s + 1) % E == 0)
+            marks.update(s for s in range(nsteps) if (s + 1) % A == 0)
+            win_exch = []
+            ea_last = self._read(8, (2, max(self.nrep - 1, 1)), np.int64)
+            acc_win = self.accepted_per_rung().sum()
         marks = sorted(marks)
         for m in marks:
             n = m + 1 - it
             if n > 0:
-                check(lib.chm_sampler_run(self._S, n, int(exchange_every)))
+                check(lib.chm_sampler_run(self._S, n, E))
                 it = m + 1
+            if A and E and (h0 + m + 1) % E == 0:
+                rates, ea_last = self._exchange_outcomes(ea_last)
+                win_exch.append(rates)
+            if A and (m + 1) % A == 0:
+                acc_now = self.accepted_per_rung().sum()
+                wa = float(acc_now - acc_win) / (A * self.nrep)
+                self.adapt(wa, np.mean(win_exch, axis=0) if win_exch else None)
+                win_exch, acc_win = [], acc_now
             if (self.collect_after is not None and m >= self.collect_after
                     and (m + 1) % self.collect_every == 0):
                 self._collect()

@@ -113,6 +113,66 @@ def find_classes(lattice, a0, rmax_ms, legs=(3, 4)):
     return [PathClass(k) for k in keys]
 
 
+LIB_FILE = "library.json"
+
+
+class PathLibrary:
+    """A multiple-scattering path library on disk (chimera/mslibrary.py:261-313):
+    `library.json` (elements, nearest-neighbour distance d_nn, the class representatives
+    in units of a0/2) and, per absorber element, one feffNNNN.dat per library entry in
+    entry order (class-major, slot 0 fastest).  `prepare(k)` puts every entry's F, delta
+    and lambda on the model grid; `params` gives the per-row inputs of the table at the
+    current lattice and sigma^2."""
+
+    def __init__(self, root, composition=None):
+        import json
+        import os
+        from .scattering import read_feffdat
+        with open(os.path.join(root, LIB_FILE)) as fh:
+            js = json.load(fh)
+        self.name = os.path.basename(os.path.normpath(root))
+        self.elements = tuple(js["elements"])
+        self.nsp = len(self.elements)
+        self.d_nn = float(js["d_nn"])
+        self.unit = float(js["unit"])
+        self.lattice = js.get("lattice", "fcc")
+        self.classes = [PathClass(c["rep"]) for c in js["classes"]]
+        self.nblk = int(sum(self.nsp ** c.nslot for c in self.classes))
+        comp = np.ones(self.nsp) if composition is None else np.asarray(composition, float)
+        self.composition = comp / comp.sum()
+        self.paths = {}
+        for el in self.elements:
+            d = os.path.join(root, el)
+            files = sorted(f for f in os.listdir(d) if f.startswith("feff0") and f.endswith(".dat"))
+            if len(files) != self.nblk:
+                raise AssertionError(f"{el}: {len(files)} files for {self.nblk} entries")
+            self.paths[el] = [read_feffdat(os.path.join(d, f)) for f in files]
+        self._grid = None
+        self.F = self.D = self.L = self.reff = None
+
+    def summary(self):
+        return [dict(absorber=a, n_classes=len(self.classes), n_entries=self.nblk,
+                     total_degeneracy=int(sum(c.deg for c in self.classes)))
+                for a in self.elements]
+
+    def prepare(self, k_phys):
+        """Interpolate every entry onto the model grid (absorber-major rows)."""
+        p = np.asarray(k_phys, np.float64)
+        recs = [d for a in self.elements for d in self.paths[a]]
+        self._grid = p
+        self.F = np.array([np.interp(p, d["p"], d["f"]) for d in recs])
+        self.D = np.array([np.interp(p, d["p"], d["delta"]) for d in recs])
+        self.L = np.array([np.interp(p, d["p"], d["lam"]) for d in recs])
+        self.reff = np.array([d["reff"] for d in recs])
+        return self
+
+    def params(self, sigma2_nn, r_nn, c_ms=1.3, a_scale=1.0):
+        """Per-row r and sigma^2 of the table (mslibrary.py:304-313): the file's half path
+        length times the lattice scale, sigma^2 = c_ms sigma2_nn r / (2 r_nn)."""
+        r = self.reff * float(a_scale)
+        return r, c_ms * sigma2_nn * r / (2.0 * r_nn)
+
+
 class MSLibrary:
     """Path classes, entries, device stencil and synthetic table rows of a supercell.
 
@@ -123,10 +183,24 @@ class MSLibrary:
     with r the class's half path length.
     """
 
-    def __init__(self, cell, nsp, rmax_ms, sigma2_nn=0.006, r_nn=None, c_ms=1.3, amp=1.0):
+    def __init__(self, cell, nsp, rmax_ms, sigma2_nn=0.006, r_nn=None, c_ms=1.3, amp=1.0,
+                 library=None):
+        """`library`: a PathLibrary (prepared on the forward model's k grid) whose
+        classes and FEFF tables replace the enumerated classes and synthetic rows; the
+        table's lattice scale is then cell.unit / library.unit."""
         self.cell = cell
         self.nsp = int(nsp)
-        self.classes = find_classes(cell.lattice, cell.a0, rmax_ms)
+        self.library = library
+        if library is not None:
+            if library.nsp != self.nsp:
+                raise ValueError(f"the library holds {library.nsp} species, the cell {self.nsp}")
+            if library.lattice != cell.lattice:
+                raise ValueError(f"a {library.lattice} library on a {cell.lattice} cell")
+            if getattr(library, "F", None) is None:
+                raise ValueError("prepare() the path library on the model k grid first")
+            self.classes = list(library.classes)
+        else:
+            self.classes = find_classes(cell.lattice, cell.a0, rmax_ms)
         self.unit = cell.unit
         self.nblk = int(sum(self.nsp ** c.nslot for c in self.classes))
         self.cls_off = np.cumsum([0] + [self.nsp ** c.nslot for c in self.classes])[:-1]
@@ -136,8 +210,51 @@ class MSLibrary:
         self.sigma2_nn = float(sigma2_nn)
         self.r_nn = float(r_nn) if r_nn is not None else float(
             np.sqrt((lattice_offsets(cell.lattice, 2.0) ** 2).sum(1)).min() * self.unit)
+        self.d_nn = self.r_nn
         self.c_ms, self.amp = float(c_ms), float(amp)
+        self.a_scale = 1.0 if library is None else self.unit / library.unit
         self._build_stencil()
+
+    def table(self, fm):
+        """(nsp * nblk, nk) term rows, one occurrence of every entry (absorber-major), on
+        the device: synthetic rows from the forward model's tables, or the library's
+        tabulated F, delta, lambda at the current parameters."""
+        if self.library is not None:
+            r, s2 = self.library.params(self.sigma2_nn, self.r_nn, self.c_ms, self.a_scale)
+            lib = self.library
+            if lib.F.shape[1] != fm.nk:
+                raise ValueError("the library was prepared on another k grid")
+            return fm.tab_rows(r, s2, lib.F, lib.D, lib.L, amp=self.amp)
+        r, s2, ab, nev, ev = self.table_rows()
+        return fm.term_rows(r, s2, np.zeros(r.size), ab, nev, ev, amp=self.amp)
+
+    def set_params(self, sigma2_nn, r_nn, c_ms=1.3, amp=1.0, a_scale=1.0):
+        """Table parameters at the current lattice (chimera/mslibrary.py:304-313, 410-412):
+        nearest-neighbour sigma^2 and distance, Debye-Waller scale c_ms, amplitude, and
+        the lattice scale applied to every path length."""
+        self.sigma2_nn, self.r_nn = float(sigma2_nn), float(r_nn)
+        self.c_ms, self.amp, self.a_scale = float(c_ms), float(amp), float(a_scale)
+        return self
+
+    def expected_counts(self, composition):
+        """(nsp * nblk,) occurrences of every entry around one absorber of each species in
+        a random alloy of `composition` (chimera/mslibrary.py:315-321): the class's
+        directed image count times the probability of the slot species, the slots
+        being distinct sites."""
+        c = np.asarray(composition, np.float64)
+        c = c / c.sum()
+        blk = np.array([self.classes[g].deg * np.prod([c[s] for s in sp]) for g, sp in self.entries()])
+        return np.tile(blk, self.nsp)
+
+    def chi_fast(self, fm, composition):
+        """Composition-averaged chi_MS (nsp, nk) per absorbing atom
+        (chimera/mslibrary.py:329-333, the drop-in for multiscat.py:133-145): the entry
+        rows evaluated on the device at the current parameters, weighted by the
+        random-alloy occurrences."""
+        T = self.table(fm)
+        w = self.expected_counts(composition)
+        nb = self.nblk
+        return np.stack([w[a * nb:(a + 1) * nb] @ T[a * nb:(a + 1) * nb] for a in range(self.nsp)])
 
     # ------------------------------------------------------------- entries
     def entries(self):
@@ -162,7 +279,7 @@ class MSLibrary:
         for a in range(self.nsp):
             for g, sp in ent:
                 cl = self.classes[g]
-                rr = cl.reff_units() * self.unit
+                rr = cl.reff_units() * self.unit * self.a_scale
                 seq = [a if t < 0 else sp[t] for t in cl.slot_of]
                 r[q] = rr
                 s2[q] = self.c_ms * self.sigma2_nn * rr / (2.0 * self.r_nn)

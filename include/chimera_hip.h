@@ -92,6 +92,43 @@ int chm_fm_term_rows(chm_ctx *ctx, int nrows, const double *r, const double *s2,
                      const double *c3, const int32_t *absorber, const int32_t *nev,
                      const int32_t *ev /* (nrows, 3) */, double amp, double *out);
 
+/* Term table rows from TABULATED per-row F, D, L (nrows, nk, on the model grid):
+ *   amp * s02 * F / (k r^2) * exp(-2 r / max(L, 1e-6) - 2 k^2 s2) * sin(2 k r + D)
+ * with the row's (r, s2).  The exact multiple-scattering table of a path library read
+ * from FEFF output, PathLibrary.table (chimera/mslibrary.py:304-313).  out (nrows, nk). */
+int chm_fm_tab_rows(chm_ctx *ctx, int nrows, const double *r, const double *s2,
+                    const double *F, const double *D, const double *L, double amp, double *out);
+
+/* Adjoint of the path sum: dL/dpos (nconf, natoms, 3) given dL/dchi (nconf, nsp, nk) of
+ * the UNNORMALISED chi of chm_fm_chi at positions pos and species (the caller folds the
+ * per-species normalisation into dL/dchi).  Per pair dT/dr_eff summed over k with the
+ * absorber's dL/dchi (dr / C3 / sigma^2-by-type tables applied), then -g e_ij to the
+ * absorber and +g e_ij to the scatterer, gathered per atom in a fixed order
+ * (deterministic).  ForwardModel.grad_pos (forward.py:209-234), kernels chi_adjoint_r
+ * and scatter_force (kernels.py:166-237). */
+int chm_fm_grad_pos(chm_ctx *ctx, int nconf, const double *pos, const int32_t *species,
+                    const double *gchi, double *gpos);
+
+/* Relaxed-occupancy path sum, UNNORMALISED chi (nconf, nsp, nk): every pair contributes
+ * sum_ab w_i[a] w_j[b] T_ab(r_ij, k) with per-site weights w (nconf, natoms, nsp); no
+ * cumulant tables.  ForwardModel.chi_soft (forward.py:190-206), kernel chi_forward_soft
+ * (kernels.py:514-589). */
+int chm_fm_chi_soft(chm_ctx *ctx, int nconf, const double *pos, const double *w, double *chi);
+
+/* Adjoint of chm_fm_chi_soft: dL/dpos (nconf, natoms, 3) and dL/dw (nconf, natoms, nsp)
+ * given dL/dchi of the UNNORMALISED soft chi.  ForwardModel.grad_soft (forward.py:236-262;
+ * the normalisation term is the caller's), kernel chi_adjoint_soft (kernels.py:592-660). */
+int chm_fm_grad_soft(chm_ctx *ctx, int nconf, const double *pos, const double *w,
+                     const double *gchi, double *gpos, double *gw);
+
+/* Average device time (ms) of one launch sequence over `iters`, HIP events on the
+ * context stream, on the inputs of the last call of that kind with the same nconf:
+ * what 0 = chm_fm_chi FP64 (distances + types + path sum + partial reduction), 1 = the
+ * same in FP32, 2 = the hard adjoint's dL/dr kernel, 3 = the soft path sum (+ reduction),
+ * 4 = the soft adjoint's per-pair kernel.  No counterpart in the reference (it has no
+ * profiler hooks). */
+int chm_fm_profile(chm_ctx *ctx, int what, int nconf, int iters, double *ms);
+
 /* ------------------------------------------------------------------ transform */
 
 /* One windowed transform per channel, folded with the model-grid resample into a
@@ -210,6 +247,12 @@ int chm_sampler_set_precision(chm_sampler *s, int incremental, int reanchor_ever
 /* One replica-exchange sweep now (method_a_rmc.py:332-373); run() also exchanges on
  * its own every exchange_every steps. */
 int chm_sampler_exchange(chm_sampler *s);
+
+/* Replace the temperature ladder (nrep betas, rung order, each finite and > 0) of a
+ * running chain; every slot takes the beta of the rung it currently holds.  The ladder
+ * retuning of RMCSampler.adapt (method_a_rmc.py:376-398) computes the new values on the
+ * host, as the reference does, and hands them over here. */
+int chm_sampler_set_beta(chm_sampler *s, const double *beta);
 
 /* Unnormalised chi(k) (nconf, nsp, nk) of count vectors (nconf, nsp*nrow_e):
  * counts @ basis_k per edge (SupercellMS.chi_of, chimera/mslibrary.py:453-457, and the

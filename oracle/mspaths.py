@@ -132,6 +132,16 @@ def entry_list(classes, nsp):
     return out
 
 
+def expected_counts(classes, nsp, composition):
+    """(nsp * nblk,) mean occurrences of every entry around one absorber in a random
+    alloy (chimera/mslibrary.py:315-321): the class's directed image count times the
+    product of the slot species' concentrations (slots are distinct sites)."""
+    c = np.asarray(composition, float)
+    c = c / c.sum()
+    blk = np.array([classes[g].deg * np.prod([c[s] for s in sp]) for g, sp in entry_list(classes, nsp)])
+    return np.tile(blk, nsp)
+
+
 def class_offsets(classes, nsp):
     return np.cumsum([0] + [nsp ** c.nslot for c in classes])[:-1].astype(np.int64)
 

@@ -12,6 +12,7 @@ Follows methods/method_a_rmc.py and exafs_gpu/kernels.py:
   * best retention                                      method_a_rmc.py:301-307
   * replica exchange, alternating neighbour pairs       method_a_rmc.py:332-373
   * ladder beta = 1/geomspace(tmin, tmax, nrep)         method_a_rmc.py:118
+  * ladder retuning from window exchange rates         method_a_rmc.py:376-425
   * collection rungs nearest collect_beta               method_a_rmc.py:457-476
 
 Decisions that differ from the reference (the reference's own choice cannot be
@@ -95,6 +96,20 @@ def exchange_uniforms(seed, iex, npairs):
     return start, us
 
 
+def adapt_ladder(beta, window_exch):
+    """Temperature ladder retuned from the exchange rates of a window
+    (method_a_rmc.py:376-398, ladder part; the displacement-step part has no swap-only
+    effect).  Working in log beta with both endpoints fixed: a neighbour gap whose
+    exchange rate r (clipped to [0.02, 0.98]) is above the mean rate r_bar widens, one
+    below it narrows, each by sqrt(r / r_bar); then every gap is rescaled by the one
+    factor that restores the original span."""
+    lb = np.log(np.asarray(beta, np.float64))
+    r = np.clip(np.asarray(window_exch, np.float64), 0.02, 0.98)
+    gaps = np.diff(lb) * (r / max(r.mean(), 1e-6)) ** 0.5
+    gaps = gaps * ((lb[-1] - lb[0]) / gaps.sum())
+    return np.exp(np.concatenate([[lb[0]], lb[0] + np.cumsum(gaps)]))
+
+
 def affected_pairs(a, b, nbr, rev, i_idx, j_idx):
     """propose_build's affected directed pairs with dedup (kernels.py:448-468)."""
     out = []
@@ -116,9 +131,19 @@ class Problem:
 
     def __init__(self, k, f, delta, lam, s02, ideal, i_idx, j_idx, shift, shell, sig2,
                  transforms, chan_edge, target, sigma, R=None, mask=None,
-                 ms=None, T=None):
+                 ms=None, T=None, dr=None, c3=None, s2types=None, chi_extra=None):
+        """dr / c3 / s2types: the cumulant tables of exafs_gpu/forward.py:104-145
+        (r_eff = r + dr[shell, a, b], sigma^2 + s2types[shell, a, b], phase
+        - 4/3 k^3 c3[shell]); chi_extra (nsp, nk): the fixed term added to the
+        normalised chi before transforming (chimera/multift.py:58-61)."""
         self.k, self.f, self.delta, self.lam, self.s02 = k, f, delta, lam, s02
         self.nsp = f.shape[0]
+        nshell = int(np.asarray(shell).max()) + 1
+        self.dr = np.zeros((nshell, self.nsp, self.nsp)) if dr is None else np.asarray(dr, float)
+        self.c3 = np.zeros(nshell) if c3 is None else np.asarray(c3, float)
+        self.s2types = (np.zeros((nshell, self.nsp, self.nsp)) if s2types is None
+                        else np.asarray(s2types, float))
+        self.chi_extra = None if chi_extra is None else np.asarray(chi_extra, float)
         self.ideal = ideal
         self.i, self.j, self.shift, self.shell = i_idx, j_idx, shift, shell
         self.sig2 = np.broadcast_to(np.asarray(sig2, float), i_idx.shape).copy()
@@ -165,11 +190,14 @@ class Problem:
         k = self.k
         nz = k > 0
         kz = k[nz][None, :]
-        r = self.r[pairs][:, None]
         sa, sb = t // nsp, t % nsp
+        sh = self.shell[pairs]
+        r = (self.r[pairs] + self.dr[sh, sa, sb])[:, None]
+        s2 = (self.sig2[pairs] + self.s2types[sh, sa, sb])[:, None]
         amp = (self.s02 * self.f[sb][:, nz] / (kz * r * r)
-               * np.exp(-2.0 * r / self.lam[nz][None, :] - 2.0 * kz * kz * self.sig2[pairs][:, None]))
-        term = amp * np.sin(2.0 * kz * r + self.delta[sa, sb][:, nz])
+               * np.exp(-2.0 * r / self.lam[nz][None, :] - 2.0 * kz * kz * s2))
+        term = amp * np.sin(2.0 * kz * r + self.delta[sa, sb][:, nz]
+                            - (4.0 / 3.0) * kz ** 3 * self.c3[sh][:, None])
         out = np.zeros((nsp, k.size))
         for s in range(nsp):
             out[s, nz] = term[sa == s].sum(0)
@@ -178,7 +206,8 @@ class Problem:
     def chi_full(self, species):
         from .forward import chi_pairs
         out = chi_pairs(self.k, self.f, self.delta, self.lam, self.s02, self.ideal, species,
-                        self.i, self.j, self.shift, self.sig2, normalise=False)
+                        self.i, self.j, self.shift, self.sig2, shell=self.shell, dr=self.dr,
+                        c3=self.c3, s2types=self.s2types, normalise=False)
         if self.ms is not None:
             C = self.ms.counts(species)
             nb = self.ms.nblk
@@ -188,6 +217,8 @@ class Problem:
 
     def to_R(self, chi):
         """(nsp, nk) normalised chi -> (nchan, nwin) complex on the fit window."""
+        if self.chi_extra is not None:
+            chi = chi + self.chi_extra
         if self.R is None:
             out = [t.to_R(chi[e])[self.mask] for t, e in zip(self.transforms, self.chan_edge)]
         else:
@@ -294,32 +325,50 @@ class PlainSampler:
         self.log.append(rec)
         return acc_all
 
-    def replica_exchange(self):
-        """method_a_rmc.py:332-373 with the counter-based uniforms."""
+    def replica_exchange(self, return_rates=False):
+        """method_a_rmc.py:332-373 with the counter-based uniforms.  With return_rates,
+        also this sweep's 0/1 outcome per neighbour pair, the pairs not tried holding
+        the mean over the tried ones (method_a_rmc.py:366-372)."""
         self.iex += 1
         n = self.nrep
         start, us = exchange_uniforms(self.seed, self.iex, n - 1)
         order = np.arange(n)
         nsw = 0
+        outcome = np.full(n - 1, np.nan)
         for i in range(start, n - 1, 2):
             d = (self.beta[i] - self.beta[i + 1]) * (self.chi2[order[i]] - self.chi2[order[i + 1]]) * self.P.nfit
+            outcome[i] = 0.0
             if d >= 0 or us[i] < np.exp(max(d, -60.0)):
                 order[i], order[i + 1] = order[i + 1], order[i]
                 nsw += 1
+                outcome[i] = 1.0
         if nsw:
             self.species = self.species[order]
             self.chi = self.chi[order]
             self.chi2 = self.chi2[order]
             self.best_chi2 = self.best_chi2[order]
             self.best_sp = self.best_sp[order]
+        if return_rates:
+            tried = ~np.isnan(outcome)
+            fill = outcome[tried].mean() if tried.any() else 0.3
+            return order, np.where(tried, outcome, fill)
         return order
 
-    def run(self, nsteps, exchange_every=25):
+    def run(self, nsteps, exchange_every=25, adapt_every=0):
+        """method_a_rmc.py:401-425: steps, exchange every `exchange_every`, and with
+        `adapt_every` the ladder retuned from each window's exchange rates."""
         t0 = time.perf_counter()
+        window = []
         for it in range(nsteps):
             self.step()
             if exchange_every and (it + 1) % exchange_every == 0:
-                self.replica_exchange()
+                _, rates = self.replica_exchange(return_rates=True)
+                if adapt_every:
+                    window.append(rates)
+            if adapt_every and (it + 1) % adapt_every == 0:
+                if window and self.nrep >= 4:
+                    self.beta = adapt_ladder(self.beta, np.mean(window, axis=0))
+                window = []
         return time.perf_counter() - t0
 
     def collect_rungs(self):

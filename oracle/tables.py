@@ -95,3 +95,48 @@ def ms_entry_table(k, zs, f, lam, entries, reff, s02=0.9, sigma2_nn=0.006, r_nn=
                        * np.sin(2.0 * kk * r + dl[nz]))
             rows.append(row)
     return np.array(rows)
+
+
+ETOK = 0.2624682843   # k^2 [A^-2] per eV (2 m_e / hbar^2), chimera/model.py:94
+
+
+def k_exp_axis(k_phys, dE0):
+    """The experimental k axis for an E0 error dE0 (chimera/model.py:120-130, k_axis
+    "constant"): k_exp = sqrt(max(k^2 - ETOK dE0, 1e-4))."""
+    k = np.asarray(k_phys, float)
+    return np.sqrt(np.maximum(k ** 2 - ETOK * dE0, 1e-4))
+
+
+def feff_columns_to_path(cols, reff):
+    """feffNNNN.dat columns -> (p, f, delta, lambda) on the internal-momentum axis
+    (exafs_gpu/scattering.py:158-178): rows with k > 0.3; amp = mag * red,
+    pha = unwrap(2*phc + phase); f = amp p / k, delta = pha - 2 (p - k) R_eff."""
+    c = np.asarray(cols, float)
+    c = c[c[:, 0] > 0.3]
+    k, phc, mag, pha, red, lam, p = c.T
+    return p, mag * red * p / k, np.unwrap(phc + pha) - 2.0 * (p - k) * reff, lam
+
+
+def read_feff_columns(fn):
+    """(nleg, deg, reff, columns) of a feffNNNN.dat (exafs_gpu/scattering.py:181-190)."""
+    lines = open(fn).read().splitlines()
+    g = [i for i, ln in enumerate(lines) if "nleg, deg, reff" in ln][0]
+    h = [i for i, ln in enumerate(lines) if ln.strip().startswith("k ") and "real[2*phc]" in ln][0]
+    nleg, deg, reff = [float(x) for x in lines[g].split()[:3]]
+    cols = np.array([[float(x) for x in ln.split()] for ln in lines[h + 1:] if ln.strip()])
+    return int(nleg), deg, reff, cols
+
+
+def tabulated_table(k, F, D, L, reff, s02, sigma2_nn, r_nn, c_ms=1.3, amp=1.0, a_scale=1.0):
+    """(nrows, nk) table of one occurrence of every library entry from tabulated F, D, L
+    on the model grid (chimera/mslibrary.py:304-313); 0 at k = 0."""
+    k = np.asarray(k, float)
+    out = np.zeros((len(reff), k.size))
+    nz = k > 0
+    kk = k[nz][None, :]
+    r = (np.asarray(reff, float) * a_scale)[:, None]
+    s2 = c_ms * sigma2_nn * r / (2.0 * r_nn)
+    out[:, nz] = (amp * s02 * F[:, nz] / (kk * r * r)
+                  * np.exp(-2.0 * r / np.maximum(L[:, nz], 1e-6) - 2.0 * kk * kk * s2)
+                  * np.sin(2.0 * kk * r + D[:, nz]))
+    return out

@@ -143,20 +143,20 @@ def test_chi_of_counts_matches_pair_space(tern108):
                                rtol=0, atol=1e-12 * np.abs(ref).max())
 
 
-def _compare_trajectory(P, O, nsteps, exch, seed, tie_tol=0.0, c2_rtol=1e-9):
+def _compare_trajectory(P, O, nsteps, exch, seed, tie_tol=0.0, c2_rtol=1e-9, adapt=0):
     """Device and oracle move by move.  With tie_tol > 0 a differing decision is
     allowed only at a near tie (|x + log u| < tie_tol); the comparison stops there."""
     S = P.sampler
     S.seed = seed
     S.set_recording(log_steps=nsteps)
     S.init_state(P.cfg0, P.cell.ideal)
-    _say(f"[run] device {nsteps} steps nrep={S.nrep} exchange_every={exch}")
+    _say(f"[run] device {nsteps} steps nrep={S.nrep} exchange_every={exch} adapt_every={adapt}")
     S.run(nsteps, exchange_every=exch, log_every=10 ** 9)
     log = S.move_log(nsteps)                                    # (steps, slot, 4)
     R = OR.PlainSampler(O, nrep=S.nrep, seed=seed)
     R.init_state(P.cfg0)
     _say("[run] oracle")
-    R.run(nsteps, exchange_every=exch)
+    R.run(nsteps, exchange_every=exch, adapt_every=adapt)
     for t in range(nsteps):
         by_rung = {int(r[3]): r for r in log[t]}
         for rung in range(S.nrep):
@@ -226,6 +226,25 @@ def test_precision_modes_running_chiR(tern108, prec, acc, rean, tol):
 def test_sampler_trajectory_binary_ss(bin500):
     P, O = bin500
     _compare_trajectory(P, O, 40, 5, seed=11)
+
+
+def test_ladder_adaptation(tern108):
+    """adapt_every (method_a_rmc.py:376-425): the device chain with its ladder retuned
+    from the exchange rates of every window follows the oracle move by move, and both
+    end on the same ladder (the same host arithmetic on the same rates)."""
+    P, O = tern108
+    S = P.sampler
+    beta0 = S.beta
+    S.adapt_every = 20
+    try:
+        S2, R = _compare_trajectory(P, O, 80, 5, seed=13, adapt=20)
+        assert not np.allclose(S.beta, beta0, rtol=1e-6)          # the ladder moved
+        np.testing.assert_allclose(S.beta[[0, -1]], beta0[[0, -1]], rtol=1e-13)
+        np.testing.assert_allclose(S.beta, R.beta, rtol=1e-14)
+    finally:
+        S.adapt_every = 0
+        S._beta_rung = beta0
+        S.init_state(P.cfg0, P.cell.ideal)
 
 
 def test_null_moves_when_candidates_repeat(bin500):

@@ -210,3 +210,77 @@ def test_thermal_sigma2_springs_in_series():
     # a bond direction perpendicular to the springs sees no relative motion
     cell2 = SimpleNamespace(ideal=np.array([[0.0, 0, 0], [0, 2.5, 0], [0, 5.0, 0]]))
     assert np.allclose(thermal_sigma2(Phi, cell2, i[:1], j[:1], np.zeros((1, 3)), T), 0.0)
+
+
+def test_force_constant_matrix_springs():
+    """Nearest-neighbour springs (exafs_gpu/lattice.py:97-111): symmetric, a rigid
+    translation costs nothing, and one bond's block is k_bond e e^T + k_angle (1 - e e^T);
+    the thermal sigma^2 from it is one value per shell (cubic symmetry) and grows with
+    distance."""
+    from chimera_amd.forward import thermal_sigma2
+    from chimera_amd.spectrum import force_constant_matrix
+    c = CrystalSupercell("fcc", (3, 3, 3), 3.59)
+    i, j, sh = c.neighbour_list(5.2)
+    Phi = force_constant_matrix(c, i, j, sh, k_bond=2.6, k_angle=0.35)
+    np.testing.assert_allclose(Phi, Phi.T, atol=1e-13)
+    for ax in range(3):
+        t = np.zeros((c.natoms, 3))
+        t[:, ax] = 1.0
+        assert np.abs(Phi @ t.reshape(-1)).max() < 1e-12
+    d = np.linalg.norm(c.ideal[j] + sh - c.ideal[i], axis=1)
+    for p in np.flatnonzero(i == 0)[:20]:
+        e = (c.ideal[j[p]] + sh[p] - c.ideal[0]) / d[p]
+        want = (-(2.6 * np.outer(e, e) + 0.35 * (np.eye(3) - np.outer(e, e)))
+                if d[p] < 2.6 else np.zeros((3, 3)))
+        np.testing.assert_allclose(Phi[0:3, 3 * j[p]:3 * j[p] + 3], want, atol=1e-12)
+    np.testing.assert_allclose(Phi[0:3, 0:3], (2.6 + 2 * 0.35) * 4 * np.eye(3), atol=1e-12)
+    shell, radii = c.shells(i, j, sh)
+    s2 = thermal_sigma2(Phi, c, i, j, sh, 300.0)
+    for s in range(radii.size):
+        v = s2[shell == s]
+        assert np.ptp(v) <= 1e-10 * v.mean()
+
+
+def test_k_exp_axis_and_expected_counts_match_oracle():
+    from chimera_amd.spectrum import k_exp_axis
+    k = k_grid()
+    for dE0 in (0.0, 2.0, -3.5):
+        np.testing.assert_array_equal(k_exp_axis(k, dE0), OTab.k_exp_axis(k, dE0))
+    c = CrystalSupercell("fcc", (3, 3, 3), 3.59)
+    lib = MSLibrary(c, 3, 5.2)
+    cl = OM.enumerate_classes("fcc", 3.59, 5.2)
+    for comp in ([1, 1, 1], [0.2, 0.3, 0.5]):
+        np.testing.assert_allclose(lib.expected_counts(comp), OM.expected_counts(cl, 3, comp),
+                                   rtol=1e-14)
+
+
+def test_path_library_reader_matches_oracle(tmp_path):
+    """PathLibrary (chimera/mslibrary.py:261-313) on a FEFF-format library: the same
+    classes, entries and stencil as the enumerated library, and F / delta / lambda on the
+    model grid equal to the oracle's own reading of the files."""
+    from chimera_amd.mspaths import PathLibrary
+    from tests.feffio import write_library
+    from tests.test_oracle import _fdl
+    c = CrystalSupercell("fcc", (3, 3, 3), 3.59)
+    ref = MSLibrary(c, 2, 5.2)
+    root = write_library(str(tmp_path / "lib"), ("Fe", "Ni"), ref.classes, c.unit,
+                         c.unit * np.sqrt(2), _fdl([26, 28]))
+    k = k_grid()
+    lib = PathLibrary(root).prepare(k)
+    ms = MSLibrary(c, 2, 5.2, library=lib)
+    assert ms.nblk == ref.nblk == lib.nblk
+    assert np.array_equal(ms.stencil, ref.stencil)
+    rows = 0
+    for a, el in enumerate(("Fe", "Ni")):
+        for n in range(lib.nblk):
+            fn = os.path.join(root, el, f"feff{n + 1:04d}.dat")
+            nleg, deg, reff, cols = OTab.read_feff_columns(fn)
+            p, f, d, lam = OTab.feff_columns_to_path(cols, reff)
+            np.testing.assert_allclose(lib.F[rows], np.interp(k, p, f), rtol=1e-15, atol=0)
+            np.testing.assert_allclose(lib.D[rows], np.interp(k, p, d), rtol=1e-15, atol=0)
+            np.testing.assert_allclose(lib.L[rows], np.interp(k, p, lam), rtol=1e-15, atol=0)
+            assert lib.reff[rows] == reff
+            rows += 1
+    with pytest.raises(AssertionError):
+        os.remove(os.path.join(root, "Ni", "feff0001.dat"))
+        PathLibrary(root)

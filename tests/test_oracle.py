@@ -256,3 +256,210 @@ def test_sampler_runs_and_improves():
     for c in range(4):
         np.testing.assert_allclose(S.chi[c], P.chi_full(S.species[c]), rtol=0,
                                    atol=1e-11 * np.abs(S.chi[c]).max())
+
+
+def test_adapt_ladder_closed_forms():
+    """Ladder retuning (method_a_rmc.py:389-398) pinned by cases with known answers."""
+    beta = np.exp(-np.arange(3.0))                      # log gaps -1, -1
+    # rates 0.8 / 0.2: sqrt(0.8) : sqrt(0.2) = 2 : 1, span 2 kept -> gaps -4/3, -2/3
+    nb = OR.adapt_ladder(beta, [0.8, 0.2])
+    np.testing.assert_allclose(np.log(nb), [0.0, -4.0 / 3.0, -2.0], rtol=0, atol=1e-14)
+    # equal rates leave a ladder unchanged, whatever its spacing
+    b = 1.0 / np.geomspace(0.02, 5.0, 8)
+    np.testing.assert_allclose(OR.adapt_ladder(b, np.full(7, 0.37)), b, rtol=1e-13)
+    b2 = np.sort(np.random.default_rng(0).uniform(0.1, 40.0, 6))[::-1]
+    np.testing.assert_allclose(OR.adapt_ladder(b2, np.full(5, 0.6)), b2, rtol=1e-13)
+    # endpoints fixed, order kept, and rates beyond [0.02, 0.98] act as the clip values
+    r = np.array([0.0, 0.5, 1.0, 0.3, 0.9, 0.05, 0.7])
+    nb = OR.adapt_ladder(b, r)
+    assert abs(nb[0] / b[0] - 1) < 1e-14 and abs(nb[-1] / b[-1] - 1) < 1e-13
+    assert np.all(np.diff(nb) < 0)
+    np.testing.assert_array_equal(nb, OR.adapt_ladder(b, np.clip(r, 0.02, 0.98)))
+    # a pair that exchanges more freely than average moves apart (its log gap grows)
+    g0, g1 = np.diff(np.log(b)), np.diff(np.log(nb))
+    assert abs(g1[2]) > abs(g0[2]) and abs(g1[0]) < abs(g0[0])
+
+
+def test_exchange_rates_filled_and_adaptive_run():
+    """Per-pair sweep outcomes: tried pairs 0/1, untried ones the mean of the tried
+    (method_a_rmc.py:366-372); an adaptive run retunes the ladder inside its span."""
+    P = _toy_problem()
+    S = OR.PlainSampler(P, nrep=5, seed=3)
+    S.init_state(OL.random_config(108, [1, 1], 5))
+    _, rates = S.replica_exchange(return_rates=True)
+    start, _ = OR.exchange_uniforms(S.seed, S.iex, 4)
+    tried = np.zeros(4, bool)
+    tried[start::2] = True
+    assert set(np.unique(rates[tried])) <= {0.0, 1.0}
+    np.testing.assert_allclose(rates[~tried], rates[tried].mean())
+    b0 = S.beta.copy()
+    S.run(40, exchange_every=5, adapt_every=20)
+    assert not np.allclose(S.beta, b0) and abs(S.beta[0] / b0[0] - 1) < 1e-14
+    assert abs(S.beta[-1] / b0[-1] - 1) < 1e-13 and np.all(np.diff(S.beta) < 0)
+
+
+def test_ms_expected_counts_random_alloy(fcc108):
+    """Random-alloy entry occurrences (chimera/mslibrary.py:315-321): exact for a pure
+    crystal (every path of the class, all-species-s entry), and the mean of the exact
+    counts over i.i.d. random occupations for a mixture."""
+    cell, i, j, sh, shell, radii = fcc108
+    cl = OM.enumerate_classes("fcc", 3.59, 5.2)
+    sc = OM.SupercellPaths(cl, 2, cell.ideal, 3.59, (3, 3, 3))
+    nb = sc.nblk
+    for s in range(2):
+        comp = np.eye(2)[s]
+        E = OM.expected_counts(cl, 2, comp)
+        C = sc.counts(np.full(108, s))
+        np.testing.assert_allclose(E[s * nb:(s + 1) * nb], C[s * nb:(s + 1) * nb] / 108.0, rtol=0,
+                                   atol=1e-12)
+    comp = np.array([0.3, 0.7])
+    E = OM.expected_counts(cl, 2, comp)
+    rng = np.random.default_rng(4)
+    acc = np.zeros(2 * nb)
+    nab = np.zeros(2)
+    for _ in range(300):
+        sp = (rng.random(108) < comp[1]).astype(np.int64)
+        acc += sc.counts(sp)
+        nab += np.bincount(sp, minlength=2)
+    mean = np.concatenate([acc[a * nb:(a + 1) * nb] / nab[a] for a in range(2)])
+    big = E > 0.5
+    np.testing.assert_allclose(mean[big], E[big], rtol=0.08)
+    np.testing.assert_allclose(mean.sum(), E.sum(), rtol=1e-12)   # every path counted once
+
+
+def test_k_exp_axis_closed_form():
+    """chimera/model.py:120-130: k_exp^2 = k^2 - ETOK dE0 above the 1e-4 floor."""
+    k = np.linspace(0.0, 16.0, 321)
+    np.testing.assert_allclose(OTab.k_exp_axis(k, 0.0)[k >= 0.01], k[k >= 0.01], rtol=1e-15)
+    for dE0 in (-3.0, 2.5):
+        ke = OTab.k_exp_axis(k, dE0)
+        ok = k * k - OTab.ETOK * dE0 > 1e-4
+        np.testing.assert_allclose(ke[ok] ** 2 + OTab.ETOK * dE0, k[ok] ** 2, rtol=1e-12)
+        np.testing.assert_allclose(ke[~ok], 0.01)
+        assert np.all(np.diff(ke) >= 0)
+    # 10 eV is k^2 = 2.6247 A^-2 (2 m_e 10 eV / hbar^2)
+    assert abs(OTab.k_exp_axis(np.array([0.0]), -10.0)[0] ** 2 - 2.624682843) < 1e-9
+
+
+def test_transform_resample_nonuniform_axis_is_linear_interp():
+    """A transform against a non-uniform model axis (an E0-shifted k_exp) resamples by
+    linear interpolation: equal to np.interp inside the axis, zero outside."""
+    k = np.linspace(0.0, 16.0, 321)
+    ke = OTab.k_exp_axis(k, 3.0)
+    t = OT.Transform(ke, 3.0, 14.0, 2, 1.0)
+    chi = np.sin(1.7 * k) * np.exp(-0.1 * k)
+    got = t.resample(chi)
+    inside = (t.k_uniform >= ke[0]) & (t.k_uniform <= ke[-1])
+    np.testing.assert_allclose(got[inside], np.interp(t.k_uniform[inside], ke, chi), rtol=0,
+                               atol=1e-14)
+    assert np.all(got[~inside] == 0.0)
+
+
+def _small_displaced(seed=0, nsp=2):
+    cell = OL.Supercell("fcc", (2, 2, 2), 3.59)
+    i, j, sh = cell.neighbour_list(3.0)                   # first shell: 12 per site
+    zs = [24, 26, 28][:nsp]
+    k, f, d, lam, s02 = OTab.synthetic_ss(OTab.k_grid(), zs)
+    rng = np.random.default_rng(seed)
+    pos = cell.ideal + rng.normal(0, 0.05, cell.ideal.shape)
+    sig2 = 0.004 + 0.002 * rng.random(i.size)
+    return cell, i, j, sh, k, f, d, lam, s02, pos, sig2, rng
+
+
+def test_grad_pos_finite_differences():
+    """grad_pos (kernels.py:166-237) against central differences of chi_pairs, with
+    cumulant tables set."""
+    cell, i, j, sh, k, f, d, lam, s02, pos, sig2, rng = _small_displaced(1, 3)
+    sp = rng.integers(0, 3, cell.natoms)
+    shell = np.zeros(i.size, int)
+    dr = rng.normal(0, 0.01, (1, 3, 3))
+    c3 = np.array([3e-4])
+    s2t = np.abs(rng.normal(0, 5e-4, (1, 3, 3)))
+    g = rng.normal(0, 1, (3, k.size))
+    kw = dict(shell=shell, dr=dr, c3=c3, s2types=s2t)
+    G = OF.grad_pos(k, f, d, lam, s02, pos, sp, i, j, sh, sig2, g, **kw)
+    h = 1e-6
+    for atom, ax in [(0, 0), (5, 2), (17, 1), (31, 0)]:
+        pp, pm = pos.copy(), pos.copy()
+        pp[atom, ax] += h
+        pm[atom, ax] -= h
+        Lp = (g * OF.chi_pairs(k, f, d, lam, s02, pp, sp, i, j, sh, sig2, normalise=False, **kw)).sum()
+        Lm = (g * OF.chi_pairs(k, f, d, lam, s02, pm, sp, i, j, sh, sig2, normalise=False, **kw)).sum()
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(G[atom, ax] - fd) <= 1e-6 * max(1.0, abs(fd)), (atom, ax, G[atom, ax], fd)
+    np.testing.assert_allclose(G.sum(0), 0.0, atol=1e-9 * np.abs(G).max())   # Newton's third law
+
+
+def test_chi_soft_one_hot_and_grad_soft_finite_differences():
+    """chi_soft (kernels.py:514-589) equals chi_pairs for one-hot weights; grad_soft
+    (kernels.py:592-660 + the normalisation term of forward.py:252-258) against central
+    differences in positions and weights."""
+    cell, i, j, sh, k, f, d, lam, s02, pos, sig2, rng = _small_displaced(2, 2)
+    sp = rng.integers(0, 2, cell.natoms)
+    w1 = np.eye(2)[sp]
+    np.testing.assert_allclose(OF.chi_soft(k, f, d, lam, s02, pos, w1, i, j, sh, sig2),
+                               OF.chi_pairs(k, f, d, lam, s02, pos, sp, i, j, sh, sig2),
+                               rtol=0, atol=1e-12)
+    w = rng.dirichlet([1.0, 1.0], cell.natoms)
+    g = rng.normal(0, 1, (2, k.size))
+
+    def L(p_, w_):
+        return (g * OF.chi_soft(k, f, d, lam, s02, p_, w_, i, j, sh, sig2)).sum()
+
+    Gp, Gw = OF.grad_soft(k, f, d, lam, s02, pos, w, i, j, sh, sig2, g)
+    h = 1e-6
+    for atom, ax in [(0, 1), (9, 0), (30, 2)]:
+        pp, pm = pos.copy(), pos.copy()
+        pp[atom, ax] += h
+        pm[atom, ax] -= h
+        fd = (L(pp, w) - L(pm, w)) / (2 * h)
+        assert abs(Gp[atom, ax] - fd) <= 1e-6 * max(1.0, abs(fd))
+    for atom, s in [(0, 0), (3, 1), (22, 0)]:
+        wp, wm = w.copy(), w.copy()
+        wp[atom, s] += h
+        wm[atom, s] -= h
+        fd = (L(pos, wp) - L(pos, wm)) / (2 * h)
+        assert abs(Gw[atom, s] - fd) <= 1e-6 * max(1.0, abs(fd)), (atom, s, Gw[atom, s], fd)
+
+
+def _fdl(zs):
+    def fdl(a, seq, p):
+        F = np.ones_like(p)
+        for s in seq:
+            F = F * (zs[s] / 26.0) * np.exp(-(p - 4.0) ** 2 / 18.0)
+        delta = -0.9 * p + 1.2 + 0.05 * np.mean([zs[s] for s in seq]) + 0.1 * a
+        return F, delta, 6.0 + 0.3 * p
+    return fdl
+
+
+def test_feff_convention_round_trip(tmp_path):
+    """FEFF columns written from chosen f(p), delta(p), lambda(p) through the documented
+    convention (k from the Fermi level, amp = mag*red, pha = 2phc + phase) read back to the
+    same functions on the p axis (exafs_gpu/scattering.py:158-190)."""
+    from tests.feffio import write_library
+    cl = OM.enumerate_classes("fcc", 3.59, 4.0)
+    zs = [24, 28]
+    root = write_library(str(tmp_path / "lib"), ("Cr", "Ni"), cl, 3.59 / 2, 3.59 / np.sqrt(2),
+                         _fdl(zs), p0=0.6)
+    fn = sorted((tmp_path / "lib" / "Ni").iterdir())[3]
+    nleg, deg, reff, cols = OTab.read_feff_columns(str(fn))
+    p, f, delta, lam = OTab.feff_columns_to_path(cols, reff)
+    np.testing.assert_allclose(p ** 2 - cols[cols[:, 0] > 0.3, 0] ** 2, 0.36, rtol=1e-9)
+    # entry 3 of the Ni (index 1) block: first class, slot species from the entry code
+    g, sp = OM.entry_list(cl, 2)[3]
+    seq = [1 if t < 0 else sp[t] for t in cl[g].slot_of]
+    F, D, L = _fdl(zs)(1, seq, p)
+    np.testing.assert_allclose(f, F, rtol=1e-9)
+    np.testing.assert_allclose(delta, D, rtol=0, atol=1e-8)   # 13 digits in the file
+    np.testing.assert_allclose(lam, L, rtol=1e-11)
+    assert nleg == len(seq) + 1 and abs(reff - cl[g].reff_units() * 3.59 / 2) < 1e-7
+    # the tabulated table at the synthetic functions equals the synthetic closed form
+    k = OTab.k_grid()
+    Fk, Dk, Lk = (np.interp(k, p, v)[None] for v in (f, delta, lam))
+    T = OTab.tabulated_table(k, Fk, Dk, Lk, [reff], 0.9, 0.006, 2.54)
+    nz = (k > 0) & (k >= p[0])
+    kk = k[nz]
+    r = reff
+    want = (0.9 * Fk[0, nz] / (kk * r * r) * np.exp(-2 * r / Lk[0, nz] - 2 * kk * kk * 1.3 * 0.006 * r / (2 * 2.54))
+            * np.sin(2 * kk * r + Dk[0, nz]))
+    np.testing.assert_allclose(T[0, nz], want, rtol=1e-12, atol=1e-15)
