@@ -145,15 +145,18 @@ def forward_bench(P, S, iters=10):
     sp = S.species
     nconf = sp.shape[0]
     pos = np.repeat(P.cell.ideal[None], nconf, 0)
-    fm.chi(pos, sp)
-    fm.chi(pos, sp, precision="f32")
+    c64 = fm.chi(pos, sp, normalise=False)
+    c32 = fm.chi(pos, sp, normalise=False, precision="f32")
     g = np.ones((nconf, P.nsp, P.k.size))
     fm.grad_pos(pos, sp, g)
     w = np.eye(P.nsp)[sp]
     fm.chi_soft(pos, w)
     terms = float(nconf) * fm.npairs * int(np.count_nonzero(P.k > 0))
+    err = np.abs(c32 - c64)
     out = {"configs": nconf, "pairs": int(fm.npairs), "k_points": int(P.k.size),
-           "terms_per_launch": terms}
+           "terms_per_launch": terms,
+           "f32_vs_f64": {"max_abs": float(f"{err.max():.3g}"),
+                          "max_rel": float(f"{err.max() / np.abs(c64).max():.3g}")}}
     for key, what in (("path_sum_f64", "chi"), ("path_sum_f32", "chi_f32"),
                       ("adjoint_dr_f64", "adjoint"), ("soft_path_sum_f64", "soft")):
         ms = fm.profile(what, nconf, iters)
@@ -261,6 +264,8 @@ def main():
                     help="only run the precision study (STEPS steps per mode) and exit")
     ap.add_argument("--no-forward", action="store_true",
                     help="skip the full chi(k) path-sum / adjoint timings")
+    ap.add_argument("--forward-only", action="store_true",
+                    help="only time the forward-model kernels (path sums, adjoint) and exit")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-uniform proposal comparison")
     ap.add_argument("--ess-steps", type=int, default=100000)
@@ -278,6 +283,10 @@ def main():
                     help="only run a plain device copy of MIB MiB three times (the known-size "
                          "kernel that scales hardware byte counters) and exit")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (replicas shard across GPUs by "
+              "design, but no multi-GPU path is built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 4
@@ -295,10 +304,11 @@ def main():
         from chimera_amd import precision_study
         precision_study.main(["--steps", str(args.precision_study)])
         return
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (replicas shard across GPUs by "
-              "design, but no multi-GPU path is built); use --gpus 1", file=sys.stderr)
-        sys.exit(2)
+    if args.forward_only:
+        from chimera_amd import workloads as W
+        P = W.build(args.workload)
+        print(json.dumps({"workload": args.workload, "forward_model": forward_bench(P, P.sampler)}))
+        return
 
     from chimera_amd import workloads as W
     print(f"[bench] building {args.workload}", file=sys.stderr, flush=True)

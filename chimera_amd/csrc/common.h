@@ -78,6 +78,7 @@ struct chm_ctx {
     int nsp = 0, nk = 0;
     double s02 = 1.0;
     chm::DevBuf k, f, delta, lam;
+    chm::DevBuf f32, drev32;   // FP32 f and delta / 2 pi for the FP32 path sum
     std::vector<double> h_k;
     // topology
     int natoms = 0, npairs = 0, nshell = 1;

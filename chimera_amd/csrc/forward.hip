@@ -87,6 +87,15 @@ extern "C" int chm_fm_set_tables(chm_ctx *ctx, int nsp, int nk, const double *k,
     if ((rc = upload(ctx, ctx->f, f, sizeof(double) * nsp * nk))) return rc;
     if ((rc = upload(ctx, ctx->delta, delta, sizeof(double) * nsp * nsp * nk))) return rc;
     if ((rc = upload(ctx, ctx->lam, lam, sizeof(double) * nk))) return rc;
+    // FP32 copies for the FP32 path sum: f, and delta in revolutions (delta / 2 pi)
+    {
+        std::vector<float> f32((size_t)nsp * nk), d32((size_t)nsp * nsp * nk);
+        for (size_t q = 0; q < f32.size(); ++q) f32[q] = (float)f[q];
+        for (size_t q = 0; q < d32.size(); ++q) d32[q] = (float)(delta[q] / 6.28318530717958647692);
+        if ((rc = upload(ctx, ctx->f32, f32.data(), sizeof(float) * f32.size()))) return rc;
+        if ((rc = upload(ctx, ctx->drev32, d32.data(), sizeof(float) * d32.size()))) return rc;
+        CHM_HIP(hipStreamSynchronize(ctx->stream));   // the host vectors go out of scope
+    }
     CHM_HIP(hipStreamSynchronize(ctx->stream));
     return CHM_OK;
 }
@@ -409,6 +418,142 @@ __global__ __launch_bounds__(256) void k_chi_pairs(
     }
 }
 
+// FP32 path sum.  Same layout as k_chi_pairs (lanes own k, pair tiles in LDS), with
+// every per-term operation in FP32 on the hardware units: the phase in revolutions,
+// sin on v_sin_f32 (sin(2 pi x)), the exponential on v_exp_f32 (2^x).  The phase
+// k r / pi is formed exactly enough without FP64: k/pi and r are split into an 11-bit
+// head and a tail, head x head is exact in FP32 and its integer part is dropped exactly,
+// the three small cross products carry the rest (error ~1e-7 revolution at k r ~ 90).
+// Terms are summed in FP32 over one tile of 256 pairs and the tile sums in FP64.
+struct PairRecF {
+    float rhi, rlo;   // r_eff = rhi + rlo, rhi with an 11-bit significand
+    float r;          // r_eff
+    float l2ir2;      // log2(1 / r_eff^2), folded into the exponential
+    float ea;         // -2 log2(e) sigma^2 (times k^2 gives the Debye-Waller exponent)
+    float c3;         // c3tab[shell]
+    int t, pad;
+};
+
+__device__ __forceinline__ float head11(float x) {
+    return __int_as_float(__float_as_int(x) & (int)0xFFFFE000);
+}
+
+// One lane per k point with k > 0 (the k = 0 term is 0): the workgroup has
+// roundup(nkz, 64) lanes (320 for the 0..16 A^-1 grid at 0.05: five full wavefronts),
+// lane l owns k index k0 + l, and a tile stages blockDim pairs.
+template <int NSP>
+__global__ __launch_bounds__(512) void k_chi_pairs_f32(
+    const double *__restrict__ r_all, const int *__restrict__ pt_all,
+    const int *__restrict__ shell, const double *__restrict__ sig2,
+    const float *__restrict__ ftab32, const float *__restrict__ drev32,
+    const double *__restrict__ lamtab, const double *__restrict__ ktab,
+    const double *__restrict__ drtab, const double *__restrict__ c3tab,
+    const double *__restrict__ s2tab, double *__restrict__ partial, int npairs, int nk, int k0,
+    int pairs_per_wg, double s02) {
+    extern __shared__ double smem[];
+    float *sf = reinterpret_cast<float *>(smem);            // f (NSP, nk)
+    float *sd = sf + NSP * nk;                              // delta / 2 pi (NSP*NSP, nk)
+    const int words = (NSP + NSP * NSP) * nk;
+    PairRecF *tile = reinterpret_cast<PairRecF *>(sf + ((words + 7) & ~7));
+    const int ic = blockIdx.y;
+    const int w = blockIdx.x;
+    const int nwg = gridDim.x;
+    const int nthr = blockDim.x;
+    for (int q = threadIdx.x; q < NSP * nk; q += nthr) sf[q] = ftab32[q];
+    for (int q = threadIdx.x; q < NSP * NSP * nk; q += nthr) sd[q] = drev32[q];
+
+    constexpr double kLog2e = 1.44269504088896340736;
+    const int ik = k0 + (int)threadIdx.x;
+    const bool live = ik < nk;
+    const int ikc = live ? ik : nk - 1;                     // a valid column for dead lanes
+    const double kv = live ? ktab[ik] : 0.0;
+    const double lv = live ? lamtab[ik] : 1.0;
+    const double kp = kv * kInvPi;                          // revolutions of 2 k r per unit r
+    const float khi = head11((float)kp);
+    const float klo = (float)(kp - (double)khi);
+    const float c1 = (float)(-2.0 * kLog2e / lv);
+    const float k2 = (float)(kv * kv);
+    const float c3k = (float)((4.0 / 3.0) * kv * kv * kv / kTwoPi);
+    const double pref = (live && kv > 0.0) ? s02 / kv : 0.0;  // applied to the FP64 sums
+    double acc[NSP];
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) acc[s] = 0.0;
+
+    const double *rc = r_all + (long)ic * npairs;
+    const int *tc = pt_all + (long)ic * npairs;
+    const int pbeg = w * pairs_per_wg;
+    const int pend = min(npairs, pbeg + pairs_per_wg);
+
+    for (int p0 = pbeg; p0 < pend; p0 += nthr) {
+        __syncthreads();
+        {
+            const int p = p0 + (int)threadIdx.x;
+            PairRecF rec;
+            if (p < pend) {
+                const int t = tc[p];
+                const int sh = shell[p];
+                const double re = rc[p] + drtab[sh * NSP * NSP + t];
+                rec.r = (float)re;
+                rec.rhi = head11(rec.r);
+                rec.rlo = (float)(re - (double)rec.rhi);
+                rec.l2ir2 = (float)(-2.0 * log2(re));
+                rec.ea = (float)(-2.0 * kLog2e * (sig2[p] + s2tab[sh * NSP * NSP + t]));
+                rec.c3 = (float)c3tab[sh];
+                rec.t = t;
+            } else {
+                rec.r = rec.rhi = 1.f;
+                rec.rlo = 0.f;
+                rec.l2ir2 = -1000.f;   // 2^-1000 = 0: contributes exactly 0
+                rec.ea = 0.f;
+                rec.c3 = 0.f;
+                rec.t = 0;
+            }
+            rec.pad = 0;
+            tile[threadIdx.x] = rec;
+        }
+        __syncthreads();
+        const int nt = __builtin_amdgcn_readfirstlane(min(nthr, pend - p0));
+        // pairs are absorber-sorted in runs (every pair of atom i has i's species), so
+        // terms go into one running sum, moved to the species' sum when the absorber
+        // species changes (a uniform branch).  The pair type is uniform: its table
+        // offsets are scalar arithmetic.
+        float run = 0.f;
+        int run_sa = __builtin_amdgcn_readfirstlane(tile[0].t) / NSP;
+        for (int q = 0; q < nt; ++q) {
+            const PairRecF rec = tile[q];
+            const int t = __builtin_amdgcn_readfirstlane(rec.t);
+            const int sa = t / NSP;
+            if (sa != run_sa) {
+#pragma unroll
+                for (int s = 0; s < NSP; ++s)
+                    if (run_sa == s) acc[s] += (double)run;
+                run = 0.f;
+                run_sa = sa;
+            }
+            const int sb = t - sa * NSP;
+            const float x = khi * rec.rhi;                      // exact
+            float ph = x - rintf(x);
+            ph += fmaf(khi, rec.rlo, klo * rec.r);
+            ph += sd[t * nk + ikc] - c3k * rec.c3;
+            const float sn = __builtin_amdgcn_sinf(ph);
+            const float e = __builtin_amdgcn_exp2f(fmaf(c1, rec.r, fmaf(k2, rec.ea, rec.l2ir2)));
+            run = fmaf(sf[sb * nk + ikc] * e, sn, run);
+        }
+#pragma unroll
+        for (int s = 0; s < NSP; ++s)
+            if (run_sa == s) acc[s] += (double)run;
+    }
+    double *out = partial + ((long)ic * nwg + w) * NSP * nk;
+    // columns below k0 (k <= 0) are zero; lane l also writes column l < k0
+    for (int q = threadIdx.x; q < k0; q += nthr)
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) out[s * nk + q] = 0.0;
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < NSP; ++s) out[s * nk + ik] = pref * acc[s];
+    }
+}
+
 // chi[ic][s][k] = sum over workgroups w (fixed order) of partial[ic][w][s][k]
 __global__ void k_reduce_partials(const double *__restrict__ partial, double *__restrict__ chi,
                                   int nwg, int n_per_conf, int nconf) {
@@ -437,14 +582,29 @@ int launch_chi(chm_ctx *ctx, int nconf, int precision) {
     if (rc) return rc;
     dim3 grid(nwg, nconf);
     if (precision == CHM_PREC_F32) {
-        hipFuncSetAttribute((const void *)k_chi_pairs<NSP, true>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        k_chi_pairs<NSP, true><<<grid, 256, lds, ctx->stream>>>(
+        // lanes over the k > 0 points only (the grid is ascending)
+        int k0 = 0;
+        while (k0 < nk && !(ctx->h_k[k0] > 0.0)) ++k0;
+        const int nkz = std::max(1, nk - k0);
+        const int nthr = std::min(512, (nkz + 63) / 64 * 64);
+        CHM_REQUIRE(nkz <= 512, CHM_E_ARG, "at most 512 k points above 0");
+        int tiles = (npairs + nthr - 1) / nthr;
+        int nwg32 = std::max(1, std::min(tiles, (2048 + nconf - 1) / nconf));
+        const int ppw32 = ((tiles + nwg32 - 1) / nwg32) * nthr;
+        nwg32 = (npairs + ppw32 - 1) / ppw32;
+        if ((rc = ctx->partial.ensure(sizeof(double) * (size_t)nconf * nwg32 * NSP * nk))) return rc;
+        const size_t words = (size_t)(NSP + NSP * NSP) * nk;
+        const size_t lds32 = sizeof(float) * ((words + 7) & ~(size_t)7) + sizeof(PairRecF) * nthr;
+        CHM_REQUIRE(lds32 <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+        hipFuncSetAttribute((const void *)k_chi_pairs_f32<NSP>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32);
+        k_chi_pairs_f32<NSP><<<dim3(nwg32, nconf), nthr, lds32, ctx->stream>>>(
             ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(),
-            ctx->sig2.as<double>(), ctx->f.as<double>(), ctx->delta.as<double>(),
+            ctx->sig2.as<double>(), ctx->f32.as<float>(), ctx->drev32.as<float>(),
             ctx->lam.as<double>(), ctx->k.as<double>(), ctx->drtab.as<double>(),
             ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), ctx->partial.as<double>(), npairs,
-            nk, ppw, ctx->s02);
+            nk, k0, ppw32, ctx->s02);
+        nwg = nwg32;
     } else {
         hipFuncSetAttribute((const void *)k_chi_pairs<NSP, false>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
