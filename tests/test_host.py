@@ -284,3 +284,21 @@ def test_path_library_reader_matches_oracle(tmp_path):
     with pytest.raises(AssertionError):
         os.remove(os.path.join(root, "Ni", "feff0001.dat"))
         PathLibrary(root)
+
+
+def test_shell_model_directions():
+    """dN_direction keeps every absorber's coordination and the pair symmetry; dr_from_free
+    gives symmetric offsets of zero composition-weighted mean (chimera/shellmodel.py:45-104)."""
+    from chimera_amd.shellmodel import dN_direction, dr_from_free, sro_directions
+    for nsp in (2, 3, 4):
+        assert len(sro_directions(nsp)) == nsp * (nsp - 1) // 2
+        for a, b in sro_directions(nsp):
+            d = dN_direction(nsp, 4, 2, a, b)
+            np.testing.assert_array_equal(d.sum(axis=(1, 2)), 0)      # coordination kept
+            np.testing.assert_array_equal(d, d.transpose(1, 0, 2))
+        comp = np.arange(1, nsp + 1, dtype=float)
+        comp /= comp.sum()
+        nfree = nsp * (nsp + 1) // 2 - 1
+        dr = dr_from_free(np.random.default_rng(nsp).normal(0, 0.01, nfree), comp)
+        np.testing.assert_allclose(dr, dr.T)
+        assert abs(comp @ dr @ comp) < 1e-16

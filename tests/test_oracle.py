@@ -463,3 +463,13 @@ def test_feff_convention_round_trip(tmp_path):
     want = (0.9 * Fk[0, nz] / (kk * r * r) * np.exp(-2 * r / Lk[0, nz] - 2 * kk * kk * 1.3 * 0.006 * r / (2 * 2.54))
             * np.sin(2 * kk * r + Dk[0, nz]))
     np.testing.assert_allclose(T[0, nz], want, rtol=1e-12, atol=1e-15)
+
+
+def test_posterior_cov_qr_equals_normal_equations():
+    """oracle.shellmodel.posterior_cov (QR of [J; sqrt(P)]) equals (J^T J + P)^-1."""
+    from oracle import shellmodel as OS
+    rng = np.random.default_rng(5)
+    J = rng.normal(size=(200, 6))
+    P = rng.uniform(0.1, 2.0, 6)
+    np.testing.assert_allclose(OS.posterior_cov(J, P), np.linalg.inv(J.T @ J + np.diag(P)),
+                               rtol=1e-10, atol=1e-14)
