@@ -110,6 +110,25 @@ def warren_cowley(species, i_idx, j_idx, shell, s, nsp):
     return alpha_from_counts(N, np.bincount(species, minlength=nsp))
 
 
+def warren_cowley_radius(species, i_idx, j_idx, shift, cell, shell_r, tol=0.15, comp=None,
+                         nsp=None):
+    """alpha_AB = 1 - P(B|A)/c_B over the pairs whose ideal-lattice distance lies within
+    `tol` of `shell_r` (exafs_gpu/lattice.py:167-185: the shell chosen by radius)."""
+    species = np.asarray(species)
+    n = int(nsp) if nsp is not None else int(species.max()) + 1
+    i_idx, j_idx = np.asarray(i_idx), np.asarray(j_idx)
+    d = np.linalg.norm(cell.ideal[j_idx] + np.asarray(shift, float) - cell.ideal[i_idx], axis=1)
+    m = np.abs(d - float(shell_r)) < tol
+    N = np.zeros((n, n))
+    np.add.at(N, (species[i_idx[m]], species[j_idx[m]]), 1.0)
+    if comp is None:
+        return alpha_from_counts(N, np.bincount(species, minlength=n))
+    c = np.asarray(comp, float)
+    tot = N.sum(1, keepdims=True)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(tot > 0, 1.0 - (N / np.maximum(tot, 1)) / c[None, :], 0.0)
+
+
 def alpha_from_counts(N, natoms_of):
     """Warren-Cowley matrix from ordered pair counts N[a, b] of one shell."""
     N = np.asarray(N, float)

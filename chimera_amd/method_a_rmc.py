@@ -531,15 +531,23 @@ class RMCSampler:
         for a, b in zip(sp, c2):
             self.samples.append((a.astype(np.int32), float(b)))
 
-    def posterior_wc(self, shell=0):
-        """Warren-Cowley alpha (upper triangle) of shell `shell` over the collected
-        samples: (mean, std, n) (method_a_rmc.py:477-491)."""
-        from .lattice import warren_cowley
+    def posterior_wc(self, shell_r, i_idx=None, j_idx=None, shift=None, cell=None, nsp=None,
+                     tol=0.15):
+        """Warren-Cowley alpha (upper triangle) over the collected samples, for the pairs
+        within `tol` of the radius `shell_r`: (mean, std, n) (method_a_rmc.py:477-491).
+        The pair lists and cell default to the forward model's."""
+        from .lattice import warren_cowley_radius
         if not self.samples:
             raise RuntimeError("no samples collected; set collect_after in the constructor")
-        n = self.nsp
-        arr = np.array([warren_cowley(sp, self.fm.i_idx, self.fm.j_idx, self.fm.shell, shell,
-                                      n)[np.triu_indices(n)] for sp, _ in self.samples])
+        fm = self.fm
+        i_idx = fm.i_idx if i_idx is None else i_idx
+        j_idx = fm.j_idx if j_idx is None else j_idx
+        shift = fm.d_shift if shift is None else shift
+        cell = fm.cell if cell is None else cell
+        n = self.nsp if nsp is None else int(nsp)
+        iu = np.triu_indices(n)
+        arr = np.array([warren_cowley_radius(sp, i_idx, j_idx, shift, cell, shell_r, tol=tol,
+                                             nsp=n)[iu] for sp, _ in self.samples])
         return arr.mean(0), arr.std(0), len(arr)
 
     def alpha_series(self, nsteps, shell=0):

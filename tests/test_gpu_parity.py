@@ -326,3 +326,67 @@ def test_workload_4000_consistency():
     assert np.array_equal(S.counts, S.counts_of(sp))
     np.testing.assert_allclose(S.chi2, S.chi2_of(sp), rtol=1e-9)
     assert S.best_chi2.min() < c2_0.min()
+
+
+def test_collection_and_posterior_wc(bin500):
+    """Posterior collection (method_a_rmc.py:401-491): every collect_every steps after
+    collect_after the rungs nearest collect_beta are stored; posterior_wc averages the
+    Warren-Cowley parameters of the stored configurations.  Checked against the oracle
+    chain snapshotted at the same steps and the oracle's Warren-Cowley by shell index."""
+    from oracle import lattice as OL
+    P, O = bin500
+    S = P.sampler
+    S.collect_after, S.collect_every, S.n_collect = 0, 10, 3
+    try:
+        S.seed = 17
+        S.init_state(P.cfg0, P.cell.ideal)
+        _say("[run] device with collection")
+        S.run(40, exchange_every=5, log_every=10 ** 9)
+        R = OR.PlainSampler(O, nrep=S.nrep, seed=17, n_collect=3)
+        R.init_state(P.cfg0)
+        rungs = R.collect_rungs()
+        assert np.array_equal(np.sort(S.collect_rungs()), np.sort(rungs))
+        snaps = []
+        for _ in range(4):
+            R.run(10, exchange_every=5)
+            snaps += [R.species[r].copy() for r in S.collect_rungs()]
+        assert len(S.samples) == len(snaps) == 12
+        for (sp, c2), ref in zip(S.samples, snaps):
+            assert np.array_equal(sp, ref)
+        mean, std, n = S.posterior_wc(float(P.radii[0]))
+        iu = np.triu_indices(2)
+        ref = np.array([OL.warren_cowley(sp, P.i, P.shell, P.j, 0, 2)[iu] for sp in snaps])
+        assert n == 12
+        np.testing.assert_allclose(mean, ref.mean(0), rtol=0, atol=1e-12)
+        np.testing.assert_allclose(std, ref.std(0), rtol=0, atol=1e-12)
+        # the radius form of the reference (pair lists and cell given explicitly)
+        m2, _, _ = S.posterior_wc(float(P.radii[1]), P.i, P.j, P.shift, P.cell, nsp=2)
+        ref1 = np.array([OL.warren_cowley(sp, P.i, P.shell, P.j, 1, 2)[iu] for sp in snaps])
+        np.testing.assert_allclose(m2, ref1.mean(0), rtol=0, atol=1e-12)
+    finally:
+        S.collect_after, S.collect_every, S.n_collect = None, 25, 4
+        S.samples = []
+
+
+def test_target_preparation(bin500):
+    """Target and model-error sigma of the workload (chimera/fitrun.py:267-288): the data's
+    chi(R) over the random start configuration's per-edge amplitude, sigma the RMS residual
+    of that configuration at its amplitude, per component, over the amplitude.  Recomputed
+    with the oracle's transform and pair sum."""
+    P, O = bin500
+    t = P.tables
+    chi_d = P.chi_data
+    trs = O.transforms
+    tgt = OT.multi_to_R(trs, P.chan_edge, chi_d, R=P.R)
+    cnt = np.bincount(P.cfg0, minlength=P.nsp)
+    chi0 = OF.chi_pairs(P.k, t.f, t.delta, t.lam, t.s02, P.cell.ideal, P.cfg0, P.i, P.j,
+                        P.shift, P.sig2, normalise=False) / np.maximum(cnt, 1)[:, None]
+    m0 = OT.multi_to_R(trs, P.chan_edge, chi0, R=P.R)
+    sc = np.empty(P.nchan)
+    for e in range(P.nsp):
+        ce = P.chan_edge == e
+        sc[ce] = (m0[ce].conj() * tgt[ce]).real.sum() / (np.abs(m0[ce]) ** 2).sum()
+    np.testing.assert_allclose(P.scales0, sc, rtol=1e-10)
+    np.testing.assert_allclose(P.target, tgt / sc[:, None], rtol=1e-10, atol=1e-14)
+    sig = np.sqrt((np.abs(m0 * sc[:, None] - tgt) ** 2).mean(1) / 2.0) / sc
+    np.testing.assert_allclose(P.sigma[:, 0], sig, rtol=1e-9)

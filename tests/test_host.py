@@ -302,3 +302,16 @@ def test_shell_model_directions():
         dr = dr_from_free(np.random.default_rng(nsp).normal(0, 0.01, nfree), comp)
         np.testing.assert_allclose(dr, dr.T)
         assert abs(comp @ dr @ comp) < 1e-16
+
+
+def test_warren_cowley_radius_matches_shell_index():
+    """The reference's radius form of Warren-Cowley (exafs_gpu/lattice.py:167-185) selects
+    exactly the pairs of one lattice shell when tol is below half the shell spacing."""
+    from chimera_amd.lattice import random_config, warren_cowley_radius
+    c = CrystalSupercell("fcc", (3, 3, 3), 3.59)
+    i, j, sh = c.neighbour_list(5.2)
+    shell, radii = c.shells(i, j, sh)
+    sp = random_config(108, [1, 2, 1], 4)
+    for s in range(4):
+        np.testing.assert_allclose(warren_cowley_radius(sp, i, j, sh, c, radii[s], nsp=3),
+                                   OL.warren_cowley(sp, i, shell, j, s, 3), rtol=0, atol=1e-14)
