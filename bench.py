@@ -27,35 +27,58 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def cpu_baseline(P, seconds=15.0, procs=1):
-    """The oracle's plain sampler (the reference algorithm step for step, FP64 NumPy)
-    on this host: replica-moves/s over a bounded sample of the same workload."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
-    try:
-        from threadpoolctl import threadpool_limits
-        limit = threadpool_limits(1)
-    except Exception:                                    # noqa: BLE001
-        limit = None
-    from oracle import rmc as OR
+def cpu_baseline(P, seconds=15.0):
+    """The oracle's plain sampler (the reference algorithm step for step, FP64 NumPy) on this
+    host, in the two arrangements of the baseline protocol: (i) one process with every
+    core's BLAS threads running all replicas; (ii) one single-threaded process per core,
+    each running its share of the replicas (replicas are independent between exchanges).
+    Each runs a bounded sample of about `seconds`; the better is the baseline.  The workers
+    are separate Python processes that import only NumPy and the oracle (no GPU)."""
+    import pickle
+    import subprocess
+    import tempfile
     from oracle.build import problem_from
     O = problem_from(P)
     nrep = P.sampler.nrep
-    R = OR.PlainSampler(O, nrep=nrep, seed=1)
-    R.init_state(P.cfg0)
-    t0 = time.perf_counter()
-    R.run(1, exchange_every=25)                          # warm-up step
-    t1 = time.perf_counter()
-    per_step = t1 - t0
-    n = max(1, int(seconds / max(per_step, 1e-6)))
-    t0 = time.perf_counter()
-    R.run(n, exchange_every=25)
-    dt = time.perf_counter() - t0
-    if limit is not None:
-        limit.unregister() if hasattr(limit, "unregister") else None
-    return {"value": nrep * n / dt, "unit": "replica swap moves/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{n} steps x {nrep} replicas of the same workload ({dt:.1f} s), "
-                      "FP64 NumPy oracle of the reference algorithm, 1 thread"}
+    cores = max(1, min(16, os.cpu_count() or 1))       # the host's share for one GPU
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "problem.pkl")
+        with open(path, "wb") as fh:
+            pickle.dump((O, P.cfg0), fh)
+
+        def arrangement(threads, reps):
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads),
+                       MKL_NUM_THREADS=str(threads))
+            procs = [subprocess.Popen([sys.executable, "-m", "oracle.cpu_worker", path,
+                                       str(seconds), str(r), str(1 + q)], cwd=ROOT, env=env,
+                                      stdout=subprocess.PIPE, text=True)
+                     for q, r in enumerate(reps)]
+            outs = []
+            for pr in procs:
+                out, _ = pr.communicate()
+                if pr.returncode != 0:
+                    raise RuntimeError(f"CPU baseline worker failed with code {pr.returncode}")
+                outs.append(json.loads(out.strip().splitlines()[-1]))
+            return sum(o["moves"] / o["seconds"] for o in outs), outs
+
+        v1, o1 = arrangement(cores, [nrep])
+        nproc = min(cores, nrep)
+        reps = [nrep // nproc + (1 if q < nrep % nproc else 0) for q in range(nproc)]
+        v2, o2 = arrangement(1, reps)
+    lscpu = ""
+    try:
+        lscpu = next((ln.split(":", 1)[1].strip() for ln in
+                      subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines()
+                      if ln.startswith("Model name")), "")
+    except OSError:
+        pass
+    best = max((v1, cores, "one process, %d BLAS threads, %d replicas" % (cores, nrep)),
+               (v2, nproc, "%d single-threaded processes, %s replicas each" % (nproc, "/".join(map(str, sorted(set(reps)))))))
+    return {"value": best[0], "unit": "replica swap moves/s", "cores": best[1], "kind": "port",
+            "sample": (f"{best[2]}; (i) 1 process x {cores} threads: {v1:.1f} moves/s over "
+                       f"{o1[0]['steps']} steps, (ii) {nproc} processes x 1 thread: {v2:.1f} moves/s "
+                       f"over {o2[0]['steps']} steps each (about {seconds:.0f} s per arrangement, same "
+                       f"workload, FP64 NumPy oracle of the reference algorithm, host {lscpu})")}
 
 
 def ess_run(P, S, burn, steps, exchange_every, setup):

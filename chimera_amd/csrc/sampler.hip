@@ -451,12 +451,13 @@ __device__ __forceinline__ void gemm_delta_bf16_tile(const Geo &g, int nrep, con
                   (unsigned long long)O[0].bump);
 }
 
+template <int KB = KBW>
 __device__ __forceinline__ void gemm_delta_bf16_body(const Geo &g, int nrep, const int *__restrict__ dcounts,
                                                      const bf16x8 *__restrict__ Tb,
                                                      const double *__restrict__ mcur, const TileOut &O, int ct,
                                                      int rbi, int e) {
     const int *const dc[1] = {dcounts};
-    gemm_delta_bf16_tile<1, KBW>(g, nrep, dc, Tb, mcur, &O, ct, rbi, e);
+    gemm_delta_bf16_tile<1, KB>(g, nrep, dc, Tb, mcur, &O, ct, rbi, e);
 }
 
 __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const int *__restrict__ dcounts,
@@ -2611,7 +2612,10 @@ struct GemmProb {
     TileOut O;
 };
 
-template <int kMode>   // 0 exact FP64, 1 split-BF16 delta, 2 FP32 delta
+// kMode 0 exact FP64, 1 split-BF16 delta, 2 FP32 delta.  KB: K blocks each wave holds in
+// registers; a batch whose problems all have nkb <= 4 runs the KB = 1 instance (a quarter
+// of the operand registers, so more tiles are in flight per CU).
+template <int kMode, int KB = KBW>
 __global__ __launch_bounds__(256) void k_gemm_grouped(const GemmProb *__restrict__ probs,
                                                        const int4 *__restrict__ tiles) {
     const int4 tl = tiles[blockIdx.x];
@@ -2619,7 +2623,7 @@ __global__ __launch_bounds__(256) void k_gemm_grouped(const GemmProb *__restrict
     if (kMode == 0)
         gemm_abs_body(P.g, P.nrep, P.counts, P.dc, P.basisR, P.O, tl.y, tl.z, tl.w);
     else if (kMode == 1)
-        gemm_delta_bf16_body(P.g, P.nrep, P.dc, P.Tb, P.mcur, P.O, tl.y, tl.z, tl.w);
+        gemm_delta_bf16_body<KB>(P.g, P.nrep, P.dc, P.Tb, P.mcur, P.O, tl.y, tl.z, tl.w);
     else
         gemm_delta_f32_body(P.g, P.nrep, P.dc, P.basis32, P.mcur, P.O, tl.y, tl.z, tl.w);
 }
@@ -2666,7 +2670,7 @@ __global__ void k_exchange_grouped(const ExchProb *__restrict__ probs, int parit
 struct chm_batch {
     chm_ctx *ctx = nullptr;
     std::vector<chm_sampler *> S;
-    int mode = 0, anchor = 0, nslot = 0, nsplit_max = 1, ntile = 0;
+    int mode = 0, anchor = 0, nslot = 0, nsplit_max = 1, ntile = 0, max_nkb = 0;
     size_t lds_gemm_max = 0, lds_slot_max = 0;
     chm::DevBuf gemm[2], gemm_nb, reanchor, tiles, slot, slots, exch;
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -2692,6 +2696,8 @@ int batch_gemm(chm_batch *B, const GemmProb *probs, int mode) {
     const int4 *tiles = B->tiles.as<int4>();
     if (mode == 0)
         k_gemm_grouped<0><<<B->ntile, 256, B->lds_gemm_max, st>>>(probs, tiles);
+    else if (mode == 1 && B->max_nkb <= 4)
+        k_gemm_grouped<1, 1><<<B->ntile, 256, 0, st>>>(probs, tiles);
     else if (mode == 1)
         k_gemm_grouped<1><<<B->ntile, 256, 0, st>>>(probs, tiles);
     else
@@ -2764,6 +2770,7 @@ extern "C" int chm_batch_create(chm_ctx *ctx, int n, chm_sampler *const *sampler
     for (int p = 0; p < n; ++p) {
         chm_sampler *S = B->S[p];
         const Geo &g = S->g;
+        B->max_nkb = std::max(B->max_nkb, g.nkb);
         for (int e = 0; e < g.nsp; ++e)
             for (int rb = 0; rb < (S->nrep + RB - 1) / RB; ++rb)
                 for (int ct = 0; ct < g.nct; ++ct) tiles.push_back(make_int4(p, ct, rb, e));

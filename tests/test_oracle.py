@@ -473,3 +473,22 @@ def test_posterior_cov_qr_equals_normal_equations():
     P = rng.uniform(0.1, 2.0, 6)
     np.testing.assert_allclose(OS.posterior_cov(J, P), np.linalg.inv(J.T @ J + np.diag(P)),
                                rtol=1e-10, atol=1e-14)
+
+
+def test_cpu_worker_runs(tmp_path):
+    """The CPU-baseline worker (a separate process, NumPy only) runs the plain sampler on a
+    pickled problem and reports its replica moves and wall time."""
+    import json
+    import pickle
+    import subprocess
+    import sys
+    P = _toy_problem()
+    path = tmp_path / "p.pkl"
+    with open(path, "wb") as fh:
+        pickle.dump((P, OL.random_config(108, [1, 1], 5)), fh)
+    root = str(__import__("pathlib").Path(__file__).resolve().parents[1])
+    out = subprocess.run([sys.executable, "-m", "oracle.cpu_worker", str(path), "0.3", "3", "1"],
+                         cwd=root, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    assert r["moves"] == 3 * r["steps"] and r["seconds"] > 0
