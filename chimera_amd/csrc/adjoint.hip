@@ -11,12 +11,18 @@
 // is the scatterer), always in the same order: the gradients are deterministic (no
 // floating-point atomics).
 //
-// Relaxed-occupancy forward: lanes own k points (as the hard path sum), a pair record
-// carries both sites' weight vectors; T_ab = base f_b sin(2kr + delta_ab) is formed from
-// one sincos(2kr) per (pair, k) and the tables f_b cos(delta_ab), f_b sin(delta_ab).
+// Relaxed-occupancy forward: lanes own the k > 0 points, one each (as the hard path
+// sum), a pair record carries both sites' weight vectors; T_ab = base f_b sin(2kr +
+// delta_ab) is formed from one sincos(2kr) per (pair, k) and the tables f_b cos(delta_ab),
+// f_b sin(delta_ab).
+//
+// Phases are carried in revolutions (2 k r / 2 pi = (k / pi) r) and sin / cos / exp are
+// the range-specific polynomials of fp64math.h.  The
+// per-k constants sit in LDS as one KRow per k point, read by all lanes at once.
 #include <cmath>
 
 #include "common.h"
+#include "fp64math.h"
 
 namespace chm {
 int distances_device(chm_ctx *ctx, int nconf, bool want_evec);
@@ -29,7 +35,31 @@ int reduce_partials(chm_ctx *ctx, const double *partial, double *out, int nwg, i
 namespace {
 
 constexpr int kLanes = 256;
-constexpr int kKq = 2;   // k points per lane in the soft forward: nk <= 512
+constexpr int kMaxK = 512;   // k points above 0 in the soft forward (one lane each)
+constexpr double kInvPi = 0.31830988618379067154;
+constexpr double kC3Rev = (4.0 / 3.0) * 0.15915494309189533577;   // (4/3) / (2 pi)
+
+// Per-k constants of a term: e^(c1 r + c2 sigma^2) is the damping, kp r the phase
+// 2 k r in revolutions, c3k C3 the cumulant phase in revolutions, pref = S0^2 / k
+// (0 for k <= 0, where the term is defined as 0), tk = 2 k = d(2 k r)/dr.
+struct KRow {
+    double kp, c1, c2, pref, c3k, tk;
+};
+
+__device__ inline void stage_krows(KRow *rows, const double *__restrict__ ktab,
+                                   const double *__restrict__ lamtab, int nk, double s02) {
+    for (int q = threadIdx.x; q < nk; q += blockDim.x) {
+        const double kv = ktab[q];
+        KRow kr;
+        kr.kp = kv * kInvPi;
+        kr.c1 = -2.0 / lamtab[q];
+        kr.c2 = -2.0 * kv * kv;
+        kr.pref = kv > 0.0 ? s02 / kv : 0.0;
+        kr.c3k = kC3Rev * kv * kv * kv;
+        kr.tk = 2.0 * kv;
+        rows[q] = kr;
+    }
+}
 
 int upload(chm_ctx *ctx, chm::DevBuf &b, const void *h, size_t n) {
     int rc = b.ensure(n);
@@ -55,23 +85,19 @@ __global__ __launch_bounds__(256) void k_adjoint_r(
     const double *__restrict__ lamtab, const double *__restrict__ ktab,
     const double *__restrict__ drtab, const double *__restrict__ c3tab,
     const double *__restrict__ s2tab, const double *__restrict__ gchi,
-    double *__restrict__ gr, int npairs, int nk, double s02) {
+    double *__restrict__ gr, int npairs, int nk, double s02, const chm::PolyCoefs pc) {
     extern __shared__ double smem[];
     double *sf = smem;                  // f (NSP, nk)
-    double *sd = sf + NSP * nk;         // delta (NSP*NSP, nk)
+    double *sd = sf + NSP * nk;         // delta / 2 pi (NSP*NSP, nk)
     double *sg = sd + NSP * NSP * nk;   // this configuration's dL/dchi (NSP, nk)
-    double *sk = sg + NSP * nk;         // k
-    double *sl = sk + nk;               // -2 / lambda
+    KRow *sk = reinterpret_cast<KRow *>(sg + NSP * nk);
     const int ic = blockIdx.y;
     for (int q = threadIdx.x; q < NSP * nk; q += kLanes) {
         sf[q] = ftab[q];
         sg[q] = gchi[(long)ic * NSP * nk + q];
     }
-    for (int q = threadIdx.x; q < NSP * NSP * nk; q += kLanes) sd[q] = dtab[q];
-    for (int q = threadIdx.x; q < nk; q += kLanes) {
-        sk[q] = ktab[q];
-        sl[q] = -2.0 / lamtab[q];
-    }
+    for (int q = threadIdx.x; q < NSP * NSP * nk; q += kLanes) sd[q] = dtab[q] * (0.5 * kInvPi);
+    stage_krows(sk, ktab, lamtab, nk, s02);
     __syncthreads();
     const int p = blockIdx.x * kLanes + threadIdx.x;
     if (p >= npairs) return;
@@ -81,6 +107,7 @@ __global__ __launch_bounds__(256) void k_adjoint_r(
     const int sa = t / NSP, sb = t - sa * NSP;
     const double re = r_all[o] + drtab[sh * NSP * NSP + t];
     const double ir = 1.0 / re;
+    const double ir2 = ir * ir;
     const double s2 = sig2[p] + s2tab[sh * NSP * NSP + t];
     const double c3 = c3tab[sh];
     const double *gs = sg + sa * nk;
@@ -89,13 +116,13 @@ __global__ __launch_bounds__(256) void k_adjoint_r(
     double acc = 0.0;
     for (int ik = 0; ik < nk; ++ik) {
         const double g = gs[ik];
-        const double kv = sk[ik];
-        if (g == 0.0 || !(kv > 0.0)) continue;   // the k = 0 term is defined as 0
-        const double c1 = sl[ik];
-        const double amp = s02 * fs[ik] * ir * ir / kv * exp(c1 * re - 2.0 * kv * kv * s2);
+        const KRow kr = sk[ik];
+        if (g == 0.0 || kr.pref == 0.0) continue;   // the k = 0 term is defined as 0
+        const double ph = fma(kr.kp, re, fma(-kr.c3k, c3, ds[ik]));
         double sn, cs;
-        sincos(2.0 * kv * re + ds[ik] - (4.0 / 3.0) * kv * kv * kv * c3, &sn, &cs);
-        acc += g * amp * ((c1 - 2.0 * ir) * sn + 2.0 * kv * cs);
+        chm::sincos2pi(ph, pc, &sn, &cs);
+        const double amp = kr.pref * fs[ik] * ir2 * chm::exp_fin(fma(kr.c1, re, kr.c2 * s2), pc);
+        acc += g * amp * ((kr.c1 - 2.0 * ir) * sn + kr.tk * cs);
     }
     gr[o] = acc;
 }
@@ -158,56 +185,53 @@ __global__ void k_soft_tables(const double *__restrict__ ftab, const double *__r
 
 template <int NSP>
 struct SoftRec {
-    double r, ir2, s2;
+    double r, lir2, s2;   // r, ln(1 / r^2) (folded into the exponent), sigma^2
     double wi[NSP], wj[NSP];
 };
 
+// One lane per k point above 0 (lane l owns k index k0 + l); each round stages blockDim
+// pair records with both sites' weights.
 template <int NSP>
-__global__ __launch_bounds__(256) void k_chi_soft(
+__global__ __launch_bounds__(512) void k_chi_soft(
     const double *__restrict__ r_all, const double *__restrict__ w_all,
     const int *__restrict__ iidx, const int *__restrict__ jidx,
     const double *__restrict__ sig2, const double *__restrict__ fctab,
     const double *__restrict__ fstab, const double *__restrict__ lamtab,
     const double *__restrict__ ktab, double *__restrict__ partial, int npairs, int natoms,
-    int nk, int pairs_per_wg, double s02) {
+    int nk, int k0, int pairs_per_wg, double s02, const chm::PolyCoefs pc) {
     extern __shared__ double smem[];
     double *sfc = smem;
     double *sfs = sfc + NSP * NSP * nk;
     SoftRec<NSP> *tile = reinterpret_cast<SoftRec<NSP> *>(sfs + NSP * NSP * nk);
     const int ic = blockIdx.y, wg = blockIdx.x, nwg = gridDim.x;
-    for (int q = threadIdx.x; q < NSP * NSP * nk; q += kLanes) {
+    const int nthr = blockDim.x;
+    for (int q = threadIdx.x; q < NSP * NSP * nk; q += nthr) {
         sfc[q] = fctab[q];
         sfs[q] = fstab[q];
     }
-    double kk[kKq], c1[kKq], c2[kKq], pref[kKq];
-    bool live[kKq];
+    const int ik = k0 + (int)threadIdx.x;
+    const bool live = ik < nk;
+    const int ikc = live ? ik : nk - 1;
+    const double kv = live ? ktab[ik] : 0.0;
+    const double kp = kv * kInvPi;
+    const double c1 = -2.0 / (live ? lamtab[ik] : 1.0);
+    const double c2 = -2.0 * kv * kv;
+    const double pref = (live && kv > 0.0) ? s02 / kv : 0.0;
+    double acc[NSP];
 #pragma unroll
-    for (int u = 0; u < kKq; ++u) {
-        const int ik = threadIdx.x + u * kLanes;
-        live[u] = ik < nk;
-        const double kv = live[u] ? ktab[ik] : 0.0;
-        kk[u] = kv;
-        c1[u] = -2.0 / (live[u] ? lamtab[ik] : 1.0);
-        c2[u] = -2.0 * kv * kv;
-        pref[u] = kv > 0.0 ? s02 / kv : 0.0;
-    }
-    double acc[kKq][NSP];
-#pragma unroll
-    for (int u = 0; u < kKq; ++u)
-#pragma unroll
-        for (int s = 0; s < NSP; ++s) acc[u][s] = 0.0;
+    for (int s = 0; s < NSP; ++s) acc[s] = 0.0;
     const double *rc = r_all + (long)ic * npairs;
     const double *wc = w_all + (long)ic * natoms * NSP;
     const int pbeg = wg * pairs_per_wg, pend = min(npairs, pbeg + pairs_per_wg);
-    for (int p0 = pbeg; p0 < pend; p0 += kLanes) {
+    for (int p0 = pbeg; p0 < pend; p0 += nthr) {
         __syncthreads();
         {
-            const int p = p0 + threadIdx.x;
+            const int p = p0 + (int)threadIdx.x;
             SoftRec<NSP> rec;
             if (p < pend) {
                 const double re = rc[p];
                 rec.r = re;
-                rec.ir2 = 1.0 / (re * re);
+                rec.lir2 = -2.0 * log(re);
                 rec.s2 = sig2[p];
                 const double *wi = wc + (long)iidx[p] * NSP;
                 const double *wj = wc + (long)jidx[p] * NSP;
@@ -218,7 +242,7 @@ __global__ __launch_bounds__(256) void k_chi_soft(
                 }
             } else {
                 rec.r = 1.0;
-                rec.ir2 = 0.0;   // contributes exactly 0
+                rec.lir2 = -2000.0;   // e^-2000 = 0: contributes exactly 0
                 rec.s2 = 0.0;
 #pragma unroll
                 for (int s = 0; s < NSP; ++s) rec.wi[s] = rec.wj[s] = 0.0;
@@ -226,36 +250,31 @@ __global__ __launch_bounds__(256) void k_chi_soft(
             tile[threadIdx.x] = rec;
         }
         __syncthreads();
-        const int nt = min(kLanes, pend - p0);
+        const int nt = min(nthr, pend - p0);
         for (int q = 0; q < nt; ++q) {
             const SoftRec<NSP> rec = tile[q];
+            const double base = chm::exp_fin(fma(c1, rec.r, fma(c2, rec.s2, rec.lir2)), pc);
+            double sn, cs;
+            chm::sincos2pi(kp * rec.r, pc, &sn, &cs);
 #pragma unroll
-            for (int u = 0; u < kKq; ++u) {
-                if (!live[u]) continue;
-                const int ik = threadIdx.x + u * kLanes;
-                const double base = pref[u] * rec.ir2 * exp(c1[u] * rec.r + c2[u] * rec.s2);
-                double sn, cs;
-                sincos(2.0 * kk[u] * rec.r, &sn, &cs);
+            for (int a = 0; a < NSP; ++a) {
+                double s = 0.0;
 #pragma unroll
-                for (int a = 0; a < NSP; ++a) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int b = 0; b < NSP; ++b) {
-                        const int t = (a * NSP + b) * nk + ik;
-                        s += rec.wj[b] * (sn * sfc[t] + cs * sfs[t]);
-                    }
-                    acc[u][a] += rec.wi[a] * base * s;
+                for (int b = 0; b < NSP; ++b) {
+                    const int t = (a * NSP + b) * nk + ikc;
+                    s += rec.wj[b] * (sn * sfc[t] + cs * sfs[t]);
                 }
+                acc[a] += rec.wi[a] * base * s;
             }
         }
     }
     double *out = partial + ((long)ic * nwg + wg) * NSP * nk;
+    for (int q = threadIdx.x; q < k0; q += nthr)
 #pragma unroll
-    for (int u = 0; u < kKq; ++u) {
-        const int ik = threadIdx.x + u * kLanes;
-        if (!live[u]) continue;
+        for (int s = 0; s < NSP; ++s) out[s * nk + q] = 0.0;
+    if (live) {
 #pragma unroll
-        for (int s = 0; s < NSP; ++s) out[s * nk + ik] = acc[u][s];
+        for (int s = 0; s < NSP; ++s) out[s * nk + ik] = pref * acc[s];
     }
 }
 
@@ -268,29 +287,26 @@ __global__ __launch_bounds__(256) void k_adjoint_soft(
     const double *__restrict__ fstab, const double *__restrict__ lamtab,
     const double *__restrict__ ktab, const double *__restrict__ gchi,
     double *__restrict__ gr, double *__restrict__ gwi_out, double *__restrict__ gwj_out,
-    int npairs, int natoms, int nk, double s02) {
+    int npairs, int natoms, int nk, double s02, const chm::PolyCoefs pc) {
     extern __shared__ double smem[];
     double *sfc = smem;
     double *sfs = sfc + NSP * NSP * nk;
     double *sg = sfs + NSP * NSP * nk;
-    double *sk = sg + NSP * nk;
-    double *sl = sk + nk;
+    KRow *sk = reinterpret_cast<KRow *>(sg + NSP * nk);
     const int ic = blockIdx.y;
     for (int q = threadIdx.x; q < NSP * NSP * nk; q += kLanes) {
         sfc[q] = fctab[q];
         sfs[q] = fstab[q];
     }
     for (int q = threadIdx.x; q < NSP * nk; q += kLanes) sg[q] = gchi[(long)ic * NSP * nk + q];
-    for (int q = threadIdx.x; q < nk; q += kLanes) {
-        sk[q] = ktab[q];
-        sl[q] = -2.0 / lamtab[q];
-    }
+    stage_krows(sk, ktab, lamtab, nk, s02);
     __syncthreads();
     const int p = blockIdx.x * kLanes + threadIdx.x;
     if (p >= npairs) return;
     const long o = (long)ic * npairs + p;
     const double re = r_all[o];
     const double ir = 1.0 / re;
+    const double ir2 = ir * ir;
     const double s2 = sig2[p];
     double wi[NSP], wj[NSP], gwi[NSP], gwj[NSP];
     const double *wc = w_all + (long)ic * natoms * NSP;
@@ -302,13 +318,12 @@ __global__ __launch_bounds__(256) void k_adjoint_soft(
     }
     double acc = 0.0;
     for (int ik = 0; ik < nk; ++ik) {
-        const double kv = sk[ik];
-        if (!(kv > 0.0)) continue;
-        const double c1 = sl[ik];
-        const double base = s02 / kv * ir * ir * exp(c1 * re - 2.0 * kv * kv * s2);
-        const double dfac = c1 - 2.0 * ir;
+        const KRow kr = sk[ik];
+        if (kr.pref == 0.0) continue;
+        const double base = kr.pref * ir2 * chm::exp_fin(fma(kr.c1, re, kr.c2 * s2), pc);
+        const double dfac = kr.c1 - 2.0 * ir;
         double sn, cs;
-        sincos(2.0 * kv * re, &sn, &cs);
+        chm::sincos2pi(kr.kp * re, pc, &sn, &cs);
 #pragma unroll
         for (int a = 0; a < NSP; ++a) {
             const double ga = sg[a * nk + ik];
@@ -324,7 +339,7 @@ __global__ __launch_bounds__(256) void k_adjoint_soft(
                 gwj[b] += ga * wi[a] * base * snv;
             }
             gwi[a] += ga * base * st;
-            acc += ga * wi[a] * base * (dfac * st + 2.0 * kv * dt);
+            acc += ga * wi[a] * base * (dfac * st + kr.tk * dt);
         }
     }
     gr[o] = acc;
@@ -338,8 +353,8 @@ __global__ __launch_bounds__(256) void k_adjoint_soft(
 // ---------------------------------------------------------------- launchers
 
 size_t adjoint_lds(int nsp, int nk, bool soft) {
-    return soft ? sizeof(double) * (2 * nsp * nsp + nsp + 2) * nk
-                : sizeof(double) * (nsp + nsp * nsp + nsp + 2) * nk;
+    return soft ? sizeof(double) * (2 * nsp * nsp + nsp) * nk + sizeof(KRow) * nk
+                : sizeof(double) * (nsp + nsp * nsp + nsp) * nk + sizeof(KRow) * nk;
 }
 
 template <int NSP>
@@ -354,7 +369,7 @@ int launch_adjoint_r(chm_ctx *ctx, int nconf) {
         ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(), ctx->sig2.as<double>(),
         ctx->f.as<double>(), ctx->delta.as<double>(), ctx->lam.as<double>(), ctx->k.as<double>(),
         ctx->drtab.as<double>(), ctx->c3tab.as<double>(), ctx->s2tab.as<double>(),
-        ctx->gchi.as<double>(), ctx->gr.as<double>(), npairs, nk, ctx->s02);
+        ctx->gchi.as<double>(), ctx->gr.as<double>(), npairs, nk, ctx->s02, chm::kPolyCoefs);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -362,22 +377,26 @@ int launch_adjoint_r(chm_ctx *ctx, int nconf) {
 template <int NSP>
 int launch_chi_soft(chm_ctx *ctx, int nconf) {
     const int nk = ctx->nk, npairs = ctx->npairs;
-    const size_t lds = sizeof(double) * 2 * NSP * NSP * nk + sizeof(SoftRec<NSP>) * kLanes;
+    int k0 = 0;
+    while (k0 < nk && !(ctx->h_k[k0] > 0.0)) ++k0;
+    const int nkz = std::max(1, nk - k0);
+    CHM_REQUIRE(nkz <= kMaxK, CHM_E_ARG, "at most 512 k points above 0");
+    const int nthr = std::min(kMaxK, (nkz + 63) / 64 * 64);
+    const size_t lds = sizeof(double) * 2 * NSP * NSP * nk + sizeof(SoftRec<NSP>) * nthr;
     CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
-    int ntiles = (npairs + kLanes - 1) / kLanes;
-    int want = (2048 + nconf - 1) / nconf;
-    int nwg = std::max(1, std::min(ntiles, want));
-    const int ppw = ((ntiles + nwg - 1) / nwg) * kLanes;
+    const int rounds = (npairs + nthr - 1) / nthr;
+    int nwg = std::max(1, std::min(rounds, (2048 + nconf - 1) / nconf));
+    const int ppw = ((rounds + nwg - 1) / nwg) * nthr;
     nwg = (npairs + ppw - 1) / ppw;
     int rc = ctx->partial.ensure(sizeof(double) * (size_t)nconf * nwg * NSP * nk);
     if (rc) return rc;
     hipFuncSetAttribute((const void *)k_chi_soft<NSP>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds);
-    k_chi_soft<NSP><<<dim3(nwg, nconf), kLanes, lds, ctx->stream>>>(
+    k_chi_soft<NSP><<<dim3(nwg, nconf), nthr, lds, ctx->stream>>>(
         ctx->r.as<double>(), ctx->w.as<double>(), ctx->i_idx.as<int>(), ctx->j_idx.as<int>(),
         ctx->sig2.as<double>(), ctx->soft_fc.as<double>(), ctx->soft_fs.as<double>(),
         ctx->lam.as<double>(), ctx->k.as<double>(), ctx->partial.as<double>(), npairs, ctx->natoms,
-        nk, ppw, ctx->s02);
+        nk, k0, ppw, ctx->s02, chm::kPolyCoefs);
     CHM_LAUNCH_CHECK();
     if ((rc = ctx->chi.ensure(sizeof(double) * (size_t)nconf * NSP * nk))) return rc;
     return chm::reduce_partials(ctx, ctx->partial.as<double>(), ctx->chi.as<double>(), nwg,
@@ -396,7 +415,8 @@ int launch_adjoint_soft(chm_ctx *ctx, int nconf) {
         ctx->r.as<double>(), ctx->w.as<double>(), ctx->i_idx.as<int>(), ctx->j_idx.as<int>(),
         ctx->sig2.as<double>(), ctx->soft_fc.as<double>(), ctx->soft_fs.as<double>(),
         ctx->lam.as<double>(), ctx->k.as<double>(), ctx->gchi.as<double>(), ctx->gr.as<double>(),
-        ctx->gwi.as<double>(), ctx->gwj.as<double>(), npairs, ctx->natoms, nk, ctx->s02);
+        ctx->gwi.as<double>(), ctx->gwj.as<double>(), npairs, ctx->natoms, nk, ctx->s02,
+        chm::kPolyCoefs);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -507,7 +527,6 @@ extern "C" int chm_fm_chi_soft(chm_ctx *ctx, int nconf, const double *pos, const
                                double *chi) {
     CHM_REQUIRE(ctx && ctx->npairs > 0, CHM_E_STATE, "set tables and topology first");
     CHM_REQUIRE(nconf > 0 && pos && w && chi, CHM_E_ARG, "bad arguments");
-    CHM_REQUIRE(ctx->nk <= kLanes * kKq, CHM_E_ARG, "at most 512 k points");
     CHM_HIP(hipSetDevice(ctx->device));
     int rc = prepare_soft(ctx, nconf, pos, w, nullptr);
     if (rc) return rc;

@@ -1,17 +1,18 @@
 // Context, tables, topology and the pair-space path sum (single scattering) for gfx950.
 //
 // Path sum layout: a workgroup owns one configuration and a contiguous run of pairs;
-// its 256 lanes own k points (lane, lane+256, ...).  Pair records are staged through
-// LDS in tiles of 256 (one coalesced load per tile, k-independent parts precomputed
-// by the loading lane), the per-k tables f[b](k), delta[t](k) sit in LDS indexed
-// [row][k] so a wavefront's 64 lanes read 64 consecutive words.  Every pair record
-// read from HBM is reused for all nk points.  Each workgroup writes one partial
-// chi (nsp, nk); a second kernel sums the partials in a fixed order, so results are
-// deterministic.
+// its lanes own the k > 0 points, one each (five full wavefronts for 320 points).
+// Pair records are staged through LDS one per lane per round (one coalesced load,
+// k-independent parts precomputed by the loading lane), the per-k tables f[b](k),
+// delta[t](k) sit in LDS indexed [row][k] so a wavefront's 64 lanes read 64 consecutive
+// words.  Every pair record read from HBM is reused for all nk points.  Each workgroup
+// writes one partial chi (nsp, nk); a second kernel sums the partials in a fixed order,
+// so results are deterministic.
 #include <cmath>
 #include <cstring>
 
 #include "common.h"
+#include "fp64math.h"
 
 namespace chm {
 
@@ -282,139 +283,157 @@ extern "C" int chm_fm_ptypes(chm_ctx *ctx, int nconf, const int32_t *species, in
 
 namespace {
 
-constexpr int kTile = 256;   // pairs staged per LDS tile == block size
-constexpr int kKq = 2;       // k points per lane: nk <= 512
+constexpr int kMaxK = 512;    // k points with k > 0 (one lane each, at most 8 waves)
 constexpr double kInvPi = 0.31830988618379067154;
 constexpr double kTwoPi = 6.28318530717958647692;
 
+// FP64 path sum.  Lane l of a workgroup of roundup(nkz, 64) lanes owns k index k0 + l
+// (k0 = first k > 0; the k = 0 term is 0), so the 0..16 A^-1 grid at 0.05 runs five full
+// wavefronts.  Each round stages blockDim pairs into LDS (one per lane, with its
+// k-independent parts: r_eff, ln(1/r_eff^2), sigma^2, C3) and every lane then sums all
+// of them at its k.  The phase is carried in revolutions, (k/pi) r + delta/2pi - C3 term,
+// formed by two fused multiply-adds, and sin(2 pi x) / e^x are the range-specific
+// polynomials of fp64math.h (the sine as cos2pi of the phase less 1/4, the 1/4 folded
+// into the delta table); two pairs are evaluated per step so that their dependent
+// polynomial chains interleave.  Pairs of one absorber are contiguous, so terms go into
+// one running sum that moves to the species' sum when the absorber species changes (a
+// uniform branch); the pair type is uniform across the wave, so table offsets are
+// scalar.
 struct PairRec {
-    double r;     // r_eff = r + dr[shell][type]
-    double ir2;   // 1 / r_eff^2
-    double s2;    // sigma^2 + s2tab[shell][type]
-    double c3;    // c3tab[shell]
-    int t;        // pair type
-    int sa;       // absorber species
+    double r;      // r_eff = r + dr[shell][type]
+    double lir2;   // ln(1 / r_eff^2), folded into the exponent
+    double s2;     // sigma^2 + s2tab[shell][type]
+    double c3;     // c3tab[shell]
+    int t, pad;    // pair type
 };
 
-// Stage a tile of pairs: lane q loads pair p0 + q and precomputes its k-independent
-// parts.
-__device__ inline void stage_tile(PairRec *tile, int p0, int npairs, int nsp,
-                                  const double *__restrict__ r, const int *__restrict__ pt,
-                                  const int *__restrict__ shell, const double *__restrict__ sig2,
-                                  const double *__restrict__ drtab,
-                                  const double *__restrict__ c3tab,
-                                  const double *__restrict__ s2tab) {
-    int q = threadIdx.x;
-    int p = p0 + q;
-    PairRec rec;
-    if (p < npairs) {
-        int t = pt[p];
-        int sh = shell[p];
-        double re = r[p] + drtab[sh * nsp * nsp + t];
-        rec.r = re;
-        rec.ir2 = 1.0 / (re * re);
-        rec.s2 = sig2[p] + s2tab[sh * nsp * nsp + t];
-        rec.c3 = c3tab[sh];
-        rec.t = t;
-        rec.sa = t / nsp;
-    } else {
-        rec.r = 1.0;
-        rec.ir2 = 0.0;  // contributes exactly 0
-        rec.s2 = 0.0;
-        rec.c3 = 0.0;
-        rec.t = 0;
-        rec.sa = 0;
-    }
-    tile[q] = rec;
+// f_b(k) e^(...) and sin(phase) of one term at this lane's k: (k/pi) r + (delta/2pi - 1/4)
+// - C3 term is the phase less a quarter revolution, so the sine is cos2pi of it.
+template <int NSP>
+__device__ __forceinline__ void pair_term(const PairRec &rec, int t, const double *sf,
+                                          const double *sd, int nk, int ikc, double kp,
+                                          double c1, double c2, double c3k,
+                                          const chm::PolyCoefs &pc, double &fe, double &sn) {
+    const int sb = t % NSP;
+    const double ph = fma(kp, rec.r, fma(-c3k, rec.c3, sd[t * nk + ikc]));
+    fe = sf[sb * nk + ikc] * chm::exp_fin(fma(c1, rec.r, fma(c2, rec.s2, rec.lir2)), pc);
+    sn = chm::cos2pi(ph, pc);
 }
 
-template <int NSP, bool F32>
-__global__ __launch_bounds__(256) void k_chi_pairs(
+template <int NSP>
+__global__ __launch_bounds__(512) void k_chi_pairs(
     const double *__restrict__ r_all, const int *__restrict__ pt_all,
     const int *__restrict__ shell, const double *__restrict__ sig2,
     const double *__restrict__ ftab, const double *__restrict__ dtab,
     const double *__restrict__ lamtab, const double *__restrict__ ktab,
     const double *__restrict__ drtab, const double *__restrict__ c3tab,
-    const double *__restrict__ s2tab, double *__restrict__ partial, int npairs, int nk,
-    int pairs_per_wg, double s02) {
+    const double *__restrict__ s2tab, double *__restrict__ partial, int npairs, int nk, int k0,
+    int pairs_per_wg, double s02, const chm::PolyCoefs pc) {
     extern __shared__ double smem[];
-    // LDS: f (NSP, nk), delta (NSP*NSP, nk), pair tile
-    double *sf = smem;
-    double *sd = sf + NSP * nk;
+    double *sf = smem;                                       // f (NSP, nk)
+    double *sd = sf + NSP * nk;                              // delta/2pi - 1/4 (NSP*NSP, nk)
     PairRec *tile = reinterpret_cast<PairRec *>(sd + NSP * NSP * nk);
-
     const int ic = blockIdx.y;
     const int w = blockIdx.x;
     const int nwg = gridDim.x;
-    for (int q = threadIdx.x; q < NSP * nk; q += blockDim.x) sf[q] = ftab[q];
-    for (int q = threadIdx.x; q < NSP * NSP * nk; q += blockDim.x) sd[q] = dtab[q];
+    const int nthr = blockDim.x;
+    constexpr double kInv2Pi = 1.0 / kTwoPi;
+    for (int q = threadIdx.x; q < NSP * nk; q += nthr) sf[q] = ftab[q];
+    for (int q = threadIdx.x; q < NSP * NSP * nk; q += nthr)
+        sd[q] = fma(dtab[q], kInv2Pi, -0.25);
 
-    // per-lane k constants
-    double kk[kKq], c1[kKq], c2[kKq], pref[kKq], c3k[kKq];
-    bool live[kKq];
+    const int ik = k0 + (int)threadIdx.x;
+    const bool live = ik < nk;
+    const int ikc = live ? ik : nk - 1;                     // a valid column for dead lanes
+    const double kv = live ? ktab[ik] : 0.0;
+    const double lv = live ? lamtab[ik] : 1.0;
+    const double kp = kv * kInvPi;                          // revolutions of 2 k r per unit r
+    const double c1 = -2.0 / lv;
+    const double c2 = -2.0 * kv * kv;
+    const double c3k = (4.0 / 3.0) * kv * kv * kv * kInv2Pi;
+    const double pref = (live && kv > 0.0) ? s02 / kv : 0.0;
+    double acc[NSP];
 #pragma unroll
-    for (int u = 0; u < kKq; ++u) {
-        int ik = threadIdx.x + u * blockDim.x;
-        live[u] = ik < nk;
-        double kv = live[u] ? ktab[ik] : 0.0;
-        double lv = live[u] ? lamtab[ik] : 1.0;
-        kk[u] = kv;
-        c1[u] = -2.0 / lv;
-        c2[u] = -2.0 * kv * kv;
-        pref[u] = (kv > 0.0) ? s02 / kv : 0.0;
-        c3k[u] = (4.0 / 3.0) * kv * kv * kv;
-    }
-    double acc[kKq][NSP];
-#pragma unroll
-    for (int u = 0; u < kKq; ++u)
-#pragma unroll
-        for (int s = 0; s < NSP; ++s) acc[u][s] = 0.0;
+    for (int s = 0; s < NSP; ++s) acc[s] = 0.0;
 
     const double *rc = r_all + (long)ic * npairs;
     const int *tc = pt_all + (long)ic * npairs;
     const int pbeg = w * pairs_per_wg;
     const int pend = min(npairs, pbeg + pairs_per_wg);
 
-    for (int p0 = pbeg; p0 < pend; p0 += kTile) {
+    for (int p0 = pbeg; p0 < pend; p0 += nthr) {
         __syncthreads();
-        stage_tile(tile, p0, pend, NSP, rc, tc, shell, sig2, drtab, c3tab, s2tab);
+        {
+            const int p = p0 + (int)threadIdx.x;
+            PairRec rec;
+            if (p < pend) {
+                const int t = tc[p];
+                const int sh = shell[p];
+                const double re = rc[p] + drtab[sh * NSP * NSP + t];
+                rec.r = re;
+                rec.lir2 = -2.0 * log(re);
+                rec.s2 = sig2[p] + s2tab[sh * NSP * NSP + t];
+                rec.c3 = c3tab[sh];
+                rec.t = t;
+            } else {
+                rec.r = 1.0;
+                rec.lir2 = -2000.0;   // e^-2000 = 0: contributes exactly 0
+                rec.s2 = 0.0;
+                rec.c3 = 0.0;
+                rec.t = 0;
+            }
+            rec.pad = 0;
+            tile[threadIdx.x] = rec;
+        }
         __syncthreads();
-        const int nt = min(kTile, pend - p0);
-        for (int q = 0; q < nt; ++q) {
-            const PairRec rec = tile[q];
-#pragma unroll
-            for (int u = 0; u < kKq; ++u) {
-                if (!live[u]) continue;
-                const int ik = threadIdx.x + u * blockDim.x;
-                const int sb = rec.t - rec.sa * NSP;
-                double v;
-                if (F32) {
-                    // phase reduced in FP64 (revolutions), everything else FP32
-                    double rev = kk[u] * kInvPi * rec.r;
-                    double fr = rev - rint(rev);
-                    float th = (float)(fr * kTwoPi) + (float)sd[rec.t * nk + ik] -
-                               (float)c3k[u] * (float)rec.c3;
-                    float e = expf((float)c1[u] * (float)rec.r + (float)c2[u] * (float)rec.s2);
-                    v = (double)((float)pref[u] * (float)sf[sb * nk + ik] * (float)rec.ir2 * e *
-                                 sinf(th));
-                } else {
-                    double e = exp(c1[u] * rec.r + c2[u] * rec.s2);
-                    double th = 2.0 * kk[u] * rec.r + sd[rec.t * nk + ik] - c3k[u] * rec.c3;
-                    v = pref[u] * sf[sb * nk + ik] * rec.ir2 * e * sin(th);
-                }
+        const int nt = __builtin_amdgcn_readfirstlane(min(nthr, pend - p0));
+        double run = 0.0;
+        int run_sa = __builtin_amdgcn_readfirstlane(tile[0].t) / NSP;
+        auto add = [&](int sa, double fe, double sn) {
+            if (sa != run_sa) {
 #pragma unroll
                 for (int s = 0; s < NSP; ++s)
-                    if (rec.sa == s) acc[u][s] += v;
+                    if (run_sa == s) acc[s] += run;
+                run = 0.0;
+                run_sa = sa;
+            }
+            run = fma(fe, sn, run);
+        };
+        // two pairs per step: their two polynomial chains interleave
+        int q = 0;
+        for (; q + 1 < nt; q += 2) {
+            const PairRec ra = tile[q], rb = tile[q + 1];
+            const int ta = __builtin_amdgcn_readfirstlane(ra.t);
+            const int tb = __builtin_amdgcn_readfirstlane(rb.t);
+            double fea, sna, feb, snb;
+            pair_term<NSP>(ra, ta, sf, sd, nk, ikc, kp, c1, c2, c3k, pc, fea, sna);
+            pair_term<NSP>(rb, tb, sf, sd, nk, ikc, kp, c1, c2, c3k, pc, feb, snb);
+            if (ta / NSP == run_sa && tb / NSP == run_sa) {    // both terms feed the run:
+                run = fma(fea, sna, run);                      // no branch between the two
+                run = fma(feb, snb, run);                      // evaluations
+            } else {
+                add(ta / NSP, fea, sna);
+                add(tb / NSP, feb, snb);
             }
         }
+        if (q < nt) {
+            const PairRec ra = tile[q];
+            const int ta = __builtin_amdgcn_readfirstlane(ra.t);
+            double fea, sna;
+            pair_term<NSP>(ra, ta, sf, sd, nk, ikc, kp, c1, c2, c3k, pc, fea, sna);
+            add(ta / NSP, fea, sna);
+        }
+#pragma unroll
+        for (int s = 0; s < NSP; ++s)
+            if (run_sa == s) acc[s] += run;
     }
     double *out = partial + ((long)ic * nwg + w) * NSP * nk;
+    for (int q = threadIdx.x; q < k0; q += nthr)
 #pragma unroll
-    for (int u = 0; u < kKq; ++u) {
-        int ik = threadIdx.x + u * blockDim.x;
-        if (!live[u]) continue;
+        for (int s = 0; s < NSP; ++s) out[s * nk + q] = 0.0;
+    if (live) {
 #pragma unroll
-        for (int s = 0; s < NSP; ++s) out[s * nk + ik] = acc[u][s];
+        for (int s = 0; s < NSP; ++s) out[s * nk + ik] = pref * acc[s];
     }
 }
 
@@ -570,50 +589,43 @@ template <int NSP>
 int launch_chi(chm_ctx *ctx, int nconf, int precision) {
     const int nk = ctx->nk;
     const int npairs = ctx->npairs;
-    int ntiles = (npairs + kTile - 1) / kTile;
-    int want = (2048 + nconf - 1) / nconf;  // ~2048 workgroups over the device
-    int nwg = std::max(1, std::min(ntiles, want));
-    int tiles_per = (ntiles + nwg - 1) / nwg;
-    int ppw = tiles_per * kTile;
+    // lanes over the k > 0 points only (the grid is ascending)
+    int k0 = 0;
+    while (k0 < nk && !(ctx->h_k[k0] > 0.0)) ++k0;
+    const int nkz = std::max(1, nk - k0);
+    CHM_REQUIRE(nkz <= kMaxK, CHM_E_ARG, "at most 512 k points above 0");
+    const int nthr = std::min(kMaxK, (nkz + 63) / 64 * 64);
+    // ~2048 workgroups over the device, whole staging rounds each
+    const int rounds = (npairs + nthr - 1) / nthr;
+    int nwg = std::max(1, std::min(rounds, (2048 + nconf - 1) / nconf));
+    const int ppw = ((rounds + nwg - 1) / nwg) * nthr;
     nwg = (npairs + ppw - 1) / ppw;
-    size_t lds = sizeof(double) * (NSP + NSP * NSP) * nk + sizeof(PairRec) * kTile;
-    CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
     int rc = ctx->partial.ensure(sizeof(double) * (size_t)nconf * nwg * NSP * nk);
     if (rc) return rc;
-    dim3 grid(nwg, nconf);
+    const dim3 grid(nwg, nconf);
     if (precision == CHM_PREC_F32) {
-        // lanes over the k > 0 points only (the grid is ascending)
-        int k0 = 0;
-        while (k0 < nk && !(ctx->h_k[k0] > 0.0)) ++k0;
-        const int nkz = std::max(1, nk - k0);
-        const int nthr = std::min(512, (nkz + 63) / 64 * 64);
-        CHM_REQUIRE(nkz <= 512, CHM_E_ARG, "at most 512 k points above 0");
-        int tiles = (npairs + nthr - 1) / nthr;
-        int nwg32 = std::max(1, std::min(tiles, (2048 + nconf - 1) / nconf));
-        const int ppw32 = ((tiles + nwg32 - 1) / nwg32) * nthr;
-        nwg32 = (npairs + ppw32 - 1) / ppw32;
-        if ((rc = ctx->partial.ensure(sizeof(double) * (size_t)nconf * nwg32 * NSP * nk))) return rc;
         const size_t words = (size_t)(NSP + NSP * NSP) * nk;
-        const size_t lds32 = sizeof(float) * ((words + 7) & ~(size_t)7) + sizeof(PairRecF) * nthr;
-        CHM_REQUIRE(lds32 <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+        const size_t lds = sizeof(float) * ((words + 7) & ~(size_t)7) + sizeof(PairRecF) * nthr;
+        CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
         hipFuncSetAttribute((const void *)k_chi_pairs_f32<NSP>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32);
-        k_chi_pairs_f32<NSP><<<dim3(nwg32, nconf), nthr, lds32, ctx->stream>>>(
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k_chi_pairs_f32<NSP><<<grid, nthr, lds, ctx->stream>>>(
             ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(),
             ctx->sig2.as<double>(), ctx->f32.as<float>(), ctx->drev32.as<float>(),
             ctx->lam.as<double>(), ctx->k.as<double>(), ctx->drtab.as<double>(),
             ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), ctx->partial.as<double>(), npairs,
-            nk, k0, ppw32, ctx->s02);
-        nwg = nwg32;
+            nk, k0, ppw, ctx->s02);
     } else {
-        hipFuncSetAttribute((const void *)k_chi_pairs<NSP, false>,
+        const size_t lds = sizeof(double) * (NSP + NSP * NSP) * nk + sizeof(PairRec) * nthr;
+        CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+        hipFuncSetAttribute((const void *)k_chi_pairs<NSP>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        k_chi_pairs<NSP, false><<<grid, 256, lds, ctx->stream>>>(
+        k_chi_pairs<NSP><<<grid, nthr, lds, ctx->stream>>>(
             ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(),
             ctx->sig2.as<double>(), ctx->f.as<double>(), ctx->delta.as<double>(),
             ctx->lam.as<double>(), ctx->k.as<double>(), ctx->drtab.as<double>(),
             ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), ctx->partial.as<double>(), npairs,
-            nk, ppw, ctx->s02);
+            nk, k0, ppw, ctx->s02, chm::kPolyCoefs);
     }
     CHM_LAUNCH_CHECK();
     long n = (long)nconf * NSP * nk;
@@ -675,7 +687,6 @@ extern "C" int chm_fm_chi(chm_ctx *ctx, int nconf, const double *pos, const int3
     CHM_REQUIRE(nconf > 0 && pos && species && chi, CHM_E_ARG, "bad arguments");
     CHM_REQUIRE(precision == CHM_PREC_F64 || precision == CHM_PREC_F32, CHM_E_ARG,
                 "precision must be CHM_PREC_F64 or CHM_PREC_F32");
-    CHM_REQUIRE(ctx->nk <= 256 * kKq, CHM_E_ARG, "at most 512 k points");
     int rc = check_species(ctx, nconf, species);
     if (rc) return rc;
     CHM_HIP(hipSetDevice(ctx->device));
