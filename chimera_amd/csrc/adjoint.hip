@@ -36,6 +36,7 @@ namespace {
 
 constexpr int kLanes = 256;
 constexpr int kMaxK = 512;   // k points above 0 in the soft forward (one lane each)
+constexpr int kSoftRegNsp = 3;   // soft forward: table columns in registers up to this
 constexpr double kInvPi = 0.31830988618379067154;
 constexpr double kC3Rev = (4.0 / 3.0) * 0.15915494309189533577;   // (4/3) / (2 pi)
 
@@ -200,18 +201,30 @@ __global__ __launch_bounds__(512) void k_chi_soft(
     const double *__restrict__ ktab, double *__restrict__ partial, int npairs, int natoms,
     int nk, int k0, int pairs_per_wg, double s02, const chm::PolyCoefs pc) {
     extern __shared__ double smem[];
+    // the lane's own column of f cos(delta), f sin(delta): in registers up to 3 species
+    // (18 values), else in LDS
+    constexpr bool kReg = NSP <= kSoftRegNsp;
     double *sfc = smem;
-    double *sfs = sfc + NSP * NSP * nk;
-    SoftRec<NSP> *tile = reinterpret_cast<SoftRec<NSP> *>(sfs + NSP * NSP * nk);
+    double *sfs = sfc + (kReg ? 0 : NSP * NSP * nk);
+    SoftRec<NSP> *tile = reinterpret_cast<SoftRec<NSP> *>(sfs + (kReg ? 0 : NSP * NSP * nk));
     const int ic = blockIdx.y, wg = blockIdx.x, nwg = gridDim.x;
     const int nthr = blockDim.x;
-    for (int q = threadIdx.x; q < NSP * NSP * nk; q += nthr) {
-        sfc[q] = fctab[q];
-        sfs[q] = fstab[q];
-    }
     const int ik = k0 + (int)threadIdx.x;
     const bool live = ik < nk;
     const int ikc = live ? ik : nk - 1;
+    double fcr[kReg ? NSP * NSP : 1], fsr[kReg ? NSP * NSP : 1];
+    if constexpr (kReg) {
+#pragma unroll
+        for (int q = 0; q < NSP * NSP; ++q) {
+            fcr[q] = fctab[q * nk + ikc];
+            fsr[q] = fstab[q * nk + ikc];
+        }
+    } else {
+        for (int q = threadIdx.x; q < NSP * NSP * nk; q += nthr) {
+            sfc[q] = fctab[q];
+            sfs[q] = fstab[q];
+        }
+    }
     const double kv = live ? ktab[ik] : 0.0;
     const double kp = kv * kInvPi;
     const double c1 = -2.0 / (live ? lamtab[ik] : 1.0);
@@ -262,7 +275,9 @@ __global__ __launch_bounds__(512) void k_chi_soft(
 #pragma unroll
                 for (int b = 0; b < NSP; ++b) {
                     const int t = (a * NSP + b) * nk + ikc;
-                    s += rec.wj[b] * (sn * sfc[t] + cs * sfs[t]);
+                    const double fc = kReg ? fcr[a * NSP + b] : sfc[t];
+                    const double fs = kReg ? fsr[a * NSP + b] : sfs[t];
+                    s += rec.wj[b] * (sn * fc + cs * fs);
                 }
                 acc[a] += rec.wi[a] * base * s;
             }
@@ -382,7 +397,8 @@ int launch_chi_soft(chm_ctx *ctx, int nconf) {
     const int nkz = std::max(1, nk - k0);
     CHM_REQUIRE(nkz <= kMaxK, CHM_E_ARG, "at most 512 k points above 0");
     const int nthr = std::min(kMaxK, (nkz + 63) / 64 * 64);
-    const size_t lds = sizeof(double) * 2 * NSP * NSP * nk + sizeof(SoftRec<NSP>) * nthr;
+    const size_t lds = sizeof(double) * (NSP <= kSoftRegNsp ? 0 : 2 * NSP * NSP * nk) +
+                       sizeof(SoftRec<NSP>) * nthr;
     CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
     const int rounds = (npairs + nthr - 1) / nthr;
     int nwg = std::max(1, std::min(rounds, (2048 + nconf - 1) / nconf));
