@@ -162,7 +162,7 @@ def learned_vs_uniform(P, S, args):
 def forward_bench(P, S, iters=10):
     """Full chi(k) evaluations of the replicas' configurations: the pair-space path sum
     (FP64 and FP32, with distances and pair types), its adjoint dL/dr, and the
-    relaxed-occupancy path sum, timed with HIP events on the forward model's stream.
+    relaxed-occupancy path sum and its adjoint (dL/dr, dL/dw), timed with HIP events on the forward model's stream.
     Work unit: one path term = one (pair, k) evaluation (exp, sin, amplitude)."""
     fm = P.fm
     sp = S.species
@@ -174,6 +174,7 @@ def forward_bench(P, S, iters=10):
     fm.grad_pos(pos, sp, g)
     w = np.eye(P.nsp)[sp]
     fm.chi_soft(pos, w)
+    fm.grad_soft(pos, w, g)
     terms = float(nconf) * fm.npairs * int(np.count_nonzero(P.k > 0))
     err = np.abs(c32 - c64)
     out = {"configs": nconf, "pairs": int(fm.npairs), "k_points": int(P.k.size),
@@ -181,7 +182,8 @@ def forward_bench(P, S, iters=10):
            "f32_vs_f64": {"max_abs": float(f"{err.max():.3g}"),
                           "max_rel": float(f"{err.max() / np.abs(c64).max():.3g}")}}
     for key, what in (("path_sum_f64", "chi"), ("path_sum_f32", "chi_f32"),
-                      ("adjoint_dr_f64", "adjoint"), ("soft_path_sum_f64", "soft")):
+                      ("adjoint_dr_f64", "adjoint"), ("soft_path_sum_f64", "soft"),
+                      ("soft_adjoint_f64", "soft_adjoint")):
         ms = fm.profile(what, nconf, iters)
         out[key] = {"ms": round(ms, 4), "terms_per_s": float(f"{terms / (ms * 1e-3):.4g}")}
     out["chi_evals_per_s_f32"] = round(nconf / (out["path_sum_f32"]["ms"] * 1e-3), 1)

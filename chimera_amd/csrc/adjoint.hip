@@ -1,9 +1,10 @@
 // Adjoints of the pair-space path sum and the relaxed-occupancy path sum (gfx950), and
 // HIP-event timing of the forward-model kernels.
 //
-// dL/dr (hard and soft occupancy): a workgroup owns 256 pairs of one configuration, one
-// pair per lane.  The per-k tables (f, delta or f cos(delta) / f sin(delta), k,
-// -2/lambda) and the configuration's dL/dchi are staged in LDS once; every lane walks
+// dL/dr (hard and soft occupancy): a workgroup owns 1024 pairs of one configuration, one
+// pair per lane (16 wavefronts share one copy of the staged tables).  The per-k tables
+// (f, delta or f cos(delta) / f sin(delta), k, -2/lambda) and the configuration's dL/dchi
+// are staged in LDS once; every lane walks
 // the k grid with its pair's accumulators in registers, so all 64 lanes of a wavefront
 // read the same k column of a table row (a handful of distinct LDS words per step).
 // dL/dpos and dL/dw: one lane per (configuration, atom) gathers its pairs through CSR
@@ -34,7 +35,7 @@ int reduce_partials(chm_ctx *ctx, const double *partial, double *out, int nwg, i
 
 namespace {
 
-constexpr int kLanes = 256;
+constexpr int kLanes = 1024;   // adjoint workgroups: pairs per workgroup (tables staged once)
 constexpr int kMaxK = 512;   // k points above 0 in the soft forward (one lane each)
 constexpr int kSoftRegNsp = 3;   // soft forward: table columns in registers up to this
 constexpr double kInvPi = 0.31830988618379067154;
@@ -79,7 +80,7 @@ int download(chm_ctx *ctx, void *h, const chm::DevBuf &b, size_t n) {
 
 // gr[ic][p] = sum_k gchi[ic][a_p][k] dT_p/dr (r_eff = r + dr, sigma^2 + s2tab, C3 phase)
 template <int NSP>
-__global__ __launch_bounds__(256) void k_adjoint_r(
+__global__ __launch_bounds__(1024) void k_adjoint_r(
     const double *__restrict__ r_all, const int *__restrict__ pt_all,
     const int *__restrict__ shell, const double *__restrict__ sig2,
     const double *__restrict__ ftab, const double *__restrict__ dtab,
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(512) void k_chi_soft(
 
 // Per pair: gr (dL/dr), gwi[s] (dL/dw_i[s]), gwj[s] (dL/dw_j[s]) of the soft path sum.
 template <int NSP>
-__global__ __launch_bounds__(256) void k_adjoint_soft(
+__global__ __launch_bounds__(1024) void k_adjoint_soft(
     const double *__restrict__ r_all, const double *__restrict__ w_all,
     const int *__restrict__ iidx, const int *__restrict__ jidx,
     const double *__restrict__ sig2, const double *__restrict__ fctab,
