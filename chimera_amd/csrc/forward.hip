@@ -443,7 +443,8 @@ __global__ __launch_bounds__(512) void k_chi_pairs(
 // k r / pi is formed exactly enough without FP64: k/pi and r are split into an 11-bit
 // head and a tail, head x head is exact in FP32 and its integer part is dropped exactly,
 // the three small cross products carry the rest (error ~1e-7 revolution at k r ~ 90).
-// Terms are summed in FP32 over one tile of 256 pairs and the tile sums in FP64.
+// Terms are summed in FP32 over each run of one absorber species within a staging round,
+// and the run sums in FP64.
 struct PairRecF {
     float rhi, rlo;   // r_eff = rhi + rlo, rhi with an 11-bit significand
     float r;          // r_eff
