@@ -129,6 +129,71 @@ __global__ __launch_bounds__(1024) void k_adjoint_r(
     gr[o] = acc;
 }
 
+// The same dL/dr on a uniform k grid with C3 = 0: e^{2ikr} advances along k by one
+// complex rotation e^{2i dk r} per point (4 FMAs) and is recomputed exactly every
+// kAnchorEvery points (and at the first), so no per-term sine or cosine is evaluated.
+// The tables are f_b cos(delta_t), f_b sin(delta_t) (one row per pair type):
+// f sin(2kr + delta) = s fc + c fs and f cos(2kr + delta) = c fc - s fs.
+constexpr int kAnchorEvery = 32;
+
+template <int NSP>
+__global__ __launch_bounds__(1024) void k_adjoint_r_rot(
+    const double *__restrict__ r_all, const int *__restrict__ pt_all,
+    const int *__restrict__ shell, const double *__restrict__ sig2,
+    const double *__restrict__ fctab, const double *__restrict__ fstab,
+    const double *__restrict__ lamtab, const double *__restrict__ ktab,
+    const double *__restrict__ drtab, const double *__restrict__ s2tab,
+    const double *__restrict__ gchi, double *__restrict__ gr, int npairs, int nk, double dkp,
+    double s02, const chm::PolyCoefs pc) {
+    extern __shared__ double smem[];
+    double *sfc = smem;                  // f_b cos(delta_t) (NSP*NSP, nk)
+    double *sfs = sfc + NSP * NSP * nk;  // f_b sin(delta_t)
+    double *sg = sfs + NSP * NSP * nk;   // this configuration's dL/dchi (NSP, nk)
+    KRow *sk = reinterpret_cast<KRow *>(sg + NSP * nk);
+    const int ic = blockIdx.y;
+    for (int q = threadIdx.x; q < NSP * NSP * nk; q += kLanes) {
+        sfc[q] = fctab[q];
+        sfs[q] = fstab[q];
+    }
+    for (int q = threadIdx.x; q < NSP * nk; q += kLanes) sg[q] = gchi[(long)ic * NSP * nk + q];
+    stage_krows(sk, ktab, lamtab, nk, s02);
+    __syncthreads();
+    const int p = blockIdx.x * kLanes + threadIdx.x;
+    if (p >= npairs) return;
+    const long o = (long)ic * npairs + p;
+    const int t = pt_all[o];
+    const int sh = shell[p];
+    const int sa = t / NSP;
+    const double re = r_all[o] + drtab[sh * NSP * NSP + t];
+    const double ir = 1.0 / re;
+    const double ir2 = ir * ir;
+    const double s2 = sig2[p] + s2tab[sh * NSP * NSP + t];
+    const double *gs = sg + sa * nk;
+    const double *fc = sfc + t * nk;
+    const double *fs = sfs + t * nk;
+    double rs, rc;                       // e^{2i dk r}
+    chm::sincos2pi(dkp * re, pc, &rs, &rc);
+    double s = 0.0, c = 1.0;             // e^{2ikr}
+    double acc = 0.0;
+    for (int ik = 0; ik < nk; ++ik) {
+        const KRow kr = sk[ik];
+        if (ik % kAnchorEvery == 0) {
+            chm::sincos2pi(kr.kp * re, pc, &s, &c);
+        } else {
+            const double c2 = fma(c, rc, -s * rs);
+            s = fma(s, rc, c * rs);
+            c = c2;
+        }
+        const double g = gs[ik];
+        if (g == 0.0 || kr.pref == 0.0) continue;   // the k = 0 term is defined as 0
+        const double fsn = fma(s, fc[ik], c * fs[ik]);    // f sin(2kr + delta)
+        const double fcs = fma(c, fc[ik], -s * fs[ik]);   // f cos(2kr + delta)
+        const double amp = kr.pref * ir2 * chm::exp_fin(fma(kr.c1, re, kr.c2 * s2), pc);
+        acc += g * amp * fma(kr.c1 - 2.0 * ir, fsn, kr.tk * fcs);
+    }
+    gr[o] = acc;
+}
+
 // gpos[ic][a] = sum_{p: i_p = a} -gr_p e_p + sum_{p: j_p = a} gr_p e_p; with gwi/gwj also
 // gw[ic][a][s] = sum_{p: i_p = a} gwi_p[s] + sum_{p: j_p = a} gwj_p[s].
 __global__ void k_gather_atoms(const double *__restrict__ gr, const double *__restrict__ evec,
@@ -373,14 +438,48 @@ size_t adjoint_lds(int nsp, int nk, bool soft) {
                 : sizeof(double) * (nsp + nsp * nsp + nsp) * nk + sizeof(KRow) * nk;
 }
 
+int soft_tables(chm_ctx *ctx);
+
+// k grid uniform to rounding (|k_i - k_0 - i dk| <= 8 eps max|k|): the rotation
+// recurrence then tracks 2kr to ~1e-13 rad; returns dk / pi, or 0 when it is not.
+double uniform_dkp(const chm_ctx *ctx) {
+    const int nk = ctx->nk;
+    if (nk < 2) return 0.0;
+    const std::vector<double> &k = ctx->h_k;
+    const double dk = (k[nk - 1] - k[0]) / (nk - 1);
+    double kmax = 0.0;
+    for (double v : k) kmax = std::max(kmax, std::fabs(v));
+    const double tol = 8.0 * 2.220446049250313e-16 * kmax;
+    if (!(dk > 0.0)) return 0.0;
+    for (int i = 0; i < nk; ++i)
+        if (std::fabs(k[i] - (k[0] + i * dk)) > tol) return 0.0;
+    return dk * kInvPi;
+}
+
 template <int NSP>
 int launch_adjoint_r(chm_ctx *ctx, int nconf) {
     const int nk = ctx->nk, npairs = ctx->npairs;
+    const dim3 grid((npairs + kLanes - 1) / kLanes, nconf);
+    const double dkp = uniform_dkp(ctx);
+    const size_t lds_rot = sizeof(double) * (2 * NSP * NSP + NSP) * nk + sizeof(KRow) * nk;
+    if (ctx->c3_zero && dkp > 0.0 && lds_rot <= 80 * 1024) {
+        int rc = soft_tables(ctx);       // f cos(delta), f sin(delta)
+        if (rc) return rc;
+        hipFuncSetAttribute((const void *)k_adjoint_r_rot<NSP>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rot);
+        k_adjoint_r_rot<NSP><<<grid, kLanes, lds_rot, ctx->stream>>>(
+            ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(),
+            ctx->sig2.as<double>(), ctx->soft_fc.as<double>(), ctx->soft_fs.as<double>(),
+            ctx->lam.as<double>(), ctx->k.as<double>(), ctx->drtab.as<double>(),
+            ctx->s2tab.as<double>(), ctx->gchi.as<double>(), ctx->gr.as<double>(), npairs, nk,
+            dkp, ctx->s02, chm::kPolyCoefs);
+        CHM_LAUNCH_CHECK();
+        return CHM_OK;
+    }
     const size_t lds = adjoint_lds(NSP, nk, false);
     CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
     hipFuncSetAttribute((const void *)k_adjoint_r<NSP>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds);
-    dim3 grid((npairs + kLanes - 1) / kLanes, nconf);
     k_adjoint_r<NSP><<<grid, kLanes, lds, ctx->stream>>>(
         ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(), ctx->sig2.as<double>(),
         ctx->f.as<double>(), ctx->delta.as<double>(), ctx->lam.as<double>(), ctx->k.as<double>(),

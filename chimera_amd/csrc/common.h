@@ -84,6 +84,7 @@ struct chm_ctx {
     int natoms = 0, npairs = 0, nshell = 1;
     chm::DevBuf i_idx, j_idx, shift, shell, sig2;
     chm::DevBuf drtab, c3tab, s2tab;
+    bool c3_zero = true;   // every shell's C3 is 0 (the adjoint may use its k recurrence)
     // pairs of each atom as absorber / as scatterer (CSR, pair order), for the adjoints
     chm::DevBuf csr_ioff, csr_i, csr_joff, csr_j;
     // scratch for the forward path

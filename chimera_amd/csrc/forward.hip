@@ -114,6 +114,8 @@ extern "C" int chm_fm_set_cumulants(chm_ctx *ctx, int nshell, const double *drta
     if ((rc = upload(ctx, ctx->s2tab, s2tab, n3))) return rc;
     if ((rc = upload(ctx, ctx->c3tab, c3tab, sizeof(double) * nshell))) return rc;
     ctx->nshell = nshell;
+    ctx->c3_zero = true;
+    for (int q = 0; q < nshell; ++q) ctx->c3_zero = ctx->c3_zero && c3tab[q] == 0.0;
     CHM_HIP(hipStreamSynchronize(ctx->stream));
     return CHM_OK;
 }
@@ -168,6 +170,7 @@ extern "C" int chm_fm_set_topology(chm_ctx *ctx, int natoms, int npairs, const i
     if ((rc = upload(ctx, ctx->drtab, z3.data(), sizeof(double) * z3.size()))) return rc;
     if ((rc = upload(ctx, ctx->s2tab, z3.data(), sizeof(double) * z3.size()))) return rc;
     if ((rc = upload(ctx, ctx->c3tab, z1.data(), sizeof(double) * z1.size()))) return rc;
+    ctx->c3_zero = true;
     CHM_HIP(hipStreamSynchronize(ctx->stream));
     return CHM_OK;
 }

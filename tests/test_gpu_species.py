@@ -1,7 +1,9 @@
 """The forward-model kernels at every species count they are built for (2..6): FP64 and
 FP32 path sums, dL/dpos, the relaxed-occupancy path sum and its adjoint, through the C ABI
 against the FP64 oracle, on a small fcc cell (two shells) with per-pair sigma^2 and
-cumulant tables; also a k grid without a k = 0 point (every lane live)."""
+cumulant tables; also a k grid without a k = 0 point (every lane live) and a non-uniform
+one.  dL/dpos is checked with C3 != 0 (per-term sine and cosine) and with C3 = 0, where a
+uniform grid whose tables fit takes the k-rotation recurrence."""
 import sys
 
 import numpy as np
@@ -34,10 +36,16 @@ def _model(nsp, k):
 
 
 @pytest.mark.parametrize("nsp,kind", [(2, "grid"), (4, "grid"), (5, "grid"), (6, "grid"),
-                                      (2, "no_zero"), (4, "no_zero")])
+                                      (2, "no_zero"), (4, "no_zero"), (3, "nonuniform")])
 def test_forward_kernels_all_species_counts(nsp, kind):
     from chimera_amd.scattering import k_grid
-    k = k_grid(16.0, 0.1) if kind == "grid" else np.linspace(0.5, 18.0, 320)
+    if kind == "grid":
+        k = k_grid(16.0, 0.1)
+    elif kind == "no_zero":
+        k = np.linspace(0.5, 18.0, 320)
+    else:
+        k = np.linspace(0.5, 18.0, 320)
+        k = k + 0.02 * np.sin(k)            # increasing, not uniform
     _say(f"[test] forward kernels nsp={nsp} k={kind} ({k.size} points)")
     fm, cell, i, j, sh, shell, tab, sig2 = _model(nsp, k)
     rng = np.random.default_rng(10 + nsp)
@@ -64,7 +72,13 @@ def test_forward_kernels_all_species_counts(nsp, kind):
         rg = OF.grad_pos(*oargs, pos[c], sp[c], i, j, sh, sig2, g[c], shell=shell, dr=dr, c3=c3,
                          s2types=s2t)
         assert np.abs(gp[c] - rg).max() <= 1e-11 * np.abs(rg).max()
-    fm.set_dr(np.zeros_like(dr)).set_c3(np.zeros_like(c3)).set_sig2_types(np.zeros_like(s2t))
+    fm.set_c3(np.zeros_like(c3))
+    gp0 = fm.grad_pos(pos, sp, g, normalise=False)
+    for c in range(2):
+        rg = OF.grad_pos(*oargs, pos[c], sp[c], i, j, sh, sig2, g[c], shell=shell, dr=dr,
+                         s2types=s2t)
+        assert np.abs(gp0[c] - rg).max() <= 1e-11 * np.abs(rg).max()
+    fm.set_dr(np.zeros_like(dr)).set_sig2_types(np.zeros_like(s2t))
 
     # relaxed occupancy
     w = rng.dirichlet(np.ones(nsp), (2, cell.natoms))
