@@ -368,7 +368,7 @@ __global__ __launch_bounds__(1024) void k_adjoint_soft(
     const double *__restrict__ fstab, const double *__restrict__ lamtab,
     const double *__restrict__ ktab, const double *__restrict__ gchi,
     double *__restrict__ gr, double *__restrict__ gwi_out, double *__restrict__ gwj_out,
-    int npairs, int natoms, int nk, double s02, const chm::PolyCoefs pc) {
+    int npairs, int natoms, int nk, double dkp, double s02, const chm::PolyCoefs pc) {
     extern __shared__ double smem[];
     double *sfc = smem;
     double *sfs = sfc + NSP * NSP * nk;
@@ -398,13 +398,21 @@ __global__ __launch_bounds__(1024) void k_adjoint_soft(
         gwi[s] = gwj[s] = 0.0;
     }
     double acc = 0.0;
+    // uniform k grid (dkp > 0): e^{2ikr} by the rotation recurrence of k_adjoint_r_rot
+    double rs = 0.0, rc = 1.0, sn = 0.0, cs = 1.0;
+    if (dkp > 0.0) chm::sincos2pi(dkp * re, pc, &rs, &rc);
     for (int ik = 0; ik < nk; ++ik) {
         const KRow kr = sk[ik];
+        if (dkp == 0.0 || ik % kAnchorEvery == 0) {
+            chm::sincos2pi(kr.kp * re, pc, &sn, &cs);
+        } else {
+            const double c2 = fma(cs, rc, -sn * rs);
+            sn = fma(sn, rc, cs * rs);
+            cs = c2;
+        }
         if (kr.pref == 0.0) continue;
         const double base = kr.pref * ir2 * chm::exp_fin(fma(kr.c1, re, kr.c2 * s2), pc);
         const double dfac = kr.c1 - 2.0 * ir;
-        double sn, cs;
-        chm::sincos2pi(kr.kp * re, pc, &sn, &cs);
 #pragma unroll
         for (int a = 0; a < NSP; ++a) {
             const double ga = sg[a * nk + ik];
@@ -531,8 +539,8 @@ int launch_adjoint_soft(chm_ctx *ctx, int nconf) {
         ctx->r.as<double>(), ctx->w.as<double>(), ctx->i_idx.as<int>(), ctx->j_idx.as<int>(),
         ctx->sig2.as<double>(), ctx->soft_fc.as<double>(), ctx->soft_fs.as<double>(),
         ctx->lam.as<double>(), ctx->k.as<double>(), ctx->gchi.as<double>(), ctx->gr.as<double>(),
-        ctx->gwi.as<double>(), ctx->gwj.as<double>(), npairs, ctx->natoms, nk, ctx->s02,
-        chm::kPolyCoefs);
+        ctx->gwi.as<double>(), ctx->gwj.as<double>(), npairs, ctx->natoms, nk, uniform_dkp(ctx),
+        ctx->s02, chm::kPolyCoefs);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
