@@ -413,23 +413,36 @@ __global__ __launch_bounds__(1024) void k_adjoint_soft(
         if (kr.pref == 0.0) continue;
         const double base = kr.pref * ir2 * chm::exp_fin(fma(kr.c1, re, kr.c2 * s2), pc);
         const double dfac = kr.c1 - 2.0 * ir;
+        // with A_a = sum_b w_j[b] fc_ab, B_a = sum_b w_j[b] fs_ab the pair's weighted
+        // f sin / f cos over b are sn A + cs B and cs A - sn B; with u_a = g_a w_i[a],
+        // C_b = sum_a u_a fc_ab, D_b = sum_a u_a fs_ab the w_j gradient is
+        // base (sn C_b + cs D_b)
+        double C[NSP], D[NSP];
+#pragma unroll
+        for (int b = 0; b < NSP; ++b) C[b] = D[b] = 0.0;
+        double accv = 0.0;
 #pragma unroll
         for (int a = 0; a < NSP; ++a) {
             const double ga = sg[a * nk + ik];
-            if (ga == 0.0) continue;
-            double st = 0.0, dt = 0.0;
+            const double u = ga * wi[a];
+            double A = 0.0, B = 0.0;
 #pragma unroll
             for (int b = 0; b < NSP; ++b) {
                 const int t = (a * NSP + b) * nk + ik;
-                const double snv = sn * sfc[t] + cs * sfs[t];   // f sin(2kr + delta)
-                const double csv = cs * sfc[t] - sn * sfs[t];   // f cos(2kr + delta)
-                st += wj[b] * snv;
-                dt += wj[b] * csv;
-                gwj[b] += ga * wi[a] * base * snv;
+                const double fc = sfc[t], fs = sfs[t];
+                A = fma(wj[b], fc, A);
+                B = fma(wj[b], fs, B);
+                C[b] = fma(u, fc, C[b]);
+                D[b] = fma(u, fs, D[b]);
             }
-            gwi[a] += ga * base * st;
-            acc += ga * wi[a] * base * (dfac * st + kr.tk * dt);
+            const double st = fma(sn, A, cs * B);    // sum_b w_j[b] f sin(2kr + delta_ab)
+            const double dt = fma(cs, A, -sn * B);   // sum_b w_j[b] f cos(2kr + delta_ab)
+            gwi[a] = fma(ga * base, st, gwi[a]);
+            accv = fma(u, fma(dfac, st, kr.tk * dt), accv);
         }
+        acc = fma(base, accv, acc);
+#pragma unroll
+        for (int b = 0; b < NSP; ++b) gwj[b] = fma(base, fma(sn, C[b], cs * D[b]), gwj[b]);
     }
     gr[o] = acc;
 #pragma unroll
