@@ -335,17 +335,18 @@ __global__ __launch_bounds__(512) void k_chi_soft(
             const double base = chm::exp_fin(fma(c1, rec.r, fma(c2, rec.s2, rec.lir2)), pc);
             double sn, cs;
             chm::sincos2pi(kp * rec.r, pc, &sn, &cs);
+            // sum_b w_j[b] f sin(2kr + delta_ab) = sn A_a + cs B_a with
+            // A_a = sum_b w_j[b] fc_ab, B_a = sum_b w_j[b] fs_ab
 #pragma unroll
             for (int a = 0; a < NSP; ++a) {
-                double s = 0.0;
+                double A = 0.0, B = 0.0;
 #pragma unroll
                 for (int b = 0; b < NSP; ++b) {
                     const int t = (a * NSP + b) * nk + ikc;
-                    const double fc = kReg ? fcr[a * NSP + b] : sfc[t];
-                    const double fs = kReg ? fsr[a * NSP + b] : sfs[t];
-                    s += rec.wj[b] * (sn * fc + cs * fs);
+                    A = fma(rec.wj[b], kReg ? fcr[a * NSP + b] : sfc[t], A);
+                    B = fma(rec.wj[b], kReg ? fsr[a * NSP + b] : sfs[t], B);
                 }
-                acc[a] += rec.wi[a] * base * s;
+                acc[a] = fma(rec.wi[a] * base, fma(sn, A, cs * B), acc[a]);
             }
         }
     }
