@@ -24,15 +24,39 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MIN_ESS_STEPS = 2000          # shorter alpha series give no usable autocorrelation estimate
 FP64_PEAK_TFLOPS = 78.6
 
 
-def cpu_baseline(P, seconds=15.0):
+def host_cores():
+    """(cores this process may run on, cores the host has, the host CPU's model name)."""
+    import subprocess
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    model = ""
+    try:
+        model = next((ln.split(":", 1)[1].strip() for ln in
+                      subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines()
+                      if ln.startswith("Model name")), "")
+    except OSError:
+        pass
+    return usable, os.cpu_count() or usable, model
+
+
+# the host cores that go with one GPU on a shared node: worker pools are sized to this share
+CPU_SHARE = 16
+
+
+def cpu_baseline(P, seconds=15.0, warmup=200):
     """The oracle's plain sampler (the reference algorithm step for step, FP64 NumPy) on this
     host, in the two arrangements of the baseline protocol: (i) one process with every
     core's BLAS threads running all replicas; (ii) one single-threaded process per core,
     each running its share of the replicas (replicas are independent between exchanges).
-    Each runs a bounded sample of about `seconds`; the better is the baseline.  The workers
+    Each process makes `warmup` untimed steps, then a bounded timed sample of about
+    `seconds`; the better arrangement is the baseline.  The cores used are the host's
+    share for one GPU (CPU_SHARE, or fewer if this process may use fewer).  The workers
     are separate Python processes that import only NumPy and the oracle (no GPU)."""
     import pickle
     import subprocess
@@ -40,7 +64,8 @@ def cpu_baseline(P, seconds=15.0):
     from oracle.build import problem_from
     O = problem_from(P)
     nrep = P.sampler.nrep
-    cores = max(1, min(16, os.cpu_count() or 1))       # the host's share for one GPU
+    usable, total, model = host_cores()
+    cores = max(1, min(CPU_SHARE, usable))
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "problem.pkl")
         with open(path, "wb") as fh:
@@ -50,8 +75,8 @@ def cpu_baseline(P, seconds=15.0):
             env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads),
                        MKL_NUM_THREADS=str(threads))
             procs = [subprocess.Popen([sys.executable, "-m", "oracle.cpu_worker", path,
-                                       str(seconds), str(r), str(1 + q)], cwd=ROOT, env=env,
-                                      stdout=subprocess.PIPE, text=True)
+                                       str(seconds), str(r), str(1 + q), str(warmup)], cwd=ROOT,
+                                      env=env, stdout=subprocess.PIPE, text=True)
                      for q, r in enumerate(reps)]
             outs = []
             for pr in procs:
@@ -65,20 +90,22 @@ def cpu_baseline(P, seconds=15.0):
         nproc = min(cores, nrep)
         reps = [nrep // nproc + (1 if q < nrep % nproc else 0) for q in range(nproc)]
         v2, o2 = arrangement(1, reps)
-    lscpu = ""
-    try:
-        lscpu = next((ln.split(":", 1)[1].strip() for ln in
-                      subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines()
-                      if ln.startswith("Model name")), "")
-    except OSError:
-        pass
     best = max((v1, cores, "one process, %d BLAS threads, %d replicas" % (cores, nrep)),
                (v2, nproc, "%d single-threaded processes, %s replicas each" % (nproc, "/".join(map(str, sorted(set(reps)))))))
     return {"value": best[0], "unit": "replica swap moves/s", "cores": best[1], "kind": "port",
+            "host": {"cpu_model": model, "host_cores": total, "usable_cores": usable,
+                     "cores_per_gpu_share": CPU_SHARE, "numpy": np.__version__,
+                     "threads_env": "OMP_NUM_THREADS=OPENBLAS_NUM_THREADS=MKL_NUM_THREADS set per arrangement"},
+            "arrangements": {"one_process_blas_threads": {"threads": cores, "moves_per_s": round(v1, 1),
+                                                          "steps": o1[0]["steps"], "warmup": o1[0]["warmup"]},
+                             "processes_single_thread": {"processes": nproc, "moves_per_s": round(v2, 1),
+                                                         "steps": o2[0]["steps"], "warmup": o2[0]["warmup"]}},
             "sample": (f"{best[2]}; (i) 1 process x {cores} threads: {v1:.1f} moves/s over "
                        f"{o1[0]['steps']} steps, (ii) {nproc} processes x 1 thread: {v2:.1f} moves/s "
-                       f"over {o2[0]['steps']} steps each (about {seconds:.0f} s per arrangement, same "
-                       f"workload, FP64 NumPy oracle of the reference algorithm, host {lscpu})")}
+                       f"over {o2[0]['steps']} steps each, each after {warmup} untimed warm-up steps "
+                       f"(about {seconds:.0f} s timed per arrangement, same workload, FP64 NumPy oracle "
+                       f"of the reference algorithm; host {model}, {total} cores, {usable} usable, "
+                       f"{CPU_SHARE} per GPU)")}
 
 
 def ess_run(P, S, burn, steps, exchange_every, setup):
@@ -89,9 +116,10 @@ def ess_run(P, S, burn, steps, exchange_every, setup):
     from chimera_amd.ess import ess
     S.init_state(P.cfg0, P.cell.ideal)
     setup()
+    # recording is set before the burn-in, whose first run captures the step graphs
+    S.set_recording(log_steps=0, series_steps=burn + steps)
     S.run(burn, exchange_every=exchange_every, log_every=10 ** 9)
     a0 = S.accepted_per_rung().copy()
-    S.set_recording(log_steps=0, series_steps=steps)
     P.fm.ctx.sync()
     t0 = time.perf_counter()
     S.run(steps, exchange_every=exchange_every, log_every=10 ** 9)
@@ -99,7 +127,7 @@ def ess_run(P, S, burn, steps, exchange_every, setup):
     dt = time.perf_counter() - t0
     cr = int(S.collect_rungs()[0])
     acc = (S.accepted_per_rung() - a0) / steps
-    al = S.alpha_series(steps, shell=0)
+    al = S.alpha_series(burn + steps, shell=0)[burn:]
     iu = np.triu_indices(P.nsp, 1)
     e = min(ess(al[:, a, b]) for a, b in zip(*iu))
     bsp, _, bc2 = S.best()
@@ -279,6 +307,7 @@ def main():
     ap.add_argument("--workload", default="fcc_crconi_4000")
     ap.add_argument("--exchange-every", type=int, default=25)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-warmup", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=200)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "f64", "f32"])
@@ -353,9 +382,10 @@ def main():
     print(f"[bench] tuned: {tuned}", file=sys.stderr, flush=True)
     print(f"[bench] built in {time.time() - t_build:.1f} s; warmup {args.warmup}",
           file=sys.stderr, flush=True)
+    # pair counts of the likelihood-temperature rung after every warm-up and timed step (the
+    # alpha series); set before the warm-up, whose first run captures the step graphs
+    S.set_recording(log_steps=0, series_steps=args.warmup + args.steps)
     S.run(args.warmup, exchange_every=args.exchange_every, log_every=10 ** 9)
-    # pair counts of the likelihood-temperature rung after every timed step (alpha series)
-    S.set_recording(log_steps=0, series_steps=args.steps)
     P.fm.ctx.sync()
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -415,15 +445,17 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(P, args.cpu_seconds)
+        cpu = cpu_baseline(P, args.cpu_seconds, args.cpu_warmup)
 
-    # alpha_1 ESS/s of the likelihood-temperature rung over the timed steps, and the
-    # R-factor of the best state (reported beside the rate)
+    # alpha_1 ESS/s of the likelihood-temperature rung over the timed steps (only when the
+    # series is long enough for the autocorrelation estimate), and the R-factor of the best
+    # state (reported beside the rate)
     from chimera_amd.ess import ess
-    al = S.alpha_series(args.steps, shell=0)
+    al = S.alpha_series(args.warmup + args.steps, shell=0)[args.warmup:]
     iu = np.triu_indices(P.nsp, 1)
-    ess_pairs = [ess(al[:, a, b]) for a, b in zip(*iu)]
-    ess_min = float(min(ess_pairs))
+    ess_min = None
+    if args.steps >= MIN_ESS_STEPS:
+        ess_min = float(min(ess(al[:, a, b]) for a, b in zip(*iu)))
     bsp, _, bc2 = S.best()
     cnt = np.bincount(bsp, minlength=P.nsp)
     chib = S.chi_k_of(bsp[None])[0] / np.maximum(cnt, 1)[:, None]
@@ -460,8 +492,8 @@ def main():
         "gpu_over_cpu": None if cpu is None else round(moves / cpu["value"], 1),
         "best_chi2": round(float(bc2), 5),
         "best_r_factor": float(f"{rfac:.4g}"),
-        "alpha1_ess": round(ess_min, 1),
-        "alpha1_ess_per_s": round(ess_min / dt, 2),
+        "alpha1_ess": None if ess_min is None else round(ess_min, 1),
+        "alpha1_ess_per_s": None if ess_min is None else round(ess_min / dt, 2),
         "alpha1_mean": {f"{a}{b}": round(float(al[:, a, b].mean()), 5) for a, b in zip(*iu)},
         "fp64_reanchors": nanchor,
         "forward_model": fwd,

@@ -78,6 +78,12 @@ _SIGS = {
     "chm_sampler_set_tuning": (C.c_int, [vp, C.c_int]),
     "chm_sampler_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "chm_sampler_set_pairs": (C.c_int, [vp, C.c_int]),
+    "chm_sampler_reserve_marks": (C.c_int, [vp, C.c_int]),
+    "chm_sampler_mark": (C.c_int, [vp, C.c_int]),
+    "chm_sampler_read_marks": (C.c_int, [vp, C.c_int, f64p]),
+    "chm_sampler_state_size": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+    "chm_sampler_export": (C.c_int, [vp, vp, C.c_int64]),
+    "chm_sampler_import": (C.c_int, [vp, vp, C.c_int64]),
     "chm_lp_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                               C.POINTER(C.c_int)]),
     "chm_lp_set_params": (C.c_int, [vp, vp]),

@@ -795,7 +795,7 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
                 if (pp[4]) {
                     // Metropolis-Hastings: pp[5] holds log q(x'->x) - log q(x->x')
                     const double x = beta * (c2t - c2p) * A.nfit - (double)__int_as_float(pp[5]);
-                    acc = (x <= 0.0) || (u_acc < exp(-fmin(x, 60.0)));
+                    acc = metropolis(x, u_acc);
                 }
                 if (acc) {
                     lsp[pp[0]] = (uint8_t)pp[2];
@@ -930,8 +930,24 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
                     A.dc_prop + (long)s * g.nrow_w);
 }
 
-__global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A) {
-    slot_step_body(g, A, blockIdx.y, blockIdx.x, gridDim.x);
+// Workgroup -> (replica slot s, split) such that all nsplit workgroups of slot s run on
+// XCD s mod 8: consecutive workgroup ids are dispatched to the eight XCDs in turn, so ids
+// congruent mod 8 share one XCD and its L2, and the slot's species, tile sums, counts and
+// path records are fetched into one L2 instead of eight.  1-D grid of
+// nsplit * ceil(nrep / 8) * 8 workgroups; the ones past the last slot exit at once.
+__device__ __forceinline__ bool xcd_slot(int nrep, int nsplit, int &s, int &split) {
+    const int L = blockIdx.x;
+    const int j = L >> 3;
+    const int m = j / nsplit;
+    split = j - m * nsplit;
+    s = (L & 7) + 8 * m;
+    return s < nrep;
+}
+
+__global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A, int nsplit) {
+    int s, split;
+    if (!xcd_slot(A.nrep, nsplit, s, split)) return;
+    slot_step_body(g, A, s, split, nsplit);
 }
 
 // ---------------------------------------------------------------- paired steps
@@ -1147,7 +1163,7 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
             for (int q = 0; q < kPropInts; ++q) p1[q] = pr[0][q];
             if (p1[4]) {
                 const double x = beta * (c2_1 - c2p) * A.nfit - (double)__int_as_float(p1[5]);
-                acc1 = (x <= 0.0) || (u1 < exp(-fmin(x, 60.0)));
+                acc1 = metropolis(x, u1);
             }
             const double cur1 = acc1 ? c2_1 : c2p;
             int q2[kPropInts];
@@ -1156,7 +1172,7 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
             const double c2_2 = acc1 ? c2_2a : c2_2r;
             if (q2[4]) {
                 const double x = beta * (c2_2 - cur1) * A.nfit - (double)__int_as_float(q2[5]);
-                acc2 = (x <= 0.0) || (u2 < exp(-fmin(x, 60.0)));
+                acc2 = metropolis(x, u2);
             }
             const double cur2 = acc2 ? c2_2 : cur1;
             if (acc1) {
@@ -1376,8 +1392,10 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
     }
 }
 
-__global__ __launch_bounds__(256) void k_slot_pair(Geo g, PairArgs A) {
-    slot_pair_body(g, A, blockIdx.y, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(256) void k_slot_pair(Geo g, PairArgs A, int nsplit) {
+    int s, split;
+    if (!xcd_slot(A.nrep, nsplit, s, split)) return;
+    slot_pair_body(g, A, s, split, nsplit);
 }
 
 constexpr int kPairKB = 3;   // K blocks per wave in the paired transform
@@ -1446,7 +1464,7 @@ __device__ __forceinline__ void exchange_body(int nrep, const double *__restrict
         int si = slot_of[i], sj = slot_of[i + 1];
         double d = (beta[i] - beta[i + 1]) * (c2[si] - c2[sj]) * nfit;
         double u = u01(splitmix64(hx ^ ((uint64_t)(i + 1) << 32)));
-        bool ok = (d >= 0.0) || (u < exp(fmax(d, -60.0)));
+        bool ok = metropolis(-d, u);
         att[i] += 1;
         if (ok) {
             acc[i] += 1;
@@ -1587,7 +1605,8 @@ int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanch
     A.c2_prev = c2_prev ? c2_prev : c2row(S, n - 1);
     A.draws = (accept == 1 && S->incremental == 1) ? S->draws.as<int>() : nullptr;   // see launch_gemm_delta
     A.c2_next = c2row(S, n);
-    k_slot_step<<<dim3(S->nsplit, S->nrep), 256, S->lds_k1, S->ctx->stream>>>(S->g, A);
+    k_slot_step<<<dim3(S->nsplit * ((S->nrep + 7) & ~7)), 256, S->lds_k1, S->ctx->stream>>>(S->g, A,
+                                                                                             S->nsplit);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -1727,7 +1746,8 @@ int launch_slot_pair(chm_sampler *S, int accept, int propose, int k, const doubl
     A.gr2 = S->gr2;
     A.nfit = S->nfit;
     A.draws = accept ? S->draws.as<int>() : nullptr;   // made by launch_gemm_pair just before
-    k_slot_pair<<<dim3(S->nsplit, S->nrep), 256, S->lds_pair, S->ctx->stream>>>(g, A);
+    k_slot_pair<<<dim3(S->nsplit * ((S->nrep + 7) & ~7)), 256, S->lds_pair, S->ctx->stream>>>(g, A,
+                                                                                              S->nsplit);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -2204,7 +2224,7 @@ extern "C" int chm_sampler_run(chm_sampler *S, int nsteps, int exchange_every) {
             if (E > 0 && S->hstep % E == 0 && (rc = launch_exchange(S, S->hstep))) return rc;
         }
     }
-    CHM_HIP(hipStreamSynchronize(st));
+    // no host sync: every read and every state change synchronises the stream first
     return CHM_OK;
 }
 
@@ -2589,6 +2609,220 @@ extern "C" int chm_sampler_set_pairs(chm_sampler *S, int on) {
     CHM_HIP(hipStreamSynchronize(S->ctx->stream));
     drop_graph(S);
     S->pairs = on ? 1 : 0;
+    return CHM_OK;
+}
+
+// ======================================================================== run marks
+namespace {
+
+// One wavefront: the smallest best chi^2 over the slots, the accepted-move total and the
+// device wall clock, for RMCSampler.run's history without a host round trip.
+__global__ __launch_bounds__(64) void k_mark(int nrep, const double *__restrict__ best_c2,
+                                             const long long *__restrict__ acc_rung,
+                                             double *__restrict__ out) {
+    double b = __builtin_huge_val();
+    long long a = 0;
+    for (int s = threadIdx.x; s < nrep; s += 64) {
+        b = fmin(b, best_c2[s]);
+        a += acc_rung[s];
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        b = fmin(b, __shfl_xor(b, m));
+        a += __shfl_xor(a, m);
+    }
+    if (threadIdx.x == 0) {
+        out[0] = b;
+        out[1] = (double)a;
+        out[2] = (double)wall_clock64();
+    }
+}
+
+}  // namespace
+
+extern "C" int chm_sampler_reserve_marks(chm_sampler *S, int n) {
+    CHM_REQUIRE(S && n >= 0, CHM_E_ARG, "bad arguments");
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    if (n > S->mark_cap) {
+        CHM_HIP(hipStreamSynchronize(S->ctx->stream));   // a mark may still be in flight
+        int rc = S->marks.ensure(sizeof(double) * 3 * (size_t)n);
+        if (rc) return rc;
+        S->mark_cap = n;
+    }
+    return CHM_OK;
+}
+
+extern "C" int chm_sampler_mark(chm_sampler *S, int idx) {
+    CHM_REQUIRE(S && S->inited, CHM_E_STATE, "init_state first");
+    CHM_REQUIRE(idx >= 0 && idx < S->mark_cap, CHM_E_ARG, "mark index beyond the reserved capacity");
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    k_mark<<<1, 64, 0, S->ctx->stream>>>(S->nrep, S->best_c2.as<double>(), S->acc_rung.as<long long>(),
+                                         S->marks.as<double>() + 3 * (size_t)idx);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
+extern "C" int chm_sampler_read_marks(chm_sampler *S, int n, double *out) {
+    CHM_REQUIRE(S && out && n >= 1 && n <= S->mark_cap, CHM_E_ARG, "bad mark count");
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    CHM_HIP(hipStreamSynchronize(S->ctx->stream));
+    CHM_HIP(hipMemcpy(out, S->marks.p, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost));
+    int khz = 0;
+    CHM_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, S->ctx->device));
+    CHM_REQUIRE(khz > 0, CHM_E_HIP, "device wall clock rate unavailable");
+    const double t0 = out[2];
+    for (int q = 0; q < n; ++q) out[3 * q + 2] = (out[3 * q + 2] - t0) / (1e3 * khz);
+    return CHM_OK;
+}
+
+// ======================================================================== checkpoint
+// The complete chain state between two run() calls: the next call proposes from it (the
+// proposal and count-change rows are rebuilt at the start of every segment), so restoring
+// these buffers and the step count continues the chain bit for bit.
+namespace {
+
+constexpr uint32_t kStateMagic = 0x53534D43u;   // "CMSS"
+constexpr uint32_t kStateVersion = 1;
+constexpr int kStateSect = 17;
+
+struct StateHeader {
+    uint32_t magic, version;
+    int32_t natoms, nsp, nrow_tot, ncol_tot, nrep, natoms_pad, incremental, nsect;
+    int64_t hstep;
+    uint64_t seed;
+    int64_t off[kStateSect], bytes[kStateSect];
+};
+
+// species, counts, chi^2 (both step-parity rows), running chi(R), best chi^2, best
+// species, rung_of, slot_of, beta (rung), beta (slot), step counters, exchange counter,
+// accepted per rung, accepted since re-anchor, re-anchors, exchange attempts / accepts
+void state_sections(chm_sampler *S, chm::DevBuf *b[kStateSect], size_t n[kStateSect]) {
+    const size_t r = S->nrep;
+    const Geo &g = S->g;
+    chm::DevBuf *bb[kStateSect] = {&S->species, &S->counts,   &S->c2,       &S->mcur,    &S->best_c2,
+                                   &S->best_sp, &S->rung_of,  &S->slot_of,  &S->beta,    &S->beta_slot,
+                                   &S->stepc,   &S->iex,      &S->acc_rung, &S->anchor,  &S->nanchor,
+                                   &S->exch_att, &S->exch_acc};
+    const size_t nn[kStateSect] = {r * g.natoms_pad, 4 * r * g.nrow_tot, 16 * r, 8 * r * g.ncol_tot, 8 * r,
+                                   r * g.natoms_pad, 4 * r, 4 * r, 8 * r, 8 * r,
+                                   8 * r, 8, 8 * r, 4 * r, 8 * r,
+                                   8 * r, 8 * r};
+    for (int q = 0; q < kStateSect; ++q) {
+        b[q] = bb[q];
+        n[q] = nn[q];
+    }
+}
+
+StateHeader state_header(chm_sampler *S) {
+    StateHeader h{};
+    h.magic = kStateMagic;
+    h.version = kStateVersion;
+    h.natoms = S->g.natoms;
+    h.nsp = S->g.nsp;
+    h.nrow_tot = S->g.nrow_tot;
+    h.ncol_tot = S->g.ncol_tot;
+    h.nrep = S->nrep;
+    h.natoms_pad = S->g.natoms_pad;
+    h.incremental = S->incremental;
+    h.nsect = kStateSect;
+    h.hstep = S->hstep;
+    h.seed = S->seed;
+    chm::DevBuf *b[kStateSect];
+    size_t n[kStateSect];
+    state_sections(S, b, n);
+    int64_t off = (sizeof(StateHeader) + 7) & ~(size_t)7;
+    for (int q = 0; q < kStateSect; ++q) {
+        h.off[q] = off;
+        h.bytes[q] = (int64_t)n[q];
+        off += ((int64_t)n[q] + 7) & ~(int64_t)7;
+    }
+    return h;
+}
+
+int64_t state_bytes(const StateHeader &h) {
+    return h.off[kStateSect - 1] + ((h.bytes[kStateSect - 1] + 7) & ~(int64_t)7);
+}
+
+}  // namespace
+
+extern "C" int chm_sampler_state_size(chm_sampler *S, int64_t *nbytes) {
+    CHM_REQUIRE(S && nbytes, CHM_E_ARG, "bad arguments");
+    *nbytes = state_bytes(state_header(S));
+    return CHM_OK;
+}
+
+extern "C" int chm_sampler_export(chm_sampler *S, void *host, int64_t nbytes) {
+    CHM_REQUIRE(S && host, CHM_E_ARG, "bad arguments");
+    CHM_REQUIRE(S->inited, CHM_E_STATE, "init_state first");
+    const StateHeader h = state_header(S);
+    CHM_REQUIRE(nbytes >= state_bytes(h), CHM_E_ARG, "host buffer too small");
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    CHM_HIP(hipStreamSynchronize(S->ctx->stream));
+    char *o = static_cast<char *>(host);
+    std::memset(o, 0, (size_t)state_bytes(h));
+    std::memcpy(o, &h, sizeof(h));
+    chm::DevBuf *b[kStateSect];
+    size_t n[kStateSect];
+    state_sections(S, b, n);
+    for (int q = 0; q < kStateSect; ++q)
+        CHM_HIP(hipMemcpy(o + h.off[q], b[q]->p, n[q], hipMemcpyDeviceToHost));
+    return CHM_OK;
+}
+
+extern "C" int chm_sampler_import(chm_sampler *S, const void *host, int64_t nbytes) {
+    CHM_REQUIRE(S && host, CHM_E_ARG, "bad arguments");
+    const StateHeader want = state_header(S);
+    CHM_REQUIRE(nbytes >= (int64_t)sizeof(StateHeader), CHM_E_ARG, "state blob too short");
+    StateHeader h;
+    std::memcpy(&h, host, sizeof(h));
+    CHM_REQUIRE(h.magic == kStateMagic && h.version == kStateVersion, CHM_E_ARG,
+                "not a sampler state blob of this version");
+    CHM_REQUIRE(h.natoms == want.natoms && h.nsp == want.nsp && h.nrow_tot == want.nrow_tot &&
+                    h.ncol_tot == want.ncol_tot && h.nrep == want.nrep && h.natoms_pad == want.natoms_pad &&
+                    h.nsect == kStateSect,
+                CHM_E_ARG, "state blob was saved from a sampler of another shape");
+    CHM_REQUIRE(h.incremental == S->incremental, CHM_E_ARG,
+                "state blob was saved in another transform precision (set_precision first)");
+    for (int q = 0; q < kStateSect; ++q)
+        CHM_REQUIRE(h.off[q] == want.off[q] && h.bytes[q] == want.bytes[q], CHM_E_ARG,
+                    "state blob layout does not match");
+    CHM_REQUIRE(nbytes >= state_bytes(want), CHM_E_ARG, "state blob too short");
+    const char *in = static_cast<const char *>(host);
+    const Geo &g = S->g;
+    const int nrep = S->nrep;
+    // host-side validation of every index and value a kernel will follow
+    const uint8_t *sp = reinterpret_cast<const uint8_t *>(in + h.off[0]);
+    const uint8_t *bsp = reinterpret_cast<const uint8_t *>(in + h.off[5]);
+    for (long q = 0; q < (long)nrep * g.natoms_pad; ++q)
+        CHM_REQUIRE(sp[q] < g.nsp && bsp[q] < g.nsp, CHM_E_ARG, "species out of range in state blob");
+    const int32_t *rung_of = reinterpret_cast<const int32_t *>(in + h.off[6]);
+    const int32_t *slot_of = reinterpret_cast<const int32_t *>(in + h.off[7]);
+    for (int r = 0; r < nrep; ++r) {
+        CHM_REQUIRE(rung_of[r] >= 0 && rung_of[r] < nrep && slot_of[r] >= 0 && slot_of[r] < nrep &&
+                        slot_of[rung_of[r]] == r,
+                    CHM_E_ARG, "rung / slot maps in state blob are not inverse permutations");
+    }
+    const double *beta = reinterpret_cast<const double *>(in + h.off[8]);
+    for (int r = 0; r < nrep; ++r)
+        CHM_REQUIRE(std::isfinite(beta[r]) && beta[r] > 0.0, CHM_E_ARG, "bad beta in state blob");
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    hipStream_t st = S->ctx->stream;
+    CHM_HIP(hipStreamSynchronize(st));
+    drop_graph(S);   // captured launches carry the recording base
+    chm::DevBuf *b[kStateSect];
+    size_t n[kStateSect];
+    state_sections(S, b, n);
+    for (int q = 0; q < kStateSect; ++q)
+        CHM_HIP(hipMemcpy(b[q]->p, in + h.off[q], n[q], hipMemcpyHostToDevice));
+    // no step is pending: the count-change and proposal rows are rebuilt by the next segment
+    CHM_HIP(hipMemsetAsync(S->dcounts.p, 0, S->dcounts.bytes, st));
+    CHM_HIP(hipMemsetAsync(S->pdc.p, 0, S->pdc.bytes, st));
+    if (S->log_cap) CHM_HIP(hipMemsetAsync(S->log.p, 0xff, S->log.bytes, st));
+    CHM_HIP(hipStreamSynchronize(st));
+    S->hstep = h.hstep;
+    S->seed = h.seed;
+    S->rec_base = h.hstep;
+    S->inited = true;
     return CHM_OK;
 }
 

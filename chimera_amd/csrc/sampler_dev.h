@@ -23,6 +23,14 @@ __device__ __forceinline__ double u01(uint64_t h) {
     return (double)(h >> 11) * (1.0 / 9007199254740992.0);
 }
 
+// Metropolis test of methods/method_a_rmc.py:280-285 for x = beta * d(chi^2) * nfit (minus
+// the log proposal ratio): accept if x <= 0 or u < exp(-min(x, 60)).  A NaN x is rejected,
+// as the reference's cp.minimum / cp.exp propagate it (fmin alone would drop the NaN and
+// accept with probability e^-60).
+__device__ __forceinline__ bool metropolis(double x, double u) {
+    return (x <= 0.0) || (!__builtin_isnan(x) && u < exp(-fmin(x, 60.0)));
+}
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 

@@ -62,6 +62,10 @@ struct chm_sampler {
     size_t lds_pair = 0;
     chm::DevBuf pdc, pprop, pmt, ppart, pc2;
     chm::DevBuf draws;           // (nrep, 24) random numbers made by the transform's extra workgroup
+    // run-history snapshots taken on the stream (no host sync): per mark the smallest best
+    // chi^2, the accepted-move total and the device wall clock
+    chm::DevBuf marks;
+    int mark_cap = 0;
 };
 
 namespace chm_smp {

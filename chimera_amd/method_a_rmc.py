@@ -37,6 +37,20 @@ from ._lib import check, f64, i32
 from .lattice import alpha_from_counts
 
 
+def state_sections(blob):
+    """{name: (byte offset, byte count)} of the sections of a save_state blob (the
+    header written by chm_sampler_export: magic, version, eight int32 sizes, step count,
+    seed, then 17 offsets and 17 byte counts)."""
+    b = np.frombuffer(bytes(blob) if not isinstance(blob, np.ndarray) else blob.tobytes(), np.uint8)
+    magic, version = b[:8].view(np.uint32)
+    if magic != 0x53534D43 or version != 1:
+        raise ValueError("not a sampler state blob")
+    nsect = int(b[8:40].view(np.int32)[7])
+    tab = b[56:56 + 16 * nsect].view(np.int64)
+    return {name: (int(tab[q]), int(tab[nsect + q]))
+            for q, name in enumerate(RMCSampler._STATE_SECTIONS[:nsect])}
+
+
 def _beta_ladder(tmin, tmax, nrep):
     return 1.0 / np.geomspace(tmin, tmax, nrep)
 
@@ -117,12 +131,14 @@ class RMCSampler:
         self.nfit = self.nwin * self.nchan * 2
         self._tm = target[:, self.mask]
         self._sm = np.asarray(sigma_R)[:, self.mask]
-        self.exch_attempts = np.zeros(max(self.nrep - 1, 1))
-        self.exch_accepted = np.zeros(max(self.nrep - 1, 1))
+        self._exchange_rate = None
+        self._rate_pending = False
         self._S = None
         self._comp = None
         self._log_cap = 0
         self._build_hooks = []   # weak methods re-applied to every new device sampler
+        self._best_verified = None
+        self._verify_pending = False
 
     # ------------------------------------------------------------------ basis
     def _lattice_geometry(self):
@@ -238,6 +254,7 @@ class RMCSampler:
 
     # ------------------------------------------------------------------ state
     def init_state(self, species0, pos0):
+        self._settle()
         species0 = i32(species0)
         pos0 = np.asarray(pos0, float)
         if not np.allclose(pos0, self.fm.cell.ideal, atol=1e-9):
@@ -274,6 +291,37 @@ class RMCSampler:
         if self._S is not None:
             check(_lib.load().chm_sampler_set_recording(self._S, self._log_cap, self._series_cap,
                                                         int(series_rung)))
+
+    # ------------------------------------------------------------- checkpoint
+    _STATE_SECTIONS = ("species", "counts", "chi2_rows", "running_chiR", "best_chi2",
+                       "best_species", "rung_of", "slot_of", "beta", "beta_slot", "step_counter",
+                       "exchange_counter", "accepted_per_rung", "accepted_since_anchor",
+                       "reanchors", "exchange_attempts", "exchange_accepts")
+
+    def save_state(self):
+        """The whole chain state between runs as bytes (species, counts, chi^2, running
+        chi(R), best state, rung/slot maps, ladder, counters, step count, seed).
+        load_state on a sampler of the same problem, precision and composition (made
+        and initialised the same way) continues the chain bit for bit."""
+        self._settle()
+        lib = _lib.load()
+        n = _lib.C.c_int64()
+        check(lib.chm_sampler_state_size(self._S, _lib.C.byref(n)))
+        buf = np.zeros(n.value, np.uint8)
+        check(lib.chm_sampler_export(self._S, buf.ctypes.data, buf.nbytes))
+        return buf.tobytes()
+
+    def load_state(self, blob):
+        """Continue from a blob written by save_state (see there).  Collected samples
+        are kept; the ladder is taken from the blob."""
+        self._settle()
+        if self._S is None:
+            raise RuntimeError("init_state first (the blob replaces its state)")
+        buf = np.frombuffer(bytes(blob), np.uint8).copy()
+        check(_lib.load().chm_sampler_import(self._S, buf.ctypes.data, buf.nbytes))
+        sec = state_sections(buf)
+        off, nb = sec["beta"]
+        self._beta_rung = buf[off:off + nb].view(np.float64).copy()
 
     def _read(self, what, shape, dtype):
         out = np.empty(shape, dtype)
@@ -342,6 +390,7 @@ class RMCSampler:
     # ------------------------------------------------------------------ moves
     def step(self):
         """One swap step for every replica; returns the accepted flags (rung order)."""
+        self._settle()
         before = self.accepted_per_rung()
         check(_lib.load().chm_sampler_run(self._S, 1, 0))
         return (self.accepted_per_rung() - before) > 0
@@ -349,6 +398,7 @@ class RMCSampler:
     def replica_exchange(self, return_rates=False):
         """One exchange sweep on the device (method_a_rmc.py:332-373).  Returns the
         number of swaps (and per-pair 0/1 rates of this sweep, NaN where not tried)."""
+        self._settle()
         m = max(self.nrep - 1, 1)
         ea0 = self._read(8, (2, m), np.int64)
         check(_lib.load().chm_sampler_exchange(self._S))
@@ -397,29 +447,47 @@ class RMCSampler:
     def run(self, nsteps, exchange_every=25, log_every=500, callback=None):
         """Run the chain on the device (method_a_rmc.py:401-455).  With adapt_every the
         run stops at every exchange to read that sweep's per-pair outcomes and retunes
-        the ladder every adapt_every steps, as the reference's loop does."""
+        the ladder every adapt_every steps, as the reference's loop does.
+
+        The history tuples (step, best chi^2, acceptance, seconds) are those of the
+        reference's log points (step 1 and every log_every steps).  Without a callback
+        they are snapshots taken on the device stream and read once at the end (the
+        seconds from the device clock), so logging costs no host round trip; with a
+        callback the run stops at every log point to call it.  best_verified (the
+        retained best's chi^2 recomputed from scratch) is evaluated when first read."""
         hist = []
         t0 = time.time()
         lib = _lib.load()
-        acc0 = self.accepted_per_rung().sum()
+        self._best_verified, self._verify_pending = None, False
+        self._rate_pending = False    # recomputed from the cumulative counters at the end
         it = 0
         E = int(exchange_every)
         A = self.adapt_every
         marks = set()
         if self.collect_after is not None:
-            marks.update(s for s in range(int(self.collect_after), nsteps)
-                         if (s + 1) % self.collect_every == 0)
-        marks.update(s for s in range(nsteps) if (s + 1) % log_every == 0 or s == 0)
+            ce = self.collect_every
+            first = int(self.collect_after)
+            first += (-(first + 1)) % ce                  # first s >= collect_after, (s+1) % ce == 0
+            marks.update(range(first, nsteps, ce))
+        logs = set(range(log_every - 1, nsteps, log_every)) | ({0} if nsteps > 0 else set())
+        marks.update(logs)
         if A:
             h0 = int(self._read(10, (1,), np.int64)[0])
             if E:
-                marks.update(s for s in range(nsteps) if (h0 + s + 1) % E == 0)
-            marks.update(s for s in range(nsteps) if (s + 1) % A == 0)
+                marks.update(range((-(h0 + 1)) % E, nsteps, E))
+            marks.update(range(A - 1, nsteps, A))
             win_exch = []
             ea_last = self._read(8, (2, max(self.nrep - 1, 1)), np.int64)
             acc_win = self.accepted_per_rung().sum()
-        marks = sorted(marks)
-        for m in marks:
+        sync_logs = callback is not None
+        log_steps = []
+        if logs:
+            if sync_logs:
+                acc0 = self.accepted_per_rung().sum()
+            else:
+                check(lib.chm_sampler_reserve_marks(self._S, len(logs) + 1))
+                check(lib.chm_sampler_mark(self._S, 0))
+        for m in sorted(marks):
             n = m + 1 - it
             if n > 0:
                 check(lib.chm_sampler_run(self._S, n, E))
@@ -435,24 +503,76 @@ class RMCSampler:
             if (self.collect_after is not None and m >= self.collect_after
                     and (m + 1) % self.collect_every == 0):
                 self._collect()
-            if (m + 1) % log_every == 0 or m == 0:
-                best = float(self.best_chi2.min())
-                acc = (self.accepted_per_rung().sum() - acc0) / ((m + 1) * self.nrep)
-                hist.append((m + 1, best, float(acc), time.time() - t0))
-                if callback:
+            if m in logs:
+                if sync_logs:
+                    best = float(self.best_chi2.min())
+                    acc = (self.accepted_per_rung().sum() - acc0) / ((m + 1) * self.nrep)
+                    hist.append((m + 1, best, float(acc), time.time() - t0))
                     callback(m + 1, best, hist[-1][2])
+                else:
+                    log_steps.append(m + 1)
+                    check(lib.chm_sampler_mark(self._S, len(log_steps)))
         if it < nsteps:
             check(lib.chm_sampler_run(self._S, nsteps - it, int(exchange_every)))
+        if log_steps:
+            mk = np.empty((len(log_steps) + 1, 3))
+            check(lib.chm_sampler_read_marks(self._S, len(log_steps) + 1, mk))
+            for q, st in enumerate(log_steps, 1):
+                hist.append((st, float(mk[q, 0]), float((mk[q, 1] - mk[0, 1]) / (st * self.nrep)),
+                             float(mk[q, 2])))
         self.walltime = time.time() - t0
         self.history = hist
-        ea = self._read(8, (2, max(self.nrep - 1, 1)), np.int64)
-        self.exch_attempts, self.exch_accepted = ea[0].astype(float), ea[1].astype(float)
-        with np.errstate(invalid="ignore", divide="ignore"):
-            self.exchange_rate = np.where(self.exch_attempts > 0,
-                                          self.exch_accepted / np.maximum(self.exch_attempts, 1),
-                                          np.nan)
-        self.best_verified = self.verify_best()
+        self._verify_pending = self._rate_pending = True
         return hist
+
+    # Replica-exchange bookkeeping (method_a_rmc.py:332-373, 451-454): the attempt and
+    # accept counters are live; exchange_rate is the value at the end of the last run, read
+    # from the device when first used (a run needs no host round trip to report it).
+    def _exchange_counts(self):
+        m = max(self.nrep - 1, 1)
+        if self._S is None:
+            return np.zeros((2, m))
+        return self._read(8, (2, m), np.int64).astype(float)
+
+    @property
+    def exch_attempts(self):
+        return self._exchange_counts()[0]
+
+    @property
+    def exch_accepted(self):
+        return self._exchange_counts()[1]
+
+    @property
+    def exchange_rate(self):
+        if self._rate_pending:
+            att, acc = self._exchange_counts()
+            with np.errstate(invalid="ignore", divide="ignore"):
+                self._exchange_rate = np.where(att > 0, acc / np.maximum(att, 1), np.nan)
+            self._rate_pending = False
+        return self._exchange_rate
+
+    @property
+    def best_verified(self):
+        """chi^2 of the best configuration retained by the last run, recomputed from
+        scratch (method_a_rmc.py:439-448).  Evaluated on first read (a pair-space path
+        sum, the multiple-scattering counts and the transform), so a run that nobody
+        verifies costs nothing; anything that changes the chain first settles it."""
+        if self._verify_pending:
+            self._best_verified = self.verify_best()
+            self._verify_pending = False
+        return self._best_verified
+
+    @best_verified.setter
+    def best_verified(self, value):
+        self._best_verified, self._verify_pending = value, False
+
+    def _settle(self):
+        # the values the last run would have reported, before the chain can change them
+        if self._S is not None:
+            if self._verify_pending:
+                self.best_verified
+            if self._rate_pending:
+                self.exchange_rate
 
     def verify_best(self):
         """chi^2 of the retained best, recomputed from scratch: pair-space FP64 path sum

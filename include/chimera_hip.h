@@ -29,7 +29,9 @@ extern "C" {
 #define CHM_E_ASSERT (-4)  /* a reference assertion failed (maps to AssertionError) */
 
 #define CHM_PREC_F64 0     /* FP64 path sum */
-#define CHM_PREC_F32 1     /* FP32 path sum (FP64 phase reduction), FP64 accumulation */
+#define CHM_PREC_F32 1     /* FP32 path sum: FP32 terms on the hardware sine / exponential
+                              units, summed in FP32 per run of one absorber species and in
+                              FP64 across runs and workgroups */
 
 typedef struct chm_ctx chm_ctx;
 typedef struct chm_ft chm_ft;
@@ -196,8 +198,10 @@ int chm_sampler_init(chm_sampler *s, const int32_t *species0, const double *beta
                      uint64_t seed);
 
 /* Run nsteps swap steps (one proposal per replica each), replica exchange every
- * exchange_every steps (0 = never).  Device-resident; the steps between host calls
- * are replayed from a captured hipGraph.  RMCSampler.run/step/replica_exchange
+ * exchange_every steps (0 = never).  Device-resident; whole exchange periods are
+ * replayed from a captured hipGraph.  Returns once the work is enqueued on the
+ * sampler's stream (every read and every state-changing call waits for it; kernel
+ * faults surface there or at chm_ctx_sync).  RMCSampler.run/step/replica_exchange
  * (method_a_rmc.py:250-308, 332-373, 401-455). */
 int chm_sampler_run(chm_sampler *s, int nsteps, int exchange_every);
 
@@ -258,6 +262,28 @@ int chm_sampler_set_beta(chm_sampler *s, const double *beta);
  * counts @ basis_k per edge (SupercellMS.chi_of, chimera/mslibrary.py:453-457, and the
  * pair-count model ShellModel.chi_k, chimera/shellmodel.py:411-413). */
 int chm_sampler_chik(chm_sampler *s, int nconf, const int32_t *counts, double *chi);
+
+/* Run-history snapshots without a host round trip.  reserve_marks sizes the buffer for
+ * n marks; mark(idx) enqueues on the sampler's stream a snapshot of the smallest best
+ * chi^2 over the replicas, the total of accepted moves and the device wall clock;
+ * read_marks copies the first n marks (n, 3): best chi^2, accepted moves, seconds since
+ * mark 0.  RMCSampler.run's `history` tuples (methods/method_a_rmc.py:430-435), which
+ * the reference fills with a host sync at every log point. */
+int chm_sampler_reserve_marks(chm_sampler *s, int n);
+int chm_sampler_mark(chm_sampler *s, int idx);
+int chm_sampler_read_marks(chm_sampler *s, int n, double *out);
+
+/* Checkpoint: the whole chain state between two chm_sampler_run calls (species, counts,
+ * chi^2, running chi(R), best state, rung <-> slot maps, ladder, step / exchange
+ * counters, acceptance and re-anchor counts, step count and seed) as one host blob of
+ * state_size bytes.  Importing it into a sampler of the same shape and precision (made
+ * and initialised as the saving one was) continues the chain bit for bit; recording
+ * restarts at the imported step.  The learned proposal's weights are saved separately
+ * (chm_lp_get_params).  No counterpart in the reference (init_state accepts a warm
+ * start, chimera/fitrun.py:317-320, but nothing saves a running chain). */
+int chm_sampler_state_size(chm_sampler *s, int64_t *nbytes);
+int chm_sampler_export(chm_sampler *s, void *host, int64_t nbytes);
+int chm_sampler_import(chm_sampler *s, const void *host, int64_t nbytes);
 
 /* Launch shape of the per-replica kernel: workgroups per replica slot for the
  * accept/propose/gather kernel (1..64).  Chosen per problem by the autotuner
