@@ -117,6 +117,20 @@ inline void ctx_release(chm_ctx *c) {
     delete c;
 }
 
+// Launch shape of the pair-space path sum (forward.hip): first lane's k index, lanes per
+// workgroup, workgroups per configuration, pairs per workgroup.
+struct ChiShape {
+    int k0, nthr, nwg, ppw;
+};
+int chi_shape(const chm_ctx *ctx, int nconf, ChiShape &sh);
+// Path sum of nconf configurations from device distances r and pair types pt (nconf,
+// npairs) into chi (nconf, nsp, nk), partial (nconf, nwg, nsp, nk) holding the per-
+// workgroup sums; on stream st, nothing allocated.
+int chi_into(chm_ctx *ctx, hipStream_t st, int nconf, int precision, const ChiShape &sh,
+             const double *r, const int *pt, double *partial, double *chi);
+// Pair distances of nconf configurations (device positions) into r, on stream st.
+int distances_into(chm_ctx *ctx, hipStream_t st, int nconf, const double *pos, double *r);
+
 // Generic FP64 GEMM on the context stream: C[b] (M x N) = A[b] (M x K) @ B[b] (K x N),
 // row-major with leading dimensions and per-batch element strides.
 int gemm_f64(hipStream_t st, int nbatch, int M, int N, int K, const double *A, int lda,

@@ -10,6 +10,7 @@
 // so results are deterministic.
 #include <cmath>
 #include <cstring>
+#include <string>
 
 #include "common.h"
 #include "fp64math.h"
@@ -577,65 +578,37 @@ __global__ __launch_bounds__(512) void k_chi_pairs_f32(
     }
 }
 
-// chi[ic][s][k] = sum over workgroups w (fixed order) of partial[ic][w][s][k]
-__global__ void k_reduce_partials(const double *__restrict__ partial, double *__restrict__ chi,
-                                  int nwg, int n_per_conf, int nconf) {
-    long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= (long)n_per_conf * nconf) return;
-    int ic = (int)(tid / n_per_conf), q = (int)(tid % n_per_conf);
-    const double *p = partial + (long)ic * nwg * n_per_conf + q;
+// chi[ic][q] = sum over workgroups w of partial[ic][w][q], q < n_per_conf, in a fixed
+// order: a 1024-lane workgroup takes 64 consecutive q (one per lane, coalesced rows) of one
+// configuration; wave v sums w = v, v + 16, ... in turn and the 16 wave sums are added in
+// wave order.  (One lane per q walking every w serially left a handful of CUs busy for
+// ~0.2 ms at one configuration.)
+constexpr int kRedWaves = 16;
+__global__ __launch_bounds__(1024) void k_reduce_partials(const double *__restrict__ partial,
+                                                          double *__restrict__ chi, int nwg,
+                                                          int n_per_conf, int nconf) {
+    __shared__ double red[kRedWaves][64];
+    const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+    const int q = blockIdx.x * 64 + lane, ic = blockIdx.y;
     double s = 0.0;
-    for (int w = 0; w < nwg; ++w) s += p[(long)w * n_per_conf];
-    chi[tid] = s;
+    if (q < n_per_conf) {
+        const double *p = partial + (long)ic * nwg * n_per_conf + q;
+        for (int w = v; w < nwg; w += kRedWaves) s += p[(long)w * n_per_conf];
+    }
+    red[v][lane] = s;
+    __syncthreads();
+    if (v == 0 && q < n_per_conf) {
+        double t = red[0][lane];
+#pragma unroll
+        for (int z = 1; z < kRedWaves; ++z) t += red[z][lane];
+        chi[(long)ic * n_per_conf + q] = t;
+    }
 }
 
-template <int NSP>
-int launch_chi(chm_ctx *ctx, int nconf, int precision) {
-    const int nk = ctx->nk;
-    const int npairs = ctx->npairs;
-    // lanes over the k > 0 points only (the grid is ascending)
-    int k0 = 0;
-    while (k0 < nk && !(ctx->h_k[k0] > 0.0)) ++k0;
-    const int nkz = std::max(1, nk - k0);
-    CHM_REQUIRE(nkz <= kMaxK, CHM_E_ARG, "at most 512 k points above 0");
-    const int nthr = std::min(kMaxK, (nkz + 63) / 64 * 64);
-    // ~2048 workgroups over the device, whole staging rounds each
-    const int rounds = (npairs + nthr - 1) / nthr;
-    int nwg = std::max(1, std::min(rounds, (2048 + nconf - 1) / nconf));
-    const int ppw = ((rounds + nwg - 1) / nwg) * nthr;
-    nwg = (npairs + ppw - 1) / ppw;
-    int rc = ctx->partial.ensure(sizeof(double) * (size_t)nconf * nwg * NSP * nk);
-    if (rc) return rc;
-    const dim3 grid(nwg, nconf);
-    if (precision == CHM_PREC_F32) {
-        const size_t words = (size_t)(NSP + NSP * NSP) * nk;
-        const size_t lds = sizeof(float) * ((words + 7) & ~(size_t)7) + sizeof(PairRecF) * nthr;
-        CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
-        hipFuncSetAttribute((const void *)k_chi_pairs_f32<NSP>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        k_chi_pairs_f32<NSP><<<grid, nthr, lds, ctx->stream>>>(
-            ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(),
-            ctx->sig2.as<double>(), ctx->f32.as<float>(), ctx->drev32.as<float>(),
-            ctx->lam.as<double>(), ctx->k.as<double>(), ctx->drtab.as<double>(),
-            ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), ctx->partial.as<double>(), npairs,
-            nk, k0, ppw, ctx->s02);
-    } else {
-        const size_t lds = sizeof(double) * (NSP + NSP * NSP) * nk + sizeof(PairRec) * nthr;
-        CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
-        hipFuncSetAttribute((const void *)k_chi_pairs<NSP>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        k_chi_pairs<NSP><<<grid, nthr, lds, ctx->stream>>>(
-            ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(),
-            ctx->sig2.as<double>(), ctx->f.as<double>(), ctx->delta.as<double>(),
-            ctx->lam.as<double>(), ctx->k.as<double>(), ctx->drtab.as<double>(),
-            ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), ctx->partial.as<double>(), npairs,
-            nk, k0, ppw, ctx->s02, chm::kPolyCoefs);
-    }
-    CHM_LAUNCH_CHECK();
-    long n = (long)nconf * NSP * nk;
-    if ((rc = ctx->chi.ensure(sizeof(double) * n))) return rc;
-    k_reduce_partials<<<dim3((unsigned)((n + 255) / 256)), 256, 0, ctx->stream>>>(
-        ctx->partial.as<double>(), ctx->chi.as<double>(), nwg, NSP * nk, nconf);
+int launch_reduce(hipStream_t st, const double *partial, double *out, int nwg, int n_per_conf,
+                  int nconf) {
+    const dim3 grid((unsigned)((n_per_conf + 63) / 64), (unsigned)nconf);
+    k_reduce_partials<<<grid, 64 * kRedWaves, 0, st>>>(partial, out, nwg, n_per_conf, nconf);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -643,6 +616,85 @@ int launch_chi(chm_ctx *ctx, int nconf, int precision) {
 }  // namespace
 
 namespace chm {
+// Launch shape of the path sum for nconf configurations: lanes over the k > 0 points only
+// (the grid is ascending), ~2048 workgroups over the device, whole staging rounds each.
+int chi_shape(const chm_ctx *ctx, int nconf, ChiShape &sh) {
+    int k0 = 0;
+    while (k0 < ctx->nk && !(ctx->h_k[k0] > 0.0)) ++k0;
+    const int nkz = std::max(1, ctx->nk - k0);
+    CHM_REQUIRE(nkz <= kMaxK, CHM_E_ARG, "at most 512 k points above 0");
+    sh.k0 = k0;
+    sh.nthr = std::min(kMaxK, (nkz + 63) / 64 * 64);
+    const int rounds = (ctx->npairs + sh.nthr - 1) / sh.nthr;
+    int nwg = std::max(1, std::min(rounds, (2048 + nconf - 1) / nconf));
+    sh.ppw = ((rounds + nwg - 1) / nwg) * sh.nthr;
+    sh.nwg = (ctx->npairs + sh.ppw - 1) / sh.ppw;
+    return CHM_OK;
+}
+}  // namespace chm
+
+namespace {
+
+// The path sum of nconf configurations from their distances r and pair types pt (both
+// (nconf, npairs)) into chi (nconf, NSP, nk), with partial (nconf, nwg, NSP, nk) as the
+// per-workgroup sums, on stream st.
+template <int NSP>
+int launch_chi_into(chm_ctx *ctx, hipStream_t st, int nconf, int precision, const chm::ChiShape &sh,
+                    const double *r, const int *pt, double *partial, double *chi) {
+    const int nk = ctx->nk;
+    const dim3 grid(sh.nwg, nconf);
+    if (precision == CHM_PREC_F32) {
+        const size_t words = (size_t)(NSP + NSP * NSP) * nk;
+        const size_t lds = sizeof(float) * ((words + 7) & ~(size_t)7) + sizeof(PairRecF) * sh.nthr;
+        CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+        hipFuncSetAttribute((const void *)k_chi_pairs_f32<NSP>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k_chi_pairs_f32<NSP><<<grid, sh.nthr, lds, st>>>(
+            r, pt, ctx->shell.as<int>(), ctx->sig2.as<double>(), ctx->f32.as<float>(),
+            ctx->drev32.as<float>(), ctx->lam.as<double>(), ctx->k.as<double>(),
+            ctx->drtab.as<double>(), ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), partial,
+            ctx->npairs, nk, sh.k0, sh.ppw, ctx->s02);
+    } else {
+        const size_t lds = sizeof(double) * (NSP + NSP * NSP) * nk + sizeof(PairRec) * sh.nthr;
+        CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+        hipFuncSetAttribute((const void *)k_chi_pairs<NSP>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k_chi_pairs<NSP><<<grid, sh.nthr, lds, st>>>(
+            r, pt, ctx->shell.as<int>(), ctx->sig2.as<double>(), ctx->f.as<double>(),
+            ctx->delta.as<double>(), ctx->lam.as<double>(), ctx->k.as<double>(),
+            ctx->drtab.as<double>(), ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), partial,
+            ctx->npairs, nk, sh.k0, sh.ppw, ctx->s02, chm::kPolyCoefs);
+    }
+    CHM_LAUNCH_CHECK();
+    return launch_reduce(st, partial, chi, sh.nwg, NSP * nk, nconf);
+}
+
+}  // namespace
+
+namespace chm {
+int chi_into(chm_ctx *ctx, hipStream_t st, int nconf, int precision, const ChiShape &sh,
+             const double *r, const int *pt, double *partial, double *chi) {
+    switch (ctx->nsp) {
+        case 2: return launch_chi_into<2>(ctx, st, nconf, precision, sh, r, pt, partial, chi);
+        case 3: return launch_chi_into<3>(ctx, st, nconf, precision, sh, r, pt, partial, chi);
+        case 4: return launch_chi_into<4>(ctx, st, nconf, precision, sh, r, pt, partial, chi);
+        case 5: return launch_chi_into<5>(ctx, st, nconf, precision, sh, r, pt, partial, chi);
+        case 6: return launch_chi_into<6>(ctx, st, nconf, precision, sh, r, pt, partial, chi);
+    }
+    chm::set_error("unsupported species count");
+    return CHM_E_ARG;
+}
+
+// Distances of nconf configurations (positions pos, device) into r (device), on stream st.
+int distances_into(chm_ctx *ctx, hipStream_t st, int nconf, const double *pos, double *r) {
+    const long tot = (long)nconf * ctx->npairs;
+    k_distances<<<dim3((unsigned)((tot + 255) / 256)), 256, 0, st>>>(
+        pos, ctx->i_idx.as<int>(), ctx->j_idx.as<int>(), ctx->shift.as<double>(), r, nullptr,
+        ctx->npairs, ctx->natoms, nconf);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
 int distances_device(chm_ctx *ctx, int nconf, bool want_evec) {
     return run_distances(ctx, nconf, want_evec);
 }
@@ -661,11 +713,7 @@ int ptypes_device(chm_ctx *ctx, int nconf) {
 
 int reduce_partials(chm_ctx *ctx, const double *partial, double *out, int nwg, int n_per_conf,
                     int nconf) {
-    long n = (long)nconf * n_per_conf;
-    k_reduce_partials<<<dim3((unsigned)((n + 255) / 256)), 256, 0, ctx->stream>>>(
-        partial, out, nwg, n_per_conf, nconf);
-    CHM_LAUNCH_CHECK();
-    return CHM_OK;
+    return launch_reduce(ctx->stream, partial, out, nwg, n_per_conf, nconf);
 }
 
 // Path sum over the context's pos/species buffers already on the device.
@@ -673,15 +721,12 @@ int chi_device(chm_ctx *ctx, int nconf, int precision) {
     int rc = run_distances(ctx, nconf, false);
     if (rc) return rc;
     if ((rc = ptypes_device(ctx, nconf))) return rc;
-    switch (ctx->nsp) {
-        case 2: return launch_chi<2>(ctx, nconf, precision);
-        case 3: return launch_chi<3>(ctx, nconf, precision);
-        case 4: return launch_chi<4>(ctx, nconf, precision);
-        case 5: return launch_chi<5>(ctx, nconf, precision);
-        case 6: return launch_chi<6>(ctx, nconf, precision);
-    }
-    set_error("unsupported species count");
-    return CHM_E_ARG;
+    ChiShape sh;
+    if ((rc = chi_shape(ctx, nconf, sh))) return rc;
+    if ((rc = ctx->partial.ensure(sizeof(double) * (size_t)nconf * sh.nwg * ctx->nsp * ctx->nk))) return rc;
+    if ((rc = ctx->chi.ensure(sizeof(double) * (size_t)nconf * ctx->nsp * ctx->nk))) return rc;
+    return chi_into(ctx, ctx->stream, nconf, precision, sh, ctx->r.as<double>(), ctx->ptype.as<int>(),
+                    ctx->partial.as<double>(), ctx->chi.as<double>());
 }
 }  // namespace chm
 

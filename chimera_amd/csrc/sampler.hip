@@ -87,6 +87,15 @@ __global__ __launch_bounds__(256) void k_full_counts(
     }
 }
 
+// workgroups per configuration of k_full_counts: ~8192 items each, and enough workgroups
+// over the device when few configurations are counted (each workgroup ends in one integer
+// atomic per non-zero row, so the split stays modest)
+int full_counts_split(const Geo &g, int nconf) {
+    const long long nitems = (long long)g.natoms * g.ncoord + (long long)g.natoms * g.nabsrec;
+    const long long cap = std::max(64, 256 / std::max(1, nconf));
+    return (int)std::max(1LL, std::min(cap, nitems / 8192 + 1));
+}
+
 // ---------------------------------------------------------------- K2
 // Both trial-transform kernels end in the same epilogue: for a 32-slot x 16-column tile
 // of one edge, m' (the trial chi(R) window values) is written out and, per slot, four
@@ -2162,9 +2171,7 @@ extern "C" int chm_sampler_init(chm_sampler *S, const int32_t *species0, const d
     S->seed = seed;
     S->hstep = 0;
     S->rec_base = 0;
-    long long npairs = (long long)g.natoms * g.ncoord;
-    long long nitems = npairs + (long long)g.natoms * g.nabsrec;
-    int nsplit = (int)std::max(1LL, std::min(64LL, nitems / 8192 + 1));
+    int nsplit = full_counts_split(g, nrep);
     k_full_counts<<<dim3(nsplit, nrep), 256, S->lds_k1, st>>>(
         g, S->species.as<uint8_t>(), S->site_xyz.as<int>(), S->nbr_j.as<int>(), S->nbr_sh.as<int>(),
         S->stencil.as<int4>(), S->counts.as<int>());
@@ -2386,8 +2393,7 @@ extern "C" int chm_sampler_counts(chm_sampler *S, int nconf, const int32_t *spec
             h[(size_t)c * g.natoms_pad + q] = (uint8_t)species[(size_t)c * g.natoms + q];
     CHM_HIP(hipMemcpyAsync(sp.p, h.data(), h.size(), hipMemcpyHostToDevice, st));
     CHM_HIP(hipMemsetAsync(cn.p, 0, cn.bytes, st));
-    long long nitems = (long long)g.natoms * g.ncoord + (long long)g.natoms * g.nabsrec;
-    int nsplit = (int)std::max(1LL, std::min(64LL, nitems / 8192 + 1));
+    int nsplit = full_counts_split(g, nconf);
     k_full_counts<<<dim3(nsplit, nconf), 256, S->lds_k1, st>>>(
         g, sp.as<uint8_t>(), S->site_xyz.as<int>(), S->nbr_j.as<int>(), S->nbr_sh.as<int>(),
         S->stencil.as<int4>(), cn.as<int>());
