@@ -49,6 +49,7 @@ _SIGS = {
     "chm_fm_distances": (C.c_int, [vp, C.c_int, f64p, f64p, vp]),
     "chm_fm_ptypes": (C.c_int, [vp, C.c_int, i32p, i32p]),
     "chm_fm_chi": (C.c_int, [vp, C.c_int, f64p, i32p, f64p, C.c_int]),
+    "chm_fm_chi_noalloc": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, vp]),
     "chm_fm_term_rows": (C.c_int, [vp, C.c_int, f64p, f64p, f64p, i32p, i32p, i32p, C.c_double,
                                    f64p]),
     "chm_ft_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, i32p, i32p, f64p, f64p,

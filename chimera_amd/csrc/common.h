@@ -128,6 +128,9 @@ int chi_shape(const chm_ctx *ctx, int nconf, ChiShape &sh);
 // workgroup sums; on stream st, nothing allocated.
 int chi_into(chm_ctx *ctx, hipStream_t st, int nconf, int precision, const ChiShape &sh,
              const double *r, const int *pt, double *partial, double *chi);
+// noalloc.hip: the device address of the allocation-free path sum's static scratch array,
+// looked up once per device (called by chm_ctx_create with the device current).
+int scratch_init(int device);
 // Pair distances of nconf configurations (device positions) into r, on stream st.
 int distances_into(chm_ctx *ctx, hipStream_t st, int nconf, const double *pos, double *r);
 

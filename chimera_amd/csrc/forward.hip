@@ -31,6 +31,8 @@ extern "C" int chm_ctx_create(int device, chm_ctx **out) {
     CHM_HIP(hipGetDeviceCount(&ndev));
     CHM_REQUIRE(device >= 0 && device < ndev, CHM_E_ARG, "chm_ctx_create: no such device");
     CHM_HIP(hipSetDevice(device));
+    int rc = chm::scratch_init(device);   // noalloc.hip: once per device, device current
+    if (rc) return rc;
     auto *c = new chm_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

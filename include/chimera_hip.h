@@ -84,6 +84,20 @@ int chm_fm_ptypes(chm_ctx *ctx, int nconf, const int32_t *species, int32_t *ptyp
 int chm_fm_chi(chm_ctx *ctx, int nconf, const double *pos, const int32_t *species,
                double *chi, int precision);
 
+/* The path sum of chm_fm_chi (same kernels, same result) on DEVICE pointers, allocating
+ * nothing: d_pos (nconf, natoms, 3), d_species (nconf, natoms), d_chi (nconf, nsp, nk)
+ * unnormalised, all device memory owned by the caller; enqueued on `stream` (a
+ * hipStream_t, or NULL for the context's stream) and not waited for.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (work captured into
+ * a graph, a serving loop).  Its intermediate arrays live in one static device array of
+ * 32 MiB per device (held until the process ends); a call whose arrays do not fit
+ * (about 8 MB per configuration at 216,000 pairs) fails with CHM_E_ARG before anything
+ * is launched.  Species are not checked on the host: a configuration with a species
+ * outside [0, nsp) gets NaN rows.  Calls that overlap on one device (two streams or two
+ * threads) share that array and race. */
+int chm_fm_chi_noalloc(chm_ctx *ctx, int nconf, const double *d_pos, const int32_t *d_species,
+                       double *d_chi, int precision, void *stream);
+
 /* Term table rows: row q gets
  *   s02 * prod_{m<nev} f[ev_m](k) / (k r^2) * exp(-2 r/lambda(k) - 2 k^2 s2)
  *       * sin(2 k r + mean_m delta[a*nsp+ev_m](k) - 4/3 k^3 c3)
