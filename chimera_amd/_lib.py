@@ -57,6 +57,7 @@ _SIGS = {
     "chm_ft_destroy": (C.c_int, [vp]),
     "chm_ft_apply": (C.c_int, [vp, C.c_int, f64p, vp, f64p]),
     "chm_ft_apply_edge": (C.c_int, [vp, C.c_int, C.c_int, f64p, f64p]),
+    "chm_ft_resample": (C.c_int, [vp, C.c_int, C.c_int, f64p, f64p]),
     "chm_sampler_create": (C.c_int, [vp, C.POINTER(Geom), C.c_int, C.c_int, i32p, i32p, i32p, vp,
                                      f64p, f64p, f64p, f64p, C.c_int, C.POINTER(vp)]),
     "chm_sampler_destroy": (C.c_int, [vp]),
@@ -90,7 +91,7 @@ _SIGS = {
     "chm_lp_set_params": (C.c_int, [vp, vp]),
     "chm_lp_get_params": (C.c_int, [vp, vp]),
     "chm_lp_configure": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
-                                   C.c_double]),
+                                   C.c_double, C.c_int]),
     "chm_lp_set_records": (C.c_int, [vp, C.c_int]),
     "chm_lp_read_records": (C.c_int, [vp, C.POINTER(C.c_int64), vp, vp, vp, C.c_int64]),
     "chm_lp_eval": (C.c_int, [vp, C.c_int, vp, vp]),

@@ -103,6 +103,8 @@ struct chm_ft {
     std::vector<int> edge_first, edge_nkw;  // channels of each edge (contiguous)
     chm::DevBuf D;       // (nsp, nk, nkw_e * nR * 2): per edge, channels edge-major
     chm::DevBuf in, out, extra;
+    int nfft = 0;
+    chm::DevBuf ridx, rfrac, rins;   // (nchan, nfft) resample index, fraction, inside flag
 };
 
 namespace chm {

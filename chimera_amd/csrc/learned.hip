@@ -33,8 +33,6 @@ namespace {
 
 using namespace chm_smp;
 
-constexpr uint64_t kLpSalt = 0x2545F4914F6CDD1Dull;
-constexpr uint64_t kCoinSalt = 0x5851F42D4C957F2Dull;
 constexpr uint64_t kSelSalt = 0x14057B7EF767814Full;
 constexpr uint64_t kCandMul = 0xD6E8FEB86659FD93ull;
 constexpr int HID = 256;      // hidden width
@@ -60,15 +58,16 @@ struct LPArgs {
     uint64_t seed;
     int ncand, nocc, noccp, nfeat, kp, r2max;
     double eps, lam, kappa, beta_max;
+    int lift;            // lifted proposal: tilt kappa d dU with d = dir[slot]
+    const int *dir;      // (nrep) +-1
     LPW W;
     float *logit;        // (nrep, ncand)
+    float *gcand;        // (nrep, ncand) network output of each candidate (lifted proposal)
+    int *ducand;         // (nrep, ncand) its first-shell change dU
     int *prop_next;      // (nrep, kPropInts)
     uint8_t *pend_occ;   // (nrep, noccp)
 };
 
-__device__ __forceinline__ uint64_t lp_stream(uint64_t seed, uint64_t step, int rung) {
-    return splitmix64(seed ^ (step * kGolden) ^ ((uint64_t)rung << 32) ^ kLpSalt);
-}
 
 __device__ __forceinline__ void candidate(uint64_t h0, int i, int natoms, int &a, int &b) {
     const uint64_t h = splitmix64(h0 ^ ((uint64_t)(i + 1) * kCandMul));
@@ -100,7 +99,13 @@ __device__ __forceinline__ int dist2(const Geo &g, const int *p, const int *q) {
 // the uniform proposal on this step: a state-independent coin, and every slot colder
 // than beta_max (there the network's error would be multiplied by a large beta)
 __device__ __forceinline__ bool lp_uniform(const LPArgs &A, uint64_t h0, double beta) {
-    return u01(splitmix64(h0 ^ kCoinSalt)) < A.eps || beta > A.beta_max;
+    return lp_coin_uniform(h0, beta, A.eps, A.beta_max);
+}
+
+// the first-shell tilt of a candidate's logit: kappa |dU| (symmetric), or kappa d dU for a
+// lifted proposal moving in direction d = +-1 (the reverse move is scored with -d)
+__device__ __forceinline__ double tilt(const LPArgs &A, int du, int d) {
+    return A.lift ? A.kappa * (double)(d * du) : A.kappa * fabs((double)du);
 }
 
 // site of occupancy entry q of candidate (a, b): the stencil neighbours of a, of b, then
@@ -400,8 +405,12 @@ __global__ __launch_bounds__(NT) void k_lp_score(LPArgs A) {
     mlp_rows<true>(A.W, &F, L.X, L.ldx, L.H, gp, gout);
     if (t < MR) {
         const bool valid = L.lsp[rows[t].x] != L.lsp[rows[t].y];
-        const float lg = (float)(-A.lam * beta * 0.5 * (double)gout[t] + A.kappa * fabs((double)du[t]));
+        const float lg = (float)(-A.lam * beta * 0.5 * (double)gout[t] + tilt(A, du[t], A.lift ? A.dir[s] : 1));
         A.logit[(long)s * A.ncand + i0 + t] = valid ? lg : -__builtin_huge_valf();
+        if (A.lift) {
+            A.gcand[(long)s * A.ncand + i0 + t] = gout[t];
+            A.ducand[(long)s * A.ncand + i0 + t] = du[t];
+        }
     }
 }
 
@@ -515,7 +524,14 @@ __global__ __launch_bounds__(NT) void k_lp_select(LPArgs A) {
             site_xyz_of(g, b, qj[0], qj[1], qj[2]);
             if (t == 0) nalist = 0;
             bool same = false;
-            if (t < MAXC) lrev[t] = lg;
+            // a candidate the swap does not touch keeps its network output and dU; its
+            // reverse logit differs from the forward one only by the lifted tilt's sign
+            // (the same expression as the scorer's at the proposed state with -d)
+            float lu = lg;
+            if (A.lift && t < A.ncand && lg > -__builtin_huge_valf())
+                lu = (float)(-A.lam * beta * 0.5 * (double)A.gcand[(long)s * A.ncand + t] +
+                             tilt(A, A.ducand[(long)s * A.ncand + t], -A.dir[s]));
+            if (t < MAXC) lrev[t] = lu;
             __syncthreads();
             if (t < A.ncand) {
                 int ca, cb, pc[3], qc[3];
@@ -546,7 +562,7 @@ __global__ __launch_bounds__(NT) void k_lp_select(LPArgs A) {
                     const int ca = rows[t].x, cb = rows[t].y;
                     const bool valid = species_swapped(L.lsp, ca, a, b) != species_swapped(L.lsp, cb, a, b);
                     const float l2 = (float)(-A.lam * beta * 0.5 * (double)gout[t] +
-                                             A.kappa * fabs((double)du[t]));
+                                             tilt(A, du[t], A.lift ? -A.dir[s] : 1));
                     lrev[alist[c0 + t]] = valid ? l2 : -__builtin_huge_valf();
                 }
                 __syncthreads();
@@ -837,6 +853,8 @@ LPArgs lp_args(chm_sampler *S) {
     A.lam = L.lam;
     A.kappa = L.kappa;
     A.beta_max = L.beta_max;
+    A.lift = L.lift;
+    A.dir = L.dir.as<int>();
     const Offs o = offsets(L.kp);
     A.W.w1t = L.w1t.as<__bf16>();
     A.W.w2t = L.w2t.as<__bf16>();
@@ -846,6 +864,8 @@ LPArgs lp_args(chm_sampler *S) {
     A.W.w3q = L.w3q.as<float>();
     A.W.kp = L.kp;
     A.logit = L.logit.as<float>();
+    A.gcand = L.gcand.as<float>();
+    A.ducand = L.ducand.as<int>();
     A.pend_occ = L.pend_occ.as<uint8_t>();
     return A;
 }
@@ -955,6 +975,8 @@ int lp_prepare(chm_sampler *S) {
         (rc = L.w1.ensure(2 * (size_t)L.kp * HID)) || (rc = L.w2t.ensure(2 * (size_t)HID * HID)) ||
         (rc = L.w2.ensure(2 * (size_t)HID * HID)) || (rc = L.w3q.ensure(4 * HID)) ||
         (rc = L.logit.ensure(4 * (size_t)S->nrep * MAXC)) ||
+        (rc = L.gcand.ensure(4 * (size_t)S->nrep * MAXC)) ||
+        (rc = L.ducand.ensure(4 * (size_t)S->nrep * MAXC)) ||
         (rc = L.pend_occ.ensure((size_t)S->nrep * L.noccp)))
         return rc;
     CHM_HIP(hipMemsetAsync(L.pend_occ.p, 0, L.pend_occ.bytes, S->ctx->stream));
@@ -1020,7 +1042,7 @@ extern "C" int chm_lp_get_params(chm_sampler *S, float *flat) {
 }
 
 extern "C" int chm_lp_configure(chm_sampler *S, int mode, int ncand, double eps, double lam, double kappa,
-                                double beta_max) {
+                                double beta_max, int lift) {
     CHM_REQUIRE(S, CHM_E_ARG, "null sampler");
     CHM_REQUIRE(mode == 0 || mode == 1, CHM_E_ARG, "mode must be 0 (uniform) or 1 (learned)");
     CHM_REQUIRE(ncand >= MR && ncand <= MAXC && ncand % MR == 0, CHM_E_ARG,
@@ -1039,6 +1061,16 @@ extern "C" int chm_lp_configure(chm_sampler *S, int mode, int ncand, double eps,
     L.lam = lam;
     L.kappa = kappa;
     L.beta_max = beta_max;
+    L.lift = (mode && lift) ? 1 : 0;
+    if (L.lift) {
+        // directions start alternating over the slots (the stationary law of d is +-1
+        // with equal probability, independent of the configuration)
+        int rc = L.dir.ensure(sizeof(int) * S->nrep);
+        if (rc) return rc;
+        std::vector<int> d(S->nrep);
+        for (int q = 0; q < S->nrep; ++q) d[q] = (q & 1) ? -1 : 1;
+        CHM_HIP(hipMemcpy(L.dir.p, d.data(), sizeof(int) * S->nrep, hipMemcpyHostToDevice));
+    }
     return CHM_OK;
 }
 

@@ -629,6 +629,8 @@ struct SlotArgs {
     int propose;            // 1: propose + gather the next step; 2: gather prop_next
     double nfit;
     const int *draws;       // (nrep, kDrawInts) made by the transform just before, or null
+    int *lp_dir;            // lifted learned proposal: direction per slot, flipped when a
+    double lp_eps, lp_beta_max;   //   learned step is not accepted (null: not lifted)
 };
 
 __device__ double block_sum(double v, double *red) {
@@ -822,6 +824,11 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
                         A.species[base_sp + pp[1]] = (uint8_t)pp[3];
                         atomicAdd(reinterpret_cast<unsigned long long *>(A.acc_rung + rung), 1ull);
                     }
+                    // a lifted learned step that is not accepted reverses its direction
+                    // (a uniform step, on the coin or a cold rung, keeps it)
+                    if (A.lp_dir && !acc &&
+                        !lp_coin_uniform(lp_stream(A.seed, stepc, rung), beta, A.lp_eps, A.lp_beta_max))
+                        A.lp_dir[s] = -A.lp_dir[s];
                     if (A.log && ri >= 0 && ri < A.log_cap) {
                         int *L = A.log + ((long)ri * A.nrep + s) * 4;
                         L[0] = pp[0];
@@ -1555,6 +1562,9 @@ SlotArgs slot_args(chm_sampler *S, int accept, int propose) {
     A.rec_occ = L.rec_cap ? L.rec_occ.as<uint8_t>() : nullptr;
     A.rec_y = L.rec_cap ? L.rec_y.as<float>() : nullptr;
     A.rec_meta = L.rec_cap ? L.rec_meta.as<int>() : nullptr;
+    A.lp_dir = (L.mode && L.lift) ? L.dir.as<int>() : nullptr;
+    A.lp_eps = L.eps;
+    A.lp_beta_max = L.beta_max;
     A.seed = S->seed;
     A.profile = S->profile;
     A.nrep = S->nrep;

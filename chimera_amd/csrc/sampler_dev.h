@@ -23,6 +23,19 @@ __device__ __forceinline__ double u01(uint64_t h) {
     return (double)(h >> 11) * (1.0 / 9007199254740992.0);
 }
 
+// Learned proposal: the per-step stream of a rung and the state-independent coin that
+// makes a step use the uniform proposal instead (probability eps, and every slot colder
+// than beta_max).  Shared by the proposal kernels and the accept kernel (which flips the
+// direction of a lifted learned step that was not accepted).
+constexpr uint64_t kLpSalt = 0x2545F4914F6CDD1Dull;
+constexpr uint64_t kCoinSalt = 0x5851F42D4C957F2Dull;
+__device__ __forceinline__ uint64_t lp_stream(uint64_t seed, uint64_t step, int rung) {
+    return splitmix64(seed ^ (step * kGolden) ^ ((uint64_t)rung << 32) ^ kLpSalt);
+}
+__device__ __forceinline__ bool lp_coin_uniform(uint64_t h0, double beta, double eps, double beta_max) {
+    return u01(splitmix64(h0 ^ kCoinSalt)) < eps || beta > beta_max;
+}
+
 // Metropolis test of methods/method_a_rmc.py:280-285 for x = beta * d(chi^2) * nfit (minus
 // the log proposal ratio): accept if x <= 0 or u < exp(-min(x, 60)).  A NaN x is rejected,
 // as the reference's cp.minimum / cp.exp propagate it (fmin alone would drop the NaN and

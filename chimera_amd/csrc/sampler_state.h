@@ -12,6 +12,8 @@ struct chm_lp {
     double lam = 1.0;        // logit = -lam * beta * g / 2 + kappa * |dU|
     double kappa = 0.0;
     double beta_max = 1e300;  // slots at a larger beta (colder) use the uniform proposal
+    int lift = 0;            // lifted: logit tilt kappa * d * dU, d = dir[slot] flips when a
+    chm::DevBuf dir;         //   learned step is not accepted (nrep ints, +-1)
     int nocc = 0, noccp = 0; // occupancy entries per candidate (2 ncoord + 2), padded to 16
     int nfeat = 0, kp = 0;   // one-hot features nsp * nocc, padded to 32
     int nparam = 0;          // flat FP32 parameters
@@ -21,6 +23,7 @@ struct chm_lp {
     chm::DevBuf params, adam_m, adam_v;          // FP32 masters and Adam moments
     chm::DevBuf w1t, w1, w2t, w2, w3q;           // bf16 copies in both orders; w3 rounded
     chm::DevBuf logit;                           // (nrep, ncand) scores of the pending step
+    chm::DevBuf gcand, ducand;                   // (nrep, ncand) network output and dU of each
     chm::DevBuf pend_occ;                        // (nrep, noccp) occupancy of each pending proposal
     chm::DevBuf rec_occ, rec_y, rec_meta;        // move-record ring
     int rec_cap = 0;

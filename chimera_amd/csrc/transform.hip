@@ -102,6 +102,21 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K,
     }
 }
 
+// out[r][u] = (a + (b - a) frac) inside with a, b = chi[r][idx], chi[r][idx + 1]: the
+// reference's linear resample onto the uniform grid (exafs_gpu/fourier.py:69-73), each
+// operation rounded on its own (no fused multiply-add) so the result is NumPy's bit for bit
+__global__ void k_resample(const double *__restrict__ chi, const int *__restrict__ idx,
+                           const double *__restrict__ frac, const double *__restrict__ inside,
+                           double *__restrict__ out, int nrows, int nk, int nfft) {
+#pragma clang fp contract(off)
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long)nrows * nfft) return;
+    const int r = (int)(tid / nfft), u = (int)(tid % nfft);
+    const int i = idx[u];
+    const double a = chi[(long)r * nk + i], b = chi[(long)r * nk + i + 1];
+    out[tid] = (a + (b - a) * frac[u]) * inside[u];
+}
+
 __global__ void k_add_extra(double *__restrict__ x, const double *__restrict__ extra, long n,
                             int per) {
     long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -154,7 +169,9 @@ extern "C" int chm_ft_create(chm_ctx *ctx, int nchan, int nk_model, int nfft, in
         ft->edge_nkw.push_back(nkw);
     }
     const int ncol_e = nkw * nR * 2;
-    chm::DevBuf didx, dfrac, dins, dw, dk, dR, dce, dkwl;
+    chm::DevBuf dw, dk, dR, dce, dkwl;
+    chm::DevBuf &didx = ft->ridx, &dfrac = ft->rfrac, &dins = ft->rins;   // kept for resample
+    ft->nfft = nfft;
     auto up = [&](chm::DevBuf &b, const void *h, size_t n) -> int {
         int rc = b.ensure(n);
         if (rc) return rc;
@@ -251,6 +268,26 @@ extern "C" int chm_ft_apply_edge(chm_ft *ft, int edge, int nrows, const double *
     if (rc) return rc;
     CHM_HIP(hipMemcpyAsync(out, ft->out.p, sizeof(double) * nout, hipMemcpyDeviceToHost,
                            ctx->stream));
+    CHM_HIP(hipStreamSynchronize(ctx->stream));
+    return CHM_OK;
+}
+
+extern "C" int chm_ft_resample(chm_ft *ft, int channel, int nrows, const double *chi, double *out) {
+    CHM_REQUIRE(ft && nrows > 0 && chi && out, CHM_E_ARG, "bad arguments");
+    CHM_REQUIRE(channel >= 0 && channel < ft->nchan, CHM_E_ARG, "no such channel");
+    chm_ctx *ctx = ft->ctx;
+    CHM_HIP(hipSetDevice(ctx->device));
+    const size_t nin = (size_t)nrows * ft->nk, nout = (size_t)nrows * ft->nfft;
+    int rc;
+    if ((rc = ft->in.ensure(sizeof(double) * nin)) || (rc = ft->out.ensure(sizeof(double) * nout)))
+        return rc;
+    CHM_HIP(hipMemcpyAsync(ft->in.p, chi, sizeof(double) * nin, hipMemcpyHostToDevice, ctx->stream));
+    const size_t off = (size_t)channel * ft->nfft;
+    k_resample<<<dim3((unsigned)((nout + 255) / 256)), 256, 0, ctx->stream>>>(
+        ft->in.as<double>(), ft->ridx.as<int>() + off, ft->rfrac.as<double>() + off,
+        ft->rins.as<double>() + off, ft->out.as<double>(), nrows, ft->nk, ft->nfft);
+    CHM_LAUNCH_CHECK();
+    CHM_HIP(hipMemcpyAsync(out, ft->out.p, sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
     CHM_HIP(hipStreamSynchronize(ctx->stream));
     return CHM_OK;
 }

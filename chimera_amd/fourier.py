@@ -8,8 +8,9 @@ kernel into one matrix per channel and `to_R` is a GEMM against it.  The result 
 numpy's forward FFT of the windowed, resampled chi at the FFT grid points
 R_j = j pi/(nfft dk); `R` may also be any other grid (the sampler's 0.02 A window).
 
-Not provided: `resample` on its own (the transform folds it) and the Method-B adjoints
-`grad_resample` / `grad_magnitude` (fourier.py:86-121; no caller on this path).
+`resample` (the linear resample onto the uniform grid on its own) runs on the device
+too.  Not provided: the Method-B adjoints `grad_resample` / `grad_magnitude`
+(fourier.py:86-121; no caller on this path).
 """
 from __future__ import annotations
 
@@ -76,6 +77,19 @@ class FourierTransform:
         lead = chi.shape[:-1]
         out = self._device().apply(chi.reshape(-1, 1, chi.shape[-1]))
         return out.reshape(lead + (self.R.size,))
+
+    def resample(self, chi):
+        """chi (..., nk) on the model grid -> (..., nfft) on the uniform grid
+        (fourier.py:69-73): linear interpolation with the clamped fraction, zero outside
+        the model grid; on the device, NumPy's arithmetic bit for bit."""
+        chi = f64(chi)
+        lead = chi.shape[:-1]
+        rows = chi.reshape(-1, chi.shape[-1])
+        if rows.shape[1] != self.k_model.size:
+            raise ValueError(f"chi has {rows.shape[1]} points, the model grid {self.k_model.size}")
+        out = np.empty((rows.shape[0], self.nfft))
+        check(_lib.load().chm_ft_resample(self._device().h, 0, rows.shape[0], f64(rows), out))
+        return out.reshape(lead + (self.nfft,))
 
     def magnitude(self, chi):
         return np.abs(self.to_R(chi))

@@ -35,12 +35,14 @@ HID = 256
 
 
 class LearnedProposal:
-    def __init__(self, sampler, seed=0, ncand=256, eps=0.1, lam=1.0, kappa=0.0, beta_max=None):
+    def __init__(self, sampler, seed=0, ncand=256, eps=0.1, lam=1.0, kappa=0.0, beta_max=None,
+                 lift=False):
         if sampler._S is None:
             raise RuntimeError("call init_state on the sampler first")
         self.S = sampler
         self.ncand, self.eps, self.lam, self.kappa = int(ncand), float(eps), float(lam), float(kappa)
         self.beta_max = 1e300 if beta_max is None else float(beta_max)
+        self.lift = bool(lift)
         n = [C.c_int(0) for _ in range(4)]
         check(_lib.load().chm_lp_info(sampler._S, *[C.byref(v) for v in n]))
         self.nparam, self.nocc, self.nfeat, self.kp = (int(v.value) for v in n)
@@ -87,7 +89,7 @@ class LearnedProposal:
         return out
 
     # ------------------------------------------------------------ proposal mode
-    def enable(self, on=True, ncand=None, eps=None, lam=None, kappa=None, beta_max=None):
+    def enable(self, on=True, ncand=None, eps=None, lam=None, kappa=None, beta_max=None, lift=None):
         if ncand is not None:
             self.ncand = int(ncand)
         if eps is not None:
@@ -98,9 +100,11 @@ class LearnedProposal:
             self.kappa = float(kappa)
         if beta_max is not None:
             self.beta_max = float(beta_max)
+        if lift is not None:
+            self.lift = bool(lift)
         self.mode = 1 if on else 0
         check(_lib.load().chm_lp_configure(self.S._S, self.mode, self.ncand, self.eps, self.lam,
-                                           self.kappa, self.beta_max))
+                                           self.kappa, self.beta_max, int(self.lift)))
 
     def disable(self):
         self.enable(False)

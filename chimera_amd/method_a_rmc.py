@@ -102,6 +102,9 @@ class RMCSampler:
         self.natoms = fm.natoms
         self.npairs = fm.npairs
         self.nk = fm.nk
+        if ms_library is not None and not hasattr(ms_library, "stencil"):
+            from .mspaths import MSLibrary
+            ms_library = MSLibrary.from_protocol(ms_library, fm.cell)
         self.ms = ms_library
 
         i_np, j_np = np.asarray(fm.i_idx), np.asarray(fm.j_idx)

@@ -184,14 +184,19 @@ class MSLibrary:
     """
 
     def __init__(self, cell, nsp, rmax_ms, sigma2_nn=0.006, r_nn=None, c_ms=1.3, amp=1.0,
-                 library=None):
+                 library=None, classes=None, fixed_table=None):
         """`library`: a PathLibrary (prepared on the forward model's k grid) whose
         classes and FEFF tables replace the enumerated classes and synthetic rows; the
-        table's lattice scale is then cell.unit / library.unit."""
+        table's lattice scale is then cell.unit / library.unit.  `classes` (PathClass
+        list) and `fixed_table` ((nsp * nblk, nk) rows) instead take both from a
+        library made elsewhere (see from_protocol)."""
         self.cell = cell
         self.nsp = int(nsp)
         self.library = library
-        if library is not None:
+        self.fixed_table = None if fixed_table is None else np.ascontiguousarray(fixed_table, np.float64)
+        if classes is not None:
+            self.classes = list(classes)
+        elif library is not None:
             if library.nsp != self.nsp:
                 raise ValueError(f"the library holds {library.nsp} species, the cell {self.nsp}")
             if library.lattice != cell.lattice:
@@ -215,10 +220,42 @@ class MSLibrary:
         self.a_scale = 1.0 if library is None else self.unit / library.unit
         self._build_stencil()
 
+    @classmethod
+    def from_protocol(cls, lib_obj, cell):
+        """A sampler-side library from an object of the reference's ms_library protocol
+        (chimera/mslibrary.py:341-492, SupercellMS): its path classes (`lib.classes`, each
+        with the representative `rep` in units of a0/2; the slot labelling, and so every
+        entry index, follows from the representative) and its entry table `T`
+        ((nsp * nblk, nk), absorber-major, one occurrence per entry, as set_params left
+        it; a device array is copied to the host).  The per-step protocol calls
+        (init / delta / commit / permute / full) are what this sampler does on the device
+        in count space, so only the classes and the table are taken."""
+        lib = getattr(lib_obj, "lib", None)
+        if lib is None or not hasattr(lib, "classes"):
+            raise ValueError("ms_library must be a chimera_amd MSLibrary or an object with "
+                             "lib.classes and T (the reference's SupercellMS protocol)")
+        T = getattr(lib_obj, "T", None)
+        if T is None:
+            raise ValueError("ms_library.T is not set: call its set_params(...) first")
+        T = T.get() if hasattr(T, "get") else np.asarray(T)
+        nsp = int(getattr(lib_obj, "nsp", getattr(lib, "nsp", 0)))
+        classes = [PathClass(np.asarray(c.rep, np.int64)) for c in lib.classes]
+        out = cls(cell, nsp, 0.0, classes=classes, fixed_table=T)
+        if out.fixed_table.shape[0] != nsp * out.nblk:
+            raise ValueError(f"T has {out.fixed_table.shape[0]} rows, the classes give "
+                             f"{nsp} x {out.nblk} entries")
+        if hasattr(lib_obj, "nblk") and int(lib_obj.nblk) != out.nblk:
+            raise ValueError("the object's nblk does not match its classes")
+        return out
+
     def table(self, fm):
         """(nsp * nblk, nk) term rows, one occurrence of every entry (absorber-major), on
         the device: synthetic rows from the forward model's tables, or the library's
         tabulated F, delta, lambda at the current parameters."""
+        if self.fixed_table is not None:
+            if self.fixed_table.shape[1] != fm.nk:
+                raise ValueError("the library table is on another k grid")
+            return self.fixed_table
         if self.library is not None:
             r, s2 = self.library.params(self.sigma2_nn, self.r_nn, self.c_ms, self.a_scale)
             lib = self.library

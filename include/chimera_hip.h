@@ -169,6 +169,11 @@ int chm_ft_apply(chm_ft *ft, int nrows, const double *chi, const double *chi_ext
  * out (nrows, nkw_e, nR, 2).  Used to transform count-space basis rows. */
 int chm_ft_apply_edge(chm_ft *ft, int edge, int nrows, const double *rows, double *out);
 
+/* Linear resample of nrows chi rows (nrows, nk) from the model grid onto channel
+ * `channel`'s uniform grid (nrows, nfft): FourierTransform.resample
+ * (exafs_gpu/fourier.py:69-73), the same arithmetic as the reference, bit for bit. */
+int chm_ft_resample(chm_ft *ft, int channel, int nrows, const double *chi, double *out);
+
 /* -------------------------------------------------------------------- sampler */
 
 /* Problem geometry of a count-space swap sampler (every pair and path term depends
@@ -345,9 +350,13 @@ int chm_lp_set_params(chm_sampler *s, const float *flat);
 int chm_lp_get_params(chm_sampler *s, float *flat);
 /* mode 0: uniform proposals (the plain sampler); mode 1: learned proposal with the
  * uniform one mixed in at probability eps and used on every slot whose beta exceeds
- * beta_max (cold rungs).  ncand: multiple of 32 in 32..256. */
+ * beta_max (cold rungs).  ncand: multiple of 32 in 32..256.  lift = 1: a lifted
+ * (direction-persistent) proposal: each slot carries d = +-1, the tilt is kappa * d * dU
+ * instead of kappa * |dU|, the reverse probability is scored with -d, and d flips when a
+ * learned step is not accepted (skew detailed balance; the posterior of the
+ * configurations is unchanged). */
 int chm_lp_configure(chm_sampler *s, int mode, int ncand, double eps, double lam, double kappa,
-                     double beta_max);
+                     double beta_max, int lift);
 /* Move records: a ring of `capacity` (occupancy, nfit * dchi2, rung | accepted << 16)
  * entries written by every step (null moves: meta -1).  0 disables. */
 int chm_lp_set_records(chm_sampler *s, int capacity);

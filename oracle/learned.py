@@ -27,6 +27,13 @@ where q sums the probabilities of every candidate naming the same unordered pair
 With probability eps, and on every rung colder than beta_max, a step uses the plain
 uniform proposal instead (a state-independent mixture of two kernels that each keep
 the posterior).
+
+Lifted variant (lift=True): each chain carries a direction d = +-1; the tilt is
+kappa * d * dU, the reverse probability is scored with -d, and d flips whenever a learned
+step is not accepted (also when no candidate is valid).  Skew detailed balance
+pi(x) T_d(x, y) = pi(y) T_-d(y, x) then keeps pi x (1/2, 1/2) stationary (Diaconis,
+Holmes & Neal, Ann. Appl. Probab. 10 (2000) 726; Turitsyn, Chertkov & Vucelja, Physica D
+240 (2011) 410); pinned by exact enumeration in tests/test_oracle_learned.py.
 """
 from __future__ import annotations
 
@@ -235,7 +242,13 @@ def select_uniform(seed, stepc, key):
     return u01(splitmix64((h0 ^ SEL_SALT) & M64))
 
 
-def logits(flat, sp, st, cands, nsp, beta, lam, kappa):
+def tilt(kappa, du, d=None):
+    """kappa |dU| (symmetric), or kappa d dU for a lifted proposal in direction d."""
+    du = np.asarray(du, float)
+    return kappa * np.abs(du) if d is None else kappa * d * du
+
+
+def logits(flat, sp, st, cands, nsp, beta, lam, kappa, d=None):
     occs, du, valid = [], [], []
     for a, b in cands:
         occ, na, nb = occupancy(sp, st, a, b)
@@ -244,7 +257,7 @@ def logits(flat, sp, st, cands, nsp, beta, lam, kappa):
         valid.append(sp[a] != sp[b])
     occs = np.array(occs)
     g, _ = forward(flat, occs, nsp)
-    lg = -lam * beta * g / 2.0 + kappa * np.abs(np.array(du, float))
+    lg = -lam * beta * g / 2.0 + tilt(kappa, du, d)
     lg = np.where(np.array(valid), lg, -np.inf)
     return lg, g, occs
 
@@ -270,28 +283,29 @@ def log_q(lg, same):
     return float(np.log(p[same].sum()) - np.log(p.sum()))
 
 
-def log_q_ratio(flat, sp, st, nsp, cands, lg, j, beta, lam, kappa):
+def log_q_ratio(flat, sp, st, nsp, cands, lg, j, beta, lam, kappa, d=None):
     """log q(x' -> x) - log q(x -> x') for candidate j of the set `cands` at state sp
-    (lg: the logits at sp)."""
+    (lg: the logits at sp; a lifted proposal in direction d is reversed with -d)."""
     a, b = cands[j]
-    same = np.array([{c, d} == {a, b} for c, d in cands])
+    same = np.array([{c, e} == {a, b} for c, e in cands])
     sp2 = sp.copy()
     sp2[a], sp2[b] = sp[b], sp[a]
-    lg2, _, _ = logits(flat, sp2, st, cands, nsp, beta, lam, kappa)
+    lg2, _, _ = logits(flat, sp2, st, cands, nsp, beta, lam, kappa, None if d is None else -d)
     return log_q(lg2, same) - log_q(lg, same)
 
 
-def propose_learned(flat, sp, st, nsp, seed, stepc, key, beta, ncand, lam, kappa):
+def propose_learned(flat, sp, st, nsp, seed, stepc, key, beta, ncand, lam, kappa, d=None):
     """(a, b, na, nb, sw, logqr, occ_j, margin) of the learned proposal at state sp
-    (margin: how far the selecting uniform fell from a bin edge, relative)."""
+    (margin: how far the selecting uniform fell from a bin edge, relative; d: the
+    direction of a lifted proposal)."""
     natoms = sp.size
     cands = candidates(seed, stepc, key, natoms, ncand)
-    lg, _, occs = logits(flat, sp, st, cands, nsp, beta, lam, kappa)
+    lg, _, occs = logits(flat, sp, st, cands, nsp, beta, lam, kappa, d)
     j, margin = select(lg, select_uniform(seed, stepc, key), with_margin=True)
     if j < 0:
         return 0, 0, int(sp[0]), int(sp[0]), 0, 0.0, None, 1.0
     a, b = cands[j]
-    logqr = log_q_ratio(flat, sp, st, nsp, cands, lg, j, beta, lam, kappa)
+    logqr = log_q_ratio(flat, sp, st, nsp, cands, lg, j, beta, lam, kappa, d)
     return a, b, int(sp[b]), int(sp[a]), 1, logqr, occs[j], margin
 
 
@@ -300,13 +314,17 @@ class LearnedSampler:
     """The plain sampler (oracle.rmc.PlainSampler) with the learned proposal mixed in:
     the checker of the device sampler in learned mode."""
 
-    def __init__(self, plain, flat, stencil, ncand=256, eps=0.0, lam=1.0, kappa=0.0, beta_max=1e300):
+    def __init__(self, plain, flat, stencil, ncand=256, eps=0.0, lam=1.0, kappa=0.0, beta_max=1e300,
+                 lift=False):
         self.S = plain
         self.flat = np.asarray(flat, np.float32)
         self.st = stencil
         self.ncand, self.eps, self.lam, self.kappa = ncand, eps, lam, kappa
         self.beta_max = beta_max
         self.margins = []
+        self.lift = bool(lift)
+        # directions start alternating over the chains (d travels with its configuration)
+        self.dir = np.where(np.arange(plain.nrep) % 2 == 1, -1, 1)
 
     def step(self):
         from .rmc import accept_uniform, propose
@@ -317,15 +335,18 @@ class LearnedSampler:
             sp = S.species[c]
             u = accept_uniform(S.seed, S.stepc, c)
             margin = 1.0
-            if uses_uniform(S.seed, S.stepc, c, self.eps) or S.beta[c] > self.beta_max:
+            learned = not (uses_uniform(S.seed, S.stepc, c, self.eps) or S.beta[c] > self.beta_max)
+            if not learned:
                 a, b, na, nb, sw = propose(sp, S.seed, S.stepc, c)
                 lq = 0.0
             else:
                 a, b, na, nb, sw, lq, _, margin = propose_learned(
                     self.flat, sp, self.st, P.nsp, S.seed, S.stepc, c, S.beta[c], self.ncand,
-                    self.lam, self.kappa)
+                    self.lam, self.kappa, int(self.dir[c]) if self.lift else None)
             self.margins.append((S.stepc, c, margin))
             if not sw:
+                if self.lift and learned:
+                    self.dir[c] = -self.dir[c]
                 rec.append((a, b, 0, S.chi2[c], 0.0, u))
                 continue
             dchi, _ = P.delta_chi(sp, a, b, na, nb)
@@ -337,6 +358,8 @@ class LearnedSampler:
                 sp[a], sp[b] = na, nb
                 S.chi[c] = trial
                 S.chi2[c] = c2
+            elif self.lift and learned:
+                self.dir[c] = -self.dir[c]
             rec.append((a, b, int(acc), c2, x, u))
         imp = S.chi2 < S.best_chi2
         S.best_chi2 = np.where(imp, S.chi2, S.best_chi2)
@@ -347,4 +370,5 @@ class LearnedSampler:
         for it in range(nsteps):
             self.step()
             if exchange_every and (it + 1) % exchange_every == 0:
-                self.S.replica_exchange()
+                order = self.S.replica_exchange()
+                self.dir = self.dir[order]

@@ -130,10 +130,11 @@ def test_move_records_match_oracle(tern108):
     assert not bad, bad[:12]
 
 
-def test_learned_trajectory_matches_oracle(tern108):
+@pytest.mark.parametrize("lift", [False, True])
+def test_learned_trajectory_matches_oracle(tern108, lift):
     """Device sampler with the learned proposal (eps 0.3 uniform mixture, tilt on, the two
-    coldest rungs uniform) move by move against the oracle's restatement with the same
-    parameters.  A differing choice
+    coldest rungs uniform; symmetric or lifted tilt) move by move against the oracle's
+    restatement with the same parameters.  A differing choice
     is allowed only at a near tie of the selecting uniform or of the acceptance; the
     comparison stops there."""
     P, O = tern108
@@ -145,15 +146,15 @@ def test_learned_trajectory_matches_oracle(tern108):
     S.seed = seed
     S.set_recording(log_steps=nsteps)
     S.init_state(P.cfg0, P.cell.ideal)
-    lp.enable(ncand=64, eps=0.3, lam=1.0, kappa=0.2, beta_max=10.0)
-    _say(f"[run] device learned {nsteps} steps")
+    lp.enable(ncand=64, eps=0.3, lam=1.0, kappa=0.2, beta_max=10.0, lift=lift)
+    _say(f"[run] device learned {nsteps} steps lift={lift}")
     S.run(nsteps, exchange_every=exch, log_every=10 ** 9)
     log = S.move_log(nsteps)
     lp.disable()
     R = OR.PlainSampler(O, nrep=S.nrep, seed=seed)
     R.init_state(P.cfg0)
     L = LP.LearnedSampler(R, flat, _stencil(P), ncand=64, eps=0.3, lam=1.0, kappa=0.2,
-                          beta_max=10.0)
+                          beta_max=10.0, lift=lift)
     _say("[run] oracle learned")
     L.run(nsteps, exchange_every=exch)
     margins = {(s, c): m for s, c, m in L.margins}

@@ -492,3 +492,85 @@ def test_cpu_worker_runs(tmp_path):
     assert out.returncode == 0, out.stderr
     r = json.loads(out.stdout.strip().splitlines()[-1])
     assert r["moves"] == 3 * r["steps"] and r["seconds"] > 0
+
+
+# ------------------------------------------------------------------ fit scales, R-factor
+def _fit_data(seed, complex_=True, nchan=6, nR=40):
+    rng = np.random.default_rng(seed)
+    m = rng.standard_normal((nchan, nR)) + (1j * rng.standard_normal((nchan, nR)) if complex_ else 0)
+    d = rng.standard_normal((nchan, nR)) + (1j * rng.standard_normal((nchan, nR)) if complex_ else 0)
+    mask = np.ones(nR, bool)
+    mask[:5] = mask[-3:] = False
+    return m, d, mask
+
+
+@pytest.mark.parametrize("complex_", [True, False])
+def test_fit_scales_minimise_the_residual(complex_):
+    """chimera/model.py:497-512: the fitted amplitude (per channel and per edge) is the
+    minimum of the windowed residual; any perturbation raises the R-factor."""
+    from oracle import fitting as OFIT
+    m, d, mask = _fit_data(1, complex_)
+    edge = np.repeat(np.arange(3), 2)
+    for e in (None, edge):
+        s = OFIT.fit_scales(m, d, mask, e)
+        r0 = OFIT.r_factor(m, d, mask, s)
+        rng = np.random.default_rng(2)
+        for _ in range(20):
+            dv = rng.standard_normal(s.size) * 1e-3
+            if e is not None:
+                dv = dv[e]                    # one amplitude per edge
+            assert OFIT.r_factor(m, d, mask, s + dv) > r0
+
+
+def test_r_factor_closed_forms():
+    """R = 0 for data = s0 * model (and the fit returns s0); with data = s0 m + n, n
+    orthogonal to m on the window, the fit returns s0 and R = |n|^2/(s0^2|m|^2 + |n|^2);
+    data scaled by c scales the amplitude by c and leaves R; per-edge fitting equals a
+    fit over the edge's channels laid end to end (chimera/model.py:497-540)."""
+    from oracle import fitting as OFIT
+    m, d, mask = _fit_data(3)
+    s0 = 0.83
+    s = OFIT.fit_scales(m, s0 * m, mask)
+    np.testing.assert_allclose(s, s0, rtol=1e-14)
+    assert OFIT.r_factor(m, s0 * m, mask, s) < 1e-28
+    # noise orthogonal to each channel's model on the window (Re<m, n> = 0)
+    rng = np.random.default_rng(4)
+    n = rng.standard_normal(m.shape) + 1j * rng.standard_normal(m.shape)
+    n[:, ~mask] = 0
+    for c in range(m.shape[0]):
+        mc = np.where(mask, m[c], 0)
+        n[c] -= np.real(np.vdot(mc, n[c])) / np.real(np.vdot(mc, mc)) * mc
+    data = s0 * m + n
+    s = OFIT.fit_scales(m, data, mask)
+    np.testing.assert_allclose(s, s0, rtol=1e-12)
+    nm = np.sum(np.abs(n[:, mask]) ** 2)
+    mm = np.sum(np.abs(m[:, mask]) ** 2)
+    np.testing.assert_allclose(OFIT.r_factor(m, data, mask, s), nm / (s0 ** 2 * mm + nm), rtol=1e-12)
+    # scale covariance
+    s1 = OFIT.fit_scales(m, d, mask)
+    np.testing.assert_allclose(OFIT.fit_scales(m, 2.5 * d, mask), 2.5 * s1, rtol=1e-13)
+    np.testing.assert_allclose(OFIT.r_factor(m, 2.5 * d, mask, 2.5 * s1),
+                               OFIT.r_factor(m, d, mask, s1), rtol=1e-12)
+    # per edge == end-to-end channels of the edge
+    edge = np.repeat(np.arange(3), 2)
+    se = OFIT.fit_scales(m, d, mask, edge)
+    for e in range(3):
+        cat_m = np.concatenate([m[c][mask] for c in np.flatnonzero(edge == e)])[None]
+        cat_d = np.concatenate([d[c][mask] for c in np.flatnonzero(edge == e)])[None]
+        one = OFIT.fit_scales(cat_m, cat_d, np.ones(cat_m.shape[1], bool))
+        np.testing.assert_allclose(se[edge == e], one[0], rtol=1e-12)
+
+
+@pytest.mark.parametrize("complex_", [True, False])
+def test_host_fit_scales_and_r_factor_equal_the_oracle(complex_):
+    """chimera_amd.workloads.fit_scales / r_factor (the host post-processing of a fit)
+    against the oracle on random channels, per channel and per edge."""
+    from chimera_amd import workloads as W
+    from oracle import fitting as OFIT
+    m, d, mask = _fit_data(5, complex_)
+    edge = np.repeat(np.arange(3), 2)
+    for e in (None, edge):
+        s_w, s_o = W.fit_scales(m, d, mask, e), OFIT.fit_scales(m, d, mask, e)
+        np.testing.assert_allclose(s_w, s_o, rtol=1e-13)
+        np.testing.assert_allclose(W.r_factor(m, d, mask, s_w), OFIT.r_factor(m, d, mask, s_o),
+                                   rtol=1e-13)

@@ -112,6 +112,80 @@ def test_learned_proposal_detailed_balance(lam, kappa):
     assert T.sum() > 1.0                                    # moves do happen
 
 
+@pytest.mark.parametrize("lam,kappa", [(1.0, 0.7), (0.5, 2.0)])
+def test_lifted_proposal_skew_balance_and_stationarity(lam, kappa):
+    """The lifted proposal on a FIXED candidate set over every state x direction of a
+    binary 8-site toy: skew detailed balance pi(x) T_d(x, y) = pi(y) T_-d(y, x) for every
+    move, and with the direction flipped on every non-accepted step the product law
+    pi x (1/2, 1/2) is stationary (while the unlifted kernel with the signed tilt, a
+    mistake one could make, is not reversible)."""
+    cell, st = _toy()
+    nsp, n = 2, cell.natoms
+    nfeat = nsp * (2 * st.ncoord + 2)
+    flat = LP.init_params(nfeat, 2 * st.ncoord + 2, seed=5, w3_scale=3.0)
+    cands = [(0, 3), (1, 6), (3, 0), (2, 5), (4, 7), (0, 3), (5, 5), (6, 1), (2, 7)]
+    states = [np.array(c) for c in itertools.product((0, 1), repeat=n) if sum(c) == n // 2]
+    index = {tuple(s): k for k, s in enumerate(states)}
+    ns = len(states)
+    rng = np.random.default_rng(11)
+    E = rng.standard_normal(ns) * 1.5
+    beta = 0.8
+    pi = np.exp(-beta * E)
+    pi /= pi.sum()
+    # T[(x, d) -> (y, d')] with d index 0: +1, 1: -1
+    T = np.zeros((2 * ns, 2 * ns))
+    Td = {1: np.zeros((ns, ns)), -1: np.zeros((ns, ns))}
+    for k, sp in enumerate(states):
+        for di, d in enumerate((1, -1)):
+            lg, _, _ = LP.logits(flat, sp, st, cands, nsp, beta, lam, kappa, d)
+            moved = 0.0
+            if np.isfinite(lg).any():
+                p = np.exp(lg - lg.max())
+                p /= p.sum()
+                for jj, (a, b) in enumerate(cands):
+                    if not np.isfinite(lg[jj]):
+                        continue
+                    y = sp.copy()
+                    y[a], y[b] = sp[b], sp[a]
+                    ky = index[tuple(y)]
+                    lqr = LP.log_q_ratio(flat, sp, st, nsp, cands, lg, jj, beta, lam, kappa, d)
+                    A = min(1.0, np.exp(-beta * (E[ky] - E[k]) + lqr))
+                    T[di * ns + k, di * ns + ky] += p[jj] * A
+                    Td[d][k, ky] += p[jj] * A
+                    moved += p[jj] * A
+            T[di * ns + k, (1 - di) * ns + k] += 1.0 - moved        # not accepted: flip d
+    for d in (1, -1):
+        F = pi[:, None] * Td[d]
+        assert np.allclose(F, (pi[:, None] * Td[-d]).T, rtol=1e-10, atol=1e-15)
+    assert np.allclose(T.sum(1), 1.0)
+    mu = np.concatenate([pi, pi]) / 2.0
+    assert np.allclose(mu @ T, mu, rtol=1e-10, atol=1e-15)
+    assert Td[1].sum() > 1.0
+    # a plausible mistake, the reverse move scored in the forward direction (d, not -d)
+    # with the same flips, breaks the stationarity of pi x (1/2, 1/2)
+    Tw = np.zeros((2 * ns, 2 * ns))
+    for k, sp in enumerate(states):
+        for di, d in enumerate((1, -1)):
+            lg, _, _ = LP.logits(flat, sp, st, cands, nsp, beta, lam, kappa, d)
+            moved = 0.0
+            if np.isfinite(lg).any():
+                p = np.exp(lg - lg.max())
+                p /= p.sum()
+                for jj, (a, b) in enumerate(cands):
+                    if not np.isfinite(lg[jj]):
+                        continue
+                    y = sp.copy()
+                    y[a], y[b] = sp[b], sp[a]
+                    ky = index[tuple(y)]
+                    same = np.array([{c, e} == {a, b} for c, e in cands])
+                    lg2, _, _ = LP.logits(flat, y, st, cands, nsp, beta, lam, kappa, d)
+                    A = min(1.0, np.exp(-beta * (E[ky] - E[k]) + LP.log_q(lg2, same) - LP.log_q(lg, same)))
+                    Tw[di * ns + k, di * ns + ky] += p[jj] * A
+                    moved += p[jj] * A
+            Tw[di * ns + k, (1 - di) * ns + k] += 1.0 - moved
+    assert np.abs(mu @ Tw - mu).max() > 1e-4
+
+
 def test_candidate_stream_is_uniform_and_state_free():
     c1 = LP.candidates(11, 5, 3, 4000, 256)
     c2 = LP.candidates(11, 5, 3, 4000, 256)
