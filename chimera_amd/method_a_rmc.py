@@ -687,7 +687,8 @@ class RMCSampler:
 
     # ------------------------------------------------------------- teardown
     def _destroy(self):
-        if self._S is not None and _lib._lib is not None:
+        # (a constructor that raised leaves no handle)
+        if getattr(self, "_S", None) is not None and _lib._lib is not None:
             _lib._lib.chm_sampler_destroy(self._S)
         self._S = None
 
