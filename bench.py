@@ -216,6 +216,26 @@ def forward_bench(P, S, iters=10):
         out[key] = {"ms": round(ms, 4), "terms_per_s": float(f"{terms / (ms * 1e-3):.4g}")}
     out["chi_evals_per_s_f32"] = round(nconf / (out["path_sum_f32"]["ms"] * 1e-3), 1)
     out["chi_evals_per_s_f64"] = round(nconf / (out["path_sum_f64"]["ms"] * 1e-3), 1)
+    if P.ms is not None:
+        # full chi(k) with multiple scattering, end to end from the host (species in,
+        # chi(k) out): FP64 pair-space sum of the single-scattering pairs + the entry
+        # counts of every 3-/4-leg path from scratch + counts x entry table.  The work
+        # unit is the survey's: (ordered SS pairs + reversal-merged MS paths) x nk terms.
+        best = float("inf")
+        for _ in range(5):
+            fm.ctx.sync()
+            t0 = time.perf_counter()
+            S.chi_k_of(sp)
+            fm.ctx.sync()
+            best = min(best, time.perf_counter() - t0)
+        p_ms = int(P.cell.natoms) * P.ms.merged_per_absorber
+        terms_ms = float(nconf) * (fm.npairs + p_ms) * int(P.k.size)
+        out["full_chi_with_ms_f64"] = {
+            "ms": round(1e3 * best, 4), "configs": nconf, "ms_paths_merged": p_ms,
+            "chi_evals_per_s": round(nconf / best, 1),
+            "path_term_equivalents_per_s": float(f"{terms_ms / best:.4g}"),
+            "note": "host to host: species upload, pair-space FP64 SS sum, MS entry counts "
+                    "from scratch and the entry-table product, chi(k) download"}
     # the transcendental bound of an FP32 term (exp, sin and a reciprocal on the
     # quarter-rate unit: 256 CUs x 16 lanes x 2.4 GHz / 3 = 3.28e12 terms/s)
     out["path_sum_f32"]["frac_of_transcendental_bound"] = round(
@@ -223,38 +243,62 @@ def forward_bench(P, S, iters=10):
     return out
 
 
-def batch_bench(args):
-    """Uneven problems stepped together (one launch per kernel per step, one hipGraph per
-    exchange period): aggregate replica swap moves/s."""
+def batch_leg(nprob, steps, warmup, exchange_every, precision, reanchor_every=512,
+              sequential=False, log=None):
+    """The uneven batch (batch_specs(nprob): fcc/bcc supercells of 256-4000 sites, 1-3
+    edges, series of 2-6 points, 16 replicas each) stepped together: one launch per kernel
+    per step, one hipGraph per exchange period; aggregate replica swap moves/s over
+    `steps` timed steps.  With `sequential`, also the same problems stepped one after
+    another for comparison."""
     from chimera_amd import batch as BT
-    log = lambda m: print(m, file=sys.stderr, flush=True)       # noqa: E731
-    specs = BT.batch_specs(args.batch)
+    specs = BT.batch_specs(nprob)
     t0 = time.perf_counter()
     probs = BT.build(specs)
     t_build = time.perf_counter() - t0
-    log(f"[bench] built {len(probs)} problems in {t_build:.1f} s")
-    B = BT.Batch(probs, precision=args.precision, reanchor_every=args.reanchor_every, log=log)
-    B.run(args.warmup, exchange_every=args.exchange_every)
-    steps = args.steps
-    dt = B.run(steps, exchange_every=args.exchange_every)
+    if log:
+        log(f"[bench] built {len(probs)} problems in {t_build:.1f} s")
+    B = BT.Batch(probs, precision=precision, reanchor_every=reanchor_every, log=log)
+    B.run(warmup, exchange_every=exchange_every)
+    dt = B.run(steps, exchange_every=exchange_every)
     nrep = B.replicas
     moves = nrep * steps / dt
     prof = B.profile(50)
     sites = [int(P.cell.natoms) for P in probs]
-    nsplits = [B._split(P.sampler) for P in probs]
-    # the same problems stepped one after another (each its own launches and graphs)
+    out = {"value": round(moves, 1), "unit": "replica swap moves/s", "steps": steps,
+           "warmup": warmup, "ms_per_step": round(1e3 * dt / steps, 5),
+           "problems": len(probs), "replicas": nrep, "sites_total": int(sum(sites)),
+           "sites": {str(n): sites.count(n) for n in sorted(set(sites))},
+           "lattices": {lat: sum(P.w.lattice == lat for P in probs) for lat in ("fcc", "bcc")},
+           "edges": {str(k): sum(len(P.w.edges or range(P.nsp)) == k for P in probs) for k in (1, 2, 3)},
+           "kernels_ms": {k: round(v, 5) for k, v in prof.items()},
+           "autotune": B.tuned, "nsplit": [B._split(P.sampler) for P in probs],
+           "build_s": round(t_build, 1)}
     B.close()
-    seq_steps = max(200, steps // 10)
-    t_seq = 0.0
-    for P in probs:
-        S = P.sampler
-        S.run(args.exchange_every, exchange_every=args.exchange_every, log_every=10 ** 9)   # graphs
-        P.fm.ctx.sync()
-        t1 = time.perf_counter()
-        S.run(seq_steps, exchange_every=args.exchange_every, log_every=10 ** 9)
-        P.fm.ctx.sync()
-        t_seq += time.perf_counter() - t1
-    seq_moves = nrep * seq_steps / t_seq
+    if sequential:
+        seq_steps = max(200, steps // 10)
+        t_seq = 0.0
+        for P in probs:
+            S = P.sampler
+            S.run(exchange_every, exchange_every=exchange_every, log_every=10 ** 9)   # graphs
+            P.fm.ctx.sync()
+            t1 = time.perf_counter()
+            S.run(seq_steps, exchange_every=exchange_every, log_every=10 ** 9)
+            P.fm.ctx.sync()
+            t_seq += time.perf_counter() - t1
+        out["sequential_moves_per_s"] = round(nrep * seq_steps / t_seq, 1)
+        out["batched_over_sequential"] = round(moves / out["sequential_moves_per_s"], 2)
+    del probs, B
+    return out
+
+
+def batch_bench(args):
+    """Uneven problems stepped together (one launch per kernel per step, one hipGraph per
+    exchange period): aggregate replica swap moves/s."""
+    log = lambda m: print(m, file=sys.stderr, flush=True)       # noqa: E731
+    r = batch_leg(args.batch, args.steps, args.warmup, args.exchange_every, args.precision,
+                  args.reanchor_every, sequential=True, log=log)
+    moves, steps, dt = r["value"], args.steps, r["ms_per_step"] * args.steps * 1e-3
+    nrep = r["replicas"]
     out = {
         "metric": "aggregate replica swap moves/s, batched uneven RMC problems",
         "value": round(moves, 1), "unit": "replica swap moves/s", "n_gpus": 1,
@@ -263,18 +307,16 @@ def batch_bench(args):
         "dtype": "f64 chi^2; " + {"bf16x3": "bf16x3 MFMA delta + f64 accumulator",
                                   "f32": "f32 delta + f64 accumulator", "f64": "f64 transform"}[args.precision],
         "data": "synthetic (seeded tables, seeded reference configurations + noise)",
-        "config": {"workload": f"batch of {len(probs)} uneven problems", "problems": len(probs),
-                   "replicas": nrep, "sites_total": int(sum(sites)),
-                   "sites": {str(n): sites.count(n) for n in sorted(set(sites))},
-                   "lattices": {lat: sum(P.w.lattice == lat for P in probs) for lat in ("fcc", "bcc")},
-                   "edges": {str(k): sum(len(P.w.edges or range(P.nsp)) == k for P in probs) for k in (1, 2, 3)},
+        "config": {"workload": f"batch of {r['problems']} uneven problems", "problems": r["problems"],
+                   "replicas": nrep, "sites_total": r["sites_total"], "sites": r["sites"],
+                   "lattices": r["lattices"], "edges": r["edges"],
                    "exchange_every": args.exchange_every},
-        "kernels_ms": {k: round(v, 5) for k, v in prof.items()},
-        "sequential_moves_per_s": round(seq_moves, 1),
-        "batched_over_sequential": round(moves / seq_moves, 2),
-        "autotune": B.tuned,
-        "nsplit": nsplits,
-        "build_s": round(t_build, 1),
+        "kernels_ms": r["kernels_ms"],
+        "sequential_moves_per_s": r["sequential_moves_per_s"],
+        "batched_over_sequential": r["batched_over_sequential"],
+        "autotune": r["autotune"],
+        "nsplit": r["nsplit"],
+        "build_s": r["build_s"],
     }
     print(json.dumps(out))
 
@@ -322,6 +364,13 @@ def main():
                     help="only time the forward-model kernels (path sums, adjoint) and exit")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-uniform proposal comparison")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the short precision study (every transform / accumulator mode)")
+    ap.add_argument("--precision-steps", type=int, default=2000)
+    ap.add_argument("--no-batch-leg", action="store_true",
+                    help="skip the short batch of uneven problems")
+    ap.add_argument("--batch-leg-problems", type=int, default=64)
+    ap.add_argument("--batch-leg-steps", type=int, default=2000)
     ap.add_argument("--ess-steps", type=int, default=100000)
     ap.add_argument("--ess-burn", type=int, default=10000)
     ap.add_argument("--train-steps", type=int, default=40000)
@@ -466,11 +515,25 @@ def main():
     if not args.no_forward:
         print("[bench] forward model", file=sys.stderr, flush=True)
         fwd = forward_bench(P, S)
+    c3 = None
+    if not args.no_precision:
+        print("[bench] precision study (short)", file=sys.stderr, flush=True)
+        from chimera_amd import precision_study
+        res = precision_study.study(args.workload, steps=args.precision_steps, problem=P)
+        c3 = {"steps_per_mode": args.precision_steps, "chik_f32_vs_f64": res["chik_f32_vs_f64"],
+              "modes": [{k: v for k, v in r.items() if k not in ("steps",)} for r in res["modes"]]}
+        S.precision, S.reanchor_every = args.precision, args.reanchor_every
+        S.init_state(P.cfg0, P.cell.ideal)
     lvu = None
     if not args.no_learned:
         lvu = learned_vs_uniform(P, S, args)
         print(f"[bench] learned/uniform alpha_1 ESS/s: {lvu['alpha1_ess_per_s_ratio']}",
               file=sys.stderr, flush=True)
+    c2 = None
+    if not args.no_batch_leg:
+        print("[bench] batch of uneven problems (short)", file=sys.stderr, flush=True)
+        c2 = batch_leg(args.batch_leg_problems, args.batch_leg_steps, 100, args.exchange_every,
+                       args.precision, args.reanchor_every)
     out = {
         "metric": "RMC swap moves/s & alpha_1 ESS/s at fixed R-factor, learned vs uniform proposal",
         "value": round(moves, 1),
@@ -498,6 +561,8 @@ def main():
         "fp64_reanchors": nanchor,
         "forward_model": fwd,
         "learned_vs_uniform": lvu,
+        "precision_study": c3,
+        "batch_uneven": c2,
         "autotune": {"nsplit": tuned["nsplit"],
                      "step_ms": {k: round(v, 5) for k, v in tuned["times_ms"].items()}},
     }

@@ -349,6 +349,10 @@ class MSLibrary:
         self.stencil = np.ascontiguousarray(np.array(stencil, np.int64).astype(np.int8).reshape(-1, 16))
         self.nstencil = self.stencil.shape[0]
         self.directed_per_absorber = int(sum(c.deg for c in self.classes))
+        # a path and its reverse are one term (chimera/mslibrary.py:29-31): directed images
+        # pair up except the palindromic ones
+        pal = sum(int(np.array_equal(img, img[::-1])) for c in self.classes for img, _ in c.images)
+        self.merged_per_absorber = (self.directed_per_absorber + pal) // 2
 
     @staticmethod
     def _rec(abs_off, slots, w, cls_off):

@@ -4,7 +4,8 @@ transform is an FP64 FFT of the full trial chi every step).
 For one workload it reports, against the exact FP64 path (FP64 pair-space path sum;
 chi(R) recomputed in FP64 from the integer counts; the FP64 path is checked against the
 FP64 NumPy oracle by the test suite):
-  * chi(k) of the FP32 path sum (FP64 phase reduction, FP64 accumulation);
+  * chi(k) of the FP32 path sum (FP32 terms on the hardware sine / exponential units,
+    summed in FP32 per run of one absorber species and in FP64 across runs);
   * Re/Im chi(R) of the running accumulator after a run, chi^2 and the R-factor computed
     from it, for split-BF16 and FP32 transforms, FP64 and FP32 accumulators, with and
     without the FP64 re-anchor;
@@ -91,9 +92,10 @@ def run_mode(P, label, precision, accumulator, reanchor, steps, seed, exchange_e
     return out
 
 
-def study(workload="fcc_crconi_4000", steps=20000, seed=7, modes=MODES):
+def study(workload="fcc_crconi_4000", steps=20000, seed=7, modes=MODES, problem=None):
     from . import workloads as W
-    P = W.build(workload)
+    P = W.build(workload) if problem is None else problem
+    saved = (P.sampler.precision, P.sampler.accumulator, P.sampler.reanchor_every, P.sampler.seed)
     res = {"workload": workload, "chik_f32_vs_f64": chik_errors(P)}
     rows = []
     ref_log = None
@@ -106,9 +108,14 @@ def study(workload="fcc_crconi_4000", steps=20000, seed=7, modes=MODES):
         by = lambda L: {(t, int(x[3])): int(x[2]) for t in range(L.shape[0]) for x in L[t]}
         a, b = by(ref_log), by(log)
         r["decisions_equal_to_exact"] = float(np.mean([a[k] == b.get(k, -1) for k in a]))
+        # alpha_1..alpha_4 means against the exact chain's (same seed and start)
+        r["alpha_abs_err_vs_exact"] = float(np.abs(np.asarray(r["alpha"]) -
+                                                   np.asarray(rows[0]["alpha"] if rows else r["alpha"])).max())
         rows.append(r)
     res["modes"] = rows
-    P.sampler.precision, P.sampler.accumulator, P.sampler.reanchor_every = "f64", "f64", 512
+    P.sampler.precision, P.sampler.accumulator, P.sampler.reanchor_every, P.sampler.seed = saved
+    P.sampler.set_recording(0, 0)
+    P.sampler.init_state(P.cfg0, P.cell.ideal)
     return res
 
 
