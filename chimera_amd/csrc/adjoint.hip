@@ -87,46 +87,57 @@ __global__ __launch_bounds__(1024) void k_adjoint_r(
     const double *__restrict__ lamtab, const double *__restrict__ ktab,
     const double *__restrict__ drtab, const double *__restrict__ c3tab,
     const double *__restrict__ s2tab, const double *__restrict__ gchi,
-    double *__restrict__ gr, int npairs, int nk, double s02, const chm::PolyCoefs pc) {
+    double *__restrict__ gr, int npairs, int nk, int kw, double s02, const chm::PolyCoefs pc) {
     extern __shared__ double smem[];
-    double *sf = smem;                  // f (NSP, nk)
-    double *sd = sf + NSP * nk;         // delta / 2 pi (NSP*NSP, nk)
-    double *sg = sd + NSP * NSP * nk;   // this configuration's dL/dchi (NSP, nk)
-    KRow *sk = reinterpret_cast<KRow *>(sg + NSP * nk);
+    double *sf = smem;                  // f (NSP, kw): the columns of the current k chunk
+    double *sd = sf + NSP * kw;         // delta / 2 pi (NSP*NSP, kw)
+    double *sg = sd + NSP * NSP * kw;   // this configuration's dL/dchi (NSP, kw)
+    KRow *sk = reinterpret_cast<KRow *>(sg + NSP * kw);
     const int ic = blockIdx.y;
-    for (int q = threadIdx.x; q < NSP * nk; q += kLanes) {
-        sf[q] = ftab[q];
-        sg[q] = gchi[(long)ic * NSP * nk + q];
-    }
-    for (int q = threadIdx.x; q < NSP * NSP * nk; q += kLanes) sd[q] = dtab[q] * (0.5 * kInvPi);
-    stage_krows(sk, ktab, lamtab, nk, s02);
-    __syncthreads();
     const int p = blockIdx.x * kLanes + threadIdx.x;
-    if (p >= npairs) return;
-    const long o = (long)ic * npairs + p;
+    const bool valid = p < npairs;
+    const long o = (long)ic * npairs + (valid ? p : 0);
     const int t = pt_all[o];
-    const int sh = shell[p];
+    const int sh = shell[valid ? p : 0];
     const int sa = t / NSP, sb = t - sa * NSP;
     const double re = r_all[o] + drtab[sh * NSP * NSP + t];
     const double ir = 1.0 / re;
     const double ir2 = ir * ir;
-    const double s2 = sig2[p] + s2tab[sh * NSP * NSP + t];
+    const double s2 = sig2[valid ? p : 0] + s2tab[sh * NSP * NSP + t];
     const double c3 = c3tab[sh];
-    const double *gs = sg + sa * nk;
-    const double *fs = sf + sb * nk;
-    const double *ds = sd + t * nk;
     double acc = 0.0;
-    for (int ik = 0; ik < nk; ++ik) {
-        const double g = gs[ik];
-        const KRow kr = sk[ik];
-        if (g == 0.0 || kr.pref == 0.0) continue;   // the k = 0 term is defined as 0
-        const double ph = fma(kr.kp, re, fma(-kr.c3k, c3, ds[ik]));
-        double sn, cs;
-        chm::sincos2pi(ph, pc, &sn, &cs);
-        const double amp = kr.pref * fs[ik] * ir2 * chm::exp_fin(fma(kr.c1, re, kr.c2 * s2), pc);
-        acc += g * amp * ((kr.c1 - 2.0 * ir) * sn + kr.tk * cs);
+    // the k grid in chunks of kw columns (one chunk when the tables fit the LDS), each
+    // staged by the workgroup and summed by every lane in k order
+    for (int kb = 0; kb < nk; kb += kw) {
+        const int kc = min(kw, nk - kb);
+        __syncthreads();                 // the previous chunk's columns are no longer read
+        for (int q = threadIdx.x; q < NSP * kc; q += kLanes) {
+            const int row = q / kc, c = q - row * kc;
+            sf[row * kw + c] = ftab[row * nk + kb + c];
+            sg[row * kw + c] = gchi[((long)ic * NSP + row) * nk + kb + c];
+        }
+        for (int q = threadIdx.x; q < NSP * NSP * kc; q += kLanes) {
+            const int row = q / kc, c = q - row * kc;
+            sd[row * kw + c] = dtab[row * nk + kb + c] * (0.5 * kInvPi);
+        }
+        stage_krows(sk, ktab + kb, lamtab + kb, kc, s02);
+        __syncthreads();
+        if (!valid) continue;
+        const double *gs = sg + sa * kw;
+        const double *fs = sf + sb * kw;
+        const double *ds = sd + t * kw;
+        for (int j = 0; j < kc; ++j) {
+            const double g = gs[j];
+            const KRow kr = sk[j];
+            if (g == 0.0 || kr.pref == 0.0) continue;   // the k = 0 term is defined as 0
+            const double ph = fma(kr.kp, re, fma(-kr.c3k, c3, ds[j]));
+            double sn, cs;
+            chm::sincos2pi(ph, pc, &sn, &cs);
+            const double amp = kr.pref * fs[j] * ir2 * chm::exp_fin(fma(kr.c1, re, kr.c2 * s2), pc);
+            acc += g * amp * ((kr.c1 - 2.0 * ir) * sn + kr.tk * cs);
+        }
     }
-    gr[o] = acc;
+    if (valid) gr[o] = acc;
 }
 
 // The same dL/dr on a uniform k grid with C3 = 0: e^{2ikr} advances along k by one
@@ -265,19 +276,23 @@ __global__ __launch_bounds__(512) void k_chi_soft(
     const double *__restrict__ sig2, const double *__restrict__ fctab,
     const double *__restrict__ fstab, const double *__restrict__ lamtab,
     const double *__restrict__ ktab, double *__restrict__ partial, int npairs, int natoms,
-    int nk, int k0, int pairs_per_wg, double s02, const chm::PolyCoefs pc) {
+    int nk, int k0, int pairs_per_wg, int kspan, double s02, const chm::PolyCoefs pc) {
     extern __shared__ double smem[];
     // the lane's own column of f cos(delta), f sin(delta): in registers up to 3 species
-    // (18 values), else in LDS
+    // (18 values), else the workgroup's columns in LDS
     constexpr bool kReg = NSP <= kSoftRegNsp;
     double *sfc = smem;
-    double *sfs = sfc + (kReg ? 0 : NSP * NSP * nk);
-    SoftRec<NSP> *tile = reinterpret_cast<SoftRec<NSP> *>(sfs + (kReg ? 0 : NSP * NSP * nk));
+    double *sfs = sfc + (kReg ? 0 : NSP * NSP * kspan);
+    SoftRec<NSP> *tile = reinterpret_cast<SoftRec<NSP> *>(sfs + (kReg ? 0 : NSP * NSP * kspan));
     const int ic = blockIdx.y, wg = blockIdx.x, nwg = gridDim.x;
     const int nthr = blockDim.x;
-    const int ik = k0 + (int)threadIdx.x;
-    const bool live = ik < nk;
-    const int ikc = live ? ik : nk - 1;
+    // this workgroup's k columns kb .. kb + kc - 1: chunk blockIdx.z of kspan columns
+    const int kb = k0 + (int)blockIdx.z * kspan;
+    const int kc = min(kspan, nk - kb);
+    const int ik = kb + (int)threadIdx.x;
+    const bool live = (int)threadIdx.x < kc;
+    const int ikl = live ? (int)threadIdx.x : kc - 1;   // column within the chunk
+    const int ikc = kb + ikl;                             // column of the k grid
     double fcr[kReg ? NSP * NSP : 1], fsr[kReg ? NSP * NSP : 1];
     if constexpr (kReg) {
 #pragma unroll
@@ -286,9 +301,10 @@ __global__ __launch_bounds__(512) void k_chi_soft(
             fsr[q] = fstab[q * nk + ikc];
         }
     } else {
-        for (int q = threadIdx.x; q < NSP * NSP * nk; q += nthr) {
-            sfc[q] = fctab[q];
-            sfs[q] = fstab[q];
+        for (int q = threadIdx.x; q < NSP * NSP * kc; q += nthr) {
+            const int row = q / kc, c = q - row * kc;
+            sfc[row * kspan + c] = fctab[row * nk + kb + c];
+            sfs[row * kspan + c] = fstab[row * nk + kb + c];
         }
     }
     const double kv = live ? ktab[ik] : 0.0;
@@ -342,7 +358,7 @@ __global__ __launch_bounds__(512) void k_chi_soft(
                 double A = 0.0, B = 0.0;
 #pragma unroll
                 for (int b = 0; b < NSP; ++b) {
-                    const int t = (a * NSP + b) * nk + ikc;
+                    const int t = (a * NSP + b) * kspan + ikl;
                     A = fma(rec.wj[b], kReg ? fcr[a * NSP + b] : sfc[t], A);
                     B = fma(rec.wj[b], kReg ? fsr[a * NSP + b] : sfs[t], B);
                 }
@@ -351,9 +367,10 @@ __global__ __launch_bounds__(512) void k_chi_soft(
         }
     }
     double *out = partial + ((long)ic * nwg + wg) * NSP * nk;
-    for (int q = threadIdx.x; q < k0; q += nthr)
+    if (blockIdx.z == 0)
+        for (int q = threadIdx.x; q < k0; q += nthr)
 #pragma unroll
-        for (int s = 0; s < NSP; ++s) out[s * nk + q] = 0.0;
+            for (int s = 0; s < NSP; ++s) out[s * nk + q] = 0.0;
     if (live) {
 #pragma unroll
         for (int s = 0; s < NSP; ++s) out[s * nk + ik] = pref * acc[s];
@@ -369,41 +386,53 @@ __global__ __launch_bounds__(1024) void k_adjoint_soft(
     const double *__restrict__ fstab, const double *__restrict__ lamtab,
     const double *__restrict__ ktab, const double *__restrict__ gchi,
     double *__restrict__ gr, double *__restrict__ gwi_out, double *__restrict__ gwj_out,
-    int npairs, int natoms, int nk, double dkp, double s02, const chm::PolyCoefs pc) {
+    int npairs, int natoms, int nk, int kw, double dkp, double s02, const chm::PolyCoefs pc) {
     extern __shared__ double smem[];
-    double *sfc = smem;
-    double *sfs = sfc + NSP * NSP * nk;
-    double *sg = sfs + NSP * NSP * nk;
-    KRow *sk = reinterpret_cast<KRow *>(sg + NSP * nk);
+    double *sfc = smem;                   // the columns of the current k chunk (kw wide)
+    double *sfs = sfc + NSP * NSP * kw;
+    double *sg = sfs + NSP * NSP * kw;
+    KRow *sk = reinterpret_cast<KRow *>(sg + NSP * kw);
     const int ic = blockIdx.y;
-    for (int q = threadIdx.x; q < NSP * NSP * nk; q += kLanes) {
-        sfc[q] = fctab[q];
-        sfs[q] = fstab[q];
-    }
-    for (int q = threadIdx.x; q < NSP * nk; q += kLanes) sg[q] = gchi[(long)ic * NSP * nk + q];
-    stage_krows(sk, ktab, lamtab, nk, s02);
-    __syncthreads();
     const int p = blockIdx.x * kLanes + threadIdx.x;
-    if (p >= npairs) return;
-    const long o = (long)ic * npairs + p;
+    const bool valid = p < npairs;
+    const int pp = valid ? p : 0;
+    const long o = (long)ic * npairs + pp;
     const double re = r_all[o];
     const double ir = 1.0 / re;
     const double ir2 = ir * ir;
-    const double s2 = sig2[p];
+    const double s2 = sig2[pp];
     double wi[NSP], wj[NSP], gwi[NSP], gwj[NSP];
     const double *wc = w_all + (long)ic * natoms * NSP;
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
-        wi[s] = wc[(long)iidx[p] * NSP + s];
-        wj[s] = wc[(long)jidx[p] * NSP + s];
+        wi[s] = wc[(long)iidx[pp] * NSP + s];
+        wj[s] = wc[(long)jidx[pp] * NSP + s];
         gwi[s] = gwj[s] = 0.0;
     }
     double acc = 0.0;
     // uniform k grid (dkp > 0): e^{2ikr} by the rotation recurrence of k_adjoint_r_rot
     double rs = 0.0, rc = 1.0, sn = 0.0, cs = 1.0;
     if (dkp > 0.0) chm::sincos2pi(dkp * re, pc, &rs, &rc);
-    for (int ik = 0; ik < nk; ++ik) {
-        const KRow kr = sk[ik];
+    // the k grid in chunks of kw columns (one chunk when the tables fit the LDS), each
+    // staged by the workgroup and summed by every lane in k order
+    for (int kb = 0; kb < nk; kb += kw) {
+      const int kc = min(kw, nk - kb);
+      __syncthreads();                   // the previous chunk's columns are no longer read
+      for (int q = threadIdx.x; q < NSP * NSP * kc; q += kLanes) {
+          const int row = q / kc, c = q - row * kc;
+          sfc[row * kw + c] = fctab[row * nk + kb + c];
+          sfs[row * kw + c] = fstab[row * nk + kb + c];
+      }
+      for (int q = threadIdx.x; q < NSP * kc; q += kLanes) {
+          const int row = q / kc, c = q - row * kc;
+          sg[row * kw + c] = gchi[((long)ic * NSP + row) * nk + kb + c];
+      }
+      stage_krows(sk, ktab + kb, lamtab + kb, kc, s02);
+      __syncthreads();
+      if (!valid) continue;
+      for (int ik = kb; ik < kb + kc; ++ik) {
+        const int j = ik - kb;
+        const KRow kr = sk[j];
         if (dkp == 0.0 || ik % kAnchorEvery == 0) {
             chm::sincos2pi(kr.kp * re, pc, &sn, &cs);
         } else {
@@ -424,12 +453,12 @@ __global__ __launch_bounds__(1024) void k_adjoint_soft(
         double accv = 0.0;
 #pragma unroll
         for (int a = 0; a < NSP; ++a) {
-            const double ga = sg[a * nk + ik];
+            const double ga = sg[a * kw + j];
             const double u = ga * wi[a];
             double A = 0.0, B = 0.0;
 #pragma unroll
             for (int b = 0; b < NSP; ++b) {
-                const int t = (a * NSP + b) * nk + ik;
+                const int t = (a * NSP + b) * kw + j;
                 const double fc = sfc[t], fs = sfs[t];
                 A = fma(wj[b], fc, A);
                 B = fma(wj[b], fs, B);
@@ -444,7 +473,9 @@ __global__ __launch_bounds__(1024) void k_adjoint_soft(
         acc = fma(base, accv, acc);
 #pragma unroll
         for (int b = 0; b < NSP; ++b) gwj[b] = fma(base, fma(sn, C[b], cs * D[b]), gwj[b]);
+      }
     }
+    if (!valid) return;
     gr[o] = acc;
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
@@ -458,6 +489,13 @@ __global__ __launch_bounds__(1024) void k_adjoint_soft(
 size_t adjoint_lds(int nsp, int nk, bool soft) {
     return soft ? sizeof(double) * (2 * nsp * nsp + nsp) * nk + sizeof(KRow) * nk
                 : sizeof(double) * (nsp + nsp * nsp + nsp) * nk + sizeof(KRow) * nk;
+}
+
+// k columns per chunk of the adjoint kernels: all of them when the tables fit 160 KB of
+// LDS, else the most that do (the kernels then restage chunk by chunk)
+int adjoint_kw(int nsp, int nk, bool soft) {
+    const size_t per = adjoint_lds(nsp, 1, soft);
+    return (int)std::max<size_t>(1, std::min<size_t>(nk, (160 * 1024) / per));
 }
 
 int soft_tables(chm_ctx *ctx);
@@ -498,15 +536,15 @@ int launch_adjoint_r(chm_ctx *ctx, int nconf) {
         CHM_LAUNCH_CHECK();
         return CHM_OK;
     }
-    const size_t lds = adjoint_lds(NSP, nk, false);
-    CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+    const int kw = adjoint_kw(NSP, nk, false);
+    const size_t lds = adjoint_lds(NSP, kw, false);
     hipFuncSetAttribute((const void *)k_adjoint_r<NSP>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds);
     k_adjoint_r<NSP><<<grid, kLanes, lds, ctx->stream>>>(
         ctx->r.as<double>(), ctx->ptype.as<int>(), ctx->shell.as<int>(), ctx->sig2.as<double>(),
         ctx->f.as<double>(), ctx->delta.as<double>(), ctx->lam.as<double>(), ctx->k.as<double>(),
         ctx->drtab.as<double>(), ctx->c3tab.as<double>(), ctx->s2tab.as<double>(),
-        ctx->gchi.as<double>(), ctx->gr.as<double>(), npairs, nk, ctx->s02, chm::kPolyCoefs);
+        ctx->gchi.as<double>(), ctx->gr.as<double>(), npairs, nk, kw, ctx->s02, chm::kPolyCoefs);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -516,12 +554,20 @@ int launch_chi_soft(chm_ctx *ctx, int nconf) {
     const int nk = ctx->nk, npairs = ctx->npairs;
     int k0 = 0;
     while (k0 < nk && !(ctx->h_k[k0] > 0.0)) ++k0;
-    const int nkz = std::max(1, nk - k0);
-    CHM_REQUIRE(nkz <= kMaxK, CHM_E_ARG, "at most 512 k points above 0");
-    const int nthr = std::min(kMaxK, (nkz + 63) / 64 * 64);
-    const size_t lds = sizeof(double) * (NSP <= kSoftRegNsp ? 0 : 2 * NSP * NSP * nk) +
-                       sizeof(SoftRec<NSP>) * nthr;
-    CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+    CHM_REQUIRE(k0 < nk, CHM_E_ARG, "no k point above 0");
+    const int nkz = nk - k0;
+    // k chunks (grid z): at most kMaxK lanes each, and (beyond three species) the chunk's
+    // f cos(delta) / f sin(delta) columns plus one pair tile within 160 KB of LDS
+    auto lds_of = [&](int span) {
+        const int thr = (span + 63) / 64 * 64;
+        return sizeof(double) * (NSP <= kSoftRegNsp ? 0 : 2 * NSP * NSP * (size_t)span) +
+               sizeof(SoftRec<NSP>) * thr;
+    };
+    int nkc = (nkz + kMaxK - 1) / kMaxK;
+    while (lds_of((nkz + nkc - 1) / nkc) > 160 * 1024) ++nkc;
+    const int kspan = (nkz + nkc - 1) / nkc;
+    const int nthr = (kspan + 63) / 64 * 64;
+    const size_t lds = lds_of(kspan);
     const int rounds = (npairs + nthr - 1) / nthr;
     int nwg = std::max(1, std::min(rounds, (2048 + nconf - 1) / nconf));
     const int ppw = ((rounds + nwg - 1) / nwg) * nthr;
@@ -530,11 +576,11 @@ int launch_chi_soft(chm_ctx *ctx, int nconf) {
     if (rc) return rc;
     hipFuncSetAttribute((const void *)k_chi_soft<NSP>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         (int)lds);
-    k_chi_soft<NSP><<<dim3(nwg, nconf), nthr, lds, ctx->stream>>>(
+    k_chi_soft<NSP><<<dim3(nwg, nconf, nkc), nthr, lds, ctx->stream>>>(
         ctx->r.as<double>(), ctx->w.as<double>(), ctx->i_idx.as<int>(), ctx->j_idx.as<int>(),
         ctx->sig2.as<double>(), ctx->soft_fc.as<double>(), ctx->soft_fs.as<double>(),
         ctx->lam.as<double>(), ctx->k.as<double>(), ctx->partial.as<double>(), npairs, ctx->natoms,
-        nk, k0, ppw, ctx->s02, chm::kPolyCoefs);
+        nk, k0, ppw, kspan, ctx->s02, chm::kPolyCoefs);
     CHM_LAUNCH_CHECK();
     if ((rc = ctx->chi.ensure(sizeof(double) * (size_t)nconf * NSP * nk))) return rc;
     return chm::reduce_partials(ctx, ctx->partial.as<double>(), ctx->chi.as<double>(), nwg,
@@ -544,8 +590,8 @@ int launch_chi_soft(chm_ctx *ctx, int nconf) {
 template <int NSP>
 int launch_adjoint_soft(chm_ctx *ctx, int nconf) {
     const int nk = ctx->nk, npairs = ctx->npairs;
-    const size_t lds = adjoint_lds(NSP, nk, true);
-    CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
+    const int kw = adjoint_kw(NSP, nk, true);
+    const size_t lds = adjoint_lds(NSP, kw, true);
     hipFuncSetAttribute((const void *)k_adjoint_soft<NSP>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     dim3 grid((npairs + kLanes - 1) / kLanes, nconf);
@@ -553,7 +599,7 @@ int launch_adjoint_soft(chm_ctx *ctx, int nconf) {
         ctx->r.as<double>(), ctx->w.as<double>(), ctx->i_idx.as<int>(), ctx->j_idx.as<int>(),
         ctx->sig2.as<double>(), ctx->soft_fc.as<double>(), ctx->soft_fs.as<double>(),
         ctx->lam.as<double>(), ctx->k.as<double>(), ctx->gchi.as<double>(), ctx->gr.as<double>(),
-        ctx->gwi.as<double>(), ctx->gwj.as<double>(), npairs, ctx->natoms, nk, uniform_dkp(ctx),
+        ctx->gwi.as<double>(), ctx->gwj.as<double>(), npairs, ctx->natoms, nk, kw, uniform_dkp(ctx),
         ctx->s02, chm::kPolyCoefs);
     CHM_LAUNCH_CHECK();
     return CHM_OK;

@@ -119,10 +119,11 @@ inline void ctx_release(chm_ctx *c) {
     delete c;
 }
 
-// Launch shape of the pair-space path sum (forward.hip): first lane's k index, lanes per
-// workgroup, workgroups per configuration, pairs per workgroup.
+// Launch shape of the pair-space path sum (forward.hip): first k index above 0, lanes per
+// workgroup, workgroups per configuration, pairs per workgroup, k chunks and their width.
 struct ChiShape {
     int k0, nthr, nwg, ppw;
+    int nkc, kspan;   // chunks of the k > 0 points (grid z) and columns per chunk
 };
 int chi_shape(const chm_ctx *ctx, int nconf, ChiShape &sh);
 // Path sum of nconf configurations from device distances r and pair types pt (nconf,

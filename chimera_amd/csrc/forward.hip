@@ -334,23 +334,31 @@ __global__ __launch_bounds__(512) void k_chi_pairs(
     const double *__restrict__ lamtab, const double *__restrict__ ktab,
     const double *__restrict__ drtab, const double *__restrict__ c3tab,
     const double *__restrict__ s2tab, double *__restrict__ partial, int npairs, int nk, int k0,
-    int pairs_per_wg, double s02, const chm::PolyCoefs pc) {
+    int pairs_per_wg, int kspan, double s02, const chm::PolyCoefs pc) {
     extern __shared__ double smem[];
-    double *sf = smem;                                       // f (NSP, nk)
-    double *sd = sf + NSP * nk;                              // delta/2pi - 1/4 (NSP*NSP, nk)
-    PairRec *tile = reinterpret_cast<PairRec *>(sd + NSP * NSP * nk);
+    double *sf = smem;                                       // f (NSP, kspan)
+    double *sd = sf + NSP * kspan;                           // delta/2pi - 1/4 (NSP*NSP, kspan)
+    PairRec *tile = reinterpret_cast<PairRec *>(sd + NSP * NSP * kspan);
     const int ic = blockIdx.y;
     const int w = blockIdx.x;
     const int nwg = gridDim.x;
     const int nthr = blockDim.x;
     constexpr double kInv2Pi = 1.0 / kTwoPi;
-    for (int q = threadIdx.x; q < NSP * nk; q += nthr) sf[q] = ftab[q];
-    for (int q = threadIdx.x; q < NSP * NSP * nk; q += nthr)
-        sd[q] = fma(dtab[q], kInv2Pi, -0.25);
+    // this workgroup's k columns kb .. kb + kc - 1: chunk blockIdx.z of kspan columns
+    const int kb = k0 + (int)blockIdx.z * kspan;
+    const int kc = min(kspan, nk - kb);
+    for (int q = threadIdx.x; q < NSP * kc; q += nthr) {
+        const int row = q / kc, c = q - row * kc;
+        sf[row * kspan + c] = ftab[row * nk + kb + c];
+    }
+    for (int q = threadIdx.x; q < NSP * NSP * kc; q += nthr) {
+        const int row = q / kc, c = q - row * kc;
+        sd[row * kspan + c] = fma(dtab[row * nk + kb + c], kInv2Pi, -0.25);
+    }
 
-    const int ik = k0 + (int)threadIdx.x;
-    const bool live = ik < nk;
-    const int ikc = live ? ik : nk - 1;                     // a valid column for dead lanes
+    const int ik = kb + (int)threadIdx.x;
+    const bool live = (int)threadIdx.x < kc;
+    const int ikc = live ? (int)threadIdx.x : kc - 1;      // LDS column (valid for dead lanes)
     const double kv = live ? ktab[ik] : 0.0;
     const double lv = live ? lamtab[ik] : 1.0;
     const double kp = kv * kInvPi;                          // revolutions of 2 k r per unit r
@@ -412,8 +420,8 @@ __global__ __launch_bounds__(512) void k_chi_pairs(
             const int ta = __builtin_amdgcn_readfirstlane(ra.t);
             const int tb = __builtin_amdgcn_readfirstlane(rb.t);
             double fea, sna, feb, snb;
-            pair_term<NSP>(ra, ta, sf, sd, nk, ikc, kp, c1, c2, c3k, pc, fea, sna);
-            pair_term<NSP>(rb, tb, sf, sd, nk, ikc, kp, c1, c2, c3k, pc, feb, snb);
+            pair_term<NSP>(ra, ta, sf, sd, kspan, ikc, kp, c1, c2, c3k, pc, fea, sna);
+            pair_term<NSP>(rb, tb, sf, sd, kspan, ikc, kp, c1, c2, c3k, pc, feb, snb);
             if (ta / NSP == run_sa && tb / NSP == run_sa) {    // both terms feed the run:
                 run = fma(fea, sna, run);                      // no branch between the two
                 run = fma(feb, snb, run);                      // evaluations
@@ -426,7 +434,7 @@ __global__ __launch_bounds__(512) void k_chi_pairs(
             const PairRec ra = tile[q];
             const int ta = __builtin_amdgcn_readfirstlane(ra.t);
             double fea, sna;
-            pair_term<NSP>(ra, ta, sf, sd, nk, ikc, kp, c1, c2, c3k, pc, fea, sna);
+            pair_term<NSP>(ra, ta, sf, sd, kspan, ikc, kp, c1, c2, c3k, pc, fea, sna);
             add(ta / NSP, fea, sna);
         }
 #pragma unroll
@@ -434,9 +442,10 @@ __global__ __launch_bounds__(512) void k_chi_pairs(
             if (run_sa == s) acc[s] += run;
     }
     double *out = partial + ((long)ic * nwg + w) * NSP * nk;
-    for (int q = threadIdx.x; q < k0; q += nthr)
+    if (blockIdx.z == 0)
+        for (int q = threadIdx.x; q < k0; q += nthr)
 #pragma unroll
-        for (int s = 0; s < NSP; ++s) out[s * nk + q] = 0.0;
+            for (int s = 0; s < NSP; ++s) out[s * nk + q] = 0.0;
     if (live) {
 #pragma unroll
         for (int s = 0; s < NSP; ++s) out[s * nk + ik] = pref * acc[s];
@@ -475,23 +484,32 @@ __global__ __launch_bounds__(512) void k_chi_pairs_f32(
     const double *__restrict__ lamtab, const double *__restrict__ ktab,
     const double *__restrict__ drtab, const double *__restrict__ c3tab,
     const double *__restrict__ s2tab, double *__restrict__ partial, int npairs, int nk, int k0,
-    int pairs_per_wg, double s02) {
+    int pairs_per_wg, int kspan, double s02) {
     extern __shared__ double smem[];
-    float *sf = reinterpret_cast<float *>(smem);            // f (NSP, nk)
-    float *sd = sf + NSP * nk;                              // delta / 2 pi (NSP*NSP, nk)
-    const int words = (NSP + NSP * NSP) * nk;
+    float *sf = reinterpret_cast<float *>(smem);            // f (NSP, kspan)
+    float *sd = sf + NSP * kspan;                           // delta / 2 pi (NSP*NSP, kspan)
+    const int words = (NSP + NSP * NSP) * kspan;
     PairRecF *tile = reinterpret_cast<PairRecF *>(sf + ((words + 7) & ~7));
     const int ic = blockIdx.y;
     const int w = blockIdx.x;
     const int nwg = gridDim.x;
     const int nthr = blockDim.x;
-    for (int q = threadIdx.x; q < NSP * nk; q += nthr) sf[q] = ftab32[q];
-    for (int q = threadIdx.x; q < NSP * NSP * nk; q += nthr) sd[q] = drev32[q];
+    // this workgroup's k columns kb .. kb + kc - 1: chunk blockIdx.z of kspan columns
+    const int kb = k0 + (int)blockIdx.z * kspan;
+    const int kc = min(kspan, nk - kb);
+    for (int q = threadIdx.x; q < NSP * kc; q += nthr) {
+        const int row = q / kc, c = q - row * kc;
+        sf[row * kspan + c] = ftab32[row * nk + kb + c];
+    }
+    for (int q = threadIdx.x; q < NSP * NSP * kc; q += nthr) {
+        const int row = q / kc, c = q - row * kc;
+        sd[row * kspan + c] = drev32[row * nk + kb + c];
+    }
 
     constexpr double kLog2e = 1.44269504088896340736;
-    const int ik = k0 + (int)threadIdx.x;
-    const bool live = ik < nk;
-    const int ikc = live ? ik : nk - 1;                     // a valid column for dead lanes
+    const int ik = kb + (int)threadIdx.x;
+    const bool live = (int)threadIdx.x < kc;
+    const int ikc = live ? (int)threadIdx.x : kc - 1;      // LDS column (valid for dead lanes)
     const double kv = live ? ktab[ik] : 0.0;
     const double lv = live ? lamtab[ik] : 1.0;
     const double kp = kv * kInvPi;                          // revolutions of 2 k r per unit r
@@ -560,20 +578,21 @@ __global__ __launch_bounds__(512) void k_chi_pairs_f32(
             const float x = khi * rec.rhi;                      // exact
             float ph = x - rintf(x);
             ph += fmaf(khi, rec.rlo, klo * rec.r);
-            ph += sd[t * nk + ikc] - c3k * rec.c3;
+            ph += sd[t * kspan + ikc] - c3k * rec.c3;
             const float sn = __builtin_amdgcn_sinf(ph);
             const float e = __builtin_amdgcn_exp2f(fmaf(c1, rec.r, fmaf(k2, rec.ea, rec.l2ir2)));
-            run = fmaf(sf[sb * nk + ikc] * e, sn, run);
+            run = fmaf(sf[sb * kspan + ikc] * e, sn, run);
         }
 #pragma unroll
         for (int s = 0; s < NSP; ++s)
             if (run_sa == s) acc[s] += (double)run;
     }
     double *out = partial + ((long)ic * nwg + w) * NSP * nk;
-    // columns below k0 (k <= 0) are zero; lane l also writes column l < k0
-    for (int q = threadIdx.x; q < k0; q += nthr)
+    // columns below k0 (k <= 0) are zero, written by the first chunk
+    if (blockIdx.z == 0)
+        for (int q = threadIdx.x; q < k0; q += nthr)
 #pragma unroll
-        for (int s = 0; s < NSP; ++s) out[s * nk + q] = 0.0;
+            for (int s = 0; s < NSP; ++s) out[s * nk + q] = 0.0;
     if (live) {
 #pragma unroll
         for (int s = 0; s < NSP; ++s) out[s * nk + ik] = pref * acc[s];
@@ -623,10 +642,21 @@ namespace chm {
 int chi_shape(const chm_ctx *ctx, int nconf, ChiShape &sh) {
     int k0 = 0;
     while (k0 < ctx->nk && !(ctx->h_k[k0] > 0.0)) ++k0;
-    const int nkz = std::max(1, ctx->nk - k0);
-    CHM_REQUIRE(nkz <= kMaxK, CHM_E_ARG, "at most 512 k points above 0");
+    CHM_REQUIRE(k0 < ctx->nk, CHM_E_ARG, "no k point above 0");
+    const int nkz = ctx->nk - k0;
+    // k chunks (grid z): at most kMaxK lanes each, and the chunk's FP64 table columns plus
+    // one pair tile within 160 KB of LDS (six species at 512 points take two chunks)
+    const size_t nsp = (size_t)ctx->nsp;
+    int nkc = (nkz + kMaxK - 1) / kMaxK;
+    for (;; ++nkc) {
+        const int span = (nkz + nkc - 1) / nkc;
+        const int thr = (span + 63) / 64 * 64;
+        if (sizeof(double) * (nsp + nsp * nsp) * (size_t)span + sizeof(PairRec) * thr <= 160 * 1024) break;
+    }
     sh.k0 = k0;
-    sh.nthr = std::min(kMaxK, (nkz + 63) / 64 * 64);
+    sh.nkc = nkc;
+    sh.kspan = (nkz + nkc - 1) / nkc;
+    sh.nthr = (sh.kspan + 63) / 64 * 64;
     const int rounds = (ctx->npairs + sh.nthr - 1) / sh.nthr;
     int nwg = std::max(1, std::min(rounds, (2048 + nconf - 1) / nconf));
     sh.ppw = ((rounds + nwg - 1) / nwg) * sh.nthr;
@@ -644,9 +674,9 @@ template <int NSP>
 int launch_chi_into(chm_ctx *ctx, hipStream_t st, int nconf, int precision, const chm::ChiShape &sh,
                     const double *r, const int *pt, double *partial, double *chi) {
     const int nk = ctx->nk;
-    const dim3 grid(sh.nwg, nconf);
+    const dim3 grid(sh.nwg, nconf, sh.nkc);
     if (precision == CHM_PREC_F32) {
-        const size_t words = (size_t)(NSP + NSP * NSP) * nk;
+        const size_t words = (size_t)(NSP + NSP * NSP) * sh.kspan;
         const size_t lds = sizeof(float) * ((words + 7) & ~(size_t)7) + sizeof(PairRecF) * sh.nthr;
         CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
         hipFuncSetAttribute((const void *)k_chi_pairs_f32<NSP>,
@@ -655,9 +685,9 @@ int launch_chi_into(chm_ctx *ctx, hipStream_t st, int nconf, int precision, cons
             r, pt, ctx->shell.as<int>(), ctx->sig2.as<double>(), ctx->f32.as<float>(),
             ctx->drev32.as<float>(), ctx->lam.as<double>(), ctx->k.as<double>(),
             ctx->drtab.as<double>(), ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), partial,
-            ctx->npairs, nk, sh.k0, sh.ppw, ctx->s02);
+            ctx->npairs, nk, sh.k0, sh.ppw, sh.kspan, ctx->s02);
     } else {
-        const size_t lds = sizeof(double) * (NSP + NSP * NSP) * nk + sizeof(PairRec) * sh.nthr;
+        const size_t lds = sizeof(double) * (NSP + NSP * NSP) * sh.kspan + sizeof(PairRec) * sh.nthr;
         CHM_REQUIRE(lds <= 160 * 1024, CHM_E_ARG, "tables too large for LDS");
         hipFuncSetAttribute((const void *)k_chi_pairs<NSP>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -665,7 +695,7 @@ int launch_chi_into(chm_ctx *ctx, hipStream_t st, int nconf, int precision, cons
             r, pt, ctx->shell.as<int>(), ctx->sig2.as<double>(), ctx->f.as<double>(),
             ctx->delta.as<double>(), ctx->lam.as<double>(), ctx->k.as<double>(),
             ctx->drtab.as<double>(), ctx->c3tab.as<double>(), ctx->s2tab.as<double>(), partial,
-            ctx->npairs, nk, sh.k0, sh.ppw, ctx->s02, chm::kPolyCoefs);
+            ctx->npairs, nk, sh.k0, sh.ppw, sh.kspan, ctx->s02, chm::kPolyCoefs);
     }
     CHM_LAUNCH_CHECK();
     return launch_reduce(st, partial, chi, sh.nwg, NSP * nk, nconf);

@@ -73,6 +73,10 @@ __host__ __device__ __forceinline__ double exp_fin(double x, const PolyCoefs &c)
     constexpr double kRound = 6755399441055744.0;   // 1.5 * 2^52
     constexpr double kLn2Hi = 0.6931471805599453;
     constexpr double kLn2Lo = 2.3190468138462996e-17;
+    // below -1100 the result is 0 anyway; the clamp keeps the integer part n in range
+    // (an argument below about -1.5e9 would wrap it).  A comparison, not fmax, so that a
+    // NaN argument stays NaN.
+    x = (x < -1100.0) ? -1100.0 : x;
     const double t = fma(x, 1.4426950408889634, kRound);
     const double n = t - kRound;
     const int ni = (int)(uint32_t)__builtin_bit_cast(uint64_t, t);

@@ -56,3 +56,8 @@ def test_exp(lib):
     assert _eval(lib, 3, np.array([0.0]))[0] == 1.0
     np.testing.assert_array_equal(_eval(lib, 3, np.array([-800.0, -2000.0, 800.0])),
                                   [0.0, 0.0, np.inf])
+    # huge negative arguments (c1 = -2 / lambda with lambda near 0): 0, as exp gives, not a
+    # wrapped exponent; NaN stays NaN
+    big = np.array([-1100.0, -5e8, -1.6e9, -3e9, -1e12, -1e300, -np.inf])
+    np.testing.assert_array_equal(_eval(lib, 3, big), np.zeros(big.size))
+    assert np.isnan(_eval(lib, 3, np.array([np.nan]))[0])

@@ -1,8 +1,8 @@
 """The forward-model kernels at every species count they are built for (2..6): FP64 and
 FP32 path sums, dL/dpos, the relaxed-occupancy path sum and its adjoint, through the C ABI
 against the FP64 oracle, on a small fcc cell (two shells) with per-pair sigma^2 and
-cumulant tables; also a k grid without a k = 0 point (every lane live) and a non-uniform
-one.  dL/dpos is checked with C3 != 0 (per-term sine and cosine) and with C3 = 0, where a
+cumulant tables; also a k grid without a k = 0 point (every lane live), a non-uniform
+one, the workloads' default grid and a fine grid of 801 points.  dL/dpos is checked with C3 != 0 (per-term sine and cosine) and with C3 = 0, where a
 uniform grid whose tables fit takes the k-rotation recurrence."""
 import sys
 
@@ -36,11 +36,19 @@ def _model(nsp, k):
 
 
 @pytest.mark.parametrize("nsp,kind", [(2, "grid"), (4, "grid"), (5, "grid"), (6, "grid"),
-                                      (2, "no_zero"), (4, "no_zero"), (3, "nonuniform")])
+                                      (2, "no_zero"), (4, "no_zero"), (3, "nonuniform"),
+                                      (6, "default"), (3, "fine"), (6, "fine")])
 def test_forward_kernels_all_species_counts(nsp, kind):
+    """kind "default": the workloads' 0..16 grid at 0.05 (321 points; at six species the
+    relaxed-occupancy tables need two k chunks); "fine": 0..20 at 0.025 (801 points: more
+    k points than one workgroup has lanes, so every kernel works in k chunks)."""
     from chimera_amd.scattering import k_grid
     if kind == "grid":
         k = k_grid(16.0, 0.1)
+    elif kind == "default":
+        k = k_grid()
+    elif kind == "fine":
+        k = k_grid(20.0, 0.025)
     elif kind == "no_zero":
         k = np.linspace(0.5, 18.0, 320)
     else:
