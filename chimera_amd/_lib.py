@@ -103,6 +103,8 @@ _SIGS = {
     "chm_batch_run": (C.c_int, [vp, C.c_int, C.c_int]),
     "chm_batch_destroy": (C.c_int, [vp]),
     "chm_batch_profile": (C.c_int, [vp, C.c_int, vp]),
+    "chm_batch_set_transform": (C.c_int, [vp, C.c_int]),
+    "chm_batch_get_shape": (C.c_int, [vp, C.POINTER(C.c_int)]),
 }
 
 EXPORTS = tuple(_SIGS)

@@ -68,10 +68,11 @@ def _items(S):
 class Batch:
     """Step many problems (workloads.Problem objects) together on the device.
 
-    tune=True picks the path-list partitioning of the whole batch: every problem's
-    replicas split their gather over ceil(items / (256 * ipl)) workgroups, for the
-    items-per-lane ipl whose batched step is fastest (measured with HIP events on the
-    live batch; cached by the batch's shape)."""
+    tune=True picks, by measurement on the live batch (HIP events; cached by the batch's
+    shape), two launch shapes together: the path-list partitioning (every problem's
+    replicas split their gather over ceil(items / (256 * ipl)) workgroups, items per lane
+    ipl) and the transform's tile shape (K blocks each wavefront holds: 1 when every
+    problem's rows fit four K blocks, or 4)."""
 
     def __init__(self, problems, precision="bf16x3", reanchor_every=512, tune=True, log=None):
         self.P = list(problems)
@@ -100,7 +101,7 @@ class Batch:
             n = max(1, min(64, -(-_items(S) // (256 * ipl))))
             check(_lib.load().chm_sampler_set_tuning(S._S, n))
 
-    def _create(self):
+    def _create(self, kb=None):
         self.close()
         handles = (C.c_void_p * len(self.P))(*[P.sampler._S.value for P in self.P])
         self._handles = handles
@@ -108,28 +109,41 @@ class Batch:
         check(_lib.load().chm_batch_create(self.ctx.h, len(self.P), C.cast(handles, C.c_void_p),
                                            C.byref(h)))
         self.h = h
+        if kb is None:
+            kb = (self.tuned or {}).get("kb_per_wave", 0)
+        check(_lib.load().chm_batch_set_transform(self.h, int(kb)))
+
+    def _max_nkb(self):
+        n = C.c_int(0)
+        check(_lib.load().chm_batch_get_shape(self.h, C.byref(n)))
+        return int(n.value)
 
     def _tune(self, log=None, iters=20):
         from . import autotune
         key = "batch|" + "|".join(autotune._key(P.sampler) for P in self.P)
         cache = autotune._load()
-        if key in cache:
+        if key in cache and "kb_per_wave" in cache[key]:
             ipl = int(cache[key]["items_per_lane"])
             self._set_ipl(ipl)
             return cache[key]
         times = {}
-        for ipl in ITEMS_PER_LANE:
-            self._set_ipl(ipl)
-            self._create()
-            times[ipl] = self.profile(iters)["step_ms"]
-            if log:
-                log(f"[batch] tune: {ipl} items/lane -> {times[ipl]:.4f} ms/step")
-        best = min(times, key=times.get)
+        self._create(0)
+        kbs = (1, 4) if self._max_nkb() <= 4 else (4,)
+        for kb in kbs:
+            for ipl in ITEMS_PER_LANE:
+                self._set_ipl(ipl)
+                self._create(kb)
+                times[(kb, ipl)] = self.profile(iters)["step_ms"]
+                if log:
+                    log(f"[batch] tune: {kb} K blocks/wave, {ipl} items/lane -> "
+                        f"{times[(kb, ipl)]:.4f} ms/step")
+        kb_best, best = min(times, key=times.get)
         self._set_ipl(best)
         self.close()
         for P in self.P:                          # the timing ran steps: start again
             P.sampler.init_state(P.cfg0, P.cell.ideal)
-        rec = {"items_per_lane": int(best), "step_ms": {str(k): v for k, v in times.items()}}
+        rec = {"items_per_lane": int(best), "kb_per_wave": int(kb_best),
+               "step_ms": {f"kb{k}_ipl{i}": v for (k, i), v in times.items()}}
         cache[key] = rec
         autotune._save(cache)
         return rec

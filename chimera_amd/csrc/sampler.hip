@@ -2921,6 +2921,7 @@ struct chm_batch {
     chm_ctx *ctx = nullptr;
     std::vector<chm_sampler *> S;
     int mode = 0, anchor = 0, nslot = 0, nsplit_max = 1, ntile = 0, max_nkb = 0;
+    int kb_wave = 0;   // split-BF16 transform: K blocks per wave (0: 1 when every nkb <= 4, else 4)
     size_t lds_gemm_max = 0, lds_slot_max = 0;
     chm::DevBuf gemm[2], gemm_nb, reanchor, tiles, slot, slots, exch;
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -2946,7 +2947,7 @@ int batch_gemm(chm_batch *B, const GemmProb *probs, int mode) {
     const int4 *tiles = B->tiles.as<int4>();
     if (mode == 0)
         k_gemm_grouped<0><<<B->ntile, 256, B->lds_gemm_max, st>>>(probs, tiles);
-    else if (mode == 1 && B->max_nkb <= 4)
+    else if (mode == 1 && B->max_nkb <= 4 && B->kb_wave != KBW)
         k_gemm_grouped<1, 1><<<B->ntile, 256, 0, st>>>(probs, tiles);
     else if (mode == 1)
         k_gemm_grouped<1><<<B->ntile, 256, 0, st>>>(probs, tiles);
@@ -3185,6 +3186,25 @@ extern "C" int chm_batch_run(chm_batch *B, int nsteps, int exchange_every) {
     }
     CHM_HIP(hipStreamSynchronize(st));
     for (chm_sampler *S : B->S) S->hstep = B->hstep;
+    return CHM_OK;
+}
+
+extern "C" int chm_batch_set_transform(chm_batch *B, int kb_per_wave) {
+    CHM_REQUIRE(B, CHM_E_ARG, "null batch");
+    CHM_REQUIRE(kb_per_wave == 0 || kb_per_wave == 1 || kb_per_wave == KBW, CHM_E_ARG,
+                "K blocks per wave must be 0 (automatic), 1 or 4");
+    CHM_REQUIRE(kb_per_wave != 1 || B->max_nkb <= 4, CHM_E_ARG,
+                "one K block per wave needs every problem's rows in at most 4 K blocks");
+    CHM_HIP(hipSetDevice(B->ctx->device));
+    CHM_HIP(hipStreamSynchronize(B->ctx->stream));
+    batch_drop_graph(B);
+    B->kb_wave = kb_per_wave;
+    return CHM_OK;
+}
+
+extern "C" int chm_batch_get_shape(chm_batch *B, int *max_nkb) {
+    CHM_REQUIRE(B && max_nkb, CHM_E_ARG, "bad arguments");
+    *max_nkb = B->max_nkb;
     return CHM_OK;
 }
 

@@ -399,6 +399,15 @@ int chm_batch_destroy(chm_batch *b);
  * steps of every chain (no exchange, no re-anchor). */
 int chm_batch_profile(chm_batch *b, int iters, double *ms);
 
+/* Tile shape of the batched split-BF16 transform: the K blocks (32 basis rows each) every
+ * one of a tile's four wavefronts holds in registers.  1 (every problem's rows must fit
+ * 4 K blocks: a quarter of the operand registers, more tiles in flight per CU), 4 (any
+ * problem up to 16 K blocks), or 0 (1 when it fits, else 4).  Chosen by the batch
+ * autotuner (chimera_amd/batch.py) by measurement.  get_shape returns the largest K-block
+ * count over the problems. */
+int chm_batch_set_transform(chm_batch *b, int kb_per_wave);
+int chm_batch_get_shape(chm_batch *b, int *max_nkb);
+
 #ifdef __cplusplus
 }
 #endif
