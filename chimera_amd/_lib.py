@@ -99,6 +99,22 @@ _SIGS = {
     "chm_lp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_uint64,
                                vp]),
     "chm_lp_profile": (C.c_int, [vp, C.c_int, vp]),
+    "chm_psampler_create": (C.c_int, [vp, C.c_int, f64p, i32p, C.c_int, C.c_int, f64p, vp, f64p,
+                                      f64p, C.c_int, C.c_int, C.POINTER(vp)]),
+    "chm_psampler_destroy": (C.c_int, [vp]),
+    "chm_psampler_set_moves": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
+    "chm_psampler_set_prior": (C.c_int, [vp, vp, C.c_double]),
+    "chm_psampler_set_ms": (C.c_int, [vp, vp, i32p, i8p, f64p]),
+    "chm_psampler_init": (C.c_int, [vp, i32p, f64p, vp, f64p, C.c_uint64]),
+    "chm_psampler_run": (C.c_int, [vp, C.c_int, C.c_int]),
+    "chm_psampler_exchange": (C.c_int, [vp]),
+    "chm_psampler_set_beta": (C.c_int, [vp, f64p]),
+    "chm_psampler_set_recording": (C.c_int, [vp, C.c_int]),
+    "chm_psampler_read": (C.c_int, [vp, C.c_int, vp, C.c_int64]),
+    "chm_psampler_chi2": (C.c_int, [vp, C.c_int, f64p, f64p]),
+    "chm_psampler_reserve_marks": (C.c_int, [vp, C.c_int]),
+    "chm_psampler_mark": (C.c_int, [vp, C.c_int]),
+    "chm_psampler_read_marks": (C.c_int, [vp, C.c_int, f64p]),
     "chm_batch_create": (C.c_int, [vp, C.c_int, vp, C.POINTER(vp)]),
     "chm_batch_run": (C.c_int, [vp, C.c_int, C.c_int]),
     "chm_batch_destroy": (C.c_int, [vp]),

@@ -1463,37 +1463,6 @@ __global__ __launch_bounds__(256) void k_eval_rows(Geo g, const double *__restri
 }
 
 // ---------------------------------------------------------------- K4
-__device__ __forceinline__ void exchange_body(int nrep, const double *__restrict__ beta, const double *__restrict__ c2,
-                           int *__restrict__ slot_of, int *__restrict__ rung_of,
-                           double *__restrict__ beta_slot, uint64_t *__restrict__ iex,
-                           long long *__restrict__ att, long long *__restrict__ acc, uint64_t seed,
-                           double nfit) {
-    __shared__ uint64_t hx;
-    if (threadIdx.x == 0) {
-        uint64_t n = iex[0] + 1;
-        iex[0] = n;
-        hx = splitmix64(seed ^ (n * kGolden) ^ kExcSalt);
-    }
-    __syncthreads();
-    const int start = (int)(hx >> 63);
-    for (int i = start + 2 * threadIdx.x; i < nrep - 1; i += 2 * blockDim.x) {
-        int si = slot_of[i], sj = slot_of[i + 1];
-        double d = (beta[i] - beta[i + 1]) * (c2[si] - c2[sj]) * nfit;
-        double u = u01(splitmix64(hx ^ ((uint64_t)(i + 1) << 32)));
-        bool ok = metropolis(-d, u);
-        att[i] += 1;
-        if (ok) {
-            acc[i] += 1;
-            slot_of[i] = sj;
-            slot_of[i + 1] = si;
-            rung_of[sj] = i;
-            rung_of[si] = i + 1;
-            beta_slot[sj] = beta[i];
-            beta_slot[si] = beta[i + 1];
-        }
-    }
-}
-
 __global__ void k_exchange(int nrep, const double *__restrict__ beta, const double *__restrict__ c2,
                            int *__restrict__ slot_of, int *__restrict__ rung_of,
                            double *__restrict__ beta_slot, uint64_t *__restrict__ iex,
@@ -1835,31 +1804,12 @@ void drop_graph(chm_sampler *S) {
 }
 }  // namespace chm_smp
 
-extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, int nk,
-                                  const int32_t *site_xyz, const int32_t *nbr_j,
-                                  const int32_t *nbr_shell, const int8_t *stencil,
-                                  const double *basis_R, const double *basis_k,
-                                  const double *target, const double *sigma, int profile_scale,
-                                  chm_sampler **out) {
-    CHM_REQUIRE(ctx && gm && out, CHM_E_ARG, "bad arguments");
-    CHM_REQUIRE(nrep >= 1 && nk >= 2, CHM_E_ARG, "bad nrep / nk");
-    CHM_REQUIRE(gm->nsp >= 2 && gm->nsp <= chm::kMaxNsp, CHM_E_ARG, "2..6 species");
-    CHM_REQUIRE(gm->natoms > 0 && gm->ncoord > 0 && gm->nshell > 0, CHM_E_ARG, "bad geometry");
-    CHM_REQUIRE(gm->nkw > 0 && gm->nwin > 0, CHM_E_ARG, "bad window");
-    CHM_REQUIRE(gm->nblk >= 0 && gm->nstencil >= 0 && gm->nabsrec >= 0 &&
-                    gm->nabsrec <= gm->nstencil,
-                CHM_E_ARG, "bad stencil sizes");
-    CHM_REQUIRE(gm->nblk == 0 || gm->nstencil > 0, CHM_E_ARG, "MS entries need a stencil");
-    CHM_REQUIRE(site_xyz && nbr_j && nbr_shell && basis_R && basis_k && target && sigma,
-                CHM_E_ARG, "null input");
-    CHM_REQUIRE(gm->nstencil == 0 || stencil, CHM_E_ARG, "null stencil");
-    for (int d = 0; d < 3; ++d)
-        CHM_REQUIRE(gm->box[d] == 2 * gm->ncell[d] && gm->box[d] < 1024, CHM_E_ARG,
-                    "box must be 2*ncell (< 1024)");
-    CHM_REQUIRE(gm->nbasis >= 1 && gm->natoms == gm->nbasis * gm->ncell[0] * gm->ncell[1] *
-                                                     gm->ncell[2],
-                CHM_E_ARG, "natoms != nbasis * cells");
-    Geo g{};
+namespace chm_smp {
+// The lattice geometry of a sampler from its host description: derived sizes, the basis
+// tables packed into registers, and every site's coordinates checked to map back to its
+// index (shared by the count-space and the pair-space samplers).
+int geo_from(const chm_geom *gm, const int32_t *site_xyz, Geo &g) {
+    g = Geo{};
     g.natoms = gm->natoms;
     g.natoms_pad = (gm->natoms + 15) & ~15;
     g.nsp = gm->nsp;
@@ -1908,11 +1858,6 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
     g.rnb = 1.0f / (float)g.nbasis;
     g.rnc1 = 1.0f / (float)g.ncell[1];
     g.rnc2 = 1.0f / (float)g.ncell[2];
-    // host-side validation of every index a kernel will follow
-    for (long q = 0; q < (long)g.natoms * g.ncoord; ++q) {
-        CHM_REQUIRE(nbr_j[q] >= 0 && nbr_j[q] < g.natoms, CHM_E_ARG, "neighbour out of range");
-        CHM_REQUIRE(nbr_shell[q] >= 0 && nbr_shell[q] < g.nshell, CHM_E_ARG, "shell out of range");
-    }
     for (int q = 0; q < g.natoms; ++q) {
         int v = site_xyz[q];
         int x = v & 1023, y = (v >> 10) & 1023, z = (v >> 20) & 1023;
@@ -1921,6 +1866,64 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         int cell = ((x >> 1) * g.ncell[1] + (y >> 1)) * g.ncell[2] + (z >> 1);
         CHM_REQUIRE(b >= 0 && cell * g.nbasis + b == q, CHM_E_ARG,
                     "site coordinates do not map back to their index");
+    }
+    return CHM_OK;
+}
+
+// Every multiple-scattering stencil record in range: slot count, weight, entry offset
+// within the absorber's block, offsets within the box, absorber-role records first.
+int check_stencil(const Geo &g, const int8_t *stencil) {
+    for (int q = 0; q < g.nstencil; ++q) {
+        const int8_t *rb = stencil + 16 * q;
+        int nslot = rb[12], w = rb[13];
+        int cls_off = (int)(uint8_t)rb[14] | ((int)rb[15] << 8);
+        CHM_REQUIRE(nslot >= 1 && nslot <= 3 && w >= 1 && w <= 2, CHM_E_ARG, "bad stencil record");
+        int pw = 1;
+        for (int z = 0; z < nslot; ++z) pw *= g.nsp;
+        CHM_REQUIRE(cls_off >= 0 && cls_off + pw <= g.nblk, CHM_E_ARG, "stencil entry out of range");
+        for (int t = 0; t < 3 * (nslot + 1); ++t)
+            CHM_REQUIRE(rb[t] > -g.box[t % 3] && rb[t] < g.box[t % 3], CHM_E_ARG,
+                        "stencil offset exceeds the box");
+        CHM_REQUIRE((q < g.nabsrec) == (rb[0] == 0 && rb[1] == 0 && rb[2] == 0), CHM_E_ARG,
+                    "absorber-role records must come first");
+    }
+    return CHM_OK;
+}
+}  // namespace chm_smp
+
+extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, int nk,
+                                  const int32_t *site_xyz, const int32_t *nbr_j,
+                                  const int32_t *nbr_shell, const int8_t *stencil,
+                                  const double *basis_R, const double *basis_k,
+                                  const double *target, const double *sigma, int profile_scale,
+                                  chm_sampler **out) {
+    CHM_REQUIRE(ctx && gm && out, CHM_E_ARG, "bad arguments");
+    CHM_REQUIRE(nrep >= 1 && nk >= 2, CHM_E_ARG, "bad nrep / nk");
+    CHM_REQUIRE(gm->nsp >= 2 && gm->nsp <= chm::kMaxNsp, CHM_E_ARG, "2..6 species");
+    CHM_REQUIRE(gm->natoms > 0 && gm->ncoord > 0 && gm->nshell > 0, CHM_E_ARG, "bad geometry");
+    CHM_REQUIRE(gm->nkw > 0 && gm->nwin > 0, CHM_E_ARG, "bad window");
+    CHM_REQUIRE(gm->nblk >= 0 && gm->nstencil >= 0 && gm->nabsrec >= 0 &&
+                    gm->nabsrec <= gm->nstencil,
+                CHM_E_ARG, "bad stencil sizes");
+    CHM_REQUIRE(gm->nblk == 0 || gm->nstencil > 0, CHM_E_ARG, "MS entries need a stencil");
+    CHM_REQUIRE(site_xyz && nbr_j && nbr_shell && basis_R && basis_k && target && sigma,
+                CHM_E_ARG, "null input");
+    CHM_REQUIRE(gm->nstencil == 0 || stencil, CHM_E_ARG, "null stencil");
+    for (int d = 0; d < 3; ++d)
+        CHM_REQUIRE(gm->box[d] == 2 * gm->ncell[d] && gm->box[d] < 1024, CHM_E_ARG,
+                    "box must be 2*ncell (< 1024)");
+    CHM_REQUIRE(gm->nbasis >= 1 && gm->natoms == gm->nbasis * gm->ncell[0] * gm->ncell[1] *
+                                                     gm->ncell[2],
+                CHM_E_ARG, "natoms != nbasis * cells");
+    Geo g;
+    {
+        int rc0 = geo_from(gm, site_xyz, g);
+        if (rc0) return rc0;
+    }
+    // host-side validation of every index a kernel will follow
+    for (long q = 0; q < (long)g.natoms * g.ncoord; ++q) {
+        CHM_REQUIRE(nbr_j[q] >= 0 && nbr_j[q] < g.natoms, CHM_E_ARG, "neighbour out of range");
+        CHM_REQUIRE(nbr_shell[q] >= 0 && nbr_shell[q] < g.nshell, CHM_E_ARG, "shell out of range");
     }
     // the neighbour offsets of site 0 form a stencil that must reproduce every site's list
     std::vector<int> ss(g.ncoord), nbr_st;
@@ -1979,19 +1982,9 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
                         "the neighbour list is not translation invariant (a Bravais lattice is required)");
         }
     }
-    for (int q = 0; q < g.nstencil; ++q) {
-        const int8_t *rb = stencil + 16 * q;
-        int nslot = rb[12], w = rb[13];
-        int cls_off = (int)(uint8_t)rb[14] | ((int)rb[15] << 8);
-        CHM_REQUIRE(nslot >= 1 && nslot <= 3 && w >= 1 && w <= 2, CHM_E_ARG, "bad stencil record");
-        int pw = 1;
-        for (int z = 0; z < nslot; ++z) pw *= g.nsp;
-        CHM_REQUIRE(cls_off >= 0 && cls_off + pw <= g.nblk, CHM_E_ARG, "stencil entry out of range");
-        for (int t = 0; t < 3 * (nslot + 1); ++t)
-            CHM_REQUIRE(rb[t] > -g.box[t % 3] && rb[t] < g.box[t % 3], CHM_E_ARG,
-                        "stencil offset exceeds the box");
-        CHM_REQUIRE((q < g.nabsrec) == (rb[0] == 0 && rb[1] == 0 && rb[2] == 0), CHM_E_ARG,
-                    "absorber-role records must come first");
+    {
+        int rc0 = check_stencil(g, stencil);
+        if (rc0) return rc0;
     }
     // largest squared offset a gather visits (neighbours and path sites)
     int gr2 = r2max;

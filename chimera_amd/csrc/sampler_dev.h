@@ -390,4 +390,40 @@ __device__ __forceinline__ int species_swapped(const uint8_t *lsp, int site, int
     return lsp[site];
 }
 
+// One replica-exchange sweep (methods/method_a_rmc.py:332-373): exchange number iex + 1
+// picks the parity of the first rung from its stream; neighbour rungs i, i + 1 of that
+// parity swap temperatures with probability min(1, e^d), d = (beta_i - beta_i+1)
+// (chi2_i - chi2_i+1) nfit: the rung <-> slot maps and the slots' betas change.
+__device__ __forceinline__ void exchange_body(int nrep, const double *__restrict__ beta, const double *__restrict__ c2,
+                           int *__restrict__ slot_of, int *__restrict__ rung_of,
+                           double *__restrict__ beta_slot, uint64_t *__restrict__ iex,
+                           long long *__restrict__ att, long long *__restrict__ acc, uint64_t seed,
+                           double nfit) {
+    __shared__ uint64_t hx;
+    if (threadIdx.x == 0) {
+        uint64_t n = iex[0] + 1;
+        iex[0] = n;
+        hx = splitmix64(seed ^ (n * kGolden) ^ kExcSalt);
+    }
+    __syncthreads();
+    const int start = (int)(hx >> 63);
+    for (int i = start + 2 * threadIdx.x; i < nrep - 1; i += 2 * blockDim.x) {
+        int si = slot_of[i], sj = slot_of[i + 1];
+        double d = (beta[i] - beta[i + 1]) * (c2[si] - c2[sj]) * nfit;
+        double u = u01(splitmix64(hx ^ ((uint64_t)(i + 1) << 32)));
+        bool ok = metropolis(-d, u);
+        att[i] += 1;
+        if (ok) {
+            acc[i] += 1;
+            slot_of[i] = sj;
+            slot_of[i + 1] = si;
+            rung_of[sj] = i;
+            rung_of[si] = i + 1;
+            beta_slot[sj] = beta[i];
+            beta_slot[si] = beta[i + 1];
+        }
+    }
+}
+
+
 }  // namespace chm_smp

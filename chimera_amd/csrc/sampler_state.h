@@ -93,5 +93,9 @@ int lp_launch_score(chm_sampler *S);
 int lp_launch_select(chm_sampler *S, long long n);
 int lp_prepare(chm_sampler *S);   // buffers for the current geometry (at create)
 void drop_graph(chm_sampler *S);  // sampler.hip: forget the captured step graphs
+// sampler.hip: lattice geometry from the host description (sites checked), and the check
+// of multiple-scattering stencil records against it
+int geo_from(const chm_geom *gm, const int32_t *site_xyz, Geo &g);
+int check_stencil(const Geo &g, const int8_t *stencil);
 
 }  // namespace chm_smp

@@ -26,9 +26,9 @@ Decisions (recorded, the reference's own choice cannot be replayed without CuPy)
 """
 from __future__ import annotations
 
-import time
-
+import inspect
 import sys
+import time
 
 import numpy as np
 
@@ -55,13 +55,34 @@ def _beta_ladder(tmin, tmax, nrep):
     return 1.0 / np.geomspace(tmin, tmax, nrep)
 
 
+def _needs_pair_space(target, swap_frac, energy_prior):
+    """Whether a sampler with these arguments needs the pair-space engine: displacement
+    moves, the energy prior or an |chi(R)| (real) target."""
+    return swap_frac != 1.0 or energy_prior is not None or not np.iscomplexobj(target)
+
+
 class RMCSampler:
+    engine = "count"
+
+    def __new__(cls, *args, **kwargs):
+        # The reference's full move set (displacements, energy prior, |chi(R)| targets) runs
+        # on the pair-space engine; the swap-only chain on a complex target on count space.
+        if cls is RMCSampler:
+            b = _SIG.bind(None, *args, **kwargs)
+            b.apply_defaults()
+            a = b.arguments
+            if _needs_pair_space(a["target"], a["swap_frac"], a["energy_prior"]):
+                from .pair_rmc import PairRMCSampler
+                cls = PairRMCSampler
+        return object.__new__(cls)
+
     def __init__(self, fm, ft, target, mask, sigma_R, nrep=48, tmin=0.02, tmax=5.0, seed=0,
                  swap_frac=0.75, disp_sigma=0.015, max_disp=0.30, nsp=None,
                  collect_after=None, collect_every=25, adapt_every=0, target_accept=0.35,
                  profile_scale=True, collect_beta=0.5, n_collect=4, energy_prior=None,
                  ms_library=None, precision="f64", reanchor_every=512, accumulator="f64",
                  paired=True):
+        self._ctor = {k: v for k, v in locals().items() if k not in ("self", "__class__")}
         """Arguments as methods/method_a_rmc.py:64-69, plus `precision`: "f64" (the trial
         chi(R) recomputed from the integer counts in FP64 every step), "bf16x3" (the
         change of chi(R) in split-BF16 MFMA with FP32 accumulation, added to a running
@@ -261,7 +282,9 @@ class RMCSampler:
         species0 = i32(species0)
         pos0 = np.asarray(pos0, float)
         if not np.allclose(pos0, self.fm.cell.ideal, atol=1e-9):
-            raise ValueError("the device sampler starts from the ideal lattice positions")
+            # an off-lattice start: the chain continues on the pair-space engine
+            self._to_pair_space()
+            return self.init_state(species0, pos0)
         comp = np.bincount(species0, minlength=self.nsp)
         if self._S is None or not np.array_equal(comp, self._comp):
             self._destroy()
@@ -283,6 +306,16 @@ class RMCSampler:
                                            int(self.seed) & ((1 << 64) - 1)))
         self.samples = []
         self._t_run = 0.0
+
+    def _to_pair_space(self):
+        """Become a PairRMCSampler with the same constructor arguments (collected samples
+        are kept)."""
+        from .pair_rmc import PairRMCSampler
+        samples = self.samples
+        self._destroy()
+        self.__class__ = PairRMCSampler
+        PairRMCSampler.__init__(self, **self._ctor)
+        self.samples = samples
 
     def set_recording(self, log_steps=0, series_steps=0, series_rung=None):
         """Keep the move log (a, b, accepted, rung per step and replica) and the
@@ -330,6 +363,35 @@ class RMCSampler:
         out = np.empty(shape, dtype)
         check(_lib.load().chm_sampler_read(self._S, what, out.ctypes.data, out.nbytes))
         return out
+
+    # Device calls of the run loop, exchange bookkeeping and history marks (the pair-space
+    # engine, chimera_amd.pair_rmc, supplies its own).
+    def _lib_run(self, n, E):
+        check(_lib.load().chm_sampler_run(self._S, int(n), int(E)))
+
+    def _lib_set_beta(self, ladder):
+        check(_lib.load().chm_sampler_set_beta(self._S, f64(ladder)))
+
+    def _lib_exchange(self):
+        check(_lib.load().chm_sampler_exchange(self._S))
+
+    def _lib_reserve_marks(self, n):
+        check(_lib.load().chm_sampler_reserve_marks(self._S, int(n)))
+
+    def _lib_mark(self, i):
+        check(_lib.load().chm_sampler_mark(self._S, int(i)))
+
+    def _lib_read_marks(self, n):
+        mk = np.empty((n, 3))
+        check(_lib.load().chm_sampler_read_marks(self._S, int(n), mk))
+        return mk
+
+    def _steps_done(self):
+        return int(self._read(10, (1,), np.int64)[0])
+
+    def _exch_raw(self):
+        """(2, nrep - 1) int64 exchange attempts and accepts per neighbour pair."""
+        return self._read(8, (2, max(self.nrep - 1, 1)), np.int64)
 
     # ------------------------------------------------------------- properties
     @property
@@ -395,17 +457,16 @@ class RMCSampler:
         """One swap step for every replica; returns the accepted flags (rung order)."""
         self._settle()
         before = self.accepted_per_rung()
-        check(_lib.load().chm_sampler_run(self._S, 1, 0))
+        self._lib_run(1, 0)
         return (self.accepted_per_rung() - before) > 0
 
     def replica_exchange(self, return_rates=False):
         """One exchange sweep on the device (method_a_rmc.py:332-373).  Returns the
         number of swaps (and per-pair 0/1 rates of this sweep, NaN where not tried)."""
         self._settle()
-        m = max(self.nrep - 1, 1)
-        ea0 = self._read(8, (2, m), np.int64)
-        check(_lib.load().chm_sampler_exchange(self._S))
-        ea1 = self._read(8, (2, m), np.int64)
+        ea0 = self._exch_raw()
+        self._lib_exchange()
+        ea1 = self._exch_raw()
         tried = (ea1[0] - ea0[0]) > 0
         ok = (ea1[1] - ea0[1]) > 0
         nsw = int(ok.sum())
@@ -434,14 +495,14 @@ class RMCSampler:
         step = np.diff(logb) * (rate / max(rate.mean(), 1e-6)) ** 0.5
         step = step * ((logb[-1] - logb[0]) / step.sum())
         ladder = np.exp(np.concatenate([[logb[0]], logb[0] + np.cumsum(step)]))
-        check(_lib.load().chm_sampler_set_beta(self._S, f64(ladder)))
+        self._lib_set_beta(ladder)
         self._beta_rung = ladder
 
     def _exchange_outcomes(self, since):
         """0/1 outcome per neighbour pair of the exchange sweep made since the attempt /
         accept counters read `since`, untried pairs filled with the mean of the tried
         ones (method_a_rmc.py:366-372); returns (rates, counters now)."""
-        ea = self._read(8, (2, max(self.nrep - 1, 1)), np.int64)
+        ea = self._exch_raw()
         tried = (ea[0] - since[0]) > 0
         ok = (ea[1] - since[1]) > 0
         fill = ok[tried].mean() if tried.any() else 0.3
@@ -460,7 +521,6 @@ class RMCSampler:
         retained best's chi^2 recomputed from scratch) is evaluated when first read."""
         hist = []
         t0 = time.time()
-        lib = _lib.load()
         self._best_verified, self._verify_pending = None, False
         self._rate_pending = False    # recomputed from the cumulative counters at the end
         it = 0
@@ -475,12 +535,12 @@ class RMCSampler:
         logs = set(range(log_every - 1, nsteps, log_every)) | ({0} if nsteps > 0 else set())
         marks.update(logs)
         if A:
-            h0 = int(self._read(10, (1,), np.int64)[0])
+            h0 = self._steps_done()
             if E:
                 marks.update(range((-(h0 + 1)) % E, nsteps, E))
             marks.update(range(A - 1, nsteps, A))
             win_exch = []
-            ea_last = self._read(8, (2, max(self.nrep - 1, 1)), np.int64)
+            ea_last = self._exch_raw()
             acc_win = self.accepted_per_rung().sum()
         sync_logs = callback is not None
         log_steps = []
@@ -488,12 +548,12 @@ class RMCSampler:
             if sync_logs:
                 acc0 = self.accepted_per_rung().sum()
             else:
-                check(lib.chm_sampler_reserve_marks(self._S, len(logs) + 1))
-                check(lib.chm_sampler_mark(self._S, 0))
+                self._lib_reserve_marks(len(logs) + 1)
+                self._lib_mark(0)
         for m in sorted(marks):
             n = m + 1 - it
             if n > 0:
-                check(lib.chm_sampler_run(self._S, n, E))
+                self._lib_run(n, E)
                 it = m + 1
             if A and E and (h0 + m + 1) % E == 0:
                 rates, ea_last = self._exchange_outcomes(ea_last)
@@ -514,12 +574,11 @@ class RMCSampler:
                     callback(m + 1, best, hist[-1][2])
                 else:
                     log_steps.append(m + 1)
-                    check(lib.chm_sampler_mark(self._S, len(log_steps)))
+                    self._lib_mark(len(log_steps))
         if it < nsteps:
-            check(lib.chm_sampler_run(self._S, nsteps - it, int(exchange_every)))
+            self._lib_run(nsteps - it, int(exchange_every))
         if log_steps:
-            mk = np.empty((len(log_steps) + 1, 3))
-            check(lib.chm_sampler_read_marks(self._S, len(log_steps) + 1, mk))
+            mk = self._lib_read_marks(len(log_steps) + 1)
             for q, st in enumerate(log_steps, 1):
                 hist.append((st, float(mk[q, 0]), float((mk[q, 1] - mk[0, 1]) / (st * self.nrep)),
                              float(mk[q, 2])))
@@ -535,7 +594,7 @@ class RMCSampler:
         m = max(self.nrep - 1, 1)
         if self._S is None:
             return np.zeros((2, m))
-        return self._read(8, (2, m), np.int64).astype(float)
+        return self._exch_raw().astype(float)
 
     @property
     def exch_attempts(self):
@@ -697,3 +756,6 @@ class RMCSampler:
         if _finalizing():
             return
         self._destroy()
+
+
+_SIG = inspect.signature(RMCSampler.__init__)

@@ -37,6 +37,7 @@ typedef struct chm_ctx chm_ctx;
 typedef struct chm_ft chm_ft;
 typedef struct chm_sampler chm_sampler;
 typedef struct chm_batch chm_batch;
+typedef struct chm_psampler chm_psampler;
 
 const char *chm_last_error(void);
 int chm_version(void);
@@ -380,6 +381,74 @@ int chm_lp_train(chm_sampler *s, int iters, int batch, float lr, int max_rung, f
  * the current state, with every slot on the learned path; the chain does not advance
  * (the selection writes into a scratch proposal row). */
 int chm_lp_profile(chm_sampler *s, int iters, double *ms);
+
+/* ------------------------------------------------- pair-space sampler (all moves) */
+
+/* The reference sampler with every move it makes (methods/method_a_rmc.py:63-496):
+ * swap and single-atom displacement moves (swap_frac, disp_sigma, max_disp), any start
+ * positions, complex or |chi(R)| targets, the Boltzmann energy prior and multiple
+ * scattering from the configuration's species.  State per replica: species, positions,
+ * the distance and type of every pair of the context's topology, unnormalised chi(k),
+ * chi^2, best state with positions.  One step: proposal + affected pairs
+ * (exafs_gpu/kernels.py:362-488), Delta chi over them (kernels.py:243-325), trial
+ * chi(R), profiled chi^2, Metropolis, commit (kernels.py:490-510), best retention.
+ *
+ * Create on a context whose tables and topology are set (pairs i-sorted, ncoord per
+ * site):
+ *   ideal (natoms, 3): the ideal sites the displacement clamp is measured from;
+ *   rev (npairs): index of the reverse pair (method_a_rmc.py:41-50);
+ *   Te (nsp, nk, nkw*nwin*2): per edge, the window transform of a unit chi(k) at each
+ *       k (resample + window + DFT at the window's R points, Re/Im interleaved), NOT
+ *       divided by the absorber count (init divides by the start's composition);
+ *   Rx (nsp, nkw*nwin*2) or NULL: the same transform of MultiEdgeFT.chi_extra;
+ *   target: complex (nsp*nkw, nwin, 2) or, with magnitude, |chi(R)| (nsp*nkw, nwin);
+ *   sigma (nsp*nkw, nwin) > 0.  RMCSampler.__init__ (method_a_rmc.py:64-163). */
+int chm_psampler_create(chm_ctx *ctx, int nrep, const double *ideal, const int32_t *rev, int nkw,
+                        int nwin, const double *Te, const double *Rx, const double *target,
+                        const double *sigma, int magnitude, int profile_scale,
+                        chm_psampler **out);
+int chm_psampler_destroy(chm_psampler *s);
+/* Move parameters: swap_frac in [0, 1], disp_sigma >= 0, max_disp > 0
+ * (method_a_rmc.py:64-66, the displacement step retuned by adapt, :376-398). */
+int chm_psampler_set_moves(chm_psampler *s, double swap_frac, double disp_sigma, double max_disp);
+/* Energy prior: etab (nshell, nsp*nsp) V/2 per directed pair, ebeta = 1/(kB T); NULL
+ * clears it (method_a_rmc.py:147-157, 310-326). */
+int chm_psampler_set_prior(chm_psampler *s, const double *etab, double ebeta);
+/* Multiple scattering from the species along every 3- and 4-leg path of the lattice
+ * (SupercellMS, chimera/mslibrary.py:341-492): the lattice geometry and stencil as for
+ * chm_sampler_create (nblk > 0), and the entry table tms (nsp*nblk, nk). */
+int chm_psampler_set_ms(chm_psampler *s, const chm_geom *g, const int32_t *site_xyz,
+                        const int8_t *stencil, const double *tms);
+/* Every replica starts at (species0 (natoms), pos0 (natoms, 3)); chi_ms (nsp, nk) is the
+ * start's multiple-scattering chi (NULL: none); betas per rung; seed.
+ * RMCSampler.init_state (method_a_rmc.py:166-184). */
+int chm_psampler_init(chm_psampler *s, const int32_t *species0, const double *pos0,
+                      const double *chi_ms, const double *beta, uint64_t seed);
+/* nsteps steps, replica exchange every exchange_every steps (0 = never); exchange periods
+ * replayed from a captured hipGraph; asynchronous (method_a_rmc.py:250-308, 332-373, 401-455). */
+int chm_psampler_run(chm_psampler *s, int nsteps, int exchange_every);
+/* One exchange sweep now (method_a_rmc.py:332-373); synchronous. */
+int chm_psampler_exchange(chm_psampler *s);
+/* New ladder (rung order); each slot takes the beta of the rung it holds (:376-398). */
+int chm_psampler_set_beta(chm_psampler *s, const double *beta);
+/* Keep the last log_steps steps' moves (a, b, is-swap | accepted << 1, slot per rung)
+ * and trial chi^2 on the device; 0 disables. */
+int chm_psampler_set_recording(chm_psampler *s, int log_steps);
+/* Read state back, rung order unless noted.  what: 0 species (nrep, natoms) int32,
+ * 1 chi2 (nrep), 2 best chi2, 3 best species, 4 positions (nrep, natoms, 3), 5 best
+ * positions, 6 chi (nrep, nsp, nk), 7 pair distances (nrep, npairs), 8 pair types int32,
+ * 9 accepted moves per rung int64, 10 exchange attempts and accepts (2, nrep-1) int64,
+ * 11 completed steps (1) int64, 12 move log (log_steps, nrep, 4) int32, 13 logged trial
+ * chi^2 (log_steps, nrep), 14 rung of each slot int32.  nbytes must match. */
+int chm_psampler_read(chm_psampler *s, int what, void *host, int64_t nbytes);
+/* chi^2 of nconf unnormalised chi (nconf, nsp, nk) with the sampler's target, window and
+ * profiling (method_a_rmc.py:211-227, best_verified :439-448). */
+int chm_psampler_chi2(chm_psampler *s, int nconf, const double *chi, double *c2);
+/* Run-history snapshots on the stream: smallest best chi^2, accepted-move total, device
+ * clock (as chm_sampler_mark).  read_marks converts the clock to seconds since mark 0. */
+int chm_psampler_reserve_marks(chm_psampler *s, int n);
+int chm_psampler_mark(chm_psampler *s, int idx);
+int chm_psampler_read_marks(chm_psampler *s, int n, double *out);
 
 /* ------------------------------------------------------------------- batches
  * Many independent problems stepped together: every kernel of a step is one launch for
