@@ -67,13 +67,19 @@ struct PsArgs {
     const int *pt;
     const double *etab;   // (nshell, nsp*nsp) or null
     PsMove *mv;
-    int *aff;             // (nrep, maxaff) pair ids of the move, compacted in pair order
-    double *rnew;         // (nrep, maxaff) new distance
-    int *tnew;            // (nrep, maxaff) new type
-    int2 *msrow;          // (nrep, msmax) (MS entry, count change), compacted in entry order
+    struct AffRec *aff;   // (nrep, maxaff) the move's pairs, compacted in pair order
+    int *dcount;          // (nrep, g.nrow_tot) MS entry-count change of a swap, summed over
+                          //   the split workgroups with integer atomics (zeroed by P4)
     int has_ms;
     Geo g;
     const int4 *ms_sten;
+};
+
+// One pair a move touches: its index, shell and sigma^2, its distance and type before and
+// after the move (everything the Delta chi and the commit read, in one record).
+struct AffRec {
+    double rnew, rold, sig2;
+    int p, tnew, told, sh;
 };
 
 // The Gaussian step of a displacement (kernels.py:405-421): two Box-Muller pairs from the
@@ -151,11 +157,14 @@ __device__ double block_sum_d(double v, double *red) {
     return t;
 }
 
-constexpr int kPropThreads = 1024;
+constexpr int kPropThreads = 256;
 
 // ------------------------------------------------------------------------------ P1
-// One workgroup of 1024 lanes per slot.  The slot's species go to LDS first (every read
-// of the proposal, the pair types and the MS gather is then an LDS read).
+// grid (nsplit, slots), 256 lanes.  Every workgroup of a slot stages the slot's species in
+// LDS and draws the same proposal; workgroup 0 builds the affected pairs and the energy
+// change; with multiple scattering every workgroup gathers its share of the paths through
+// the two swapped sites into an LDS histogram and adds its nonzero bins to the slot's
+// count-change row with integer atomics (order-independent, so deterministic).
 __global__ __launch_bounds__(kPropThreads) void k_ps_propose(PsArgs A) {
     extern __shared__ int lds[];
     // [MS entry-count change (g.nrow_tot ints), MS only][species (natoms_pad bytes)]
@@ -164,7 +173,7 @@ __global__ __launch_bounds__(kPropThreads) void k_ps_propose(PsArgs A) {
     __shared__ PsMove M;
     __shared__ int wcnt[16];
     __shared__ double red[16];
-    const int s = blockIdx.x;
+    const int s = blockIdx.y, split = blockIdx.x, nsplit = gridDim.x;
     const int rung = A.rung_of[s];
     const uint64_t n = A.stepc[s] + 1;
     const uint8_t *sp = A.sp + (size_t)s * A.natoms_pad;
@@ -214,66 +223,6 @@ __global__ __launch_bounds__(kPropThreads) void k_ps_propose(PsArgs A) {
     }
     __syncthreads();
     const int a = M.a, b = M.b, na = M.na, nb = M.nb, sw = M.sw;
-    const int C = A.ncoord, nn = A.nsp * A.nsp;
-    const double *rs = A.r + (size_t)s * A.npairs;
-    const int *ts = A.pt + (size_t)s * A.npairs;
-    int *aff = A.aff + (size_t)s * A.maxaff;
-    double *rnew = A.rnew + (size_t)s * A.maxaff;
-    int *tnew = A.tnew + (size_t)s * A.maxaff;
-    double de = 0.0;
-    int base = 0;
-    for (int q0 = 0; q0 < A.maxaff; q0 += blockDim.x) {
-        const int q = q0 + (int)threadIdx.x;
-        // item q: block q / C (0: out-pairs of a, 1: their reverses, 2, 3: the same of b),
-        // the o-th pair of the site's i-sorted run; the pair's ends i, j follow from the
-        // block without reading the pair list (a reverse pair runs j -> site)
-        int p = -1, i = 0, j = 0;
-        if (q < A.maxaff) {
-            const int blk = q / C, o = q - blk * C;
-            const int site = blk < 2 ? a : b;
-            const int p0 = site * C + o;
-            const int other = A.jidx[p0];
-            const int rv = A.rev[p0];
-            if (blk & 1) {
-                p = rv;
-                i = other;
-                j = site;
-            } else {
-                p = p0;
-                i = site;
-                j = other;
-            }
-            // dedup (kernels.py:448-468): b's blocks drop everything for a displacement
-            // and every pair whose other end is a
-            if (blk >= 2 && (b == a || other == a)) p = -1;
-        }
-        double rn = 0.0;
-        int tn = 0;
-        if (p >= 0) {
-            if (sw) {
-                rn = rs[p];
-            } else {
-                const double *pi = (i == a) ? M.np : pos + 3 * i;
-                const double *pj = (j == a) ? M.np : pos + 3 * j;
-                rn = pair_distance(pi, pj, A.shift + 3 * p);
-            }
-            const int si = (i == a) ? na : ((i == b) ? nb : (int)lsp[i]);
-            const int sj = (j == a) ? na : ((j == b) ? nb : (int)lsp[j]);
-            tn = si * A.nsp + sj;
-            if (A.etab) {
-                const int sh = A.shell[p];
-                de += A.etab[sh * nn + tn] - A.etab[sh * nn + ts[p]];
-            }
-        }
-        const int at = compact_index(p >= 0, wcnt, base);
-        if (p >= 0) {
-            aff[at] = p;
-            rnew[at] = rn;
-            tnew[at] = tn;
-        }
-    }
-    de = block_sum_d(de, red);
-    int nms = 0;
     if (A.has_ms && sw) {
         const Geo &g = A.g;
         for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) hist[q] = 0;
@@ -282,29 +231,92 @@ __global__ __launch_bounds__(kPropThreads) void k_ps_propose(PsArgs A) {
         W.p = Prop{a, b, na, nb, 1};
         site_xyz_of(g, a, W.ax, W.ay, W.az);
         site_xyz_of(g, b, W.bx, W.by, W.bz);
-        for (int it = threadIdx.x; it < 2 * g.nstencil; it += blockDim.x)
-            gather_item_sp(g, W, SpPlain{lsp}, hist, it, 0, A.ms_sten[it < g.nstencil ? it : it - g.nstencil]);
-        __syncthreads();
-        int2 *rows = A.msrow + (size_t)s * A.msmax;
-        int mb = 0;
-        for (int q0 = 0; q0 < g.nrow_tot; q0 += blockDim.x) {
-            const int q = q0 + (int)threadIdx.x;
-            int e = 0, j = -1, v = 0;
-            if (q < g.nrow_tot) {
-                e = q / g.rstride;
-                j = q - e * g.rstride - g.nss_rows;
-                v = hist[q];
+        // this workgroup's items, each lane's stencil records requested together first
+        // with several workgroups the first builds the affected pairs meanwhile and
+        // gathers nothing
+        constexpr int kItems = 8;
+        const int ng = nsplit > 1 ? nsplit - 1 : 1, gi = nsplit > 1 ? split - 1 : 0;
+        const int stride = ng * (int)blockDim.x;
+        for (int it0 = (gi < 0 ? 2 * g.nstencil : gi * (int)blockDim.x); it0 < 2 * g.nstencil; it0 += kItems * stride) {
+            int4 rec[kItems];
+#pragma unroll
+            for (int u = 0; u < kItems; ++u) {
+                const int it = it0 + u * stride + (int)threadIdx.x;
+                const int m = it < g.nstencil ? it : it - g.nstencil;
+                rec[u] = A.ms_sten[min(max(m, 0), g.nstencil - 1)];
             }
-            const bool keep = v != 0 && j >= 0 && j < g.nblk;
-            const int at = compact_index(keep, wcnt, mb);
-            if (keep) rows[at] = make_int2(e * g.nblk + j, v);
+#pragma unroll
+            for (int u = 0; u < kItems; ++u) {
+                const int it = it0 + u * stride + (int)threadIdx.x;
+                if (it < 2 * g.nstencil) gather_item_sp(g, W, SpPlain{lsp}, hist, it, 0, rec[u]);
+            }
         }
-        nms = mb;
+        __syncthreads();
+        int *dc = A.dcount + (size_t)s * g.nrow_tot;
+        for (int q = threadIdx.x; q < g.nrow_tot; q += blockDim.x) {
+            const int v = hist[q];
+            if (v != 0) atomicAdd(&dc[q], v);
+        }
     }
+    if (split != 0) return;
+    const int C = A.ncoord, nn = A.nsp * A.nsp;
+    const double *rs = A.r + (size_t)s * A.npairs;
+    const int *ts = A.pt + (size_t)s * A.npairs;
+    AffRec *aff = A.aff + (size_t)s * A.maxaff;
+    double de = 0.0;
+    int base = 0;
+    for (int q0 = 0; q0 < A.maxaff; q0 += blockDim.x) {
+        const int q = q0 + (int)threadIdx.x;
+        // item q: block q / C (0: out-pairs of a, 1: their reverses, 2, 3: the same of b),
+        // the o-th pair of the site's i-sorted run; the pair's ends follow from the block
+        // without reading the pair list (a reverse pair runs other -> site).  Every load
+        // below has an in-range index whether the item is kept or not (no branch around it),
+        // so each round of loads is issued together.
+        const bool in = q < A.maxaff;
+        const int qq = in ? q : 0;
+        const int blk = qq / C, o = qq - blk * C;
+        const int site = blk < 2 ? a : b;
+        const int p0 = site * C + o;
+        const int other = A.jidx[p0];
+        const int rv = A.rev[p0];
+        const bool rvs = (blk & 1) != 0;
+        const int p = rvs ? rv : p0;
+        const int i = rvs ? other : site, j = rvs ? site : other;
+        // dedup (kernels.py:448-468): b's blocks drop everything for a displacement and
+        // every pair whose other end is a
+        const bool keep = in && !(blk >= 2 && (b == a || other == a));
+        const double rold = rs[p];
+        const int told = ts[p];
+        const int sh = A.shell[p];
+        const double s2p = A.sig2[p];
+        double rn = rold;
+        if (!sw) {
+            const double *pi = (i == a) ? M.np : pos + 3 * i;
+            const double *pj = (j == a) ? M.np : pos + 3 * j;
+            rn = pair_distance(pi, pj, A.shift + 3 * p);
+        }
+        const int si = (i == a) ? na : ((i == b) ? nb : (int)lsp[i]);
+        const int sj = (j == a) ? na : ((j == b) ? nb : (int)lsp[j]);
+        const int tn = si * A.nsp + sj;
+        if (A.etab && keep) de += A.etab[sh * nn + tn] - A.etab[sh * nn + told];
+        const int at = compact_index(keep, wcnt, base);
+        if (keep) {
+            AffRec rec;
+            rec.rnew = rn;
+            rec.rold = rold;
+            rec.sig2 = s2p;
+            rec.p = p;
+            rec.tnew = tn;
+            rec.told = told;
+            rec.sh = sh;
+            aff[at] = rec;
+        }
+    }
+    if (A.etab) de = block_sum_d(de, red);
     if (threadIdx.x == 0) {
         PsMove o = M;
         o.naff = base;
-        o.nms = nms;
+        o.nms = 0;
         o.pad = 0;
         o.de = de;
         A.mv[s] = o;
@@ -327,15 +339,10 @@ struct PsDeltaArgs {
     int nrep, npairs, nk, k0, maxaff, msmax, nblk;
     double s02;
     const PsMove *mv;
-    const int *aff;
-    const double *rnew;
-    const int *tnew;
-    const double *r;
-    const int *pt;
-    const int *shell;
-    const double *sig2, *ftab, *dtab, *lamtab, *ktab, *drtab, *c3tab, *s2tab;
-    const int2 *msrow;
-    const double *tms;     // (nsp * nblk, nk) MS table rows
+    const AffRec *aff;
+    const double *ftab, *dtab, *lamtab, *ktab, *drtab, *c3tab, *s2tab;
+    const double *dms;     // (ngrp, nrep, nsp, nk) MS chi change of the move in entry groups, or null
+    int ngrp;
     const double *chi;     // (nrep, nsp, nk) current unnormalised chi
     double *trial;         // (nrep, nsp, nk) chi + Delta chi
     double *trialT;        // (nsp, nk, nrep) the same, k-major for the transform
@@ -371,18 +378,16 @@ __global__ __launch_bounds__(1024) void k_ps_delta(PsDeltaArgs A, const chm::Pol
     const int nn = NSP * NSP;
     // stage every term record of the move (2 naff <= 8 ncoord <= blockDim)
     for (int t = threadIdx.x; t < nterm; t += blockDim.x) {
-        const int q = t >> 1;
-        const int p = A.aff[(size_t)s * A.maxaff + q];
+        const AffRec ar = A.aff[(size_t)s * A.maxaff + (t >> 1)];
         const bool nw = (t & 1) == 0;
-        const double rr = nw ? A.rnew[(size_t)s * A.maxaff + q] : A.r[(size_t)s * A.npairs + p];
-        const int tt = nw ? A.tnew[(size_t)s * A.maxaff + q] : A.pt[(size_t)s * A.npairs + p];
-        const int sh = A.shell[p];
-        const double re = rr + A.drtab[sh * nn + tt];
+        const double rr = nw ? ar.rnew : ar.rold;
+        const int tt = nw ? ar.tnew : ar.told;
+        const double re = rr + A.drtab[ar.sh * nn + tt];
         TermRec rec;
         rec.r = re;
         rec.lir2 = -2.0 * log(re);
-        rec.s2 = A.sig2[p] + A.s2tab[sh * nn + tt];
-        rec.c3 = A.c3tab[sh];
+        rec.s2 = ar.sig2 + A.s2tab[ar.sh * nn + tt];
+        rec.c3 = A.c3tab[ar.sh];
         rec.t = tt;
         rec.sign = nw ? 1 : -1;
         tile[t] = rec;
@@ -396,9 +401,9 @@ __global__ __launch_bounds__(1024) void k_ps_delta(PsDeltaArgs A, const chm::Pol
     const double c1 = -2.0 / lv, c2 = -2.0 * kv * kv;
     const double c3k = (4.0 / 3.0) * kv * kv * kv * kInv2Pi;
     const double pref = (live && kv > 0.0) ? A.s02 / kv : 0.0;
-    double acc[NSP], accm[NSP];
+    double acc[NSP];
 #pragma unroll
-    for (int e = 0; e < NSP; ++e) acc[e] = accm[e] = 0.0;
+    for (int e = 0; e < NSP; ++e) acc[e] = 0.0;
     const int per = (nterm + kDeltaWaves - 1) / kDeltaWaves;
     const int tb = min(nterm, w * per), te = min(nterm, tb + per);
     for (int q = tb; q < te; ++q) {
@@ -412,84 +417,260 @@ __global__ __launch_bounds__(1024) void k_ps_delta(PsDeltaArgs A, const chm::Pol
         for (int e = 0; e < NSP; ++e)
             if (sa == e) acc[e] += v;
     }
-    // MS entry rows of a swap: change x table row
-    const int2 *rows = A.msrow + (size_t)s * A.msmax;
-    const int mper = (mv.nms + kDeltaWaves - 1) / kDeltaWaves;
-    const int mb = min(mv.nms, w * mper), me = min(mv.nms, mb + mper);
-    for (int j = mb; j < me; ++j) {
-        const int2 rw = rows[j];
-        const int e = rw.x / A.nblk;
-        const double v = (double)rw.y * (live ? A.tms[(size_t)rw.x * nk + ik] : 0.0);
 #pragma unroll
-        for (int z = 0; z < NSP; ++z)
-            if (e == z) accm[z] += v;
-    }
-#pragma unroll
-    for (int e = 0; e < NSP; ++e) red[(w * NSP + e) * 64 + lane] = fma(pref, acc[e], accm[e]);
+    for (int e = 0; e < NSP; ++e) red[(w * NSP + e) * 64 + lane] = pref * acc[e];
     __syncthreads();
     // wavefront e < NSP adds the groups of absorber e in group order
     if (w < NSP && live) {
         double d = 0.0;
         for (int z = 0; z < kDeltaWaves; ++z) d += red[(z * NSP + w) * 64 + lane];
         const size_t at = ((size_t)s * NSP + w) * nk + ik;
-        const double t = A.chi[at] + d;
+        double m = 0.0;
+        if (A.dms)
+            for (int g = 0; g < A.ngrp; ++g) m += A.dms[(size_t)g * A.nrep * NSP * nk + at];
+        const double t = A.chi[at] + d + m;
         A.trial[at] = t;
         A.trialT[((size_t)w * nk + ik) * A.nrep + s] = t;
     }
     if (blockIdx.x == 0 && w < NSP) {
-        // k points below k0: MS rows only, summed over the whole move by this wavefront
-        for (int q = 0; q < A.k0; ++q) {
-            double v = 0.0;
-            for (int j = lane; j < mv.nms; j += 64) {
-                const int2 rw = rows[j];
-                if (rw.x / A.nblk == w) v += (double)rw.y * A.tms[(size_t)rw.x * nk + q];
-            }
-            v = wave_sum(v);
-            if (lane == 0) {
-                const size_t at = ((size_t)s * NSP + w) * nk + q;
-                const double t = A.chi[at] + v;
-                A.trial[at] = t;
-                A.trialT[((size_t)w * nk + q) * A.nrep + s] = t;
-            }
+        // k points below k0: no pair term, the MS change only
+        for (int q = lane; q < A.k0; q += 64) {
+            const size_t at = ((size_t)s * NSP + w) * nk + q;
+            double m = 0.0;
+            if (A.dms)
+                for (int g = 0; g < A.ngrp; ++g) m += A.dms[(size_t)g * A.nrep * NSP * nk + at];
+            const double t = A.chi[at] + m;
+            A.trial[at] = t;
+            A.trialT[((size_t)w * nk + q) * A.nrep + s] = t;
+        }
+    }
+}
+
+// MS chi change of every slot's swap: grid (32-point chunks of all k, edges, 32-slot
+// groups x entry groups), 256 lanes, lane = (k, four slots).  Almost every entry of a
+// swap's two sites changes, so this is a dense product over the entries,
+// dms[g][s][e][k] = sum_{j in group g} dC[s][e][j] T[e][j][k]: the group's count changes
+// ([entry][slot]) and table rows ([entry][k]) are staged in LDS with every lane's loads
+// requested together, then each lane runs over the entries with one 16-byte count read
+// and one table read per four multiply-adds.  The delta kernel adds the groups' partials.
+constexpr int kMsRowsMax = 128;   // entries per group
+__global__ __launch_bounds__(256) void k_ps_msgemm(int nrep, int nk, int nsp, int nblk, int rstride, int nss_rows,
+                                                  int nrow_tot, int ngrp, const int *__restrict__ dcount,
+                                                  const double *__restrict__ tms, double *__restrict__ dms) {
+    __shared__ int4 dcl[kMsRowsMax * 8];           // (entry, slot quad)
+    __shared__ double tt[kMsRowsMax * 32];         // (entry, k)
+    const int e = blockIdx.y, k0 = blockIdx.x * 32;
+    const int sg = blockIdx.z / ngrp, grp = blockIdx.z - sg * ngrp, s0 = sg * 32;
+    const int rpg = (nblk + ngrp - 1) / ngrp;
+    const int jb = min(nblk, grp * rpg), jn = min(nblk, jb + rpg) - jb;   // jn <= kMsRowsMax
+    const int kk = threadIdx.x & 31, sq = threadIdx.x >> 5;
+    {
+        const Buf bd = make_buf(dcount, (uint32_t)(sizeof(int) * (size_t)nrep * nrow_tot));
+        const Buf bt = make_buf(tms, (uint32_t)(sizeof(double) * (size_t)nsp * nblk * nk));
+        constexpr int U = kMsRowsMax * 32 / 256;
+        int vd[U];
+        double vt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = u * 256 + (int)threadIdx.x;
+            const int j = q >> 5, c = q & 31;
+            const int s = s0 + c, k = k0 + c;
+            vd[u] = buf_i32(bd, j < jn && s < nrep,
+                            4u * ((uint32_t)s * (uint32_t)nrow_tot + (uint32_t)(e * rstride + nss_rows + jb + j)));
+            vt[u] = buf_f64(bt, j < jn && k < nk, (uint32_t)(sizeof(double) * (((size_t)e * nblk + jb + j) * nk + k)));
+        }
+        int *dci = reinterpret_cast<int *>(dcl);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = u * 256 + (int)threadIdx.x;
+            dci[q] = vd[u];
+            tt[q] = vt[u];
+        }
+    }
+    __syncthreads();
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll 4
+    for (int j = 0; j < jn; ++j) {
+        const int4 d = dcl[j * 8 + sq];
+        const double t = tt[j * 32 + kk];
+        a0 = fma((double)d.x, t, a0);
+        a1 = fma((double)d.y, t, a1);
+        a2 = fma((double)d.z, t, a2);
+        a3 = fma((double)d.w, t, a3);
+    }
+    const int k = k0 + kk;
+    if (k < nk) {
+        const double a[4] = {a0, a1, a2, a3};
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int s = s0 + 4 * sq + h;
+            if (s < nrep) dms[(((size_t)grp * nrep + s) * nsp + e) * nk + k] = a[h];
         }
     }
 }
 
 // ------------------------------------------------------------------------------ P3
-// Trial chi(R): grid (column blocks of 16 of one edge, edges, slot groups of 32), 256
-// lanes.  The edge's trial chi of the group's 32 slots over the window's nonzero k rows
-// [kb0, kb1) is staged in LDS (k-major, one coalesced row per k); lane (column c, slot
-// pair) accumulates two slots against Te[e][k][c] in FP64 over k.
-constexpr int kTfCols = 16;
-__global__ __launch_bounds__(256) void k_ps_transform(int nrep, int nk, int kb0, int kb1, int ncol_e,
-                                                      const double *__restrict__ trialT,
-                                                      const double *__restrict__ Te, double *__restrict__ Rt) {
-    extern __shared__ double sa[];   // (kb1 - kb0, 32)
-    const int e = blockIdx.y, s0 = blockIdx.z * 32;
-    const int nkk = kb1 - kb0;
-    for (int q = threadIdx.x; q < nkk * 32; q += blockDim.x) {
-        const int k = q >> 5, c = q & 31;
-        sa[q] = (s0 + c < nrep) ? trialT[((size_t)e * nk + kb0 + k) * nrep + s0 + c] : 0.0;
-    }
-    __syncthreads();
-    const int col = blockIdx.x * kTfCols + (threadIdx.x & (kTfCols - 1));
-    const int sp = threadIdx.x / kTfCols;          // slot pair 0..15
-    const bool ok = col < ncol_e;
-    const double *B = Te + ((size_t)e * nk + kb0) * ncol_e + (ok ? col : 0);
+// Trial chi(R) and its residual sums: grid (column blocks of 16 of one edge, edges, slot
+// groups of 32), 256 lanes.  Over the window's nonzero k rows [kb0, kb1) in tiles of 128:
+// the edge's trial chi of the group's 32 slots (k-major) and the tile of the DFT matrix
+// Te[e][k][cols] are staged in LDS, every lane's loads requested together (bounds-checked
+// buffer loads, no branch); lane (column c, slot pair) accumulates two slots in FP64.
+// The epilogue forms the model m (the column, or |z| of a point for an |chi(R)| target)
+// and its residual d = m - t against the target, and reduces over the block's columns
+//   U1 = sum d t,  U2 = sum d^2,  W1 = sum w d t,  W2 = sum w d^2    (w = 1/sigma^2)
+// per slot: with T0 = sum t^2 and Q0 = sum w t^2 of the edge, the profiled scale is
+// s = (T0 + U1) / (T0 + 2 U1 + U2) and the edge's share of chi^2 nfit is
+// s^2 W2 + 2 s (s - 1) W1 + (s - 1)^2 Q0 (method_a_rmc.py:190-227); every term is small
+// near a fit, so nothing cancels.
+constexpr int kTfCols = 16, kTfK = 128;
+struct TfArgs {
+    int nrep, nk, nsp, kb0, kb1, ncol_e, nbc, magnitude;
+    const double *trialT;  // (nsp, nk, nrep)
+    const double *Te;      // (nsp, nk, ncol_e) DFT matrix / absorber count
+    const double *Rx;      // (nsp * ncol_e) transform of the constant term
+    const double *tgt;     // complex: (nsp * ncol_e); magnitude: (nsp * ncol_e / 2)
+    const double *wgt;     // (nsp * ncol_e / 2) 1 / sigma^2 per point
+    double *sums;          // (nrep, nsp, nbc, 4)
+};
+
+__global__ __launch_bounds__(256) void k_ps_transform(TfArgs A) {
+    __shared__ double sa[kTfK * 32];
+    __shared__ double sb[kTfK * kTfCols];
+    const int e = blockIdx.y, s0 = blockIdx.z * 32, c0 = blockIdx.x * kTfCols;
+    const int nrep = A.nrep, nk = A.nk, ncol_e = A.ncol_e;
+    const Buf bA = make_buf(A.trialT, (uint32_t)(sizeof(double) * (size_t)A.nsp * nk * nrep));
+    const Buf bB = make_buf(A.Te, (uint32_t)(sizeof(double) * (size_t)A.nsp * nk * ncol_e));
+    const int c = threadIdx.x & (kTfCols - 1), sp = threadIdx.x / kTfCols;
     double a0 = 0.0, a1 = 0.0;
+    double va[kTfK * 32 / 256], vb[kTfK * kTfCols / 256];
+    auto load_tile = [&](int k0) {
+        const int kn = min(kTfK, A.kb1 - k0);
+#pragma unroll
+        for (int u = 0; u < kTfK * 32 / 256; ++u) {
+            const int q = u * 256 + threadIdx.x, k = q >> 5, cc = q & 31;
+            const bool ok = k < kn && s0 + cc < nrep;
+            va[u] = buf_f64(bA, ok, (uint32_t)(sizeof(double) * (((size_t)e * nk + k0 + k) * nrep + s0 + cc)));
+        }
+#pragma unroll
+        for (int u = 0; u < kTfK * kTfCols / 256; ++u) {
+            const int q = u * 256 + threadIdx.x, k = q / kTfCols, cc = q % kTfCols;
+            const bool ok = k < kn && c0 + cc < ncol_e;
+            vb[u] = buf_f64(bB, ok, (uint32_t)(sizeof(double) * (((size_t)e * nk + k0 + k) * ncol_e + c0 + cc)));
+        }
+    };
+    load_tile(A.kb0);
+    for (int k0 = A.kb0; k0 < A.kb1; k0 += kTfK) {
+        const int kn = min(kTfK, A.kb1 - k0);
+        __syncthreads();   // the previous tile's reads are done
+#pragma unroll
+        for (int u = 0; u < kTfK * 32 / 256; ++u) sa[u * 256 + threadIdx.x] = va[u];
+#pragma unroll
+        for (int u = 0; u < kTfK * kTfCols / 256; ++u) sb[u * 256 + threadIdx.x] = vb[u];
+        __syncthreads();
+        if (k0 + kTfK < A.kb1) load_tile(k0 + kTfK);   // in flight during this tile
 #pragma unroll 8
-    for (int k = 0; k < nkk; ++k) {
-        const double bv = B[(size_t)k * ncol_e];
-        const double2 x = *reinterpret_cast<const double2 *>(sa + k * 32 + 2 * sp);
-        a0 = fma(x.x, bv, a0);
-        a1 = fma(x.y, bv, a1);
+        for (int k = 0; k < kn; ++k) {
+            const double2 x = *reinterpret_cast<const double2 *>(sa + k * 32 + 2 * sp);
+            const double bv = sb[k * kTfCols + c];
+            a0 = fma(x.x, bv, a0);
+            a1 = fma(x.y, bv, a1);
+        }
     }
-    if (ok) {
-        const int s = s0 + 2 * sp;
-        const size_t ld = (size_t)gridDim.y * ncol_e;
-        if (s < nrep) Rt[(size_t)s * ld + (size_t)e * ncol_e + col] = a0;
-        if (s + 1 < nrep) Rt[(size_t)(s + 1) * ld + (size_t)e * ncol_e + col] = a1;
+    // residual sums of the block's columns, per slot
+    const int col = c0 + c;
+    const bool ok = col < ncol_e;
+    const size_t gc = (size_t)e * ncol_e + (ok ? col : 0);    // column over all edges
+    const size_t pt = gc >> 1;                                  // its R point
+    const double rx = ok ? A.Rx[gc] : 0.0;
+    const double w = ok ? A.wgt[pt] : 0.0;
+    const double tc = ok ? (A.magnitude ? A.tgt[pt] : A.tgt[gc]) : 0.0;
+    double S4[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const double z = (h ? a1 : a0) + rx;
+        const double zp = __shfl_xor(z, 1);     // the point's other part (lane c ^ 1)
+        double d, ww, t;
+        if (A.magnitude) {
+            const bool use = ok && (c & 1) == 0;
+            t = use ? tc : 0.0;
+            d = use ? hypot(z, zp) - tc : 0.0;
+            ww = use ? w : 0.0;
+        } else {
+            t = tc;
+            d = ok ? z - tc : 0.0;
+            ww = w;
+        }
+        S4[h][0] = d * t;
+        S4[h][1] = d * d;
+        S4[h][2] = ww * d * t;
+        S4[h][3] = ww * d * d;
     }
+#pragma unroll
+    for (int m = kTfCols / 2; m >= 1; m >>= 1)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S4[h][v] += __shfl_xor(S4[h][v], m);
+    if (c == 0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int s = s0 + 2 * sp + h;
+            if (s >= nrep) continue;
+            double *o = A.sums + (((size_t)s * A.nsp + e) * A.nbc + blockIdx.x) * 4;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) o[v] = S4[h][v];
+        }
+    }
+}
+
+// chi^2 of slot s from its residual sums (nsp, nbc, 4), by one wavefront (every lane gets
+// it): the blocks are added in block order per lane and the lanes in butterfly order.
+__device__ double c2_from_sums(const double *sums, int nsp, int nbc, const double *q0, int profile,
+                               double nfit) {
+    const int lane = threadIdx.x & 63;
+    // every edge's first two block rounds requested together (bounds-checked, no branch)
+    const Buf bs = make_buf(sums, (uint32_t)(sizeof(double) * 4 * (size_t)nsp * nbc));
+    double P[chm::kMaxNsp][2][4];
+#pragma unroll
+    for (int e = 0; e < chm::kMaxNsp; ++e)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int b = lane + 64 * h;
+            const bool ok = e < nsp && b < nbc;
+            const uint32_t off = (uint32_t)(sizeof(double) * 4 * ((size_t)(ok ? e : 0) * nbc + b));
+#pragma unroll
+            for (int v = 0; v < 4; ++v) P[e][h][v] = buf_f64(bs, ok, off + 8u * v);
+        }
+    double U[chm::kMaxNsp][4];
+#pragma unroll
+    for (int e = 0; e < chm::kMaxNsp; ++e)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) U[e][v] = P[e][0][v] + P[e][1][v];
+    for (int e = 0; e < nsp; ++e)
+        for (int b = lane + 128; b < nbc; b += 64)
+            for (int v = 0; v < 4; ++v) U[e][v] += sums[((size_t)e * nbc + b) * 4 + v];
+    // one butterfly for all the sums: each step's exchanges are independent
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+        for (int e = 0; e < chm::kMaxNsp; ++e)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) U[e][v] += __shfl_xor(U[e][v], m);
+    double tot = 0.0;
+#pragma unroll
+    for (int e = 0; e < chm::kMaxNsp; ++e) {
+        if (e >= nsp) break;
+        const double T0 = q0[2 * e], Q0 = q0[2 * e + 1];
+        double sc = 1.0, sm1 = 0.0;
+        if (profile) {
+            const double P2 = fmax(U[e][1] + 2.0 * U[e][0] + T0, 1e-30);
+            const double P1 = U[e][0] + T0;
+            sc = P1 / P2;
+            sm1 = (P1 - P2) / P2;
+        }
+        tot += sc * sc * U[e][3] + 2.0 * sc * sm1 * U[e][2] + sm1 * sm1 * Q0;
+    }
+    return tot / nfit;
 }
 
 // ------------------------------------------------------------------------------ P4
@@ -501,109 +682,49 @@ struct PsAcceptArgs {
     const int *rung_of;
     uint64_t *stepc;
     const double *beta_slot;
-    const double *Rt;        // (nrep, nsp * ncol_e) trial chi(R) without the constant term
-    const double *Rx;        // (nsp * ncol_e) transform of the constant term (zeros if none)
-    const double *tgt;       // complex: (nsp * ncol_e) re/im; magnitude: (nsp*nkw*nwin)
-    const double *isig;      // (nsp*nkw*nwin) 1 / sigma
+    const double *sums;      // (nrep, nsp, nbc, 4) residual sums of the trial chi(R)
+    const double *q0;        // (nsp, 2) per edge T0 = sum t^2, Q0 = sum t^2 / sigma^2
+    int nbc;                 // column blocks per edge
     const PsMove *mv;
-    const int *aff;
-    const double *rnew;
-    const int *tnew;
+    const AffRec *aff;
     const double *trial;
     uint8_t *sp;
     double *pos, *r, *chi, *c2, *best_c2, *best_pos;
     int *pt;
     uint8_t *best_sp;
     long long *acc_rung;
+    int *dcount;             // (nrep, nrow_tot) zeroed for the next step (MS only)
+    int nrow_tot;
     int4 *log;               // (log_cap, nrep): a, b, is-swap | accepted << 1, slot
     double *log_c2;          // (log_cap, nrep): trial chi^2
     int log_cap;
     long long rec_base;
 };
 
-constexpr int kAccThreads = 1024;
-constexpr int kAccPts = 4;    // R points per lane held in registers (npt <= 4096)
-
-// Reduced chi^2 of one window row Rt (nsp * ncol_e, without the constant term) with the
-// profiled per-edge scale (method_a_rmc.py:180-227), by all lanes of the workgroup (every
-// lane gets it).  Each lane's points are loaded once, up front.
-__device__ double row_chi2(const PsAcceptArgs &A, const double *Rt, double *red) {
-    const int npt_e = A.nkw * A.nwin, npt = A.nsp * npt_e;
-    double zr[kAccPts], zi[kAccPts], tr[kAccPts], ti[kAccPts], is[kAccPts];
-    int pe[kAccPts];
-#pragma unroll
-    for (int u = 0; u < kAccPts; ++u) {
-        const int q = threadIdx.x + u * blockDim.x;
-        const bool ok = q < npt;
-        const int qq = ok ? q : 0;
-        zr[u] = ok ? Rt[2 * qq] + A.Rx[2 * qq] : 0.0;
-        zi[u] = ok ? Rt[2 * qq + 1] + A.Rx[2 * qq + 1] : 0.0;
-        tr[u] = ok ? (A.magnitude ? A.tgt[qq] : A.tgt[2 * qq]) : 0.0;
-        ti[u] = (ok && !A.magnitude) ? A.tgt[2 * qq + 1] : 0.0;
-        is[u] = ok ? A.isig[qq] : 0.0;
-        pe[u] = ok ? qq / npt_e : -1;
-    }
-    double m[kAccPts];
-#pragma unroll
-    for (int u = 0; u < kAccPts; ++u) m[u] = A.magnitude ? hypot(zr[u], zi[u]) : 0.0;
-    double sc[chm::kMaxNsp];
-#pragma unroll
-    for (int e = 0; e < chm::kMaxNsp; ++e) sc[e] = 1.0;
-    if (A.profile) {
-        for (int e = 0; e < A.nsp; ++e) {
-            double nu = 0.0, de = 0.0;
-#pragma unroll
-            for (int u = 0; u < kAccPts; ++u) {
-                if (pe[u] != e) continue;
-                if (A.magnitude) {
-                    nu += m[u] * tr[u];
-                    de += m[u] * m[u];
-                } else {
-                    nu += zr[u] * tr[u] + zi[u] * ti[u];
-                    de += zr[u] * zr[u] + zi[u] * zi[u];
-                }
-            }
-            const double N = block_sum_d(nu, red);
-            const double D = block_sum_d(de, red + 16);
-            sc[e] = N / fmax(D, 1e-30);
-        }
-    }
-    double sq = 0.0;
-#pragma unroll
-    for (int u = 0; u < kAccPts; ++u) {
-        if (pe[u] < 0) continue;
-        double se = 1.0;
-#pragma unroll
-        for (int z = 0; z < chm::kMaxNsp; ++z)
-            if (z == pe[u]) se = sc[z];
-        if (A.magnitude) {
-            const double d = (m[u] * se - tr[u]) * is[u];
-            sq += d * d;
-        } else {
-            const double dr = (zr[u] * se - tr[u]) * is[u];
-            const double di = (zi[u] * se - ti[u]) * is[u];
-            sq += dr * dr + di * di;
-        }
-    }
-    return block_sum_d(sq, red + 32) / A.nfit;
-}
+constexpr int kAccThreads = 256;
 
 __global__ __launch_bounds__(kAccThreads) void k_ps_accept(PsAcceptArgs A) {
-    __shared__ double red[48];
+    __shared__ double red[1];
     __shared__ int accs, imps;
     const int s = blockIdx.x;
     const int rung = A.rung_of[s];
     const uint64_t n = A.stepc[s] + 1;
     const PsMove mv = A.mv[s];
-    const double c2t = row_chi2(A, A.Rt + (size_t)s * A.nsp * A.ncol_e, red);
+    const double bslot = A.beta_slot[s], c2c = A.c2[s], bestc = A.best_c2[s];
+    if (threadIdx.x < 64) {
+        const double v = c2_from_sums(A.sums + (size_t)s * A.nsp * A.nbc * 4, A.nsp, A.nbc, A.q0, A.profile, A.nfit);
+        if (threadIdx.x == 0) red[0] = v;
+    }
+    __syncthreads();
+    const double c2t = red[0];
     if (threadIdx.x == 0) {
-        double x = A.beta_slot[s] * (c2t - A.c2[s]) * A.nfit;
+        double x = bslot * (c2t - c2c) * A.nfit;
         if (A.has_prior) x += A.ebeta * mv.de;
         const uint64_t h = splitmix64(A.seed ^ (n * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
         const bool acc = metropolis(x, u01(splitmix64(h)));
         accs = acc ? 1 : 0;
-        const double cur = acc ? c2t : A.c2[s];
-        imps = (cur < A.best_c2[s]) ? 1 : 0;
+        const double cur = acc ? c2t : c2c;
+        imps = (cur < bestc) ? 1 : 0;
         if (A.log_cap > 0) {
             const long long ri = (long long)n - 1 - A.rec_base;
             if (ri >= 0) {
@@ -616,9 +737,9 @@ __global__ __launch_bounds__(kAccThreads) void k_ps_accept(PsAcceptArgs A) {
     __syncthreads();
     if (accs) {
         for (int q = threadIdx.x; q < mv.naff; q += blockDim.x) {
-            const int p = A.aff[(size_t)s * A.maxaff + q];
-            A.r[(size_t)s * A.npairs + p] = A.rnew[(size_t)s * A.maxaff + q];
-            A.pt[(size_t)s * A.npairs + p] = A.tnew[(size_t)s * A.maxaff + q];
+            const AffRec ar = A.aff[(size_t)s * A.maxaff + q];
+            A.r[(size_t)s * A.npairs + ar.p] = ar.rnew;
+            A.pt[(size_t)s * A.npairs + ar.p] = ar.tnew;
         }
         const size_t ck = (size_t)A.nsp * A.nk;
         for (size_t q = threadIdx.x; q < ck; q += blockDim.x) A.chi[s * ck + q] = A.trial[s * ck + q];
@@ -638,20 +759,54 @@ __global__ __launch_bounds__(kAccThreads) void k_ps_accept(PsAcceptArgs A) {
     }
     if (imps) {
         __syncthreads();   // the committed species / position are visible to every lane
-        for (int q = threadIdx.x; q < A.natoms; q += blockDim.x)
-            A.best_sp[(size_t)s * A.natoms_pad + q] = A.sp[(size_t)s * A.natoms_pad + q];
-        for (int q = threadIdx.x; q < 3 * A.natoms; q += blockDim.x)
-            A.best_pos[(size_t)s * A.natoms * 3 + q] = A.pos[(size_t)s * A.natoms * 3 + q];
-        if (threadIdx.x == 0) A.best_c2[s] = accs ? c2t : A.c2[s];
+        // species as 16-byte words, positions as doubles: each lane's loads issued together
+        {
+            const uint4 *src = reinterpret_cast<const uint4 *>(A.sp + (size_t)s * A.natoms_pad);
+            uint4 *dst = reinterpret_cast<uint4 *>(A.best_sp + (size_t)s * A.natoms_pad);
+            for (int q = threadIdx.x; q < A.natoms_pad / 16; q += blockDim.x) dst[q] = src[q];
+        }
+        {
+            const int np = 3 * A.natoms;
+            const double *src = A.pos + (size_t)s * np;
+            double *dst = A.best_pos + (size_t)s * np;
+            const Buf bp = make_buf(src, (uint32_t)(sizeof(double) * np));
+            constexpr int U = 16;
+            for (int q0 = 0; q0 < np; q0 += U * (int)blockDim.x) {
+                double v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int q = q0 + u * (int)blockDim.x + (int)threadIdx.x;
+                    v[u] = buf_f64(bp, q < np, 8u * (uint32_t)q);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int q = q0 + u * (int)blockDim.x + (int)threadIdx.x;
+                    if (q < np) dst[q] = v[u];
+                }
+            }
+        }
+        if (threadIdx.x == 0) A.best_c2[s] = accs ? c2t : c2c;
     }
+    if (A.dcount)
+        for (int q = threadIdx.x; q < A.nrow_tot; q += blockDim.x) A.dcount[(size_t)s * A.nrow_tot + q] = 0;
     if (threadIdx.x == 0) A.stepc[s] = n;
 }
 
-// chi^2 of rows of Rt (for init and chm_psampler_chi2): the accept kernel's sums
-__global__ __launch_bounds__(kAccThreads) void k_ps_chi2(PsAcceptArgs A, double *out) {
-    __shared__ double red[48];
-    const double c2 = row_chi2(A, A.Rt + (size_t)blockIdx.x * A.nsp * A.ncol_e, red);
-    if (threadIdx.x == 0) out[blockIdx.x] = c2;
+// chi^2 of every row of `sums` (for init and chm_psampler_chi2), one wavefront per row
+__global__ __launch_bounds__(64) void k_ps_chi2(const double *__restrict__ sums, int nsp, int nbc,
+                                               const double *__restrict__ q0, int profile, double nfit,
+                                               double *__restrict__ out) {
+    const double v = c2_from_sums(sums + (size_t)blockIdx.x * nsp * nbc * 4, nsp, nbc, q0, profile, nfit);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+// chi rows (nconf, nsp, nk) -> k-major (nsp, nk, nconf), the transform's input layout
+__global__ void k_ps_to_kmajor(int nconf, int nsp, int nk, const double *__restrict__ chi, double *__restrict__ outT) {
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long)nconf * nsp * nk) return;
+    const int s = (int)(tid / ((long)nsp * nk));
+    const int q = (int)(tid - (long)s * nsp * nk);
+    outT[(size_t)q * nconf + s] = chi[tid];
 }
 
 __global__ void k_ps_exchange(int nrep, const double *__restrict__ beta, const double *__restrict__ c2,
@@ -726,17 +881,21 @@ struct chm_psampler {
     int k0 = 0, nchunk = 0;          // first k > 0; 64-point chunks of the k > 0 points
     int kb0 = 0, kb1 = 0;            // k rows of the window transform that are not all zero
     Geo g{};
-    size_t lds_delta = 0, lds_prop = 0, lds_tf = 0;
+    size_t lds_delta = 0, lds_prop = 0;
     uint64_t seed = 0;
     bool inited = false;
     long long rec_base = 0;
     long long hstep = 0;   // steps completed (host mirror of the device counters)
     int log_cap = 0, mark_cap = 0;
     double cnt[chm::kMaxNsp] = {0};
-    chm::DevBuf rev, ideal, etab, Te, Te_raw, Rx, tgt, isig, ms_sten, tms;
+    chm::DevBuf rev, ideal, etab, Te, Te_raw, Rx, tgt, wgt, q0, ms_sten, tms;
+    int nbc = 1;   // 16-column blocks per edge of the transform
     chm::DevBuf sp, pos, r, pt, chi, c2, best_c2, best_sp, best_pos, rung_of, slot_of, beta, beta_slot,
         stepc, iex, acc_rung, exch_att, exch_acc;
-    chm::DevBuf mv, aff, rnew, tnew, msrow, trial, trialT, Rt, log, log_c2, marks, tmp, tmp2;
+    chm::DevBuf mv, aff, dcount, dms, trial, trialT, sums, log, log_c2, marks, tmp, tmp2;
+    int nsplit = 1;   // workgroups per slot of the proposal (MS gather share)
+    int ms_groups = 1;   // entry groups of the MS product (partials added by the delta kernel)
+    static int ms_groups_for(int nblk) { return std::max(4, (nblk + kMsRowsMax - 1) / kMsRowsMax); }
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
@@ -784,10 +943,8 @@ PsArgs prop_args(chm_psampler *S) {
     A.pt = S->pt.as<int>();
     A.etab = S->has_prior ? S->etab.as<double>() : nullptr;
     A.mv = S->mv.as<PsMove>();
-    A.aff = S->aff.as<int>();
-    A.rnew = S->rnew.as<double>();
-    A.tnew = S->tnew.as<int>();
-    A.msrow = S->msrow.as<int2>();
+    A.aff = S->aff.as<AffRec>();
+    A.dcount = S->has_ms ? S->dcount.as<int>() : nullptr;
     A.has_ms = S->has_ms;
     A.g = S->g;
     A.ms_sten = S->ms_sten.as<int4>();
@@ -815,14 +972,11 @@ PsAcceptArgs accept_args(chm_psampler *S) {
     A.rung_of = S->rung_of.as<int>();
     A.stepc = S->stepc.as<uint64_t>();
     A.beta_slot = S->beta_slot.as<double>();
-    A.Rt = S->Rt.as<double>();
-    A.Rx = S->Rx.as<double>();
-    A.tgt = S->tgt.as<double>();
-    A.isig = S->isig.as<double>();
+    A.sums = S->sums.as<double>();
+    A.q0 = S->q0.as<double>();
+    A.nbc = S->nbc;
     A.mv = S->mv.as<PsMove>();
-    A.aff = S->aff.as<int>();
-    A.rnew = S->rnew.as<double>();
-    A.tnew = S->tnew.as<int>();
+    A.aff = S->aff.as<AffRec>();
     A.trial = S->trial.as<double>();
     A.sp = S->sp.as<uint8_t>();
     A.pos = S->pos.as<double>();
@@ -834,6 +988,8 @@ PsAcceptArgs accept_args(chm_psampler *S) {
     A.pt = S->pt.as<int>();
     A.best_sp = S->best_sp.as<uint8_t>();
     A.acc_rung = S->acc_rung.as<long long>();
+    A.dcount = S->has_ms ? S->dcount.as<int>() : nullptr;
+    A.nrow_tot = S->g.nrow_tot;
     A.log = S->log_cap ? S->log.as<int4>() : nullptr;
     A.log_c2 = S->log_cap ? S->log_c2.as<double>() : nullptr;
     A.log_cap = S->log_cap;
@@ -854,13 +1010,7 @@ int launch_delta(chm_psampler *S, hipStream_t st) {
     A.nblk = std::max(1, S->g.nblk);
     A.s02 = c->s02;
     A.mv = S->mv.as<PsMove>();
-    A.aff = S->aff.as<int>();
-    A.rnew = S->rnew.as<double>();
-    A.tnew = S->tnew.as<int>();
-    A.r = S->r.as<double>();
-    A.pt = S->pt.as<int>();
-    A.shell = c->shell.as<int>();
-    A.sig2 = c->sig2.as<double>();
+    A.aff = S->aff.as<AffRec>();
     A.ftab = c->f.as<double>();
     A.dtab = c->delta.as<double>();
     A.lamtab = c->lam.as<double>();
@@ -868,8 +1018,8 @@ int launch_delta(chm_psampler *S, hipStream_t st) {
     A.drtab = c->drtab.as<double>();
     A.c3tab = c->c3tab.as<double>();
     A.s2tab = c->s2tab.as<double>();
-    A.msrow = S->msrow.as<int2>();
-    A.tms = S->has_ms ? S->tms.as<double>() : c->k.as<double>();
+    A.dms = S->has_ms ? S->dms.as<double>() : nullptr;
+    A.ngrp = S->ms_groups;
     A.chi = S->chi.as<double>();
     A.trial = S->trial.as<double>();
     A.trialT = S->trialT.as<double>();
@@ -887,10 +1037,39 @@ int delta_attr(chm_psampler *S) {
     return CHM_OK;
 }
 
+// trial chi(R) residual sums of nconf k-major chi rows
+int launch_transform(chm_psampler *S, hipStream_t st, int nconf, const double *trialT, double *sums) {
+    TfArgs T{};
+    T.nrep = nconf;
+    T.nk = S->nk;
+    T.nsp = S->nsp;
+    T.kb0 = S->kb0;
+    T.kb1 = S->kb1;
+    T.ncol_e = S->ncol_e;
+    T.nbc = S->nbc;
+    T.magnitude = S->magnitude;
+    T.trialT = trialT;
+    T.Te = S->Te.as<double>();
+    T.Rx = S->Rx.as<double>();
+    T.tgt = S->tgt.as<double>();
+    T.wgt = S->wgt.as<double>();
+    T.sums = sums;
+    k_ps_transform<<<dim3(S->nbc, S->nsp, (nconf + 31) / 32), 256, 0, st>>>(T);
+    CHM_LAUNCH_CHECK();
+    return CHM_OK;
+}
+
 int launch_step(chm_psampler *S, hipStream_t st) {
     {
         PsArgs A = prop_args(S);
-        k_ps_propose<<<S->nrep, kPropThreads, S->lds_prop, st>>>(A);
+        k_ps_propose<<<dim3(S->nsplit, S->nrep), kPropThreads, S->lds_prop, st>>>(A);
+        CHM_LAUNCH_CHECK();
+    }
+    if (S->has_ms) {
+        const Geo &g = S->g;
+        k_ps_msgemm<<<dim3((S->nk + 31) / 32, S->nsp, ((S->nrep + 31) / 32) * S->ms_groups), 256, 0, st>>>(
+            S->nrep, S->nk, S->nsp, g.nblk, g.rstride, g.nss_rows, g.nrow_tot, S->ms_groups, S->dcount.as<int>(),
+            S->tms.as<double>(), S->dms.as<double>());
         CHM_LAUNCH_CHECK();
     }
     int rc = CHM_E_ARG;
@@ -902,10 +1081,7 @@ int launch_step(chm_psampler *S, hipStream_t st) {
         case 6: rc = launch_delta<6>(S, st); break;
     }
     if (rc) return rc;
-    k_ps_transform<<<dim3((S->ncol_e + kTfCols - 1) / kTfCols, S->nsp, (S->nrep + 31) / 32), 256, S->lds_tf,
-                     st>>>(S->nrep, S->nk, S->kb0, S->kb1, S->ncol_e, S->trialT.as<double>(), S->Te.as<double>(),
-                           S->Rt.as<double>());
-    CHM_LAUNCH_CHECK();
+    if ((rc = launch_transform(S, st, S->nrep, S->trialT.as<double>(), S->sums.as<double>()))) return rc;
     k_ps_accept<<<S->nrep, kAccThreads, 0, st>>>(accept_args(S));
     CHM_LAUNCH_CHECK();
     return CHM_OK;
@@ -970,8 +1146,6 @@ extern "C" int chm_psampler_create(chm_ctx *ctx, int nrep, const double *ideal, 
     while (S->k0 < S->nk && !(ctx->h_k[S->k0] > 0.0)) ++S->k0;
     if (S->k0 >= S->nk) return fail((chm::set_error("no k point above 0"), CHM_E_ARG));
     S->nchunk = (S->nk - S->k0 + 63) / 64;
-    if ((size_t)S->nsp * nkw * nwin > (size_t)kAccThreads * kAccPts)
-        return fail((chm::set_error("fit window too large (at most 4096 points over all channels)"), CHM_E_ARG));
     S->lds_prop = (size_t)S->natoms_pad;
     if (S->lds_prop > 120 * 1024)
         return fail((chm::set_error("at most 122880 sites (the proposal stages the species in LDS)"), CHM_E_ARG));
@@ -995,11 +1169,17 @@ extern "C" int chm_psampler_create(chm_ctx *ctx, int nrep, const double *ideal, 
         return CHM_OK;
     };
     const size_t npt = (size_t)S->nsp * nkw * nwin;
-    std::vector<double> isig(npt);
+    std::vector<double> wgt(npt), q0(2 * (size_t)S->nsp, 0.0);
     for (size_t q = 0; q < npt; ++q) {
         if (!(sigma[q] > 0.0)) return fail((chm::set_error("sigma must be positive"), CHM_E_ARG));
-        isig[q] = 1.0 / sigma[q];
+        wgt[q] = 1.0 / (sigma[q] * sigma[q]);
+        const size_t e = q / ((size_t)nkw * nwin);
+        const double t2 = magnitude ? target[q] * target[q]
+                                    : target[2 * q] * target[2 * q] + target[2 * q + 1] * target[2 * q + 1];
+        q0[2 * e] += t2;
+        q0[2 * e + 1] += t2 * wgt[q];
     }
+    S->nbc = (S->ncol_e + kTfCols - 1) / kTfCols;
     std::vector<double> rx((size_t)S->nsp * S->ncol_e, 0.0);
     if (Rx) std::copy(Rx, Rx + rx.size(), rx.begin());
     const size_t te_n = (size_t)S->nsp * S->nk * S->ncol_e;
@@ -1020,18 +1200,12 @@ extern "C" int chm_psampler_create(chm_ctx *ctx, int nrep, const double *ideal, 
         S->kb0 = 0;
         S->kb1 = 1;
     }
-    S->lds_tf = sizeof(double) * 32 * (size_t)(S->kb1 - S->kb0);
-    if (S->lds_tf > 160 * 1024) return fail((chm::set_error("window transform spans too many k points"), CHM_E_ARG));
-    {
-        hipError_t e = hipFuncSetAttribute((const void *)k_ps_transform, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)S->lds_tf);
-        if (e != hipSuccess) return fail((chm::set_error(hipGetErrorString(e)), CHM_E_HIP));
-    }
+
     if ((rc = up(S->rev, rev, sizeof(int) * npairs)) || (rc = up(S->ideal, ideal, sizeof(double) * 3 * natoms)) ||
         (rc = up(S->Te_raw, Te, sizeof(double) * te_n)) || (rc = S->Te.ensure(sizeof(double) * te_n)) ||
         (rc = up(S->Rx, rx.data(), sizeof(double) * rx.size())) ||
         (rc = up(S->tgt, target, sizeof(double) * npt * (magnitude ? 1 : 2))) ||
-        (rc = up(S->isig, isig.data(), sizeof(double) * npt)))
+        (rc = up(S->wgt, wgt.data(), sizeof(double) * npt)) || (rc = up(S->q0, q0.data(), sizeof(double) * q0.size())))
         return fail(rc);
     CHM_HIP(hipStreamSynchronize(ctx->stream));
     const size_t nr = nrep;
@@ -1047,12 +1221,10 @@ extern "C" int chm_psampler_create(chm_ctx *ctx, int nrep, const double *ideal, 
         (rc = S->stepc.ensure(sizeof(uint64_t) * nr)) || (rc = S->iex.ensure(sizeof(uint64_t))) ||
         (rc = S->acc_rung.ensure(sizeof(long long) * nr)) ||
         (rc = S->exch_att.ensure(sizeof(long long) * nr)) || (rc = S->exch_acc.ensure(sizeof(long long) * nr)) ||
-        (rc = S->mv.ensure(sizeof(PsMove) * nr)) || (rc = S->aff.ensure(sizeof(int) * nr * S->maxaff)) ||
-        (rc = S->rnew.ensure(sizeof(double) * nr * S->maxaff)) ||
-        (rc = S->tnew.ensure(sizeof(int) * nr * S->maxaff)) ||
-        (rc = S->msrow.ensure(sizeof(int2) * nr * S->msmax)) ||
+        (rc = S->mv.ensure(sizeof(PsMove) * nr)) || (rc = S->aff.ensure(sizeof(AffRec) * nr * S->maxaff)) ||
+
         (rc = S->trialT.ensure(sizeof(double) * nr * S->nsp * S->nk)) ||
-        (rc = S->Rt.ensure(sizeof(double) * nr * S->nsp * S->ncol_e)) ||
+        (rc = S->sums.ensure(sizeof(double) * nr * S->nsp * S->nbc * 4)) ||
         (rc = S->tmp.ensure(sizeof(double) * nr)))
         return fail(rc);
     CHM_HIP(hipMemsetAsync(S->acc_rung.p, 0, S->acc_rung.bytes, ctx->stream));
@@ -1123,13 +1295,26 @@ extern "C" int chm_psampler_set_ms(chm_psampler *S, const chm_geom *gm, const in
     const int msmax = g.nsp * g.nblk;
     if ((rc = S->ms_sten.ensure(16 * (size_t)g.nstencil)) ||
         (rc = S->tms.ensure(sizeof(double) * (size_t)g.nsp * g.nblk * S->nk)) ||
-        (rc = S->msrow.ensure(sizeof(int2) * (size_t)S->nrep * msmax)))
+        (rc = S->dcount.ensure(sizeof(int) * (size_t)S->nrep * g.nrow_tot)) ||
+        (rc = S->dms.ensure(sizeof(double) * (size_t)S->ms_groups_for(g.nblk) * S->nrep * g.nsp * S->nk)))
         return rc;
+    S->ms_groups = S->ms_groups_for(g.nblk);
+    CHM_HIP(hipMemset(S->dcount.p, 0, S->dcount.bytes));
+    // proposal workgroups per slot: about 3 gather items per lane, at most 16
+    S->nsplit = 1 + std::max(1, std::min(16, (2 * g.nstencil + 3 * kPropThreads - 1) / (3 * kPropThreads)));
     CHM_HIP(hipMemcpy(S->ms_sten.p, stencil, 16 * (size_t)g.nstencil, hipMemcpyHostToDevice));
     CHM_HIP(hipMemcpy(S->tms.p, tms, sizeof(double) * (size_t)g.nsp * g.nblk * S->nk, hipMemcpyHostToDevice));
     S->g = g;
     S->msmax = msmax;
     S->has_ms = 1;
+    switch (S->nsp) {
+        case 2: rc = delta_attr<2>(S); break;
+        case 3: rc = delta_attr<3>(S); break;
+        case 4: rc = delta_attr<4>(S); break;
+        case 5: rc = delta_attr<5>(S); break;
+        case 6: rc = delta_attr<6>(S); break;
+    }
+    if (rc) return rc;
     S->lds_prop = sizeof(int) * (size_t)((g.nrow_tot + 3) & ~3) + (size_t)S->natoms_pad;
     CHM_REQUIRE(S->lds_prop <= 160 * 1024, CHM_E_ARG, "sites and MS entries too many for the LDS of the proposal");
     CHM_HIP(hipFuncSetAttribute((const void *)k_ps_propose, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1213,12 +1398,16 @@ extern "C" int chm_psampler_init(chm_psampler *S, const int32_t *species0, const
     k_ps_copy_rows<<<dim3((unsigned)(((long)nr * S->npairs + 255) / 256)), 256, 0, st>>>(
         nr, S->npairs, S->r.as<double>(), S->r.as<double>());
     CHM_LAUNCH_CHECK();
-    // chi^2 of the start
-    if ((rc = chm::gemm_f64(st, nsp, nr, S->ncol_e, S->nk, S->chi.as<double>(), nsp * S->nk, S->nk,
-                            S->Te.as<double>(), S->ncol_e, (long)S->nk * S->ncol_e, S->Rt.as<double>(),
-                            nsp * S->ncol_e, S->ncol_e)))
-        return rc;
-    k_ps_chi2<<<nr, kAccThreads, 0, st>>>(accept_args(S), S->c2.as<double>());
+    // chi^2 of the start: the step's transform and residual sums on chi
+    {
+        const long tot = (long)nr * nsp * S->nk;
+        k_ps_to_kmajor<<<dim3((unsigned)((tot + 255) / 256)), 256, 0, st>>>(nr, nsp, S->nk, S->chi.as<double>(),
+                                                                            S->trialT.as<double>());
+        CHM_LAUNCH_CHECK();
+    }
+    if ((rc = launch_transform(S, st, nr, S->trialT.as<double>(), S->sums.as<double>()))) return rc;
+    k_ps_chi2<<<nr, 64, 0, st>>>(S->sums.as<double>(), nsp, S->nbc, S->q0.as<double>(), S->profile, S->nfit,
+                                 S->c2.as<double>());
     CHM_LAUNCH_CHECK();
     CHM_HIP(hipMemcpyAsync(S->best_c2.p, S->c2.p, sizeof(double) * nr, hipMemcpyDeviceToDevice, st));
     std::vector<int> ident(nr);
@@ -1232,6 +1421,7 @@ extern "C" int chm_psampler_init(chm_psampler *S, const int32_t *species0, const
     CHM_HIP(hipMemsetAsync(S->acc_rung.p, 0, S->acc_rung.bytes, st));
     CHM_HIP(hipMemsetAsync(S->exch_att.p, 0, S->exch_att.bytes, st));
     CHM_HIP(hipMemsetAsync(S->exch_acc.p, 0, S->exch_acc.bytes, st));
+    if (S->has_ms) CHM_HIP(hipMemsetAsync(S->dcount.p, 0, S->dcount.bytes, st));
     CHM_HIP(hipStreamSynchronize(st));   // sp0, p0, part, chi1 go out of scope
     S->rec_base = 0;
     S->hstep = 0;
@@ -1455,20 +1645,22 @@ extern "C" int chm_psampler_chi2(chm_psampler *S, int nconf, const double *chi, 
     CHM_REQUIRE(S && S->inited && chi && out && nconf >= 1, CHM_E_ARG, "bad arguments");
     CHM_HIP(hipSetDevice(S->ctx->device));
     hipStream_t st = S->ctx->stream;
-    const size_t nkk = (size_t)S->nsp * S->nk, ncol = (size_t)S->nsp * S->ncol_e;
-    chm::DevBuf in, R, c2;
+    const size_t nkk = (size_t)S->nsp * S->nk;
+    chm::DevBuf in, inT, sums, c2;
     int rc;
-    if ((rc = in.ensure(sizeof(double) * nconf * nkk)) || (rc = R.ensure(sizeof(double) * nconf * ncol)) ||
-        (rc = c2.ensure(sizeof(double) * nconf)))
+    if ((rc = in.ensure(sizeof(double) * nconf * nkk)) || (rc = inT.ensure(sizeof(double) * nconf * nkk)) ||
+        (rc = sums.ensure(sizeof(double) * nconf * S->nsp * S->nbc * 4)) || (rc = c2.ensure(sizeof(double) * nconf)))
         return rc;
     CHM_HIP(hipMemcpyAsync(in.p, chi, sizeof(double) * nconf * nkk, hipMemcpyHostToDevice, st));
-    if ((rc = chm::gemm_f64(st, S->nsp, nconf, S->ncol_e, S->nk, in.as<double>(), S->nsp * S->nk, S->nk,
-                            S->Te.as<double>(), S->ncol_e, (long)S->nk * S->ncol_e, R.as<double>(),
-                            S->nsp * S->ncol_e, S->ncol_e)))
-        return rc;
-    PsAcceptArgs A = accept_args(S);
-    A.Rt = R.as<double>();
-    k_ps_chi2<<<nconf, kAccThreads, 0, st>>>(A, c2.as<double>());
+    {
+        const long tot = (long)nconf * nkk;
+        k_ps_to_kmajor<<<dim3((unsigned)((tot + 255) / 256)), 256, 0, st>>>(nconf, S->nsp, S->nk, in.as<double>(),
+                                                                            inT.as<double>());
+        CHM_LAUNCH_CHECK();
+    }
+    if ((rc = launch_transform(S, st, nconf, inT.as<double>(), sums.as<double>()))) return rc;
+    k_ps_chi2<<<nconf, 64, 0, st>>>(sums.as<double>(), S->nsp, S->nbc, S->q0.as<double>(), S->profile, S->nfit,
+                                    c2.as<double>());
     CHM_LAUNCH_CHECK();
     CHM_HIP(hipMemcpyAsync(out, c2.p, sizeof(double) * nconf, hipMemcpyDeviceToHost, st));
     CHM_HIP(hipStreamSynchronize(st));
