@@ -243,6 +243,38 @@ def forward_bench(P, S, iters=10):
     return out
 
 
+def pair_leg(P, steps, warmup, exchange_every, seed=1):
+    """The reference sampler's own default move mix (methods/method_a_rmc.py:64-66:
+    swap_frac 0.75, disp_sigma 0.015, max_disp 0.30) on the pair-space engine, same
+    workload and replica count: replica moves/s over `steps` timed steps (swaps and
+    displacements counted alike), the acceptance, and a check that the tracked chi^2 of
+    every replica equals a recompute from its species and positions."""
+    from chimera_amd.method_a_rmc import RMCSampler
+    S = RMCSampler(P.fm, P.ft, P.target, P.mask, P.sigma, nrep=P.sampler.nrep, seed=seed,
+                   ms_library=P.ms)
+    S.init_state(P.cfg0, P.cell.ideal)
+    S.run(warmup, exchange_every=exchange_every, log_every=10 ** 9)
+    P.fm.ctx.sync()
+    a0 = S.accepted_per_rung().sum()
+    t0 = time.perf_counter()
+    S.run(steps, exchange_every=exchange_every, log_every=10 ** 9)
+    P.fm.ctx.sync()
+    dt = time.perf_counter() - t0
+    acc = (S.accepted_per_rung().sum() - a0) / (steps * S.nrep)
+    sp, pos = S.species, S.pos
+    chk = S.chi2_of(sp, pos)
+    err = float(np.max(np.abs(chk - S.chi2) / np.abs(chk)))
+    out = {"engine": S.engine, "swap_frac": S.swap_frac, "disp_sigma": S.disp_sigma,
+           "max_disp": S.max_disp, "steps": steps, "warmup": warmup,
+           "ms_per_step": round(1e3 * dt / steps, 5),
+           "replica_moves_per_s": round(S.nrep * steps / dt, 1),
+           "acceptance": round(float(acc), 4), "max_displacement_A": round(float(
+               np.linalg.norm(pos - P.cell.ideal[None], axis=-1).max()), 4),
+           "chi2_tracked_vs_recomputed_max_rel": float(f"{err:.3g}")}
+    S._destroy()
+    return out
+
+
 def batch_leg(nprob, steps, warmup, exchange_every, precision, reanchor_every=512,
               sequential=False, log=None):
     """The uneven batch (batch_specs(nprob): fcc/bcc supercells of 256-4000 sites, 1-3
@@ -370,6 +402,11 @@ def main():
     ap.add_argument("--no-batch-leg", action="store_true",
                     help="skip the short batch of uneven problems")
     ap.add_argument("--batch-leg-problems", type=int, default=64)
+    ap.add_argument("--no-pair-leg", action="store_true",
+                    help="skip the pair-space engine leg (swap_frac 0.75 with displacements)")
+    ap.add_argument("--pair-leg-steps", type=int, default=2000)
+    ap.add_argument("--pair-only", action="store_true",
+                    help="only the pair-space engine leg (--pair-leg-steps timed after --warmup)")
     ap.add_argument("--batch-leg-steps", type=int, default=2000)
     ap.add_argument("--ess-steps", type=int, default=100000)
     ap.add_argument("--ess-burn", type=int, default=10000)
@@ -406,6 +443,13 @@ def main():
     if args.precision_study:
         from chimera_amd import precision_study
         precision_study.main(["--steps", str(args.precision_study)])
+        return
+    if args.pair_only:
+        from chimera_amd import workloads as W
+        P = W.build(args.workload)
+        print(json.dumps({"workload": args.workload,
+                          "pair_engine_default_moves": pair_leg(P, args.pair_leg_steps, args.warmup,
+                                                                args.exchange_every)}))
         return
     if args.forward_only:
         from chimera_amd import workloads as W
@@ -529,6 +573,10 @@ def main():
         lvu = learned_vs_uniform(P, S, args)
         print(f"[bench] learned/uniform alpha_1 ESS/s: {lvu['alpha1_ess_per_s_ratio']}",
               file=sys.stderr, flush=True)
+    pl = None
+    if not args.no_pair_leg:
+        print("[bench] pair-space engine, the reference's default move mix", file=sys.stderr, flush=True)
+        pl = pair_leg(P, args.pair_leg_steps, 200, args.exchange_every)
     c2 = None
     if not args.no_batch_leg:
         print("[bench] batch of uneven problems (short)", file=sys.stderr, flush=True)
@@ -563,6 +611,7 @@ def main():
         "learned_vs_uniform": lvu,
         "precision_study": c3,
         "batch_uneven": c2,
+        "pair_engine_default_moves": pl,
         "autotune": {"nsplit": tuned["nsplit"],
                      "step_ms": {k: round(v, 5) for k, v in tuned["times_ms"].items()}},
     }
