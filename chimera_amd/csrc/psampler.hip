@@ -1546,7 +1546,8 @@ __global__ void k_ps_gather_rows(int nrep, long row, const int *__restrict__ slo
 // 4 positions (nrep, natoms, 3), 5 best positions, 6 chi (nrep, nsp, nk), 7 r (nrep, npairs),
 // 8 pair types (nrep, npairs) int32, 9 accepted per rung (nrep) int64, 10 exchange attempts
 // and accepts (2, nrep - 1) int64, 11 completed steps (1) int64, 12 move log (log_cap, nrep, 4)
-// int32, 13 logged trial chi^2 (log_cap, nrep), 14 rung_of (nrep) int32.  Rung order except
+// int32, 13 logged trial chi^2 (log_cap, nrep), 14 rung_of (nrep) int32, 15 energy-prior
+// change of each replica's last proposal (nrep).  Rung order except
 // where noted (the log is indexed by rung already; rung_of by slot).
 extern "C" int chm_psampler_read(chm_psampler *S, int what, void *out, int64_t nbytes) {
     const size_t bytes = nbytes < 0 ? 0 : (size_t)nbytes;
@@ -1601,6 +1602,18 @@ extern "C" int chm_psampler_read(chm_psampler *S, int what, void *out, int64_t n
         case 12: return plain(S->log, sizeof(int4) * (size_t)S->log_cap * nr);
         case 13: return plain(S->log_c2, sizeof(double) * (size_t)S->log_cap * nr);
         case 14: return plain(S->rung_of, sizeof(int) * nr);
+        case 15: {
+            // energy-prior change of each slot's last proposal, rung order
+            CHM_REQUIRE(bytes == sizeof(double) * nr, CHM_E_ARG, "output size does not match");
+            std::vector<PsMove> mv(nr);
+            std::vector<int> slot_of(nr);
+            CHM_HIP(hipMemcpyAsync(mv.data(), S->mv.p, sizeof(PsMove) * nr, hipMemcpyDeviceToHost, st));
+            CHM_HIP(hipMemcpyAsync(slot_of.data(), S->slot_of.p, sizeof(int) * nr, hipMemcpyDeviceToHost, st));
+            CHM_HIP(hipStreamSynchronize(st));
+            double *o = reinterpret_cast<double *>(out);
+            for (int q = 0; q < nr; ++q) o[q] = mv[slot_of[q]].de;
+            return CHM_OK;
+        }
     }
     chm::set_error("unknown read code");
     return CHM_E_ARG;

@@ -338,6 +338,14 @@ class PairRMCSampler(RMCSampler):
         super().adapt(window_acc, window_exch)
         self._push_moves()
 
+    def energy_delta(self):
+        """Energy change (eV) of the move each configuration last proposed, under the
+        pair interactions (method_a_rmc.py:310-319), in the current rung order (the value
+        travels with its configuration through exchanges)."""
+        if self._etab is None:
+            raise RuntimeError("no energy_prior was given")
+        return self._read(15, (self.nrep,), np.float64)
+
     def energy(self, species=None):
         """Pair-interaction energy (eV) of every replica's configuration
         (method_a_rmc.py:321-326), or of the given configurations."""

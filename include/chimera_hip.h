@@ -439,7 +439,8 @@ int chm_psampler_set_recording(chm_psampler *s, int log_steps);
  * positions, 6 chi (nrep, nsp, nk), 7 pair distances (nrep, npairs), 8 pair types int32,
  * 9 accepted moves per rung int64, 10 exchange attempts and accepts (2, nrep-1) int64,
  * 11 completed steps (1) int64, 12 move log (log_steps, nrep, 4) int32, 13 logged trial
- * chi^2 (log_steps, nrep), 14 rung of each slot int32.  nbytes must match. */
+ * chi^2 (log_steps, nrep), 14 rung of each slot int32, 15 energy-prior change of each
+ * replica's last proposal (eV; method_a_rmc.py:310-319).  nbytes must match. */
 int chm_psampler_read(chm_psampler *s, int what, void *host, int64_t nbytes);
 /* chi^2 of nconf unnormalised chi (nconf, nsp, nk) with the sampler's target, window and
  * profiling (method_a_rmc.py:211-227, best_verified :439-448). */

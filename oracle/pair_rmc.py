@@ -273,6 +273,7 @@ class PairSampler:
             x = self.beta[c] * (c2 - self.chi2[c]) * self.nfit
             if self.etab is not None:
                 x = x + self.ebeta * self.energy_delta(c, aff, t_new)
+            de = self.energy_delta(c, aff, t_new) if self.etab is not None else 0.0
             u = accept_uniform(self.seed, self.stepc, c)
             acc = bool((x <= 0) or (u < math.exp(-min(x, 60.0))))
             if acc:
@@ -284,7 +285,7 @@ class PairSampler:
                 self.chi[c] = trial
                 self.chi2[c] = c2
             acc_all[c] = acc
-            rec.append((a, b, sw, int(acc), c2, x, u, newp))
+            rec.append((a, b, sw, int(acc), c2, x, u, newp, de))
         imp = self.chi2 < self.best_chi2
         self.best_chi2 = np.where(imp, self.chi2, self.best_chi2)
         self.best_sp[imp] = self.species[imp]

@@ -53,7 +53,7 @@ def _compare(P, O, S, ref, nsteps, exch, start_pos=None, tie_tol=1e-6):
     kinds = set()
     for t in range(nsteps):
         for rung in range(S.nrep):
-            a, b, sw, acc, c2, x, u, newp = ref.log[t][rung]
+            a, b, sw, acc, c2, x, u = ref.log[t][rung][:7]
             g = log[t, rung]
             assert (int(g[0]), int(g[1]), int(g[2]) & 1) == (a, b, sw), f"step {t} rung {rung}"
             np.testing.assert_allclose(lc2[t, rung], c2, rtol=1e-9, err_msg=f"step {t} rung {rung}")
@@ -164,3 +164,32 @@ def test_adapt_retunes_displacement(tern108):
     assert S.disp_sigma > 0.001            # a tiny step is almost always accepted: it grows
     want = S.chi_k_of(S.species, S.pos)
     np.testing.assert_allclose(S.chi, want, rtol=0, atol=1e-10 * np.abs(want).max())
+
+
+def test_trajectory_binary_unprofiled_prior():
+    """Two species without multiple scattering, the scale not profiled
+    (profile_scale=False), the energy prior, swap_frac 0.4, 500 sites: move by move
+    against the oracle; energy_delta() holds the last proposals' energy change."""
+    from oracle.build import problem_from
+    P = _problem("pb500", (5, 5, 5), ("Fe", "Ni"), False, nrep=4)
+    O = problem_from(P)
+    V = np.array([[[0.0, -0.025], [-0.025, 0.0]], [[0.0, 0.01], [0.01, 0.0]]])
+    S = _sampler(P, nrep=4, swap_frac=0.4, disp_sigma=0.02, seed=11, profile_scale=False,
+                 energy_prior=(V, 700.0), ms_library=None)
+    ref = PR.PairSampler(O, nrep=4, seed=11, swap_frac=0.4, disp_sigma=0.02, profile_scale=False,
+                         energy_prior=(V, 700.0))
+    kinds = _compare(P, O, S, ref, 35, 10)                   # no exchange after the last step
+    if kinds is not None:
+        want = [r[8] for r in ref.log[-1]]                    # rung order
+        np.testing.assert_allclose(S.energy_delta(), want, rtol=1e-12, atol=1e-15)
+
+
+def test_trajectory_four_species():
+    """Four species (the NSP = 4 kernels), complex target, no MS, default move mix."""
+    from oracle.build import problem_from
+    P = _problem("pq108", (3, 3, 3), ("Cr", "Mn", "Fe", "Ni"), False, nrep=4)
+    O = problem_from(P)
+    S = _sampler(P, nrep=4, seed=3, ms_library=None)
+    assert S.engine == "pair" and S.swap_frac == 0.75
+    ref = PR.PairSampler(O, nrep=4, seed=3)
+    _compare(P, O, S, ref, 30, 10)
