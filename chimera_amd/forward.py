@@ -85,6 +85,48 @@ class ForwardModel:
         self._sig2 = f64(v).reshape(self.npairs)
         check(_lib.load().chm_fm_set_sig2(self.ctx.h, self._sig2))
 
+    # The reference's other device attributes (forward.py:70-92), which its callers read:
+    # NumPy copies of what the library holds (shapes and flattening as the reference's).
+    @property
+    def d_i(self):
+        return self.i_idx
+
+    @property
+    def d_j(self):
+        return self.j_idx
+
+    @property
+    def d_shell(self):
+        return self.shell
+
+    @property
+    def d_k(self):
+        return self.k
+
+    @property
+    def d_f(self):
+        return self.f
+
+    @property
+    def d_delta(self):
+        return np.ascontiguousarray(self.delta.reshape(self.nsp * self.nsp, -1))
+
+    @property
+    def d_lam(self):
+        return self.lam
+
+    @property
+    def d_dr(self):
+        return self.dr.reshape(-1)
+
+    @property
+    def d_c3(self):
+        return self.c3
+
+    @property
+    def d_s2tab(self):
+        return self.s2types.reshape(-1)
+
     def _upload_cumulants(self):
         check(_lib.load().chm_fm_set_cumulants(self.ctx.h, self.nshell, f64(self.dr.reshape(-1)),
                                                f64(self.c3), f64(self.s2types.reshape(-1))))
@@ -164,6 +206,21 @@ class ForwardModel:
             cnt = np.stack([(species == s).sum(axis=1) for s in range(self.nsp)], axis=1)
             chi /= np.maximum(cnt, 1.0)[:, :, None]
         return chi
+
+    def chi_reference(self, pos, species, sigma2=None):
+        """Normalised chi (nconf, nsp, nk) with an optional per-pair sigma^2 override
+        (forward.py:265-293, the reference's validation form: pair terms by absorber,
+        divided by the absorber's atom count).  Evaluated by the device FP64 path sum,
+        the kernel the tests validate against the NumPy oracle; the override is uploaded
+        for the call and the model's own sigma^2 restored after it."""
+        if sigma2 is None:
+            return self.chi(pos, species, normalise=True)
+        keep = self._sig2.copy()
+        self.d_sig2 = np.broadcast_to(np.asarray(sigma2, np.float64), (self.npairs,))
+        try:
+            return self.chi(pos, species, normalise=True)
+        finally:
+            self.d_sig2 = keep
 
     def tab_rows(self, r, s2, F, D, L, amp=1.0):
         """Device evaluation of term rows from tabulated F, D, L (nrows, nk) on this

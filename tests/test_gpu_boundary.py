@@ -134,3 +134,28 @@ def test_in_place_device_attribute_updates(tern):
     finally:
         fm.d_shift = shift0
         fm.d_sig2 = sig0
+
+
+def test_chi_reference_and_device_attributes(tern):
+    """ForwardModel.chi_reference (exafs_gpu/forward.py:265-293: normalised by absorber atom
+    count, optional sigma^2 override) equals the oracle's normalised pair sum at both sigma^2
+    values and leaves the model's own sigma^2 in place; the reference's device attribute
+    names read back the model's arrays in the reference's shapes."""
+    P = tern
+    fm = P.fm
+    rng = np.random.default_rng(4)
+    pos = P.cell.ideal + rng.normal(0, 0.03, P.cell.ideal.shape)
+    sp = P.cfg0
+    sig_alt = np.full(fm.npairs, 0.009)
+    for s2 in (None, sig_alt):
+        got = fm.chi_reference(pos[None], sp[None], sigma2=s2)[0]
+        want = OF.chi_pairs(P.k, P.tables.f, P.tables.delta, P.tables.lam, P.tables.s02, pos, sp,
+                            P.i, P.j, P.shift, P.sig2 if s2 is None else s2, normalise=True)
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    assert np.array_equal(fm.d_sig2, P.sig2)
+    n = fm.nsp
+    assert np.array_equal(fm.d_i, P.i) and np.array_equal(fm.d_j, P.j)
+    assert np.array_equal(fm.d_shell, P.shell) and np.array_equal(fm.d_k, P.k)
+    assert fm.d_delta.shape == (n * n, fm.nk) and fm.d_f.shape == (n, fm.nk)
+    assert fm.d_dr.shape == (fm.nshell * n * n,) and fm.d_s2tab.shape == (fm.nshell * n * n,)
+    assert fm.d_c3.shape == (fm.nshell,) and fm.d_lam.shape == (fm.nk,)
