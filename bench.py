@@ -163,7 +163,7 @@ def learned_vs_uniform(P, S, args):
     corr = float(np.corrcoef(g, yy)[0, 1])
     lp.record(0)
     settings = dict(ncand=args.lp_ncand, lam=1.0, kappa=args.lp_kappa, eps=0.05,
-                    beta_max=args.lp_beta_max)
+                    beta_max=args.lp_beta_max, lift=not args.lp_symmetric)
     print(f"[bench] learned: trained {args.train_iters} its, loss {loss2:.3f}, corr {corr:.3f}; "
           "ESS runs", file=sys.stderr, flush=True)
     uni = ess_run(P, S, args.ess_burn, args.ess_steps, args.exchange_every, lambda: lp.disable())
@@ -171,8 +171,10 @@ def learned_vs_uniform(P, S, args):
                   lambda: lp.enable(**settings))
     prof = lp.profile(50)
     lp.disable()
-    # MFMA work of one scorer launch: every slot's candidates through both layers
-    flops = 2.0 * S.nrep * args.lp_ncand * (lp.kp * 256 + 256 * 256 + 256)
+    # MFMA work of one scoring launch (every slot learned): layer 2 and the output of each
+    # slot's candidates at the committed state (layer 1 is a sum of two per-site rows; the
+    # re-scored rows near the pending swap are not counted)
+    flops = 2.0 * S.nrep * args.lp_ncand * (256 * 256 + 256)
     return {"settings": settings,
             "training": {"records": int(nv), "record_steps": args.train_steps,
                          "record_s": round(t_rec, 2), "iters": args.train_iters, "batch": 1024,
@@ -413,7 +415,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=40000)
     ap.add_argument("--train-iters", type=int, default=8000)
     ap.add_argument("--lp-ncand", type=int, default=256)
-    ap.add_argument("--lp-kappa", type=float, default=1.0)
+    ap.add_argument("--lp-kappa", type=float, default=0.5)
+    ap.add_argument("--lp-symmetric", action="store_true",
+                    help="symmetric tilt kappa |dU| instead of the lifted kappa d dU")
     ap.add_argument("--lp-beta-max", type=float, default=3.0)
     ap.add_argument("--nsplit", type=int, default=0,
                     help="workgroups per replica in the slot kernel (0: autotune)")

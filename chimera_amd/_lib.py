@@ -92,6 +92,7 @@ _SIGS = {
     "chm_lp_get_params": (C.c_int, [vp, vp]),
     "chm_lp_configure": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_int]),
+    "chm_lp_set_tilt_weights": (C.c_int, [vp, vp, C.c_int]),
     "chm_lp_set_records": (C.c_int, [vp, C.c_int]),
     "chm_lp_read_records": (C.c_int, [vp, C.POINTER(C.c_int64), vp, vp, vp, C.c_int64]),
     "chm_lp_eval": (C.c_int, [vp, C.c_int, vp, vp]),

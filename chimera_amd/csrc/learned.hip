@@ -1,83 +1,29 @@
-// Learned swap proposal for the device sampler (gfx950).  No counterpart in the
-// reference: its swap proposal is uniform (exafs_gpu/kernels.py:362-398).
+// Learned swap proposal for the device sampler (gfx950): the per-site layer-1 sums, the
+// scoring kernel of a segment's first proposal, the network evaluation for tests, the
+// trainer and the C ABI.  No counterpart in the reference: its swap proposal is uniform
+// (exafs_gpu/kernels.py:362-398).  The network, the per-site sums and the speculative
+// scoring that runs inside the transform launches are described in learned_dev.h.
 //
-// Per replica and step, `ncand` candidate pairs (a, b) are drawn from a counter-based
-// stream that does not depend on the state.  Each is scored by an MLP over the one-hot
-// species occupancy of a's and b's stencil neighbours and of a and b themselves
-// (nocc = 2 ncoord + 2 entries, nsp classes each):
-//     Z1 = b1 + X W1,  H1 = bf16(silu(Z1)),  Z2 = b2 + H1 W2,  H2 = silu(Z2),
-//     g = b3 + H2 . w3          (W1, W2, w3 in bf16, every sum in FP32)
-// on v_mfma_f32_16x16x32_bf16, 32 candidates per workgroup.  g predicts the energy change
-// nfit (chi2(x') - chi2(x)) of the swap; logit = -lam beta g / 2 + kappa |dU| (dU: change of
-// unlike first-shell bonds), -inf when a and b hold the same species.  One candidate is
-// drawn from the softmax; its reverse probability comes from the same candidate set
-// scored at x' (only candidates whose occupancy the swap changes are re-scored: those
-// within the stencil radius of a or b), and the acceptance uses
-//     x = beta dE - log(q(x'->x) / q(x->x')),
-// with q summing every candidate that names the same unordered pair.  With probability
-// eps (a state-independent coin) a step uses the uniform proposal instead.
-//
-// Training: the sampler writes a record (occupancy, dE, rung, accepted) of every swap
-// it proposes; `chm_lp_train` runs Adam on minibatches of the ring with an FP32 forward
-// and backward pass (same bf16 weights and H1 rounding as the scorer).
+// Training: the sampler writes a record (occupancy, dE, rung, accepted) of every swap it
+// proposes; `chm_lp_train` runs Adam on minibatches of the ring with an FP32 forward and
+// backward pass (bf16 W1 and W2 with the same H1 rounding as the scorer).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
 #include "../../include/chimera_hip.h"
 #include "common.h"
+#include "learned_dev.h"
 #include "sampler_dev.h"
 #include "sampler_state.h"
 
 namespace {
 
 using namespace chm_smp;
-
-constexpr uint64_t kSelSalt = 0x14057B7EF767814Full;
-constexpr uint64_t kCandMul = 0xD6E8FEB86659FD93ull;
-constexpr int HID = 256;      // hidden width
-constexpr int MR = 32;        // candidate rows per MLP tile
-constexpr int MAXC = 256;     // candidates per replica and step, at most
-constexpr int LDH = HID + 8;  // H1 row stride in LDS (bf16), staggers the banks
-
-struct LPW {
-    const __bf16 *w1t;   // (HID, kp): row n = hidden unit
-    const __bf16 *w2t;   // (HID, HID): row n = output unit
-    const float *b1, *b2, *w3q, *b3;
-    int kp;
-};
-
-struct LPArgs {
-    Geo g;
-    const uint8_t *species;
-    const int *rung_of;
-    const double *beta_slot;
-    const uint64_t *stepc;
-    const int *ss_sten;
-    const int *nbr;      // (natoms, ncoord) neighbour sites in stencil order
-    uint64_t seed;
-    int ncand, nocc, noccp, nfeat, kp, r2max;
-    double eps, lam, kappa, beta_max;
-    int lift;            // lifted proposal: tilt kappa d dU with d = dir[slot]
-    const int *dir;      // (nrep) +-1
-    LPW W;
-    float *logit;        // (nrep, ncand)
-    float *gcand;        // (nrep, ncand) network output of each candidate (lifted proposal)
-    int *ducand;         // (nrep, ncand) its first-shell change dU
-    int *prop_next;      // (nrep, kPropInts)
-    uint8_t *pend_occ;   // (nrep, noccp)
-};
-
-
-__device__ __forceinline__ void candidate(uint64_t h0, int i, int natoms, int &a, int &b) {
-    const uint64_t h = splitmix64(h0 ^ ((uint64_t)(i + 1) * kCandMul));
-    a = (int)(((h & 0xFFFFFFFFull) * (uint64_t)natoms) >> 32);
-    b = (int)(((h >> 32) * (uint64_t)natoms) >> 32);
-}
+using namespace chm_lp_dev;
 
 __device__ __forceinline__ float silu(float z) { return z / (1.f + expf(-z)); }
-// the scorer's SiLU: hardware exp2 and reciprocal (a few ulp; the trainer keeps silu)
-__device__ __forceinline__ float silu_fast(float z) { return z * __frcp_rn(1.f + __expf(-z)); }
 __device__ __forceinline__ float dsilu(float z) {
     const float s = 1.f / (1.f + expf(-z));
     return s * (1.f + z * (1.f - s));
@@ -85,547 +31,61 @@ __device__ __forceinline__ float dsilu(float z) {
 __device__ __forceinline__ float bf16_round(float f) { return bf16_to_float(bf16_rne(f)); }
 __device__ __forceinline__ float bf(__bf16 v) { return bf16_to_float(__builtin_bit_cast(unsigned short, v)); }
 
-__device__ __forceinline__ int centred(int d, int n) {
-    d = (d < 0) ? d + n : d;
-    return (2 * d > n) ? d - n : d;
-}
-__device__ __forceinline__ int dist2(const Geo &g, const int *p, const int *q) {
-    const int dx = centred(p[0] - q[0], g.box[0]);
-    const int dy = centred(p[1] - q[1], g.box[1]);
-    const int dz = centred(p[2] - q[2], g.box[2]);
-    return dx * dx + dy * dy + dz * dz;
-}
-
-// the uniform proposal on this step: a state-independent coin, and every slot colder
-// than beta_max (there the network's error would be multiplied by a large beta)
-__device__ __forceinline__ bool lp_uniform(const LPArgs &A, uint64_t h0, double beta) {
-    return lp_coin_uniform(h0, beta, A.eps, A.beta_max);
-}
-
-// the first-shell tilt of a candidate's logit: kappa |dU| (symmetric), or kappa d dU for a
-// lifted proposal moving in direction d = +-1 (the reverse move is scored with -d)
-__device__ __forceinline__ double tilt(const LPArgs &A, int du, int d) {
-    return A.lift ? A.kappa * (double)(d * du) : A.kappa * fabs((double)du);
-}
-
-// site of occupancy entry q of candidate (a, b): the stencil neighbours of a, of b, then
-// a and b themselves
-__device__ __forceinline__ int occ_site_tbl(const LPArgs &A, int a, int b, int q) {
-    const int nc = A.g.ncoord;
-    if (q < nc) return A.nbr[(long)a * nc + q];
-    if (q < 2 * nc) return A.nbr[(long)b * nc + q - nc];
-    return q == 2 * nc ? a : b;
-}
-
-// Occupancy rows of nrows candidates (rows[r] = (a, b)) with sites xa, xb exchanged
-// virtually (xa < 0: as stored) into occ (MR x noccp), the one-hot features into X
-// (MR x ldx bf16, zero beyond nfeat and beyond nrows) and dU into du.  shell0[q] marks
-// the first-shell stencil entries.  The neighbour-table loads of a lane are all issued
-// before any is used.  Ends with the workgroup synchronised.
-constexpr int IPT = 8;   // occupancy entries per lane (MR * nocc <= IPT * blockDim.x)
-
-// idx / d for 0 <= idx < 2^20, 0 < d < 2^12 (a float reciprocal; the quotient is exact)
-__device__ __forceinline__ int small_div(int idx, int d, float inv_d) {
-    int r = (int)(((float)idx + 0.5f) * inv_d);
-    return r;
-}
-
-// the neighbour-table loads of this lane's occupancy entries (issued, not waited for)
-__device__ __forceinline__ void row_sites(const LPArgs &A, const int2 *rows, int nrows, int (&site)[IPT]) {
-    const int n = nrows * A.nocc;
-#pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        const int idx = k * blockDim.x + threadIdx.x;
-        site[k] = -1;
-        if (idx < n) {
-            const int r = small_div(idx, A.nocc, 1.f / (float)A.nocc), q = idx - r * A.nocc;
-            site[k] = occ_site_tbl(A, rows[r].x, rows[r].y, q);
-        }
-    }
-}
-
-// Occupancy rows of nrows candidates (rows[r] = (a, b), sites from row_sites) with sites
-// xa, xb exchanged virtually (xa < 0: as stored) into occ (MR x noccp), the one-hot
-// features into X (MR x ldx bf16, zero beyond nfeat and beyond nrows) and dU into du
-// (zeroed by the caller).  shell0[q] marks the first-shell stencil entries.  Ends with
-// the workgroup synchronised.
-__device__ void build_rows(const LPArgs &A, const uint8_t *lsp, const uint8_t *shell0, const int2 *rows,
-                           int nrows, const int (&site)[IPT], int xa, int xb, uint8_t *occ, __bf16 *X,
-                           int ldx, int *du) {
+// ------------------------------------------------------------------ per-site sums
+// grid (natoms, nrep), HID threads: A and B rows of every site of every slot from scratch,
+// into both copies
+__global__ __launch_bounds__(HID) void k_lp_cache_build(LPArgs A, int *cache0, int *cache1) {
+    __shared__ int occ[256 + 1];
     const Geo &g = A.g;
-    const int t = threadIdx.x;
-    const int n = nrows * A.nocc;
-#pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        const int idx = k * blockDim.x + t;
-        if (idx >= n) continue;
-        const int r = small_div(idx, A.nocc, 1.f / (float)A.nocc), q = idx - r * A.nocc;
-        const int a = rows[r].x, b = rows[r].y;
-        const int sp = species_swapped(lsp, site[k], xa, xb);
-        occ[r * A.noccp + q] = (uint8_t)sp;
-        if (q < 2 * g.ncoord) {
-            const bool on_a = q < g.ncoord;
-            if (shell0[on_a ? q : q - g.ncoord]) {
-                const int sa = species_swapped(lsp, a, xa, xb), sb = species_swapped(lsp, b, xa, xb);
-                int cnt = 0;
-                if (on_a && site[k] != b) cnt = (sp == sa) - (sp == sb);
-                if (!on_a && site[k] != a) cnt = (sp == sb) - (sp == sa);
-                if (cnt) atomicAdd(&du[r], cnt);
-            }
-        }
-    }
+    const int c = blockIdx.x, s = blockIdx.y, h = threadIdx.x;
+    const uint8_t *sp = A.species + (long)s * g.natoms_pad;
+    for (int q = h; q < g.ncoord; q += blockDim.x) occ[q] = sp[A.nbr[(long)c * g.ncoord + q]];
+    if (h == 0) occ[g.ncoord] = sp[c];
     __syncthreads();
-    const int nchunk = A.kp / 8;
-    for (int idx = t; idx < MR * nchunk; idx += blockDim.x) {
-        const int r = idx / nchunk, c = idx - r * nchunk;
-        bf16x8 v;
-        int q = (8 * c) / g.nsp, s = (8 * c) - q * g.nsp;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = 8 * c + i;
-            const bool on = r < nrows && k < A.nfeat && occ[r * A.noccp + q] == s;
-            v[i] = __builtin_bit_cast(__bf16, (unsigned short)(on ? 0x3F80 : 0));
-            if (++s == g.nsp) {
-                s = 0;
-                ++q;
-            }
-        }
-        *reinterpret_cast<bf16x8 *>(X + r * ldx + 8 * c) = v;
-    }
-    __syncthreads();
-}
-
-// first-shell flags of the stencil entries into LDS (at most 256 entries)
-__device__ __forceinline__ void load_shell0(const LPArgs &A, uint8_t *shell0) {
-    for (int q = threadIdx.x; q < A.g.ncoord; q += blockDim.x) shell0[q] = ((A.ss_sten[q] >> 24) & 0xFF) == 0;
-}
-
-// This lane's share of the network: 8 waves x 32 hidden units (two 16-column tiles).  The
-// weight fragments are loaded once, into registers, at the start of a kernel, so their
-// latency overlaps the occupancy gathers.
-constexpr int NT = 512;    // threads per workgroup of the scorer and the selection
-constexpr int KT1 = 12;    // layer-1 K tiles held in registers (kp <= 384 in one pass)
-struct Frag {
-    bf16x8 w1[KT1][2], w2[HID / 32][2];
-    float b1[2], b2[2], w3[2];
-};
-
-__device__ __forceinline__ void load_frag(const LPW &W, Frag &F) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int lr = lane & 15, lk = 8 * (lane >> 4);
-    const int nk1 = W.kp / 32;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const int n = 32 * wave + 16 * c + lr;
-        const __bf16 *w1 = W.w1t + (long)n * W.kp + lk;
-        const __bf16 *w2 = W.w2t + (long)n * HID + lk;
-#pragma unroll
-        for (int kt = 0; kt < KT1; ++kt)
-            if (kt < nk1) F.w1[kt][c] = *reinterpret_cast<const bf16x8 *>(w1 + 32 * kt);
-#pragma unroll
-        for (int kt = 0; kt < HID / 32; ++kt) F.w2[kt][c] = *reinterpret_cast<const bf16x8 *>(w2 + 32 * kt);
-        F.b1[c] = W.b1[n];
-        F.b2[c] = W.b2[n];
-        F.w3[c] = W.w3q[n];
+    for (int role = 0; role < 2; ++role) {
+        int acc = A.w1q[(long)feat_own(g, role, occ[g.ncoord]) * HID + h];
+        for (int q = 0; q < g.ncoord; ++q) acc += A.w1q[(long)feat_nbr(g, role, q, occ[q]) * HID + h];
+        const long o = (((long)s * 2 + role) * g.natoms + c) * HID + h;
+        cache0[o] = acc;
+        cache1[o] = acc;
     }
 }
 
-// g of the MR rows whose features are in X (LDS); H (MR x LDH bf16) and gp (8 x MR) are LDS
-// scratch; result in gout (LDS).  Every thread of the (NT-thread) workgroup calls it.
-// With F the weight fragments come from registers (loaded earlier by load_frag);
-// without, each K tile's fragments are loaded one tile ahead (fewer registers).
-template <bool kHeld>
-__device__ void mlp_rows(const LPW &W, const Frag *F, const __bf16 *X, int ldx, __bf16 *H, float *gp,
-                         float *gout) {
-    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const int lr = lane & 15, lk = 8 * (lane >> 4);
-    const int n0 = 32 * wave + lr, n1 = n0 + 16;
-    const __bf16 *w1a = W.w1t + (long)n0 * W.kp + lk, *w1b = W.w1t + (long)n1 * W.kp + lk;
-    const __bf16 *w2a = W.w2t + (long)n0 * HID + lk, *w2b = W.w2t + (long)n1 * HID + lk;
-    float b1[2], b2[2], w3[2];
-    if (kHeld) {
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            b1[c] = F->b1[c];
-            b2[c] = F->b2[c];
-            w3[c] = F->w3[c];
-        }
-    } else {
-        b1[0] = W.b1[n0];
-        b1[1] = W.b1[n1];
-        b2[0] = W.b2[n0];
-        b2[1] = W.b2[n1];
-        w3[0] = W.w3q[n0];
-        w3[1] = W.w3q[n1];
-    }
-    f32x4 acc[2][2];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 2; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int nk1 = W.kp / 32;
-    auto step = [&](const bf16x8 &b0, const bf16x8 &b1v, int kt) {
-        const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(X + lr * ldx + 32 * kt + lk);
-        const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(X + (16 + lr) * ldx + 32 * kt + lk);
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1v, acc[0][1], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1v, acc[1][1], 0, 0, 0);
-    };
-    bf16x8 nb0, nb1;
-    if (kHeld) {
-#pragma unroll
-        for (int kt = 0; kt < KT1; ++kt)
-            if (kt < nk1) step(F->w1[kt][0], F->w1[kt][1], kt);
-        for (int kt = KT1; kt < nk1; ++kt)   // features beyond the register-held tiles
-            step(*reinterpret_cast<const bf16x8 *>(w1a + 32 * kt),
-                 *reinterpret_cast<const bf16x8 *>(w1b + 32 * kt), kt);
-    } else {
-        nb0 = *reinterpret_cast<const bf16x8 *>(w1a);
-        nb1 = *reinterpret_cast<const bf16x8 *>(w1b);
-        for (int kt = 0; kt < nk1; ++kt) {
-            const bf16x8 b0 = nb0, b1v = nb1;
-            if (kt + 1 < nk1) {
-                nb0 = *reinterpret_cast<const bf16x8 *>(w1a + 32 * (kt + 1));
-                nb1 = *reinterpret_cast<const bf16x8 *>(w1b + 32 * (kt + 1));
-            }
-            step(b0, b1v, kt);
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const int n = c ? n1 : n0;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int row = 16 * r + 4 * (lane >> 4) + i;
-                H[row * LDH + n] = __builtin_bit_cast(__bf16, bf16_rne(silu_fast(acc[r][c][i] + b1[c])));
-            }
-            acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
-    __syncthreads();
-    if (!kHeld) {
-        nb0 = *reinterpret_cast<const bf16x8 *>(w2a);
-        nb1 = *reinterpret_cast<const bf16x8 *>(w2b);
-    }
-#pragma unroll
-    for (int kt = 0; kt < HID / 32; ++kt) {
-        bf16x8 b0, b1v;
-        if (kHeld) {
-            b0 = F->w2[kt][0];
-            b1v = F->w2[kt][1];
-        } else {
-            b0 = nb0;
-            b1v = nb1;
-            if (kt + 1 < HID / 32) {
-                nb0 = *reinterpret_cast<const bf16x8 *>(w2a + 32 * (kt + 1));
-                nb1 = *reinterpret_cast<const bf16x8 *>(w2b + 32 * (kt + 1));
-            }
-        }
-        const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(H + lr * LDH + 32 * kt + lk);
-        const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(H + (16 + lr) * LDH + 32 * kt + lk);
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1v, acc[0][1], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1v, acc[1][1], 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            float v = silu_fast(acc[r][0][i] + b2[0]) * w3[0] + silu_fast(acc[r][1][i] + b2[1]) * w3[1];
-#pragma unroll
-            for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, 16);
-            if (lr == 0) gp[wave * MR + 16 * r + 4 * (lane >> 4) + i] = v;
-        }
-    __syncthreads();
-    if (t < MR) {
-        float v = 0.f;
-#pragma unroll
-        for (int w = 0; w < NT / 64; ++w) v += gp[w * MR + t];
-        gout[t] = W.b3[0] + v;
-    }
-    __syncthreads();
-}
-
-struct RowLds {
-    uint8_t *lsp, *occ;
-    __bf16 *X, *H;
-    int ldx;
-};
-
-__device__ RowLds carve(const LPArgs &A, char *base) {
-    RowLds L;
-    char *p = base;
-    L.lsp = reinterpret_cast<uint8_t *>(p);
-    p += (A.g.natoms_pad + 15) & ~15;
-    L.ldx = A.kp + 8;
-    L.X = reinterpret_cast<__bf16 *>(p);
-    p += (size_t)MR * L.ldx * 2;
-    L.H = reinterpret_cast<__bf16 *>(p);
-    p += (size_t)MR * LDH * 2;
-    L.occ = reinterpret_cast<uint8_t *>(p);
-    return L;
-}
-
-// ------------------------------------------------------------------ scoring
-// grid (ncand / MR, nrep), NT threads: logits of the candidates of step stepc + 1 at the
-// current state
-__global__ __launch_bounds__(NT) void k_lp_score(LPArgs A) {
+// ------------------------------------------------------------------ a segment's first scores
+// scoring workgroups only (mode 0): step n+1's candidates at the configuration after n steps
+__global__ __launch_bounds__(TB) void k_lp_tiles(LPArgs A) {
     extern __shared__ __align__(16) char lds_raw[];
-    __shared__ int2 rows[MR];
-    __shared__ int du[MR];
-    __shared__ float gp[(NT / 64) * MR], gout[MR];
-    __shared__ uint8_t shell0[256];
-    const Geo &g = A.g;
-    const int s = blockIdx.y, t = threadIdx.x;
-    const uint64_t n1 = A.stepc[s] + 1;
-    const int rung = A.rung_of[s];
-    const double beta = A.beta_slot[s];
-    const uint64_t h0 = lp_stream(A.seed, n1, rung);
-    if (lp_uniform(A, h0, beta)) return;   // a uniform step: nothing to score
-    RowLds L = carve(A, lds_raw);
-    load_species(L.lsp, A.species + (long)s * g.natoms_pad, g.natoms_pad);
-    load_shell0(A, shell0);
-    const int i0 = blockIdx.x * MR;
-    if (t < MR) {
-        int a, b;
-        candidate(h0, i0 + t, g.natoms, a, b);
-        rows[t] = make_int2(a, b);
-        du[t] = 0;
-    }
-    __syncthreads();
-    // neighbour loads first, then the weights: the occupancy work waits only for the former
-    int site[IPT];
-    row_sites(A, rows, MR, site);
-    Frag F;
-    load_frag(A.W, F);
-    build_rows(A, L.lsp, shell0, rows, MR, site, -1, -1, L.occ, L.X, L.ldx, du);
-    mlp_rows<true>(A.W, &F, L.X, L.ldx, L.H, gp, gout);
-    if (t < MR) {
-        const bool valid = L.lsp[rows[t].x] != L.lsp[rows[t].y];
-        const float lg = (float)(-A.lam * beta * 0.5 * (double)gout[t] + tilt(A, du[t], A.lift ? A.dir[s] : 1));
-        A.logit[(long)s * A.ncand + i0 + t] = valid ? lg : -__builtin_huge_valf();
-        if (A.lift) {
-            A.gcand[(long)s * A.ncand + i0 + t] = gout[t];
-            A.ducand[(long)s * A.ncand + i0 + t] = du[t];
-        }
-    }
-}
-
-// ------------------------------------------------------------------ selection
-constexpr int NW = NT / 64;
-
-__device__ __forceinline__ float block_max_f(float v, float *red) {
-    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    float r = red[0];
-    for (int w = 1; w < NW; ++w) r = fmaxf(r, red[w]);
-    __syncthreads();
-    return r;
-}
-
-__device__ __forceinline__ double block_sum_d(double v, double *red) {
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double r = red[0];
-    for (int w = 1; w < NW; ++w) r += red[w];
-    __syncthreads();
-    return r;
-}
-
-// inclusive scan over the workgroup's lanes (index order); returns (exclusive, inclusive)
-__device__ __forceinline__ void block_scan_d(double v, double *red, double &excl, double &incl) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double x = v;
-    for (int m = 1; m < 64; m <<= 1) {
-        const double y = __shfl_up(x, m, 64);
-        if (lane >= m) x += y;
-    }
-    __syncthreads();
-    if (lane == 63) red[wave] = x;
-    __syncthreads();
-    double off = 0.0;
-    for (int w = 0; w < wave; ++w) off += red[w];
-    incl = off + x;
-    const double prev = __shfl_up(x, 1, 64);
-    excl = off + (lane == 0 ? 0.0 : prev);
-    __syncthreads();
-}
-
-// grid (1, nrep), NT threads: the proposal of step stepc + 1 (learned, or uniform on the
-// coin / above beta_max), its log q ratio and its occupancy record; the slot kernel then
-// gathers its count changes
-__global__ __launch_bounds__(NT) void k_lp_select(LPArgs A) {
-    extern __shared__ __align__(16) char lds_raw[];
-    __shared__ int2 rows[MR];
-    __shared__ int du[MR];
-    __shared__ float gp[NW * MR], gout[MR];
-    __shared__ float lrev[MAXC];
-    __shared__ int alist[MAXC];
-    __shared__ int nalist, jsel;
-    __shared__ double redd[NW];
-    __shared__ float redf[NW];
-    __shared__ uint8_t shell0[256];
-    __shared__ Prop P;
-    const Geo &g = A.g;
-    const int s = blockIdx.y, t = threadIdx.x;
-    RowLds L = carve(A, lds_raw);
-    const uint64_t n1 = A.stepc[s] + 1;
-    const int rung = A.rung_of[s];
-    const double beta = A.beta_slot[s];
-    const uint64_t h0 = lp_stream(A.seed, n1, rung);
-    const bool uniform = lp_uniform(A, h0, beta);
-    const float lg = (!uniform && t < A.ncand) ? A.logit[(long)s * A.ncand + t] : -__builtin_huge_valf();
-    load_species(L.lsp, A.species + (long)s * g.natoms_pad, g.natoms_pad);
-    load_shell0(A, shell0);
-    if (t == 0) jsel = -1;
-    __syncthreads();
-    float logqr = 0.f;
-    if (uniform) {
-        if (t == 0) P = propose(L.lsp, g.natoms, A.seed, n1, (uint64_t)rung);
-        __syncthreads();
-    } else {
-        const float m = block_max_f(lg, redf);
-        const bool any = m > -__builtin_huge_valf();
-        const double p = (any && lg > -__builtin_huge_valf()) ? exp((double)(lg - m)) : 0.0;
-        double excl, incl;
-        block_scan_d(p, redd, excl, incl);
-        const double Z = block_sum_d(p, redd);
-        const double target = u01(splitmix64(h0 ^ kSelSalt)) * Z;
-        if (any && p > 0.0 && excl <= target && target < incl) jsel = t;
-        __syncthreads();
-        if (any && jsel < 0) {   // target rounded past the last bin: the last candidate with p > 0
-            if (p > 0.0) atomicMax(&jsel, t);
-            __syncthreads();
-        }
-        const int j = jsel;
-        if (t == 0) {
-            if (j < 0) {
-                P = Prop{0, 0, L.lsp[0], L.lsp[0], 0};
-            } else {
-                int a, b;
-                candidate(h0, j, g.natoms, a, b);
-                P = Prop{a, b, L.lsp[b], L.lsp[a], 1};
-            }
-        }
-        __syncthreads();
-        if (j >= 0) {
-            // Hastings ratio: the candidate set scored at x' = swap(x, a, b); only the
-            // candidates within the stencil radius of a or b change their occupancy
-            const int a = P.a, b = P.b;
-            int pj[3], qj[3];
-            site_xyz_of(g, a, pj[0], pj[1], pj[2]);
-            site_xyz_of(g, b, qj[0], qj[1], qj[2]);
-            if (t == 0) nalist = 0;
-            bool same = false;
-            // a candidate the swap does not touch keeps its network output and dU; its
-            // reverse logit differs from the forward one only by the lifted tilt's sign
-            // (the same expression as the scorer's at the proposed state with -d)
-            float lu = lg;
-            if (A.lift && t < A.ncand && lg > -__builtin_huge_valf())
-                lu = (float)(-A.lam * beta * 0.5 * (double)A.gcand[(long)s * A.ncand + t] +
-                             tilt(A, A.ducand[(long)s * A.ncand + t], -A.dir[s]));
-            if (t < MAXC) lrev[t] = lu;
-            __syncthreads();
-            if (t < A.ncand) {
-                int ca, cb, pc[3], qc[3];
-                candidate(h0, t, g.natoms, ca, cb);
-                same = (ca == a && cb == b) || (ca == b && cb == a);
-                site_xyz_of(g, ca, pc[0], pc[1], pc[2]);
-                site_xyz_of(g, cb, qc[0], qc[1], qc[2]);
-                const bool touched = dist2(g, pc, pj) <= A.r2max || dist2(g, pc, qj) <= A.r2max ||
-                                     dist2(g, qc, pj) <= A.r2max || dist2(g, qc, qj) <= A.r2max;
-                if (touched) alist[atomicAdd(&nalist, 1)] = t;
-            }
-            __syncthreads();
-            const int na = nalist;
-            for (int c0 = 0; c0 < na; c0 += MR) {
-                const int nr = min(MR, na - c0);
-                if (t < nr) {
-                    int ca, cb;
-                    candidate(h0, alist[c0 + t], g.natoms, ca, cb);
-                    rows[t] = make_int2(ca, cb);
-                }
-                if (t < MR) du[t] = 0;
-                __syncthreads();
-                int site[IPT];
-                row_sites(A, rows, nr, site);
-                build_rows(A, L.lsp, shell0, rows, nr, site, a, b, L.occ, L.X, L.ldx, du);
-                mlp_rows<false>(A.W, nullptr, L.X, L.ldx, L.H, gp, gout);
-                if (t < nr) {
-                    const int ca = rows[t].x, cb = rows[t].y;
-                    const bool valid = species_swapped(L.lsp, ca, a, b) != species_swapped(L.lsp, cb, a, b);
-                    const float l2 = (float)(-A.lam * beta * 0.5 * (double)gout[t] +
-                                             tilt(A, du[t], A.lift ? -A.dir[s] : 1));
-                    lrev[alist[c0 + t]] = valid ? l2 : -__builtin_huge_valf();
-                }
-                __syncthreads();
-            }
-            const float lr2 = (t < A.ncand) ? lrev[t] : -__builtin_huge_valf();
-            const float m2 = block_max_f(lr2, redf);
-            const double p2 = (lr2 > -__builtin_huge_valf()) ? exp((double)(lr2 - m2)) : 0.0;
-            const double Z2 = block_sum_d(p2, redd);
-            const double D1 = block_sum_d(same ? p : 0.0, redd);
-            const double D2 = block_sum_d(same ? p2 : 0.0, redd);
-            logqr = (float)((log(D2) - log(Z2)) - (log(D1) - log(Z)));
-        }
-    }
-    const Prop p = P;
-    if (t == 0) {
-        int *o = A.prop_next + kPropInts * s;
-        o[0] = p.a;
-        o[1] = p.b;
-        o[2] = p.na;
-        o[3] = p.nb;
-        o[4] = p.sw;
-        o[5] = __float_as_int(logqr);
-    }
-    // occupancy of the proposal at x, for its move record
-    if (A.pend_occ && p.sw)
-        for (int q = t; q < A.nocc; q += blockDim.x)
-            A.pend_occ[(long)s * A.noccp + q] = L.lsp[occ_site_tbl(A, p.a, p.b, q)];
+    lp_block(A, blockIdx.x, lds_raw);
 }
 
 // ------------------------------------------------------------------ evaluation (tests)
-// grid ceil(n / MR), NT threads: g of given occupancy rows through the scorer's MFMA path
-__global__ __launch_bounds__(NT) void k_lp_eval(LPArgs A, int n, const uint8_t *__restrict__ occ_in,
+// grid ceil(n / MR), TB threads: g of given occupancy rows (n, nocc) through the scorer's
+// network (fixed-point layer 1 summed from the occupancy, layer 2 on MFMA)
+__global__ __launch_bounds__(TB) void k_lp_eval(LPArgs A, int nocc, int n, const uint8_t *__restrict__ occ_in,
                                                 float *__restrict__ out) {
     extern __shared__ __align__(16) char lds_raw[];
-    __shared__ float gp[NW * MR], gout[MR];
-    Frag F;
-    load_frag(A.W, F);
-    RowLds L = carve(A, lds_raw);
+    __shared__ float gp[TW * MP], gout[MP];
+    __bf16 *H = reinterpret_cast<__bf16 *>(lds_raw);
     const int r0 = blockIdx.x * MR, nr = min(MR, n - r0), t = threadIdx.x;
-    for (int idx = t; idx < MR * A.noccp; idx += blockDim.x) {
-        const int r = idx / A.noccp, q = idx - r * A.noccp;
-        L.occ[idx] = (r < nr && q < A.nocc) ? occ_in[(long)(r0 + r) * A.nocc + q] : 0;
-    }
-    __syncthreads();
-    const int nchunk = A.kp / 8;
-    for (int idx = t; idx < MR * nchunk; idx += blockDim.x) {
-        const int r = idx / nchunk, c = idx - r * nchunk;
-        bf16x8 v;
-        int q = (8 * c) / A.g.nsp, s = (8 * c) - q * A.g.nsp;
+    const int r = t >> 3;
+    const bool on = r < nr;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = 8 * c + i;
-            const bool on = r < nr && k < A.nfeat && L.occ[r * A.noccp + q] == s;
-            v[i] = __builtin_bit_cast(__bf16, (unsigned short)(on ? 0x3F80 : 0));
-            if (++s == A.g.nsp) {
-                s = 0;
-                ++q;
+    for (int hh = 0; hh < 2; ++hh) {
+        const int ch = (t & 7) + 8 * hh;
+        int acc[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] = 0;
+        if (on) {
+            const uint8_t *o = occ_in + (long)(r0 + r) * nocc;
+            for (int q = 0; q < nocc; ++q) {
+                const int sq = o[q] < A.g.nsp ? o[q] : 0;   // (rows are validated on the host)
+                add_row16(acc, A.w1q + (long)(q * A.g.nsp + sq) * HID + 16 * ch, 1);
             }
         }
-        *reinterpret_cast<bf16x8 *>(L.X + r * L.ldx + 8 * c) = v;
+        h1_store(A.W, H, r, ch, acc, on);
     }
     __syncthreads();
-    mlp_rows<true>(A.W, &F, L.X, L.ldx, L.H, gp, gout);
+    layer2(A.W, H, 2, gp, gout);
     if (t < nr) out[r0 + t] = gout[t];
 }
 
@@ -793,17 +253,18 @@ __global__ void k_lp_adam(int n, float *__restrict__ p, const float *__restrict_
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
 }
 
-// bf16 copies of W1 (both orders), W2 (both orders) and the rounded w3
+// bf16 W1 (the trainer's), the scorer's fixed-point W1q (bf16(W1) in units of 2^-16, half to
+// even, clamped to |w| <= 2), bf16 W2 in both orders and the rounded w3
 __global__ void k_lp_quant(int kp, const float *__restrict__ p, int off_w2, int off_w3, __bf16 *__restrict__ w1,
-                           __bf16 *__restrict__ w1t, __bf16 *__restrict__ w2, __bf16 *__restrict__ w2t,
+                           int *__restrict__ w1q, __bf16 *__restrict__ w2, __bf16 *__restrict__ w2t,
                            float *__restrict__ w3q) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n1 = (long)kp * HID, n2 = (long)HID * HID;
     if (i < n1) {
-        const int f = (int)(i / HID), n = (int)(i % HID);
-        const __bf16 v = __builtin_bit_cast(__bf16, bf16_rne(p[i]));
-        w1[i] = v;
-        w1t[(long)n * kp + f] = v;
+        const unsigned short h = bf16_rne(p[i]);
+        w1[i] = __builtin_bit_cast(__bf16, h);
+        const float q = rintf(bf16_to_float(h) * 65536.f);
+        w1q[i] = (int)fminf(fmaxf(q, (float)-kW1QMax), (float)kW1QMax);
     } else if (i < n1 + n2) {
         const long j = i - n1;
         const int k = (int)(j / HID), n = (int)(j % HID);
@@ -832,6 +293,7 @@ Offs offsets(int kp) {
     return o;
 }
 
+// the parts of the scoring / folding arguments that do not depend on the step
 LPArgs lp_args(chm_sampler *S) {
     const chm_lp &L = S->lp;
     LPArgs A{};
@@ -839,41 +301,32 @@ LPArgs lp_args(chm_sampler *S) {
     A.species = S->species.as<uint8_t>();
     A.rung_of = S->rung_of.as<int>();
     A.beta_slot = S->beta_slot.as<double>();
-    A.stepc = S->stepc.as<uint64_t>();
     A.ss_sten = S->ss_sten.as<int>();
     A.nbr = S->nbr_st.as<int>();
+    A.posmap = L.posmap.as<signed char>();
+    A.pm_r = L.pm_r;
+    for (int q = 0; q < 128; ++q) A.opp[q] = L.opp[q];
     A.seed = S->seed;
     A.ncand = L.ncand;
-    A.nocc = L.nocc;
-    A.noccp = L.noccp;
-    A.nfeat = L.nfeat;
-    A.kp = L.kp;
+    A.nrep = S->nrep;
     A.r2max = L.r2max;
     A.eps = L.eps;
     A.lam = L.lam;
     A.kappa = L.kappa;
     A.beta_max = L.beta_max;
     A.lift = L.lift;
-    A.dir = L.dir.as<int>();
+    for (int q = 0; q < 64; ++q) A.wpair[q] = L.wpair[q];
     const Offs o = offsets(L.kp);
-    A.W.w1t = L.w1t.as<__bf16>();
     A.W.w2t = L.w2t.as<__bf16>();
     A.W.b1 = L.params.as<float>() + o.b1;
     A.W.b2 = L.params.as<float>() + o.b2;
     A.W.b3 = L.params.as<float>() + o.b3;
     A.W.w3q = L.w3q.as<float>();
-    A.W.kp = L.kp;
-    A.logit = L.logit.as<float>();
-    A.gcand = L.gcand.as<float>();
-    A.ducand = L.ducand.as<int>();
-    A.pend_occ = L.pend_occ.as<uint8_t>();
+    A.w1q = L.w1q.as<int>();
+    A.cs = L.cs.as<int>();
+    A.lq = L.lq.as<double>();
+    A.dmax = L.dmax.as<float>();
     return A;
-}
-
-size_t lds_rows(const chm_sampler *S) {
-    const chm_lp &L = S->lp;
-    return (size_t)((S->g.natoms_pad + 15) & ~15) + (size_t)MR * (L.kp + 8) * 2 + (size_t)MR * LDH * 2 +
-           (size_t)MR * L.noccp;
 }
 
 int quantise(chm_sampler *S) {
@@ -881,9 +334,10 @@ int quantise(chm_sampler *S) {
     const Offs o = offsets(L.kp);
     const long n = (long)L.kp * HID + HID * HID + HID;
     k_lp_quant<<<(unsigned)((n + 255) / 256), 256, 0, S->ctx->stream>>>(
-        L.kp, L.params.as<float>(), o.w2, o.w3, L.w1.as<__bf16>(), L.w1t.as<__bf16>(), L.w2.as<__bf16>(),
+        L.kp, L.params.as<float>(), o.w2, o.w3, L.w1.as<__bf16>(), L.w1q.as<int>(), L.w2.as<__bf16>(),
         L.w2t.as<__bf16>(), L.w3q.as<float>());
     CHM_LAUNCH_CHECK();
+    L.cache_ok = false;   // the per-site sums depend on W1
     return CHM_OK;
 }
 
@@ -967,39 +421,121 @@ int lp_prepare(chm_sampler *S) {
     L.nfeat = g.nsp * L.nocc;
     L.kp = (L.nfeat + 31) & ~31;
     L.nparam = offsets(L.kp).n;
-    if (L.kp > 1024 || g.ncoord > 256)
+    for (int q = 0; q < 64; ++q) L.wpair[q] = (q / 8 != q % 8) ? 1 : 0;   // unlike bonds
+    if (L.kp > 1024 || g.ncoord > 127 || g.nsp > kMaxSpW)
         return chm::set_error("learned proposal: too many occupancy features"), CHM_E_ARG;
+    hipStream_t st = S->ctx->stream;
+    // stencil position of every offset in the cube |d| <= pm_r (units a0/2), -1 elsewhere
+    std::vector<int> ss(g.ncoord);
+    CHM_HIP(hipMemcpy(ss.data(), S->ss_sten.p, sizeof(int) * g.ncoord, hipMemcpyDeviceToHost));
+    auto comp = [](int v, int k) { return (int)(signed char)((v >> (8 * k)) & 0xFF); };
+    int R = 0;
+    for (int q = 0; q < g.ncoord; ++q)
+        for (int k = 0; k < 3; ++k) R = std::max(R, std::abs(comp(ss[q], k)));
+    L.pm_r = R;
+    const int side = 2 * R + 1;
+    if ((size_t)side * side * side > (size_t)kPosMapMax)
+        return chm::set_error("learned proposal: stencil offsets too long"), CHM_E_ARG;
+    std::vector<signed char> pm((size_t)side * side * side, -1);
+    auto at = [&](int dx, int dy, int dz) -> signed char & {
+        return pm[((size_t)(dx + R) * side + (dy + R)) * side + (dz + R)];
+    };
+    int n1 = 0;
+    for (int q = 0; q < g.ncoord; ++q) {
+        at(comp(ss[q], 0), comp(ss[q], 1), comp(ss[q], 2)) = (signed char)q;
+        n1 += ((ss[q] >> 24) & 0xFF) == 0;
+    }
+    if (n1 > kMaxFs) return chm::set_error("learned proposal: too many first-shell neighbours"), CHM_E_ARG;
+    // the opposite offset of every stencil position (the stencil of a Bravais lattice holds
+    // both)
+    for (int q = 0; q < g.ncoord; ++q) {
+        const int o = at(-comp(ss[q], 0), -comp(ss[q], 1), -comp(ss[q], 2));
+        if (o < 0) return chm::set_error("learned proposal: the stencil is not symmetric"), CHM_E_ARG;
+        L.opp[q] = (signed char)o;
+    }
+    const size_t nr = (size_t)S->nrep, nc = MAXC;
     int rc;
     if ((rc = L.params.ensure(4 * (size_t)L.nparam)) || (rc = L.adam_m.ensure(4 * (size_t)L.nparam)) ||
-        (rc = L.adam_v.ensure(4 * (size_t)L.nparam)) || (rc = L.w1t.ensure(2 * (size_t)L.kp * HID)) ||
+        (rc = L.adam_v.ensure(4 * (size_t)L.nparam)) || (rc = L.w1q.ensure(4 * (size_t)L.kp * HID)) ||
         (rc = L.w1.ensure(2 * (size_t)L.kp * HID)) || (rc = L.w2t.ensure(2 * (size_t)HID * HID)) ||
         (rc = L.w2.ensure(2 * (size_t)HID * HID)) || (rc = L.w3q.ensure(4 * HID)) ||
-        (rc = L.logit.ensure(4 * (size_t)S->nrep * MAXC)) ||
-        (rc = L.gcand.ensure(4 * (size_t)S->nrep * MAXC)) ||
-        (rc = L.ducand.ensure(4 * (size_t)S->nrep * MAXC)) ||
-        (rc = L.pend_occ.ensure((size_t)S->nrep * L.noccp)))
+        (rc = L.posmap.ensure(pm.size())) || (rc = L.rec.ensure(4 * 2 * kRecInts * nr)) ||
+        (rc = L.sg.ensure(4 * 2 * 2 * nr * nc)) || (rc = L.sdu.ensure(4 * 2 * 2 * nr * nc)) ||
+        (rc = L.st8.ensure(2 * nr * nc)) || (rc = L.cs.ensure(4 * nr)) ||
+        (rc = L.lq.ensure(8 * nr * MAXT * kLqDoubles)) || (rc = L.dmax.ensure(4 * nr * MAXT * 2)) ||
+        (rc = L.dir.ensure(4 * 2 * nr)) || (rc = L.lpn.ensure(8 * nr)) || (rc = L.pend_occ.ensure(nr * L.noccp)))
         return rc;
-    CHM_HIP(hipMemsetAsync(L.pend_occ.p, 0, L.pend_occ.bytes, S->ctx->stream));
-    if (lds_rows(S) > 140 * 1024) return chm::set_error("learned proposal: LDS too small"), CHM_E_ARG;
-    if (MR * L.nocc > IPT * NT) return chm::set_error("learned proposal: stencil too large"), CHM_E_ARG;
-    hipFuncSetAttribute((const void *)k_lp_score, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows(S));
-    hipFuncSetAttribute((const void *)k_lp_eval, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows(S));
-    hipFuncSetAttribute((const void *)k_lp_select, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)lds_rows(S));
+    CHM_HIP(hipMemcpyAsync(L.posmap.p, pm.data(), pm.size(), hipMemcpyHostToDevice, st));
+    CHM_HIP(hipMemsetAsync(L.pend_occ.p, 0, L.pend_occ.bytes, st));
+    CHM_HIP(hipMemsetAsync(L.rec.p, 0, L.rec.bytes, st));
+    CHM_HIP(hipMemsetAsync(L.cs.p, 0, L.cs.bytes, st));
+    CHM_HIP(hipMemsetAsync(L.st8.p, 0, L.st8.bytes, st));
+    CHM_HIP(hipStreamSynchronize(st));
     return CHM_OK;
 }
 
-int lp_launch_score(chm_sampler *S) {
+// The two copies of the per-site layer-1 sums of every slot from its species (after a
+// change of W1, of the species from the host, or of the proposal mode), no swap pending
+int lp_ensure_cache(chm_sampler *S) {
+    chm_lp &L = S->lp;
+    if (L.cache_ok) return CHM_OK;
+    const size_t one = 4 * (size_t)S->nrep * 2 * S->g.natoms * HID;
+    int rc = L.cache.ensure(2 * one);
+    if (rc) return rc;
     LPArgs A = lp_args(S);
-    k_lp_score<<<dim3(S->lp.ncand / MR, S->nrep), NT, lds_rows(S), S->ctx->stream>>>(A);
+    int *c0 = L.cache.as<int>();
+    k_lp_cache_build<<<dim3(S->g.natoms, S->nrep), HID, 0, S->ctx->stream>>>(A, c0, c0 + one / 4);
     CHM_LAUNCH_CHECK();
+    CHM_HIP(hipMemsetAsync(L.rec.p, 0, L.rec.bytes, S->ctx->stream));
+    L.cache_ok = true;
     return CHM_OK;
 }
 
-int lp_launch_select(chm_sampler *S, long long n) {
+size_t lp_tile_lds(const chm_sampler *S) { return tile_lds(S->g.natoms_pad); }
+
+// The scoring / folding arguments of the transform launch of step n (mode 1; 2 for the
+// last step of a segment) or of a segment's first scoring launch after n steps (mode 0)
+LPArgs lp_step_args(chm_sampler *S, int mode, long long n) {
+    chm_lp &L = S->lp;
     LPArgs A = lp_args(S);
-    A.prop_next = prow(S, n + 1);
-    k_lp_select<<<dim3(1, S->nrep), NT, lds_rows(S), S->ctx->stream>>>(A);
+    A.mode = mode;
+    A.stepc = S->stepc.as<uint64_t>();
+    A.lpn = L.lpn.as<uint64_t>();
+    const size_t one = (size_t)S->nrep * 2 * S->g.natoms * HID;
+    int *c0 = L.cache.as<int>();
+    const size_t np = (size_t)S->nrep * 2 * MAXC, nt = (size_t)S->nrep * MAXC;
+    const long long nx = n + 1;
+    int *rec = L.rec.as<int>();
+    auto recp = [&](long long k) { return rec + (size_t)(k & 1) * kRecInts * S->nrep; };
+    int *dir = L.dir.as<int>();
+    if (mode == 0) {
+        A.cache_rd = c0 + (size_t)(nx & 1) * one;
+        A.lag = recp(n);
+        A.dir = dir + (size_t)(nx & 1) * S->nrep;
+    } else {
+        A.cache_rd = c0 + (size_t)(n & 1) * one;
+        A.cache_wr = c0 + (size_t)(nx & 1) * one;
+        A.lag = recp(n - 1);
+        A.lag2 = recp(n);
+        A.prop_n = prow(S, n);
+        A.dir = dir + (size_t)(n & 1) * S->nrep;
+        A.sg_in = L.sg.as<float>() + (size_t)(n & 1) * np;
+        A.sdu_in = L.sdu.as<int>() + (size_t)(n & 1) * np;
+        A.st_in = L.st8.as<int8_t>() + (size_t)(n & 1) * nt;
+    }
+    A.sg_out = L.sg.as<float>() + (size_t)(nx & 1) * np;
+    A.sdu_out = L.sdu.as<int>() + (size_t)(nx & 1) * np;
+    A.st_out = L.st8.as<int8_t>() + (size_t)(nx & 1) * nt;
+    const int slots8 = (S->nrep + 7) / 8 * 8;
+    A.ntile_blocks = (L.ncand / MR) * slots8;
+    A.nfold_blocks = mode == 0 ? 0 : kFoldPerSlot * slots8;
+    return A;
+}
+
+// a segment's first scoring launch, after h steps
+int lp_launch_tiles0(chm_sampler *S, long long h) {
+    LPArgs A = lp_step_args(S, 0, h);
+    k_lp_tiles<<<(unsigned)A.ntile_blocks, TB, lp_tile_lds(S), S->ctx->stream>>>(A);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -1062,15 +598,32 @@ extern "C" int chm_lp_configure(chm_sampler *S, int mode, int ncand, double eps,
     L.kappa = kappa;
     L.beta_max = beta_max;
     L.lift = (mode && lift) ? 1 : 0;
+    L.cache_ok = false;   // the sums were not kept while the proposal was off
     if (L.lift) {
         // directions start alternating over the slots (the stationary law of d is +-1
-        // with equal probability, independent of the configuration)
-        int rc = L.dir.ensure(sizeof(int) * S->nrep);
-        if (rc) return rc;
-        std::vector<int> d(S->nrep);
-        for (int q = 0; q < S->nrep; ++q) d[q] = (q & 1) ? -1 : 1;
-        CHM_HIP(hipMemcpy(L.dir.p, d.data(), sizeof(int) * S->nrep, hipMemcpyHostToDevice));
+        // with equal probability, independent of the configuration); two copies, the
+        // direction before and after a step, alternate by step
+        std::vector<int> d(2 * (size_t)S->nrep);
+        for (int q = 0; q < 2 * S->nrep; ++q) d[q] = ((q % S->nrep) & 1) ? -1 : 1;
+        CHM_HIP(hipMemcpy(L.dir.p, d.data(), sizeof(int) * d.size(), hipMemcpyHostToDevice));
     }
+    return CHM_OK;
+}
+
+extern "C" int chm_lp_set_tilt_weights(chm_sampler *S, const int32_t *w, int nsp) {
+    CHM_REQUIRE(S && w, CHM_E_ARG, "bad arguments");
+    CHM_REQUIRE(nsp == S->g.nsp, CHM_E_ARG, "weights must be nsp x nsp");
+    for (int i = 0; i < nsp; ++i)
+        for (int j = 0; j < nsp; ++j) {
+            CHM_REQUIRE(w[i * nsp + j] == w[j * nsp + i], CHM_E_ARG, "weights must be symmetric");
+            CHM_REQUIRE(w[i * nsp + j] >= -63 && w[i * nsp + j] <= 63, CHM_E_ARG, "weights must be in [-63, 63]");
+        }
+    CHM_HIP(hipSetDevice(S->ctx->device));
+    CHM_HIP(hipStreamSynchronize(S->ctx->stream));
+    drop_graph(S);
+    for (int q = 0; q < 64; ++q) S->lp.wpair[q] = 0;
+    for (int i = 0; i < nsp; ++i)
+        for (int j = 0; j < nsp; ++j) S->lp.wpair[8 * i + j] = (signed char)w[i * nsp + j];
     return CHM_OK;
 }
 
@@ -1128,7 +681,8 @@ extern "C" int chm_lp_eval(chm_sampler *S, int n, const uint8_t *occ, float *g) 
     if ((rc = din.ensure((size_t)n * L.nocc)) || (rc = dout.ensure(4 * (size_t)n))) return rc;
     CHM_HIP(hipMemcpyAsync(din.p, occ, (size_t)n * L.nocc, hipMemcpyHostToDevice, st));
     LPArgs A = lp_args(S);
-    k_lp_eval<<<(unsigned)((n + MR - 1) / MR), NT, lds_rows(S), st>>>(A, n, din.as<uint8_t>(), dout.as<float>());
+    k_lp_eval<<<(unsigned)((n + MR - 1) / MR), TB, (size_t)MR * LDH * 2, st>>>(A, L.nocc, n, din.as<uint8_t>(),
+                                                                                dout.as<float>());
     CHM_LAUNCH_CHECK();
     CHM_HIP(hipMemcpyAsync(g, dout.p, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
     CHM_HIP(hipStreamSynchronize(st));
@@ -1171,28 +725,33 @@ extern "C" int chm_lp_profile(chm_sampler *S, int iters, double *ms) {
     CHM_REQUIRE(L.have_params, CHM_E_STATE, "set the network parameters first");
     CHM_HIP(hipSetDevice(S->ctx->device));
     hipStream_t st = S->ctx->stream;
-    chm::DevBuf pr;
-    int rc;
-    if ((rc = pr.ensure(sizeof(int) * (size_t)S->nrep * kPropInts))) return rc;
-    // both kernels on the current state; the selection writes into scratch rows, so the
-    // chain is not advanced (the scores and the pending occupancy are rewritten by the
-    // next real step)
-    LPArgs A = lp_args(S);
-    A.prop_next = pr.as<int>();
-    A.eps = 0.0;   // time the learned path on every slot
-    A.beta_max = 1e300;
+    int rc = lp_ensure_cache(S);
+    if (rc) return rc;
+    // the scoring workgroups of a step on the current state (every slot learned, the last
+    // step's proposal as the swap; they rewrite only buffers the next segment's first
+    // launches rewrite again), then the folding workgroups alone; the per-site sums are
+    // rebuilt before the next run
+    const double eps = L.eps, bmax = L.beta_max;
+    L.eps = 0.0;
+    L.beta_max = 1e300;
+    LPArgs A = lp_step_args(S, 1, S->hstep);
+    L.eps = eps;
+    L.beta_max = bmax;
+    LPArgs As = A, Af = A;
+    As.nfold_blocks = 0;
+    Af.ntile_blocks = 0;
     hipEvent_t e0, e1, e2;
     CHM_HIP(hipEventCreate(&e0));
     CHM_HIP(hipEventCreate(&e1));
     CHM_HIP(hipEventCreate(&e2));
     CHM_HIP(hipEventRecord(e0, st));
     for (int i = 0; i < iters; ++i)
-        k_lp_score<<<dim3(L.ncand / MR, S->nrep), NT, lds_rows(S), st>>>(A);
+        k_lp_tiles<<<(unsigned)As.ntile_blocks, TB, lp_tile_lds(S), st>>>(As);
     CHM_HIP(hipEventRecord(e1, st));
-    for (int i = 0; i < iters; ++i)
-        k_lp_select<<<dim3(1, S->nrep), NT, lds_rows(S), st>>>(A);
-    CHM_HIP(hipEventRecord(e2, st));
+    for (int i = 0; i < iters; ++i) k_lp_tiles<<<(unsigned)Af.nfold_blocks, TB, lp_tile_lds(S), st>>>(Af);
     CHM_LAUNCH_CHECK();
+    L.cache_ok = false;
+    CHM_HIP(hipEventRecord(e2, st));
     CHM_HIP(hipEventSynchronize(e2));
     float t1 = 0.f, t2 = 0.f;
     CHM_HIP(hipEventElapsedTime(&t1, e0, e1));

@@ -1,0 +1,786 @@
+// Device side of the learned swap proposal (no counterpart in the reference: its swap
+// proposal is uniform, exafs_gpu/kernels.py:362-398), shared by the learned-proposal
+// kernels (learned.hip) and the sampler's transform and per-replica kernels (sampler.hip).
+//
+// Network of a candidate swap (a, b): one-hot species occupancy of a's and b's stencil
+// neighbours and of a and b (nocc = 2 ncoord + 2 entries, nsp classes each):
+//     Z1 = b1 + X W1q,  H1 = bf16(silu(Z1)),  Z2 = b2 + H1 W2,  H2 = silu(Z2),
+//     g = b3 + H2 . w3          (W2, w3 in bf16, sums in FP32; W2 on MFMA)
+// g predicts the swap's energy change nfit (chi2(x') - chi2(x)); a candidate's logit is
+// -lam beta g / 2 + tilt(dU) (dU: change of the weighted first-shell bond count), -inf when
+// a and b hold one species.
+//
+// Layer 1 is linear in the one-hot occupancy: X W1q = A[a] + B[b], the two per-site sums
+// A[c] = sum over c's stencil positions p of W1q[(p, species)] + W1q[(own a, species of c)],
+// B[c] with b's weight blocks.  W1q is bf16(W1) rounded to a multiple of 2^-16 and clamped
+// to |w| <= 2, held as integers, so every sum is exact (int32, and below 2^24, exact in FP32
+// too): the device keeps A and B of every site of every replica slot and updates them by
+// the weight rows a swap changes, with no drift.  Two copies alternate by step: the
+// transform of step n reads copy n & 1 (the configuration before step n - 1, whose swap it
+// applies on the fly) and brings copy (n + 1) & 1 up to the configuration after step n - 1
+// by folding in the swaps of steps n - 2 and n - 1; nothing reads a copy while it is
+// written.
+//
+// Speculative scoring.  The scores of step n+1's candidates depend on whether step n is
+// accepted; the scoring workgroups of step n's transform launch score them at the
+// configuration after step n - 1 and, where step n's swap reaches them, with that swap
+// made; they also score step n's candidates near its swap with the swap made (its
+// reverse probability).  Each 32-candidate tile writes its g / dU and partial softmax
+// sums; the per-replica kernel that decides step n merges them into step n's Hastings
+// ratio and draws step n+1's proposal for the outcome it decided.
+#pragma once
+
+#include <climits>
+#include <cstdint>
+
+#include "common.h"
+#include "sampler_dev.h"
+#include "sampler_state.h"
+
+namespace chm_lp_dev {
+
+using namespace chm_smp;
+
+constexpr int HID = 256;      // hidden width
+constexpr int MR = 32;        // candidates per tile and rows per network pass
+constexpr int MAXC = 256;     // candidates per replica and step, at most
+constexpr int MAXT = MAXC / MR;
+constexpr int LDH = HID + 8;  // H1 row stride in LDS (bf16), staggers the banks
+constexpr int TB = 256;       // threads of a scoring / folding workgroup
+constexpr int TW = TB / 64;   // its waves
+constexpr int kMaxSpW = 8;    // species dimension of the bond-weight table
+constexpr int kMaxFs = 16;    // first-shell stencil entries, at most
+constexpr int kNCorr = 8;     // layer-1 corrections of a row: 2 candidate sites x 4 changed sites
+constexpr int kRecInts = 8;   // swap record: valid, a, b, old species of a, old species of b
+constexpr int kLqDoubles = 6; // per tile: forward (max, Z, D) and reverse (max, Z, D) of log q
+constexpr int kFoldPerSlot = 8;   // folding workgroups per replica slot
+constexpr int MP = 48;        // rows of one network pass: a tile's candidates, then re-scored ones
+constexpr int kPosMapMax = 4096;  // bytes of the offset -> stencil position cube staged in LDS
+constexpr float kW1Inv = 1.f / 65536.f;
+constexpr int kW1QMax = 131072;
+
+struct LPW {
+    const __bf16 *w2t;   // (HID, HID): row n = output unit
+    const float *b1, *b2, *w3q, *b3;
+};
+
+// Everything the scoring and folding workgroups of one launch read; built by the host
+// for one step (mode 1: step n with step n+1's candidates; 2: step n only, the last of a
+// segment; 0: a segment's first candidates after n steps, no fold).
+struct LPArgs {
+    Geo g;
+    const uint8_t *species;
+    const int *rung_of;
+    const double *beta_slot;
+    const int *ss_sten;
+    const int *nbr;            // (natoms, ncoord) neighbour sites in stencil order
+    const signed char *posmap; // stencil position of an offset (cube of side 2 pm_r + 1), or -1
+    int pm_r;
+    signed char opp[128];      // stencil position of the opposite offset of each position
+    uint64_t seed;
+    int ncand, nrep, r2max;
+    double eps, lam, kappa, beta_max;
+    signed char wpair[64];     // first-shell bond weights w[s][t] (8 x 8) of the tilt's dU
+    int lift;                  // lifted proposal: tilt kappa d dU with d = dir[slot]
+    const int *dir;            // (nrep) +-1
+    LPW W;
+    const int *w1q;            // (kp, HID) layer-1 weights, integers in units of 2^-16
+    int mode;
+    // the step number, read on the device (the launch may sit in a captured graph that
+    // replays it at other steps): steps done (mode 0) or the step the per-replica launch
+    // before this one set up (mode 1, 2)
+    const uint64_t *stepc;
+    const uint64_t *lpn;
+    const int *cache_rd;       // (nrep, 2, natoms, HID): the configuration before the lag swap
+    int *cache_wr;             // the other copy: brought up to the committed configuration
+    const int *lag;            // (nrep, kRecInts) the swap of the last decided step (or null rec)
+    const int *lag2;           // the one before (folded together with lag), mode 1 / 2
+    const int *prop_n;         // (nrep, kPropInts) step n's proposal
+    const int *cs;             // (nrep) 1: step n - 1's swap was accepted (mode 1 / 2)
+    const float *sg_in;        // step n's candidates as drawn: (nrep, 2, ncand) g at x (0) and
+    const int *sdu_in;         //   with step n-1's swap (1), their dU, and (nrep, ncand) whether
+    const int8_t *st_in;       //   that swap reached them
+    float *sg_out;             // the same for step n+1's candidates
+    int *sdu_out;
+    int8_t *st_out;
+    double *lq;                // (nrep, MAXT, kLqDoubles) partial sums of step n's log q
+    float *dmax;               // (nrep, MAXT, 2) tile maxima of step n+1's logits: accepted,
+                               //   not accepted
+    int ntile_blocks, nfold_blocks;
+};
+
+__device__ __forceinline__ float silu_fast(float z) { return z * __frcp_rn(1.f + __expf(-z)); }
+
+__device__ __forceinline__ int centred(int d, int n) {
+    d = (d < 0) ? d + n : d;
+    return (2 * d > n) ? d - n : d;
+}
+__device__ __forceinline__ int dist2_sites(const Geo &g, int p, int q) {
+    int px, py, pz, qx, qy, qz;
+    site_xyz_of(g, p, px, py, pz);
+    site_xyz_of(g, q, qx, qy, qz);
+    const int dx = centred(qx - px, g.box[0]), dy = centred(qy - py, g.box[1]), dz = centred(qz - pz, g.box[2]);
+    return dx * dx + dy * dy + dz * dz;
+}
+// stencil position of site x around site c, or -1 (x is not one of c's stencil neighbours);
+// pm: the offset -> position cube (in LDS)
+__device__ __forceinline__ int stencil_pos(const LPArgs &A, const signed char *pm, int c, int x) {
+    int cx, cy, cz, xx, xy, xz;
+    site_xyz_of(A.g, c, cx, cy, cz);
+    site_xyz_of(A.g, x, xx, xy, xz);
+    const int dx = centred(xx - cx, A.g.box[0]), dy = centred(xy - cy, A.g.box[1]), dz = centred(xz - cz, A.g.box[2]);
+    const int R = A.pm_r, side = 2 * R + 1;
+    if (dx < -R || dx > R || dy < -R || dy > R || dz < -R || dz > R) return -1;
+    return pm[((dx + R) * side + (dy + R)) * side + (dz + R)];
+}
+
+// the tilt of a candidate's logit: kappa |dU| (symmetric), or kappa d dU for a lifted
+// proposal moving in direction d = +-1 (the reverse move is scored with -d)
+__device__ __forceinline__ float logit_of(double lam, double kappa, int lift, float g, int du, double beta, int d) {
+    if (du == INT_MIN) return -__builtin_huge_valf();
+    const double tl = lift ? kappa * (double)(d * du) : kappa * fabs((double)du);
+    return (float)(-lam * beta * 0.5 * (double)g + tl);
+}
+
+__device__ __forceinline__ int feat_nbr(const Geo &g, int role, int p, int s) {
+    return (role * g.ncoord + p) * g.nsp + s;
+}
+__device__ __forceinline__ int feat_own(const Geo &g, int role, int s) { return (2 * g.ncoord + role) * g.nsp + s; }
+
+// A swap record: sites a, b and their species before it (v = 0: no swap)
+struct Rec {
+    int v, a, b, oa, ob;
+};
+__device__ __forceinline__ Rec load_rec(const int *p) { return Rec{p[0], p[1], p[2], p[3], p[4]}; }
+// species of x before swap r, given its species after it
+__device__ __forceinline__ int undo(const Rec &r, int x, int sp) {
+    if (r.v && x == r.a) return r.oa;
+    if (r.v && x == r.b) return r.ob;
+    return sp;
+}
+
+// The layer-1 correction of candidate site c (role) for a changed site x: the weight row it
+// holds at the target state minus the one the copy summed; (-1, -1) when x is neither c nor
+// one of c's stencil neighbours, or its species did not change.
+__device__ __forceinline__ int2 corr_entry(const LPArgs &A, const signed char *pm, int c, int role, int x, int s_tgt,
+                                           int s_old) {
+    if (s_tgt == s_old) return make_int2(-1, -1);
+    if (x == c) return make_int2(feat_own(A.g, role, s_tgt), feat_own(A.g, role, s_old));
+    const int p = stencil_pos(A, pm, c, x);
+    if (p < 0) return make_int2(-1, -1);
+    return make_int2(feat_nbr(A.g, role, p, s_tgt), feat_nbr(A.g, role, p, s_old));
+}
+
+__device__ __forceinline__ void add_row16(int (&acc)[16], const int *row, int sign) {
+    const int4 *p = reinterpret_cast<const int4 *>(row);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int4 v = p[k];
+        acc[4 * k] += sign * v.x;
+        acc[4 * k + 1] += sign * v.y;
+        acc[4 * k + 2] += sign * v.z;
+        acc[4 * k + 3] += sign * v.w;
+    }
+}
+
+// g of the first 16 nrt rows (nrt <= MP / 16) whose H1 (bf16, MP x LDH) is in LDS: Z2 = b2 +
+// H1 W2 on v_mfma_f32_16x16x32_bf16 (wave w: hidden units [64 w, 64 w + 64), four 16-column
+// tiles), then b3 + silu(Z2) . w3; gp (TW x MP) is LDS scratch, the result goes to gout
+// (LDS).  The W2 fragments are loaded in two halves of K (64 registers each), so the
+// scoring workgroups fit beside the transform's in one launch.  Every thread of the
+// TB-thread workgroup calls it.
+__device__ inline void layer2(const LPW &W, const __bf16 *H, int nrt, float *gp, float *gout) {
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int lr = lane & 15, lk = 8 * (lane >> 4);
+    constexpr int RT = MP / 16;
+    f32x4 acc[RT][4];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        bf16x8 B[4][4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const __bf16 *w2 = W.w2t + (long)(64 * wave + 16 * c + lr) * HID + lk + 128 * half;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) B[k][c] = *reinterpret_cast<const bf16x8 *>(w2 + 32 * k);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int kt = 4 * half + k;
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                if (r >= nrt) break;
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(H + (16 * r + lr) * LDH + 32 * kt + lk);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, B[k][c], acc[r][c], 0, 0, 0);
+            }
+        }
+    }
+    float b2[4], w3[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        b2[c] = W.b2[64 * wave + 16 * c + lr];
+        w3[c] = W.w3q[64 * wave + 16 * c + lr];
+    }
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        if (r >= nrt) break;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float v = 0.f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v += silu_fast(acc[r][c][i] + b2[c]) * w3[c];
+#pragma unroll
+            for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, 16);
+            if (lr == 0) gp[wave * MP + 16 * r + 4 * (lane >> 4) + i] = v;
+        }
+    }
+    __syncthreads();
+    if (t < 16 * nrt) {
+        float v = 0.f;
+#pragma unroll
+        for (int w = 0; w < TW; ++w) v += gp[w * MP + t];
+        gout[t] = W.b3[0] + v;
+    }
+    __syncthreads();
+}
+
+// H1 of row r, columns [16 ch, 16 ch + 16): silu(b1 + 2^-16 sum) rounded to bf16, into LDS
+__device__ __forceinline__ void h1_store(const LPW &W, __bf16 *H, int r, int ch, const int (&acc)[16], bool on) {
+    bf16x8 v0, v1;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int col = 16 * ch + i;
+        const float z = (float)acc[i] * kW1Inv + W.b1[col];
+        const __bf16 h = __builtin_bit_cast(__bf16, on ? bf16_rne(silu_fast(z)) : (unsigned short)0);
+        if (i < 8)
+            v0[i] = h;
+        else
+            v1[i - 8] = h;
+    }
+    *reinterpret_cast<bf16x8 *>(H + r * LDH + 16 * ch) = v0;
+    *reinterpret_cast<bf16x8 *>(H + r * LDH + 16 * ch + 8) = v1;
+}
+
+// Rows of one network pass, in LDS: candidate (a, b) at the committed state (v = 0) or with
+// the proposal swapped (v = 1); per row the layer-1 corrections, dU and g.
+struct Rows {
+    int a[MP], b[MP], v[MP];
+    int2 corr[MP][kNCorr];
+    int ncorr[MP];
+    int du[MP];
+    float gout[MP];
+    float gp[TW * MP];
+};
+
+// g and dU of rows[0, nrows) (nrows <= MP) of slot s.  The copy `cache_rd` holds the
+// configuration before the lag swap; the committed species (lsp) have it made; rows with
+// v = 1 also have the proposal (pa, pb) made.  fs: first-shell stencil positions (n1); wl:
+// bond weights; pm: offset -> stencil position cube.
+__device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const signed char *pm, const Rec &L, int pa,
+                                  int pb, const int *fs, int n1, const signed char *wl, int s, int nrows, Rows &R,
+                                  __bf16 *H) {
+    const Geo &g = A.g;
+    const int t = threadIdx.x;
+    // changed sites relative to the copy: the lag swap's, then the proposal's not among them
+    int X[4], nx = 0, nxv;
+    if (L.v) {
+        X[nx++] = L.a;
+        X[nx++] = L.b;
+    }
+    nxv = nx;
+    if (pa >= 0) {
+        if (!(L.v && (pa == L.a || pa == L.b))) X[nxv++] = pa;
+        if (!(L.v && (pb == L.a || pb == L.b))) X[nxv++] = pb;
+    }
+    // corrections: (row, candidate site, changed site), two per thread
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = t + TB * h, r = q >> 3, k = q & 7, role = k >> 2, xi = k & 3;
+        int2 e = make_int2(-1, -1);
+        if (r < nrows) {
+            const int v = R.v[r];
+            const int c = role ? R.b[r] : R.a[r];
+            if (xi < (v ? nxv : nx)) {
+                const int x = X[xi];
+                const int st = v ? species_swapped(lsp, x, pa, pb) : (int)lsp[x];
+                e = corr_entry(A, pm, c, role, x, st, undo(L, x, lsp[x]));
+            }
+        }
+        if (r < MP) R.corr[r][k] = e;
+    }
+    __syncthreads();
+    // each row's corrections moved to its front, their count in ncorr
+    if (t < nrows) {
+        int m = 0;
+#pragma unroll
+        for (int k = 0; k < kNCorr; ++k) {
+            const int2 e = R.corr[t][k];
+            if (e.x >= 0) R.corr[t][m++] = e;
+        }
+        R.ncorr[t] = m;
+    }
+    // dU at the row's state: 4 lanes per row over the first-shell entries of a, then b (the
+    // neighbour-table loads of a lane are all issued before any is used)
+    {
+        constexpr int NE = 2 * kMaxFs / 4;
+        const int r = t >> 2, l4 = t & 3;
+        int cnt = 0, sa = 0, sb = 0;
+        if (r < nrows) {
+            const int a = R.a[r], b = R.b[r], v = R.v[r];
+            int y[NE];
+#pragma unroll
+            for (int j = 0; j < NE; ++j) {
+                const int e = l4 + 4 * j;
+                y[j] = -1;
+                if (e < 2 * n1) y[j] = A.nbr[(long)(e < n1 ? a : b) * g.ncoord + fs[e < n1 ? e : e - n1]];
+            }
+            sa = v ? species_swapped(lsp, a, pa, pb) : lsp[a];
+            sb = v ? species_swapped(lsp, b, pa, pb) : lsp[b];
+#pragma unroll
+            for (int j = 0; j < NE; ++j) {
+                const int e = l4 + 4 * j;
+                const bool on_a = e < n1;
+                if (e >= 2 * n1 || y[j] == (on_a ? b : a)) continue;   // the a-b bond keeps its pair
+                const int sy = v ? species_swapped(lsp, y[j], pa, pb) : lsp[y[j]];
+                // bond (own species, sy) becomes (other species, sy)
+                cnt += on_a ? (wl[sb * kMaxSpW + sy] - wl[sa * kMaxSpW + sy])
+                            : (wl[sa * kMaxSpW + sy] - wl[sb * kMaxSpW + sy]);
+            }
+        }
+        cnt += __shfl_xor(cnt, 1, 4);
+        cnt += __shfl_xor(cnt, 2, 4);
+        if (r < nrows && l4 == 0) R.du[r] = (sa != sb) ? cnt : INT_MIN;
+    }
+    __syncthreads();
+    // H1: thread = (row, 64 columns), in two halves of 32: the two per-site rows, then the
+    // corrections
+    {
+        const int r = t >> 2, q4 = t & 3;
+        const bool on = r < nrows;
+        const long base = (long)s * 2 * g.natoms;
+        const int *ra = A.cache_rd + (base + (on ? R.a[r] : 0)) * HID;
+        const int *rb = A.cache_rd + (base + g.natoms + (on ? R.b[r] : 0)) * HID;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ch0 = 4 * q4 + 2 * h;   // chunks ch0, ch0 + 1 of 16 columns
+            int acc0[16], acc1[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc0[i] = acc1[i] = 0;
+            if (on) {
+                add_row16(acc0, ra + 16 * ch0, 1);
+                add_row16(acc1, ra + 16 * (ch0 + 1), 1);
+                add_row16(acc0, rb + 16 * ch0, 1);
+                add_row16(acc1, rb + 16 * (ch0 + 1), 1);
+                // the row's corrections (most rows have none, a touched one one or two)
+                const int nk = R.ncorr[r];
+#pragma unroll 1
+                for (int k = 0; k < nk; ++k) {
+                    const int2 e = R.corr[r][k];
+                    const int *wn = A.w1q + (long)e.x * HID, *wo = A.w1q + (long)e.y * HID;
+                    add_row16(acc0, wn + 16 * ch0, 1);
+                    add_row16(acc1, wn + 16 * (ch0 + 1), 1);
+                    add_row16(acc0, wo + 16 * ch0, -1);
+                    add_row16(acc1, wo + 16 * (ch0 + 1), -1);
+                }
+            }
+            if (r < MP) {
+                h1_store(A.W, H, r, ch0, acc0, on);
+                h1_store(A.W, H, r, ch0 + 1, acc1, on);
+            }
+        }
+    }
+    __syncthreads();
+    layer2(A.W, H, (nrows + 15) >> 4, R.gp, R.gout);
+}
+
+template <int NWV>
+__device__ __forceinline__ float block_max_f(float v, float *red) {
+    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = red[0];
+    for (int w = 1; w < NWV; ++w) r = fmaxf(r, red[w]);
+    __syncthreads();
+    return r;
+}
+
+// inclusive scan over the workgroup's lanes (index order); returns (exclusive, inclusive)
+// and the total
+template <int NWV>
+__device__ __forceinline__ void block_scan_d(double v, double *red, double &excl, double &incl, double &tot) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double x = v;
+    for (int m = 1; m < 64; m <<= 1) {
+        const double y = __shfl_up(x, m, 64);
+        if (lane >= m) x += y;
+    }
+    __syncthreads();
+    if (lane == 63) red[wave] = x;
+    __syncthreads();
+    double off = 0.0, all = 0.0;
+    for (int w = 0; w < NWV; ++w) {
+        if (w < wave) off += red[w];
+        all += red[w];
+    }
+    incl = off + x;
+    const double prev = __shfl_up(x, 1, 64);
+    excl = off + (lane == 0 ? 0.0 : prev);
+    tot = all;
+    __syncthreads();
+}
+
+// The softmax draw over the workgroup's lanes (lane = candidate, lg its logit, m the maximum
+// logit, from the tiles): the lane whose bin [excl, incl) of the cumulative weights holds
+// u Z, the last lane with weight when u Z rounds past the end, -1 when no candidate is
+// valid.  Every thread calls it and gets the same answer.
+template <int NWV>
+__device__ inline int softmax_draw(float lg, float m, double u, double *redd, int *jsh) {
+    const bool any = m > -__builtin_huge_valf();
+    const double p = (any && lg > -__builtin_huge_valf()) ? exp((double)(lg - m)) : 0.0;
+    double excl, incl, Z;
+    block_scan_d<NWV>(p, redd, excl, incl, Z);
+    const double target = u * Z;
+    if (threadIdx.x == 0) *jsh = -1;
+    __syncthreads();
+    if (any && p > 0.0 && excl <= target && target < incl) *jsh = threadIdx.x;
+    __syncthreads();
+    if (any && *jsh < 0) {   // the target rounded past the last bin: the last candidate with p > 0
+        if (p > 0.0) atomicMax(jsh, (int)threadIdx.x);
+        __syncthreads();
+    }
+    const int j = *jsh;
+    __syncthreads();
+    return j;
+}
+
+// log of the softmax share of the lanes with `same` set, from per-tile partial sums:
+// (max, sum of e^(l - max), sum over `same`) of ntile tiles, merged
+__device__ inline double merge_share(const double *part, int ntile, int off) {
+    double m = -__builtin_huge_val();
+    for (int k = 0; k < ntile; ++k) m = fmax(m, part[k * kLqDoubles + off]);
+    double Z = 0.0, D = 0.0;
+    for (int k = 0; k < ntile; ++k) {
+        const double mk = part[k * kLqDoubles + off];
+        if (mk == -__builtin_huge_val()) continue;
+        const double f = exp(mk - m);
+        Z += part[k * kLqDoubles + off + 1] * f;
+        D += part[k * kLqDoubles + off + 2] * f;
+    }
+    return log(D) - log(Z);
+}
+
+// the first-shell stencil positions and the bond weights into LDS; returns how many
+__device__ inline int load_fs(const LPArgs &A, int *fs, signed char *wl, int *nfs) {
+    if (threadIdx.x == 0) *nfs = 0;
+    if (threadIdx.x < kMaxSpW * kMaxSpW) wl[threadIdx.x] = A.wpair[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < 64) {   // one wave, positions in stencil order
+        int base = 0;
+        for (int q0 = 0; q0 < A.g.ncoord; q0 += 64) {
+            const int q = q0 + (int)threadIdx.x;
+            const bool on = q < A.g.ncoord && ((A.ss_sten[q] >> 24) & 0xFF) == 0;
+            const uint64_t m = __ballot(on);
+            if (on) {
+                const int idx = base + __popcll(m & ((1ull << threadIdx.x) - 1ull));
+                if (idx < kMaxFs) fs[idx] = q;
+            }
+            base += __popcll(m);
+        }
+        if (threadIdx.x == 0) *nfs = min(base, kMaxFs);
+    }
+    __syncthreads();
+    return *nfs;
+}
+
+// partial softmax sums of a tile's lanes (lane < MR): max, sum e^(l - max), sum over `same`
+__device__ __forceinline__ void tile_partial(float l, bool same, double &m, double &Z, double &D) {
+    float mf = l;
+#pragma unroll
+    for (int k = 16; k >= 1; k >>= 1) mf = fmaxf(mf, __shfl_xor(mf, k, 32));
+    const double p = (mf > -__builtin_huge_valf() && l > -__builtin_huge_valf()) ? exp((double)(l - mf)) : 0.0;
+    double z = p, d = same ? p : 0.0;
+#pragma unroll
+    for (int k = 16; k >= 1; k >>= 1) {
+        z += __shfl_xor(z, k, 32);
+        d += __shfl_xor(d, k, 32);
+    }
+    m = (double)mf;
+    Z = z;
+    D = d;
+}
+
+// dynamic LDS of a scoring workgroup: the slot's species, H1, the offset -> position cube
+__host__ __device__ inline size_t tile_lds(int natoms_pad) {
+    return (size_t)((natoms_pad + 15) & ~15) + (size_t)MP * LDH * 2 + kPosMapMax;
+}
+
+// One scoring workgroup: tile k of slot s (TB threads).  Candidates [k MR, (k+1) MR):
+//   (i)   of step n+1 at the committed configuration x (x = the copy with the lag swap made),
+//   (ii)  of step n+1 within the radius of step n's swap P, at x with P made,
+//   (iii) of step n within the radius of P, at x with P made (the reverse set),
+// in one network pass of (i) followed by the rows of (ii) and (iii) (more passes only when
+// more than MR of those); writes g / dU of (i), (ii), the partial sums of step n's log q
+// ratio and the tile maxima of step n+1's logits for either outcome of step n.
+__device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) {
+    __shared__ Rows R;
+    __shared__ int fs[kMaxFs];
+    __shared__ signed char wl[kMaxSpW * kMaxSpW];
+    __shared__ int li[2 * MR], nl, nfs;
+    __shared__ float gAv[MR], gBv[MR], gRv[MR];
+    __shared__ int duAv[MR], duBv[MR], duRv[MR], tBv[MR], tRv[MR];
+    const Geo &g = A.g;
+    const int t = threadIdx.x;
+    const int rung = A.rung_of[s];
+    const double beta = A.beta_slot[s];
+    const uint64_t n = (A.mode == 0) ? A.stepc[s] : A.lpn[s], nx = n + 1;
+    const uint64_t h_next = lp_stream(A.seed, nx, rung), h_cur = lp_stream(A.seed, n, rung);
+    const bool learn_next = A.mode != 2 && !lp_coin_uniform(h_next, beta, A.eps, A.beta_max);
+    const bool learn_cur = A.mode != 0 && !lp_coin_uniform(h_cur, beta, A.eps, A.beta_max);
+    Prop P{-1, -1, 0, 0, 0};
+    if (A.mode != 0) {
+        const int *pp = A.prop_n + kPropInts * s;
+        P = Prop{pp[0], pp[1], pp[2], pp[3], pp[4]};
+    }
+    const bool has_p = P.sw != 0;
+    const bool need_i = learn_next, need_ii = learn_next && has_p, need_iii = learn_cur && has_p;
+    if (!need_i && !need_iii) return;   // (the same for every tile of the slot)
+    const Rec LG = load_rec(A.lag + kRecInts * s);
+    uint8_t *lsp = reinterpret_cast<uint8_t *>(lds_raw);
+    __bf16 *H = reinterpret_cast<__bf16 *>(lds_raw + ((g.natoms_pad + 15) & ~15));
+    signed char *pm = reinterpret_cast<signed char *>(lds_raw + ((g.natoms_pad + 15) & ~15) + (size_t)MP * LDH * 2);
+    const int i0 = tile * MR;
+    const long cbase = (long)s * A.ncand;
+    load_species(lsp, A.species + (long)s * g.natoms_pad, g.natoms_pad);
+    {
+        const int side = 2 * A.pm_r + 1, npm = side * side * side;
+        for (int q = t; q < npm; q += TB) pm[q] = A.posmap[q];
+    }
+    const int n1 = load_fs(A, fs, wl, &nfs);
+    const int d0 = A.lift ? A.dir[s] : 1;
+    // rows (ii) / (iii): candidates within the stencil radius of P (a superset of the ones a
+    // correction reaches; the others score as at x), in candidate order
+    if (t < 64) {
+        const int kind = t < MR ? 1 : 2;
+        const int c = t & (MR - 1);
+        const bool want = kind == 1 ? need_ii : need_iii;
+        bool touched = false;
+        if (want) {
+            int ca, cb;
+            lp_candidate(kind == 1 ? h_next : h_cur, i0 + c, g.natoms, ca, cb);
+            touched = dist2_sites(g, ca, P.a) <= A.r2max || dist2_sites(g, ca, P.b) <= A.r2max ||
+                      dist2_sites(g, cb, P.a) <= A.r2max || dist2_sites(g, cb, P.b) <= A.r2max;
+        }
+        const uint64_t m = __ballot(touched);
+        if (touched) li[__popcll(m & ((1ull << t) - 1ull))] = t;   // t: kind and offset
+        if (t == 0) nl = __popcll(m);
+        if (kind == 1) {
+            tBv[c] = touched;
+            if (need_i) A.st_out[cbase + i0 + c] = (int8_t)touched;
+        } else {
+            tRv[c] = touched;
+        }
+    }
+    __syncthreads();
+    // passes: (i) rows first (when step n+1 is learned), then the re-scored rows
+    const int n0 = need_i ? MR : 0, ntot = n0 + nl;
+    for (int q0 = 0; q0 < ntot; q0 += MP) {
+        const int nr = min(MP, ntot - q0);
+        if (t < nr) {
+            const int q = q0 + t;
+            int a, b, v;
+            if (q < n0) {
+                lp_candidate(h_next, i0 + q, g.natoms, a, b);
+                v = 0;
+            } else {
+                const int e = li[q - n0];
+                lp_candidate(e < MR ? h_next : h_cur, i0 + (e & (MR - 1)), g.natoms, a, b);
+                v = 1;
+            }
+            R.a[t] = a;
+            R.b[t] = b;
+            R.v[t] = v;
+        }
+        __syncthreads();
+        score_rows(A, lsp, pm, LG, P.a, P.b, fs, n1, wl, s, nr, R, H);
+        if (t < nr) {
+            const int q = q0 + t;
+            const float gv = R.gout[t];
+            const int dv = R.du[t];
+            if (q < n0) {
+                gAv[q] = gv;
+                duAv[q] = dv;
+                A.sg_out[cbase * 2 + i0 + q] = gv;
+                A.sdu_out[cbase * 2 + i0 + q] = dv;
+            } else {
+                const int e = li[q - n0], c = e & (MR - 1);
+                if (e < MR) {
+                    gBv[c] = gv;
+                    duBv[c] = dv;
+                    A.sg_out[cbase * 2 + A.ncand + i0 + c] = gv;
+                    A.sdu_out[cbase * 2 + A.ncand + i0 + c] = dv;
+                } else {
+                    gRv[c] = gv;
+                    duRv[c] = dv;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (t >= MR) return;
+    // lanes 0..31 of wave 0: candidate i0 + t of step n (partial log q) and of step n+1
+    // (tile maxima of its logits for either outcome)
+    double *lq = A.lq + ((long)s * MAXT + tile) * kLqDoubles;
+    if (need_iii) {
+        const int c = i0 + t;
+        const bool tb = A.cs[s] && A.st_in[cbase + c];
+        const float fg = A.sg_in[cbase * 2 + (tb ? A.ncand : 0) + c];
+        const int fdu = A.sdu_in[cbase * 2 + (tb ? A.ncand : 0) + c];
+        const float rg = tRv[t] ? gRv[t] : fg;
+        const int rdu = tRv[t] ? duRv[t] : fdu;
+        int ca, cb;
+        lp_candidate(h_cur, c, g.natoms, ca, cb);
+        const bool same = (ca == P.a && cb == P.b) || (ca == P.b && cb == P.a);
+        double m, Z, D;
+        tile_partial(logit_of(A.lam, A.kappa, A.lift, fg, fdu, beta, d0), same, m, Z, D);
+        if (t == 0) {
+            lq[0] = m;
+            lq[1] = Z;
+            lq[2] = D;
+        }
+        tile_partial(logit_of(A.lam, A.kappa, A.lift, rg, rdu, beta, -d0), same, m, Z, D);
+        if (t == 0) {
+            lq[3] = m;
+            lq[4] = Z;
+            lq[5] = D;
+        }
+    }
+    if (need_i) {
+        // the direction after step n: kept if it is accepted, reversed if a learned step n
+        // is not
+        const int dR = (A.mode != 0 && learn_cur) ? -d0 : d0;
+        const float lR = logit_of(A.lam, A.kappa, A.lift, gAv[t], duAv[t], beta, dR);
+        const bool tb = need_ii && tBv[t];
+        const float lA = logit_of(A.lam, A.kappa, A.lift, tb ? gBv[t] : gAv[t], tb ? duBv[t] : duAv[t], beta, d0);
+        float mA = lA, mR = lR;
+#pragma unroll
+        for (int k = 16; k >= 1; k >>= 1) {
+            mA = fmaxf(mA, __shfl_xor(mA, k, 32));
+            mR = fmaxf(mR, __shfl_xor(mR, k, 32));
+        }
+        if (t == 0) {
+            A.dmax[((long)s * MAXT + tile) * 2] = mA;
+            A.dmax[((long)s * MAXT + tile) * 2 + 1] = mR;
+        }
+    }
+}
+
+// One folding workgroup (part `part` of kFoldPerSlot for slot s, TB threads): brings the
+// copy cache_wr from the configuration before swaps lag2 and lag up to the committed one.
+// A swap changes the rows of its sites and of their stencil neighbours.  Work item = (site,
+// role, 16 columns) over the changed sites and their stencil neighbours, each site taken
+// under the first changed site that reaches it (so no row is written twice); the item adds
+// the weight-row changes of every changed site that is the site itself or one of its
+// stencil neighbours.
+__device__ inline void lp_fold(const LPArgs &A, int s, int part, char *lds_raw) {
+    const Geo &g = A.g;
+    const int nc = g.ncoord;
+    const Rec L1 = load_rec(A.lag + kRecInts * s), L2 = load_rec(A.lag2 + kRecInts * s);
+    int X[4], nX = 0;
+    if (L2.v) {
+        X[nX++] = L2.a;
+        X[nX++] = L2.b;
+    }
+    if (L1.v) {
+        if (!(L2.v && (L1.a == L2.a || L1.a == L2.b))) X[nX++] = L1.a;
+        if (!(L2.v && (L1.b == L2.a || L1.b == L2.b))) X[nX++] = L1.b;
+    }
+    if (nX == 0) return;
+    signed char *pm = reinterpret_cast<signed char *>(lds_raw);
+    {
+        const int side = 2 * A.pm_r + 1, npm = side * side * side;
+        for (int q = threadIdx.x; q < npm; q += TB) pm[q] = A.posmap[q];
+    }
+    const uint8_t *gsp = A.species + (long)s * g.natoms_pad;
+    int snow[4], sold[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        snow[k] = sold[k] = 0;
+        if (k < nX) {
+            snow[k] = gsp[X[k]];
+            sold[k] = undo(L2, X[k], undo(L1, X[k], snow[k]));
+        }
+    }
+    __syncthreads();
+    const int nitems = nX * (nc + 1) * 32;
+    for (int it = part * TB + threadIdx.x; it < nitems; it += kFoldPerSlot * TB) {
+        const int k = it >> 5, role = (it >> 4) & 1, ch = it & 15;
+        const int ci = k / (nc + 1), q = k - ci * (nc + 1);
+        int xc = X[0];
+#pragma unroll
+        for (int j = 1; j < 4; ++j)
+            if (j == ci) xc = X[j];
+        const int c = (q == 0) ? xc : A.nbr[(long)xc * nc + q - 1];
+        bool dup = false;
+        int2 e[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            e[j] = make_int2(-1, -1);
+            if (j < nX) {
+                // position of changed site j around c (own: -2, none: -1)
+                const int pj = (c == X[j]) ? -2 : stencil_pos(A, pm, c, X[j]);
+                if (j < ci && pj != -1) dup = true;   // an earlier changed site owns c
+                if (pj != -1 && snow[j] != sold[j])
+                    e[j] = pj == -2 ? make_int2(feat_own(g, role, snow[j]), feat_own(g, role, sold[j]))
+                                    : make_int2(feat_nbr(g, role, pj, snow[j]), feat_nbr(g, role, pj, sold[j]));
+            }
+        }
+        if (dup || (e[0].x < 0 && e[1].x < 0 && e[2].x < 0 && e[3].x < 0)) continue;
+        int acc[16];
+        int *row = A.cache_wr + (((long)s * 2 + role) * g.natoms + c) * HID + 16 * ch;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] = 0;
+        add_row16(acc, row, 1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (e[j].x >= 0) {
+                add_row16(acc, A.w1q + (long)e[j].x * HID + 16 * ch, 1);
+                add_row16(acc, A.w1q + (long)e[j].y * HID + 16 * ch, -1);
+            }
+        int4 *dst = reinterpret_cast<int4 *>(row);
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) dst[qq] = make_int4(acc[4 * qq], acc[4 * qq + 1], acc[4 * qq + 2], acc[4 * qq + 3]);
+    }
+}
+
+// The scoring and folding workgroups of a launch: block index b in [0, ntile_blocks) is
+// a scoring tile, [ntile_blocks, ntile_blocks + nfold_blocks) a folding part; the ones of
+// one slot sit on one XCD (consecutive ids go to the eight XCDs in turn).
+__device__ inline void lp_block(const LPArgs &A, int b, char *lds_raw) {
+    const int ntile = A.ncand / MR;
+    if (b < A.ntile_blocks) {
+        const int j = b >> 3, m = j / ntile;
+        const int tile = j - m * ntile, s = (b & 7) + 8 * m;
+        if (s < A.nrep) lp_tile(A, s, tile, lds_raw);
+        return;
+    }
+    b -= A.ntile_blocks;
+    if (b >= A.nfold_blocks) return;
+    const int j = b >> 3, m = j / kFoldPerSlot;
+    const int part = j - m * kFoldPerSlot, s = (b & 7) + 8 * m;
+    if (s < A.nrep) lp_fold(A, s, part, lds_raw);
+}
+
+}  // namespace chm_lp_dev
+
+struct chm_sampler;
+namespace chm_smp {
+// learned.hip: the scoring / folding arguments of the transform launch of step n (mode 1;
+// 2 for a segment's last step) or of a segment's first scoring launch after n steps (0)
+chm_lp_dev::LPArgs lp_step_args(chm_sampler *S, int mode, long long n);
+}  // namespace chm_smp

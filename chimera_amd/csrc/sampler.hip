@@ -31,6 +31,7 @@
 #include "common.h"
 #include "sampler_dev.h"
 #include "sampler_state.h"
+#include "learned_dev.h"
 
 namespace {
 
@@ -235,9 +236,26 @@ __device__ __forceinline__ void gemm_abs_body(const Geo &g, int nrep, const int 
         atomicAdd(reinterpret_cast<unsigned long long *>(O.stepc + r0 + threadIdx.x), 1ull);
 }
 
+// The learned proposal's scoring and folding workgroups ride along in a transform launch:
+// the last LP.ntile_blocks + LP.nfold_blocks workgroups of grid row (y, z) = (0, 0) run
+// them (independent of the transform: they read the committed configuration and write
+// buffers the transform does not touch); those of the other rows exit.
+__device__ __forceinline__ bool lp_riders(const chm_lp_dev::LPArgs &LP) {
+    const int nlp = LP.ntile_blocks + LP.nfold_blocks;
+    const int x0 = (int)gridDim.x - nlp;
+    if (nlp == 0 || (int)blockIdx.x < x0) return false;
+    if (blockIdx.y == 0 && blockIdx.z == 0) {
+        extern __shared__ __align__(16) char lp_lds[];
+        chm_lp_dev::lp_block(LP, (int)blockIdx.x - x0, lp_lds);
+    }
+    return true;
+}
+
 __global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int *__restrict__ counts,
                                                        const int *__restrict__ dcounts,
-                                                       const double *__restrict__ basisR, TileOut O) {
+                                                       const double *__restrict__ basisR, TileOut O,
+                                                       chm_lp_dev::LPArgs LP) {
+    if (lp_riders(LP)) return;
     gemm_abs_body(g, nrep, counts, dcounts, basisR, O, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -471,7 +489,9 @@ __device__ __forceinline__ void gemm_delta_bf16_body(const Geo &g, int nrep, con
 
 __global__ __launch_bounds__(256) void k_gemm_delta_bf16(Geo g, int nrep, const int *__restrict__ dcounts,
                                                           const bf16x8 *__restrict__ Tb,
-                                                          const double *__restrict__ mcur, TileOut O) {
+                                                          const double *__restrict__ mcur, TileOut O,
+                                                          chm_lp_dev::LPArgs LP) {
+    if (lp_riders(LP)) return;
     if ((int)blockIdx.x == g.nct) {   // the extra column of workgroups: draws + step counter
         if (blockIdx.z == 0) compute_draws(O, nrep, blockIdx.y * RB);
         return;
@@ -538,7 +558,9 @@ __device__ __forceinline__ void gemm_delta_f32_body(const Geo &g, int nrep, cons
 
 __global__ __launch_bounds__(256) void k_gemm_delta_f32(Geo g, int nrep, const int *__restrict__ dcounts,
                                                          const float *__restrict__ basis32,
-                                                         const double *__restrict__ mcur, TileOut O) {
+                                                         const double *__restrict__ mcur, TileOut O,
+                                                         chm_lp_dev::LPArgs LP) {
+    if (lp_riders(LP)) return;
     gemm_delta_f32_body(g, nrep, dcounts, basis32, mcur, O, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -626,11 +648,26 @@ struct SlotArgs {
     int profile;
     int nrep;               // replica slots of the problem
     int accept;             // 0: none, 1: step, 2: init (store chi^2, no decision)
-    int propose;            // 1: propose + gather the next step; 2: gather prop_next
+    int propose;            // 1: propose + gather the next step; 4: the same with the learned
+                            //    proposal (the scoring kernel's draw for this step's outcome)
     double nfit;
     const int *draws;       // (nrep, kDrawInts) made by the transform just before, or null
-    int *lp_dir;            // lifted learned proposal: direction per slot, flipped when a
-    double lp_eps, lp_beta_max;   //   learned step is not accepted (null: not lifted)
+    // learned proposal (lp_lq null: off).  The scoring workgroups of this step's transform
+    // left per-tile partial sums of its log q ratio and, for the next step's candidates,
+    // g / dU at the committed configuration and with this step's swap made, and the tile
+    // maxima of their logits for either outcome.
+    const double *lp_lq;    // (nrep, MAXT, 6)
+    const float *lp_dmax;   // (nrep, MAXT, 2)
+    const float *lp_sg;     // (nrep, 2, ncand)
+    const int *lp_sdu;
+    const int8_t *lp_st;    // (nrep, ncand)
+    const int *lp_dir_rd;   // lifted: direction before this step (null: not lifted)
+    int *lp_dir_wr;         //   after it (reversed when a learned step is not accepted)
+    int *lp_rec;            // this step's swap record for the per-site sums
+    int *lp_cs;             // this step's outcome for the next scoring workgroups
+    uint64_t *lp_lpn;       // the step the next transform's scoring workgroups serve
+    int lp_ncand, lp_lift;
+    double lp_eps, lp_beta_max, lp_lam, lp_kappa;
 };
 
 __device__ double block_sum(double v, double *red) {
@@ -698,9 +735,22 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     int pp[kPropInts];                       // the proposal being accepted
 #pragma unroll
     for (int q = 0; q < kPropInts; ++q) pp[q] = A.prop_acc[kPropInts * s + q];
-    int pn[5];                               // a proposal written by the learned selection
-#pragma unroll
-    for (int q = 0; q < 5; ++q) pn[q] = A.prop_next[kPropInts * s + q];
+    // learned proposal: this lane's candidate of the next step (g / dU at the committed
+    // configuration and with this step's swap made) and the direction before this step
+    float lpgA = 0.f, lpgB = 0.f;
+    int lpdA = INT_MIN, lpdB = INT_MIN, lptB = 0, lpd0 = 1;
+    if (A.lp_lq) {
+        if (A.propose == 4 && t < A.lp_ncand) {
+            const long cb = (long)s * A.lp_ncand;
+            lpgA = A.lp_sg[2 * cb + t];
+            lpdA = A.lp_sdu[2 * cb + t];
+            lpgB = A.lp_sg[2 * cb + A.lp_ncand + t];
+            lpdB = A.lp_sdu[2 * cb + A.lp_ncand + t];
+            lptB = A.lp_st[cb + t];
+        }
+        if (A.lp_dir_rd) lpd0 = A.lp_dir_rd[s];
+    }
+    __shared__ int lp_dnew;
     double q0v[2 * chm::kMaxNsp];            // per edge T0, Q0 (padded with zeros)
 #pragma unroll
     for (int q = 0; q < 2 * chm::kMaxNsp; ++q) q0v[q] = A.q0[q];
@@ -771,13 +821,13 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     if (A.draws) {
         const int *D = A.draws + (long)s * kDrawInts;
         if (A.accept == 1) u_acc = draw_u(D, 0);
-        if (A.propose == 1) dr = draw_p(D, 0);
+        if (A.propose) dr = draw_p(D, 0);
     } else if (t < 64) {
         if (A.accept == 1) {
             const uint64_t h = splitmix64(A.seed ^ (stepc * kGolden) ^ ((uint64_t)rung << 32) ^ kAccSalt);
             u_acc = u01(splitmix64(h));
         }
-        if (A.propose == 1) dr = propose_draws(g.natoms, A.seed, stepc + 1, (uint64_t)rung);
+        if (A.propose) dr = propose_draws(g.natoms, A.seed, stepc + 1, (uint64_t)rung);
     }
     __builtin_amdgcn_sched_barrier(0);   // every load issued before the first use
     // ------------------------------------------------ consume
@@ -803,11 +853,23 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
             double cur = c2t;
             if (A.accept == 1) {
                 acc = 0;
+                // a learned step's log q(x'->x) - log q(x->x') from the scoring tiles' partial
+                // sums; a uniform step's (0) in pp[5]
+                const bool learned = A.lp_lq && !lp_coin_uniform(lp_stream(A.seed, stepc, rung), beta,
+                                                                 A.lp_eps, A.lp_beta_max);
                 if (pp[4]) {
-                    // Metropolis-Hastings: pp[5] holds log q(x'->x) - log q(x->x')
-                    const double x = beta * (c2t - c2p) * A.nfit - (double)__int_as_float(pp[5]);
+                    double lq = (double)__int_as_float(pp[5]);
+                    if (learned) {
+                        const double *part = A.lp_lq + (long)s * chm_lp_dev::MAXT * chm_lp_dev::kLqDoubles;
+                        const int nt = A.lp_ncand / chm_lp_dev::MR;
+                        lq = chm_lp_dev::merge_share(part, nt, 3) - chm_lp_dev::merge_share(part, nt, 0);
+                    }
+                    const double x = beta * (c2t - c2p) * A.nfit - lq;
                     acc = metropolis(x, u_acc);
                 }
+                // a lifted learned step that is not accepted reverses its direction (a
+                // uniform step, on the coin or a cold rung, keeps it)
+                lp_dnew = (learned && !acc) ? -lpd0 : lpd0;
                 if (acc) {
                     lsp[pp[0]] = (uint8_t)pp[2];
                     lsp[pp[1]] = (uint8_t)pp[3];
@@ -824,11 +886,19 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
                         A.species[base_sp + pp[1]] = (uint8_t)pp[3];
                         atomicAdd(reinterpret_cast<unsigned long long *>(A.acc_rung + rung), 1ull);
                     }
-                    // a lifted learned step that is not accepted reverses its direction
-                    // (a uniform step, on the coin or a cold rung, keeps it)
-                    if (A.lp_dir && !acc &&
-                        !lp_coin_uniform(lp_stream(A.seed, stepc, rung), beta, A.lp_eps, A.lp_beta_max))
-                        A.lp_dir[s] = -A.lp_dir[s];
+                    if (A.lp_lq) {
+                        if (A.lp_dir_wr) A.lp_dir_wr[s] = lp_dnew;
+                        // this step's swap, for the per-site sums (folded in by the next
+                        // transforms' folding workgroups)
+                        int *rc = A.lp_rec + chm_lp_dev::kRecInts * s;
+                        rc[0] = acc && pp[4];
+                        rc[1] = pp[0];
+                        rc[2] = pp[1];
+                        rc[3] = pp[3];   // old species of a (= new species of b)
+                        rc[4] = pp[2];
+                        A.lp_cs[s] = acc;
+                        A.lp_lpn[s] = stepc + 1;
+                    }
                     if (A.log && ri >= 0 && ri < A.log_cap) {
                         int *L = A.log + ((long)ri * A.nrep + s) * 4;
                         L[0] = pp[0];
@@ -912,15 +982,43 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     }
     if (!A.propose) return;
     // ------------------------------------------------ propose + gather of the next step
-    if (A.propose == 2) {
-        // the learned proposal (or its uniform fallback) chosen by the selection kernel
-        const Prop p{pn[0], pn[1], pn[2], pn[3], pn[4]};
-        if (p.sw) gather_swap(g, p, A.ss_sten, A.ms_sten, lsp, hist, first, stride, recs,
-                              A.dc_prop + (long)s * g.nrow_w);
-        return;
+    // the learned proposal: a softmax draw over the next step's candidates as the scoring
+    // tiles left them for this step's outcome (every workgroup draws the same one)
+    const uint64_t hn = lp_stream(A.seed, stepc + 1, rung);
+    const bool learned_next = A.propose == 4 && !lp_coin_uniform(hn, beta, A.lp_eps, A.lp_beta_max);
+    int jl = -1;
+    if (learned_next) {
+        __shared__ double lp_red[4];
+        __shared__ int lp_j;
+        const bool accd = A.accept == 1 && (flag & 1);
+        const int dout = (A.accept == 1) ? lp_dnew : lpd0;
+        const int nt = A.lp_ncand / chm_lp_dev::MR;
+        float m = -__builtin_huge_valf();
+        for (int k = 0; k < nt; ++k)
+            m = fmaxf(m, A.lp_dmax[((long)s * chm_lp_dev::MAXT + k) * 2 + (accd ? 0 : 1)]);
+        const bool useB = accd && lptB;
+        const float lg = (t < A.lp_ncand)
+                             ? chm_lp_dev::logit_of(A.lp_lam, A.lp_kappa, A.lp_lift, useB ? lpgB : lpgA,
+                                                    useB ? lpdB : lpdA, beta, dout)
+                             : -__builtin_huge_valf();
+        jl = chm_lp_dev::softmax_draw<4>(lg, m, u01(splitmix64(hn ^ kSelSalt)), lp_red, &lp_j);
     }
     if (t == 0) {
-        P = propose_pick(lsp, dr);
+        if (learned_next) {
+            if (jl >= 0) {
+                int a, b;
+                lp_candidate(hn, jl, g.natoms, a, b);
+                P = Prop{a, b, lsp[b], lsp[a], 1};
+            } else {
+                P = Prop{0, 0, lsp[0], lsp[0], 0};   // no valid candidate: a null move
+            }
+        } else {
+            P = propose_pick(lsp, dr);
+        }
+        if (lead && A.propose == 4 && A.accept != 1) {
+            A.lp_cs[s] = 0;
+            A.lp_lpn[s] = stepc + 1;
+        }
         if (lead) {
             int *o = A.prop_next + kPropInts * s;
             o[0] = P.a;
@@ -1526,14 +1624,17 @@ SlotArgs slot_args(chm_sampler *S, int accept, int propose) {
     const chm_lp &L = S->lp;
     A.nocc = L.nocc;
     A.noccp = L.noccp;
-    A.pend_occ = (L.rec_cap || L.mode) ? L.pend_occ.as<uint8_t>() : nullptr;
+    A.pend_occ = L.rec_cap ? L.pend_occ.as<uint8_t>() : nullptr;
     A.rec_cap = L.rec_cap;
     A.rec_occ = L.rec_cap ? L.rec_occ.as<uint8_t>() : nullptr;
     A.rec_y = L.rec_cap ? L.rec_y.as<float>() : nullptr;
     A.rec_meta = L.rec_cap ? L.rec_meta.as<int>() : nullptr;
-    A.lp_dir = (L.mode && L.lift) ? L.dir.as<int>() : nullptr;
     A.lp_eps = L.eps;
     A.lp_beta_max = L.beta_max;
+    A.lp_lam = L.lam;
+    A.lp_kappa = L.kappa;
+    A.lp_lift = L.lift;
+    A.lp_ncand = L.ncand;
     A.seed = S->seed;
     A.profile = S->profile;
     A.nrep = S->nrep;
@@ -1549,13 +1650,23 @@ size_t lds_gemm(const Geo &g) {
     return std::max(a, p);
 }
 
-int launch_gemm_abs(chm_sampler *S, const int *dc, int bump) {
+// grid x and dynamic LDS of a transform launch with the learned proposal's workgroups (lp)
+void lp_ride(chm_sampler *S, const chm_lp_dev::LPArgs *lp, dim3 &grid, size_t &lds) {
+    if (!lp) return;
+    grid.x += (unsigned)(lp->ntile_blocks + lp->nfold_blocks);
+    lds = std::max(lds, lp_tile_lds(S));
+}
+
+int launch_gemm_abs(chm_sampler *S, const int *dc, int bump, const chm_lp_dev::LPArgs *lp = nullptr) {
     const Geo &g = S->g;
     dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
     TileOut O = tile_out(S);
     O.bump = bump;
-    k_gemm_abs_f64<<<grid, 256, lds_gemm(g), S->ctx->stream>>>(
-        g, S->nrep, S->counts.as<int>(), dc, S->basisR.as<double>(), O);
+    size_t lds = lds_gemm(g);
+    lp_ride(S, lp, grid, lds);
+    k_gemm_abs_f64<<<grid, 256, lds, S->ctx->stream>>>(g, S->nrep, S->counts.as<int>(), dc,
+                                                       S->basisR.as<double>(), O,
+                                                       lp ? *lp : chm_lp_dev::LPArgs{});
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -1569,22 +1680,48 @@ void set_draws(chm_sampler *S, TileOut &O, int mode) {
     O.natoms = S->g.natoms;
 }
 
-int launch_gemm_delta(chm_sampler *S, const int *dc, int bump) {
+int launch_gemm_delta(chm_sampler *S, const int *dc, int bump, const chm_lp_dev::LPArgs *lp = nullptr) {
     const Geo &g = S->g;
     dim3 grid(g.nct + 1, (S->nrep + RB - 1) / RB, g.nsp);
     TileOut O = tile_out(S);
     O.bump = bump;
     set_draws(S, O, 1);
-    k_gemm_delta_bf16<<<grid, 256, 0, S->ctx->stream>>>(
-        g, S->nrep, dc, reinterpret_cast<const bf16x8 *>(S->Tb.p), S->mcur.as<double>(), O);
+    size_t lds = 0;
+    lp_ride(S, lp, grid, lds);
+    k_gemm_delta_bf16<<<grid, 256, lds, S->ctx->stream>>>(g, S->nrep, dc, reinterpret_cast<const bf16x8 *>(S->Tb.p),
+                                                          S->mcur.as<double>(), O, lp ? *lp : chm_lp_dev::LPArgs{});
     CHM_LAUNCH_CHECK();
     return CHM_OK;
+}
+
+// the learned proposal's buffers of the per-replica launch that decides step n (accept 1)
+// or proposes step n + 1 after n steps (accept 0)
+void lp_slot_args(chm_sampler *S, SlotArgs &A, int accept, long long n) {
+    chm_lp &L = S->lp;
+    if (!L.mode) return;
+    const size_t np = (size_t)S->nrep * 2 * chm_lp_dev::MAXC, nt = (size_t)S->nrep * chm_lp_dev::MAXC;
+    const long long nx = n + 1;
+    A.lp_lq = L.lq.as<double>();
+    A.lp_dmax = L.dmax.as<float>();
+    A.lp_sg = L.sg.as<float>() + (size_t)(nx & 1) * np;
+    A.lp_sdu = L.sdu.as<int>() + (size_t)(nx & 1) * np;
+    A.lp_st = L.st8.as<int8_t>() + (size_t)(nx & 1) * nt;
+    if (L.lift) {
+        // decided step n: direction before it in copy n & 1, after it in copy (n + 1) & 1;
+        // a segment's first proposal reads the direction after its n steps
+        A.lp_dir_rd = L.dir.as<int>() + (size_t)((accept == 1 ? n : nx) & 1) * S->nrep;
+        A.lp_dir_wr = L.dir.as<int>() + (size_t)(nx & 1) * S->nrep;
+    }
+    A.lp_rec = L.rec.as<int>() + (size_t)(n & 1) * chm_lp_dev::kRecInts * S->nrep;
+    A.lp_cs = L.cs.as<int>();
+    A.lp_lpn = L.lpn.as<uint64_t>();
 }
 
 // accept: 0 none, 1 step n, 2 init (n = 0).  propose: step n + 1.
 int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanchor_step = 0,
                 const double *c2_prev = nullptr) {
     SlotArgs A = slot_args(S, accept, propose);
+    if (accept != 2) lp_slot_args(S, A, accept, n);
     A.reanchor_step = reanchor_step;
     A.dc_acc = drow(S, n);
     A.dc_prop = drow(S, n + 1);
@@ -1599,24 +1736,21 @@ int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanch
     return CHM_OK;
 }
 
-// propose step h + 1 after h completed steps (nothing is accepted)
+// propose step h + 1 after h completed steps (nothing is accepted); the learned proposal
+// scores step h + 1's candidates first
 int launch_propose(chm_sampler *S, long long h) {
     if (S->lp.mode) {
-        int rc = lp_launch_score(S);
-        if (!rc) rc = lp_launch_select(S, h);
-        return rc ? rc : launch_slot(S, 0, 2, h);
+        int rc = lp_launch_tiles0(S, h);
+        return rc ? rc : launch_slot(S, 0, 4, h);
     }
     return launch_slot(S, 0, 1, h);
 }
 
 // accept step n, then (propose) propose step n + 1
 int launch_accept(chm_sampler *S, long long n, int propose, int reanchor_step = 0) {
-    if (S->lp.mode) {
-        int rc = launch_slot(S, 1, 0, n, reanchor_step);
-        return (rc || !propose) ? rc : launch_propose(S, n);
-    }
-    return launch_slot(S, 1, propose, n, reanchor_step);
+    return launch_slot(S, 1, propose ? (S->lp.mode ? 4 : 1) : 0, n, reanchor_step);
 }
+
 
 int launch_reanchor(chm_sampler *S) {
     const Geo &g = S->g;
@@ -1626,26 +1760,35 @@ int launch_reanchor(chm_sampler *S) {
     O.anchor_min = S->reanchor_every;
     O.anchor_out = S->mcur.as<double>();
     k_gemm_abs_f64<<<grid, 256, lds_gemm(g), S->ctx->stream>>>(
-        g, S->nrep, S->counts.as<int>(), nullptr, S->basisR.as<double>(), O);
+        g, S->nrep, S->counts.as<int>(), nullptr, S->basisR.as<double>(), O, chm_lp_dev::LPArgs{});
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
 
-int launch_gemm_delta32(chm_sampler *S, const int *dc, int bump) {
+int launch_gemm_delta32(chm_sampler *S, const int *dc, int bump, const chm_lp_dev::LPArgs *lp = nullptr) {
     const Geo &g = S->g;
     dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
     TileOut O = tile_out(S);
     O.bump = bump;
-    const size_t lds = std::max(sizeof(float) * (size_t)g.nrow_e * (RB + 1), sizeof(float) * (size_t)KS * RB * NCW);
+    size_t lds = std::max(sizeof(float) * (size_t)g.nrow_e * (RB + 1), sizeof(float) * (size_t)KS * RB * NCW);
+    lp_ride(S, lp, grid, lds);
     k_gemm_delta_f32<<<grid, 256, lds, S->ctx->stream>>>(g, S->nrep, dc, S->basisR32.as<float>(),
-                                                          S->mcur.as<double>(), O);
+                                                          S->mcur.as<double>(), O, lp ? *lp : chm_lp_dev::LPArgs{});
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
 
-int launch_gemm(chm_sampler *S, const int *dc, int bump) {
-    if (S->incremental == 2) return launch_gemm_delta32(S, dc, bump);
-    return S->incremental ? launch_gemm_delta(S, dc, bump) : launch_gemm_abs(S, dc, bump);
+int launch_gemm(chm_sampler *S, const int *dc, int bump, const chm_lp_dev::LPArgs *lp = nullptr) {
+    if (S->incremental == 2) return launch_gemm_delta32(S, dc, bump, lp);
+    return S->incremental ? launch_gemm_delta(S, dc, bump, lp) : launch_gemm_abs(S, dc, bump, lp);
+}
+
+// the transform of step n; with the learned proposal, the same launch carries the scoring
+// workgroups (step n's reverse probability, step n+1's candidates) and the folding ones
+int launch_trial(chm_sampler *S, long long n, bool more) {
+    if (!S->lp.mode) return launch_gemm(S, drow(S, n), 1);
+    const chm_lp_dev::LPArgs lp = lp_step_args(S, more ? 1 : 2, n);
+    return launch_gemm(S, drow(S, n), 1, &lp);
 }
 
 // ------------------------------------------------ paired steps
@@ -1773,7 +1916,7 @@ int launch_segment(chm_sampler *S, long long h, int L) {
     if (!rc && anchor) rc = launch_reanchor(S);
     for (int t = 0; t < L && !rc; ++t) {
         const long long n = h + 1 + t;
-        rc = launch_gemm(S, drow(S, n), 1);
+        rc = launch_trial(S, n, t + 1 < L);
         if (!rc) rc = launch_accept(S, n, t + 1 < L ? 1 : 0, anchor && t == 0);
     }
     return rc;
@@ -2174,6 +2317,7 @@ extern "C" int chm_sampler_init(chm_sampler *S, const int32_t *species0, const d
     S->seed = seed;
     S->hstep = 0;
     S->rec_base = 0;
+    S->lp.cache_ok = false;   // new species: the learned proposal's per-site sums are stale
     int nsplit = full_counts_split(g, nrep);
     k_full_counts<<<dim3(nsplit, nrep), 256, S->lds_k1, st>>>(
         g, S->species.as<uint8_t>(), S->site_xyz.as<int>(), S->nbr_j.as<int>(), S->nbr_sh.as<int>(),
@@ -2194,6 +2338,7 @@ extern "C" int chm_sampler_run(chm_sampler *S, int nsteps, int exchange_every) {
     hipStream_t st = S->ctx->stream;
     const int E = exchange_every;
     int rc = 0;
+    if (S->lp.mode && (rc = lp_ensure_cache(S))) return rc;
     if (E > 0 && S->graph_steps != E) {
         // one exchange period per graph: propose, E x (transform, accept [+ propose]),
         // exchange.  The count-change rows alternate by step parity, so a period that
@@ -2365,11 +2510,19 @@ extern "C" int chm_sampler_read(chm_sampler *S, int what, void *host, int64_t nb
             return CHM_OK;
         }
         case 15: {
-            const int64_t n = (int64_t)4 * nrep * S->lp.ncand;
+            // the copy the next scoring launch reads (it lacks the last step's swap, 16)
+            CHM_REQUIRE(S->lp.cache_ok, CHM_E_STATE, "the learned proposal's per-site sums are not built");
+            const int64_t n = (int64_t)4 * nrep * 2 * g.natoms * 256;
             if ((rc = need(n))) return rc;
-            std::vector<float> v((size_t)nrep * S->lp.ncand);
-            CHM_HIP(hipMemcpy(v.data(), S->lp.logit.p, n, hipMemcpyDeviceToHost));
-            std::memcpy(host, v.data(), n);
+            const char *c = static_cast<const char *>(S->lp.cache.p) + (size_t)((S->hstep + 1) & 1) * n;
+            CHM_HIP(hipMemcpy(host, c, n, hipMemcpyDeviceToHost));
+            return CHM_OK;
+        }
+        case 16: {
+            const int64_t n = (int64_t)4 * nrep * chm_lp_dev::kRecInts;
+            if ((rc = need(n))) return rc;
+            const char *r = static_cast<const char *>(S->lp.rec.p) + (size_t)(S->hstep & 1) * n;
+            CHM_HIP(hipMemcpy(host, r, n, hipMemcpyDeviceToHost));
             return CHM_OK;
         }
     }
@@ -2545,7 +2698,7 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
     CHM_HIP(hipEventRecord(e0, st));
     for (int i = 0; i < iters && !rc; ++i) {
         const long long n = h + 1 + i;
-        rc = launch_gemm(S, drow(S, n), 1);
+        rc = launch_trial(S, n, true);
         if (!rc) rc = launch_accept(S, n, 1);
     }
     CHM_HIP(hipEventRecord(e1, st));
@@ -2556,7 +2709,7 @@ extern "C" int chm_sampler_profile(chm_sampler *S, int iters, double *ms) {
     ms[2] = ms[3] - ms[1];                   // accept + propose + gather per step
     ms[0] = 0.0;
     const long long n = h + 1 + iters;       // settle: transform and accept the last proposal
-    if (!rc) rc = launch_gemm(S, drow(S, n), 1);
+    if (!rc) rc = launch_trial(S, n, false);
     if (!rc) rc = launch_accept(S, n, 0);
     if (rc) return rc;
     S->hstep = n;
@@ -2832,6 +2985,7 @@ extern "C" int chm_sampler_import(chm_sampler *S, const void *host, int64_t nbyt
     S->seed = h.seed;
     S->rec_base = h.hstep;
     S->inited = true;
+    S->lp.cache_ok = false;   // new species: the learned proposal's per-site sums are stale
     return CHM_OK;
 }
 

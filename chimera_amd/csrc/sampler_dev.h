@@ -35,6 +35,14 @@ __device__ __forceinline__ uint64_t lp_stream(uint64_t seed, uint64_t step, int 
 __device__ __forceinline__ bool lp_coin_uniform(uint64_t h0, double beta, double eps, double beta_max) {
     return u01(splitmix64(h0 ^ kCoinSalt)) < eps || beta > beta_max;
 }
+// candidate pair i of a step's candidate set (stream h0), and the uniform that selects one
+constexpr uint64_t kSelSalt = 0x14057B7EF767814Full;
+constexpr uint64_t kCandMul = 0xD6E8FEB86659FD93ull;
+__device__ __forceinline__ void lp_candidate(uint64_t h0, int i, int natoms, int &a, int &b) {
+    const uint64_t h = splitmix64(h0 ^ ((uint64_t)(i + 1) * kCandMul));
+    a = (int)(((h & 0xFFFFFFFFull) * (uint64_t)natoms) >> 32);
+    b = (int)(((h >> 32) * (uint64_t)natoms) >> 32);
+}
 
 // Metropolis test of methods/method_a_rmc.py:280-285 for x = beta * d(chi^2) * nfit (minus
 // the log proposal ratio): accept if x <= 0 or u < exp(-min(x, 60)).  A NaN x is rejected,

@@ -12,18 +12,31 @@ struct chm_lp {
     double lam = 1.0;        // logit = -lam * beta * g / 2 + kappa * |dU|
     double kappa = 0.0;
     double beta_max = 1e300;  // slots at a larger beta (colder) use the uniform proposal
+    signed char wpair[64] = {};  // dU = sum of w[s][t] changes over first-shell bonds (8 x 8)
     int lift = 0;            // lifted: logit tilt kappa * d * dU, d = dir[slot] flips when a
-    chm::DevBuf dir;         //   learned step is not accepted (nrep ints, +-1)
+    chm::DevBuf dir;         //   learned step is not accepted (2 x nrep ints, +-1: before /
+                             //   after a step, alternating by step)
     int nocc = 0, noccp = 0; // occupancy entries per candidate (2 ncoord + 2), padded to 16
     int nfeat = 0, kp = 0;   // one-hot features nsp * nocc, padded to 32
     int nparam = 0;          // flat FP32 parameters
     int r2max = 0;           // largest squared stencil offset, units (a0/2)^2
-    int shell0 = 0;          // stencil shell index of the first shell
+    int pm_r = 0;            // half-side of the offset -> stencil position cube
+    signed char opp[128] = {};   // stencil position of the opposite offset of each position
     bool have_params = false;
     chm::DevBuf params, adam_m, adam_v;          // FP32 masters and Adam moments
-    chm::DevBuf w1t, w1, w2t, w2, w3q;           // bf16 copies in both orders; w3 rounded
-    chm::DevBuf logit;                           // (nrep, ncand) scores of the pending step
-    chm::DevBuf gcand, ducand;                   // (nrep, ncand) network output and dU of each
+    chm::DevBuf w1, w2t, w2, w3q;                // bf16 copies (W2 in both orders); w3 rounded
+    chm::DevBuf w1q;                             // scorer's layer 1: int32 in units of 2^-16
+    chm::DevBuf posmap;                          // int8 stencil position of an offset, or -1
+    // two copies of the per-site layer-1 sums (2 x nrep x 2 x natoms x 256 int32, valid when
+    // cache_ok) and the swap record of each step (2 x nrep x 8 ints, by step parity)
+    chm::DevBuf cache, rec;
+    bool cache_ok = false;
+    // speculative scoring: g / dU of the pending step's candidates (2 parities x nrep x 2 x
+    // 256; [.][.][1] with the previous swap made) and their touched flags (2 x nrep x 256),
+    // the outcome of the last step, per-tile partial sums of the log q ratio (nrep x 8 x 6)
+    // and tile maxima of the next logits (nrep x 8 x 2)
+    chm::DevBuf sg, sdu, st8, cs, lq, dmax;
+    chm::DevBuf lpn;     // (nrep) u64: the step the next transform launch's scoring serves
     chm::DevBuf pend_occ;                        // (nrep, noccp) occupancy of each pending proposal
     chm::DevBuf rec_occ, rec_y, rec_meta;        // move-record ring
     int rec_cap = 0;
@@ -87,10 +100,11 @@ inline double *c2row(chm_sampler *S, long long n) {
     return S->c2.as<double>() + (size_t)(n & 1) * S->nrep;
 }
 
-// learned.hip: score the candidates of step n + 1 (after step n was accepted), then
-// select one and compute its Hastings ratio into prow(n + 1) (the slot kernel gathers it)
-int lp_launch_score(chm_sampler *S);
-int lp_launch_select(chm_sampler *S, long long n);
+// learned.hip: the per-site layer-1 sums (built when stale), a segment's first scoring
+// launch after h steps, and the dynamic LDS of a scoring workgroup
+int lp_ensure_cache(chm_sampler *S);
+int lp_launch_tiles0(chm_sampler *S, long long h);
+size_t lp_tile_lds(const chm_sampler *S);
 int lp_prepare(chm_sampler *S);   // buffers for the current geometry (at create)
 void drop_graph(chm_sampler *S);  // sampler.hip: forget the captured step graphs
 // sampler.hip: lattice geometry from the host description (sites checked), and the check

@@ -15,7 +15,8 @@ the calls:
 
 Network: one-hot species occupancy of both candidate sites' stencil neighbours and of
 the two sites (nocc = 2 ncoord + 2 entries) -> 256 -> 256 -> 1 with SiLU, bf16 weights,
-FP32 accumulation.  It predicts the swap's energy change nfit * dchi2; a candidate's
+FP32 accumulation; the scorer's layer-1 weights are bf16(W1) on a 2^-16 grid, so each
+site's layer-1 sum is an exact integer the device keeps per replica and updates per move.  It predicts the swap's energy change nfit * dchi2; a candidate's
 logit is -lam * beta * g / 2 + kappa * |dU| (dU: change of unlike first-shell bonds), so
 with a perfect predictor the proposal is locally balanced and the acceptance tends to 1.
 The posterior is exact for any parameters: the acceptance carries the Hastings ratio of
@@ -106,6 +107,14 @@ class LearnedProposal:
         check(_lib.load().chm_lp_configure(self.S._S, self.mode, self.ncand, self.eps, self.lam,
                                            self.kappa, self.beta_max, int(self.lift)))
 
+    def set_tilt_weights(self, w):
+        """Bond weights (nsp, nsp) of the tilt's count: dU = the change of
+        sum_bonds w[s_i, s_j] over the first-shell bonds a swap changes (integers in
+        [-63, 63], symmetric).  The default, w[s, t] = [s != t], counts unlike bonds."""
+        w = np.ascontiguousarray(w, np.int32)
+        self.tilt_w = w.copy()
+        check(_lib.load().chm_lp_set_tilt_weights(self.S._S, w.ctypes.data, int(w.shape[0])))
+
     def disable(self):
         self.enable(False)
 
@@ -154,19 +163,29 @@ class LearnedProposal:
         return float(loss[0]), g
 
     def profile(self, iters=100):
-        """Device ms per launch of the scorer and of the selection kernel (HIP events)."""
+        """Device ms per launch (HIP events) of one step's scoring workgroups (every slot
+        learned) and of its folding workgroups, each launched alone (in a run they ride in
+        the transform's launch)."""
         ms = np.zeros(2)
         check(_lib.load().chm_lp_profile(self.S._S, int(iters), ms.ctypes.data))
-        return {"score_ms": float(ms[0]), "select_ms": float(ms[1])}
+        return {"score_ms": float(ms[0]), "fold_ms": float(ms[1])}
 
-    def logits(self):
-        """Logits (nrep, ncand) of the candidates scored last, slot order."""
-        return self.S._read(15, (self.S.nrep, self.ncand), np.float32)
+    def site_sums(self):
+        """(sums, lag): the copy of the per-site layer-1 sums the next step reads
+        (nrep, 2, natoms, 256) int32 in units of 2^-16, slot order (role 0: the site as
+        candidate a, 1: as b), and the swap it lacks per slot (nrep, 8): valid, a, b, old
+        species of a, old species of b (the last step's, if it was accepted)."""
+        S = self.S
+        sums = S._read(15, (S.nrep, 2, S.natoms, 256), np.int32)
+        lag = S._read(16, (S.nrep, 8), np.int32)
+        return sums, lag
 
     def _reapply(self, sampler):
         """A new device sampler (new composition): restore parameters, records, mode."""
         self.set_params(self._params)
         if self.rec_cap:
             self.record(self.rec_cap)
+        if getattr(self, "tilt_w", None) is not None:
+            self.set_tilt_weights(self.tilt_w)
         if self.mode:
             self.enable()

@@ -241,8 +241,9 @@ int chm_sampler_set_recording(chm_sampler *s, int log_capacity, int series_capac
  *   11 chi(k) (nrep, nsp, nk) double, rung order   12 trial chi(R) of the last step
  *      (nrep, nsp*nkw*nwin*2) double, slot order     13 running chi(R) accumulator, same
  *      layout (incremental mode)                      14 FP64 re-anchors done (nrep) int64,
- *      slot order                                     15 learned-proposal logits of the
- *      candidates last scored (nrep, ncand) float, slot order
+ *      slot order                                     15 learned proposal's per-site
+ *      layer-1 sums (nrep, 2, natoms, 256) int32 (units 2^-16), slot order, which lack
+ *      the swap in 16 (nrep, 8) int32 [valid, a, b, old species of a, of b]
  * RMCSampler attributes species/chi2/chi/best (method_a_rmc.py:166-184, 493-496). */
 int chm_sampler_read(chm_sampler *s, int what, void *host, int64_t nbytes);
 
@@ -358,6 +359,10 @@ int chm_lp_get_params(chm_sampler *s, float *flat);
  * configurations is unchanged). */
 int chm_lp_configure(chm_sampler *s, int mode, int ncand, double eps, double lam, double kappa,
                      double beta_max, int lift);
+/* Weights of the tilt's bond count: dU = sum over the first-shell bonds a swap changes of
+ * w[s][t] (new pair) - w[s][t] (old pair); w is nsp x nsp, symmetric, integers in
+ * [-63, 63].  The default is the unlike-bond indicator (w[s][t] = 1 for s != t). */
+int chm_lp_set_tilt_weights(chm_sampler *s, const int32_t *w, int nsp);
 /* Move records: a ring of `capacity` (occupancy, nfit * dchi2, rung | accepted << 16)
  * entries written by every step (null moves: meta -1).  0 disables. */
 int chm_lp_set_records(chm_sampler *s, int capacity);

@@ -114,9 +114,23 @@ def one_hot(occ, nsp, kp):
     return X
 
 
-def forward(flat, occ, nsp, rounding=True):
+W1_SCALE = 65536.0          # the scorer's layer-1 weights are integers in units of 2^-16
+W1_QMAX = 131072.0          # ... clamped to |w| <= 2, so a sum of <= 110 stays below 2^24
+
+
+def w1_fixed(W1):
+    """The scorer's layer-1 weights: bf16(W1) rounded (half to even) to a multiple of
+    2^-16 and clamped to [-2, 2].  Sums of them over one-hot features are exact in FP64 and,
+    as integers below 2^24, in FP32: the device keeps per-site layer-1 sums incrementally
+    and still gets the same value as a sum from scratch."""
+    return np.clip(np.rint(np.asarray(W1, np.float64) * W1_SCALE), -W1_QMAX, W1_QMAX) / W1_SCALE
+
+
+def forward(flat, occ, nsp, rounding=True, scorer=False):
     """g (n,) and the intermediates, FP64 with the bf16 roundings of the device
-    (rounding=False: the same network without them, for finite differences)."""
+    (rounding=False: the same network without them, for finite differences).
+    scorer=True: layer 1 with the scorer's fixed-point weights (w1_fixed), the network
+    that scores candidate swaps; the trainer differentiates the bf16 one."""
     occ = np.asarray(occ, np.int64)
     nfeat = occ.shape[1] * nsp
     p = unpack(flat, nfeat)
@@ -124,6 +138,8 @@ def forward(flat, occ, nsp, rounding=True):
     X = one_hot(occ, nsp, kp)
     rnd = bf16 if rounding else (lambda v: np.asarray(v, np.float64))
     W1, W2, w3 = rnd(p["W1"]), rnd(p["W2"]), rnd(p["w3"])
+    if scorer:
+        W1 = w1_fixed(W1)
     Z1 = X @ W1 + p["b1"]
     H1 = rnd(silu(Z1))
     Z2 = H1 @ W2 + p["b2"]
@@ -203,16 +219,26 @@ def occupancy(sp, st: Stencil, a, b):
     return np.array(occ, np.int64), na, nb
 
 
-def unlike_change(sp, st: Stencil, a, b, na, nb):
-    """Change of the number of unlike first-shell bonds when a and b swap."""
+def unlike_weights(nsp):
+    """The default bond weights of the tilt's count: 1 for an unlike pair, 0 for a like one."""
+    return 1 - np.eye(nsp, dtype=np.int64)
+
+
+def unlike_change(sp, st: Stencil, a, b, na, nb, w=None):
+    """Change of sum_bonds w[s_i, s_j] over the first-shell bonds when a and b swap
+    (w: symmetric bond weights; default: the number of unlike bonds).  The a-b bond keeps
+    its pair and is skipped."""
     A, B = int(sp[a]), int(sp[b])
+    if w is None:
+        w = unlike_weights(max(int(sp.max()), A, B) + 1)
+    w = np.asarray(w, np.int64)
     d = 0
     for q, s in enumerate(na):
         if st.shell[q] == 0 and s != b:
-            d += int(sp[s] == A) - int(sp[s] == B)
+            d += int(w[B, sp[s]]) - int(w[A, sp[s]])
     for q, s in enumerate(nb):
         if st.shell[q] == 0 and s != a:
-            d += int(sp[s] == B) - int(sp[s] == A)
+            d += int(w[A, sp[s]]) - int(w[B, sp[s]])
     return d
 
 
@@ -248,15 +274,16 @@ def tilt(kappa, du, d=None):
     return kappa * np.abs(du) if d is None else kappa * d * du
 
 
-def logits(flat, sp, st, cands, nsp, beta, lam, kappa, d=None):
+def logits(flat, sp, st, cands, nsp, beta, lam, kappa, d=None, w=None):
     occs, du, valid = [], [], []
+    w = unlike_weights(nsp) if w is None else w
     for a, b in cands:
         occ, na, nb = occupancy(sp, st, a, b)
         occs.append(occ)
-        du.append(unlike_change(sp, st, a, b, na, nb))
+        du.append(unlike_change(sp, st, a, b, na, nb, w))
         valid.append(sp[a] != sp[b])
     occs = np.array(occs)
-    g, _ = forward(flat, occs, nsp)
+    g, _ = forward(flat, occs, nsp, scorer=True)
     lg = -lam * beta * g / 2.0 + tilt(kappa, du, d)
     lg = np.where(np.array(valid), lg, -np.inf)
     return lg, g, occs
@@ -283,29 +310,29 @@ def log_q(lg, same):
     return float(np.log(p[same].sum()) - np.log(p.sum()))
 
 
-def log_q_ratio(flat, sp, st, nsp, cands, lg, j, beta, lam, kappa, d=None):
+def log_q_ratio(flat, sp, st, nsp, cands, lg, j, beta, lam, kappa, d=None, w=None):
     """log q(x' -> x) - log q(x -> x') for candidate j of the set `cands` at state sp
     (lg: the logits at sp; a lifted proposal in direction d is reversed with -d)."""
     a, b = cands[j]
     same = np.array([{c, e} == {a, b} for c, e in cands])
     sp2 = sp.copy()
     sp2[a], sp2[b] = sp[b], sp[a]
-    lg2, _, _ = logits(flat, sp2, st, cands, nsp, beta, lam, kappa, None if d is None else -d)
+    lg2, _, _ = logits(flat, sp2, st, cands, nsp, beta, lam, kappa, None if d is None else -d, w)
     return log_q(lg2, same) - log_q(lg, same)
 
 
-def propose_learned(flat, sp, st, nsp, seed, stepc, key, beta, ncand, lam, kappa, d=None):
+def propose_learned(flat, sp, st, nsp, seed, stepc, key, beta, ncand, lam, kappa, d=None, w=None):
     """(a, b, na, nb, sw, logqr, occ_j, margin) of the learned proposal at state sp
     (margin: how far the selecting uniform fell from a bin edge, relative; d: the
-    direction of a lifted proposal)."""
+    direction of a lifted proposal; w: the tilt's bond weights)."""
     natoms = sp.size
     cands = candidates(seed, stepc, key, natoms, ncand)
-    lg, _, occs = logits(flat, sp, st, cands, nsp, beta, lam, kappa, d)
+    lg, _, occs = logits(flat, sp, st, cands, nsp, beta, lam, kappa, d, w)
     j, margin = select(lg, select_uniform(seed, stepc, key), with_margin=True)
     if j < 0:
         return 0, 0, int(sp[0]), int(sp[0]), 0, 0.0, None, 1.0
     a, b = cands[j]
-    logqr = log_q_ratio(flat, sp, st, nsp, cands, lg, j, beta, lam, kappa, d)
+    logqr = log_q_ratio(flat, sp, st, nsp, cands, lg, j, beta, lam, kappa, d, w)
     return a, b, int(sp[b]), int(sp[a]), 1, logqr, occs[j], margin
 
 
@@ -315,7 +342,7 @@ class LearnedSampler:
     the checker of the device sampler in learned mode."""
 
     def __init__(self, plain, flat, stencil, ncand=256, eps=0.0, lam=1.0, kappa=0.0, beta_max=1e300,
-                 lift=False):
+                 lift=False, tilt_w=None):
         self.S = plain
         self.flat = np.asarray(flat, np.float32)
         self.st = stencil
@@ -323,6 +350,7 @@ class LearnedSampler:
         self.beta_max = beta_max
         self.margins = []
         self.lift = bool(lift)
+        self.tilt_w = None if tilt_w is None else np.asarray(tilt_w, np.int64)
         # directions start alternating over the chains (d travels with its configuration)
         self.dir = np.where(np.arange(plain.nrep) % 2 == 1, -1, 1)
 
@@ -342,7 +370,7 @@ class LearnedSampler:
             else:
                 a, b, na, nb, sw, lq, _, margin = propose_learned(
                     self.flat, sp, self.st, P.nsp, S.seed, S.stepc, c, S.beta[c], self.ncand,
-                    self.lam, self.kappa, int(self.dir[c]) if self.lift else None)
+                    self.lam, self.kappa, int(self.dir[c]) if self.lift else None, self.tilt_w)
             self.margins.append((S.stepc, c, margin))
             if not sw:
                 if self.lift and learned:

@@ -62,7 +62,7 @@ def test_network_output_matches_oracle(tern108):
     occ = rng.integers(0, P.nsp, (100, lp.nocc)).astype(np.uint8)
     _say(f"[test] MFMA scorer on {occ.shape} occupancy rows vs oracle")
     g_dev = lp.eval(occ).astype(np.float64)
-    g_or, _ = LP.forward(flat, occ, P.nsp)
+    g_or, _ = LP.forward(flat, occ, P.nsp, scorer=True)
     err = np.abs(g_dev - g_or)
     assert np.median(err) < 1e-4 * max(1.0, np.abs(g_or).max())
     assert np.all(err <= 3e-3 * np.maximum(1.0, np.abs(g_or)))
@@ -130,13 +130,18 @@ def test_move_records_match_oracle(tern108):
     assert not bad, bad[:12]
 
 
-@pytest.mark.parametrize("lift", [False, True])
-def test_learned_trajectory_matches_oracle(tern108, lift):
+_TILT_W = np.array([[0, 1, -2], [1, 0, 3], [-2, 3, 0]])
+
+
+@pytest.mark.parametrize("lift,tilt_w", [(False, None), (True, None), (True, _TILT_W)])
+def test_learned_trajectory_matches_oracle(tern108, lift, tilt_w):
     """Device sampler with the learned proposal (eps 0.3 uniform mixture, tilt on, the two
-    coldest rungs uniform; symmetric or lifted tilt) move by move against the oracle's
-    restatement with the same parameters.  A differing choice
-    is allowed only at a near tie of the selecting uniform or of the acceptance; the
-    comparison stops there."""
+    coldest rungs uniform; symmetric or lifted tilt, unlike-bond or other bond weights)
+    move by move against the oracle's restatement with the same parameters: the
+    speculative scoring (both outcomes of the pending step), the per-site layer-1 sums and
+    their lag reproduce the proposal drawn from the state each step reaches.  A differing
+    choice is allowed only at a near tie of the selecting uniform or of the acceptance;
+    the comparison stops there."""
     P, O = tern108
     S = P.sampler
     lp = _lp(P)
@@ -146,15 +151,17 @@ def test_learned_trajectory_matches_oracle(tern108, lift):
     S.seed = seed
     S.set_recording(log_steps=nsteps)
     S.init_state(P.cfg0, P.cell.ideal)
+    lp.set_tilt_weights(LP.unlike_weights(P.nsp) if tilt_w is None else tilt_w)
     lp.enable(ncand=64, eps=0.3, lam=1.0, kappa=0.2, beta_max=10.0, lift=lift)
-    _say(f"[run] device learned {nsteps} steps lift={lift}")
+    _say(f"[run] device learned {nsteps} steps lift={lift} weights={tilt_w is not None}")
     S.run(nsteps, exchange_every=exch, log_every=10 ** 9)
     log = S.move_log(nsteps)
     lp.disable()
+    lp.set_tilt_weights(LP.unlike_weights(P.nsp))
     R = OR.PlainSampler(O, nrep=S.nrep, seed=seed)
     R.init_state(P.cfg0)
     L = LP.LearnedSampler(R, flat, _stencil(P), ncand=64, eps=0.3, lam=1.0, kappa=0.2,
-                          beta_max=10.0, lift=lift)
+                          beta_max=10.0, lift=lift, tilt_w=tilt_w)
     _say("[run] oracle learned")
     L.run(nsteps, exchange_every=exch)
     margins = {(s, c): m for s, c, m in L.margins}
@@ -175,6 +182,51 @@ def test_learned_trajectory_matches_oracle(tern108, lift):
     assert learned_steps > nsteps * S.nrep // 3
     assert np.array_equal(S.species, R.species)
     np.testing.assert_allclose(S.chi2, R.chi2, rtol=1e-9)
+
+
+def _scratch_sums(P, lp, flat, sp):
+    """Per-site layer-1 sums (2, natoms, 256) of one configuration from scratch, in units
+    of 2^-16, with the oracle's fixed-point weights."""
+    kp, off, _ = LP.layout(lp.nfeat)
+    o, m = off["W1"]
+    W = np.rint(LP.w1_fixed(LP.bf16(flat[o:o + m].reshape(kp, LP.HID))) * 65536.0).astype(np.int64)
+    st = _stencil(P)
+    nc, nsp = st.ncoord, P.nsp
+    out = np.zeros((2, sp.size, LP.HID), np.int64)
+    for c in range(sp.size):
+        nb = st.neighbours(c)
+        for role in (0, 1):
+            f = [(role * nc + q) * nsp + int(sp[x]) for q, x in enumerate(nb)]
+            f.append((2 * nc + role) * nsp + int(sp[c]))
+            out[role, c] = W[f].sum(0)
+    return out
+
+
+def test_site_sums_follow_the_chain(tern108):
+    """After a learned run with exchanges, every slot's per-site layer-1 sums kept by
+    increments equal the sums from scratch at that slot's configuration with the pending
+    (lag) swap undone, exactly."""
+    P, O = tern108
+    S = P.sampler
+    lp = _lp(P)
+    flat = _random_params(lp, 9, w3_scale=0.05)
+    lp.set_params(flat)
+    S.seed = 4
+    S.init_state(P.cfg0, P.cell.ideal)
+    lp.enable(ncand=64, eps=0.2, lam=1.0, kappa=0.3, beta_max=10.0, lift=True)
+    _say("[run] device learned 60 steps, exchange every 7")
+    S.run(60, exchange_every=7, log_every=10 ** 9)
+    sums, lag = lp.site_sums()
+    lp.disable()
+    rung_of = S._read(5, (S.nrep,), np.int32)
+    sp_rung = S.species
+    for s in range(S.nrep):
+        sp = sp_rung[rung_of[s]].copy()
+        if lag[s, 0]:
+            a, b, oa, ob = lag[s, 1:5]
+            sp[a], sp[b] = oa, ob
+        want = _scratch_sums(P, lp, flat, sp)
+        np.testing.assert_array_equal(sums[s].astype(np.int64), want, err_msg=f"slot {s}")
 
 
 def test_training_reduces_loss(tern108):

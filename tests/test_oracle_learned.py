@@ -193,3 +193,64 @@ def test_candidate_stream_is_uniform_and_state_free():
     a = np.array([x for x, _ in LP.candidates(1, 1, 0, 1000, 20000)])
     counts = np.bincount(a // 100, minlength=10)
     assert counts.min() > 1800 and counts.max() < 2200
+
+
+def test_scorer_layer1_weights_fixed_point():
+    """The scorer's layer-1 weights: bf16 values of magnitude >= 2^-9 are already on the
+    2^-16 grid and stay unchanged; every value is a multiple of 2^-16 within [-2, 2]; a sum
+    of 110 of them is exact in FP32 (below 2^24 grid units), so a per-site sum kept by
+    increments equals one formed from scratch."""
+    rng = np.random.default_rng(5)
+    w = LP.bf16(rng.standard_normal(20000) * 10.0 ** rng.uniform(-5, 0.5, 20000))
+    q = LP.w1_fixed(w)
+    assert np.all(q * 65536.0 == np.rint(q * 65536.0))
+    assert np.abs(q).max() <= 2.0
+    big = (np.abs(w) >= 2.0 ** -9) & (np.abs(w) <= 2.0)
+    np.testing.assert_array_equal(q[big], w[big])
+    small = np.abs(w) < 2.0 ** -9
+    assert np.all(np.abs(q[small] - w[small]) <= 2.0 ** -17)
+    assert LP.w1_fixed(np.array([3.5, -7.0])).tolist() == [2.0, -2.0]
+    # half a grid step rounds to even
+    assert LP.w1_fixed(np.array([2.0 ** -17, 3 * 2.0 ** -17])).tolist() == [0.0, 2.0 ** -15]
+    rows = q.reshape(-1, 100)[:, :110 if q.size >= 110 else 100]
+    for r in rows[:50]:
+        pick = rng.choice(r.size, min(110, r.size), replace=False)
+        exact = r[pick].sum()
+        f32 = np.float32(0.0)
+        for v in r[pick]:
+            f32 = np.float32(f32 + np.float32(v))
+        assert float(f32) == exact
+
+
+def test_weighted_bond_change_equals_recount():
+    """dU of a swap with symmetric bond weights w equals the change of
+    sum over first-shell bonds of w[s_i, s_j] recounted over the whole configuration
+    (including swaps of two first-shell neighbours), and with the default weights it is
+    the change of the number of unlike bonds."""
+    cell = OL.Supercell("fcc", (3, 3, 3), 3.6)
+    i, j, sh = cell.neighbour_list(3.6 * 1.01)
+    d = np.linalg.norm(cell.ideal[j] + sh - cell.ideal[i], axis=1)
+    shell = np.where(d < 3.6 * 0.75, 0, 1)
+    st = LP.Stencil(cell.ideal, i, j, sh, shell, cell.box)
+    rng = np.random.default_rng(7)
+    nsp = 3
+    first = [[s for s, q in zip(st.neighbours(x), range(st.ncoord)) if st.shell[q] == 0]
+             for x in range(cell.natoms)]
+
+    def total(sp, w):
+        return sum(int(w[sp[x], sp[y]]) for x in range(cell.natoms) for y in first[x]) // 2
+
+    for trial in range(30):
+        sp = rng.integers(0, nsp, cell.natoms)
+        W = rng.integers(-5, 6, (nsp, nsp))
+        W = W + W.T
+        a = int(rng.integers(cell.natoms))
+        b = int(first[a][rng.integers(len(first[a]))]) if trial % 3 == 0 else int(rng.integers(cell.natoms))
+        if sp[a] == sp[b]:
+            continue
+        _, na, nb = LP.occupancy(sp, st, a, b)
+        sp2 = sp.copy()
+        sp2[a], sp2[b] = sp[b], sp[a]
+        assert LP.unlike_change(sp, st, a, b, na, nb, W) == total(sp2, W) - total(sp, W)
+        U = LP.unlike_weights(nsp)
+        assert LP.unlike_change(sp, st, a, b, na, nb) == total(sp2, U) - total(sp, U)
