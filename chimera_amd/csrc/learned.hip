@@ -306,6 +306,8 @@ LPArgs lp_args(chm_sampler *S) {
     A.posmap = L.posmap.as<signed char>();
     A.pm_r = L.pm_r;
     for (int q = 0; q < 128; ++q) A.opp[q] = L.opp[q];
+    for (int q = 0; q < kMaxFs; ++q) A.fs[q] = L.fs[q];
+    A.n1 = L.n1;
     A.seed = S->seed;
     A.ncand = L.ncand;
     A.nrep = S->nrep;
@@ -446,6 +448,9 @@ int lp_prepare(chm_sampler *S) {
         n1 += ((ss[q] >> 24) & 0xFF) == 0;
     }
     if (n1 > kMaxFs) return chm::set_error("learned proposal: too many first-shell neighbours"), CHM_E_ARG;
+    L.n1 = 0;
+    for (int q = 0; q < g.ncoord; ++q)
+        if (((ss[q] >> 24) & 0xFF) == 0) L.fs[L.n1++] = (signed char)q;
     // the opposite offset of every stencil position (the stencil of a Bravais lattice holds
     // both)
     for (int q = 0; q < g.ncoord; ++q) {

@@ -77,6 +77,8 @@ struct LPArgs {
     const signed char *posmap; // stencil position of an offset (cube of side 2 pm_r + 1), or -1
     int pm_r;
     signed char opp[128];      // stencil position of the opposite offset of each position
+    signed char fs[kMaxFs];    // the first-shell stencil positions (n1 of them)
+    int n1;
     uint64_t seed;
     int ncand, nrep, r2max;
     double eps, lam, kappa, beta_max;
@@ -198,6 +200,12 @@ __device__ inline void layer2(const LPW &W, const __bf16 *H, int nrt, float *gp,
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float b2[4], w3[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        b2[c] = W.b2[64 * wave + 16 * c + lr];
+        w3[c] = W.w3q[64 * wave + 16 * c + lr];
+    }
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
         bf16x8 B[4][4];
@@ -219,12 +227,6 @@ __device__ inline void layer2(const LPW &W, const __bf16 *H, int nrt, float *gp,
                     acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, B[k][c], acc[r][c], 0, 0, 0);
             }
         }
-    }
-    float b2[4], w3[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        b2[c] = W.b2[64 * wave + 16 * c + lr];
-        w3[c] = W.w3q[64 * wave + 16 * c + lr];
     }
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
@@ -267,25 +269,52 @@ __device__ __forceinline__ void h1_store(const LPW &W, __bf16 *H, int r, int ch,
 }
 
 // Rows of one network pass, in LDS: candidate (a, b) at the committed state (v = 0) or with
-// the proposal swapped (v = 1); per row the layer-1 corrections, dU and g.
+// the proposal swapped (v = 1); dU and g per row, and the flattened list of layer-1
+// corrections (row, new weight row, old weight row).
 struct Rows {
     int a[MP], b[MP], v[MP];
-    int2 corr[MP][kNCorr];
-    int ncorr[MP];
     int du[MP];
     float gout[MP];
     float gp[TW * MP];
+    int crow[MP * kNCorr], cnew[MP * kNCorr], cold[MP * kNCorr];
+    int ncl;
 };
 
 // g and dU of rows[0, nrows) (nrows <= MP) of slot s.  The copy `cache_rd` holds the
 // configuration before the lag swap; the committed species (lsp) have it made; rows with
 // v = 1 also have the proposal (pa, pb) made.  fs: first-shell stencil positions (n1); wl:
-// bond weights; pm: offset -> stencil position cube.
+// bond weights; pm: offset -> stencil position cube; zl: MP x HID int32 of LDS (the layer-1
+// sums; H1 is written over them).
 __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const signed char *pm, const Rec &L, int pa,
                                   int pb, const int *fs, int n1, const signed char *wl, int s, int nrows, Rows &R,
-                                  __bf16 *H) {
+                                  int *zl) {
     const Geo &g = A.g;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    __bf16 *H = reinterpret_cast<__bf16 *>(zl);
+    const float4 b1 = reinterpret_cast<const float4 *>(A.W.b1)[lane];   // (used in stage C)
+    // stage A: the two per-site rows of each row, one wave per row (rows wave, wave + 4, ...),
+    // every load of a lane issued before any is used (a missing row reads nothing: zero)
+    constexpr int RW = MP / TW;
+    {
+        const long base = (long)s * 2 * g.natoms;
+        const Buf bc = make_buf(A.cache_rd + base * HID, 4u * 2u * (uint32_t)g.natoms * HID);
+        int4 va[RW], vb[RW];
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int r = wave + TW * k;
+            const bool on = r < nrows;
+            const int ra = on ? R.a[r] : 0, rb = on ? R.b[r] : 0;
+            va[k] = buf_i128(bc, on, 16u * ((uint32_t)ra * HID / 4 + lane));
+            vb[k] = buf_i128(bc, on, 16u * ((uint32_t)(g.natoms + rb) * HID / 4 + lane));
+        }
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int r = wave + TW * k;
+            if (r < MP)
+                reinterpret_cast<int4 *>(zl + r * HID)[lane] =
+                    make_int4(va[k].x + vb[k].x, va[k].y + vb[k].y, va[k].z + vb[k].z, va[k].w + vb[k].w);
+        }
+    }
     // changed sites relative to the copy: the lag swap's, then the proposal's not among them
     int X[4], nx = 0, nxv;
     if (L.v) {
@@ -297,32 +326,27 @@ __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const sig
         if (!(L.v && (pa == L.a || pa == L.b))) X[nxv++] = pa;
         if (!(L.v && (pb == L.a || pb == L.b))) X[nxv++] = pb;
     }
-    // corrections: (row, candidate site, changed site), two per thread
+    if (t == 0) R.ncl = 0;
+    __syncthreads();
+    // corrections: (row, candidate site, changed site), two per thread, appended to the list
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int q = t + TB * h, r = q >> 3, k = q & 7, role = k >> 2, xi = k & 3;
-        int2 e = make_int2(-1, -1);
         if (r < nrows) {
             const int v = R.v[r];
             const int c = role ? R.b[r] : R.a[r];
             if (xi < (v ? nxv : nx)) {
                 const int x = X[xi];
                 const int st = v ? species_swapped(lsp, x, pa, pb) : (int)lsp[x];
-                e = corr_entry(A, pm, c, role, x, st, undo(L, x, lsp[x]));
+                const int2 e = corr_entry(A, pm, c, role, x, st, undo(L, x, lsp[x]));
+                if (e.x >= 0) {
+                    const int slot = atomicAdd(&R.ncl, 1);
+                    R.crow[slot] = r;
+                    R.cnew[slot] = e.x;
+                    R.cold[slot] = e.y;
+                }
             }
         }
-        if (r < MP) R.corr[r][k] = e;
-    }
-    __syncthreads();
-    // each row's corrections moved to its front, their count in ncorr
-    if (t < nrows) {
-        int m = 0;
-#pragma unroll
-        for (int k = 0; k < kNCorr; ++k) {
-            const int2 e = R.corr[t][k];
-            if (e.x >= 0) R.corr[t][m++] = e;
-        }
-        R.ncorr[t] = m;
     }
     // dU at the row's state: 4 lanes per row over the first-shell entries of a, then b (the
     // neighbour-table loads of a lane are all issued before any is used)
@@ -357,41 +381,58 @@ __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const sig
         if (r < nrows && l4 == 0) R.du[r] = (sa != sb) ? cnt : INT_MIN;
     }
     __syncthreads();
-    // H1: thread = (row, 64 columns), in two halves of 32: the two per-site rows, then the
-    // corrections
+    // stage B: the corrections, one wave per list entry (entries wave, wave + 4, ...), the
+    // loads of up to CB entries of a lane issued together; integer LDS atomics add them
     {
-        const int r = t >> 2, q4 = t & 3;
-        const bool on = r < nrows;
-        const long base = (long)s * 2 * g.natoms;
-        const int *ra = A.cache_rd + (base + (on ? R.a[r] : 0)) * HID;
-        const int *rb = A.cache_rd + (base + g.natoms + (on ? R.b[r] : 0)) * HID;
+        constexpr int CB = 8;
+        const int ncl = R.ncl;
+        const Buf bw = make_buf(A.w1q, 4u * (uint32_t)(2 * g.ncoord + 2) * (uint32_t)g.nsp * HID);
+        for (int e0 = 0; e0 < ncl; e0 += TW * CB) {
+            int4 wn[CB], wo[CB];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int ch0 = 4 * q4 + 2 * h;   // chunks ch0, ch0 + 1 of 16 columns
-            int acc0[16], acc1[16];
+            for (int k = 0; k < CB; ++k) {
+                const int e = e0 + wave + TW * k;
+                const bool on = e < ncl;
+                wn[k] = buf_i128(bw, on, 16u * ((uint32_t)(on ? R.cnew[e] : 0) * HID / 4 + lane));
+                wo[k] = buf_i128(bw, on, 16u * ((uint32_t)(on ? R.cold[e] : 0) * HID / 4 + lane));
+            }
 #pragma unroll
-            for (int i = 0; i < 16; ++i) acc0[i] = acc1[i] = 0;
-            if (on) {
-                add_row16(acc0, ra + 16 * ch0, 1);
-                add_row16(acc1, ra + 16 * (ch0 + 1), 1);
-                add_row16(acc0, rb + 16 * ch0, 1);
-                add_row16(acc1, rb + 16 * (ch0 + 1), 1);
-                // the row's corrections (most rows have none, a touched one one or two)
-                const int nk = R.ncorr[r];
-#pragma unroll 1
-                for (int k = 0; k < nk; ++k) {
-                    const int2 e = R.corr[r][k];
-                    const int *wn = A.w1q + (long)e.x * HID, *wo = A.w1q + (long)e.y * HID;
-                    add_row16(acc0, wn + 16 * ch0, 1);
-                    add_row16(acc1, wn + 16 * (ch0 + 1), 1);
-                    add_row16(acc0, wo + 16 * ch0, -1);
-                    add_row16(acc1, wo + 16 * (ch0 + 1), -1);
+            for (int k = 0; k < CB; ++k) {
+                const int e = e0 + wave + TW * k;
+                if (e < ncl) {
+                    int *z = zl + R.crow[e] * HID + 4 * lane;
+                    atomicAdd(z, wn[k].x - wo[k].x);
+                    atomicAdd(z + 1, wn[k].y - wo[k].y);
+                    atomicAdd(z + 2, wn[k].z - wo[k].z);
+                    atomicAdd(z + 3, wn[k].w - wo[k].w);
                 }
             }
-            if (r < MP) {
-                h1_store(A.W, H, r, ch0, acc0, on);
-                h1_store(A.W, H, r, ch0 + 1, acc1, on);
-            }
+        }
+    }
+    __syncthreads();
+    // stage C: H1 = bf16(silu(b1 + 2^-16 z)), written over the sums (all read first)
+    {
+        int4 zv[RW];
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int r = wave + TW * k;
+            zv[k] = (r < nrows) ? reinterpret_cast<const int4 *>(zl + r * HID)[lane] : make_int4(0, 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int r = wave + TW * k;
+            if (r >= MP) continue;
+            const bool on = r < nrows;
+            unsigned short h[4];
+            const int zz[4] = {zv[k].x, zv[k].y, zv[k].z, zv[k].w};
+            const float bb[4] = {b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) h[i] = on ? bf16_rne(silu_fast((float)zz[i] * kW1Inv + bb[i])) : 0;
+            uint2 pk;
+            pk.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
+            pk.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
+            *reinterpret_cast<uint2 *>(H + r * LDH + 4 * lane) = pk;
         }
     }
     __syncthreads();
@@ -475,29 +516,6 @@ __device__ inline double merge_share(const double *part, int ntile, int off) {
     return log(D) - log(Z);
 }
 
-// the first-shell stencil positions and the bond weights into LDS; returns how many
-__device__ inline int load_fs(const LPArgs &A, int *fs, signed char *wl, int *nfs) {
-    if (threadIdx.x == 0) *nfs = 0;
-    if (threadIdx.x < kMaxSpW * kMaxSpW) wl[threadIdx.x] = A.wpair[threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x < 64) {   // one wave, positions in stencil order
-        int base = 0;
-        for (int q0 = 0; q0 < A.g.ncoord; q0 += 64) {
-            const int q = q0 + (int)threadIdx.x;
-            const bool on = q < A.g.ncoord && ((A.ss_sten[q] >> 24) & 0xFF) == 0;
-            const uint64_t m = __ballot(on);
-            if (on) {
-                const int idx = base + __popcll(m & ((1ull << threadIdx.x) - 1ull));
-                if (idx < kMaxFs) fs[idx] = q;
-            }
-            base += __popcll(m);
-        }
-        if (threadIdx.x == 0) *nfs = min(base, kMaxFs);
-    }
-    __syncthreads();
-    return *nfs;
-}
-
 // partial softmax sums of a tile's lanes (lane < MR): max, sum e^(l - max), sum over `same`
 __device__ __forceinline__ void tile_partial(float l, bool same, double &m, double &Z, double &D) {
     float mf = l;
@@ -517,7 +535,7 @@ __device__ __forceinline__ void tile_partial(float l, bool same, double &m, doub
 
 // dynamic LDS of a scoring workgroup: the slot's species, H1, the offset -> position cube
 __host__ __device__ inline size_t tile_lds(int natoms_pad) {
-    return (size_t)((natoms_pad + 15) & ~15) + (size_t)MP * LDH * 2 + kPosMapMax;
+    return (size_t)((natoms_pad + 15) & ~15) + (size_t)MP * HID * 4 + kPosMapMax;
 }
 
 // One scoring workgroup: tile k of slot s (TB threads).  Candidates [k MR, (k+1) MR):
@@ -531,11 +549,34 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
     __shared__ Rows R;
     __shared__ int fs[kMaxFs];
     __shared__ signed char wl[kMaxSpW * kMaxSpW];
-    __shared__ int li[2 * MR], nl, nfs;
+    __shared__ int li[2 * MR], nl;
     __shared__ float gAv[MR], gBv[MR], gRv[MR];
     __shared__ int duAv[MR], duBv[MR], duRv[MR], tBv[MR], tRv[MR];
     const Geo &g = A.g;
     const int t = threadIdx.x;
+    uint8_t *lsp = reinterpret_cast<uint8_t *>(lds_raw);
+    int *zl = reinterpret_cast<int *>(lds_raw + ((g.natoms_pad + 15) & ~15));
+    signed char *pm = reinterpret_cast<signed char *>(lds_raw + ((g.natoms_pad + 15) & ~15) + (size_t)MP * HID * 4);
+    const int i0 = tile * MR;
+    const long cbase = (long)s * A.ncand;
+    // requested before anything is known of the step (one round trip): the species, the
+    // offset cube, and this lane's candidate of step n as step n - 1 left it
+    load_species(lsp, A.species + (long)s * g.natoms_pad, g.natoms_pad);
+    {
+        const int side = 2 * A.pm_r + 1, npm = side * side * side;
+        for (int q = t; q < npm; q += TB) pm[q] = A.posmap[q];
+    }
+    float fgA = 0.f, fgB = 0.f;
+    int fdA = INT_MIN, fdB = INT_MIN, ftb = 0, csv = 0;
+    if (A.mode != 0 && t < MR) {
+        const long c = i0 + t;
+        fgA = A.sg_in[cbase * 2 + c];
+        fdA = A.sdu_in[cbase * 2 + c];
+        fgB = A.sg_in[cbase * 2 + A.ncand + c];
+        fdB = A.sdu_in[cbase * 2 + A.ncand + c];
+        ftb = A.st_in[cbase + c];
+        csv = A.cs[s];
+    }
     const int rung = A.rung_of[s];
     const double beta = A.beta_slot[s];
     const uint64_t n = (A.mode == 0) ? A.stepc[s] : A.lpn[s], nx = n + 1;
@@ -551,17 +592,9 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
     const bool need_i = learn_next, need_ii = learn_next && has_p, need_iii = learn_cur && has_p;
     if (!need_i && !need_iii) return;   // (the same for every tile of the slot)
     const Rec LG = load_rec(A.lag + kRecInts * s);
-    uint8_t *lsp = reinterpret_cast<uint8_t *>(lds_raw);
-    __bf16 *H = reinterpret_cast<__bf16 *>(lds_raw + ((g.natoms_pad + 15) & ~15));
-    signed char *pm = reinterpret_cast<signed char *>(lds_raw + ((g.natoms_pad + 15) & ~15) + (size_t)MP * LDH * 2);
-    const int i0 = tile * MR;
-    const long cbase = (long)s * A.ncand;
-    load_species(lsp, A.species + (long)s * g.natoms_pad, g.natoms_pad);
-    {
-        const int side = 2 * A.pm_r + 1, npm = side * side * side;
-        for (int q = t; q < npm; q += TB) pm[q] = A.posmap[q];
-    }
-    const int n1 = load_fs(A, fs, wl, &nfs);
+    const int n1 = A.n1;
+    if (t < kMaxFs) fs[t] = A.fs[t];
+    if (t < kMaxSpW * kMaxSpW) wl[t] = A.wpair[t];
     const int d0 = A.lift ? A.dir[s] : 1;
     // rows (ii) / (iii): candidates within the stencil radius of P (a superset of the ones a
     // correction reaches; the others score as at x), in candidate order
@@ -607,7 +640,7 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
             R.v[t] = v;
         }
         __syncthreads();
-        score_rows(A, lsp, pm, LG, P.a, P.b, fs, n1, wl, s, nr, R, H);
+        score_rows(A, lsp, pm, LG, P.a, P.b, fs, n1, wl, s, nr, R, zl);
         if (t < nr) {
             const int q = q0 + t;
             const float gv = R.gout[t];
@@ -638,9 +671,9 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
     double *lq = A.lq + ((long)s * MAXT + tile) * kLqDoubles;
     if (need_iii) {
         const int c = i0 + t;
-        const bool tb = A.cs[s] && A.st_in[cbase + c];
-        const float fg = A.sg_in[cbase * 2 + (tb ? A.ncand : 0) + c];
-        const int fdu = A.sdu_in[cbase * 2 + (tb ? A.ncand : 0) + c];
+        const bool tb = csv && ftb;
+        const float fg = tb ? fgB : fgA;
+        const int fdu = tb ? fdB : fdA;
         const float rg = tRv[t] ? gRv[t] : fg;
         const int rdu = tRv[t] ? duRv[t] : fdu;
         int ca, cb;

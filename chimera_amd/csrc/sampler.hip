@@ -751,6 +751,17 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
         if (A.lp_dir_rd) lpd0 = A.lp_dir_rd[s];
     }
     __shared__ int lp_dnew;
+    // ... the scoring tiles' partial sums of this step's log q ratio and the tile maxima of
+    // the next step's logits, into LDS with the rest of the prologue
+    __shared__ double lp_lqs[chm_lp_dev::MAXT * chm_lp_dev::kLqDoubles];
+    __shared__ float lp_dms[chm_lp_dev::MAXT * 2];
+    if (A.lp_lq) {
+        const int nt = A.lp_ncand / chm_lp_dev::MR;
+        if (t < nt * chm_lp_dev::kLqDoubles)
+            lp_lqs[t] = A.lp_lq[(long)s * chm_lp_dev::MAXT * chm_lp_dev::kLqDoubles + t];
+        if (A.propose == 4 && t >= 64 && t < 64 + 2 * nt)
+            lp_dms[t - 64] = A.lp_dmax[(long)s * chm_lp_dev::MAXT * 2 + t - 64];
+    }
     double q0v[2 * chm::kMaxNsp];            // per edge T0, Q0 (padded with zeros)
 #pragma unroll
     for (int q = 0; q < 2 * chm::kMaxNsp; ++q) q0v[q] = A.q0[q];
@@ -860,9 +871,8 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
                 if (pp[4]) {
                     double lq = (double)__int_as_float(pp[5]);
                     if (learned) {
-                        const double *part = A.lp_lq + (long)s * chm_lp_dev::MAXT * chm_lp_dev::kLqDoubles;
                         const int nt = A.lp_ncand / chm_lp_dev::MR;
-                        lq = chm_lp_dev::merge_share(part, nt, 3) - chm_lp_dev::merge_share(part, nt, 0);
+                        lq = chm_lp_dev::merge_share(lp_lqs, nt, 3) - chm_lp_dev::merge_share(lp_lqs, nt, 0);
                     }
                     const double x = beta * (c2t - c2p) * A.nfit - lq;
                     acc = metropolis(x, u_acc);
@@ -994,8 +1004,7 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
         const int dout = (A.accept == 1) ? lp_dnew : lpd0;
         const int nt = A.lp_ncand / chm_lp_dev::MR;
         float m = -__builtin_huge_valf();
-        for (int k = 0; k < nt; ++k)
-            m = fmaxf(m, A.lp_dmax[((long)s * chm_lp_dev::MAXT + k) * 2 + (accd ? 0 : 1)]);
+        for (int k = 0; k < nt; ++k) m = fmaxf(m, lp_dms[2 * k + (accd ? 0 : 1)]);
         const bool useB = accd && lptB;
         const float lg = (t < A.lp_ncand)
                              ? chm_lp_dev::logit_of(A.lp_lam, A.lp_kappa, A.lp_lift, useB ? lpgB : lpgA,

@@ -22,6 +22,8 @@ struct chm_lp {
     int r2max = 0;           // largest squared stencil offset, units (a0/2)^2
     int pm_r = 0;            // half-side of the offset -> stencil position cube
     signed char opp[128] = {};   // stencil position of the opposite offset of each position
+    signed char fs[16] = {};     // first-shell stencil positions (n1 of them)
+    int n1 = 0;
     bool have_params = false;
     chm::DevBuf params, adam_m, adam_v;          // FP32 masters and Adam moments
     chm::DevBuf w1, w2t, w2, w3q;                // bf16 copies (W2 in both orders); w3 rounded
