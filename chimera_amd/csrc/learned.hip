@@ -84,8 +84,12 @@ __global__ __launch_bounds__(TB) void k_lp_eval(LPArgs A, int nocc, int n, const
         }
         h1_store(A.W, H, r, ch, acc, on);
     }
+    W2Half F;
+    float b2[4], w3[4];
+    load_w2_half(A.W, 0, F);
+    load_b2w3(A.W, b2, w3);
     __syncthreads();
-    layer2(A.W, H, 2, gp, gout);
+    layer2(A.W, H, 2, F, b2, w3, gp, gout);
     if (t < nr) out[r0 + t] = gout[t];
 }
 
@@ -438,7 +442,8 @@ int lp_prepare(chm_sampler *S) {
     const int side = 2 * R + 1;
     if ((size_t)side * side * side > (size_t)kPosMapMax)
         return chm::set_error("learned proposal: stencil offsets too long"), CHM_E_ARG;
-    std::vector<signed char> pm((size_t)side * side * side, -1);
+    // (padded to 16 bytes: the scoring workgroups read it in 16-byte pieces)
+    std::vector<signed char> pm(((size_t)side * side * side + 15) & ~(size_t)15, -1);
     auto at = [&](int dx, int dy, int dz) -> signed char & {
         return pm[((size_t)(dx + R) * side + (dy + R)) * side + (dz + R)];
     };

@@ -185,13 +185,49 @@ __device__ __forceinline__ void add_row16(int (&acc)[16], const int *row, int si
     }
 }
 
+// Sum over the 16 lanes of each DPP row (lanes 16 k .. 16 k + 15), in every lane: swaps of
+// neighbours, of pairs, of quads (half-row mirror) and of halves (row mirror).
+__device__ __forceinline__ float row16_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
+    return v;
+}
+
+// One K half of a wave's W2 fragments (hidden units [64 w, 64 w + 64), four 16-column
+// tiles), with b2 and w3 of those units
+struct W2Half {
+    bf16x8 B[4][4];
+};
+__device__ __forceinline__ void load_w2_half(const LPW &W, int half, W2Half &F) {
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int lr = lane & 15, lk = 8 * (lane >> 4);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const __bf16 *w2 = W.w2t + (long)(64 * wave + 16 * c + lr) * HID + lk + 128 * half;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) F.B[k][c] = *reinterpret_cast<const bf16x8 *>(w2 + 32 * k);
+    }
+}
+
+__device__ __forceinline__ void load_b2w3(const LPW &W, float (&b2)[4], float (&w3)[4]) {
+    const int t = threadIdx.x, wave = t >> 6, lr = t & 15;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        b2[c] = W.b2[64 * wave + 16 * c + lr];
+        w3[c] = W.w3q[64 * wave + 16 * c + lr];
+    }
+}
+
 // g of the first 16 nrt rows (nrt <= MP / 16) whose H1 (bf16, MP x LDH) is in LDS: Z2 = b2 +
-// H1 W2 on v_mfma_f32_16x16x32_bf16 (wave w: hidden units [64 w, 64 w + 64), four 16-column
-// tiles), then b3 + silu(Z2) . w3; gp (TW x MP) is LDS scratch, the result goes to gout
-// (LDS).  The W2 fragments are loaded in two halves of K (64 registers each), so the
-// scoring workgroups fit beside the transform's in one launch.  Every thread of the
-// TB-thread workgroup calls it.
-__device__ inline void layer2(const LPW &W, const __bf16 *H, int nrt, float *gp, float *gout) {
+// H1 W2 on v_mfma_f32_16x16x32_bf16 (wave w: hidden units [64 w, 64 w + 64)), then b3 +
+// silu(Z2) . w3; gp (TW x MP) is LDS scratch, the result goes to gout (LDS).  F holds the
+// first K half of the W2 fragments (requested by the caller ahead of time); the second is
+// loaded into the same registers, so the scoring workgroups fit beside the transform's in
+// one launch.  Every thread of the TB-thread workgroup calls it.
+__device__ inline void layer2(const LPW &W, const __bf16 *H, int nrt, W2Half &F, const float (&b2)[4],
+                              const float (&w3)[4], float *gp, float *gout) {
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int lr = lane & 15, lk = 8 * (lane >> 4);
     constexpr int RT = MP / 16;
@@ -200,21 +236,9 @@ __device__ inline void layer2(const LPW &W, const __bf16 *H, int nrt, float *gp,
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-    float b2[4], w3[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        b2[c] = W.b2[64 * wave + 16 * c + lr];
-        w3[c] = W.w3q[64 * wave + 16 * c + lr];
-    }
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
-        bf16x8 B[4][4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const __bf16 *w2 = W.w2t + (long)(64 * wave + 16 * c + lr) * HID + lk + 128 * half;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) B[k][c] = *reinterpret_cast<const bf16x8 *>(w2 + 32 * k);
-        }
+        if (half == 1) load_w2_half(W, 1, F);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int kt = 4 * half + k;
@@ -224,7 +248,7 @@ __device__ inline void layer2(const LPW &W, const __bf16 *H, int nrt, float *gp,
                 const bf16x8 a = *reinterpret_cast<const bf16x8 *>(H + (16 * r + lr) * LDH + 32 * kt + lk);
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, B[k][c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, F.B[k][c], acc[r][c], 0, 0, 0);
             }
         }
     }
@@ -236,8 +260,7 @@ __device__ inline void layer2(const LPW &W, const __bf16 *H, int nrt, float *gp,
             float v = 0.f;
 #pragma unroll
             for (int c = 0; c < 4; ++c) v += silu_fast(acc[r][c][i] + b2[c]) * w3[c];
-#pragma unroll
-            for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, 16);
+            v = row16_sum(v);
             if (lr == 0) gp[wave * MP + 16 * r + 4 * (lane >> 4) + i] = v;
         }
     }
@@ -268,6 +291,19 @@ __device__ __forceinline__ void h1_store(const LPW &W, __bf16 *H, int r, int ch,
     *reinterpret_cast<bf16x8 *>(H + r * LDH + 16 * ch + 8) = v1;
 }
 
+// Up to four changed sites in registers (appended in order; no dynamically indexed array)
+struct Sites4 {
+    int x0 = 0, x1 = 0, x2 = 0, x3 = 0, n = 0;
+    __device__ __forceinline__ void push(int v) {
+        x0 = (n == 0) ? v : x0;
+        x1 = (n == 1) ? v : x1;
+        x2 = (n == 2) ? v : x2;
+        x3 = (n == 3) ? v : x3;
+        ++n;
+    }
+    __device__ __forceinline__ int operator[](int i) const { return i == 0 ? x0 : i == 1 ? x1 : i == 2 ? x2 : x3; }
+};
+
 // Rows of one network pass, in LDS: candidate (a, b) at the committed state (v = 0) or with
 // the proposal swapped (v = 1); dU and g per row, and the flattened list of layer-1
 // corrections (row, new weight row, old weight row).
@@ -280,54 +316,102 @@ struct Rows {
     int ncl;
 };
 
-// g and dU of rows[0, nrows) (nrows <= MP) of slot s.  The copy `cache_rd` holds the
-// configuration before the lag swap; the committed species (lsp) have it made; rows with
-// v = 1 also have the proposal (pa, pb) made.  fs: first-shell stencil positions (n1); wl:
-// bond weights; pm: offset -> stencil position cube; zl: MP x HID int32 of LDS (the layer-1
-// sums; H1 is written over them).
+// The row list of a tile: entry q < n0 is candidate i0 + q of step n+1 at the committed state;
+// entry q >= n0 is li[q - n0] (an entry e of the touched test: e < MR candidate i0 + e of step
+// n+1, else candidate i0 + e - MR of step n) with the proposal made.  li is the calling wave's
+// own copy of the list, so a wave can name any row without a workgroup barrier.
+struct RowSrc {
+    uint64_t h_next, h_cur;
+    int i0, n0, natoms;
+    const int *li;
+    __device__ __forceinline__ void get(int q, int &a, int &b, int &v) const {
+        if (q < n0) {
+            lp_candidate(h_next, i0 + q, natoms, a, b);
+            v = 0;
+        } else {
+            const int e = li[q - n0];
+            // (a select by mask: a plain conditional of the two wave-uniform values compiled
+            // to a stack array)
+            const uint64_t m = (e < MR) ? ~0ull : 0ull;
+            lp_candidate(h_cur ^ ((h_next ^ h_cur) & m), i0 + (e & (MR - 1)), natoms, a, b);
+            v = 1;
+        }
+    }
+};
+
+// g and dU of rows [q0, q0 + nrows) (nrows <= MP) of the list RS of slot s.  The copy
+// `cache_rd` holds the configuration before the lag swap; the committed species (lsp) have
+// it made; rows with v = 1 also have the proposal (pa, pb) made.  fs: first-shell stencil
+// positions (n1); wl: bond weights; pm: offset -> stencil position cube; zl: MP x HID int32
+// of LDS (the layer-1 sums; H1 is written over them).  The per-site rows are requested
+// before the first barrier.
 __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const signed char *pm, const Rec &L, int pa,
-                                  int pb, const int *fs, int n1, const signed char *wl, int s, int nrows, Rows &R,
-                                  int *zl) {
+                                  int pb, const int *fs, int n1, const signed char *wl, int s, const RowSrc &RS,
+                                  int q0, int nrows, Rows &R, int *zl) {
     const Geo &g = A.g;
-    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     __bf16 *H = reinterpret_cast<__bf16 *>(zl);
-    const float4 b1 = reinterpret_cast<const float4 *>(A.W.b1)[lane];   // (used in stage C)
+    if (t < nrows) {
+        int a, b, v;
+        RS.get(q0 + t, a, b, v);
+        R.a[t] = a;
+        R.b[t] = b;
+        R.v[t] = v;
+    }
     // stage A: the two per-site rows of each row, one wave per row (rows wave, wave + 4, ...),
     // every load of a lane issued before any is used (a missing row reads nothing: zero)
     constexpr int RW = MP / TW;
-    {
-        const long base = (long)s * 2 * g.natoms;
-        const Buf bc = make_buf(A.cache_rd + base * HID, 4u * 2u * (uint32_t)g.natoms * HID);
-        int4 va[RW], vb[RW];
+    const Buf bc = make_buf(A.cache_rd + (long)s * 2 * g.natoms * HID, 4u * 2u * (uint32_t)g.natoms * HID);
+    // (lane k names row wave + TW k; its sites are read back as wave-uniform values)
+    int ral = 0, rbl = 0;
+    if (lane < RW && wave + TW * lane < nrows) {
+        int v;
+        RS.get(q0 + wave + TW * lane, ral, rbl, v);
+    }
+    int4 va[RW], vb[RW];
 #pragma unroll
-        for (int k = 0; k < RW; ++k) {
-            const int r = wave + TW * k;
-            const bool on = r < nrows;
-            const int ra = on ? R.a[r] : 0, rb = on ? R.b[r] : 0;
-            va[k] = buf_i128(bc, on, 16u * ((uint32_t)ra * HID / 4 + lane));
-            vb[k] = buf_i128(bc, on, 16u * ((uint32_t)(g.natoms + rb) * HID / 4 + lane));
-        }
-#pragma unroll
-        for (int k = 0; k < RW; ++k) {
-            const int r = wave + TW * k;
-            if (r < MP)
-                reinterpret_cast<int4 *>(zl + r * HID)[lane] =
-                    make_int4(va[k].x + vb[k].x, va[k].y + vb[k].y, va[k].z + vb[k].z, va[k].w + vb[k].w);
-        }
+    for (int k = 0; k < RW; ++k) {
+        const bool on = wave + TW * k < nrows;
+        const int ra = __builtin_amdgcn_readlane(ral, k), rb = __builtin_amdgcn_readlane(rbl, k);
+        va[k] = buf_i128(bc, on, 16u * ((uint32_t)ra * HID / 4 + lane));
+        vb[k] = buf_i128(bc, on, 16u * ((uint32_t)(g.natoms + rb) * HID / 4 + lane));
     }
-    // changed sites relative to the copy: the lag swap's, then the proposal's not among them
-    int X[4], nx = 0, nxv;
-    if (L.v) {
-        X[nx++] = L.a;
-        X[nx++] = L.b;
-    }
-    nxv = nx;
-    if (pa >= 0) {
-        if (!(L.v && (pa == L.a || pa == L.b))) X[nxv++] = pa;
-        if (!(L.v && (pb == L.a || pb == L.b))) X[nxv++] = pb;
-    }
+    const float4 b1 = reinterpret_cast<const float4 *>(A.W.b1)[lane];   // (used in stage C)
+    __builtin_amdgcn_sched_barrier(0);
     if (t == 0) R.ncl = 0;
     __syncthreads();
+    // dU at the row's state: 4 lanes per row over the first-shell entries of a, then b; the
+    // neighbour-table loads are issued before the per-site rows are consumed
+    constexpr int NE = 2 * kMaxFs / 4;
+    const int ur = t >> 2, l4 = t & 3;
+    int y[NE];
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+        const int e = l4 + 4 * j;
+        y[j] = -1;
+        if (ur < nrows && e < 2 * n1)
+            y[j] = A.nbr[(long)(e < n1 ? R.a[ur] : R.b[ur]) * g.ncoord + fs[e < n1 ? e : e - n1]];
+    }
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+        const int r = wave + TW * k;
+        if (r < MP)
+            reinterpret_cast<int4 *>(zl + r * HID)[lane] =
+                make_int4(va[k].x + vb[k].x, va[k].y + vb[k].y, va[k].z + vb[k].z, va[k].w + vb[k].w);
+    }
+    // changed sites relative to the copy: the lag swap's, then the proposal's not among them
+    Sites4 X;
+    if (L.v) {
+        X.push(L.a);
+        X.push(L.b);
+    }
+    const int nx = X.n;
+    if (pa >= 0) {
+        if (!(L.v && (pa == L.a || pa == L.b))) X.push(pa);
+        if (!(L.v && (pb == L.a || pb == L.b))) X.push(pb);
+    }
+    const int nxv = X.n;
     // corrections: (row, candidate site, changed site), two per thread, appended to the list
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -348,21 +432,10 @@ __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const sig
             }
         }
     }
-    // dU at the row's state: 4 lanes per row over the first-shell entries of a, then b (the
-    // neighbour-table loads of a lane are all issued before any is used)
     {
-        constexpr int NE = 2 * kMaxFs / 4;
-        const int r = t >> 2, l4 = t & 3;
         int cnt = 0, sa = 0, sb = 0;
-        if (r < nrows) {
-            const int a = R.a[r], b = R.b[r], v = R.v[r];
-            int y[NE];
-#pragma unroll
-            for (int j = 0; j < NE; ++j) {
-                const int e = l4 + 4 * j;
-                y[j] = -1;
-                if (e < 2 * n1) y[j] = A.nbr[(long)(e < n1 ? a : b) * g.ncoord + fs[e < n1 ? e : e - n1]];
-            }
+        if (ur < nrows) {
+            const int a = R.a[ur], b = R.b[ur], v = R.v[ur];
             sa = v ? species_swapped(lsp, a, pa, pb) : lsp[a];
             sb = v ? species_swapped(lsp, b, pa, pb) : lsp[b];
 #pragma unroll
@@ -378,7 +451,7 @@ __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const sig
         }
         cnt += __shfl_xor(cnt, 1, 4);
         cnt += __shfl_xor(cnt, 2, 4);
-        if (r < nrows && l4 == 0) R.du[r] = (sa != sb) ? cnt : INT_MIN;
+        if (ur < nrows && l4 == 0) R.du[ur] = (sa != sb) ? cnt : INT_MIN;
     }
     __syncthreads();
     // stage B: the corrections, one wave per list entry (entries wave, wave + 4, ...), the
@@ -410,6 +483,11 @@ __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const sig
         }
     }
     __syncthreads();
+    // the first K half of layer 2's weights and b2 / w3, requested before stage C's work
+    W2Half F;
+    float b2[4], w3[4];
+    load_w2_half(A.W, 0, F);
+    load_b2w3(A.W, b2, w3);
     // stage C: H1 = bf16(silu(b1 + 2^-16 z)), written over the sums (all read first)
     {
         int4 zv[RW];
@@ -436,84 +514,69 @@ __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const sig
         }
     }
     __syncthreads();
-    layer2(A.W, H, (nrows + 15) >> 4, R.gp, R.gout);
+    layer2(A.W, H, (nrows + 15) >> 4, F, b2, w3, R.gp, R.gout);
 }
 
-template <int NWV>
-__device__ __forceinline__ float block_max_f(float v, float *red) {
-    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    float r = red[0];
-    for (int w = 1; w < NWV; ++w) r = fmaxf(r, red[w]);
-    __syncthreads();
-    return r;
-}
-
-// inclusive scan over the workgroup's lanes (index order); returns (exclusive, inclusive)
-// and the total
-template <int NWV>
-__device__ __forceinline__ void block_scan_d(double v, double *red, double &excl, double &incl, double &tot) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double x = v;
-    for (int m = 1; m < 64; m <<= 1) {
-        const double y = __shfl_up(x, m, 64);
-        if (lane >= m) x += y;
+// log q(x' -> x) - log q(x -> x') of a step from the scoring tiles' partial sums (per tile:
+// max m of the logits, sum of e^(l - m), the same sum over the proposed pair): lanes 0..7 of
+// the calling wave hold tile `lane` of the forward set, lanes 8..15 the reverse set (tiles
+// past the last: m = -inf, z = d = 0).  Every lane of the wave calls it; lane 0 gets the
+// result.
+__device__ inline double lq_merge(double m, double z, double d) {
+    double M = m;
+#pragma unroll
+    for (int o = 4; o >= 1; o >>= 1) M = fmax(M, __shfl_xor(M, o, 8));
+    const double f = (m == -__builtin_huge_val()) ? 0.0 : exp(m - M);
+    double Z = z * f, D = d * f;
+#pragma unroll
+    for (int o = 4; o >= 1; o >>= 1) {
+        Z += __shfl_xor(Z, o, 8);
+        D += __shfl_xor(D, o, 8);
     }
-    __syncthreads();
-    if (lane == 63) red[wave] = x;
-    __syncthreads();
-    double off = 0.0, all = 0.0;
-    for (int w = 0; w < NWV; ++w) {
-        if (w < wave) off += red[w];
-        all += red[w];
-    }
-    incl = off + x;
-    const double prev = __shfl_up(x, 1, 64);
-    excl = off + (lane == 0 ? 0.0 : prev);
-    tot = all;
-    __syncthreads();
+    const double r = log(D) - log(Z);
+    return __shfl(r, 8, 64) - r;
 }
 
-// The softmax draw over the workgroup's lanes (lane = candidate, lg its logit, m the maximum
-// logit, from the tiles): the lane whose bin [excl, incl) of the cumulative weights holds
-// u Z, the last lane with weight when u Z rounds past the end, -1 when no candidate is
-// valid.  Every thread calls it and gets the same answer.
-template <int NWV>
-__device__ inline int softmax_draw(float lg, float m, double u, double *redd, int *jsh) {
+// The softmax draw over candidates held four per lane by one wave (candidate 4 lane + q has
+// logit lg[q]; m: the largest logit): the candidate whose bin [lo, hi) of the cumulative
+// weights (index order) holds u Z, else the last candidate with weight (u Z rounded past the
+// end), -1 when none is valid.  Bins meet exactly: a lane's last edge is its inclusive scan
+// value, which is the next lane's first.  Every lane of the wave calls it and gets the answer.
+__device__ inline int wave_draw4(const float (&lg)[4], float m, double u) {
+    const int lane = threadIdx.x & 63;
     const bool any = m > -__builtin_huge_valf();
-    const double p = (any && lg > -__builtin_huge_valf()) ? exp((double)(lg - m)) : 0.0;
-    double excl, incl, Z;
-    block_scan_d<NWV>(p, redd, excl, incl, Z);
-    const double target = u * Z;
-    if (threadIdx.x == 0) *jsh = -1;
-    __syncthreads();
-    if (any && p > 0.0 && excl <= target && target < incl) *jsh = threadIdx.x;
-    __syncthreads();
-    if (any && *jsh < 0) {   // the target rounded past the last bin: the last candidate with p > 0
-        if (p > 0.0) atomicMax(jsh, (int)threadIdx.x);
-        __syncthreads();
+    double p[4], c[4], run = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        p[q] = (any && lg[q] > -__builtin_huge_valf()) ? exp((double)(lg[q] - m)) : 0.0;
+        run += p[q];
+        c[q] = run;
     }
-    const int j = *jsh;
-    __syncthreads();
-    return j;
-}
-
-// log of the softmax share of the lanes with `same` set, from per-tile partial sums:
-// (max, sum of e^(l - max), sum over `same`) of ntile tiles, merged
-__device__ inline double merge_share(const double *part, int ntile, int off) {
-    double m = -__builtin_huge_val();
-    for (int k = 0; k < ntile; ++k) m = fmax(m, part[k * kLqDoubles + off]);
-    double Z = 0.0, D = 0.0;
-    for (int k = 0; k < ntile; ++k) {
-        const double mk = part[k * kLqDoubles + off];
-        if (mk == -__builtin_huge_val()) continue;
-        const double f = exp(mk - m);
-        Z += part[k * kLqDoubles + off + 1] * f;
-        D += part[k * kLqDoubles + off + 2] * f;
+    double x = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
     }
-    return log(D) - log(Z);
+    const double prev = __shfl_up(x, 1, 64);
+    const double base = lane == 0 ? 0.0 : prev;
+    const double target = u * __shfl(x, 63, 64);
+    int hit = -1, last = -1;
+    double lo = base;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double hi = (q == 3) ? x : base + c[q];
+        if (p[q] > 0.0) {
+            if (hit < 0 && lo <= target && target < hi) hit = 4 * lane + q;
+            last = 4 * lane + q;
+        }
+        lo = hi;
+    }
+    const unsigned long long bh = __ballot(hit >= 0);
+    if (bh) return __shfl(hit, __ffsll(bh) - 1, 64);
+    const unsigned long long bl = __ballot(last >= 0);
+    if (bl) return __shfl(last, 63 - __clzll((long long)bl), 64);
+    return -1;
 }
 
 // partial softmax sums of a tile's lanes (lane < MR): max, sum e^(l - max), sum over `same`
@@ -549,23 +612,37 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
     __shared__ Rows R;
     __shared__ int fs[kMaxFs];
     __shared__ signed char wl[kMaxSpW * kMaxSpW];
-    __shared__ int li[2 * MR], nl;
+    __shared__ int li4[TW][2 * MR];
     __shared__ float gAv[MR], gBv[MR], gRv[MR];
     __shared__ int duAv[MR], duBv[MR], duRv[MR], tBv[MR], tRv[MR];
     const Geo &g = A.g;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     uint8_t *lsp = reinterpret_cast<uint8_t *>(lds_raw);
     int *zl = reinterpret_cast<int *>(lds_raw + ((g.natoms_pad + 15) & ~15));
     signed char *pm = reinterpret_cast<signed char *>(lds_raw + ((g.natoms_pad + 15) & ~15) + (size_t)MP * HID * 4);
     const int i0 = tile * MR;
     const long cbase = (long)s * A.ncand;
-    // requested before anything is known of the step (one round trip): the species, the
-    // offset cube, and this lane's candidate of step n as step n - 1 left it
-    load_species(lsp, A.species + (long)s * g.natoms_pad, g.natoms_pad);
-    {
-        const int side = 2 * A.pm_r + 1, npm = side * side * side;
-        for (int q = t; q < npm; q += TB) pm[q] = A.posmap[q];
+    // requested before anything is known of the step (one round trip), the small loads the
+    // row list needs first: the step, the slot's rung and beta, step n's proposal, the lag
+    // swap and direction; then the species and the offset cube (into registers: stored into
+    // LDS once the rows are requested) and this lane's candidate of step n as step n - 1 left
+    // it
+    const int rung = A.rung_of[s];
+    const double beta = A.beta_slot[s];
+    const uint64_t n = (A.mode == 0) ? A.stepc[s] : A.lpn[s], nx = n + 1;
+    Prop P{-1, -1, 0, 0, 0};
+    if (A.mode != 0) {
+        const int *pp = A.prop_n + kPropInts * s;
+        P = Prop{pp[0], pp[1], pp[2], pp[3], pp[4]};
     }
+    const Rec LG = load_rec(A.lag + kRecInts * s);
+    const int d0 = A.lift ? A.dir[s] : 1;
+    const int nsv = g.natoms_pad / 16;
+    const uint4 *gsp = reinterpret_cast<const uint4 *>(A.species + (long)s * g.natoms_pad);
+    const uint4 spv = (t < nsv) ? gsp[t] : make_uint4(0, 0, 0, 0);
+    const int side = 2 * A.pm_r + 1, npm16 = (side * side * side + 15) / 16;   // (<= TB)
+    const uint4 pmv = (t < npm16) ? reinterpret_cast<const uint4 *>(A.posmap)[t] : make_uint4(0, 0, 0, 0);
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);
     float fgA = 0.f, fgB = 0.f;
     int fdA = INT_MIN, fdB = INT_MIN, ftb = 0, csv = 0;
     if (A.mode != 0 && t < MR) {
@@ -577,30 +654,23 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
         ftb = A.st_in[cbase + c];
         csv = A.cs[s];
     }
-    const int rung = A.rung_of[s];
-    const double beta = A.beta_slot[s];
-    const uint64_t n = (A.mode == 0) ? A.stepc[s] : A.lpn[s], nx = n + 1;
+    __builtin_amdgcn_sched_barrier(0);
     const uint64_t h_next = lp_stream(A.seed, nx, rung), h_cur = lp_stream(A.seed, n, rung);
     const bool learn_next = A.mode != 2 && !lp_coin_uniform(h_next, beta, A.eps, A.beta_max);
     const bool learn_cur = A.mode != 0 && !lp_coin_uniform(h_cur, beta, A.eps, A.beta_max);
-    Prop P{-1, -1, 0, 0, 0};
-    if (A.mode != 0) {
-        const int *pp = A.prop_n + kPropInts * s;
-        P = Prop{pp[0], pp[1], pp[2], pp[3], pp[4]};
-    }
     const bool has_p = P.sw != 0;
     const bool need_i = learn_next, need_ii = learn_next && has_p, need_iii = learn_cur && has_p;
     if (!need_i && !need_iii) return;   // (the same for every tile of the slot)
-    const Rec LG = load_rec(A.lag + kRecInts * s);
     const int n1 = A.n1;
     if (t < kMaxFs) fs[t] = A.fs[t];
     if (t < kMaxSpW * kMaxSpW) wl[t] = A.wpair[t];
-    const int d0 = A.lift ? A.dir[s] : 1;
     // rows (ii) / (iii): candidates within the stencil radius of P (a superset of the ones a
-    // correction reaches; the others score as at x), in candidate order
-    if (t < 64) {
-        const int kind = t < MR ? 1 : 2;
-        const int c = t & (MR - 1);
+    // correction reaches; the others score as at x), in candidate order; every wave makes
+    // the list itself (lane = entry: (ii) for lane < MR, (iii) else)
+    int nl;
+    {
+        const int kind = lane < MR ? 1 : 2;
+        const int c = lane & (MR - 1);
         const bool want = kind == 1 ? need_ii : need_iii;
         bool touched = false;
         if (want) {
@@ -610,37 +680,27 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
                       dist2_sites(g, cb, P.a) <= A.r2max || dist2_sites(g, cb, P.b) <= A.r2max;
         }
         const uint64_t m = __ballot(touched);
-        if (touched) li[__popcll(m & ((1ull << t) - 1ull))] = t;   // t: kind and offset
-        if (t == 0) nl = __popcll(m);
-        if (kind == 1) {
-            tBv[c] = touched;
-            if (need_i) A.st_out[cbase + i0 + c] = (int8_t)touched;
-        } else {
-            tRv[c] = touched;
+        if (touched) li4[wave_u][__popcll(m & ((1ull << lane) - 1ull))] = lane;
+        nl = __popcll(m);
+        if (wave == 0) {
+            if (kind == 1) {
+                tBv[c] = touched;
+                if (need_i) A.st_out[cbase + i0 + c] = (int8_t)touched;
+            } else {
+                tRv[c] = touched;
+            }
         }
     }
-    __syncthreads();
     // passes: (i) rows first (when step n+1 is learned), then the re-scored rows
     const int n0 = need_i ? MR : 0, ntot = n0 + nl;
+    const RowSrc RS{h_next, h_cur, i0, n0, g.natoms, li4[wave_u]};
+    // the species and the offset cube into LDS (read after the first pass's first barrier)
+    if (t < nsv) reinterpret_cast<uint4 *>(lsp)[t] = spv;
+    for (int q = t + TB; q < nsv; q += TB) reinterpret_cast<uint4 *>(lsp)[q] = gsp[q];
+    if (t < npm16) reinterpret_cast<uint4 *>(pm)[t] = pmv;
     for (int q0 = 0; q0 < ntot; q0 += MP) {
         const int nr = min(MP, ntot - q0);
-        if (t < nr) {
-            const int q = q0 + t;
-            int a, b, v;
-            if (q < n0) {
-                lp_candidate(h_next, i0 + q, g.natoms, a, b);
-                v = 0;
-            } else {
-                const int e = li[q - n0];
-                lp_candidate(e < MR ? h_next : h_cur, i0 + (e & (MR - 1)), g.natoms, a, b);
-                v = 1;
-            }
-            R.a[t] = a;
-            R.b[t] = b;
-            R.v[t] = v;
-        }
-        __syncthreads();
-        score_rows(A, lsp, pm, LG, P.a, P.b, fs, n1, wl, s, nr, R, zl);
+        score_rows(A, lsp, pm, LG, P.a, P.b, fs, n1, wl, s, RS, q0, nr, R, zl);
         if (t < nr) {
             const int q = q0 + t;
             const float gv = R.gout[t];
@@ -651,7 +711,7 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
                 A.sg_out[cbase * 2 + i0 + q] = gv;
                 A.sdu_out[cbase * 2 + i0 + q] = dv;
             } else {
-                const int e = li[q - n0], c = e & (MR - 1);
+                const int e = li4[0][q - n0], c = e & (MR - 1);
                 if (e < MR) {
                     gBv[c] = gv;
                     duBv[c] = dv;
@@ -724,15 +784,16 @@ __device__ inline void lp_fold(const LPArgs &A, int s, int part, char *lds_raw) 
     const Geo &g = A.g;
     const int nc = g.ncoord;
     const Rec L1 = load_rec(A.lag + kRecInts * s), L2 = load_rec(A.lag2 + kRecInts * s);
-    int X[4], nX = 0;
+    Sites4 X;
     if (L2.v) {
-        X[nX++] = L2.a;
-        X[nX++] = L2.b;
+        X.push(L2.a);
+        X.push(L2.b);
     }
     if (L1.v) {
-        if (!(L2.v && (L1.a == L2.a || L1.a == L2.b))) X[nX++] = L1.a;
-        if (!(L2.v && (L1.b == L2.a || L1.b == L2.b))) X[nX++] = L1.b;
+        if (!(L2.v && (L1.a == L2.a || L1.a == L2.b))) X.push(L1.a);
+        if (!(L2.v && (L1.b == L2.a || L1.b == L2.b))) X.push(L1.b);
     }
+    const int nX = X.n;
     if (nX == 0) return;
     signed char *pm = reinterpret_cast<signed char *>(lds_raw);
     {
@@ -754,10 +815,7 @@ __device__ inline void lp_fold(const LPArgs &A, int s, int part, char *lds_raw) 
     for (int it = part * TB + threadIdx.x; it < nitems; it += kFoldPerSlot * TB) {
         const int k = it >> 5, role = (it >> 4) & 1, ch = it & 15;
         const int ci = k / (nc + 1), q = k - ci * (nc + 1);
-        int xc = X[0];
-#pragma unroll
-        for (int j = 1; j < 4; ++j)
-            if (j == ci) xc = X[j];
+        const int xc = X[ci];
         const int c = (q == 0) ? xc : A.nbr[(long)xc * nc + q - 1];
         bool dup = false;
         int2 e[4];

@@ -735,33 +735,44 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     int pp[kPropInts];                       // the proposal being accepted
 #pragma unroll
     for (int q = 0; q < kPropInts; ++q) pp[q] = A.prop_acc[kPropInts * s + q];
-    // learned proposal: this lane's candidate of the next step (g / dU at the committed
-    // configuration and with this step's swap made) and the direction before this step
-    float lpgA = 0.f, lpgB = 0.f;
-    int lpdA = INT_MIN, lpdB = INT_MIN, lptB = 0, lpd0 = 1;
+    // learned proposal: the next step's draw is made for either outcome of this step before
+    // the decision, by wave 2 (this step not accepted) and wave 3 (accepted), four candidates
+    // per lane: g / dU at the committed configuration and with this step's swap made, and
+    // the tiles' maxima of the outcome's logits.  Lanes 0..15 of wave 3 take the tiles'
+    // partial sums of this step's log q ratio.
+    const int wave = t >> 6, lane = t & 63;
+    const int lp_nt = A.lp_ncand / chm_lp_dev::MR;
+    const bool drawer = A.lp_lq && A.propose == 4 && wave >= 2;
+    float4 dgA = {}, dgB = {};
+    int4 ddA = make_int4(INT_MIN, INT_MIN, INT_MIN, INT_MIN), ddB = ddA;
+    int dtB = 0, lpd0 = 1;
+    float dm[chm_lp_dev::MAXT];
+    double lqm = -__builtin_huge_val(), lqz = 0.0, lqd = 0.0;
     if (A.lp_lq) {
-        if (A.propose == 4 && t < A.lp_ncand) {
-            const long cb = (long)s * A.lp_ncand;
-            lpgA = A.lp_sg[2 * cb + t];
-            lpdA = A.lp_sdu[2 * cb + t];
-            lpgB = A.lp_sg[2 * cb + A.lp_ncand + t];
-            lpdB = A.lp_sdu[2 * cb + A.lp_ncand + t];
-            lptB = A.lp_st[cb + t];
+        const long cb = (long)s * A.lp_ncand;
+        if (drawer && 4 * lane < A.lp_ncand) {
+            dgA = reinterpret_cast<const float4 *>(A.lp_sg + 2 * cb)[lane];
+            ddA = reinterpret_cast<const int4 *>(A.lp_sdu + 2 * cb)[lane];
+            if (wave == 3) {
+                dgB = reinterpret_cast<const float4 *>(A.lp_sg + 2 * cb + A.lp_ncand)[lane];
+                ddB = reinterpret_cast<const int4 *>(A.lp_sdu + 2 * cb + A.lp_ncand)[lane];
+                dtB = reinterpret_cast<const int *>(A.lp_st + cb)[lane];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < chm_lp_dev::MAXT; ++k)
+            dm[k] = drawer ? A.lp_dmax[((long)s * chm_lp_dev::MAXT + k) * 2 + (wave == 3 ? 0 : 1)] : 0.f;
+        if (wave == 3 && lane < 16 && (lane & 7) < lp_nt) {
+            const double *q = A.lp_lq + ((long)s * chm_lp_dev::MAXT + (lane & 7)) * chm_lp_dev::kLqDoubles +
+                              3 * (lane >> 3);
+            lqm = q[0];
+            lqz = q[1];
+            lqd = q[2];
         }
         if (A.lp_dir_rd) lpd0 = A.lp_dir_rd[s];
     }
-    __shared__ int lp_dnew;
-    // ... the scoring tiles' partial sums of this step's log q ratio and the tile maxima of
-    // the next step's logits, into LDS with the rest of the prologue
-    __shared__ double lp_lqs[chm_lp_dev::MAXT * chm_lp_dev::kLqDoubles];
-    __shared__ float lp_dms[chm_lp_dev::MAXT * 2];
-    if (A.lp_lq) {
-        const int nt = A.lp_ncand / chm_lp_dev::MR;
-        if (t < nt * chm_lp_dev::kLqDoubles)
-            lp_lqs[t] = A.lp_lq[(long)s * chm_lp_dev::MAXT * chm_lp_dev::kLqDoubles + t];
-        if (A.propose == 4 && t >= 64 && t < 64 + 2 * nt)
-            lp_dms[t - 64] = A.lp_dmax[(long)s * chm_lp_dev::MAXT * 2 + t - 64];
-    }
+    __shared__ int lp_dnew, lp_jo[2];
+    __shared__ double lp_lqv;
     double q0v[2 * chm::kMaxNsp];            // per edge T0, Q0 (padded with zeros)
 #pragma unroll
     for (int q = 0; q < 2 * chm::kMaxNsp; ++q) q0v[q] = A.q0[q];
@@ -855,7 +866,40 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     }
     if (A.propose)
         for (int q = t; q < g.nrow_tot; q += nt) hist[q] = 0;
+    if (A.lp_lq && wave == 3) {
+        const double v = chm_lp_dev::lq_merge(lqm, lqz, lqd);
+        if (lane == 0) lp_lqv = v;
+    }
     __syncthreads();
+    // the learned draws of the next step (waves 2 and 3, beside the decision)
+    if (drawer) {
+        const uint64_t hn = lp_stream(A.seed, stepc + 1, rung);
+        if (!lp_coin_uniform(hn, beta, A.lp_eps, A.lp_beta_max)) {
+            const bool oacc = wave == 3;
+            // the direction after this step: kept if it is accepted, reversed if a learned
+            // step is not (a uniform step keeps it)
+            const bool learned_n = A.accept == 1 && !lp_coin_uniform(lp_stream(A.seed, stepc, rung), beta,
+                                                                     A.lp_eps, A.lp_beta_max);
+            const int dout = (oacc || !learned_n) ? lpd0 : -lpd0;
+            float m = -__builtin_huge_valf();
+#pragma unroll
+            for (int k = 0; k < chm_lp_dev::MAXT; ++k)
+                if (k < lp_nt) m = fmaxf(m, dm[k]);
+            const float ga[4] = {dgA.x, dgA.y, dgA.z, dgA.w}, gb[4] = {dgB.x, dgB.y, dgB.z, dgB.w};
+            const int da[4] = {ddA.x, ddA.y, ddA.z, ddA.w}, db[4] = {ddB.x, ddB.y, ddB.z, ddB.w};
+            float lg[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool useB = oacc && ((dtB >> (8 * q)) & 0xFF);
+                lg[q] = (4 * lane + q < A.lp_ncand)
+                            ? chm_lp_dev::logit_of(A.lp_lam, A.lp_kappa, A.lp_lift, useB ? gb[q] : ga[q],
+                                                   useB ? db[q] : da[q], beta, dout)
+                            : -__builtin_huge_valf();
+            }
+            const int j = chm_lp_dev::wave_draw4(lg, m, u01(splitmix64(hn ^ kSelSalt)));
+            if (lane == 0) lp_jo[oacc] = j;
+        }
+    }
     if (A.accept) {
         // ------------------------------------------------ decision (same in every workgroup)
         if (t == 0) {
@@ -870,10 +914,7 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
                                                                  A.lp_eps, A.lp_beta_max);
                 if (pp[4]) {
                     double lq = (double)__int_as_float(pp[5]);
-                    if (learned) {
-                        const int nt = A.lp_ncand / chm_lp_dev::MR;
-                        lq = chm_lp_dev::merge_share(lp_lqs, nt, 3) - chm_lp_dev::merge_share(lp_lqs, nt, 0);
-                    }
+                    if (learned) lq = lp_lqv;
                     const double x = beta * (c2t - c2p) * A.nfit - lq;
                     acc = metropolis(x, u_acc);
                 }
@@ -992,26 +1033,12 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     }
     if (!A.propose) return;
     // ------------------------------------------------ propose + gather of the next step
-    // the learned proposal: a softmax draw over the next step's candidates as the scoring
-    // tiles left them for this step's outcome (every workgroup draws the same one)
+    // the learned proposal: the draw made above for this step's outcome (every workgroup
+    // draws the same one)
     const uint64_t hn = lp_stream(A.seed, stepc + 1, rung);
     const bool learned_next = A.propose == 4 && !lp_coin_uniform(hn, beta, A.lp_eps, A.lp_beta_max);
-    int jl = -1;
-    if (learned_next) {
-        __shared__ double lp_red[4];
-        __shared__ int lp_j;
-        const bool accd = A.accept == 1 && (flag & 1);
-        const int dout = (A.accept == 1) ? lp_dnew : lpd0;
-        const int nt = A.lp_ncand / chm_lp_dev::MR;
-        float m = -__builtin_huge_valf();
-        for (int k = 0; k < nt; ++k) m = fmaxf(m, lp_dms[2 * k + (accd ? 0 : 1)]);
-        const bool useB = accd && lptB;
-        const float lg = (t < A.lp_ncand)
-                             ? chm_lp_dev::logit_of(A.lp_lam, A.lp_kappa, A.lp_lift, useB ? lpgB : lpgA,
-                                                    useB ? lpdB : lpdA, beta, dout)
-                             : -__builtin_huge_valf();
-        jl = chm_lp_dev::softmax_draw<4>(lg, m, u01(splitmix64(hn ^ kSelSalt)), lp_red, &lp_j);
-    }
+    if (!A.accept && learned_next) __syncthreads();   // (no decision barrier since the draws)
+    const int jl = learned_next ? lp_jo[(A.accept == 1 && (flag & 1)) ? 1 : 0] : -1;
     if (t == 0) {
         if (learned_next) {
             if (jl >= 0) {
