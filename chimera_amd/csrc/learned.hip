@@ -32,9 +32,8 @@ __device__ __forceinline__ float bf16_round(float f) { return bf16_to_float(bf16
 __device__ __forceinline__ float bf(__bf16 v) { return bf16_to_float(__builtin_bit_cast(unsigned short, v)); }
 
 // ------------------------------------------------------------------ per-site sums
-// grid (natoms, nrep), HID threads: A and B rows of every site of every slot from scratch,
-// into both copies
-__global__ __launch_bounds__(HID) void k_lp_cache_build(LPArgs A, int *cache0, int *cache1) {
+// grid (natoms, nrep), HID threads: A and B rows of every site of every slot from scratch
+__global__ __launch_bounds__(HID) void k_lp_cache_build(LPArgs A, int *cache) {
     __shared__ int occ[256 + 1];
     const Geo &g = A.g;
     const int c = blockIdx.x, s = blockIdx.y, h = threadIdx.x;
@@ -46,8 +45,7 @@ __global__ __launch_bounds__(HID) void k_lp_cache_build(LPArgs A, int *cache0, i
         int acc = A.w1q[(long)feat_own(g, role, occ[g.ncoord]) * HID + h];
         for (int q = 0; q < g.ncoord; ++q) acc += A.w1q[(long)feat_nbr(g, role, q, occ[q]) * HID + h];
         const long o = (((long)s * 2 + role) * g.natoms + c) * HID + h;
-        cache0[o] = acc;
-        cache1[o] = acc;
+        cache[o] = acc;
     }
 }
 
@@ -297,7 +295,7 @@ Offs offsets(int kp) {
     return o;
 }
 
-// the parts of the scoring / folding arguments that do not depend on the step
+// the parts of the scoring arguments that do not depend on the step
 LPArgs lp_args(chm_sampler *S) {
     const chm_lp &L = S->lp;
     LPArgs A{};
@@ -309,7 +307,6 @@ LPArgs lp_args(chm_sampler *S) {
     A.nbr = S->nbr_st.as<int>();
     A.posmap = L.posmap.as<signed char>();
     A.pm_r = L.pm_r;
-    for (int q = 0; q < 128; ++q) A.opp[q] = L.opp[q];
     for (int q = 0; q < kMaxFs; ++q) A.fs[q] = L.fs[q];
     A.n1 = L.n1;
     A.seed = S->seed;
@@ -456,20 +453,19 @@ int lp_prepare(chm_sampler *S) {
     L.n1 = 0;
     for (int q = 0; q < g.ncoord; ++q)
         if (((ss[q] >> 24) & 0xFF) == 0) L.fs[L.n1++] = (signed char)q;
-    // the opposite offset of every stencil position (the stencil of a Bravais lattice holds
-    // both)
-    for (int q = 0; q < g.ncoord; ++q) {
-        const int o = at(-comp(ss[q], 0), -comp(ss[q], 1), -comp(ss[q], 2));
-        if (o < 0) return chm::set_error("learned proposal: the stencil is not symmetric"), CHM_E_ARG;
-        L.opp[q] = (signed char)o;
-    }
+    // the fold of a swap visits the stencil of each swapped site for the sites whose stencil
+    // holds it: the same set only for a stencil that holds every offset's opposite (as that
+    // of a Bravais lattice does)
+    for (int q = 0; q < g.ncoord; ++q)
+        if (at(-comp(ss[q], 0), -comp(ss[q], 1), -comp(ss[q], 2)) < 0)
+            return chm::set_error("learned proposal: the stencil is not symmetric"), CHM_E_ARG;
     const size_t nr = (size_t)S->nrep, nc = MAXC;
     int rc;
     if ((rc = L.params.ensure(4 * (size_t)L.nparam)) || (rc = L.adam_m.ensure(4 * (size_t)L.nparam)) ||
         (rc = L.adam_v.ensure(4 * (size_t)L.nparam)) || (rc = L.w1q.ensure(4 * (size_t)L.kp * HID)) ||
         (rc = L.w1.ensure(2 * (size_t)L.kp * HID)) || (rc = L.w2t.ensure(2 * (size_t)HID * HID)) ||
         (rc = L.w2.ensure(2 * (size_t)HID * HID)) || (rc = L.w3q.ensure(4 * HID)) ||
-        (rc = L.posmap.ensure(pm.size())) || (rc = L.rec.ensure(4 * 2 * kRecInts * nr)) ||
+        (rc = L.posmap.ensure(pm.size())) ||
         (rc = L.sg.ensure(4 * 2 * 2 * nr * nc)) || (rc = L.sdu.ensure(4 * 2 * 2 * nr * nc)) ||
         (rc = L.st8.ensure(2 * nr * nc)) || (rc = L.cs.ensure(4 * nr)) ||
         (rc = L.lq.ensure(8 * nr * MAXT * kLqDoubles)) || (rc = L.dmax.ensure(4 * nr * MAXT * 2)) ||
@@ -477,56 +473,43 @@ int lp_prepare(chm_sampler *S) {
         return rc;
     CHM_HIP(hipMemcpyAsync(L.posmap.p, pm.data(), pm.size(), hipMemcpyHostToDevice, st));
     CHM_HIP(hipMemsetAsync(L.pend_occ.p, 0, L.pend_occ.bytes, st));
-    CHM_HIP(hipMemsetAsync(L.rec.p, 0, L.rec.bytes, st));
     CHM_HIP(hipMemsetAsync(L.cs.p, 0, L.cs.bytes, st));
     CHM_HIP(hipMemsetAsync(L.st8.p, 0, L.st8.bytes, st));
     CHM_HIP(hipStreamSynchronize(st));
     return CHM_OK;
 }
 
-// The two copies of the per-site layer-1 sums of every slot from its species (after a
-// change of W1, of the species from the host, or of the proposal mode), no swap pending
+// The per-site layer-1 sums of every slot from its species (after a change of W1, of the
+// species from the host, or of the proposal mode)
 int lp_ensure_cache(chm_sampler *S) {
     chm_lp &L = S->lp;
     if (L.cache_ok) return CHM_OK;
-    const size_t one = 4 * (size_t)S->nrep * 2 * S->g.natoms * HID;
-    int rc = L.cache.ensure(2 * one);
+    int rc = L.cache.ensure(4 * (size_t)S->nrep * 2 * S->g.natoms * HID);
     if (rc) return rc;
     LPArgs A = lp_args(S);
-    int *c0 = L.cache.as<int>();
-    k_lp_cache_build<<<dim3(S->g.natoms, S->nrep), HID, 0, S->ctx->stream>>>(A, c0, c0 + one / 4);
+    k_lp_cache_build<<<dim3(S->g.natoms, S->nrep), HID, 0, S->ctx->stream>>>(A, L.cache.as<int>());
     CHM_LAUNCH_CHECK();
-    CHM_HIP(hipMemsetAsync(L.rec.p, 0, L.rec.bytes, S->ctx->stream));
     L.cache_ok = true;
     return CHM_OK;
 }
 
 size_t lp_tile_lds(const chm_sampler *S) { return tile_lds(S->g.natoms_pad); }
 
-// The scoring / folding arguments of the transform launch of step n (mode 1; 2 for the
-// last step of a segment) or of a segment's first scoring launch after n steps (mode 0)
+// The scoring arguments of the transform launch of step n (mode 1; 2 for the last step of a
+// segment) or of a segment's first scoring launch after n steps (mode 0)
 LPArgs lp_step_args(chm_sampler *S, int mode, long long n) {
     chm_lp &L = S->lp;
     LPArgs A = lp_args(S);
     A.mode = mode;
     A.stepc = S->stepc.as<uint64_t>();
     A.lpn = L.lpn.as<uint64_t>();
-    const size_t one = (size_t)S->nrep * 2 * S->g.natoms * HID;
-    int *c0 = L.cache.as<int>();
+    A.cache = L.cache.as<int>();
     const size_t np = (size_t)S->nrep * 2 * MAXC, nt = (size_t)S->nrep * MAXC;
     const long long nx = n + 1;
-    int *rec = L.rec.as<int>();
-    auto recp = [&](long long k) { return rec + (size_t)(k & 1) * kRecInts * S->nrep; };
     int *dir = L.dir.as<int>();
     if (mode == 0) {
-        A.cache_rd = c0 + (size_t)(nx & 1) * one;
-        A.lag = recp(n);
         A.dir = dir + (size_t)(nx & 1) * S->nrep;
     } else {
-        A.cache_rd = c0 + (size_t)(n & 1) * one;
-        A.cache_wr = c0 + (size_t)(nx & 1) * one;
-        A.lag = recp(n - 1);
-        A.lag2 = recp(n);
         A.prop_n = prow(S, n);
         A.dir = dir + (size_t)(n & 1) * S->nrep;
         A.sg_in = L.sg.as<float>() + (size_t)(n & 1) * np;
@@ -538,7 +521,6 @@ LPArgs lp_step_args(chm_sampler *S, int mode, long long n) {
     A.st_out = L.st8.as<int8_t>() + (size_t)(nx & 1) * nt;
     const int slots8 = (S->nrep + 7) / 8 * 8;
     A.ntile_blocks = (L.ncand / MR) * slots8;
-    A.nfold_blocks = mode == 0 ? 0 : kFoldPerSlot * slots8;
     return A;
 }
 
@@ -739,38 +721,28 @@ extern "C" int chm_lp_profile(chm_sampler *S, int iters, double *ms) {
     if (rc) return rc;
     // the scoring workgroups of a step on the current state (every slot learned, the last
     // step's proposal as the swap; they rewrite only buffers the next segment's first
-    // launches rewrite again), then the folding workgroups alone; the per-site sums are
-    // rebuilt before the next run
+    // launches rewrite again).  ms[1] = 0: an accepted swap is folded into the per-site
+    // sums inside the per-replica kernel, with no launch of its own.
     const double eps = L.eps, bmax = L.beta_max;
     L.eps = 0.0;
     L.beta_max = 1e300;
-    LPArgs A = lp_step_args(S, 1, S->hstep);
+    const LPArgs A = lp_step_args(S, 1, S->hstep);
     L.eps = eps;
     L.beta_max = bmax;
-    LPArgs As = A, Af = A;
-    As.nfold_blocks = 0;
-    Af.ntile_blocks = 0;
-    hipEvent_t e0, e1, e2;
+    hipEvent_t e0, e1;
     CHM_HIP(hipEventCreate(&e0));
     CHM_HIP(hipEventCreate(&e1));
-    CHM_HIP(hipEventCreate(&e2));
     CHM_HIP(hipEventRecord(e0, st));
-    for (int i = 0; i < iters; ++i)
-        k_lp_tiles<<<(unsigned)As.ntile_blocks, TB, lp_tile_lds(S), st>>>(As);
-    CHM_HIP(hipEventRecord(e1, st));
-    for (int i = 0; i < iters; ++i) k_lp_tiles<<<(unsigned)Af.nfold_blocks, TB, lp_tile_lds(S), st>>>(Af);
+    for (int i = 0; i < iters; ++i) k_lp_tiles<<<(unsigned)A.ntile_blocks, TB, lp_tile_lds(S), st>>>(A);
     CHM_LAUNCH_CHECK();
-    L.cache_ok = false;
-    CHM_HIP(hipEventRecord(e2, st));
-    CHM_HIP(hipEventSynchronize(e2));
-    float t1 = 0.f, t2 = 0.f;
+    CHM_HIP(hipEventRecord(e1, st));
+    CHM_HIP(hipEventSynchronize(e1));
+    float t1 = 0.f;
     CHM_HIP(hipEventElapsedTime(&t1, e0, e1));
-    CHM_HIP(hipEventElapsedTime(&t2, e1, e2));
     ms[0] = t1 / iters;
-    ms[1] = t2 / iters;
+    ms[1] = 0.0;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    hipEventDestroy(e2);
     return CHM_OK;
 }
 

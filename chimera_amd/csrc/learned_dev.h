@@ -15,11 +15,10 @@
 // B[c] with b's weight blocks.  W1q is bf16(W1) rounded to a multiple of 2^-16 and clamped
 // to |w| <= 2, held as integers, so every sum is exact (int32, and below 2^24, exact in FP32
 // too): the device keeps A and B of every site of every replica slot and updates them by
-// the weight rows a swap changes, with no drift.  Two copies alternate by step: the
-// transform of step n reads copy n & 1 (the configuration before step n - 1, whose swap it
-// applies on the fly) and brings copy (n + 1) & 1 up to the configuration after step n - 1
-// by folding in the swaps of steps n - 2 and n - 1; nothing reads a copy while it is
-// written.
+// the weight rows a swap changes, with no drift.  The per-replica kernel that accepts a
+// swap folds it in (its workgroups share the rows), so the next transform's scoring
+// workgroups read the sums of the committed configuration; nothing reads them while they
+// are written.
 //
 // Speculative scoring.  The scores of step n+1's candidates depend on whether step n is
 // accepted; the scoring workgroups of step n's transform launch score them at the
@@ -50,10 +49,9 @@ constexpr int TB = 256;       // threads of a scoring / folding workgroup
 constexpr int TW = TB / 64;   // its waves
 constexpr int kMaxSpW = 8;    // species dimension of the bond-weight table
 constexpr int kMaxFs = 16;    // first-shell stencil entries, at most
-constexpr int kNCorr = 8;     // layer-1 corrections of a row: 2 candidate sites x 4 changed sites
-constexpr int kRecInts = 8;   // swap record: valid, a, b, old species of a, old species of b
+constexpr int kNCorr = 4;     // layer-1 corrections of a row: 2 candidate sites x 2 swapped sites
 constexpr int kLqDoubles = 6; // per tile: forward (max, Z, D) and reverse (max, Z, D) of log q
-constexpr int kFoldPerSlot = 8;   // folding workgroups per replica slot
+constexpr int kFoldPerSlot = 8;   // folding workgroups per replica slot (per-replica kernel)
 constexpr int MP = 48;        // rows of one network pass: a tile's candidates, then re-scored ones
 constexpr int kPosMapMax = 4096;  // bytes of the offset -> stencil position cube staged in LDS
 constexpr float kW1Inv = 1.f / 65536.f;
@@ -64,9 +62,9 @@ struct LPW {
     const float *b1, *b2, *w3q, *b3;
 };
 
-// Everything the scoring and folding workgroups of one launch read; built by the host
-// for one step (mode 1: step n with step n+1's candidates; 2: step n only, the last of a
-// segment; 0: a segment's first candidates after n steps, no fold).
+// Everything the scoring workgroups of one launch read; built by the host for one step
+// (mode 1: step n with step n+1's candidates; 2: step n only, the last of a segment; 0: a
+// segment's first candidates after n steps).
 struct LPArgs {
     Geo g;
     const uint8_t *species;
@@ -76,7 +74,6 @@ struct LPArgs {
     const int *nbr;            // (natoms, ncoord) neighbour sites in stencil order
     const signed char *posmap; // stencil position of an offset (cube of side 2 pm_r + 1), or -1
     int pm_r;
-    signed char opp[128];      // stencil position of the opposite offset of each position
     signed char fs[kMaxFs];    // the first-shell stencil positions (n1 of them)
     int n1;
     uint64_t seed;
@@ -93,10 +90,7 @@ struct LPArgs {
     // before this one set up (mode 1, 2)
     const uint64_t *stepc;
     const uint64_t *lpn;
-    const int *cache_rd;       // (nrep, 2, natoms, HID): the configuration before the lag swap
-    int *cache_wr;             // the other copy: brought up to the committed configuration
-    const int *lag;            // (nrep, kRecInts) the swap of the last decided step (or null rec)
-    const int *lag2;           // the one before (folded together with lag), mode 1 / 2
+    const int *cache;          // (nrep, 2, natoms, HID): the sums of the committed configuration
     const int *prop_n;         // (nrep, kPropInts) step n's proposal
     const int *cs;             // (nrep) 1: step n - 1's swap was accepted (mode 1 / 2)
     const float *sg_in;        // step n's candidates as drawn: (nrep, 2, ncand) g at x (0) and
@@ -108,7 +102,7 @@ struct LPArgs {
     double *lq;                // (nrep, MAXT, kLqDoubles) partial sums of step n's log q
     float *dmax;               // (nrep, MAXT, 2) tile maxima of step n+1's logits: accepted,
                                //   not accepted
-    int ntile_blocks, nfold_blocks;
+    int ntile_blocks;
 };
 
 __device__ __forceinline__ float silu_fast(float z) { return z * __frcp_rn(1.f + __expf(-z)); }
@@ -126,12 +120,12 @@ __device__ __forceinline__ int dist2_sites(const Geo &g, int p, int q) {
 }
 // stencil position of site x around site c, or -1 (x is not one of c's stencil neighbours);
 // pm: the offset -> position cube (in LDS)
-__device__ __forceinline__ int stencil_pos(const LPArgs &A, const signed char *pm, int c, int x) {
+__device__ __forceinline__ int stencil_pos(const Geo &g, int pm_r, const signed char *pm, int c, int x) {
     int cx, cy, cz, xx, xy, xz;
-    site_xyz_of(A.g, c, cx, cy, cz);
-    site_xyz_of(A.g, x, xx, xy, xz);
-    const int dx = centred(xx - cx, A.g.box[0]), dy = centred(xy - cy, A.g.box[1]), dz = centred(xz - cz, A.g.box[2]);
-    const int R = A.pm_r, side = 2 * R + 1;
+    site_xyz_of(g, c, cx, cy, cz);
+    site_xyz_of(g, x, xx, xy, xz);
+    const int dx = centred(xx - cx, g.box[0]), dy = centred(xy - cy, g.box[1]), dz = centred(xz - cz, g.box[2]);
+    const int R = pm_r, side = 2 * R + 1;
     if (dx < -R || dx > R || dy < -R || dy > R || dz < -R || dz > R) return -1;
     return pm[((dx + R) * side + (dy + R)) * side + (dz + R)];
 }
@@ -150,27 +144,13 @@ __device__ __forceinline__ int feat_nbr(const Geo &g, int role, int p, int s) {
 __device__ __forceinline__ int feat_own(const Geo &g, int role, int s) { return (2 * g.ncoord + role) * g.nsp + s; }
 
 // A swap record: sites a, b and their species before it (v = 0: no swap)
-struct Rec {
-    int v, a, b, oa, ob;
-};
-__device__ __forceinline__ Rec load_rec(const int *p) { return Rec{p[0], p[1], p[2], p[3], p[4]}; }
-// species of x before swap r, given its species after it
-__device__ __forceinline__ int undo(const Rec &r, int x, int sp) {
-    if (r.v && x == r.a) return r.oa;
-    if (r.v && x == r.b) return r.ob;
-    return sp;
-}
-
-// The layer-1 correction of candidate site c (role) for a changed site x: the weight row it
-// holds at the target state minus the one the copy summed; (-1, -1) when x is neither c nor
-// one of c's stencil neighbours, or its species did not change.
-__device__ __forceinline__ int2 corr_entry(const LPArgs &A, const signed char *pm, int c, int role, int x, int s_tgt,
-                                           int s_old) {
+__device__ __forceinline__ int2 corr_entry(const Geo &g, int pm_r, const signed char *pm, int c, int role, int x,
+                                           int s_tgt, int s_old) {
     if (s_tgt == s_old) return make_int2(-1, -1);
-    if (x == c) return make_int2(feat_own(A.g, role, s_tgt), feat_own(A.g, role, s_old));
-    const int p = stencil_pos(A, pm, c, x);
+    if (x == c) return make_int2(feat_own(g, role, s_tgt), feat_own(g, role, s_old));
+    const int p = stencil_pos(g, pm_r, pm, c, x);
     if (p < 0) return make_int2(-1, -1);
-    return make_int2(feat_nbr(A.g, role, p, s_tgt), feat_nbr(A.g, role, p, s_old));
+    return make_int2(feat_nbr(g, role, p, s_tgt), feat_nbr(g, role, p, s_old));
 }
 
 __device__ __forceinline__ void add_row16(int (&acc)[16], const int *row, int sign) {
@@ -239,6 +219,7 @@ __device__ inline void layer2(const LPW &W, const __bf16 *H, int nrt, W2Half &F,
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
         if (half == 1) load_w2_half(W, 1, F);
+        const W2Half &FH = F;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int kt = 4 * half + k;
@@ -248,7 +229,7 @@ __device__ inline void layer2(const LPW &W, const __bf16 *H, int nrt, W2Half &F,
                 const bf16x8 a = *reinterpret_cast<const bf16x8 *>(H + (16 * r + lr) * LDH + 32 * kt + lk);
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, F.B[k][c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, FH.B[k][c], acc[r][c], 0, 0, 0);
             }
         }
     }
@@ -291,19 +272,6 @@ __device__ __forceinline__ void h1_store(const LPW &W, __bf16 *H, int r, int ch,
     *reinterpret_cast<bf16x8 *>(H + r * LDH + 16 * ch + 8) = v1;
 }
 
-// Up to four changed sites in registers (appended in order; no dynamically indexed array)
-struct Sites4 {
-    int x0 = 0, x1 = 0, x2 = 0, x3 = 0, n = 0;
-    __device__ __forceinline__ void push(int v) {
-        x0 = (n == 0) ? v : x0;
-        x1 = (n == 1) ? v : x1;
-        x2 = (n == 2) ? v : x2;
-        x3 = (n == 3) ? v : x3;
-        ++n;
-    }
-    __device__ __forceinline__ int operator[](int i) const { return i == 0 ? x0 : i == 1 ? x1 : i == 2 ? x2 : x3; }
-};
-
 // Rows of one network pass, in LDS: candidate (a, b) at the committed state (v = 0) or with
 // the proposal swapped (v = 1); dU and g per row, and the flattened list of layer-1
 // corrections (row, new weight row, old weight row).
@@ -339,14 +307,14 @@ struct RowSrc {
     }
 };
 
-// g and dU of rows [q0, q0 + nrows) (nrows <= MP) of the list RS of slot s.  The copy
-// `cache_rd` holds the configuration before the lag swap; the committed species (lsp) have
-// it made; rows with v = 1 also have the proposal (pa, pb) made.  fs: first-shell stencil
+// g and dU of rows [q0, q0 + nrows) (nrows <= MP) of the list RS of slot s.  The per-site
+// sums (`cache`) and the species (lsp) are those of the committed configuration; rows with
+// v = 1 have the proposal (pa, pb) made.  fs: first-shell stencil
 // positions (n1); wl: bond weights; pm: offset -> stencil position cube; zl: MP x HID int32
 // of LDS (the layer-1 sums; H1 is written over them).  The per-site rows are requested
 // before the first barrier.
-__device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const signed char *pm, const Rec &L, int pa,
-                                  int pb, const int *fs, int n1, const signed char *wl, int s, const RowSrc &RS,
+__device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const signed char *pm, int pa, int pb,
+                                  const int *fs, int n1, const signed char *wl, int s, const RowSrc &RS,
                                   int q0, int nrows, Rows &R, int *zl) {
     const Geo &g = A.g;
     const int t = threadIdx.x, lane = t & 63;
@@ -362,7 +330,7 @@ __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const sig
     // stage A: the two per-site rows of each row, one wave per row (rows wave, wave + 4, ...),
     // every load of a lane issued before any is used (a missing row reads nothing: zero)
     constexpr int RW = MP / TW;
-    const Buf bc = make_buf(A.cache_rd + (long)s * 2 * g.natoms * HID, 4u * 2u * (uint32_t)g.natoms * HID);
+    const Buf bc = make_buf(A.cache + (long)s * 2 * g.natoms * HID, 4u * 2u * (uint32_t)g.natoms * HID);
     // (lane k names row wave + TW k; its sites are read back as wave-uniform values)
     int ral = 0, rbl = 0;
     if (lane < RW && wave + TW * lane < nrows) {
@@ -400,35 +368,18 @@ __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const sig
             reinterpret_cast<int4 *>(zl + r * HID)[lane] =
                 make_int4(va[k].x + vb[k].x, va[k].y + vb[k].y, va[k].z + vb[k].z, va[k].w + vb[k].w);
     }
-    // changed sites relative to the copy: the lag swap's, then the proposal's not among them
-    Sites4 X;
-    if (L.v) {
-        X.push(L.a);
-        X.push(L.b);
-    }
-    const int nx = X.n;
-    if (pa >= 0) {
-        if (!(L.v && (pa == L.a || pa == L.b))) X.push(pa);
-        if (!(L.v && (pb == L.a || pb == L.b))) X.push(pb);
-    }
-    const int nxv = X.n;
-    // corrections: (row, candidate site, changed site), two per thread, appended to the list
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int q = t + TB * h, r = q >> 3, k = q & 7, role = k >> 2, xi = k & 3;
-        if (r < nrows) {
-            const int v = R.v[r];
+    // corrections of the rows with the proposal made: (row, candidate site, swapped site),
+    // four per row, appended to the list
+    {
+        const int r = t >> 2, role = (t >> 1) & 1, x = (t & 1) ? pb : pa;
+        if (r < nrows && R.v[r] && pa >= 0) {
             const int c = role ? R.b[r] : R.a[r];
-            if (xi < (v ? nxv : nx)) {
-                const int x = X[xi];
-                const int st = v ? species_swapped(lsp, x, pa, pb) : (int)lsp[x];
-                const int2 e = corr_entry(A, pm, c, role, x, st, undo(L, x, lsp[x]));
-                if (e.x >= 0) {
-                    const int slot = atomicAdd(&R.ncl, 1);
-                    R.crow[slot] = r;
-                    R.cnew[slot] = e.x;
-                    R.cold[slot] = e.y;
-                }
+            const int2 e = corr_entry(g, A.pm_r, pm, c, role, x, species_swapped(lsp, x, pa, pb), lsp[x]);
+            if (e.x >= 0) {
+                const int slot = atomicAdd(&R.ncl, 1);
+                R.crow[slot] = r;
+                R.cnew[slot] = e.x;
+                R.cold[slot] = e.y;
             }
         }
     }
@@ -499,17 +450,18 @@ __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const sig
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < RW; ++k) {
-            const int r = wave + TW * k;
+            const int r = wave + TW * k;   // (wave-uniform)
             if (r >= MP) continue;
-            const bool on = r < nrows;
-            unsigned short h[4];
-            const int zz[4] = {zv[k].x, zv[k].y, zv[k].z, zv[k].w};
-            const float bb[4] = {b1.x, b1.y, b1.z, b1.w};
+            uint2 pk = make_uint2(0u, 0u);
+            if (r < nrows) {
+                unsigned short h[4];
+                const int zz[4] = {zv[k].x, zv[k].y, zv[k].z, zv[k].w};
+                const float bb[4] = {b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
-            for (int i = 0; i < 4; ++i) h[i] = on ? bf16_rne(silu_fast((float)zz[i] * kW1Inv + bb[i])) : 0;
-            uint2 pk;
-            pk.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
-            pk.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
+                for (int i = 0; i < 4; ++i) h[i] = bf16_rne(silu_fast((float)zz[i] * kW1Inv + bb[i]));
+                pk.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
+                pk.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
+            }
             *reinterpret_cast<uint2 *>(H + r * LDH + 4 * lane) = pk;
         }
     }
@@ -622,11 +574,13 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
     signed char *pm = reinterpret_cast<signed char *>(lds_raw + ((g.natoms_pad + 15) & ~15) + (size_t)MP * HID * 4);
     const int i0 = tile * MR;
     const long cbase = (long)s * A.ncand;
+    // (the scoring waves are the launch's critical path: issued ahead of the transform's
+    // waves that share their SIMDs)
+    __builtin_amdgcn_s_setprio(2);
     // requested before anything is known of the step (one round trip), the small loads the
-    // row list needs first: the step, the slot's rung and beta, step n's proposal, the lag
-    // swap and direction; then the species and the offset cube (into registers: stored into
-    // LDS once the rows are requested) and this lane's candidate of step n as step n - 1 left
-    // it
+    // row list needs first: the step, the slot's rung and beta, step n's proposal and the
+    // direction; then the species and the offset cube (into registers) and this lane's
+    // candidate of step n as step n - 1 left it
     const int rung = A.rung_of[s];
     const double beta = A.beta_slot[s];
     const uint64_t n = (A.mode == 0) ? A.stepc[s] : A.lpn[s], nx = n + 1;
@@ -635,7 +589,6 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
         const int *pp = A.prop_n + kPropInts * s;
         P = Prop{pp[0], pp[1], pp[2], pp[3], pp[4]};
     }
-    const Rec LG = load_rec(A.lag + kRecInts * s);
     const int d0 = A.lift ? A.dir[s] : 1;
     const int nsv = g.natoms_pad / 16;
     const uint4 *gsp = reinterpret_cast<const uint4 *>(A.species + (long)s * g.natoms_pad);
@@ -700,7 +653,7 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
     if (t < npm16) reinterpret_cast<uint4 *>(pm)[t] = pmv;
     for (int q0 = 0; q0 < ntot; q0 += MP) {
         const int nr = min(MP, ntot - q0);
-        score_rows(A, lsp, pm, LG, P.a, P.b, fs, n1, wl, s, RS, q0, nr, R, zl);
+        score_rows(A, lsp, pm, P.a, P.b, fs, n1, wl, s, RS, q0, nr, R, zl);
         if (t < nr) {
             const int q = q0 + t;
             const float gv = R.gout[t];
@@ -773,105 +726,80 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
     }
 }
 
-// One folding workgroup (part `part` of kFoldPerSlot for slot s, TB threads): brings the
-// copy cache_wr from the configuration before swaps lag2 and lag up to the committed one.
-// A swap changes the rows of its sites and of their stencil neighbours.  Work item = (site,
-// role, 16 columns) over the changed sites and their stencil neighbours, each site taken
-// under the first changed site that reaches it (so no row is written twice); the item adds
-// the weight-row changes of every changed site that is the site itself or one of its
-// stencil neighbours.
-__device__ inline void lp_fold(const LPArgs &A, int s, int part, char *lds_raw) {
-    const Geo &g = A.g;
-    const int nc = g.ncoord;
-    const Rec L1 = load_rec(A.lag + kRecInts * s), L2 = load_rec(A.lag2 + kRecInts * s);
-    Sites4 X;
-    if (L2.v) {
-        X.push(L2.a);
-        X.push(L2.b);
-    }
-    if (L1.v) {
-        if (!(L2.v && (L1.a == L2.a || L1.a == L2.b))) X.push(L1.a);
-        if (!(L2.v && (L1.b == L2.a || L1.b == L2.b))) X.push(L1.b);
-    }
-    const int nX = X.n;
-    if (nX == 0) return;
-    signed char *pm = reinterpret_cast<signed char *>(lds_raw);
-    {
-        const int side = 2 * A.pm_r + 1, npm = side * side * side;
-        for (int q = threadIdx.x; q < npm; q += TB) pm[q] = A.posmap[q];
-    }
-    const uint8_t *gsp = A.species + (long)s * g.natoms_pad;
-    int snow[4], sold[4];
+// Folding swap (a, b) into a slot's per-site layer-1 sums (cache: the slot's 2 x natoms x HID
+// block): site a takes species na (it held oa), b takes nb (it held ob).  The swap changes
+// the rows of a, b and their stencil neighbours.  Work item it = (site, role, 16 columns)
+// over a's and b's stencil (a's first): k = it >> 5 names the site (the stencil centre
+// ci = k / (ncoord + 1), position q = k mod (ncoord + 1), 0: the centre itself), each site
+// taken under the first of a, b that reaches it (so no row is written twice); the item adds
+// the weight-row changes of a and b where they are the site itself or one of its stencil
+// neighbours.  An item is done in three steps: its site (a neighbour-table load), its rows
+// (pm: the offset cube in LDS), and the store.
+__device__ __forceinline__ int fold_nitems(const Geo &g) { return 2 * (g.ncoord + 1) * 32; }
+__device__ __forceinline__ int fold_item_site(const Geo &g, const int *__restrict__ nbr, int a, int b, int it) {
+    const int nc = g.ncoord, k = it >> 5;
+    const int ci = k / (nc + 1), q = k - ci * (nc + 1);
+    const int xc = ci ? b : a;
+    return (q == 0) ? xc : nbr[(long)xc * nc + q - 1];
+}
+struct FoldItem {
+    int4 v[4], an[4], ao[4], bn[4], bo[4];   // the row's 16 columns; weight rows of a, b (new, old)
+    int *dst;
+    bool on;
+};
+__device__ inline void fold_item_load(const Geo &g, int pm_r, const signed char *pm, const int *w1q, int *cache,
+                                      int a, int b, int na, int nb, int oa, int ob, int it, int c, FoldItem &F) {
+    const int nc = g.ncoord, k = it >> 5, role = (it >> 4) & 1, ch = it & 15;
+    const int ci = k / (nc + 1);
+    // positions of a and b around c (the site itself: -2, outside its stencil: -1)
+    const int pa = (c == a) ? -2 : stencil_pos(g, pm_r, pm, c, a);
+    const int pb = (c == b) ? -2 : stencil_pos(g, pm_r, pm, c, b);
+    const bool owner = !(ci == 1 && pa != -1);   // a's items own the sites both reach
+    const bool ua = owner && pa != -1 && na != oa, ub = owner && pb != -1 && nb != ob;
+    const int fan = ua ? (pa == -2 ? feat_own(g, role, na) : feat_nbr(g, role, pa, na)) : 0;
+    const int fao = ua ? (pa == -2 ? feat_own(g, role, oa) : feat_nbr(g, role, pa, oa)) : 0;
+    const int fbn = ub ? (pb == -2 ? feat_own(g, role, nb) : feat_nbr(g, role, pb, nb)) : 0;
+    const int fbo = ub ? (pb == -2 ? feat_own(g, role, ob) : feat_nbr(g, role, pb, ob)) : 0;
+    F.on = ua || ub;
+    const long ro = ((long)role * g.natoms + c) * HID + 16 * ch;
+    F.dst = cache + ro;
+    const Buf bc = make_buf(cache, 4u * 2u * (uint32_t)g.natoms * HID);
+    const Buf bw = make_buf(w1q, 4u * (uint32_t)(2 * nc + 2) * (uint32_t)g.nsp * HID);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        snow[k] = sold[k] = 0;
-        if (k < nX) {
-            snow[k] = gsp[X[k]];
-            sold[k] = undo(L2, X[k], undo(L1, X[k], snow[k]));
-        }
-    }
-    __syncthreads();
-    const int nitems = nX * (nc + 1) * 32;
-    for (int it = part * TB + threadIdx.x; it < nitems; it += kFoldPerSlot * TB) {
-        const int k = it >> 5, role = (it >> 4) & 1, ch = it & 15;
-        const int ci = k / (nc + 1), q = k - ci * (nc + 1);
-        const int xc = X[ci];
-        const int c = (q == 0) ? xc : A.nbr[(long)xc * nc + q - 1];
-        bool dup = false;
-        int2 e[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            e[j] = make_int2(-1, -1);
-            if (j < nX) {
-                // position of changed site j around c (own: -2, none: -1)
-                const int pj = (c == X[j]) ? -2 : stencil_pos(A, pm, c, X[j]);
-                if (j < ci && pj != -1) dup = true;   // an earlier changed site owns c
-                if (pj != -1 && snow[j] != sold[j])
-                    e[j] = pj == -2 ? make_int2(feat_own(g, role, snow[j]), feat_own(g, role, sold[j]))
-                                    : make_int2(feat_nbr(g, role, pj, snow[j]), feat_nbr(g, role, pj, sold[j]));
-            }
-        }
-        if (dup || (e[0].x < 0 && e[1].x < 0 && e[2].x < 0 && e[3].x < 0)) continue;
-        int acc[16];
-        int *row = A.cache_wr + (((long)s * 2 + role) * g.natoms + c) * HID + 16 * ch;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) acc[i] = 0;
-        add_row16(acc, row, 1);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (e[j].x >= 0) {
-                add_row16(acc, A.w1q + (long)e[j].x * HID + 16 * ch, 1);
-                add_row16(acc, A.w1q + (long)e[j].y * HID + 16 * ch, -1);
-            }
-        int4 *dst = reinterpret_cast<int4 *>(row);
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) dst[qq] = make_int4(acc[4 * qq], acc[4 * qq + 1], acc[4 * qq + 2], acc[4 * qq + 3]);
+    for (int qq = 0; qq < 4; ++qq) {
+        F.v[qq] = buf_i128(bc, F.on, 4u * (uint32_t)ro + 16u * qq);
+        F.an[qq] = buf_i128(bw, ua, 4u * ((uint32_t)fan * HID + 16 * ch) + 16u * qq);
+        F.ao[qq] = buf_i128(bw, ua, 4u * ((uint32_t)fao * HID + 16 * ch) + 16u * qq);
+        F.bn[qq] = buf_i128(bw, ub, 4u * ((uint32_t)fbn * HID + 16 * ch) + 16u * qq);
+        F.bo[qq] = buf_i128(bw, ub, 4u * ((uint32_t)fbo * HID + 16 * ch) + 16u * qq);
     }
 }
+__device__ __forceinline__ void fold_item_store(const FoldItem &F) {
+    if (!F.on) return;
+    int4 *dst = reinterpret_cast<int4 *>(F.dst);
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq)
+        dst[qq] = make_int4(F.v[qq].x + F.an[qq].x - F.ao[qq].x + F.bn[qq].x - F.bo[qq].x,
+                            F.v[qq].y + F.an[qq].y - F.ao[qq].y + F.bn[qq].y - F.bo[qq].y,
+                            F.v[qq].z + F.an[qq].z - F.ao[qq].z + F.bn[qq].z - F.bo[qq].z,
+                            F.v[qq].w + F.an[qq].w - F.ao[qq].w + F.bn[qq].w - F.bo[qq].w);
+}
 
-// The scoring and folding workgroups of a launch: block index b in [0, ntile_blocks) is
-// a scoring tile, [ntile_blocks, ntile_blocks + nfold_blocks) a folding part; the ones of
-// one slot sit on one XCD (consecutive ids go to the eight XCDs in turn).
+// Scoring workgroup b of a launch: slot (b & 7) + 8 m, tile j - m ntile (j = b >> 3, m = j /
+// ntile), so the tiles of a slot run on the XCD of that slot's per-replica workgroups
 __device__ inline void lp_block(const LPArgs &A, int b, char *lds_raw) {
     const int ntile = A.ncand / MR;
-    if (b < A.ntile_blocks) {
-        const int j = b >> 3, m = j / ntile;
-        const int tile = j - m * ntile, s = (b & 7) + 8 * m;
-        if (s < A.nrep) lp_tile(A, s, tile, lds_raw);
-        return;
-    }
-    b -= A.ntile_blocks;
-    if (b >= A.nfold_blocks) return;
-    const int j = b >> 3, m = j / kFoldPerSlot;
-    const int part = j - m * kFoldPerSlot, s = (b & 7) + 8 * m;
-    if (s < A.nrep) lp_fold(A, s, part, lds_raw);
+    if (b >= A.ntile_blocks) return;
+    const int j = b >> 3, m = j / ntile;
+    const int tile = j - m * ntile, s = (b & 7) + 8 * m;
+    if (s < A.nrep) lp_tile(A, s, tile, lds_raw);
 }
 
 }  // namespace chm_lp_dev
 
 struct chm_sampler;
 namespace chm_smp {
-// learned.hip: the scoring / folding arguments of the transform launch of step n (mode 1;
-// 2 for a segment's last step) or of a segment's first scoring launch after n steps (0)
+// learned.hip: the scoring arguments of the transform launch of step n (mode 1; 2 for a
+// segment's last step) or of a segment's first scoring launch after n steps (0)
 chm_lp_dev::LPArgs lp_step_args(chm_sampler *S, int mode, long long n);
 }  // namespace chm_smp

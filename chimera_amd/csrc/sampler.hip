@@ -236,12 +236,11 @@ __device__ __forceinline__ void gemm_abs_body(const Geo &g, int nrep, const int 
         atomicAdd(reinterpret_cast<unsigned long long *>(O.stepc + r0 + threadIdx.x), 1ull);
 }
 
-// The learned proposal's scoring and folding workgroups ride along in a transform launch:
-// the last LP.ntile_blocks + LP.nfold_blocks workgroups of grid row (y, z) = (0, 0) run
-// them (independent of the transform: they read the committed configuration and write
+// The learned proposal's scoring workgroups ride along in a transform launch: the last
+// LP.ntile_blocks workgroups of grid row (y, z) = (0, 0) run them (independent of the transform: they read the committed configuration and write
 // buffers the transform does not touch); those of the other rows exit.
 __device__ __forceinline__ bool lp_riders(const chm_lp_dev::LPArgs &LP) {
-    const int nlp = LP.ntile_blocks + LP.nfold_blocks;
+    const int nlp = LP.ntile_blocks;
     const int x0 = (int)gridDim.x - nlp;
     if (nlp == 0 || (int)blockIdx.x < x0) return false;
     if (blockIdx.y == 0 && blockIdx.z == 0) {
@@ -663,7 +662,12 @@ struct SlotArgs {
     const int8_t *lp_st;    // (nrep, ncand)
     const int *lp_dir_rd;   // lifted: direction before this step (null: not lifted)
     int *lp_dir_wr;         //   after it (reversed when a learned step is not accepted)
-    int *lp_rec;            // this step's swap record for the per-site sums
+    int *lp_cache;          // (nrep, 2, natoms, HID) per-site layer-1 sums: an accepted swap is
+                            //   folded in here (the slot's workgroups share the rows)
+    const int *lp_w1q;      //   with the scorer's layer-1 weight rows,
+    const int *lp_nbr;      //   the stencil-ordered neighbour table
+    const signed char *lp_posmap;   //   and the offset -> stencil position cube (side 2 pm_r + 1)
+    int lp_pm_r;
     int *lp_cs;             // this step's outcome for the next scoring workgroups
     uint64_t *lp_lpn;       // the step the next transform's scoring workgroups serve
     int lp_ncand, lp_lift;
@@ -710,7 +714,10 @@ __device__ __forceinline__ double chi2_of_sums(const Geo &g, const double (*S)[4
 // is requested before the first barrier (one memory round trip); the decision is taken
 // by every workgroup, and each commits its own share of the counts, running chi(R) and
 // best configuration.
-__device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, int s, int split, int nsplit) {
+// split < nsplit: a workgroup of the step's commit and gather shares; nsplit <= split <
+// nsplit + nfold: a folding workgroup of the learned proposal (decides, folds, exits)
+__device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, int s, int split, int nsplit,
+                                               int nfold = 0) {
     extern __shared__ int lds[];
     int *hist = lds;
     uint8_t *lsp = reinterpret_cast<uint8_t *>(lds + ((g.nrow_tot + 3) & ~3));
@@ -773,6 +780,11 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     }
     __shared__ int lp_dnew, lp_jo[2];
     __shared__ double lp_lqv;
+    // ... and, to fold an accepted swap into the per-site sums, the offset cube
+    __shared__ __align__(16) signed char lp_pm[chm_lp_dev::kPosMapMax];
+    const bool lp_fold = A.lp_cache && A.accept == 1;
+    const int lp_npm16 = lp_fold ? ((2 * A.lp_pm_r + 1) * (2 * A.lp_pm_r + 1) * (2 * A.lp_pm_r + 1) + 15) / 16 : 0;
+    const uint4 lp_pmv = (t < lp_npm16) ? reinterpret_cast<const uint4 *>(A.lp_posmap)[t] : make_uint4(0, 0, 0, 0);
     double q0v[2 * chm::kMaxNsp];            // per edge T0, Q0 (padded with zeros)
 #pragma unroll
     for (int q = 0; q < 2 * chm::kMaxNsp; ++q) q0v[q] = A.q0[q];
@@ -870,6 +882,7 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
         const double v = chm_lp_dev::lq_merge(lqm, lqz, lqd);
         if (lane == 0) lp_lqv = v;
     }
+    if (t < lp_npm16) reinterpret_cast<uint4 *>(lp_pm)[t] = lp_pmv;
     __syncthreads();
     // the learned draws of the next step (waves 2 and 3, beside the decision)
     if (drawer) {
@@ -939,14 +952,6 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
                     }
                     if (A.lp_lq) {
                         if (A.lp_dir_wr) A.lp_dir_wr[s] = lp_dnew;
-                        // this step's swap, for the per-site sums (folded in by the next
-                        // transforms' folding workgroups)
-                        int *rc = A.lp_rec + chm_lp_dev::kRecInts * s;
-                        rc[0] = acc && pp[4];
-                        rc[1] = pp[0];
-                        rc[2] = pp[1];
-                        rc[3] = pp[3];   // old species of a (= new species of b)
-                        rc[4] = pp[2];
                         A.lp_cs[s] = acc;
                         A.lp_lpn[s] = stepc + 1;
                     }
@@ -977,9 +982,25 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
             }
         }
         __syncthreads();
-        // ------------------------------------------------ commit of this workgroup's share
         const int acc = flag & 1;
         const bool add = acc && A.accept == 1;
+        if (split >= nwg) {
+            // a folding workgroup (learned proposal): an accepted swap folded into the slot's
+            // per-site layer-1 sums (a takes pp[2], b takes pp[3]: the species they
+            // exchange), the items shared by the slot's folding workgroups
+            if (lp_fold && add && pp[4]) {
+                int *cache = A.lp_cache + (long)s * 2 * g.natoms * chm_lp_dev::HID;
+                for (int it = (split - nwg) * nt + t; it < chm_lp_dev::fold_nitems(g); it += nfold * nt) {
+                    chm_lp_dev::FoldItem F;
+                    chm_lp_dev::fold_item_load(g, A.lp_pm_r, lp_pm, A.lp_w1q, cache, pp[0], pp[1], pp[2], pp[3],
+                                               pp[3], pp[2], it,
+                                               chm_lp_dev::fold_item_site(g, A.lp_nbr, pp[0], pp[1], it), F);
+                    chm_lp_dev::fold_item_store(F);
+                }
+            }
+            return;
+        }
+        // ------------------------------------------------ commit of this workgroup's share
         // words rbeg + t + r nt: prefetched when inside this workgroup's chunk (rbeg == pbeg
         // whenever rend > 0: series_on commits all rows in the lead, whose chunk starts at 0)
         int2 *cn2 = reinterpret_cast<int2 *>(cn);
@@ -1094,10 +1115,10 @@ __device__ __forceinline__ bool xcd_slot(int nrep, int nsplit, int &s, int &spli
     return s < nrep;
 }
 
-__global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A, int nsplit) {
+__global__ __launch_bounds__(256) void k_slot_step(Geo g, SlotArgs A, int nsplit, int nfold) {
     int s, split;
-    if (!xcd_slot(A.nrep, nsplit, s, split)) return;
-    slot_step_body(g, A, s, split, nsplit);
+    if (!xcd_slot(A.nrep, nsplit + nfold, s, split)) return;
+    slot_step_body(g, A, s, split, nsplit, nfold);
 }
 
 // ---------------------------------------------------------------- paired steps
@@ -1689,7 +1710,7 @@ size_t lds_gemm(const Geo &g) {
 // grid x and dynamic LDS of a transform launch with the learned proposal's workgroups (lp)
 void lp_ride(chm_sampler *S, const chm_lp_dev::LPArgs *lp, dim3 &grid, size_t &lds) {
     if (!lp) return;
-    grid.x += (unsigned)(lp->ntile_blocks + lp->nfold_blocks);
+    grid.x += (unsigned)lp->ntile_blocks;
     lds = std::max(lds, lp_tile_lds(S));
 }
 
@@ -1748,7 +1769,11 @@ void lp_slot_args(chm_sampler *S, SlotArgs &A, int accept, long long n) {
         A.lp_dir_rd = L.dir.as<int>() + (size_t)((accept == 1 ? n : nx) & 1) * S->nrep;
         A.lp_dir_wr = L.dir.as<int>() + (size_t)(nx & 1) * S->nrep;
     }
-    A.lp_rec = L.rec.as<int>() + (size_t)(n & 1) * chm_lp_dev::kRecInts * S->nrep;
+    A.lp_cache = L.cache.as<int>();
+    A.lp_w1q = L.w1q.as<int>();
+    A.lp_nbr = S->nbr_st.as<int>();
+    A.lp_posmap = L.posmap.as<signed char>();
+    A.lp_pm_r = L.pm_r;
     A.lp_cs = L.cs.as<int>();
     A.lp_lpn = L.lpn.as<uint64_t>();
 }
@@ -1766,8 +1791,10 @@ int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanch
     A.c2_prev = c2_prev ? c2_prev : c2row(S, n - 1);
     A.draws = (accept == 1 && S->incremental == 1) ? S->draws.as<int>() : nullptr;   // see launch_gemm_delta
     A.c2_next = c2row(S, n);
-    k_slot_step<<<dim3(S->nsplit * ((S->nrep + 7) & ~7)), 256, S->lds_k1, S->ctx->stream>>>(S->g, A,
-                                                                                             S->nsplit);
+    // (a decided learned step: the slot's folding workgroups follow its nsplit ones)
+    const int nfold = (A.lp_cache && accept == 1) ? chm_lp_dev::kFoldPerSlot : 0;
+    k_slot_step<<<dim3((S->nsplit + nfold) * ((S->nrep + 7) & ~7)), 256, S->lds_k1, S->ctx->stream>>>(
+        S->g, A, S->nsplit, nfold);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
@@ -1820,7 +1847,7 @@ int launch_gemm(chm_sampler *S, const int *dc, int bump, const chm_lp_dev::LPArg
 }
 
 // the transform of step n; with the learned proposal, the same launch carries the scoring
-// workgroups (step n's reverse probability, step n+1's candidates) and the folding ones
+// workgroups (step n's reverse probability, step n+1's candidates)
 int launch_trial(chm_sampler *S, long long n, bool more) {
     if (!S->lp.mode) return launch_gemm(S, drow(S, n), 1);
     const chm_lp_dev::LPArgs lp = lp_step_args(S, more ? 1 : 2, n);
@@ -2546,19 +2573,11 @@ extern "C" int chm_sampler_read(chm_sampler *S, int what, void *host, int64_t nb
             return CHM_OK;
         }
         case 15: {
-            // the copy the next scoring launch reads (it lacks the last step's swap, 16)
+            // the per-site layer-1 sums (of the committed configuration)
             CHM_REQUIRE(S->lp.cache_ok, CHM_E_STATE, "the learned proposal's per-site sums are not built");
             const int64_t n = (int64_t)4 * nrep * 2 * g.natoms * 256;
             if ((rc = need(n))) return rc;
-            const char *c = static_cast<const char *>(S->lp.cache.p) + (size_t)((S->hstep + 1) & 1) * n;
-            CHM_HIP(hipMemcpy(host, c, n, hipMemcpyDeviceToHost));
-            return CHM_OK;
-        }
-        case 16: {
-            const int64_t n = (int64_t)4 * nrep * chm_lp_dev::kRecInts;
-            if ((rc = need(n))) return rc;
-            const char *r = static_cast<const char *>(S->lp.rec.p) + (size_t)(S->hstep & 1) * n;
-            CHM_HIP(hipMemcpy(host, r, n, hipMemcpyDeviceToHost));
+            CHM_HIP(hipMemcpy(host, S->lp.cache.p, n, hipMemcpyDeviceToHost));
             return CHM_OK;
         }
     }

@@ -21,7 +21,6 @@ struct chm_lp {
     int nparam = 0;          // flat FP32 parameters
     int r2max = 0;           // largest squared stencil offset, units (a0/2)^2
     int pm_r = 0;            // half-side of the offset -> stencil position cube
-    signed char opp[128] = {};   // stencil position of the opposite offset of each position
     signed char fs[16] = {};     // first-shell stencil positions (n1 of them)
     int n1 = 0;
     bool have_params = false;
@@ -29,9 +28,8 @@ struct chm_lp {
     chm::DevBuf w1, w2t, w2, w3q;                // bf16 copies (W2 in both orders); w3 rounded
     chm::DevBuf w1q;                             // scorer's layer 1: int32 in units of 2^-16
     chm::DevBuf posmap;                          // int8 stencil position of an offset, or -1
-    // two copies of the per-site layer-1 sums (2 x nrep x 2 x natoms x 256 int32, valid when
-    // cache_ok) and the swap record of each step (2 x nrep x 8 ints, by step parity)
-    chm::DevBuf cache, rec;
+    // the per-site layer-1 sums (nrep x 2 x natoms x 256 int32, valid when cache_ok)
+    chm::DevBuf cache;
     bool cache_ok = false;
     // speculative scoring: g / dU of the pending step's candidates (2 parities x nrep x 2 x
     // 256; [.][.][1] with the previous swap made) and their touched flags (2 x nrep x 256),

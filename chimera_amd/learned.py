@@ -164,21 +164,18 @@ class LearnedProposal:
 
     def profile(self, iters=100):
         """Device ms per launch (HIP events) of one step's scoring workgroups (every slot
-        learned) and of its folding workgroups, each launched alone (in a run they ride in
-        the transform's launch)."""
+        learned), launched alone (in a run they ride in the transform's launch).  An accepted
+        swap is folded into the per-site sums inside the per-replica kernel, so there is no
+        separate folding time."""
         ms = np.zeros(2)
         check(_lib.load().chm_lp_profile(self.S._S, int(iters), ms.ctypes.data))
-        return {"score_ms": float(ms[0]), "fold_ms": float(ms[1])}
+        return {"score_ms": float(ms[0])}
 
     def site_sums(self):
-        """(sums, lag): the copy of the per-site layer-1 sums the next step reads
-        (nrep, 2, natoms, 256) int32 in units of 2^-16, slot order (role 0: the site as
-        candidate a, 1: as b), and the swap it lacks per slot (nrep, 8): valid, a, b, old
-        species of a, old species of b (the last step's, if it was accepted)."""
+        """The per-site layer-1 sums of every slot's committed configuration (nrep, 2, natoms,
+        256) int32 in units of 2^-16, slot order (role 0: the site as candidate a, 1: as b)."""
         S = self.S
-        sums = S._read(15, (S.nrep, 2, S.natoms, 256), np.int32)
-        lag = S._read(16, (S.nrep, 8), np.int32)
-        return sums, lag
+        return S._read(15, (S.nrep, 2, S.natoms, 256), np.int32)
 
     def _reapply(self, sampler):
         """A new device sampler (new composition): restore parameters, records, mode."""
