@@ -175,6 +175,35 @@ __device__ __forceinline__ float row16_sum(float v) {
     return v;
 }
 
+// DPP moves (every lane of the row pattern active): neighbour swap 0xB1, pair swap 0x4E,
+// half-row mirror 0x141, row mirror 0x140
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int2 w = __builtin_bit_cast(int2, v);
+    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_update_dpp(0, w.x, kCtrl, 0xF, 0xF, false),
+                                                __builtin_amdgcn_update_dpp(0, w.y, kCtrl, 0xF, 0xF, false)));
+}
+// max / sum over lanes 0..31 (all active), in each of them: the rows of 16 by DPP, then
+// the two rows by one exchange
+__device__ __forceinline__ float max32(float v) {
+    v = fmaxf(v, dpp_f<0xB1>(v));
+    v = fmaxf(v, dpp_f<0x4E>(v));
+    v = fmaxf(v, dpp_f<0x141>(v));
+    v = fmaxf(v, dpp_f<0x140>(v));
+    return fmaxf(v, __shfl_xor(v, 16, 32));
+}
+__device__ __forceinline__ double sum32(double v) {
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    return v + __shfl_xor(v, 16, 32);
+}
+
 // One K half of a wave's W2 fragments (hidden units [64 w, 64 w + 64), four 16-column
 // tiles), with b2 and w3 of those units
 struct W2Half {
@@ -454,13 +483,8 @@ __device__ inline void score_rows(const LPArgs &A, const uint8_t *lsp, const sig
             if (r >= MP) continue;
             uint2 pk = make_uint2(0u, 0u);
             if (r < nrows) {
-                unsigned short h[4];
-                const int zz[4] = {zv[k].x, zv[k].y, zv[k].z, zv[k].w};
-                const float bb[4] = {b1.x, b1.y, b1.z, b1.w};
-#pragma unroll
-                for (int i = 0; i < 4; ++i) h[i] = bf16_rne(silu_fast((float)zz[i] * kW1Inv + bb[i]));
-                pk.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
-                pk.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
+                pk.x = pk_bf16(silu_fast((float)zv[k].x * kW1Inv + b1.x), silu_fast((float)zv[k].y * kW1Inv + b1.y));
+                pk.y = pk_bf16(silu_fast((float)zv[k].z * kW1Inv + b1.z), silu_fast((float)zv[k].w * kW1Inv + b1.w));
             }
             *reinterpret_cast<uint2 *>(H + r * LDH + 4 * lane) = pk;
         }
@@ -531,21 +555,15 @@ __device__ inline int wave_draw4(const float (&lg)[4], float m, double u) {
     return -1;
 }
 
-// partial softmax sums of a tile's lanes (lane < MR): max, sum e^(l - max), sum over `same`
+// partial softmax sums of a tile's lanes (lanes 0..MR-1 = 0..31, all active): max, sum
+// e^(l - max), sum over `same`
 __device__ __forceinline__ void tile_partial(float l, bool same, double &m, double &Z, double &D) {
-    float mf = l;
-#pragma unroll
-    for (int k = 16; k >= 1; k >>= 1) mf = fmaxf(mf, __shfl_xor(mf, k, 32));
+    static_assert(MR == 32, "tile reductions span 32 lanes");
+    const float mf = max32(l);
     const double p = (mf > -__builtin_huge_valf() && l > -__builtin_huge_valf()) ? exp((double)(l - mf)) : 0.0;
-    double z = p, d = same ? p : 0.0;
-#pragma unroll
-    for (int k = 16; k >= 1; k >>= 1) {
-        z += __shfl_xor(z, k, 32);
-        d += __shfl_xor(d, k, 32);
-    }
     m = (double)mf;
-    Z = z;
-    D = d;
+    Z = sum32(p);
+    D = sum32(same ? p : 0.0);
 }
 
 // dynamic LDS of a scoring workgroup: the slot's species, H1, the offset -> position cube
@@ -713,12 +731,7 @@ __device__ inline void lp_tile(const LPArgs &A, int s, int tile, char *lds_raw) 
         const float lR = logit_of(A.lam, A.kappa, A.lift, gAv[t], duAv[t], beta, dR);
         const bool tb = need_ii && tBv[t];
         const float lA = logit_of(A.lam, A.kappa, A.lift, tb ? gBv[t] : gAv[t], tb ? duBv[t] : duAv[t], beta, d0);
-        float mA = lA, mR = lR;
-#pragma unroll
-        for (int k = 16; k >= 1; k >>= 1) {
-            mA = fmaxf(mA, __shfl_xor(mA, k, 32));
-            mR = fmaxf(mR, __shfl_xor(mR, k, 32));
-        }
+        const float mA = max32(lA), mR = max32(lR);
         if (t == 0) {
             A.dmax[((long)s * MAXT + tile) * 2] = mA;
             A.dmax[((long)s * MAXT + tile) * 2 + 1] = mR;

@@ -60,6 +60,13 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (unsigned short)(u >> 16);
 }
+// two floats rounded to bf16 (nearest even, as bf16_rne for finite values) in one packed
+// conversion instruction: a in the low half, b in the high half
+__device__ __forceinline__ unsigned pk_bf16(float a, float b) {
+    typedef float f2_t __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f2_t){a, b}, b2_t));
+}
 __device__ __forceinline__ float bf16_to_float(unsigned short h) {
     return __builtin_bit_cast(float, (unsigned int)h << 16);
 }
