@@ -1791,10 +1791,13 @@ int launch_slot(chm_sampler *S, int accept, int propose, long long n, int reanch
     A.c2_prev = c2_prev ? c2_prev : c2row(S, n - 1);
     A.draws = (accept == 1 && S->incremental == 1) ? S->draws.as<int>() : nullptr;   // see launch_gemm_delta
     A.c2_next = c2row(S, n);
-    // (a decided learned step: the slot's folding workgroups follow its nsplit ones)
+    // (a decided learned step: the slot's folding workgroups follow its nsplit ones; the
+    // learned proposal's launches take their own split, the uniform sampler's tuned one
+    // being too many workgroups beside the folding ones)
     const int nfold = (A.lp_cache && accept == 1) ? chm_lp_dev::kFoldPerSlot : 0;
-    k_slot_step<<<dim3((S->nsplit + nfold) * ((S->nrep + 7) & ~7)), 256, S->lds_k1, S->ctx->stream>>>(
-        S->g, A, S->nsplit, nfold);
+    const int nsplit = S->lp.mode ? S->lp.nsplit : S->nsplit;
+    k_slot_step<<<dim3((nsplit + nfold) * ((S->nrep + 7) & ~7)), 256, S->lds_k1, S->ctx->stream>>>(
+        S->g, A, nsplit, nfold);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }

@@ -8,6 +8,7 @@
 struct chm_lp {
     int mode = 0;            // 0: uniform proposals only; 1: learned proposal mixed with uniform
     int ncand = 256;         // candidates per replica and step (multiple of 32, <= 256)
+    int nsplit = 8;          // commit / gather workgroups per slot of a learned step's launch
     double eps = 0.0;        // probability that a step uses the uniform proposal instead
     double lam = 1.0;        // logit = -lam * beta * g / 2 + kappa * |dU|
     double kappa = 0.0;
