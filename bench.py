@@ -396,6 +396,12 @@ def main():
                     help="skip the full chi(k) path-sum / adjoint timings")
     ap.add_argument("--forward-only", action="store_true",
                     help="only time the forward-model kernels (path sums, adjoint) and exit")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="only the learned-vs-uniform comparison (training, both ESS runs, the "
+                         "scorer profile), as one JSON line")
+    ap.add_argument("--learned-timing", type=int, default=0, metavar="STEPS",
+                    help="time STEPS steps of the learned sampler (the bench's learned settings, "
+                         "the network's initial weights, no recording) and exit")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-uniform proposal comparison")
     ap.add_argument("--no-precision", action="store_true",
@@ -418,7 +424,7 @@ def main():
     ap.add_argument("--lp-kappa", type=float, default=0.5)
     ap.add_argument("--lp-symmetric", action="store_true",
                     help="symmetric tilt kappa |dU| instead of the lifted kappa d dU")
-    ap.add_argument("--lp-beta-max", type=float, default=3.0)
+    ap.add_argument("--lp-beta-max", type=float, default=2.0)
     ap.add_argument("--nsplit", type=int, default=0,
                     help="workgroups per replica in the slot kernel (0: autotune)")
     ap.add_argument("--single-steps", action="store_true",
@@ -454,6 +460,39 @@ def main():
         print(json.dumps({"workload": args.workload,
                           "pair_engine_default_moves": pair_leg(P, args.pair_leg_steps, args.warmup,
                                                                 args.exchange_every)}))
+        return
+    if args.learned_timing:
+        from chimera_amd import workloads as W
+        from chimera_amd.learned import LearnedProposal
+        P = W.build(args.workload)
+        S = P.sampler
+        S.precision, S.reanchor_every = args.precision, args.reanchor_every
+        S.paired = not args.single_steps
+        S.init_state(P.cfg0, P.cell.ideal)
+        lp = LearnedProposal(S, seed=0)
+        lp.enable(ncand=args.lp_ncand, lam=1.0, kappa=args.lp_kappa, eps=0.05, beta_max=args.lp_beta_max,
+                  lift=not args.lp_symmetric)
+        S.run(args.warmup, exchange_every=args.exchange_every, log_every=10 ** 9)
+        P.fm.ctx.sync()
+        t0 = time.perf_counter()
+        S.run(args.learned_timing, exchange_every=args.exchange_every, log_every=10 ** 9)
+        P.fm.ctx.sync()
+        dt = time.perf_counter() - t0
+        print(json.dumps({"workload": args.workload, "learned_steps": args.learned_timing,
+                          "us_per_step": round(1e6 * dt / args.learned_timing, 3),
+                          "kernels_ms": {k: round(v, 5) for k, v in lp.profile(50).items()}}))
+        return
+    if args.learned_only:
+        from chimera_amd import workloads as W
+        P = W.build(args.workload)
+        S = P.sampler
+        S.precision, S.reanchor_every = args.precision, args.reanchor_every
+        S.paired = not args.single_steps
+        S.init_state(P.cfg0, P.cell.ideal)
+        if args.nsplit:
+            from chimera_amd import _lib
+            _lib.check(_lib.load().chm_sampler_set_tuning(S._S, args.nsplit))
+        print(json.dumps({"workload": args.workload, "learned_vs_uniform": learned_vs_uniform(P, S, args)}))
         return
     if args.forward_only:
         from chimera_amd import workloads as W
