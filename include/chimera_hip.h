@@ -338,10 +338,14 @@ int chm_sampler_profile(chm_sampler *s, int iters, double *ms);
  * by an MLP (one-hot species occupancy of both sites' stencil neighbours and of the
  * sites: nocc = 2 ncoord + 2 entries, nfeat = nsp * nocc features -> 256 -> 256 -> 1,
  * SiLU, bf16 weights, FP32 accumulation) that predicts the swap's energy change
- * nfit * dchi2; logit = -lam * beta * g / 2 + kappa * |dU| (dU: change of unlike
- * first-shell bonds); one candidate is drawn from the softmax and accepted with the
+ * nfit * dchi2; logit = -lam * beta * g / 2 + kappa * dU (dU: change of the weighted
+ * first-shell bond count); one candidate is drawn from the softmax and accepted with the
  * exact Hastings ratio (the reverse probability from the same candidate set scored at
- * the proposed state).  With probability eps a step uses the uniform proposal. */
+ * the proposed state).  With probability eps a step uses the uniform proposal.  The
+ * scorer's layer 1 is W1 rounded to bf16 and to a multiple of 2^-16 (|w| <= 2), summed in
+ * int32 from per-site rows that the sampler keeps per replica (read back with
+ * chm_sampler_read: 15 the sums, nrep x 2 x natoms x 256 int32 in slot order, 16 the swap
+ * they lack, nrep x 8 ints: valid, a, b, old species of a, of b). */
 
 /* Sizes: flat FP32 parameter count, occupancy entries, features, padded features kp.
  * Flat layout: W1 (kp, 256) feature-major | b1 (256) | W2 (256, 256) input-major |
@@ -381,10 +385,11 @@ int chm_lp_grad(chm_sampler *s, int n, const uint8_t *occ, const float *y, const
  * clipped to [-clip, clip]; loss of the last minibatch. */
 int chm_lp_train(chm_sampler *s, int iters, int batch, float lr, int max_rung, float clip,
                  uint64_t seed, float *loss);
-/* Average device time (ms, HIP events on the sampler's stream) of the candidate scorer
- * (ms[0]) and of the selection + Hastings kernel (ms[1]) over `iters` launches each on
- * the current state, with every slot on the learned path; the chain does not advance
- * (the selection writes into a scratch proposal row). */
+/* Average device time (ms, HIP events on the sampler's stream) of one step's scoring
+ * workgroups launched alone (ms[0]: every slot learned, the current state, the last
+ * proposal as the pending swap) and of the folding workgroups of one recorded swap per
+ * slot (ms[1]), over `iters` launches each.  The chain does not advance; the per-site
+ * sums are rebuilt before the next run. */
 int chm_lp_profile(chm_sampler *s, int iters, double *ms);
 
 /* ------------------------------------------------- pair-space sampler (all moves) */
