@@ -175,18 +175,6 @@ __device__ __forceinline__ float row16_sum(float v) {
     return v;
 }
 
-// DPP moves (every lane of the row pattern active): neighbour swap 0xB1, pair swap 0x4E,
-// half-row mirror 0x141, row mirror 0x140
-template <int kCtrl>
-__device__ __forceinline__ float dpp_f(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
-}
-template <int kCtrl>
-__device__ __forceinline__ double dpp_d(double v) {
-    const int2 w = __builtin_bit_cast(int2, v);
-    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_update_dpp(0, w.x, kCtrl, 0xF, 0xF, false),
-                                                __builtin_amdgcn_update_dpp(0, w.y, kCtrl, 0xF, 0xF, false)));
-}
 // max / sum over lanes 0..31 (all active), in each of them: the rows of 16 by DPP, then
 // the two rows by one exchange
 __device__ __forceinline__ float max32(float v) {

@@ -143,13 +143,10 @@ __device__ __forceinline__ void tile_epilogue(const Geo &g, const TileOut &O, in
         q1 = p1 * w;
         q2 = p2 * w;
     }
-#pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) {
-        p1 += __shfl_xor(p1, m, 16);
-        p2 += __shfl_xor(p2, m, 16);
-        q1 += __shfl_xor(q1, m, 16);
-        q2 += __shfl_xor(q2, m, 16);
-    }
+    p1 = group_sum_d<16>(p1);
+    p2 = group_sum_d<16>(p2);
+    q1 = group_sum_d<16>(q1);
+    q2 = group_sum_d<16>(q2);
     if (slot < nrep && (threadIdx.x & 15) == 0) {
         double *P = O.partials + (((long)slot * g.nsp + e) * g.nct + ct) * 4;
         P[0] = p1;
@@ -456,13 +453,10 @@ __device__ __forceinline__ void gemm_delta_bf16_tile(const Geo &g, int nrep, con
                 q1 = p1 * w_pre[h];
                 q2 = p2 * w_pre[h];
             }
-#pragma unroll
-            for (int mm = 8; mm >= 1; mm >>= 1) {
-                p1 += __shfl_xor(p1, mm, 16);
-                p2 += __shfl_xor(p2, mm, 16);
-                q1 += __shfl_xor(q1, mm, 16);
-                q2 += __shfl_xor(q2, mm, 16);
-            }
+            p1 = group_sum_d<16>(p1);
+            p2 = group_sum_d<16>(p2);
+            q1 = group_sum_d<16>(q1);
+            q2 = group_sum_d<16>(q2);
             if (slot < nrep && cl == 0) {
                 double *P = Oa.partials + (((long)slot * g.nsp + e) * g.nct + ct) * 4;
                 P[0] = p1;
@@ -793,7 +787,7 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     const uint4 *gsp = reinterpret_cast<const uint4 *>(A.species + base_sp);
     const int4 spv = buf_i128(make_buf(gsp, 16u * nv), t < nv, 16u * t);
     // per-edge sums over the column tiles: (e, j) groups of CT lanes (CT-aligned inside a
-    // wave), each lane summing tiles c0, c0 + CT, ... in order, then a butterfly over the
+    // wave), each lane summing tiles c0, c0 + CT, ... in order, then DPP sums over the
     // group's lanes
     const int G = 4 * g.nsp;
     const int CT = (G <= 16) ? 16 : 8;
@@ -873,7 +867,7 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
         for (int r = 0; r < PR; ++r) psum += pv[r];
         for (int c = c0 + PR * CT; sum_lane && c < g.nct; c += CT)
             psum += Pp[(long)((gi >> 2) * g.nct + c) * 4 + (gi & 3)];
-        for (int m = CT / 2; m >= 1; m >>= 1) psum += __shfl_xor(psum, m, CT);
+        psum = (CT == 16) ? group_sum_d<16>(psum) : group_sum_d<8>(psum);
         if (t < G * CT && c0 == 0) sums[gi >> 2][gi & 3] = psum;
     }
     if (A.propose)
@@ -1309,7 +1303,7 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
             const double *Pp = A.partials[v] + (long)s * g.nsp * g.nct * 4;
             for (int c = c0 + PR * CT; sum_lane && c < g.nct; c += CT)
                 psum += Pp[(long)((gi >> 2) * g.nct + c) * 4 + (gi & 3)];
-            for (int m = CT / 2; m >= 1; m >>= 1) psum += __shfl_xor(psum, m, CT);
+            psum = (CT == 16) ? group_sum_d<16>(psum) : group_sum_d<8>(psum);
             if (t < G * CT && c0 == 0) sums[v][gi >> 2][gi & 3] = psum;
         }
     }

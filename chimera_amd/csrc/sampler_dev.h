@@ -71,6 +71,30 @@ __device__ __forceinline__ float bf16_to_float(unsigned short h) {
     return __builtin_bit_cast(float, (unsigned int)h << 16);
 }
 
+// DPP moves (every lane of the row pattern active): neighbour swap 0xB1, pair swap 0x4E,
+// half-row mirror 0x141, row mirror 0x140
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int2 w = __builtin_bit_cast(int2, v);
+    return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_update_dpp(0, w.x, kCtrl, 0xF, 0xF, false),
+                                                __builtin_amdgcn_update_dpp(0, w.y, kCtrl, 0xF, 0xF, false)));
+}
+// sum over aligned groups of W lanes (W = 8 or 16: within the DPP rows of 16), in every lane
+// of the group: neighbours, pairs, quads (half-row mirror), halves (row mirror)
+template <int W>
+__device__ __forceinline__ double group_sum_d(double v) {
+    static_assert(W == 8 || W == 16, "groups of 8 or 16 lanes");
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    if (W == 16) v += dpp_d<0x140>(v);
+    return v;
+}
+
 // Bounds-checked loads through a buffer resource (raw buffer, stride 0): a lane whose
 // byte offset is at or beyond the resource's size gets zero and makes no memory access,
 // without a branch, so a block of such loads is issued back to back and waited on once.
