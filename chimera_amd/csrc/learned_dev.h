@@ -41,7 +41,7 @@ namespace chm_lp_dev {
 using namespace chm_smp;
 
 constexpr int HID = 256;      // hidden width
-constexpr int MR = 32;        // candidates per tile and rows per network pass
+constexpr int MR = 32;        // candidates per tile
 constexpr int MAXC = 256;     // candidates per replica and step, at most
 constexpr int MAXT = MAXC / MR;
 constexpr int LDH = HID + 8;  // H1 row stride in LDS (bf16), staggers the banks

@@ -657,7 +657,7 @@ struct SlotArgs {
     const int *lp_dir_rd;   // lifted: direction before this step (null: not lifted)
     int *lp_dir_wr;         //   after it (reversed when a learned step is not accepted)
     int *lp_cache;          // (nrep, 2, natoms, HID) per-site layer-1 sums: an accepted swap is
-                            //   folded in here (the slot's workgroups share the rows)
+                            //   folded in here by the slot's folding workgroups
     const int *lp_w1q;      //   with the scorer's layer-1 weight rows,
     const int *lp_nbr;      //   the stencil-ordered neighbour table
     const signed char *lp_posmap;   //   and the offset -> stencil position cube (side 2 pm_r + 1)
