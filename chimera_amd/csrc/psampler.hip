@@ -624,7 +624,7 @@ __global__ __launch_bounds__(256) void k_ps_transform(TfArgs A) {
 }
 
 // chi^2 of slot s from its residual sums (nsp, nbc, 4), by one wavefront (every lane gets
-// it): the blocks are added in block order per lane and the lanes in butterfly order.
+// it): the blocks are added in block order per lane, then the lanes (rows of 16, then rows).
 __device__ double c2_from_sums(const double *sums, int nsp, int nbc, const double *q0, int profile,
                                double nfit) {
     const int lane = threadIdx.x & 63;
@@ -646,16 +646,24 @@ __device__ double c2_from_sums(const double *sums, int nsp, int nbc, const doubl
     for (int e = 0; e < chm::kMaxNsp; ++e)
 #pragma unroll
         for (int v = 0; v < 4; ++v) U[e][v] = P[e][0][v] + P[e][1][v];
-    for (int e = 0; e < nsp; ++e)
+#pragma unroll
+    for (int e = 0; e < chm::kMaxNsp; ++e) {
+        if (e >= nsp) break;
         for (int b = lane + 128; b < nbc; b += 64)
+#pragma unroll
             for (int v = 0; v < 4; ++v) U[e][v] += sums[((size_t)e * nbc + b) * 4 + v];
-    // one butterfly for all the sums: each step's exchanges are independent
+    }
+    // the sums over the wavefront: each row of 16 lanes by DPP, then the four rows
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1)
+    for (int e = 0; e < chm::kMaxNsp; ++e) {
+        if (e >= nsp) break;
 #pragma unroll
-        for (int e = 0; e < chm::kMaxNsp; ++e)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) U[e][v] += __shfl_xor(U[e][v], m);
+        for (int v = 0; v < 4; ++v) {
+            double x = chm_smp::group_sum_d<16>(U[e][v]);
+            x += __shfl_xor(x, 16);
+            U[e][v] = x + __shfl_xor(x, 32);
+        }
+    }
     double tot = 0.0;
 #pragma unroll
     for (int e = 0; e < chm::kMaxNsp; ++e) {
@@ -702,11 +710,19 @@ struct PsAcceptArgs {
 };
 
 constexpr int kAccThreads = 256;
+constexpr int kAccParts = 8;   // workgroups per replica: each takes the same decision and a
+                               // share of the commit and best-state copies
 
+// Workgroup (part, slot) of the accept launch, grid kAccParts x nrep.  Every workgroup
+// decides (the same arithmetic on the same inputs); part 0 makes the scalar updates (log,
+// chi^2, counters, the moved species / position, step counter); the copies (trial chi into
+// the running chi, the affected pairs' distances and types, and on a new best the species
+// and positions into the best state, with the move applied) are split between the parts.
 __global__ __launch_bounds__(kAccThreads) void k_ps_accept(PsAcceptArgs A) {
     __shared__ double red[1];
     __shared__ int accs, imps;
-    const int s = blockIdx.x;
+    const int s = blockIdx.y, part = blockIdx.x, nparts = gridDim.x;
+    const bool lead = part == 0;
     const int rung = A.rung_of[s];
     const uint64_t n = A.stepc[s] + 1;
     const PsMove mv = A.mv[s];
@@ -725,7 +741,7 @@ __global__ __launch_bounds__(kAccThreads) void k_ps_accept(PsAcceptArgs A) {
         accs = acc ? 1 : 0;
         const double cur = acc ? c2t : c2c;
         imps = (cur < bestc) ? 1 : 0;
-        if (A.log_cap > 0) {
+        if (lead && A.log_cap > 0) {
             const long long ri = (long long)n - 1 - A.rec_base;
             if (ri >= 0) {
                 const size_t at = (size_t)(ri % A.log_cap) * A.nrep + rung;
@@ -735,15 +751,17 @@ __global__ __launch_bounds__(kAccThreads) void k_ps_accept(PsAcceptArgs A) {
         }
     }
     __syncthreads();
-    if (accs) {
-        for (int q = threadIdx.x; q < mv.naff; q += blockDim.x) {
+    const int acc = accs;
+    const int tid = part * (int)blockDim.x + (int)threadIdx.x, nthr = nparts * (int)blockDim.x;
+    if (acc) {
+        for (int q = tid; q < mv.naff; q += nthr) {
             const AffRec ar = A.aff[(size_t)s * A.maxaff + q];
             A.r[(size_t)s * A.npairs + ar.p] = ar.rnew;
             A.pt[(size_t)s * A.npairs + ar.p] = ar.tnew;
         }
         const size_t ck = (size_t)A.nsp * A.nk;
-        for (size_t q = threadIdx.x; q < ck; q += blockDim.x) A.chi[s * ck + q] = A.trial[s * ck + q];
-        if (threadIdx.x == 0) {
+        for (size_t q = tid; q < ck; q += nthr) A.chi[s * ck + q] = A.trial[s * ck + q];
+        if (lead && threadIdx.x == 0) {
             uint8_t *sp = A.sp + (size_t)s * A.natoms_pad;
             sp[mv.a] = (uint8_t)mv.na;
             sp[mv.b] = (uint8_t)mv.nb;
@@ -758,38 +776,49 @@ __global__ __launch_bounds__(kAccThreads) void k_ps_accept(PsAcceptArgs A) {
         }
     }
     if (imps) {
-        __syncthreads();   // the committed species / position are visible to every lane
-        // species as 16-byte words, positions as doubles: each lane's loads issued together
+        // the best state after this step: the species and positions as stored (the lead may
+        // be writing the move into them now), with this step's move applied when accepted
         {
             const uint4 *src = reinterpret_cast<const uint4 *>(A.sp + (size_t)s * A.natoms_pad);
             uint4 *dst = reinterpret_cast<uint4 *>(A.best_sp + (size_t)s * A.natoms_pad);
-            for (int q = threadIdx.x; q < A.natoms_pad / 16; q += blockDim.x) dst[q] = src[q];
+            for (int q = tid; q < A.natoms_pad / 16; q += nthr) {
+                uint4 w = src[q];
+                if (acc) {
+                    unsigned char *b = reinterpret_cast<unsigned char *>(&w);
+                    if (mv.a >= 16 * q && mv.a < 16 * q + 16) b[mv.a - 16 * q] = (unsigned char)mv.na;
+                    if (mv.b >= 16 * q && mv.b < 16 * q + 16) b[mv.b - 16 * q] = (unsigned char)mv.nb;
+                }
+                dst[q] = w;
+            }
         }
         {
             const int np = 3 * A.natoms;
             const double *src = A.pos + (size_t)s * np;
             double *dst = A.best_pos + (size_t)s * np;
             const Buf bp = make_buf(src, (uint32_t)(sizeof(double) * np));
-            constexpr int U = 16;
-            for (int q0 = 0; q0 < np; q0 += U * (int)blockDim.x) {
+            const bool moved = acc && !mv.sw;
+            constexpr int U = 8;
+            for (int q0 = 0; q0 < np; q0 += U * nthr) {
                 double v[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int q = q0 + u * (int)blockDim.x + (int)threadIdx.x;
+                    const int q = q0 + u * nthr + tid;
                     v[u] = buf_f64(bp, q < np, 8u * (uint32_t)q);
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int q = q0 + u * (int)blockDim.x + (int)threadIdx.x;
-                    if (q < np) dst[q] = v[u];
+                    const int q = q0 + u * nthr + tid;
+                    const int k = q - 3 * mv.a;   // (a selection: no indexed register array)
+                    const double pk = (k == 0) ? mv.np[0] : (k == 1) ? mv.np[1] : mv.np[2];
+                    if (q < np) dst[q] = (moved && k >= 0 && k < 3) ? pk : v[u];
                 }
             }
         }
-        if (threadIdx.x == 0) A.best_c2[s] = accs ? c2t : c2c;
+        if (lead && threadIdx.x == 0) A.best_c2[s] = acc ? c2t : c2c;
     }
     if (A.dcount)
-        for (int q = threadIdx.x; q < A.nrow_tot; q += blockDim.x) A.dcount[(size_t)s * A.nrow_tot + q] = 0;
-    if (threadIdx.x == 0) A.stepc[s] = n;
+        for (int q = tid; q < A.nrow_tot; q += nthr) A.dcount[(size_t)s * A.nrow_tot + q] = 0;
+    if (lead && threadIdx.x == 0) A.stepc[s] = n;
 }
 
 // chi^2 of every row of `sums` (for init and chm_psampler_chi2), one wavefront per row
@@ -1082,7 +1111,7 @@ int launch_step(chm_psampler *S, hipStream_t st) {
     }
     if (rc) return rc;
     if ((rc = launch_transform(S, st, S->nrep, S->trialT.as<double>(), S->sums.as<double>()))) return rc;
-    k_ps_accept<<<S->nrep, kAccThreads, 0, st>>>(accept_args(S));
+    k_ps_accept<<<dim3(kAccParts, S->nrep), kAccThreads, 0, st>>>(accept_args(S));
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
