@@ -418,8 +418,9 @@ def main():
     ap.add_argument("--batch-leg-steps", type=int, default=2000)
     ap.add_argument("--ess-steps", type=int, default=100000)
     ap.add_argument("--ess-burn", type=int, default=10000)
-    ap.add_argument("--train-steps", type=int, default=40000)
-    ap.add_argument("--train-iters", type=int, default=8000)
+    ap.add_argument("--train-steps", type=int, default=312500,
+                    help="uniform steps whose move records train the learned proposal (x replicas records)")
+    ap.add_argument("--train-iters", type=int, default=24000)
     ap.add_argument("--lp-ncand", type=int, default=256)
     ap.add_argument("--lp-kappa", type=float, default=0.5)
     ap.add_argument("--lp-symmetric", action="store_true",
