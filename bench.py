@@ -189,6 +189,41 @@ def learned_vs_uniform(P, S, args):
                                 "frac": round(flops / (prof["score_ms"] * 1e-3) / 1e12 / 2500.0, 6)}}
 
 
+def replica_sweep(args, counts, steps, warmup):
+    """The same workload and step at more replicas per GPU: at 32 replicas both kernels of a
+    step are dependent latency chains, so the step time grows much more slowly than the
+    work.  Each count is its own sampler (built, autotuned, warmed up, then timed)."""
+    from chimera_amd import _lib, autotune
+    from chimera_amd import workloads as W
+    out = []
+    for n in counts:
+        P = W.build(args.workload, nrep=n)
+        S = P.sampler
+        S.precision, S.reanchor_every = args.precision, args.reanchor_every
+        S.paired = not args.single_steps
+        S.init_state(P.cfg0, P.cell.ideal)
+        tuned = autotune.tune(S, use_cache=False)
+        S.run(warmup, exchange_every=args.exchange_every, log_every=10 ** 9)
+        P.fm.ctx.sync()
+        a0 = float(S.accepted_per_rung().sum())
+        t0 = time.perf_counter()
+        S.run(steps, exchange_every=args.exchange_every, log_every=10 ** 9)
+        P.fm.ctx.sync()
+        dt = time.perf_counter() - t0
+        acc = (float(S.accepted_per_rung().sum()) - a0) / (steps * n)
+        ms = np.zeros(4)
+        _lib.check(_lib.load().chm_sampler_profile(S._S, 20, ms))
+        out.append({"replicas": n, "steps": steps, "nsplit": tuned["nsplit"],
+                    "us_per_step": round(1e6 * dt / steps, 3),
+                    "replica_moves_per_s": round(n * steps / dt, 1),
+                    "transform_us_per_step": round(1e3 * ms[1], 3),
+                    "per_replica_us_per_step": round(1e3 * ms[2], 3),
+                    "accept": round(acc, 4)})
+        print(f"[bench] {n} replicas: {out[-1]['us_per_step']} us/step", file=sys.stderr, flush=True)
+        del S, P
+    return out
+
+
 def forward_bench(P, S, iters=10):
     """Full chi(k) evaluations of the replicas' configurations: the pair-space path sum
     (FP64 and FP32, with distances and pair types), its adjoint dL/dr, and the
@@ -413,6 +448,9 @@ def main():
     ap.add_argument("--no-pair-leg", action="store_true",
                     help="skip the pair-space engine leg (swap_frac 0.75 with displacements)")
     ap.add_argument("--pair-leg-steps", type=int, default=2000)
+    ap.add_argument("--no-replica-sweep", action="store_true",
+                    help="skip the same workload at 64, 128 and 256 replicas")
+    ap.add_argument("--replica-sweep-only", action="store_true")
     ap.add_argument("--pair-only", action="store_true",
                     help="only the pair-space engine leg (--pair-leg-steps timed after --warmup)")
     ap.add_argument("--batch-leg-steps", type=int, default=2000)
@@ -454,6 +492,10 @@ def main():
     if args.precision_study:
         from chimera_amd import precision_study
         precision_study.main(["--steps", str(args.precision_study)])
+        return
+    if args.replica_sweep_only:
+        print(json.dumps({"workload": args.workload,
+                          "replica_sweep": replica_sweep(args, (32, 64, 128, 256), 4000, 500)}))
         return
     if args.pair_only:
         from chimera_amd import workloads as W
@@ -621,6 +663,10 @@ def main():
     if not args.no_pair_leg:
         print("[bench] pair-space engine, the reference's default move mix", file=sys.stderr, flush=True)
         pl = pair_leg(P, args.pair_leg_steps, 200, args.exchange_every)
+    rs = None
+    if not args.no_replica_sweep:
+        print("[bench] replica-count sweep", file=sys.stderr, flush=True)
+        rs = replica_sweep(args, (64, 128, 256), 2000, 300)
     c2 = None
     if not args.no_batch_leg:
         print("[bench] batch of uneven problems (short)", file=sys.stderr, flush=True)
@@ -656,6 +702,7 @@ def main():
         "precision_study": c3,
         "batch_uneven": c2,
         "pair_engine_default_moves": pl,
+        "replica_sweep": rs,
         "autotune": {"nsplit": tuned["nsplit"],
                      "step_ms": {k: round(v, 5) for k, v in tuned["times_ms"].items()}},
     }
