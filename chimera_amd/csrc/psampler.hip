@@ -6,8 +6,8 @@
 // species, positions, the distance and type of every ordered pair, the unnormalised
 // chi(k), chi^2 and the best state with its positions.  One step, for all replicas:
 //
-//   P1 propose + affected pairs (one workgroup per slot): the splitmix64 proposal with the
-//      swap flag, up to 8 partners and the displacement fallback (Box-Muller step clamped
+//   P1 propose + affected pairs (workgroup 0 of a slot; the others gather MS paths): the
+//      splitmix64 proposal with the swap flag, up to 8 partners and the displacement fallback (Box-Muller step clamped
 //      to max_disp around the ideal site; exafs_gpu/kernels.py:362-423); the directed pairs
 //      touching the moved sites with dedup, their new distance and type (kernels.py:
 //      425-488), compacted in pair order; the energy-prior change over them
@@ -22,7 +22,8 @@
 //   P3 trial chi(R) (grid: 16-column blocks x edges x 32-slot groups): trial chi of 32
 //      slots over the window's nonzero k rows staged in LDS, times the window's DFT
 //      matrix pre-divided by the edge's absorber count (chimera/multift.py:58-63), FP64.
-//   P4 accept (one workgroup per slot): the model on the target's footing (complex or
+//   P4 accept (8 workgroups per slot, each taking the same decision and committing a
+//      share of the copies): the model on the target's footing (complex or
 //      |chi(R)|), profiled per-edge scale, chi^2 (method_a_rmc.py:180-227), Metropolis on
 //      beta dchi2 nfit + beta_E dE with the counter-based uniform (method_a_rmc.py:
 //      278-285), commit of pair distances / types, species, position, chi and chi^2
