@@ -7,8 +7,9 @@
 // chi(k), chi^2 and the best state with its positions.  One step, for all replicas:
 //
 //   P1 propose + affected pairs (workgroup 0 of a slot; the others gather MS paths): the
-//      splitmix64 proposal with the swap flag, up to 8 partners and the displacement fallback (Box-Muller step clamped
-//      to max_disp around the ideal site; exafs_gpu/kernels.py:362-423); the directed pairs
+//      splitmix64 proposal with the swap flag, up to 8 partners and the displacement
+//      fallback (Box-Muller step clamped to max_disp around the ideal site;
+//      exafs_gpu/kernels.py:362-423); the directed pairs
 //      touching the moved sites with dedup, their new distance and type (kernels.py:
 //      425-488), compacted in pair order; the energy-prior change over them
 //      (method_a_rmc.py:310-319); for a swap with multiple scattering, the entry-count
