@@ -255,6 +255,56 @@ __global__ __launch_bounds__(256) void k_gemm_abs_f64(Geo g, int nrep, const int
     gemm_abs_body(g, nrep, counts, dcounts, basisR, O, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// FP64 re-anchor of the running chi(R): anchor_out[s] = counts[s] @ basis_R for every slot
+// with anchor[s] >= anchor_min.  Grid (column tiles, slots, edges), 16 columns x 16 K
+// slices per workgroup; a workgroup of a slot that is not due exits after one scalar
+// load.  Each lane requests its K slice's counts and basis entries together (bounds-
+// checked buffer loads, no branch), then multiplies in row order; the slices are added in
+// slice order.  The same slices, order and operations as the FP64 transform, so the
+// values are those of gemm_abs_body's re-anchor mode bit for bit.
+__global__ __launch_bounds__(256) void k_reanchor_f64(Geo g, int nrep, const int *__restrict__ counts,
+                                                      const double *__restrict__ basisR,
+                                                      const int *__restrict__ anchor, int anchor_min,
+                                                      double *__restrict__ anchor_out, int acc32) {
+    __shared__ double P[KS * NCW];
+    const int slot = blockIdx.y, ct = blockIdx.x, e = blockIdx.z;
+    if (anchor[slot] < anchor_min) return;
+    const int K = g.nrow_e;
+    const int tn = threadIdx.x % NCW, ks = threadIdx.x / NCW;
+    const int c0 = ct * NCW, col = c0 + tn;
+    const int kc = (K + KS - 1) / KS;
+    const int kb = ks * kc, ke = min(K, kb + kc);
+    const bool live = col < g.ncol_e;
+    const Buf bc = make_buf(counts + (size_t)slot * g.nrow_tot + (size_t)e * g.rstride, (uint32_t)(sizeof(int) * K));
+    const Buf bb = make_buf(basisR + (size_t)e * K * g.ncol_e, (uint32_t)(sizeof(double) * (size_t)K * g.ncol_e));
+    constexpr int kBatch = 32;
+    double acc = 0.0;
+    for (int k0 = kb; k0 < ke; k0 += kBatch) {
+        int cv[kBatch];
+        double bv[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int k = k0 + j;
+            const bool ok = live && k < ke;
+            cv[j] = buf_i32(bc, ok, (uint32_t)(sizeof(int) * k));
+            bv[j] = buf_f64(bb, ok, (uint32_t)(sizeof(double) * ((size_t)k * g.ncol_e + col)));
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const double f = fma((double)cv[j], bv[j], acc);
+            acc = (k0 + j < ke) ? f : acc;
+        }
+    }
+    P[ks * NCW + tn] = acc;
+    __syncthreads();
+    if (threadIdx.x < NCW && c0 + (int)threadIdx.x < g.ncol_e) {
+        double sum = 0.0;
+        for (int z = 0; z < KS; ++z) sum += P[z * NCW + threadIdx.x];
+        anchor_out[(size_t)slot * g.ncol_tot + (size_t)e * g.ncol_e + c0 + threadIdx.x] =
+            acc32 ? (double)(float)sum : sum;
+    }
+}
+
 __device__ __forceinline__ __bf16 bf16_of_small_int(int v) {
     // |v| <= 128: exactly representable; the float's top 16 bits are the bf16
     float f = (float)v;
@@ -1814,13 +1864,9 @@ int launch_accept(chm_sampler *S, long long n, int propose, int reanchor_step = 
 
 int launch_reanchor(chm_sampler *S) {
     const Geo &g = S->g;
-    dim3 grid(g.nct, (S->nrep + RB - 1) / RB, g.nsp);
-    TileOut O = tile_out(S);
-    O.anchor = S->anchor.as<int>();
-    O.anchor_min = S->reanchor_every;
-    O.anchor_out = S->mcur.as<double>();
-    k_gemm_abs_f64<<<grid, 256, lds_gemm(g), S->ctx->stream>>>(
-        g, S->nrep, S->counts.as<int>(), nullptr, S->basisR.as<double>(), O, chm_lp_dev::LPArgs{});
+    k_reanchor_f64<<<dim3(g.nct, S->nrep, g.nsp), 256, 0, S->ctx->stream>>>(
+        g, S->nrep, S->counts.as<int>(), S->basisR.as<double>(), S->anchor.as<int>(), S->reanchor_every,
+        S->mcur.as<double>(), S->acc32);
     CHM_LAUNCH_CHECK();
     return CHM_OK;
 }
