@@ -54,6 +54,11 @@ struct PsMove {
     int sw, naff, nms, pad;
     double np[3];   // new position of a (its current position for a swap)
     double de;      // energy-prior change, eV
+    // the step's number and the slot's chi^2 and best chi^2 before it, as P1 read them: the
+    // accept workgroups decide from these, not from the counters and chi^2 that the lead
+    // accept workgroup overwrites during the same launch
+    uint64_t n;
+    double c2c, bestc;
 };
 
 struct PsArgs {
@@ -62,6 +67,7 @@ struct PsArgs {
     double swap_frac, disp_sigma, max_disp;
     const int *rung_of;
     const uint64_t *stepc;
+    const double *c2, *best_c2;   // the slot's chi^2 and best chi^2 (copied into the move)
     const uint8_t *sp;
     const double *pos, *ideal;
     const int *rev, *iidx, *jidx, *shell;
@@ -321,6 +327,9 @@ __global__ __launch_bounds__(kPropThreads) void k_ps_propose(PsArgs A) {
         o.nms = 0;
         o.pad = 0;
         o.de = de;
+        o.n = n;
+        o.c2c = A.c2[s];
+        o.bestc = A.best_c2[s];
         A.mv[s] = o;
     }
 }
@@ -716,7 +725,10 @@ constexpr int kAccParts = 8;   // workgroups per replica: each takes the same de
                                // share of the commit and best-state copies
 
 // Workgroup (part, slot) of the accept launch, grid kAccParts x nrep.  Every workgroup
-// decides (the same arithmetic on the same inputs); part 0 makes the scalar updates (log,
+// decides (the same arithmetic on the same inputs: the residual sums and the step number,
+// chi^2 and best chi^2 that P1 copied into the move, none of which this launch writes, so a
+// workgroup that starts after the lead has finished decides the same); part 0 makes the
+// scalar updates (log,
 // chi^2, counters, the moved species / position, step counter); the copies (trial chi into
 // the running chi, the affected pairs' distances and types, and on a new best the species
 // and positions into the best state, with the move applied) are split between the parts.
@@ -726,9 +738,9 @@ __global__ __launch_bounds__(kAccThreads) void k_ps_accept(PsAcceptArgs A) {
     const int s = blockIdx.y, part = blockIdx.x, nparts = gridDim.x;
     const bool lead = part == 0;
     const int rung = A.rung_of[s];
-    const uint64_t n = A.stepc[s] + 1;
     const PsMove mv = A.mv[s];
-    const double bslot = A.beta_slot[s], c2c = A.c2[s], bestc = A.best_c2[s];
+    const uint64_t n = mv.n;
+    const double bslot = A.beta_slot[s], c2c = mv.c2c, bestc = mv.bestc;
     if (threadIdx.x < 64) {
         const double v = c2_from_sums(A.sums + (size_t)s * A.nsp * A.nbc * 4, A.nsp, A.nbc, A.q0, A.profile, A.nfit);
         if (threadIdx.x == 0) red[0] = v;
@@ -961,6 +973,8 @@ PsArgs prop_args(chm_psampler *S) {
     A.max_disp = S->max_disp;
     A.rung_of = S->rung_of.as<int>();
     A.stepc = S->stepc.as<uint64_t>();
+    A.c2 = S->c2.as<double>();
+    A.best_c2 = S->best_c2.as<double>();
     A.sp = S->sp.as<uint8_t>();
     A.pos = S->pos.as<double>();
     A.ideal = S->ideal.as<double>();

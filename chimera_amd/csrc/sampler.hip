@@ -113,6 +113,8 @@ constexpr int KS = 16;    // K slices of the FP64 kernel
 struct TileOut {
     uint64_t *stepc;        // per slot: advanced by `bump` steps per transform
     int bump;
+    const double *best_c2;  // per slot: the best chi^2, copied into best_snap for the next
+    double *best_snap;      //   per-replica launch (null: no copy)
     int *draws;             // per slot (kDrawInts): the next per-replica launch's acceptance
     int draw_mode;          //   uniforms and proposal draws, made by an extra workgroup that
     const int *rung_of;     //   also advances the step counter (null: none; 1 one step,
@@ -127,6 +129,15 @@ struct TileOut {
     double *partials;       // (nrep, nsp, nct, 4)
     int acc32;              // the running chi(R) is an FP32 accumulator (precision study)
 };
+
+// Every workgroup of a per-replica launch decides "new best?" against the best chi^2 before
+// that launch, while the launch's lead workgroup overwrites best_c2; a workgroup that
+// starts after the lead has finished would read the new value.  So the trial transform
+// just before copies the value for them: workgroup (0, rows, edge 0) for its 32 slots.
+__device__ __forceinline__ void snap_best(const TileOut &O, int nrep, int r0, int ct, int e) {
+    if (O.best_snap && ct == 0 && e == 0 && threadIdx.x < RB && r0 + (int)threadIdx.x < nrep)
+        O.best_snap[r0 + threadIdx.x] = O.best_c2[r0 + threadIdx.x];
+}
 
 // lanes t..t+15 (t % 16 == 0) hold the 16 columns of one slot
 __device__ __forceinline__ void tile_epilogue(const Geo &g, const TileOut &O, int nrep, int slot,
@@ -170,6 +181,7 @@ __device__ __forceinline__ void gemm_abs_body(const Geo &g, int nrep, const int 
     const int K = g.nrow_e;
     const int ld = RB + 1;
     const bool reanchor = O.anchor_out != nullptr;
+    if (!reanchor) snap_best(O, nrep, r0, ct, e);
     if (reanchor) {
         if (threadIdx.x == 0) any_due = 0;
         __syncthreads();
@@ -422,6 +434,7 @@ __device__ __forceinline__ void gemm_delta_bf16_tile(const Geo &g, int nrep, con
     const int r0 = rbi * RB;
     const int t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63;
+    snap_best(O[0], nrep, r0, ct, e);
     // epilogue operands first, so their latency hides behind the MFMA phase
     double mc_pre[2], tg_pre[2], w_pre[2];
 #pragma unroll
@@ -554,6 +567,7 @@ __device__ __forceinline__ void gemm_delta_f32_body(const Geo &g, int nrep, cons
     const int c0 = ct * NCW;
     const int K = g.nrow_e;
     const int ld = RB + 1;
+    snap_best(O, nrep, r0, ct, e);
     for (int q = threadIdx.x; q < RB * K; q += blockDim.x) {
         const int m = q / K, k = q % K;
         const int slot = r0 + m;
@@ -667,6 +681,7 @@ struct SlotArgs {
     int *dc_prop;           // count changes of the step being proposed (zero on entry)
     uint8_t *species, *best_sp;
     double *best_c2;
+    const double *best_snap;   // best_c2 as the trial transform before this launch copied it
     const int *rung_of;
     const uint64_t *stepc;  // the pending step (advanced by the transform kernels)
     long long *acc_rung;
@@ -781,7 +796,7 @@ __device__ __forceinline__ void slot_step_body(const Geo &g, const SlotArgs &A, 
     const int rung = A.rung_of[s];
     const double beta = A.beta_slot[s];
     const double c2p = A.c2_prev[s];
-    const double best = A.best_c2[s];
+    const double best = A.best_snap[s];   // the best before this launch (see snap_best)
     const int anchor = A.anchor[s];
     int pp[kPropInts];                       // the proposal being accepted
 #pragma unroll
@@ -1189,6 +1204,7 @@ struct PairArgs {
     int *counts;
     uint8_t *species, *best_sp;
     double *best_c2;
+    const double *best_snap;   // best_c2 as the trial transform before this launch copied it
     const int *rung_of;
     const uint64_t *stepc;       // n + 2 after the round's transform (n when proposing only)
     long long *acc_rung;
@@ -1245,7 +1261,7 @@ __device__ __forceinline__ void slot_pair_body(const Geo &g, const PairArgs &A, 
     const int rung = A.rung_of[s];
     const double beta = A.beta_slot[s];
     const double c2p = A.c2_prev[s];
-    const double best = A.best_c2[s];
+    const double best = A.best_snap[s];   // the best before this launch (see snap_best)
     const int anchor = A.anchor[s];
     int pr[3][kPropInts];
 #pragma unroll
@@ -1685,6 +1701,8 @@ TileOut tile_out(chm_sampler *S) {
     TileOut O{};
     O.stepc = S->stepc.as<uint64_t>();
     O.bump = 0;
+    O.best_c2 = S->best_c2.as<double>();
+    O.best_snap = S->best_snap.as<double>();
     O.target = S->target.as<double>();
     O.wcol = S->wcol.as<double>();
     O.mtrial = S->mtrial.as<double>();
@@ -1707,6 +1725,7 @@ SlotArgs slot_args(chm_sampler *S, int accept, int propose) {
     A.species = S->species.as<uint8_t>();
     A.best_sp = S->best_sp.as<uint8_t>();
     A.best_c2 = S->best_c2.as<double>();
+    A.best_snap = S->best_snap.as<double>();
     A.rung_of = S->rung_of.as<int>();
     A.stepc = S->stepc.as<uint64_t>();
     A.acc_rung = S->acc_rung.as<long long>();
@@ -1960,6 +1979,7 @@ int launch_slot_pair(chm_sampler *S, int accept, int propose, int k, const doubl
     A.species = S->species.as<uint8_t>();
     A.best_sp = S->best_sp.as<uint8_t>();
     A.best_c2 = S->best_c2.as<double>();
+    A.best_snap = S->best_snap.as<double>();
     A.rung_of = S->rung_of.as<int>();
     A.stepc = S->stepc.as<uint64_t>();
     A.acc_rung = S->acc_rung.as<long long>();
@@ -2309,6 +2329,7 @@ extern "C" int chm_sampler_create(chm_ctx *ctx, const chm_geom *gm, int nrep, in
         (rc = S->counts.ensure(sizeof(int) * nr * g.nrow_tot)) ||
         (rc = S->dcounts.ensure(sizeof(int) * 2 * nr * g.nrow_tot)) ||
         (rc = S->c2.ensure(sizeof(double) * 2 * nr)) || (rc = S->best_c2.ensure(sizeof(double) * nr)) ||
+        (rc = S->best_snap.ensure(sizeof(double) * nr)) ||
         (rc = S->rung_of.ensure(sizeof(int) * nr)) || (rc = S->slot_of.ensure(sizeof(int) * nr)) ||
         (rc = S->beta.ensure(sizeof(double) * nr)) || (rc = S->beta_slot.ensure(sizeof(double) * nr)) ||
         (rc = S->stepc.ensure(sizeof(uint64_t) * nr)) ||
@@ -2415,6 +2436,7 @@ extern "C" int chm_sampler_init(chm_sampler *S, const int32_t *species0, const d
     CHM_HIP(hipMemsetAsync(S->anchor.p, 0, S->anchor.bytes, st));
     CHM_HIP(hipMemsetAsync(S->c2.p, 0, S->c2.bytes, st));
     CHM_HIP(hipMemsetAsync(S->best_c2.p, 0, S->best_c2.bytes, st));
+    CHM_HIP(hipMemsetAsync(S->best_snap.p, 0, S->best_snap.bytes, st));
     CHM_HIP(hipMemsetAsync(S->nanchor.p, 0, S->nanchor.bytes, st));
     CHM_HIP(hipMemsetAsync(S->pdc.p, 0, S->pdc.bytes, st));
     CHM_HIP(hipMemsetAsync(S->pprop.p, 0, S->pprop.bytes, st));

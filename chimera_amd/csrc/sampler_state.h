@@ -62,7 +62,7 @@ struct chm_sampler {
     chm::DevBuf site_xyz, nbr_j, nbr_sh, ss_sten, stencil, basisR, basisK, target, wcol, q0, Tb;
     chm::DevBuf nbr_st;   // (natoms, ncoord) neighbour sites in stencil order
     chm::DevBuf basisR32; // FP32 copy of basis_R (FP32-transform mode only)
-    chm::DevBuf species, counts, dcounts, c2, best_c2, best_sp, rung_of, slot_of, beta, beta_slot,
+    chm::DevBuf species, counts, dcounts, c2, best_c2, best_snap, best_sp, rung_of, slot_of, beta, beta_slot,
         stepc, iex,
         prop, mtrial, mcur, partials, anchor, nanchor, exch_att, exch_acc, acc_rung, log, series,
         tmp;

@@ -208,3 +208,24 @@ def test_history_from_device_marks(tern):
     finally:
         S.precision, S.paired = "f64", True
         S.init_state(P.cfg0, P.cell.ideal)
+
+
+@pytest.mark.parametrize("paired,nsplit", [(True, 24), (False, 16)])
+def test_best_state_consistent_with_many_workgroups(paired, nsplit):
+    """At 4000 sites with more per-replica workgroups than stay resident together (the
+    launch's later workgroups start after earlier ones, the lead among them, have
+    finished), every replica's best configuration still has the best chi^2 it is stored
+    with: the workgroups decide 'new best' from the copy the trial transform made, which
+    the launch's lead does not overwrite."""
+    from chimera_amd import workloads as W
+    _say(f"[build] fcc_crconi_4000 paired={paired} nsplit={nsplit}")
+    P = W.build("fcc_crconi_4000")
+    S = P.sampler
+    S.precision, S.paired = "bf16x3", paired
+    S.init_state(P.cfg0, P.cell.ideal)
+    _lib.check(_lib.load().chm_sampler_set_tuning(S._S, nsplit))
+    S.run(1500, exchange_every=25, log_every=10 ** 9)
+    best_sp = S._read(4, (S.nrep, S.natoms), np.int32)
+    rec = S.chi2_of(best_sp)
+    # the running chi(R) is split-BF16 with an FP64 accumulator: a recompute agrees to ~1e-7
+    assert np.all(np.abs(rec - S.best_chi2) <= 1e-5 * S.best_chi2)
